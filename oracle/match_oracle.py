@@ -1,0 +1,161 @@
+"""CPU oracle of the descriptor matcher (test infrastructure only).
+
+Two layers, both plain numpy:
+
+* a restatement of the reference's float32 matcher (app/planesite/matching.py): the
+  brute-force ratio test ``_ratio_match_bf`` (:86-100), ``_unique`` (:103-110) and the
+  prior-guided matcher ``_guided`` (:113-152), with cKDTree's k-nearest-within-radius replaced
+  by a brute-force spatial search of the same definition;
+* the FP64 oracle that defines the match set exactly (every GPU precision mode must return
+  this list bit for bit):
+    - candidates of query q: all of B (brute force), or the B keypoints strictly inside the
+      prior radius around pred[q] (``radius`` in px), optionally only the kc spatially nearest
+      (ties in spatial distance: lower index first);
+    - d(q, t) = sum_k (a_k - b_k)^2 accumulated in float64 in k order;
+    - best = argmin d (ties: lower t); b2 = the second smallest d (+inf if q has one
+      candidate); accept iff sqrt(b1) < ratio * sqrt(b2) in float64;
+    - one query per target: the smallest b1 wins, ties to the lower q; output sorted by t.
+
+For uint8 RootSIFT descriptors (the pipeline's format) every distance is an exact integer
+below 2^24, so the float32 restatement and the FP64 oracle differ only where
+b1 / b2 == ratio^2 exactly; tests/test_oracle_match.py checks both against each other and
+against scipy's cKDTree.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def unique(q: np.ndarray, t: np.ndarray, dd: np.ndarray):
+    """matching.py:103-110: keep one query per target (smallest dd, then earliest position);
+    output sorted by target."""
+    if len(q) == 0:
+        return q, t
+    order = np.lexsort((dd, t))
+    t_s = t[order]
+    first = np.r_[True, t_s[1:] != t_s[:-1]]
+    keep = order[first]
+    return q[keep], t[keep]
+
+
+def _two_smallest(d2: np.ndarray):
+    """Row-wise (best index, best value, second value); ties -> lower index (values are what
+    the reference's argpartition pair yields: a tie at the minimum always fails the ratio test,
+    so its index choice never reaches the output)."""
+    n = d2.shape[1]
+    best = np.argmin(d2, axis=1)
+    r = np.arange(d2.shape[0])
+    b1 = d2[r, best]
+    if n < 2:
+        return best, b1, np.full(len(r), np.inf, d2.dtype)
+    tmp = d2.copy()
+    tmp[r, best] = np.inf
+    b2 = tmp.min(axis=1)
+    return best, b1, b2
+
+
+def ratio_match_bf_f32(da: np.ndarray, db: np.ndarray, ratio: float, na=None, nb=None):
+    """matching.py:86-100 in float32: d2 = na + nb - 2 da.db, top-2, strict ratio, _unique."""
+    if len(da) < 2 or len(db) < 2:
+        return np.zeros(0, np.int64), np.zeros(0, np.int64)
+    da = np.asarray(da, np.float32)
+    db = np.asarray(db, np.float32)
+    na = (da * da).sum(1) if na is None else na
+    nb = (db * db).sum(1) if nb is None else nb
+    d2 = na[:, None] + nb[None, :] - np.float32(2.0) * (da @ db.T)
+    first, a, bmax = _two_smallest(d2)
+    b1 = np.maximum(a, 0)
+    b2 = np.maximum(bmax, 0)
+    ok = np.sqrt(b1) < np.float32(ratio) * np.sqrt(b2)
+    r = np.arange(len(da))
+    return unique(r[ok].astype(np.int64), first[ok].astype(np.int64), b1[ok].astype(np.float32))
+
+
+def guided_candidates(pred: np.ndarray, xy_b: np.ndarray, wh_b, radius: float, kc: int):
+    """matching.py:115-131: queries predicted inside B (with a margin of `radius`), each with
+    the kc spatially nearest B keypoints strictly within `radius` (kc <= 0: all of them).
+    Returns (query index array, list of candidate index arrays)."""
+    pred = np.asarray(pred, np.float64)
+    xy_b = np.asarray(xy_b, np.float64)
+    Wb, Hb = wh_b
+    inside = np.nonzero((pred[:, 0] > -radius) & (pred[:, 0] < Wb + radius)
+                        & (pred[:, 1] > -radius) & (pred[:, 1] < Hb + radius))[0]
+    qs, cands = [], []
+    if len(xy_b) == 0:
+        return np.zeros(0, np.int64), []
+    for q in inside:
+        d = np.sqrt(((xy_b - pred[q]) ** 2).sum(1))
+        idx = np.nonzero(d < radius)[0]
+        if len(idx) == 0:
+            continue
+        if kc > 0 and len(idx) > kc:
+            o = np.lexsort((idx, d[idx]))[:kc]
+            idx = np.sort(idx[o])
+        qs.append(q)
+        cands.append(idx.astype(np.int64))
+    return np.asarray(qs, np.int64), cands
+
+
+def guided_f32(da, db, qs, cands, ratio: float):
+    """matching.py:132-152 in float32 on explicit candidate lists."""
+    da = np.asarray(da, np.float32)
+    db = np.asarray(db, np.float32)
+    na = (da * da).sum(1)
+    nb = (db * db).sum(1)
+    out_q, out_t, out_d = [], [], []
+    for q, c in zip(qs, cands):
+        d2 = np.maximum(na[q] + nb[c] - np.float32(2.0) * (db[c] @ da[q]), np.float32(0))
+        best, b1, b2 = _two_smallest(d2[None, :])
+        b1, b2 = b1[0], b2[0]
+        if np.isfinite(b1) and np.sqrt(b1) < np.float32(ratio) * np.sqrt(b2):
+            out_q.append(q)
+            out_t.append(c[best[0]])
+            out_d.append(b1)
+    return unique(np.asarray(out_q, np.int64), np.asarray(out_t, np.int64), np.asarray(out_d, np.float32))
+
+
+def dist64(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """sum_k (a_k - b_k)^2 in float64, k in order (rows of a against rows of b)."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    acc = np.zeros((a.shape[0], b.shape[0]), np.float64)
+    for k in range(a.shape[1]):
+        t = a[:, k][:, None] - b[:, k][None, :]
+        acc += t * t
+    return acc
+
+
+def match_fp64(da, db, ratio: float, qs=None, cands=None, block: int = 1024):
+    """The FP64 oracle (module docstring).  Brute force when qs/cands are None (needs >= 2 on
+    each side, as matching.py:88-89), else on the given candidate lists.
+    Returns (q int64, t int64, b1 float64) sorted by t."""
+    out_q, out_t, out_d = [], [], []
+    if qs is None:
+        if len(da) < 2 or len(db) < 2:
+            return np.zeros(0, np.int64), np.zeros(0, np.int64), np.zeros(0)
+        for s0 in range(0, len(da), block):
+            d = dist64(da[s0:s0 + block], db)
+            best, b1, b2 = _two_smallest(d)
+            ok = np.sqrt(b1) < float(ratio) * np.sqrt(b2)
+            r = np.nonzero(ok)[0]
+            out_q.append(r + s0)
+            out_t.append(best[r])
+            out_d.append(b1[r])
+    else:
+        for q, c in zip(qs, cands):
+            d = dist64(np.asarray(da)[q:q + 1], np.asarray(db)[c])
+            best, b1, b2 = _two_smallest(d)
+            if np.sqrt(b1[0]) < float(ratio) * np.sqrt(b2[0]):
+                out_q.append(np.array([q]))
+                out_t.append(np.array([c[best[0]]]))
+                out_d.append(b1)
+    if not out_q:
+        return np.zeros(0, np.int64), np.zeros(0, np.int64), np.zeros(0)
+    q = np.concatenate(out_q).astype(np.int64)
+    t = np.concatenate(out_t).astype(np.int64)
+    d = np.concatenate(out_d).astype(np.float64)
+    order = np.lexsort((q, d, t))
+    t_s = t[order]
+    first = np.r_[True, t_s[1:] != t_s[:-1]]
+    keep = order[first]
+    return q[keep], t[keep], d[keep]
