@@ -1,0 +1,73 @@
+"""ctypes binding of libfsx.so (the gfx950 HIP kernels behind a plain C ABI).
+
+Every entry point returns an int status (0 = ok); a non-zero status becomes a RuntimeError
+carrying ``fsx_last_error``, so callers keep the reference's exception-driven fallback logic.
+There is no CPU fallback: if the library is missing, importing the product path fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfsx.so")
+
+_vp = C.c_void_p
+_i = C.c_int
+_sz = C.c_size_t
+
+# name -> argtypes (all return int status unless listed in _RESTYPE)
+SIGNATURES = {
+    "fsx_version": [],
+    "fsx_last_error": [C.c_char_p, _sz],
+    "fsx_device_info": [_i, C.c_char_p, _sz, C.POINTER(_i), C.POINTER(C.c_longlong)],
+    "fsx_sift_plan": [_i, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)],
+    "fsx_sift_taps": [_i, C.POINTER(C.c_float), C.POINTER(_i)],
+    "fsx_sift_workspace_bytes": [_i, _i, _i, _i, C.POINTER(_sz)],
+    "fsx_sift_detect": [_vp, _i, _i, _i, C.c_double, C.c_double, _i, _i, _i, _vp, _sz, _vp, _vp,
+                        C.POINTER(C.c_float), _vp],
+    "fsx_sift_counters": [_vp, C.POINTER(_i), _vp],
+    "fsx_sift_describe": [_vp, _vp, _i, _vp, _i, _vp, _vp, _vp],
+}
+
+_lib = None
+
+
+class FsxError(RuntimeError):
+    pass
+
+
+def load(path: str = LIB_PATH):
+    """Load libfsx.so (raises OSError if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise OSError("libfsx.so not built (%s); run fieldstitch_amd/build.py or __graft_entry__.build()" % path)
+        L = C.CDLL(path)
+        for name, args in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    L = load()
+    buf = C.create_string_buffer(1024)
+    L.fsx_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def call(name: str, *args) -> int:
+    rc = getattr(load(), name)(*args)
+    if rc != 0:
+        raise FsxError("%s failed (status %d): %s" % (name, rc, last_error()))
+    return rc
+
+
+def stream_handle(stream=None) -> int:
+    """Raw hipStream_t of a torch stream (default: the current stream)."""
+    import torch
+    s = torch.cuda.current_stream() if stream is None else stream
+    return int(s.cuda_stream)

@@ -1,0 +1,96 @@
+// Shared definitions of libfsx: error state, launch checks and the SIFT workspace layout.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include <stdio.h>
+#include <stdarg.h>
+
+#define FSX_OK 0
+#define FSX_EINVAL 1
+#define FSX_EHIP 2
+#define FSX_ECAPACITY 3
+#define FSX_ENOMEM 4
+
+namespace fsx {
+
+void set_error(const char* fmt, ...);
+
+#define FSX_HIP(call)                                                                \
+    do {                                                                             \
+        hipError_t e_ = (call);                                                      \
+        if (e_ != hipSuccess) {                                                      \
+            ::fsx::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
+            return FSX_EHIP;                                                         \
+        }                                                                            \
+    } while (0)
+
+#define FSX_LAUNCH_CHECK()                                                           \
+    do {                                                                             \
+        hipError_t e_ = hipGetLastError();                                           \
+        if (e_ != hipSuccess) {                                                      \
+            ::fsx::set_error("%s:%d kernel launch: %s", __FILE__, __LINE__, hipGetErrorString(e_)); \
+            return FSX_EHIP;                                                         \
+        }                                                                            \
+    } while (0)
+
+#define FSX_REQUIRE(cond, ...)                                                       \
+    do {                                                                             \
+        if (!(cond)) {                                                               \
+            ::fsx::set_error(__VA_ARGS__);                                           \
+            return FSX_EINVAL;                                                       \
+        }                                                                            \
+    } while (0)
+
+// ------------------------------------------------------------------------------------------
+// SIFT constants (gsift.py:13-25, 224)
+constexpr int N_LAYERS = 3;
+constexpr int BORDER = 5;
+constexpr int MAX_INTERP = 5;
+constexpr int ORI_BINS = 36;
+constexpr int ORI_R_MAX = 16;
+constexpr int MAXO = 16;
+constexpr int HIST_SIZE = 6 * 6 * 10;    // (d+2) x (d+2) x (bins+2) descriptor histogram
+
+// Octave geometry inside one image's workspace.  Offsets are in float elements.
+struct OctGeom {
+    int R, C;
+    long long dbase;     // DoG atlas: 5 layers (R, C) per octave
+    long long gbase;     // gauss layers 1..3: (3, R, C) per octave
+    long long g0base;    // gauss layer 0 (octave base): (R, C) per octave
+};
+
+// Per-image job record, stored at the head of the image's workspace (device memory).
+struct SiftJob {
+    const uint8_t* img;
+    int H, W, ld;
+    int n_oct;
+    OctGeom oct[MAXO];
+    float* DF;           // 5 * sumP
+    float* G123;         // 3 * sumP
+    float* G0;           // sumP
+    float* G4;           // P0 (scratch for layer 4)
+    // keypoint pipeline buffers
+    uint32_t* cand;      // candidate atlas indices (gauss-atlas numbering, layers 1..3)
+    float4* refined;     // 2 x float4 per refined keypoint: (o,l,r,c as int bits), (X0,X1,X2,resp)
+    uint32_t* refined_src;   // candidate index of each refined keypoint
+    float4* oriented;    // 2 x float4 per oriented keypoint: (x,y,size,angle), (resp, o, l bits, src)
+    unsigned long long* keys0;   // sort keys (2 per element) ping
+    unsigned long long* keys1;   // pong
+    int* counters;       // [0] n_cand, [1] n_refined, [2] n_oriented, [3] overflow flags
+    int cand_cap, kp_cap;
+    float contrast_f, edge_f, e1sq_f, pthr;
+    int max_kp;
+    float* kp_out;       // [7][max_kp] x,y,size,angle,response,octave(int),layer(int)
+    int* n_out;          // number of keypoints written (device)
+};
+
+struct SiftPlan {
+    int H, W, n_oct;
+    int R[MAXO], C[MAXO];
+    long long sumP, P0;
+};
+
+int sift_plan(int H, int W, SiftPlan* p);
+
+}  // namespace fsx

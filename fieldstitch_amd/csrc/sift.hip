@@ -1,0 +1,994 @@
+// SIFT detector/descriptor kernels for gfx950 (CDNA4): Gaussian scale space, DoG extrema,
+// sub-pixel refinement, orientation histograms, deterministic keypoint ordering and 128-d
+// descriptors.  Arithmetic reproduces the reference's CPU path (app/planesite/gsift.py with
+// numpy/scipy dtypes): float32 where numpy is float32 (compiled with -ffp-contract=off, so no
+// fused multiply-add), float64 where numpy promotes (scipy's correlate1d accumulation,
+// bincount, the descriptor's sample geometry).  Histograms accumulate in 64-bit fixed point
+// with LDS integer atomics, so every run gives the same bits.
+#include "fsx_common.h"
+#include <math.h>
+#include <string.h>
+#include <vector>
+
+namespace fsx {
+
+// ------------------------------------------------------------------------------------------
+// Gaussian taps (gsift.py:52-72).  Host side, float64 like numpy, normalised with numpy's
+// pairwise summation order, cast to float32.
+
+static double np_sum_d(const double* a, int n) {
+    if (n < 8) {
+        double r = 0.0;
+        for (int i = 0; i < n; i++) r += a[i];
+        return r;
+    }
+    double r[8];
+    for (int j = 0; j < 8; j++) r[j] = a[j];
+    int i;
+    for (i = 8; i < n - (n % 8); i += 8)
+        for (int j = 0; j < 8; j++) r[j] += a[i + j];
+    double res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+    for (; i < n; i++) res += a[i];
+    return res;
+}
+
+static int gauss_taps(double sigma, float* out) {
+    int k = ((int)rint(sigma * 8.0 + 1.0)) | 1;
+    if (k > 63) return -1;
+    double g[64];
+    for (int i = 0; i < k; i++) {
+        double x = (double)i - (k - 1) / 2.0;
+        g[i] = exp(-x * x / (2.0 * sigma * sigma));
+    }
+    double s = np_sum_d(g, k);
+    for (int i = 0; i < k; i++) out[i] = (float)(g[i] / s);
+    return k;
+}
+
+static void layer_sigmas(double* sig6) {
+    const double SIG = 1.6, INIT = 0.5;
+    double k = pow(2.0, 1.0 / N_LAYERS);
+    sig6[0] = sqrt(fmax(SIG * SIG - (2 * INIT) * (2 * INIT), 0.01));
+    for (int i = 1; i < N_LAYERS + 3; i++) {
+        double prev = pow(k, i - 1) * SIG;
+        double total = prev * k;
+        sig6[i] = sqrt(total * total - prev * prev);
+    }
+}
+
+// Half kernel by distance from the centre: k[0] = centre, k[j] = left tap at distance j.
+struct HalfTaps {
+    double k[16];
+};
+
+struct TapSet {
+    int r2[6];
+    HalfTaps h[6];
+    float full[6][64];
+    int len[6];
+};
+
+static const TapSet& taps() {
+    static TapSet T;
+    static bool init = false;
+    if (!init) {
+        double sig[6];
+        layer_sigmas(sig);
+        for (int i = 0; i < 6; i++) {
+            T.len[i] = gauss_taps(sig[i], T.full[i]);
+            T.r2[i] = T.len[i] / 2;
+            for (int j = 0; j <= T.r2[i]; j++) T.h[i].k[j] = (double)T.full[i][T.r2[i] - j];
+        }
+        init = true;
+    }
+    return T;
+}
+
+// gsift.py:158-172: octave count from the input's smaller side, first octave = 2x upsample,
+// each next octave = ceil halves, stop below 2*BORDER+4.
+int sift_plan(int H, int W, SiftPlan* p) {
+    memset(p, 0, sizeof(*p));
+    p->H = H;
+    p->W = W;
+    int mn = H < W ? H : W;
+    int n_oct = (int)rint(log2((double)mn) - 2.0) + 1;
+    if (n_oct < 1) n_oct = 1;
+    int r = 2 * H, c = 2 * W, n = 0;
+    for (int o = 0; o < n_oct && o < MAXO; o++) {
+        if ((r < c ? r : c) < 2 * BORDER + 4) break;
+        p->R[n] = r;
+        p->C[n] = c;
+        p->sumP += (long long)r * c;
+        n++;
+        r = (r + 1) / 2;
+        c = (c + 1) / 2;
+    }
+    p->n_oct = n;
+    p->P0 = (long long)p->R[0] * p->C[0];
+    return n;
+}
+
+// ------------------------------------------------------------------------------------------
+// Pyramid: one LDS-staged 2-D tile kernel per (octave, layer), batched over images.
+// Vertical pass (axis 0) then horizontal pass (axis 1), each rounded to float32 like
+// scipy.ndimage.correlate1d; float64 accumulation: centre tap first, then symmetric pairs
+// from the outermost inwards (scipy ni_filters.c, symmetric branch).  Mirror border =
+// reflect-101.  The epilogue writes the new gauss layer, the DoG (new - previous, float32)
+// and, for layer 3, the next octave's base (every second pixel).
+
+constexpr int TW = 64;
+constexpr int TH = 32;
+
+__device__ __forceinline__ int reflect101(int p, int n) {
+    while (p < 0 || p >= n) p = p < 0 ? -p : 2 * n - 2 - p;
+    return p;
+}
+
+// value of the 2x bilinear upsample of u8 * float32(1/255) at (y, x) (gsift.py:90-102, 160)
+__device__ __forceinline__ float up2_value(const uint8_t* __restrict__ img, int H, int W, int ld, int y, int x) {
+    const float s255 = (float)(1.0 / 255.0);
+    int iy = y >> 1, ix = x >> 1;
+    int ny = (y & 1) ? min(iy + 1, H - 1) : max(iy - 1, 0);
+    int nx = (x & 1) ? min(ix + 1, W - 1) : max(ix - 1, 0);
+    float a_ii = (float)img[(size_t)iy * ld + ix] * s255;
+    float a_ni = (float)img[(size_t)ny * ld + ix] * s255;
+    float a_in = (float)img[(size_t)iy * ld + nx] * s255;
+    float a_nn = (float)img[(size_t)ny * ld + nx] * s255;
+    float t_i = 0.75f * a_ii + 0.25f * a_ni;
+    float t_n = 0.75f * a_in + 0.25f * a_nn;
+    return 0.75f * t_i + 0.25f * t_n;
+}
+
+template <int R2, bool BASE>
+__global__ __launch_bounds__(256) void k_blur(const SiftJob* __restrict__ jobs, int o, int layer, HalfTaps tp) {
+    constexpr int SW = TW + 2 * R2;
+    constexpr int SH = TH + 2 * R2;
+    __shared__ float s_in[SH * SW];
+    __shared__ float s_mid[TH * SW];
+    const SiftJob& J = jobs[blockIdx.z];
+    if (o >= J.n_oct) return;
+    const int R = J.oct[o].R, C = J.oct[o].C;
+    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+    if (x0 >= C || y0 >= R) return;
+    const long long P = (long long)R * C;
+    const float* in = nullptr;
+    if (!BASE) {
+        if (layer == 1) in = J.G0 + J.oct[o].g0base;
+        else if (layer <= 4) in = J.G123 + J.oct[o].gbase + (long long)(layer - 2) * P;
+        else in = J.G4;
+    }
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < SH * SW; idx += 256) {
+        int ly = idx / SW, lx = idx - (idx / SW) * SW;
+        int gy = reflect101(y0 + ly - R2, R), gx = reflect101(x0 + lx - R2, C);
+        float v;
+        if (BASE) v = up2_value(J.img, J.H, J.W, J.ld, gy, gx);
+        else v = in[(size_t)gy * C + gx];
+        s_in[idx] = v;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < TH * SW; idx += 256) {
+        int ly = idx / SW, lx = idx - (idx / SW) * SW;
+        const float* col = s_in + (ly + R2) * SW + lx;
+        double s = (double)col[0] * tp.k[0];
+#pragma unroll
+        for (int j = R2; j >= 1; --j) s += ((double)col[-j * SW] + (double)col[j * SW]) * tp.k[j];
+        s_mid[idx] = (float)s;
+    }
+    __syncthreads();
+    float* outG = nullptr;
+    float* outD = nullptr;
+    float* down = nullptr;
+    int R2n = 0, C2n = 0;
+    if (BASE) {
+        outG = J.G0 + J.oct[0].g0base;
+    } else {
+        if (layer <= 3) outG = J.G123 + J.oct[o].gbase + (long long)(layer - 1) * P;
+        else if (layer == 4) outG = J.G4;
+        outD = J.DF + J.oct[o].dbase + (long long)(layer - 1) * P;
+        if (layer == 3 && o + 1 < J.n_oct) {
+            down = J.G0 + J.oct[o + 1].g0base;
+            R2n = J.oct[o + 1].R;
+            C2n = J.oct[o + 1].C;
+        }
+    }
+    const int lx = tid & (TW - 1);
+    const int gx = x0 + lx;
+    for (int ly = tid / TW; ly < TH; ly += 256 / TW) {
+        const int gy = y0 + ly;
+        const float* row = s_mid + ly * SW + lx + R2;
+        double s = (double)row[0] * tp.k[0];
+#pragma unroll
+        for (int j = R2; j >= 1; --j) s += ((double)row[-j] + (double)row[j]) * tp.k[j];
+        float g = (float)s;
+        if (gy < R && gx < C) {
+            size_t p = (size_t)gy * C + gx;
+            if (outG) outG[p] = g;
+            if (outD) outD[p] = g - s_in[(ly + R2) * SW + lx + R2];
+            if (down && !(gy & 1) && !(gx & 1)) {
+                int ry = gy >> 1, rx = gx >> 1;
+                if (ry < R2n && rx < C2n) down[(size_t)ry * C2n + rx] = g;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Extrema (gsift.py:293-319, 633-659): non-strict 3x3x3 max/min of DoG layers 1..3 with the
+// prefilter |v| > 0.8*0.5*contrast/3, outside a 5-pixel border.  Candidates are appended in
+// any order as gauss-atlas indices gbase[o] + (l-1)*R*C + r*C + c (ascending index = the
+// reference's candidate order; the final sort key carries it).
+
+constexpr int EX_TW = 64, EX_TH = 16;
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ int wave_append(bool f, int* counter) {
+    unsigned long long m = __ballot(f);
+    int cnt = __popcll(m);
+    int base = 0;
+    const int lane = lane_id();
+    const int leader = __ffsll((long long)m) - 1;
+    if (cnt && lane == leader) base = atomicAdd(counter, cnt);
+    base = __shfl(base, leader < 0 ? 0 : leader);
+    return base + __popcll(m & ((1ull << lane) - 1ull));
+}
+
+__global__ __launch_bounds__(256) void k_extrema(const SiftJob* __restrict__ jobs, int o) {
+    __shared__ float s[5][EX_TH + 2][EX_TW + 2];
+    const SiftJob& J = jobs[blockIdx.z];
+    if (o >= J.n_oct) return;
+    const int R = J.oct[o].R, C = J.oct[o].C;
+    const int x0 = blockIdx.x * EX_TW, y0 = blockIdx.y * EX_TH;
+    if (x0 >= C || y0 >= R) return;
+    const long long P = (long long)R * C;
+    const float* D = J.DF + J.oct[o].dbase;
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < 5 * (EX_TH + 2) * (EX_TW + 2); idx += 256) {
+        int l = idx / ((EX_TH + 2) * (EX_TW + 2));
+        int rem = idx - l * (EX_TH + 2) * (EX_TW + 2);
+        int ly = rem / (EX_TW + 2), lx = rem - ly * (EX_TW + 2);
+        int gy = min(max(y0 + ly - 1, 0), R - 1), gx = min(max(x0 + lx - 1, 0), C - 1);
+        s[l][ly][lx] = D[l * P + (size_t)gy * C + gx];
+    }
+    __syncthreads();
+    const float pthr = J.pthr;
+    const int lx = tid & (EX_TW - 1);
+    const int c = x0 + lx;
+    for (int l = 1; l <= N_LAYERS; l++) {
+        for (int ly = tid / EX_TW; ly < EX_TH; ly += 256 / EX_TW) {
+            const int r = y0 + ly;
+            bool f = false;
+            if (r >= BORDER && r < R - BORDER && c >= BORDER && c < C - BORDER) {
+                const float v = s[l][ly + 1][lx + 1];
+                bool mx = v > pthr, mn = v < -pthr;
+                if (mx || mn) {
+#pragma unroll
+                    for (int dl = -1; dl <= 1; dl++)
+#pragma unroll
+                        for (int dr = -1; dr <= 1; dr++)
+#pragma unroll
+                            for (int dc = -1; dc <= 1; dc++) {
+                                float u = s[l + dl][ly + 1 + dr][lx + 1 + dc];
+                                mx = mx && !(u > v);
+                                mn = mn && !(u < v);
+                            }
+                    f = mx || mn;
+                }
+            }
+            int pos = wave_append(f, &J.counters[0]);
+            if (f) {
+                if (pos < J.cand_cap)
+                    J.cand[pos] = (uint32_t)(J.oct[o].gbase + (long long)(l - 1) * P + (long long)r * C + c);
+                else
+                    atomicOr(&J.counters[3], 1);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Refinement (gsift.py:274-290 derivatives, 182-198 adjugate solve, 322-369 iteration and
+// the contrast / edge tests).  One thread per candidate.
+
+struct Derivs {
+    float v, dx, dy, ds, dxx, dyy, dss, dxy, dxs, dys;
+};
+
+__device__ __forceinline__ Derivs derivs_at(const float* __restrict__ D, long long k, long long C, long long RC) {
+    Derivs d;
+    d.v = D[k];
+    float xp1 = D[k + 1], xm1 = D[k - 1], yp1 = D[k + C], ym1 = D[k - C], sp1 = D[k + RC], sm1 = D[k - RC];
+    d.dx = (xp1 - xm1) * 0.5f;
+    d.dy = (yp1 - ym1) * 0.5f;
+    d.ds = (sp1 - sm1) * 0.5f;
+    d.dxx = (xp1 + xm1) - 2.0f * d.v;
+    d.dyy = (yp1 + ym1) - 2.0f * d.v;
+    d.dss = (sp1 + sm1) - 2.0f * d.v;
+    d.dxy = (((D[k + C + 1] - D[k + C - 1]) - D[k - C + 1]) + D[k - C - 1]) * 0.25f;
+    d.dxs = (((D[k + RC + 1] - D[k + RC - 1]) - D[k - RC + 1]) + D[k - RC - 1]) * 0.25f;
+    d.dys = (((D[k + RC + C] - D[k + RC - C]) - D[k - RC + C]) + D[k - RC - C]) * 0.25f;
+    return d;
+}
+
+__global__ __launch_bounds__(256) void k_refine(const SiftJob* __restrict__ jobs) {
+    const SiftJob& J = jobs[blockIdx.z];
+    const int n = min(J.counters[0], J.cand_cap);
+    // block-uniform loop so that every lane of a wave reaches the ballot in wave_append
+    for (int base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
+        const int i = base + threadIdx.x;
+        bool keep = false;
+        int o = 0, l = 0, r = 0, c = 0;
+        float X0 = 0.f, X1 = 0.f, X2 = 0.f, resp = 0.f;
+        uint32_t g = 0;
+        if (i < n) {
+            g = J.cand[i];
+            while (o + 1 < J.n_oct && (long long)g >= J.oct[o + 1].gbase) o++;
+            const int R = J.oct[o].R, C = J.oct[o].C;
+            const long long RC = (long long)R * C;
+            long long loc = (long long)g - J.oct[o].gbase;
+            l = (int)(loc / RC) + 1;
+            long long rem = loc - (long long)(l - 1) * RC;
+            r = (int)(rem / C);
+            c = (int)(rem - (long long)r * C);
+            const float* D = J.DF + J.oct[o].dbase;
+            bool good = false;
+            for (int it = 0; it < MAX_INTERP; it++) {
+                Derivs d = derivs_at(D, (long long)l * RC + (long long)r * C + c, C, RC);
+                const float a = d.dxx, bb = d.dxy, cc = d.dxs, dd = d.dxy, e = d.dyy, ff = d.dys, gg = d.dxs,
+                            h = d.dys, ii = d.dss;
+                const float A0 = e * ii - ff * h, B0 = -(dd * ii - ff * gg), C0 = dd * h - e * gg;
+                const float det = (a * A0 + bb * B0) + cc * C0;
+                if (!(fabsf(det) > 1e-20f)) break;
+                const float i00 = A0 / det, i01 = -(bb * ii - cc * h) / det, i02 = (bb * ff - cc * e) / det;
+                const float i10 = B0 / det, i11 = (a * ii - cc * gg) / det, i12 = -(a * ff - cc * dd) / det;
+                const float i20 = C0 / det, i21 = -(a * h - bb * gg) / det, i22 = (a * e - bb * dd) / det;
+                const float x0 = -((i00 * d.dx + i01 * d.dy) + i02 * d.ds);
+                const float x1 = -((i10 * d.dx + i11 * d.dy) + i12 * d.ds);
+                const float x2 = -((i20 * d.dx + i21 * d.dy) + i22 * d.ds);
+                if (fabsf(x0) < 0.5f && fabsf(x1) < 0.5f && fabsf(x2) < 0.5f) {
+                    good = true;
+                    X0 = x0; X1 = x1; X2 = x2;
+                    break;
+                }
+                if (!(fabsf(x0) <= 1e6f && fabsf(x1) <= 1e6f && fabsf(x2) <= 1e6f)) break;   // also NaN
+                c += (int)rintf(x0);
+                r += (int)rintf(x1);
+                l += (int)rintf(x2);
+                if (l < 1 || l > N_LAYERS || c < BORDER || c >= C - BORDER || r < BORDER || r >= R - BORDER) break;
+            }
+            if (good) {
+                Derivs d = derivs_at(D, (long long)l * RC + (long long)r * C + c, C, RC);
+                const float contr = d.v + 0.5f * ((d.dx * X0 + d.dy * X1) + d.ds * X2);
+                const float tr = d.dxx + d.dyy;
+                const float dt = d.dxx * d.dyy - d.dxy * d.dxy;
+                const float ac = fabsf(contr);
+                if (ac * 3.0f >= J.contrast_f && dt > 0.f && (tr * tr) * J.edge_f < J.e1sq_f * dt) {
+                    keep = true;
+                    resp = ac;
+                }
+            }
+        }
+        int pos = wave_append(keep, &J.counters[1]);
+        if (keep) {
+            if (pos < J.cand_cap) {
+                J.refined[2 * pos] = make_float4(__int_as_float(o), __int_as_float(l), __int_as_float(r), __int_as_float(c));
+                J.refined[2 * pos + 1] = make_float4(X0, X1, X2, resp);
+                J.refined_src[pos] = g;
+            } else {
+                atomicOr(&J.counters[3], 2);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Gradient of a gauss layer at (y, x) (gsift.py:256-267): dx = G[y,x+1]-G[y,x-1],
+// dy = G[y-1,x]-G[y+1,x]; magnitude sqrt (float32, correctly rounded); angle =
+// float32(atan2) * (180f/pi_f) in [0, 360).  Callers only ask for interior pixels.
+
+__device__ __forceinline__ void grad_at(const float* __restrict__ G, int C, int y, int x, float& mag, float& ang) {
+    const size_t k = (size_t)y * C + x;
+    const float dx = G[k + 1] - G[k - 1];
+    const float dy = G[k - C] - G[k + C];
+    const float xx = dx * dx, yy = dy * dy;
+    mag = sqrtf(xx + yy);
+    const float deg = 180.0f / 3.14159265358979323846f;
+    float a = (float)atan2((double)dy, (double)dx) * deg;
+    ang = a < 0.f ? a + 360.0f : a;
+}
+
+// Orientation (gsift.py:372-432 with the detect epilogue :462-469).  One wave per refined
+// keypoint: 36-bin histogram of the gaussian-weighted gradient magnitudes in the
+// (2*ORI_R_MAX+1)^2 window (own radius rint(4.5*scl)), smoothed, peaks >= 0.8 max with
+// parabolic interpolation.  Each peak is appended as an oriented keypoint with its sort key.
+
+__global__ __launch_bounds__(64) void k_orient(const SiftJob* __restrict__ jobs) {
+    __shared__ unsigned long long s_h[ORI_BINS];
+    __shared__ float s_f[ORI_BINS];
+    const SiftJob& J = jobs[blockIdx.z];
+    const int n = min(J.counters[1], J.cand_cap);
+    const int lane = threadIdx.x;
+    constexpr int WIN = 2 * ORI_R_MAX + 1;
+    for (int k = blockIdx.x; k < n; k += gridDim.x) {
+        const float4 a = J.refined[2 * k], b = J.refined[2 * k + 1];
+        const int o = __float_as_int(a.x), l = __float_as_int(a.y), r = __float_as_int(a.z), c = __float_as_int(a.w);
+        const int R = J.oct[o].R, C = J.oct[o].C;
+        const float e = ((float)l + b.z) / 3.0f;
+        const float scl = 1.6f * (float)pow(2.0, (double)e);
+        const int rad = (int)rintf(4.5f * scl);
+        const float sw = 1.5f * scl;
+        const float den = 2.0f * (sw * sw);
+        const float* G = J.G123 + J.oct[o].gbase + (long long)(l - 1) * R * C;
+        if (lane < ORI_BINS) s_h[lane] = 0ull;
+        __syncthreads();
+        for (int s = lane; s < WIN * WIN; s += 64) {
+            const int i = s / WIN - ORI_R_MAX, j = s - (s / WIN) * WIN - ORI_R_MAX;
+            const int y = r + i, x = c + j;
+            if (abs(i) <= rad && abs(j) <= rad && y > 0 && y < R - 1 && x > 0 && x < C - 1) {
+                float mag, ang;
+                grad_at(G, C, y, x, mag, ang);
+                const float d2 = (float)(i * i + j * j);
+                const float w = (float)exp((double)(-d2 / den));
+                const float m = mag * w;
+                int bin = (int)rintf(ang * 0.1f);
+                bin = bin >= ORI_BINS ? bin - ORI_BINS : bin;
+                atomicAdd(&s_h[bin], __double2ull_rn((double)m * 0x1p50));
+            }
+        }
+        __syncthreads();
+        if (lane < ORI_BINS) s_f[lane] = (float)((double)s_h[lane] * 0x1p-50);
+        __syncthreads();
+        float h = -1.0f, hl = 0.f, hr = 0.f;
+        if (lane < ORI_BINS) {
+            auto H = [&](int j) { return s_f[(j + ORI_BINS) % ORI_BINS]; };
+            h = ((H(lane - 2) + H(lane + 2)) * 0.0625f + (H(lane - 1) + H(lane + 1)) * 0.25f) + H(lane) * 0.375f;
+        }
+        __syncthreads();
+        if (lane < ORI_BINS) s_f[lane] = h;
+        __syncthreads();
+        float hmax = h;
+        for (int off = 32; off >= 1; off >>= 1) hmax = fmaxf(hmax, __shfl_xor(hmax, off));
+        bool peak = false;
+        float angle = 0.f;
+        if (lane < ORI_BINS) {
+            hl = s_f[(lane + ORI_BINS - 1) % ORI_BINS];
+            hr = s_f[(lane + 1) % ORI_BINS];
+            peak = h > hl && h > hr && h >= 0.8f * hmax && hmax > 0.0f;
+            if (peak) {
+                float dn = (hl - 2.0f * h) + hr;
+                if (fabsf(dn) < 1e-12f) dn = -1e-12f;
+                const float q = (0.5f * (hl - hr)) / dn;
+                double bin = (double)lane + (double)q;
+                if (bin < 0) bin += ORI_BINS;
+                if (bin >= ORI_BINS) bin -= ORI_BINS;
+                double ang = 360.0 - 10.0 * bin;
+                if (fabs(ang - 360.0) < 1e-4) ang = 0.0;
+                angle = (float)ang;
+            }
+        }
+        const unsigned long long pm = __ballot(peak);
+        const int rank = __popcll(pm & ((1ull << lane) - 1ull));
+        int pos = wave_append(peak, &J.counters[2]);
+        if (peak) {
+            if (pos < J.kp_cap) {
+                const float f = (float)ldexp(1.0, o) * 0.5f;
+                const float x = ((float)c + b.x) * f;
+                const float y = ((float)r + b.y) * f;
+                const float size = (scl * 2.0f) * f;
+                const uint32_t src = J.refined_src[k];
+                J.oriented[2 * pos] = make_float4(x, y, size, angle);
+                J.oriented[2 * pos + 1] = make_float4(b.w, __int_as_float(o), __int_as_float(l), __uint_as_float(src));
+                // stable argsort of -response (gsift.py:471): response descending, then
+                // candidate order, then peak order within the keypoint
+                const unsigned long long k0 = ((unsigned long long)(~__float_as_uint(b.w)) << 32) | src;
+                const unsigned long long k1 = ((unsigned long long)rank << 32) | (unsigned)pos;
+                J.keys0[2 * pos] = k0;
+                J.keys0[2 * pos + 1] = k1;
+            } else {
+                atomicOr(&J.counters[3], 4);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Deterministic sort of (k0, k1) key pairs: 1024-element bitonic blocks in LDS, then merge
+// passes where every element finds its output slot by binary search in the partner run.
+
+__device__ __forceinline__ bool key_less(unsigned long long a0, unsigned long long a1, unsigned long long b0,
+                                         unsigned long long b1) {
+    return a0 < b0 || (a0 == b0 && a1 < b1);
+}
+
+__global__ __launch_bounds__(256) void k_sort_local(const SiftJob* __restrict__ jobs) {
+    __shared__ unsigned long long s0[1024], s1[1024];
+    const SiftJob& J = jobs[blockIdx.z];
+    const int n = min(J.counters[2], J.kp_cap);
+    const int base = blockIdx.x * 1024;
+    if (base >= n) return;
+    for (int t = threadIdx.x; t < 1024; t += 256) {
+        int g = base + t;
+        if (g < n) {
+            s0[t] = J.keys0[2 * g];
+            s1[t] = J.keys0[2 * g + 1];
+        } else {
+            s0[t] = ~0ull;
+            s1[t] = ~0ull;
+        }
+    }
+    __syncthreads();
+    for (int k = 2; k <= 1024; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int t = threadIdx.x; t < 512; t += 256) {
+                int i = 2 * t - (t & (j - 1));
+                int p = i + j;
+                bool up = (i & k) == 0;
+                unsigned long long a0 = s0[i], a1 = s1[i], b0 = s0[p], b1 = s1[p];
+                bool sw = up ? key_less(b0, b1, a0, a1) : key_less(a0, a1, b0, b1);
+                if (sw) {
+                    s0[i] = b0; s1[i] = b1;
+                    s0[p] = a0; s1[p] = a1;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int t = threadIdx.x; t < 1024; t += 256) {
+        int g = base + t;
+        if (g < n) {
+            J.keys0[2 * g] = s0[t];
+            J.keys0[2 * g + 1] = s1[t];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_merge(const SiftJob* __restrict__ jobs, int w, int src_is_0) {
+    const SiftJob& J = jobs[blockIdx.z];
+    const int n = min(J.counters[2], J.kp_cap);
+    const unsigned long long* src = src_is_0 ? J.keys0 : J.keys1;
+    unsigned long long* dst = src_is_0 ? J.keys1 : J.keys0;
+    for (int p = blockIdx.x * 256 + threadIdx.x; p < n; p += gridDim.x * 256) {
+        const unsigned long long x0 = src[2 * p], x1 = src[2 * p + 1];
+        const int s = (p / (2 * w)) * (2 * w);
+        int lo, hi, out;
+        if (p < s + w) {
+            lo = s + w;
+            hi = min(s + 2 * w, n);
+            out = p;
+        } else {
+            lo = s;
+            hi = s + w;
+            out = p - w;
+        }
+        // count of partner elements less than x
+        int a = lo, b = hi;
+        while (a < b) {
+            int m = (a + b) >> 1;
+            if (key_less(src[2 * m], src[2 * m + 1], x0, x1)) a = m + 1;
+            else b = m;
+        }
+        out += a - lo;
+        dst[2 * out] = x0;
+        dst[2 * out + 1] = x1;
+    }
+}
+
+// emit the first min(n, max_kp) keypoints in sorted order as the detect() arrays
+__global__ __launch_bounds__(256) void k_emit(const SiftJob* __restrict__ jobs, int final_in_0) {
+    const SiftJob& J = jobs[blockIdx.z];
+    const int n = min(J.counters[2], J.kp_cap);
+    const int m = min(n, J.max_kp);
+    const unsigned long long* keys = final_in_0 ? J.keys0 : J.keys1;
+    const int stride = J.max_kp;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < m; i += gridDim.x * 256) {
+        const int idx = (int)(keys[2 * i + 1] & 0xffffffffull);
+        const float4 a = J.oriented[2 * idx], b = J.oriented[2 * idx + 1];
+        float* out = J.kp_out;
+        out[i] = a.x;
+        out[stride + i] = a.y;
+        out[2 * stride + i] = a.z;
+        out[3 * stride + i] = a.w;
+        out[4 * stride + i] = b.x;
+        ((int*)out)[5 * stride + i] = __float_as_int(b.y);
+        ((int*)out)[6 * stride + i] = __float_as_int(b.z);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *J.n_out = m;
+}
+
+__global__ void k_reset(const SiftJob* __restrict__ jobs) {
+    const SiftJob& J = jobs[blockIdx.x];
+    if (threadIdx.x < 4) J.counters[threadIdx.x] = 0;
+}
+
+// ------------------------------------------------------------------------------------------
+// Descriptor (gsift.py:480-525 describe, 800-871 _descr_flat).  One 256-thread block per
+// selected keypoint.  Sample geometry in float64 (the reference's host-side scale factors
+// are float64), angle bin in float32, trilinear split into the (6,6,10) histogram with
+// 64-bit fixed-point LDS atomics, then the orientation wrap, the 4x4x8 crop and numpy's
+// float32 normalisation (pairwise sums with 8 accumulators): L2 -> clip 0.2 -> L2 -> x512.
+// Optional RootSIFT uint8 epilogue (features.py:80-82).
+
+struct DescArgs {
+    const float* kp;      // [7][stride] detect arrays (x,y,size,angle,response,octave,layer)
+    int stride;
+    const long long* sel; // selected keypoint indices
+    int m;
+    float* desc;          // (m, 128)
+    uint8_t* u8;          // (m, 128) or null
+};
+
+__device__ __forceinline__ float wave_pairwise8(float v_lane_j_partial, int lane) {
+    // lanes 0..7 hold numpy's 8 running accumulators; combine in numpy's fixed tree
+    float r0 = __shfl(v_lane_j_partial, 0), r1 = __shfl(v_lane_j_partial, 1);
+    float r2 = __shfl(v_lane_j_partial, 2), r3 = __shfl(v_lane_j_partial, 3);
+    float r4 = __shfl(v_lane_j_partial, 4), r5 = __shfl(v_lane_j_partial, 5);
+    float r6 = __shfl(v_lane_j_partial, 6), r7 = __shfl(v_lane_j_partial, 7);
+    return ((r0 + r1) + (r2 + r3)) + ((r4 + r5) + (r6 + r7));
+}
+
+__global__ __launch_bounds__(256) void k_describe(const SiftJob* __restrict__ job, DescArgs A) {
+    __shared__ unsigned long long s_h[HIST_SIZE];
+    __shared__ float s_d[128];
+    const SiftJob& J = *job;
+    const int q = blockIdx.x;
+    if (q >= A.m) return;
+    const long long k = A.sel[q];
+    const int* kpi = (const int*)A.kp;
+    const int o = kpi[5 * (long long)A.stride + k], li = kpi[6 * (long long)A.stride + k];
+    const int R = J.oct[o].R, C = J.oct[o].C;
+    const double f = 1.0 / (ldexp(1.0, o) * 0.5);
+    const double px = (double)A.kp[k] * f, py = (double)A.kp[(long long)A.stride + k] * f;
+    const double scl = (double)A.kp[2 * (long long)A.stride + k] * f * 0.5;
+    float ori = 360.0f - A.kp[3 * (long long)A.stride + k];
+    if (fabsf(ori - 360.0f) < 1e-4f) ori = 0.0f;
+    long long rad = (long long)rint(((3.0 * scl) * 1.4142135623730951) * 5 * 0.5);
+    const long long rlim = (long long)sqrt((double)((long long)R * R + (long long)C * C));
+    rad = rad < rlim ? rad : rlim;
+    rad = rad < 23000 ? rad : 23000;   // (2 rad + 1)^2 stays below 2^31
+    const double hw = 3.0 * scl;
+    const float t = ori * (float)(3.14159265358979323846 / 180.0);
+    const double cos_t = (double)(float)cos((double)t) / hw, sin_t = (double)(float)sin((double)t) / hw;
+    const long long pr = (long long)rint(py), pc = (long long)rint(px);
+    const float* G = J.G123 + J.oct[o].gbase + (long long)(li - 1) * R * C;
+    for (int b = threadIdx.x; b < HIST_SIZE; b += 256) s_h[b] = 0ull;
+    __syncthreads();
+    const int W = (int)(2 * rad + 1);
+    const float obs = (float)(8.0 / 360.0);
+    for (int s = threadIdx.x; s < W * W; s += 256) {
+        const int i = s / W - (int)rad, j = s - (s / W) * W - (int)rad;
+        const double oi = (double)(float)i, oj = (double)(float)j;
+        const double c_rot = oj * cos_t - oi * sin_t;
+        const double r_rot = oj * sin_t + oi * cos_t;
+        const double rbin = (r_rot + 2.0) - 0.5, cbin = (c_rot + 2.0) - 0.5;
+        const long long yy = pr + i, xx = pc + j;
+        if (!(rbin > -1.0 && rbin < 4.0 && cbin > -1.0 && cbin < 4.0 && yy > 0 && yy < R - 1 && xx > 0 && xx < C - 1))
+            continue;
+        float mag, ang;
+        grad_at(G, C, (int)yy, (int)xx, mag, ang);
+        const double w = exp((c_rot * c_rot + r_rot * r_rot) * (-0.125));
+        const double mg = (double)mag * w;
+        const float obin = (ang - ori) * obs;
+        const double r0 = floor(rbin), c0 = floor(cbin);
+        const float o0f = floorf(obin);
+        const double rb = rbin - r0, cb = cbin - c0;
+        const double ob = (double)(obin - o0f);
+        int o0 = (int)o0f;
+        if (o0 < 0) o0 += 8;
+        if (o0 >= 8) o0 -= 8;
+        const double v_r1 = mg * rb, v_r0 = mg - v_r1;
+        const double v_rc11 = v_r1 * cb, v_rc10 = v_r1 - v_rc11;
+        const double v_rc01 = v_r0 * cb, v_rc00 = v_r0 - v_rc01;
+        const double v111 = v_rc11 * ob, v110 = v_rc11 - v111;
+        const double v101 = v_rc10 * ob, v100 = v_rc10 - v101;
+        const double v011 = v_rc01 * ob, v010 = v_rc01 - v011;
+        const double v001 = v_rc00 * ob, v000 = v_rc00 - v001;
+        const int idx = (((int)r0 + 1) * 6 + (int)c0 + 1) * 10 + o0;
+        atomicAdd(&s_h[idx], __double2ull_rn(v000 * 0x1p50));
+        atomicAdd(&s_h[idx + 1], __double2ull_rn(v001 * 0x1p50));
+        atomicAdd(&s_h[idx + 10], __double2ull_rn(v010 * 0x1p50));
+        atomicAdd(&s_h[idx + 11], __double2ull_rn(v011 * 0x1p50));
+        atomicAdd(&s_h[idx + 60], __double2ull_rn(v100 * 0x1p50));
+        atomicAdd(&s_h[idx + 61], __double2ull_rn(v101 * 0x1p50));
+        atomicAdd(&s_h[idx + 70], __double2ull_rn(v110 * 0x1p50));
+        atomicAdd(&s_h[idx + 71], __double2ull_rn(v111 * 0x1p50));
+    }
+    __syncthreads();
+    if (threadIdx.x < 128) {
+        const int b = threadIdx.x;
+        const int rr = b >> 5, cc = (b >> 3) & 3, ob = b & 7;
+        const int cell = ((rr + 1) * 6 + cc + 1) * 10;
+        float v = (float)((double)s_h[cell + ob] * 0x1p-50);
+        if (ob < 2) v = v + (float)((double)s_h[cell + 8 + ob] * 0x1p-50);
+        s_d[b] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        float part = 0.f;
+        if (lane < 8) {
+            part = s_d[lane] * s_d[lane];
+            for (int t2 = 1; t2 < 16; t2++) {
+                const float v = s_d[8 * t2 + lane];
+                part += v * v;
+            }
+        }
+        const float nrm = sqrtf(wave_pairwise8(part, lane));
+        const float thr = nrm * 0.2f;
+        float d0 = s_d[lane], d1 = s_d[lane + 64];
+        d0 = d0 < thr ? d0 : thr;
+        d1 = d1 < thr ? d1 : thr;
+        s_d[lane] = d0;
+        s_d[lane + 64] = d1;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_s_waitcnt(0);
+        part = 0.f;
+        if (lane < 8) {
+            part = s_d[lane] * s_d[lane];
+            for (int t2 = 1; t2 < 16; t2++) {
+                const float v = s_d[8 * t2 + lane];
+                part += v * v;
+            }
+        }
+        const float nrm2 = sqrtf(wave_pairwise8(part, lane));
+        const float sc = 512.0f / (nrm2 > 1e-7f ? nrm2 : 1e-7f);
+        const float e0 = d0 * sc, e1 = d1 * sc;
+        A.desc[(long long)q * 128 + lane] = e0;
+        A.desc[(long long)q * 128 + lane + 64] = e1;
+        if (A.u8) {
+            s_d[lane] = e0;
+            s_d[lane + 64] = e1;
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_s_waitcnt(0);
+            part = 0.f;
+            if (lane < 8) {
+                part = fabsf(s_d[lane]);
+                for (int t2 = 1; t2 < 16; t2++) part += fabsf(s_d[8 * t2 + lane]);
+            }
+            const float l1 = wave_pairwise8(part, lane) + 1e-7f;
+            float u0 = sqrtf(e0 / l1) * 512.0f, u1 = sqrtf(e1 / l1) * 512.0f;
+            u0 = u0 < 0.f ? 0.f : (u0 > 255.f ? 255.f : u0);
+            u1 = u1 < 0.f ? 0.f : (u1 > 255.f ? 255.f : u1);
+            A.u8[(long long)q * 128 + lane] = (uint8_t)u0;
+            A.u8[(long long)q * 128 + lane + 64] = (uint8_t)u1;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Host side
+
+static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct WsLayout {
+    size_t job, counters, DF, G0, G123, G4, cand, refined, refined_src, oriented, keys0, keys1, total;
+};
+
+static WsLayout ws_layout(const SiftPlan& p, int cand_cap, int kp_cap) {
+    WsLayout L;
+    size_t off = 0;
+    L.job = off; off = align256(off + sizeof(SiftJob));
+    L.counters = off; off = align256(off + 16 * sizeof(int));
+    L.DF = off; off = align256(off + 5 * (size_t)p.sumP * 4);
+    L.G0 = off; off = align256(off + (size_t)p.sumP * 4);
+    L.G123 = off; off = align256(off + 3 * (size_t)p.sumP * 4);
+    L.G4 = off; off = align256(off + (size_t)p.P0 * 4);
+    L.cand = off; off = align256(off + (size_t)cand_cap * 4);
+    L.refined = off; off = align256(off + (size_t)cand_cap * 32);
+    L.refined_src = off; off = align256(off + (size_t)cand_cap * 4);
+    L.oriented = off; off = align256(off + (size_t)kp_cap * 32);
+    L.keys0 = off; off = align256(off + (size_t)kp_cap * 16);
+    L.keys1 = off; off = align256(off + (size_t)kp_cap * 16);
+    L.total = off;
+    return L;
+}
+
+static int fill_job(SiftJob* J, const SiftPlan& p, const uint8_t* img, int ld, double contrast, double edge,
+                    int max_kp, int cand_cap, int kp_cap, void* ws, float* kp_out, int* n_out) {
+    WsLayout L = ws_layout(p, cand_cap, kp_cap);
+    char* b = (char*)ws;
+    memset(J, 0, sizeof(*J));
+    J->img = img;
+    J->H = p.H;
+    J->W = p.W;
+    J->ld = ld;
+    J->n_oct = p.n_oct;
+    long long db = 0, gb = 0, g0 = 0;
+    for (int o = 0; o < p.n_oct; o++) {
+        long long P = (long long)p.R[o] * p.C[o];
+        J->oct[o].R = p.R[o];
+        J->oct[o].C = p.C[o];
+        J->oct[o].dbase = db;
+        J->oct[o].gbase = gb;
+        J->oct[o].g0base = g0;
+        db += 5 * P;
+        gb += 3 * P;
+        g0 += P;
+    }
+    J->DF = (float*)(b + L.DF);
+    J->G0 = (float*)(b + L.G0);
+    J->G123 = (float*)(b + L.G123);
+    J->G4 = (float*)(b + L.G4);
+    J->cand = (uint32_t*)(b + L.cand);
+    J->refined = (float4*)(b + L.refined);
+    J->refined_src = (uint32_t*)(b + L.refined_src);
+    J->oriented = (float4*)(b + L.oriented);
+    J->keys0 = (unsigned long long*)(b + L.keys0);
+    J->keys1 = (unsigned long long*)(b + L.keys1);
+    J->counters = (int*)(b + L.counters);
+    J->cand_cap = cand_cap;
+    J->kp_cap = kp_cap;
+    // gsift.py:295 prefilter threshold is float64 then compared in float32; contrast and
+    // edge are weak Python scalars (float32) in the tests of gsift.py:367
+    J->pthr = (float)(0.8 * 0.5 * contrast / N_LAYERS);
+    J->contrast_f = (float)contrast;
+    J->edge_f = (float)edge;
+    J->e1sq_f = (float)((edge + 1.0) * (edge + 1.0));
+    J->max_kp = max_kp;
+    J->kp_out = kp_out;
+    J->n_out = n_out;
+    return FSX_OK;
+}
+
+template <int R2>
+static void launch_blur(const SiftJob* jobs, int n_img, int o, int layer, int R, int C, const HalfTaps& h,
+                        hipStream_t st) {
+    dim3 grid((C + TW - 1) / TW, (R + TH - 1) / TH, n_img);
+    hipLaunchKernelGGL((k_blur<R2, false>), grid, dim3(256), 0, st, jobs, o, layer, h);
+}
+
+static int launch_layer(const SiftJob* jobs, int n_img, int o, int layer, int R, int C, hipStream_t st) {
+    const TapSet& T = taps();
+    const HalfTaps& h = T.h[layer];
+    switch (T.r2[layer]) {
+        case 5: launch_blur<5>(jobs, n_img, o, layer, R, C, h, st); break;
+        case 6: launch_blur<6>(jobs, n_img, o, layer, R, C, h, st); break;
+        case 8: launch_blur<8>(jobs, n_img, o, layer, R, C, h, st); break;
+        case 10: launch_blur<10>(jobs, n_img, o, layer, R, C, h, st); break;
+        case 13: launch_blur<13>(jobs, n_img, o, layer, R, C, h, st); break;
+        default:
+            set_error("unexpected blur half-width %d", T.r2[layer]);
+            return FSX_EINVAL;
+    }
+    FSX_LAUNCH_CHECK();
+    return FSX_OK;
+}
+
+// Enqueue the whole detect pipeline for n_img images of one shape (jobs: device array).
+static int run_detect(const SiftJob* jobs, int n_img, const SiftPlan& p, int kp_cap, hipStream_t st,
+                      hipEvent_t* ev) {
+    const TapSet& T = taps();
+    if (T.r2[0] != 5) {
+        set_error("unexpected base blur half-width %d", T.r2[0]);
+        return FSX_EINVAL;
+    }
+    if (ev) FSX_HIP(hipEventRecord(ev[0], st));
+    hipLaunchKernelGGL(k_reset, dim3(n_img), dim3(64), 0, st, jobs);
+    {
+        dim3 grid((p.C[0] + TW - 1) / TW, (p.R[0] + TH - 1) / TH, n_img);
+        hipLaunchKernelGGL((k_blur<5, true>), grid, dim3(256), 0, st, jobs, 0, 0, T.h[0]);
+        FSX_LAUNCH_CHECK();
+    }
+    for (int o = 0; o < p.n_oct; o++)
+        for (int layer = 1; layer <= N_LAYERS + 2; layer++) {
+            int rc = launch_layer(jobs, n_img, o, layer, p.R[o], p.C[o], st);
+            if (rc) return rc;
+        }
+    if (ev) FSX_HIP(hipEventRecord(ev[1], st));
+    for (int o = 0; o < p.n_oct; o++) {
+        dim3 grid((p.C[o] + EX_TW - 1) / EX_TW, (p.R[o] + EX_TH - 1) / EX_TH, n_img);
+        hipLaunchKernelGGL(k_extrema, grid, dim3(256), 0, st, jobs, o);
+        FSX_LAUNCH_CHECK();
+    }
+    if (ev) FSX_HIP(hipEventRecord(ev[2], st));
+    hipLaunchKernelGGL(k_refine, dim3(256, 1, n_img), dim3(256), 0, st, jobs);
+    FSX_LAUNCH_CHECK();
+    if (ev) FSX_HIP(hipEventRecord(ev[3], st));
+    hipLaunchKernelGGL(k_orient, dim3(4096, 1, n_img), dim3(64), 0, st, jobs);
+    FSX_LAUNCH_CHECK();
+    if (ev) FSX_HIP(hipEventRecord(ev[4], st));
+    hipLaunchKernelGGL(k_sort_local, dim3(kp_cap / 1024, 1, n_img), dim3(256), 0, st, jobs);
+    FSX_LAUNCH_CHECK();
+    int in0 = 1;
+    for (int w = 1024; w < kp_cap; w <<= 1) {
+        hipLaunchKernelGGL(k_merge, dim3(std::min(kp_cap / 256, 1024), 1, n_img), dim3(256), 0, st, jobs, w, in0);
+        FSX_LAUNCH_CHECK();
+        in0 = !in0;
+    }
+    hipLaunchKernelGGL(k_emit, dim3(64, 1, n_img), dim3(256), 0, st, jobs, in0);
+    FSX_LAUNCH_CHECK();
+    if (ev) FSX_HIP(hipEventRecord(ev[5], st));
+    return FSX_OK;
+}
+
+}  // namespace fsx
+
+using namespace fsx;
+
+extern "C" {
+
+int fsx_sift_plan(int H, int W, int* n_oct, int* rows, int* cols) {
+    FSX_REQUIRE(H > 0 && W > 0, "bad image size %dx%d", H, W);
+    SiftPlan p;
+    sift_plan(H, W, &p);
+    *n_oct = p.n_oct;
+    for (int o = 0; o < p.n_oct; o++) {
+        if (rows) rows[o] = p.R[o];
+        if (cols) cols[o] = p.C[o];
+    }
+    return FSX_OK;
+}
+
+int fsx_sift_taps(int layer, float* out, int* len) {
+    FSX_REQUIRE(layer >= 0 && layer < 6, "layer %d out of range", layer);
+    const TapSet& T = taps();
+    *len = T.len[layer];
+    for (int i = 0; i < T.len[layer]; i++) out[i] = T.full[layer][i];
+    return FSX_OK;
+}
+
+int fsx_sift_workspace_bytes(int H, int W, int cand_cap, int kp_cap, size_t* bytes) {
+    FSX_REQUIRE(H >= 7 && W >= 7, "image %dx%d too small", H, W);
+    FSX_REQUIRE(cand_cap > 0 && kp_cap >= 1024 && (kp_cap & (kp_cap - 1)) == 0,
+                "kp_cap must be a power of two >= 1024 (got %d)", kp_cap);
+    SiftPlan p;
+    sift_plan(H, W, &p);
+    *bytes = ws_layout(p, cand_cap, kp_cap).total;
+    return FSX_OK;
+}
+
+int fsx_sift_detect(const uint8_t* img, int H, int W, int ld, double contrast, double edge, int max_kp,
+                    int cand_cap, int kp_cap, void* ws, size_t ws_bytes, float* kp_out, int* n_out,
+                    float* stage_ms, void* stream) {
+    FSX_REQUIRE(img && ws && kp_out && n_out, "null pointer argument");
+    FSX_REQUIRE(H >= 7 && W >= 7 && ld >= W, "bad image geometry %dx%d ld %d", H, W, ld);
+    FSX_REQUIRE(max_kp >= 1, "max_kp must be >= 1");
+    size_t need = 0;
+    int rc = fsx_sift_workspace_bytes(H, W, cand_cap, kp_cap, &need);
+    if (rc) return rc;
+    FSX_REQUIRE(ws_bytes >= need, "workspace too small: %zu < %zu bytes", ws_bytes, need);
+    hipStream_t st = (hipStream_t)stream;
+    SiftPlan p;
+    sift_plan(H, W, &p);
+    SiftJob J;
+    fill_job(&J, p, img, ld, contrast, edge, max_kp, cand_cap, kp_cap, ws, kp_out, n_out);
+    FSX_HIP(hipMemcpyAsync(ws, &J, sizeof(J), hipMemcpyHostToDevice, st));
+    hipEvent_t ev[6];
+    if (stage_ms)
+        for (int i = 0; i < 6; i++) FSX_HIP(hipEventCreate(&ev[i]));
+    rc = run_detect((const SiftJob*)ws, 1, p, kp_cap, st, stage_ms ? ev : nullptr);
+    if (rc) return rc;
+    if (stage_ms) {
+        FSX_HIP(hipEventSynchronize(ev[5]));
+        for (int i = 0; i < 5; i++) FSX_HIP(hipEventElapsedTime(&stage_ms[i], ev[i], ev[i + 1]));
+        for (int i = 0; i < 6; i++) (void)hipEventDestroy(ev[i]);
+    }
+    return FSX_OK;
+}
+
+int fsx_sift_counters(const void* ws, int* out4, void* stream) {
+    FSX_REQUIRE(ws && out4, "null pointer argument");
+    hipStream_t st = (hipStream_t)stream;
+    SiftJob J;
+    FSX_HIP(hipMemcpyAsync(&J, ws, sizeof(J), hipMemcpyDeviceToHost, st));
+    FSX_HIP(hipStreamSynchronize(st));
+    FSX_HIP(hipMemcpyAsync(out4, J.counters, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
+    FSX_HIP(hipStreamSynchronize(st));
+    return FSX_OK;
+}
+
+int fsx_sift_describe(const void* ws, const float* kp, int kp_stride, const long long* sel, int m, float* desc,
+                      uint8_t* u8_out, void* stream) {
+    FSX_REQUIRE(ws && kp && desc, "null pointer argument");
+    FSX_REQUIRE(m >= 0 && kp_stride >= 1, "bad sizes m=%d stride=%d", m, kp_stride);
+    if (m == 0) return FSX_OK;
+    FSX_REQUIRE(sel != nullptr, "null selection");
+    hipStream_t st = (hipStream_t)stream;
+    DescArgs A{kp, kp_stride, sel, m, desc, u8_out};
+    hipLaunchKernelGGL(k_describe, dim3(m), dim3(256), 0, st, (const SiftJob*)ws, A);
+    FSX_LAUNCH_CHECK();
+    return FSX_OK;
+}
+
+}  // extern "C"

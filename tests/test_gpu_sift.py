@@ -1,0 +1,120 @@
+"""GPU parity of the SIFT front end against the CPU oracle and the reference's golden vectors.
+
+Bars (north star): equal keypoint counts; x, y within 0.01 px; size and angle within 1e-3
+relative (angle compared on the circle); descriptors within 1 unit per bin.  The pyramid,
+extrema and refinement arithmetic is bit-exact, so x, y, response, octave and layer are
+expected to be identical to the oracle's bits.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from fieldstitch_amd import synth
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _angle_diff(a, b):
+    d = np.abs(a.astype(np.float64) - b.astype(np.float64)) % 360.0
+    return np.minimum(d, 360.0 - d)
+
+
+def assert_kps_match(k, ref, exact_xy=True):
+    n, nr = len(k["x"]), len(ref["x"])
+    assert n == nr, "keypoint count %d != %d" % (n, nr)
+    if n == 0:
+        return
+    if exact_xy:
+        for key in ("x", "y", "response", "octave", "layer"):
+            np.testing.assert_array_equal(k[key], ref[key], err_msg=key)
+    else:
+        assert np.abs(k["x"] - ref["x"]).max() <= 0.01
+        assert np.abs(k["y"] - ref["y"]).max() <= 0.01
+        np.testing.assert_array_equal(k["octave"], ref["octave"])
+        np.testing.assert_array_equal(k["layer"], ref["layer"])
+    rel = np.abs(k["size"].astype(np.float64) - ref["size"]) / np.abs(ref["size"].astype(np.float64))
+    assert rel.max() <= 1e-3
+    assert (_angle_diff(k["angle"], ref["angle"]) <= np.maximum(1e-3 * np.abs(ref["angle"]), 1e-3)).all()
+
+
+def _golden(name):
+    return np.load(os.path.join(GOLD, "gsift_%s.npz" % name))
+
+
+def _golden_image(name, g):
+    if "img" in g:
+        return g["img"]
+    img1, img2, _ = synth.field_pair(1)
+    return img1 if name == "c1_full_cv" else img2
+
+
+@pytest.mark.parametrize("name", ["tiny_cv", "c1_crop_cv", "c1_crop_pipe", "c1_full_cv", "c1_full2_cv"])
+def test_detect_describe_vs_golden(gpu, name):
+    from fieldstitch_amd import gsift
+    g = _golden(name)
+    img = _golden_image(name, g)
+    print("case", name, img.shape, flush=True)
+    k, st = gsift.detect(img, max_kp=int(g["max_kp"]), contrast=float(g["contrast"]), edge=float(g["edge"]))
+    ref = {key: g["kp_" + key] for key in gsift.KEYS}
+    assert_kps_match(k, ref)
+    d = gsift.describe(st, k, g["sel"])
+    assert np.abs(d - g["desc"]).max() <= 1.0
+
+
+@pytest.mark.parametrize("seed,shape,contrast,edge", [(11, (240, 320), 0.04, 10.0), (12, (301, 417), 0.015, 12.0),
+                                                      (13, (97, 131), 0.015, 12.0)])
+def test_detect_vs_oracle(gpu, seed, shape, contrast, edge):
+    from fieldstitch_amd import gsift
+    from oracle import sift_oracle as so
+    rng = np.random.default_rng(seed)
+    mos = synth.Mosaic(seed=int(rng.integers(1 << 30)))
+    mos.polys = synth.random_polygons(rng, 12, 0, 0, shape[1], shape[0])
+    img = mos.render(np.eye(3)[:2], shape[0], shape[1], rng)
+    print("case seed", seed, shape, flush=True)
+    k, st = gsift.detect(img, max_kp=100000, contrast=contrast, edge=edge)
+    kr, sr = so.detect(img, 100000, contrast, edge)
+    assert_kps_match(k, kr)
+    sel = np.arange(len(k["x"]))
+    d = gsift.describe(st, k, sel)
+    dr = so.describe(sr, kr, sel)
+    assert np.abs(d - dr).max() <= 1.0
+    assert np.abs(d - dr).mean() <= 1e-3
+
+
+def test_rootsift_u8_epilogue(gpu):
+    from fieldstitch_amd import gsift
+    from oracle import sift_oracle as so
+    g = _golden("c1_crop_cv")
+    k, st = gsift.detect(g["img"], max_kp=12000, contrast=0.04, edge=10.0)
+    sel = np.arange(len(k["x"]))
+    d, u8 = gsift.describe(st, k, sel, rootsift_u8=True)
+    ref = so.rootsift_u8(d)
+    assert (np.abs(u8.astype(int) - ref.astype(int)) <= 0).all()
+
+
+def test_deterministic(gpu):
+    from fieldstitch_amd import gsift
+    img, _, _ = synth.field_pair(1)
+    k1, s1 = gsift.detect(img, max_kp=4000, contrast=0.04, edge=10.0)
+    k2, s2 = gsift.detect(img, max_kp=4000, contrast=0.04, edge=10.0)
+    for key in gsift.KEYS:
+        np.testing.assert_array_equal(k1[key], k2[key])
+    sel = np.arange(len(k1["x"]))
+    np.testing.assert_array_equal(gsift.describe(s1, k1, sel), gsift.describe(s2, k2, sel))
+
+
+def test_edge_cases(gpu):
+    from fieldstitch_amd import gsift
+    # flat image: no keypoints, describe of an empty selection
+    k, st = gsift.detect(np.full((64, 80), 128, np.uint8), max_kp=100, contrast=0.04, edge=10.0)
+    assert len(k["x"]) == 0 and k["octave"].dtype == np.int32
+    assert gsift.describe(st, k, []).shape == (0, 128)
+    # max_kp truncation keeps the strongest
+    g = _golden("c1_crop_pipe")
+    k, st = gsift.detect(g["img"], max_kp=50, contrast=0.015, edge=12.0)
+    assert len(k["x"]) == 50
+    np.testing.assert_array_equal(k["response"], g["kp_response"][:50])
+    with pytest.raises(ValueError):
+        gsift.detect(g["img"], gpu=False)

@@ -1,0 +1,118 @@
+"""Pins of the matcher oracle (oracle/match_oracle.py): the FP64 oracle against a brute-force
+loop and the float32 restatement of matching.py, the spatial candidate search against scipy's
+cKDTree, and the tie rules on hand-made cases."""
+import numpy as np
+import pytest
+from scipy.spatial import cKDTree
+
+from oracle import match_oracle as mo
+
+
+def rootsift_like(rng, n):
+    d = rng.gamma(0.6, 1.0, (n, 128))
+    d /= d.sum(1, keepdims=True)
+    return np.clip(np.sqrt(d) * 512, 0, 255).astype(np.uint8)
+
+
+def pair_u8(rng, na, nb, n_true, noise=6):
+    a = rootsift_like(rng, na)
+    b = rootsift_like(rng, nb)
+    qa = rng.permutation(na)[:n_true]
+    tb = rng.permutation(nb)[:n_true]
+    b[tb] = np.clip(a[qa].astype(int) + rng.integers(-noise, noise + 1, (n_true, 128)), 0, 255)
+    return a, b
+
+
+def brute_loop(da, db, ratio):
+    da = np.asarray(da, np.float64)
+    db = np.asarray(db, np.float64)
+    best = {}
+    for q in range(len(da)):
+        d = [float(((da[q] - db[t]) ** 2).sum()) for t in range(len(db))]
+        o = sorted(range(len(db)), key=lambda t: (d[t], t))
+        b1, b2 = d[o[0]], d[o[1]]
+        if np.sqrt(b1) < ratio * np.sqrt(b2):
+            t = o[0]
+            if t not in best or (b1, q) < best[t]:
+                best[t] = (b1, q)
+    ts = sorted(best)
+    return np.array([best[t][1] for t in ts]), np.array(ts)
+
+
+def test_fp64_oracle_vs_loop():
+    rng = np.random.default_rng(0)
+    a, b = pair_u8(rng, 60, 70, 30)
+    q, t, _ = mo.match_fp64(a, b, 0.8)
+    ql, tl = brute_loop(a, b, 0.8)
+    np.testing.assert_array_equal(q, ql)
+    np.testing.assert_array_equal(t, tl)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_f32_restatement_equals_fp64_on_u8(seed):
+    rng = np.random.default_rng(seed)
+    a, b = pair_u8(rng, 400, 500, 200)
+    q, t = mo.ratio_match_bf_f32(a.astype(np.float32), b.astype(np.float32), 0.8)
+    q2, t2, _ = mo.match_fp64(a, b, 0.8)
+    np.testing.assert_array_equal(q, q2)
+    np.testing.assert_array_equal(t, t2)
+    assert len(q) > 100
+
+
+def test_tie_rules():
+    # best tie -> rejected; a unique target claimed by two queries -> smaller distance wins,
+    # equal distances -> lower query index
+    b = np.zeros((4, 128), np.float32)
+    b[0, 0] = 10
+    b[1, 0] = 10
+    b[2, 1] = 100
+    b[3, 2] = 100
+    a = np.zeros((4, 128), np.float32)
+    a[0, 0] = 10            # equidistant to b0 and b1 -> reject
+    a[1, 1] = 100           # exact copy of b2
+    a[2, 1] = 100           # same -> tie on target 2 -> q=1 wins
+    a[3, 2] = 99            # close to b3
+    q, t, _ = mo.match_fp64(a, b, 0.8)
+    assert list(zip(q, t)) == [(1, 2), (3, 3)]
+    q2, t2 = mo.ratio_match_bf_f32(a, b, 0.8)
+    assert list(zip(q2, t2)) == [(1, 2), (3, 3)]
+    # fewer than 2 on a side -> empty (matching.py:88-89)
+    assert len(mo.match_fp64(a[:1], b, 0.8)[0]) == 0
+
+
+def test_guided_candidates_vs_ckdtree():
+    rng = np.random.default_rng(5)
+    xy_b = rng.uniform(0, 500, (3000, 2))
+    pred = rng.uniform(-40, 540, (800, 2))
+    tree = cKDTree(xy_b)
+    for kc in (1, 4, 16, 64):
+        qs, cands = mo.guided_candidates(pred, xy_b, (500, 500), 30.0, kc)
+        inside = np.nonzero((pred[:, 0] > -30) & (pred[:, 0] < 530) & (pred[:, 1] > -30) & (pred[:, 1] < 530))[0]
+        d, idx = tree.query(pred[inside], k=kc, distance_upper_bound=30.0)
+        if kc == 1:
+            idx = idx[:, None]
+        has = (idx < len(xy_b)).any(1)
+        np.testing.assert_array_equal(qs, inside[has])
+        for c, row in zip(cands, idx[has]):
+            np.testing.assert_array_equal(c, np.sort(row[row < len(xy_b)]))
+
+
+def test_guided_single_candidate_always_passes():
+    a = np.zeros((1, 128), np.float32)
+    b = np.full((1, 128), 50, np.float32)
+    q, t, _ = mo.match_fp64(a, b, 0.8, [0], [np.array([0])])
+    assert list(q) == [0] and list(t) == [0]
+    q2, t2 = mo.guided_f32(a, b, [0], [np.array([0])], 0.8)
+    assert list(q2) == [0]
+
+
+def test_guided_f32_equals_fp64_on_u8():
+    rng = np.random.default_rng(9)
+    a, b = pair_u8(rng, 300, 320, 150)
+    xy_b = rng.uniform(0, 400, (320, 2))
+    pred = rng.uniform(0, 400, (300, 2))
+    qs, cands = mo.guided_candidates(pred, xy_b, (400, 400), 60.0, 0)
+    q, t = mo.guided_f32(a, b, qs, cands, 0.8)
+    q2, t2, _ = mo.match_fp64(a, b, 0.8, qs, cands)
+    np.testing.assert_array_equal(q, q2)
+    np.testing.assert_array_equal(t, t2)

@@ -1,0 +1,189 @@
+"""Pins of the SIFT oracle (oracle/sift_oracle.c) before it is trusted as the GPU checker:
+the reference's own CPU outputs (golden vectors from app/planesite/gsift.py), library routines
+(scipy.ndimage.correlate1d / maximum_filter), closed forms and invariants."""
+import os
+
+import numpy as np
+import pytest
+import scipy.ndimage as ndi
+
+from oracle import sift_oracle as so
+from fieldstitch_amd import synth
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+NAMES = ["tiny_cv", "c1_crop_cv", "c1_crop_pipe", "c1_full_cv", "c1_full2_cv"]
+
+
+def golden(name):
+    return np.load(os.path.join(GOLD, "gsift_%s.npz" % name))
+
+
+def golden_image(name, g):
+    if "img" in g:
+        return g["img"]
+    img1, img2, _ = synth.field_pair(1)
+    return img1 if name == "c1_full_cv" else img2
+
+
+def test_golden_inputs_regenerate():
+    import hashlib
+    img1, img2, _ = synth.field_pair(1)
+    for name, img in (("c1_full_cv", img1), ("c1_full2_cv", img2)):
+        assert hashlib.sha256(img.tobytes()).hexdigest() == str(golden(name)["img_sha"])
+
+
+# ---- S2: taps, sigmas (closed form) --------------------------------------------------
+def test_sigmas_and_tap_counts():
+    s = so.sigmas()
+    np.testing.assert_allclose(s, [1.2490, 1.2263, 1.5450, 1.9466, 2.4525, 3.0900], atol=1e-4)
+    assert [len(so.gauss_kernel(x)) for x in s] == [11, 11, 13, 17, 21, 27]
+    for x in s:
+        k = so.gauss_kernel(x).astype(np.float64)
+        assert abs(k.sum() - 1.0) < 1e-6 and np.array_equal(k, k[::-1])
+
+
+# ---- S4: blur == scipy.ndimage.correlate1d(mirror) twice, bit for bit ------------------
+@pytest.mark.parametrize("shape", [(14, 15), (31, 50), (64, 33), (100, 14)])
+def test_blur_bitexact_vs_scipy(shape):
+    rng = np.random.default_rng(shape[0] * 7 + shape[1])
+    img = rng.random(shape).astype(np.float32)
+    for s in so.sigmas():
+        k = so.gauss_kernel(s)
+        ref = ndi.correlate1d(ndi.correlate1d(img, k, axis=0, mode="mirror"), k, axis=1, mode="mirror")
+        np.testing.assert_array_equal(so.blur(img, k), ref)
+
+
+# ---- S3: upsample closed form ------------------------------------------------------------
+def test_up2_closed_form():
+    g = np.array([[0, 255], [51, 102]], np.uint8)
+    a = g.astype(np.float32) * np.float32(1.0 / 255.0)
+    out = so.up2_u8(g)
+    assert out.shape == (4, 4)
+    # corner (0,0): rows clamp -> a00, cols clamp -> a00
+    assert out[0, 0] == a[0, 0]
+    # (1,1): 0.75 of (0.75 a00 + 0.25 a10) + 0.25 of (0.75 a01 + 0.25 a11)
+    t0 = np.float32(0.75) * a[0, 0] + np.float32(0.25) * a[1, 0]
+    t1 = np.float32(0.75) * a[0, 1] + np.float32(0.25) * a[1, 1]
+    assert out[1, 1] == np.float32(0.75) * t0 + np.float32(0.25) * t1
+    # constant image stays constant
+    assert np.all(so.up2_u8(np.full((5, 7), 200, np.uint8)) == np.float32(200) * np.float32(1.0 / 255.0))
+
+
+# ---- S5: octave plan closed form -------------------------------------------------------
+def test_plan_closed_form():
+    p = so.plan(912, 1368)
+    assert p[0] == (1824, 2736) and p[-1] == (15, 22) and len(p) == 8
+    p = so.plan(1978, 2640)
+    assert p[0] == (3956, 5280) and p[-1] == (16, 21) and len(p) == 9
+    assert len(so.plan(989, 1320)) == 8
+
+
+# ---- golden vectors of the reference's CPU path ------------------------------------------
+@pytest.mark.parametrize("name", ["tiny_cv", "c1_crop_cv", "c1_crop_pipe"])
+def test_stages_vs_golden(name):
+    g = golden(name)
+    st = so.State(g["img"])
+    gs = np.array([[float(np.sum(st.gauss(o, i).astype(np.float64))) for i in range(6)]
+                   for o in range(len(st.octaves))])
+    np.testing.assert_array_equal(gs, g["st_g_sums"])
+    ds = np.array([float(np.sum(st.dog(o).astype(np.float64))) for o in range(len(st.octaves))])
+    np.testing.assert_array_equal(ds, g["st_dog_sums"])
+    if g["st_dog_oct0"].size:
+        np.testing.assert_array_equal(st.dog(0), g["st_dog_oct0"])
+    mag = np.concatenate([st.mag(o).ravel() for o in range(len(st.octaves))])[:4096]
+    np.testing.assert_array_equal(mag, g["st_mag_first"])
+    ang = np.concatenate([st.ang(o).ravel() for o in range(len(st.octaves))])[:4096]
+    assert np.abs(ang - g["st_ang_first"]).max() <= 1e-4      # numpy's float32 atan2 ulps
+    F = st.extrema(_contrast(g))
+    np.testing.assert_array_equal(F, g["st_F"])
+    o, l, r, c, X, resp, _ = st.refine(F, _contrast(g), _edge(g))
+    for a, b in ((o, "o"), (l, "l"), (r, "r"), (c, "c"), (X, "X"), (resp, "resp")):
+        np.testing.assert_array_equal(a, g["st_ref_" + b])
+    scl = np.float32(1.6) * np.power(np.float32(2.0), (l.astype(np.float32) + X[:, 2]) / np.float32(3))
+    for i in range(min(64, len(o))):
+        h = st.ori_hist(o[i], l[i], r[i], c[i], g["st_scl"][i])
+        np.testing.assert_allclose(h, g["st_hist"][i], rtol=2e-6, atol=1e-9)
+    assert np.allclose(scl, g["st_scl"], rtol=1e-6)
+
+
+def _contrast(g):
+    # fixtures store float32; the reference ran with the Python floats 0.04 / 0.015
+    return 0.04 if abs(float(g["contrast"]) - 0.04) < 1e-6 else 0.015
+
+
+def _edge(g):
+    return float(g["edge"])
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_detect_describe_vs_golden(name):
+    g = golden(name)
+    img = golden_image(name, g)
+    k, st = so.detect(img, int(g["max_kp"]), _contrast(g), _edge(g))
+    assert len(k["x"]) == len(g["kp_x"])
+    for key in ("x", "y", "response", "octave", "layer"):
+        np.testing.assert_array_equal(k[key], g["kp_" + key])
+    np.testing.assert_allclose(k["size"], g["kp_size"], rtol=1e-6)
+    # angle: 1e-3 relative (numpy's float32 atan2/exp ulps can move a sample across a bin)
+    d = np.abs(k["angle"].astype(np.float64) - g["kp_angle"]) % 360
+    assert (np.minimum(d, 360 - d) <= np.maximum(1e-3 * np.abs(g["kp_angle"]), 1e-3)).all()
+    desc = so.describe(st, k, g["sel"])
+    assert np.abs(desc - g["desc"]).max() <= 1e-2
+
+
+# ---- S9: extrema == scipy maximum/minimum_filter formulation ------------------------------
+def test_extrema_vs_scipy_filters():
+    g = golden("c1_crop_pipe")
+    st = so.State(g["img"])
+    contrast = 0.015
+    pthr = np.float32(0.8 * 0.5 * contrast / 3)
+    masks = []
+    for o in range(len(st.octaves)):
+        D = st.dog(o)
+        R, C = D.shape[1:]
+        mx = ndi.maximum_filter(D, size=3, mode="nearest")
+        mn = ndi.minimum_filter(D, size=3, mode="nearest")
+        ext = ((D >= mx) & (D > pthr)) | ((D <= mn) & (D < -pthr))
+        ext[0] = ext[-1] = False
+        ext[:, :5] = ext[:, R - 5:] = False
+        ext[:, :, :5] = ext[:, :, C - 5:] = False
+        masks.append(ext.ravel())
+    np.testing.assert_array_equal(st.extrema(contrast), np.nonzero(np.concatenate(masks))[0])
+
+
+# ---- S12: orientation peaks closed forms ---------------------------------------------------
+def test_ori_peaks_spike_and_parabola():
+    h = np.zeros(36, np.float32)
+    h[7] = 1.0
+    np.testing.assert_array_equal(so.ori_peaks(h), np.float32([360 - 70]))
+    h[0] = 1.0                        # second equal spike -> two peaks in bin order
+    a = so.ori_peaks(h)
+    assert list(a) == [0.0, np.float32(290.0)]
+    # a smoothed parabola around bin 20 with vertex at 20.3
+    j = np.arange(36)
+    h = np.maximum(0, 100 - 3 * (j - 20.3) ** 2).astype(np.float32)
+    a = so.ori_peaks(h)
+    assert len(a) == 1 and abs(a[0] - (360 - 10 * 20.3)) < 0.5
+    assert len(so.ori_peaks(np.zeros(36, np.float32))) == 0
+
+
+# ---- S14/S15: descriptor invariants --------------------------------------------------------
+def test_descriptor_invariants():
+    g = golden("c1_crop_pipe")
+    k, st = so.detect(g["img"], 12000, 0.015, 12.0)
+    d = so.describe(st, k, np.arange(len(k["x"])))
+    nrm = np.linalg.norm(d.astype(np.float64), axis=1)
+    assert np.all(np.abs(nrm - 512) < 0.05)
+    assert d.min() >= 0
+    # clipping at 0.2 of the pre-clip norm bounds every bin by ~0.2 * 512 / (post-clip norm share)
+    assert d.max() <= 512.0
+
+
+def test_rootsift_closed_form():
+    rng = np.random.default_rng(3)
+    d = (rng.random((50, 128)) * 100).astype(np.float32)
+    d[0] = 0
+    u = so.rootsift_u8(d)
+    ref = d / (np.abs(d).sum(1, keepdims=True) + np.float32(1e-7))
+    ref = np.clip(np.sqrt(ref) * np.float32(512.0), 0, 255).astype(np.uint8)
+    np.testing.assert_array_equal(u, ref)
