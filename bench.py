@@ -1,0 +1,231 @@
+"""Benchmark of the MI355X SIFT front end + guided matcher on the realistic-shape workload.
+
+Workload (one "step"): 8 synthetic nadir photos of 2640x1978 uint8 cropped from one seeded
+field mosaic along a 2x4 flight grid (75 % front / 65 % side overlap, +-3 deg rotation, +-2 %
+scale, noise sigma 2).  SIFT per photo: OpenCV-default thresholds (contrast 0.04, edge 10),
+detect int(8000*2.2) keypoints, grid-spread to 8000, describe, RootSIFT uint8.  Guided
+matching: every overlapping pair, prior = true mapping + N(0, 3 px), 60 px window, top-2,
+ratio 0.8, one query per target.
+
+Prints ONE JSON line.  ``value`` = SIFT images/s with the photos resident in HBM;
+``matching`` holds matched pairs/s of the fast and the plain FP32 GPU forms and whether the
+match set equals the FP64 oracle; ``roofline`` is for the dominant kernel (the pyramid's
+LDS strip blur, HBM-bound); ``cpu_baseline`` times the CPU oracle on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12        # MI355X HBM3E, bytes/s
+NFEAT = 8000
+CONTRAST, EDGE = 0.04, 10.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_flight(seed=2):
+    from fieldstitch_amd import synth
+    return synth.flight(seed=seed, workers=min(8, os.cpu_count() or 1))
+
+
+def pyramid_bytes(H, W):
+    """Algorithmic HBM bytes of the pyramid stage of one image: u8 in + base layer out +
+    per octave pixel 5 x (read G_i, write G_i+1, write D_i) x 4 B + 2 B downsample."""
+    from fieldstitch_amd import gsift
+    pl = gsift.plan(H, W)
+    sumP = sum(r * c for r, c in pl)
+    P0 = pl[0][0] * pl[0][1]
+    return H * W + 4 * P0 + 62 * sumP
+
+
+def sift_bytes(H, W):
+    """Algorithmic bytes of the whole SIFT front end of one image: 118 * sumP + 4 * P0 + H*W
+    (pyramid + downsample + gradient of 3 layers + extrema over 5 DoG)."""
+    from fieldstitch_amd import gsift
+    pl = gsift.plan(H, W)
+    sumP = sum(r * c for r, c in pl)
+    P0 = pl[0][0] * pl[0][1]
+    return H * W + 4 * P0 + 118 * sumP
+
+
+def cpu_sift_worker(args):
+    img, max_kp, nfeat = args
+    os.environ["OMP_NUM_THREADS"] = "1"
+    from oracle import sift_oracle as so
+    from fieldstitch_amd import features
+    t = time.perf_counter()
+    k, st = so.detect(img, max_kp, CONTRAST, EDGE)
+    lst = [features._KP(float(x), float(y), float(r)) for x, y, r in zip(k["x"], k["y"], k["response"])]
+    sel = np.array(features._spread(lst, img.shape[1], img.shape[0], nfeat), np.int64)
+    sel = sel[np.argsort(-k["response"][sel], kind="stable")]
+    d = so.describe(st, k, sel)
+    so.rootsift_u8(d)
+    return time.perf_counter() - t
+
+
+def cpu_baseline(images, n_proc):
+    """The CPU oracle (C restatement of gsift's CPU path) on one photo per process."""
+    import multiprocessing as mp
+    sample = images[:n_proc]
+    ctx = mp.get_context("spawn")
+    t0 = time.perf_counter()
+    with ctx.Pool(len(sample)) as pool:
+        per = pool.map(cpu_sift_worker, [(im, int(NFEAT * 2.2), NFEAT) for im in sample])
+    wall = time.perf_counter() - t0
+    return {"value": len(sample) / wall, "unit": "images/s", "cores": len(sample), "kind": "port",
+            "sample": "%d of the 2640x1978 photos, one per process (C oracle SIFT + spread + describe + "
+                      "RootSIFT), wall %.1f s, per-photo CPU %.1f s" % (len(sample), wall, float(np.mean(per)))}
+
+
+RADIUS, RATIO = 60.0, 0.8
+
+
+def bench_matching(fb, fl, shapes, args):
+    """Guided matching of every overlapping pair on the device-resident features: the MFMA
+    form and the plain FP32 form, timed separately; both checked against the FP64 oracle."""
+    import torch
+    from fieldstitch_amd.matching import PairMatcher
+    from oracle import match_oracle as mo
+    n_det, n_sel = fb.counts()
+    images, host = [], []
+    for i, (H, W) in enumerate(shapes):
+        xy, u8, resp, _ = fb.results(i)
+        xy64 = xy.astype(np.float64)     # reduce 1, zero distortion: und = xy
+        host.append((u8, xy64))
+        images.append((fb.u8[i, :len(xy)].contiguous(), torch.from_numpy(xy64).cuda(), (W, H)))
+    pairs = list(fl.pairs)
+    Hm = [fl.prior[p] for p in pairs]
+    pm = PairMatcher(images, pairs, Hm, radius=RADIUS, kc=0, ratio=RATIO)
+    out = {"pairs": len(pairs), "radius_px": RADIUS, "ratio": RATIO, "kc": "all in window"}
+    results = {}
+    for form in ("mfma", "plain"):
+        for _ in range(max(1, args.warmup)):
+            pm.run(form)
+        torch.cuda.synchronize()
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            pm.run(form, events=evs[k])
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / args.steps
+        contr = float(np.mean([evs[k][1].elapsed_time(evs[k][2]) for k in range(args.steps)]))
+        results[form] = pm.results()
+        out[form] = {"pairs_per_s": len(pairs) / t, "ms_per_step": t * 1e3, "contraction_ms": contr}
+    # exactness: both forms against the FP64 oracle, pair by pair
+    equal_forms = all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+                      for a, b in zip(results["mfma"], results["plain"]))
+    n_equal, n_matches, cand = 0, 0, 0
+    for p, (i, j) in enumerate(pairs):
+        q, t = mo.guided_fp64_fast(host[i][0], host[i][1], host[j][0], host[j][1], Hm[p], (shapes[j][1], shapes[j][0]),
+                                   RADIUS, RATIO)
+        got = results["mfma"][p]
+        n_equal += int(np.array_equal(got[0], q) and np.array_equal(got[1], t))
+        n_matches += len(q)
+    out["match_set_equals_fp64_oracle"] = bool(n_equal == len(pairs) and equal_forms)
+    out["pairs_equal_fp64"] = n_equal
+    out["matches_total"] = n_matches
+    out["speedup_vs_plain_fp32"] = out["plain"]["ms_per_step"] / out["mfma"]["ms_per_step"]
+    out["value"] = out["mfma"]["pairs_per_s"]
+    out["unit"] = "matched pairs/s"
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-procs", type=int, default=8)
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_blur_c2.json"))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding)",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from fieldstitch_amd import features
+
+    t0 = time.time()
+    fl = make_flight()
+    log("flight: %d photos %s, %d pairs, generated in %.1f s" % (len(fl.images), fl.images[0].shape,
+                                                                  len(fl.pairs), time.time() - t0))
+    dev = [torch.from_numpy(im).cuda() for im in fl.images]
+    shapes = [im.shape for im in fl.images]
+    fb = features.FeatureBatch(shapes, nfeat=NFEAT, contrast=CONTRAST, edge=EDGE)
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        fb.run(dev)
+    fb.check_capacity()
+    torch.cuda.synchronize()
+
+    # timed region: K steps of the SIFT front end on all photos
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(8)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        fb.run(dev, events=evs[k])
+    torch.cuda.synchronize()
+    t_sift = (time.perf_counter() - t_start) / args.steps
+    fb.check_capacity()
+    stages = {}
+    for k in range(args.steps):
+        for name, v in features.FeatureBatch.stage_times(evs[k]).items():
+            stages[name] = stages.get(name, 0.0) + v / args.steps
+    n_det, n_sel = fb.counts()
+    n_img = len(fl.images)
+    pyr_b = sum(pyramid_bytes(*s) for s in shapes)
+    sift_b = sum(sift_bytes(*s) for s in shapes)
+    pyr_s = stages["pyramid"] / 1e3
+    achieved = pyr_b / pyr_s
+    traffic = None
+    if os.path.exists(args.traffic):
+        with open(args.traffic) as f:
+            traffic = json.load(f).get("bytes_per_step")
+    res = {
+        "metric": "images/s (SIFT) + matched pairs/s vs plain FP32 GPU form, match set = FP64 oracle",
+        "value": n_img / t_sift,
+        "unit": "images/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_sift * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 blur accumulation)",
+        "data": "synthetic (seeded field mosaic, 8 photos)",
+        "config": {"workload": "8 photos 2640x1978 u8, 2x4 flight grid 75/65 overlap; SIFT contrast 0.04 edge 10, "
+                               "nfeat 8000 (detect 17600), RootSIFT u8",
+                   "photos": n_img, "nfeat": NFEAT, "pairs": len(fl.pairs)},
+        "sift": {"stage_ms": {k: round(v, 4) for k, v in stages.items()},
+                 "keypoints_detected_mean": float(n_det.mean()), "keypoints_selected_mean": float(n_sel.mean()),
+                 "algorithmic_bytes_per_step": sift_b,
+                 "sift_roofline_frac": sift_b / t_sift / HBM_PEAK},
+        "roofline": {"kernel": "k_blur (pyramid: base + 5 layers x octaves, all photos)", "bound": "hbm",
+                     "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "algorithmic_bytes_per_step": pyr_b, "time_ms_per_step": stages["pyramid"]},
+    }
+    res["matching"] = bench_matching(fb, fl, shapes, args)
+    if not args.no_cpu_baseline:
+        n_proc = min(args.cpu_procs, len(fl.images), os.cpu_count() or 1)
+        log("cpu baseline: %d processes" % n_proc)
+        res["cpu_baseline"] = cpu_baseline(fl.images, n_proc)
+        res["cpu_baseline"]["host_cpu_count"] = os.cpu_count()
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -28,6 +28,19 @@ SIGNATURES = {
                         C.POINTER(C.c_float), _vp],
     "fsx_sift_counters": [_vp, C.POINTER(_i), _vp],
     "fsx_sift_describe": [_vp, _vp, _i, _vp, _i, _vp, _vp, _vp],
+    "fsx_sift_table_bytes": [_i, C.POINTER(_sz)],
+    "fsx_sift_features_batch": [_i, C.POINTER(_vp), C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.c_double,
+                                C.c_double, _i, _i, _i, _i, C.POINTER(_vp), C.POINTER(_sz), C.POINTER(_vp), _vp,
+                                C.POINTER(_vp), C.POINTER(_vp), _vp, _sz, C.POINTER(_vp), _vp],
+    "fsx_sift_selection_offset": [_i, _i, _i, _i, C.POINTER(_sz)],
+    "fsx_match_workspace_bytes": [_i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), _i, C.POINTER(_i), C.POINTER(_i),
+                                  C.c_double, C.POINTER(_sz), C.POINTER(C.c_longlong)],
+    "fsx_match_batch": [_i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), _vp, _vp, _i, C.POINTER(_i), C.POINTER(_i),
+                        C.POINTER(_i), C.POINTER(C.c_double), C.c_double, _i, C.c_double, _i, _vp, _sz,
+                        C.POINTER(C.c_longlong), _vp, _vp],
+    "fsx_match_results": [_i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), _i, C.POINTER(_i), C.POINTER(_i),
+                          C.c_double, _vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), _vp],
+    "fsx_mfma_layout_probe": [_vp, _vp],
 }
 
 _lib = None

@@ -77,12 +77,18 @@ struct SiftJob {
     float4* oriented;    // 2 x float4 per oriented keypoint: (x,y,size,angle), (resp, o, l bits, src)
     unsigned long long* keys0;   // sort keys (2 per element) ping
     unsigned long long* keys1;   // pong
-    int* counters;       // [0] n_cand, [1] n_refined, [2] n_oriented, [3] overflow flags
+    int* counters;       // [0] n_cand, [1] n_refined, [2] n_oriented, [3] overflow flags, [4] n selected
     int cand_cap, kp_cap;
     float contrast_f, edge_f, e1sq_f, pthr;
     int max_kp;
     float* kp_out;       // [7][max_kp] x,y,size,angle,response,octave(int),layer(int)
     int* n_out;          // number of keypoints written (device)
+    // feature-batch extension: grid spread of the detected keypoints, then descriptors
+    int nfeat;           // keypoints kept per image (features._spread n)
+    long long* sel;      // [nfeat] selected keypoint indices in final (response) order
+    int* spread_tmp;     // [4 * kp_cap] scratch of the spread kernel
+    float* desc_out;     // [nfeat][128] float32 descriptors (may be null)
+    uint8_t* u8_out;     // [nfeat][128] RootSIFT uint8 (may be null)
 };
 
 struct SiftPlan {

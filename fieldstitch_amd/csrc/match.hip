@@ -1,0 +1,993 @@
+// Descriptor matcher for gfx950: the prior-guided matcher (app/planesite/matching.py:113-152
+// _guided) and the brute-force ratio matcher (matching.py:86-100 _ratio_match_bf), each with
+// the one-query-per-target rule (matching.py:103-110 _unique), batched over many image pairs.
+//
+// Match set definition (every path here returns exactly this list): candidates of query q =
+// the B keypoints strictly inside the prior radius around q's predicted position (squared
+// float64 distance), optionally only the kc spatially nearest (ties: lower index), or all of
+// B for brute force; d = sum_k (a_k - b_k)^2; best = argmin d (ties: lower index), b2 = the
+// second smallest d (+inf with one candidate); accept iff sqrt(b1) < ratio*sqrt(b2) in
+// float64; per target keep the smallest b1, ties to the lower query; output sorted by target.
+// uint8 RootSIFT descriptors make every d an exact integer < 2^24, so the float32 forms and
+// the BF16-input MFMA form (uint8 is exact in bf16, sums exact in float32) are exact.
+//
+// Two forms of the candidate contraction:
+//   plain: one thread per query, FP32 gather + dot products on VALU (the baseline form);
+//   mfma:  queries sorted by predicted grid cell, 32 spatial neighbours per wave, the union of
+//          their candidate cells contracted on MFMA (32x32x16 BF16, FP32 accumulation) with
+//          the prior-window mask and the top-2 selection fused into the epilogue.
+#include "fsx_common.h"
+#include <math.h>
+#include <string.h>
+#include <algorithm>
+#include <vector>
+
+namespace fsx {
+
+struct MImage {
+    long long off;      // first keypoint in the concatenated arrays
+    int n;              // keypoints
+    int W, H;           // image size (inside test on the B side)
+    int gw, gh;         // spatial grid (cells of `radius` px covering [-2r, W+2r) x [-2r, H+2r))
+    long long cell_off; // first cell of this image in the cell arrays
+};
+
+struct MPair {
+    int a, b;           // image indices: queries from a, targets from b
+    int dense;          // 1: brute force over all of b (no prior)
+    int pad;
+    double Hm[9];       // prior mapping a px -> b px (cv2.perspectiveTransform semantics)
+    long long q_off;    // first global query (prefix of n_a)
+    long long t_off;    // first per-target slot (prefix of n_b)
+    long long o_off;    // first output slot (prefix of min(n_a, n_b))
+};
+
+struct MatchBatch {
+    const uint8_t* desc;          // [Ntot][128] RootSIFT uint8
+    const double* xy;             // [Ntot][2] positions (undistorted px)
+    const MImage* img;
+    const MPair* pair;
+    int n_img, n_pair;
+    long long n_query, n_kp, n_cells, n_targets;
+    double radius, r2, ratio;
+    int kc;
+    int pad;
+    // scratch
+    unsigned* norms;              // [Ntot] sum of squares
+    int* cell_start;              // [n_cells + n_img]
+    int* cell_cur;                // [n_cells]
+    int* cell_idx;                // [Ntot] local keypoint indices sorted by cell
+    double* pred;                 // [n_query][2]
+    int* qcell;                   // [n_query] cell of the prediction in b's grid, -1 = not inside
+    int* best;                    // [n_query] accepted best target (local), -1 = none
+    double* b1;                   // [n_query]
+    double* kth;                  // [n_query][2] kc-th nearest (d2, idx) threshold, kc > 0
+    unsigned long long* tkey;     // [n_targets] min b1 bits per target
+    int* tq;                      // [n_targets] min query among equal b1
+    unsigned long long* qkey;     // [2 * n_query] sort keys of the mfma form
+    unsigned long long* qkey2;    // [2 * n_query] merge pong
+    int* qsorted;                 // [n_query] global query ids in (pair, cell) order
+    int* counters;                // [0] valid queries, [1] chunks, [2] overflow flags
+    int* pcount;                  // [n_pair] valid queries per pair
+    int* chunk;                   // [3 * max_chunks] (pair, start, count)
+    int max_chunks;
+    int* out_q;                   // [Σ min(n_a, n_b)]
+    int* out_t;
+    int* out_n;                   // [n_pair]
+    long long sort_cap;           // power of two >= n_query, >= 1024
+};
+
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ int find_pair(const MatchBatch& B, long long g) {
+    int lo = 0, hi = B.n_pair - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (B.pair[mid].q_off <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int find_image(const MatchBatch& B, long long k) {
+    int lo = 0, hi = B.n_img - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (B.img[mid].off <= k) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ void cell_of(const MatchBatch& B, const MImage& im, double x, double y, int& cx, int& cy) {
+    cx = (int)floor((x + 2.0 * B.radius) / B.radius);
+    cy = (int)floor((y + 2.0 * B.radius) / B.radius);
+    cx = cx < 0 ? 0 : (cx >= im.gw ? im.gw - 1 : cx);
+    cy = cy < 0 ? 0 : (cy >= im.gh ? im.gh - 1 : cy);
+}
+
+__global__ void k_m_reset(MatchBatch B) {
+    const long long tid = (long long)blockIdx.x * 256 + threadIdx.x, stride = (long long)gridDim.x * 256;
+    for (long long i = tid; i < B.n_targets; i += stride) {
+        B.tkey[i] = ~0ull;
+        B.tq[i] = 0x7fffffff;
+    }
+    for (long long i = tid; i < B.n_cells; i += stride) B.cell_cur[i] = 0;
+    for (long long i = tid; i < B.n_pair; i += stride) {
+        B.out_n[i] = 0;
+        B.pcount[i] = 0;
+    }
+    if (tid < 4) B.counters[tid] = 0;
+}
+
+// sum of squares of every descriptor (exact integer for uint8) and its grid cell count
+__global__ void k_m_norms_count(MatchBatch B) {
+    const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (k >= B.n_kp) return;
+    const uint4* d = (const uint4*)(B.desc + k * 128);
+    unsigned s = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        uint4 v = d[i];
+        s = __builtin_amdgcn_udot4(v.x, v.x, s, false);
+        s = __builtin_amdgcn_udot4(v.y, v.y, s, false);
+        s = __builtin_amdgcn_udot4(v.z, v.z, s, false);
+        s = __builtin_amdgcn_udot4(v.w, v.w, s, false);
+    }
+    B.norms[k] = s;
+    const int ii = find_image(B, k);
+    const MImage& im = B.img[ii];
+    int cx, cy;
+    cell_of(B, im, B.xy[2 * k], B.xy[2 * k + 1], cx, cy);
+    atomicAdd(&B.cell_cur[im.cell_off + (long long)cy * im.gw + cx], 1);
+}
+
+// one block per image: exclusive scan of the cell counts -> cell_start; cursors reset
+__global__ __launch_bounds__(256) void k_m_grid_scan(MatchBatch B) {
+    __shared__ int s_w[4];
+    const MImage& im = B.img[blockIdx.x];
+    const int nc = im.gw * im.gh;
+    int* start = B.cell_start + im.cell_off + blockIdx.x;   // nc + 1 entries per image
+    int* cnt = B.cell_cur + im.cell_off;
+    int carry = 0;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    for (int base = 0; base < nc; base += 256) {
+        const int i = base + tid;
+        const int v = i < nc ? cnt[i] : 0;
+        int x = v;
+        for (int off = 1; off < 64; off <<= 1) {
+            int y = __shfl_up(x, off);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) s_w[wv] = x;
+        __syncthreads();
+        int pre = 0;
+        for (int w2 = 0; w2 < wv; w2++) pre += s_w[w2];
+        const int tot = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+        if (i < nc) {
+            start[i] = carry + pre + x - v;
+            cnt[i] = 0;
+        }
+        carry += tot;
+        __syncthreads();
+    }
+    if (tid == 0) start[nc] = carry;
+}
+
+__global__ void k_m_grid_fill(MatchBatch B) {
+    const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (k >= B.n_kp) return;
+    const int ii = find_image(B, k);
+    const MImage& im = B.img[ii];
+    int cx, cy;
+    cell_of(B, im, B.xy[2 * k], B.xy[2 * k + 1], cx, cy);
+    const long long c = (long long)cy * im.gw + cx;
+    const int* start = B.cell_start + im.cell_off + ii;
+    const int slot = start[c] + atomicAdd(&B.cell_cur[im.cell_off + c], 1);
+    B.cell_idx[im.off + slot] = (int)(k - im.off);
+}
+
+// prediction (cv2.perspectiveTransform: w = (h6 x + h7 y) + h8, |w| > FLT_EPSILON ? v * (1/w)
+// : 0) and the strict inside test of matching.py:119-120
+__global__ void k_m_predict(MatchBatch B) {
+    const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (g >= B.n_query) return;
+    const int p = find_pair(B, g);
+    const MPair& P = B.pair[p];
+    const MImage& ia = B.img[P.a];
+    const MImage& ib = B.img[P.b];
+    const long long q = g - P.q_off;
+    const double x = B.xy[2 * (ia.off + q)], y = B.xy[2 * (ia.off + q) + 1];
+    double px, py;
+    if (P.dense) {
+        px = 0.0;
+        py = 0.0;
+    } else {
+        const double w = (x * P.Hm[6] + y * P.Hm[7]) + P.Hm[8];
+        if (fabs(w) > 1.1920928955078125e-07) {
+            const double iw = 1.0 / w;
+            px = ((x * P.Hm[0] + y * P.Hm[1]) + P.Hm[2]) * iw;
+            py = ((x * P.Hm[3] + y * P.Hm[4]) + P.Hm[5]) * iw;
+        } else {
+            px = 0.0;
+            py = 0.0;
+        }
+    }
+    B.pred[2 * g] = px;
+    B.pred[2 * g + 1] = py;
+    int cell = -1;
+    if (P.dense) {
+        cell = (ib.n >= 2 && ia.n >= 2) ? 0 : -1;
+    } else if (px > -B.radius && px < ib.W + B.radius && py > -B.radius && py < ib.H + B.radius && ib.n > 0) {
+        int cx, cy;
+        cell_of(B, ib, px, py, cx, cy);
+        cell = cy * ib.gw + cx;
+    }
+    B.qcell[g] = cell;
+    B.best[g] = -1;
+}
+
+// spatial (d2, idx) key comparison for the kc-nearest rule
+__device__ __forceinline__ bool sp_less(double d, int i, double d2, int i2) { return d < d2 || (d == d2 && i < i2); }
+
+// kc-th nearest spatial threshold of every query (kc > 0 only): the candidate set is
+// {in radius and (d2, idx) <= threshold}
+__global__ void k_m_knn(MatchBatch B) {
+    const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (g >= B.n_query) return;
+    const int cell = B.qcell[g];
+    double td = B.r2;
+    int ti = 0x7fffffff;
+    const int p = find_pair(B, g);
+    const MPair& P = B.pair[p];
+    if (cell >= 0 && !P.dense) {
+        constexpr int KMAX = 64;
+        double ld[KMAX];
+        int li[KMAX];
+        int nl = 0;
+        const int kc = B.kc < KMAX ? B.kc : KMAX;
+        const MImage& ib = B.img[P.b];
+        const double px = B.pred[2 * g], py = B.pred[2 * g + 1];
+        const int cx0 = cell % ib.gw, cy0 = cell / ib.gw;
+        const int* start = B.cell_start + ib.cell_off + P.b;
+        for (int cy = max(cy0 - 1, 0); cy <= min(cy0 + 1, ib.gh - 1); cy++)
+            for (int cx = max(cx0 - 1, 0); cx <= min(cx0 + 1, ib.gw - 1); cx++) {
+                const int c = cy * ib.gw + cx;
+                for (int s = start[c]; s < start[c + 1]; s++) {
+                    const int t = B.cell_idx[ib.off + s];
+                    const double dx = B.xy[2 * (ib.off + t)] - px, dy = B.xy[2 * (ib.off + t) + 1] - py;
+                    const double d = dx * dx + dy * dy;
+                    if (!(d < B.r2)) continue;
+                    if (nl == kc && !sp_less(d, t, ld[nl - 1], li[nl - 1])) continue;
+                    int j = nl < kc ? nl++ : kc - 1;
+                    while (j > 0 && sp_less(d, t, ld[j - 1], li[j - 1])) {
+                        ld[j] = ld[j - 1];
+                        li[j] = li[j - 1];
+                        j--;
+                    }
+                    ld[j] = d;
+                    li[j] = t;
+                }
+            }
+        if (nl == kc) {
+            td = ld[kc - 1];
+            ti = li[kc - 1];
+        }
+    }
+    B.kth[2 * g] = td;
+    B.kth[2 * g + 1] = __longlong_as_double((long long)ti);
+}
+
+__device__ __forceinline__ bool in_candidates(const MatchBatch& B, long long g, double d, int t) {
+    if (!(d < B.r2)) return false;
+    if (B.kc <= 0) return true;
+    const double td = B.kth[2 * g];
+    const int ti = (int)__double_as_longlong(B.kth[2 * g + 1]);
+    return d < td || (d == td && t <= ti);
+}
+
+// top-2 by (value, index): v1/i1 best, v2 second smallest value
+__device__ __forceinline__ void top2_push(float d, int t, float& v1, int& i1, float& v2) {
+    if (d < v1 || (d == v1 && t < i1)) {
+        v2 = fminf(v2, v1);
+        v1 = d;
+        i1 = t;
+    } else {
+        v2 = fminf(v2, d);
+    }
+}
+
+// ratio test in float64 + the first phase of the per-target minimum
+__device__ __forceinline__ void accept_query(const MatchBatch& B, long long g, const MPair& P, float v1, int i1, float v2) {
+    if (i1 < 0) return;
+    const double b1 = (double)v1, b2 = (double)v2;
+    if (sqrt(b1) < B.ratio * sqrt(b2)) {
+        B.best[g] = i1;
+        B.b1[g] = b1;
+        atomicMin(&B.tkey[P.t_off + i1], (unsigned long long)__double_as_longlong(b1));
+    }
+}
+
+// the plain FP32 GPU form: one thread per query, candidates gathered from global memory,
+// d = na + nb - 2 a.b with FP32 products on VALU
+__global__ __launch_bounds__(256) void k_m_plain(MatchBatch B) {
+    const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (g >= B.n_query) return;
+    const int cell = B.qcell[g];
+    if (cell < 0) return;
+    const int p = find_pair(B, g);
+    const MPair& P = B.pair[p];
+    const MImage& ia = B.img[P.a];
+    const MImage& ib = B.img[P.b];
+    const long long ka = ia.off + (g - P.q_off);
+    const uint8_t* da = B.desc + ka * 128;
+    const float na = (float)B.norms[ka];
+    float v1 = INFINITY, v2 = INFINITY;
+    int i1 = -1;
+    auto eval = [&](int t) {
+        const uint8_t* db = B.desc + (ib.off + t) * 128;
+        float dot = 0.f;
+        for (int k = 0; k < 128; k++) dot += (float)da[k] * (float)db[k];
+        const float d = fmaxf((na + (float)B.norms[ib.off + t]) - 2.0f * dot, 0.0f);
+        top2_push(d, t, v1, i1, v2);
+    };
+    if (P.dense) {
+        for (int t = 0; t < ib.n; t++) eval(t);
+    } else {
+        const double px = B.pred[2 * g], py = B.pred[2 * g + 1];
+        const int cx0 = cell % ib.gw, cy0 = cell / ib.gw;
+        const int* start = B.cell_start + ib.cell_off + P.b;
+        for (int cy = max(cy0 - 1, 0); cy <= min(cy0 + 1, ib.gh - 1); cy++)
+            for (int cx = max(cx0 - 1, 0); cx <= min(cx0 + 1, ib.gw - 1); cx++) {
+                const int c = cy * ib.gw + cx;
+                for (int s = start[c]; s < start[c + 1]; s++) {
+                    const int t = B.cell_idx[ib.off + s];
+                    const double dx = B.xy[2 * (ib.off + t)] - px, dy = B.xy[2 * (ib.off + t) + 1] - py;
+                    if (in_candidates(B, g, dx * dx + dy * dy, t)) eval(t);
+                }
+            }
+    }
+    accept_query(B, g, P, v1, i1, v2);
+}
+
+// second phase of the per-target minimum: among queries with the minimal b1, the lowest q
+__global__ void k_m_unique2(MatchBatch B) {
+    const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (g >= B.n_query) return;
+    const int t = B.best[g];
+    if (t < 0) return;
+    const int p = find_pair(B, g);
+    const MPair& P = B.pair[p];
+    if ((unsigned long long)__double_as_longlong(B.b1[g]) == B.tkey[P.t_off + t])
+        atomicMin(&B.tq[P.t_off + t], (int)(g - P.q_off));
+}
+
+// one block per pair: winners in target order -> (q, t) lists
+__global__ __launch_bounds__(256) void k_m_compact(MatchBatch B) {
+    __shared__ int s_w[4];
+    const MPair& P = B.pair[blockIdx.x];
+    const int nb = B.img[P.b].n;
+    const int* tq = B.tq + P.t_off;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    int carry = 0;
+    for (int base = 0; base < nb; base += 256) {
+        const int t = base + tid;
+        const int v = (t < nb && tq[t] != 0x7fffffff) ? 1 : 0;
+        int x = v;
+        for (int off = 1; off < 64; off <<= 1) {
+            int y = __shfl_up(x, off);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) s_w[wv] = x;
+        __syncthreads();
+        int pre = 0;
+        for (int w2 = 0; w2 < wv; w2++) pre += s_w[w2];
+        const int tot = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+        if (v) {
+            const long long o = P.o_off + carry + pre + x - 1;
+            B.out_q[o] = tq[t];
+            B.out_t[o] = t;
+        }
+        carry += tot;
+        __syncthreads();
+    }
+    if (tid == 0) B.out_n[blockIdx.x] = carry;
+}
+
+// ------------------------------------------------------------------------------------------
+// MFMA form.  Queries are sorted by (pair, 4x4-cell tile, cell) so that 32 consecutive ones
+// are spatial neighbours; a wave contracts them against the union of their 3x3 candidate
+// cells, 32 candidates per 32x32x16 BF16 MFMA tile (uint8 -> bf16 is exact).
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ unsigned long long qsort_key_hi(int p, int tilekey) {
+    return ((unsigned long long)(unsigned)p << 32) | (unsigned)tilekey;
+}
+
+// sort keys of the queries: (pair, tile-major cell) ; invalid queries sort last
+__global__ void k_m_qkeys(MatchBatch B) {
+    const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (g >= B.sort_cap) return;
+    unsigned long long k0 = ~0ull, k1 = ~0ull;
+    if (g < B.n_query) {
+        const int cell = B.qcell[g];
+        if (cell >= 0) {
+            const int p = find_pair(B, g);
+            const MPair& P = B.pair[p];
+            const MImage& ib = B.img[P.b];
+            int key = 0;
+            if (!P.dense) {
+                const int cx = cell % ib.gw, cy = cell / ib.gw;
+                const int tw = (ib.gw + 3) / 4;
+                key = (((cy >> 2) * tw + (cx >> 2)) << 4) | ((cy & 3) << 2) | (cx & 3);
+            }
+            k0 = qsort_key_hi(p, key);
+            k1 = (unsigned long long)g;
+            atomicAdd(&B.counters[0], 1);
+            atomicAdd(&B.pcount[p], 1);
+        }
+    }
+    B.qkey[2 * g] = k0;
+    B.qkey[2 * g + 1] = k1;
+}
+
+__device__ __forceinline__ bool key_lt(unsigned long long a0, unsigned long long a1, unsigned long long b0,
+                                       unsigned long long b1) {
+    return a0 < b0 || (a0 == b0 && a1 < b1);
+}
+
+// generic key-pair sort: 1024-element bitonic blocks, then rank merges (deterministic)
+__global__ __launch_bounds__(256) void k_keysort_local(unsigned long long* keys, long long n) {
+    __shared__ unsigned long long s0[1024], s1[1024];
+    const long long base = (long long)blockIdx.x * 1024;
+    if (base >= n) return;
+    for (int t = threadIdx.x; t < 1024; t += 256) {
+        long long g = base + t;
+        s0[t] = g < n ? keys[2 * g] : ~0ull;
+        s1[t] = g < n ? keys[2 * g + 1] : ~0ull;
+    }
+    __syncthreads();
+    for (int k = 2; k <= 1024; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int t = threadIdx.x; t < 512; t += 256) {
+                int i = 2 * t - (t & (j - 1));
+                int q = i + j;
+                bool up = (i & k) == 0;
+                unsigned long long a0 = s0[i], a1 = s1[i], b0 = s0[q], b1 = s1[q];
+                bool sw = up ? key_lt(b0, b1, a0, a1) : key_lt(a0, a1, b0, b1);
+                if (sw) {
+                    s0[i] = b0; s1[i] = b1;
+                    s0[q] = a0; s1[q] = a1;
+                }
+            }
+            __syncthreads();
+        }
+    for (int t = threadIdx.x; t < 1024; t += 256) {
+        long long g = base + t;
+        if (g < n) {
+            keys[2 * g] = s0[t];
+            keys[2 * g + 1] = s1[t];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_keysort_merge(const unsigned long long* src, unsigned long long* dst,
+                                                       long long n, long long w) {
+    for (long long p = (long long)blockIdx.x * 256 + threadIdx.x; p < n; p += (long long)gridDim.x * 256) {
+        const unsigned long long x0 = src[2 * p], x1 = src[2 * p + 1];
+        const long long s = (p / (2 * w)) * (2 * w);
+        long long lo, hi, out;
+        if (p < s + w) {
+            lo = s + w;
+            hi = min(s + 2 * w, n);
+            out = p;
+        } else {
+            lo = s;
+            hi = s + w;
+            out = p - w;
+        }
+        long long a = lo, b = hi;
+        while (a < b) {
+            long long m = (a + b) >> 1;
+            if (key_lt(src[2 * m], src[2 * m + 1], x0, x1)) a = m + 1;
+            else b = m;
+        }
+        out += a - lo;
+        dst[2 * out] = x0;
+        dst[2 * out + 1] = x1;
+    }
+}
+
+// chunks of <= 32 sorted queries of one pair (single block): the sorted order is by pair, so
+// pair p's queries start at the prefix sum of the per-pair counts
+__global__ __launch_bounds__(256) void k_m_chunks(MatchBatch B, int sorted_in_qkey) {
+    __shared__ int s_w[8];
+    const unsigned long long* keys = sorted_in_qkey ? B.qkey : B.qkey2;
+    const int nv = B.counters[0];
+    for (int i = threadIdx.x; i < nv; i += 256) B.qsorted[i] = (int)keys[2 * (long long)i + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    int carry_q = 0, carry_c = 0;
+    for (int base = 0; base < B.n_pair; base += 256) {
+        const int p = base + tid;
+        const int cq = p < B.n_pair ? B.pcount[p] : 0;
+        const int cc = (cq + 31) / 32;
+        int xq = cq, xc = cc;
+        for (int off = 1; off < 64; off <<= 1) {
+            int yq = __shfl_up(xq, off), yc = __shfl_up(xc, off);
+            if (lane >= off) {
+                xq += yq;
+                xc += yc;
+            }
+        }
+        if (lane == 63) {
+            s_w[wv] = xq;
+            s_w[4 + wv] = xc;
+        }
+        __syncthreads();
+        int pq = 0, pc = 0;
+        for (int w2 = 0; w2 < wv; w2++) {
+            pq += s_w[w2];
+            pc += s_w[4 + w2];
+        }
+        const int tq = s_w[0] + s_w[1] + s_w[2] + s_w[3], tc = s_w[4] + s_w[5] + s_w[6] + s_w[7];
+        const int qstart = carry_q + pq + xq - cq, cstart = carry_c + pc + xc - cc;
+        for (int c = 0; c < cc; c++) {
+            const int k = cstart + c;
+            if (k < B.max_chunks) {
+                B.chunk[3 * k] = p;
+                B.chunk[3 * k + 1] = qstart + 32 * c;
+                B.chunk[3 * k + 2] = min(32, cq - 32 * c);
+            }
+        }
+        carry_q += tq;
+        carry_c += tc;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        B.counters[1] = carry_c;
+        if (carry_c > B.max_chunks) B.counters[2] |= 1;
+    }
+}
+
+constexpr int MF_MAXCELLS = 1024;   // bitmap of candidate cells per chunk (32 x 32 cells)
+constexpr int MF_MAXCAND = 2048;    // candidate list per chunk held in LDS
+
+__device__ __forceinline__ bf16x8_t u8x8_to_bf16(uint2 v) {
+    bf16x8_t r;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const unsigned byte = ((e < 4 ? v.x : v.y) >> (8 * (e & 3))) & 0xffu;
+        const float f = (float)byte;
+        r[e] = __builtin_bit_cast(__bf16, (unsigned short)(__float_as_uint(f) >> 16));
+    }
+    return r;
+}
+
+__global__ __launch_bounds__(64) void k_m_mfma(MatchBatch B) {
+    __shared__ int s_cand[MF_MAXCAND];
+    __shared__ unsigned s_mark[MF_MAXCELLS / 32];
+    __shared__ int s_q[32], s_cell[32];
+    __shared__ double s_px[32], s_py[32];
+    __shared__ int s_n;
+    __shared__ float s_v1[64], s_v2[64];
+    __shared__ int s_i1[64];
+    const int lane = threadIdx.x;
+    const int nch = min(B.counters[1], B.max_chunks);
+    for (int ch = blockIdx.x; ch < nch; ch += gridDim.x) {
+        const int p = B.chunk[3 * ch], start = B.chunk[3 * ch + 1], cnt = B.chunk[3 * ch + 2];
+        const MPair& P = B.pair[p];
+        const MImage& ia = B.img[P.a];
+        const MImage& ib = B.img[P.b];
+        if (lane < 32) {
+            const int g = lane < cnt ? B.qsorted[start + lane] : -1;
+            s_q[lane] = g;
+            s_cell[lane] = g >= 0 ? B.qcell[g] : -1;
+            s_px[lane] = g >= 0 ? B.pred[2 * (long long)g] : 0.0;
+            s_py[lane] = g >= 0 ? B.pred[2 * (long long)g + 1] : 0.0;
+        }
+        if (lane < MF_MAXCELLS / 32) s_mark[lane] = 0u;
+        if (lane == 0) s_n = 0;
+        __syncthreads();
+        // candidate cells: union of the 3x3 neighbourhoods, inside a 32x32 window of cells
+        int cx_lo = 1 << 30, cy_lo = 1 << 30;
+        for (int i = 0; i < cnt; i++) {
+            const int c = s_cell[i];
+            cx_lo = min(cx_lo, c % ib.gw);
+            cy_lo = min(cy_lo, c / ib.gw);
+        }
+        cx_lo = max(cx_lo - 1, 0);
+        cy_lo = max(cy_lo - 1, 0);
+        bool overflow = false;
+        if (P.dense) {
+            // brute force: every keypoint of b is a candidate (streamed in list-sized passes)
+        } else if (lane < cnt) {
+            const int c = s_cell[lane];
+            const int cx0 = c % ib.gw, cy0 = c / ib.gw;
+            const int* cs = B.cell_start + ib.cell_off + P.b;
+            for (int cy = max(cy0 - 1, 0); cy <= min(cy0 + 1, ib.gh - 1); cy++)
+                for (int cx = max(cx0 - 1, 0); cx <= min(cx0 + 1, ib.gw - 1); cx++) {
+                    const int bx = cx - cx_lo, by = cy - cy_lo;
+                    const int cc = cy * ib.gw + cx;
+                    // a cell larger than the LDS list, or a window wider than 32x32 cells:
+                    // the queries of this chunk take the per-query path below
+                    if (bx < 32 && by < 32 && cs[cc + 1] - cs[cc] <= MF_MAXCAND) atomicOr(&s_mark[by], 1u << bx);
+                    else overflow = true;
+                }
+        }
+        const bool any_over = __ballot(overflow) != 0ull;
+        __syncthreads();
+        // per query lane-pair state: lanes l and l+32 handle query column l%32
+        float v1 = INFINITY, v2 = INFINITY;
+        int i1 = -1;
+        const int qc = lane & 31;
+        const int gq = s_q[qc];
+        // query operand: 8 K-steps of 8 bf16 (k = 16 s + (lane/32)*8 + e)
+        bf16x8_t qop[8];
+        float qn = 0.f;
+        if (gq >= 0) {
+            const long long ka = ia.off + (gq - P.q_off);
+            const uint2* qa = (const uint2*)(B.desc + ka * 128);
+#pragma unroll
+            for (int s = 0; s < 8; s++) qop[s] = u8x8_to_bf16(qa[2 * s + (lane >> 5)]);
+            qn = (float)B.norms[ka];
+        } else {
+#pragma unroll
+            for (int s = 0; s < 8; s++) qop[s] = u8x8_to_bf16(make_uint2(0, 0));
+        }
+        const double qpx = gq >= 0 ? s_px[qc] : 0.0, qpy = gq >= 0 ? s_py[qc] : 0.0;
+        // candidate passes
+        int pass_c = 0;           // dense: next b index; guided: next cell (row-major in window)
+        bool more = true;
+        while (more) {
+            // build the candidate list of this pass
+            if (lane == 0) {
+                int n = 0;
+                if (P.dense) {
+                    while (pass_c < ib.n && n < MF_MAXCAND) s_cand[n++] = pass_c++;
+                    more = pass_c < ib.n;
+                } else if (!any_over) {
+                    const int* cs = B.cell_start + ib.cell_off + P.b;
+                    more = false;
+                    while (pass_c < 32 * 32) {
+                        const int by = pass_c >> 5, bx = pass_c & 31;
+                        if (s_mark[by] & (1u << bx)) {
+                            const int c = (cy_lo + by) * ib.gw + cx_lo + bx;
+                            const int a0 = cs[c], a1 = cs[c + 1];
+                            if (n + (a1 - a0) > MF_MAXCAND) {
+                                more = true;
+                                break;
+                            }
+                            for (int s = a0; s < a1; s++) s_cand[n++] = B.cell_idx[ib.off + s];
+                        }
+                        pass_c++;
+                    }
+                } else {
+                    more = false;
+                }
+                s_n = n;
+            }
+            more = __shfl(more ? 1 : 0, 0) != 0;
+            pass_c = __shfl(pass_c, 0);
+            __syncthreads();
+            const int n = s_n;
+            for (int t0 = 0; t0 < n; t0 += 32) {
+                // candidate operand: row = lane%32 of this tile
+                const int tr = t0 + (lane & 31);
+                const int tc = tr < n ? s_cand[tr] : -1;
+                const uint2* cb = (const uint2*)(B.desc + (ib.off + (tc >= 0 ? tc : 0)) * 128);
+                f32x16_t acc = {};
+#pragma unroll
+                for (int s = 0; s < 8; s++) {
+                    const bf16x8_t a = u8x8_to_bf16(tc >= 0 ? cb[2 * s + (lane >> 5)] : make_uint2(0, 0));
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, qop[s], acc, 0, 0, 0);
+                }
+                // epilogue: acc[i*4+j] = dot(cand row i*8 + (lane/32)*4 + j, query col lane%32)
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const int r = t0 + i * 8 + (lane >> 5) * 4 + j;
+                        if (r >= n || gq < 0) continue;
+                        const int t = s_cand[r];
+                        if (!P.dense) {
+                            const double dx = B.xy[2 * (ib.off + t)] - qpx, dy = B.xy[2 * (ib.off + t) + 1] - qpy;
+                            if (!in_candidates(B, gq, dx * dx + dy * dy, t)) continue;
+                        }
+                        const float d = fmaxf((qn + (float)B.norms[ib.off + t]) - 2.0f * acc[i * 4 + j], 0.0f);
+                        top2_push(d, t, v1, i1, v2);
+                    }
+            }
+            __syncthreads();
+        }
+        // merge the two half-waves of each query column
+        s_v1[lane] = v1;
+        s_v2[lane] = v2;
+        s_i1[lane] = i1;
+        __syncthreads();
+        if (lane < 32 && gq >= 0) {
+            const float w1 = s_v1[lane + 32], w2 = s_v2[lane + 32];
+            const int j1 = s_i1[lane + 32];
+            if (j1 >= 0) top2_push(w1, j1, v1, i1, v2);
+            v2 = fminf(v2, w2);
+            if (any_over) {
+                // candidate cells spread beyond the 32x32 window: evaluate this query alone
+                v1 = INFINITY; v2 = INFINITY; i1 = -1;
+                const long long ka = ia.off + (gq - P.q_off);
+                const uint8_t* da = B.desc + ka * 128;
+                const int c = s_cell[lane];
+                const int cx0 = c % ib.gw, cy0 = c / ib.gw;
+                const int* cs = B.cell_start + ib.cell_off + P.b;
+                for (int cy = max(cy0 - 1, 0); cy <= min(cy0 + 1, ib.gh - 1); cy++)
+                    for (int cx = max(cx0 - 1, 0); cx <= min(cx0 + 1, ib.gw - 1); cx++) {
+                        const int cc = cy * ib.gw + cx;
+                        for (int s = cs[cc]; s < cs[cc + 1]; s++) {
+                            const int t = B.cell_idx[ib.off + s];
+                            const double dx = B.xy[2 * (ib.off + t)] - qpx, dy = B.xy[2 * (ib.off + t) + 1] - qpy;
+                            if (!in_candidates(B, gq, dx * dx + dy * dy, t)) continue;
+                            const uint8_t* db = B.desc + (ib.off + t) * 128;
+                            unsigned dot = 0;
+                            for (int k = 0; k < 128; k += 4)
+                                dot = __builtin_amdgcn_udot4(*(const unsigned*)(da + k), *(const unsigned*)(db + k), dot, false);
+                            const float d = fmaxf((qn + (float)B.norms[ib.off + t]) - 2.0f * (float)dot, 0.0f);
+                            top2_push(d, t, v1, i1, v2);
+                        }
+                    }
+            }
+            accept_query(B, gq, P, v1, i1, v2);
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Host side
+
+static inline size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct MatchPlanHost {
+    std::vector<MImage> img;
+    std::vector<MPair> pair;
+    long long n_kp = 0, n_query = 0, n_cells = 0, n_targets = 0, n_out = 0, sort_cap = 1024;
+    int max_chunks = 0;
+    size_t off_img, off_pair, off_norms, off_cstart, off_ccur, off_cidx, off_pred, off_qcell, off_best, off_b1,
+        off_kth, off_tkey, off_tq, off_qkey, off_qkey2, off_qsorted, off_cnt, off_pcount, off_chunk, off_outq, off_outt,
+        off_outn, off_batch, total;
+};
+
+static int match_plan(int n_img, const int* n_kp, const int* W, const int* H, int n_pair, const int* pa,
+                      const int* pb, const int* dense, const double* Hm, double radius, MatchPlanHost& M) {
+    FSX_REQUIRE(n_img >= 1 && n_pair >= 1, "need images and pairs");
+    FSX_REQUIRE(radius > 0, "radius must be > 0");
+    M.img.resize(n_img);
+    long long off = 0, coff = 0;
+    for (int i = 0; i < n_img; i++) {
+        FSX_REQUIRE(n_kp[i] >= 0 && W[i] > 0 && H[i] > 0, "bad image %d", i);
+        MImage& m = M.img[i];
+        m.off = off;
+        m.n = n_kp[i];
+        m.W = W[i];
+        m.H = H[i];
+        m.gw = (int)ceil((W[i] + 4.0 * radius) / radius);
+        m.gh = (int)ceil((H[i] + 4.0 * radius) / radius);
+        m.cell_off = coff;
+        off += n_kp[i];
+        coff += (long long)m.gw * m.gh;
+    }
+    M.n_kp = off;
+    M.n_cells = coff;
+    M.pair.resize(n_pair);
+    long long qo = 0, to = 0, oo = 0, chunks = 0;
+    for (int p = 0; p < n_pair; p++) {
+        FSX_REQUIRE(pa[p] >= 0 && pa[p] < n_img && pb[p] >= 0 && pb[p] < n_img, "bad pair %d", p);
+        MPair& P = M.pair[p];
+        memset(&P, 0, sizeof(P));
+        P.a = pa[p];
+        P.b = pb[p];
+        P.dense = dense ? dense[p] : 0;
+        for (int k = 0; k < 9; k++) P.Hm[k] = Hm ? Hm[9 * p + k] : (k % 4 == 0 ? 1.0 : 0.0);
+        P.q_off = qo;
+        P.t_off = to;
+        P.o_off = oo;
+        qo += M.img[P.a].n;
+        to += M.img[P.b].n;
+        oo += std::min(M.img[P.a].n, M.img[P.b].n);
+        chunks += (M.img[P.a].n + 31) / 32 + 1;
+    }
+    M.n_query = qo;
+    M.n_targets = to;
+    M.n_out = oo;
+    M.max_chunks = (int)std::max(chunks, 1LL);
+    long long cap = 1024;
+    while (cap < qo) cap <<= 1;
+    M.sort_cap = cap;
+    size_t o = 0;
+    M.off_batch = o; o = a256(o + sizeof(MatchBatch));
+    M.off_img = o; o = a256(o + sizeof(MImage) * n_img);
+    M.off_pair = o; o = a256(o + sizeof(MPair) * n_pair);
+    M.off_norms = o; o = a256(o + 4 * (size_t)std::max(M.n_kp, 1LL));
+    M.off_cstart = o; o = a256(o + 4 * (size_t)(M.n_cells + n_img));
+    M.off_ccur = o; o = a256(o + 4 * (size_t)M.n_cells);
+    M.off_cidx = o; o = a256(o + 4 * (size_t)std::max(M.n_kp, 1LL));
+    M.off_pred = o; o = a256(o + 16 * (size_t)std::max(qo, 1LL));
+    M.off_qcell = o; o = a256(o + 4 * (size_t)std::max(qo, 1LL));
+    M.off_best = o; o = a256(o + 4 * (size_t)std::max(qo, 1LL));
+    M.off_b1 = o; o = a256(o + 8 * (size_t)std::max(qo, 1LL));
+    M.off_kth = o; o = a256(o + 16 * (size_t)std::max(qo, 1LL));
+    M.off_tkey = o; o = a256(o + 8 * (size_t)std::max(to, 1LL));
+    M.off_tq = o; o = a256(o + 4 * (size_t)std::max(to, 1LL));
+    M.off_qkey = o; o = a256(o + 16 * (size_t)cap);
+    M.off_qkey2 = o; o = a256(o + 16 * (size_t)cap);
+    M.off_qsorted = o; o = a256(o + 4 * (size_t)std::max(qo, 1LL));
+    M.off_cnt = o; o = a256(o + 16 * 4);
+    M.off_pcount = o; o = a256(o + 4 * (size_t)n_pair);
+    M.off_chunk = o; o = a256(o + 12 * (size_t)M.max_chunks);
+    M.off_outq = o; o = a256(o + 4 * (size_t)std::max(oo, 1LL));
+    M.off_outt = o; o = a256(o + 4 * (size_t)std::max(oo, 1LL));
+    M.off_outn = o; o = a256(o + 4 * (size_t)n_pair);
+    M.total = o;
+    return FSX_OK;
+}
+
+static MatchBatch batch_ptrs(const MatchPlanHost& M, char* ws, const uint8_t* desc, const double* xy, double radius,
+                             int kc, double ratio) {
+    MatchBatch b;
+    memset(&b, 0, sizeof(b));
+    b.desc = desc;
+    b.xy = xy;
+    b.img = (const MImage*)(ws + M.off_img);
+    b.pair = (const MPair*)(ws + M.off_pair);
+    b.n_img = (int)M.img.size();
+    b.n_pair = (int)M.pair.size();
+    b.n_query = M.n_query;
+    b.n_kp = M.n_kp;
+    b.n_cells = M.n_cells;
+    b.n_targets = M.n_targets;
+    b.radius = radius;
+    b.r2 = radius * radius;
+    b.ratio = ratio;
+    b.kc = kc;
+    b.norms = (unsigned*)(ws + M.off_norms);
+    b.cell_start = (int*)(ws + M.off_cstart);
+    b.cell_cur = (int*)(ws + M.off_ccur);
+    b.cell_idx = (int*)(ws + M.off_cidx);
+    b.pred = (double*)(ws + M.off_pred);
+    b.qcell = (int*)(ws + M.off_qcell);
+    b.best = (int*)(ws + M.off_best);
+    b.b1 = (double*)(ws + M.off_b1);
+    b.kth = (double*)(ws + M.off_kth);
+    b.tkey = (unsigned long long*)(ws + M.off_tkey);
+    b.tq = (int*)(ws + M.off_tq);
+    b.qkey = (unsigned long long*)(ws + M.off_qkey);
+    b.qkey2 = (unsigned long long*)(ws + M.off_qkey2);
+    b.qsorted = (int*)(ws + M.off_qsorted);
+    b.counters = (int*)(ws + M.off_cnt);
+    b.pcount = (int*)(ws + M.off_pcount);
+    b.chunk = (int*)(ws + M.off_chunk);
+    b.max_chunks = M.max_chunks;
+    b.out_q = (int*)(ws + M.off_outq);
+    b.out_t = (int*)(ws + M.off_outt);
+    b.out_n = (int*)(ws + M.off_outn);
+    b.sort_cap = M.sort_cap;
+    return b;
+}
+
+static inline unsigned blocks_for(long long n) { return (unsigned)std::max(1LL, (n + 255) / 256); }
+
+}  // namespace fsx
+
+using namespace fsx;
+
+extern "C" {
+
+int fsx_match_workspace_bytes(int n_img, const int* n_kp, const int* W, const int* H, int n_pair, const int* pa,
+                              const int* pb, double radius, size_t* bytes, long long* out_capacity) {
+    FSX_REQUIRE(bytes, "null pointer argument");
+    MatchPlanHost M;
+    int rc = match_plan(n_img, n_kp, W, H, n_pair, pa, pb, nullptr, nullptr, radius, M);
+    if (rc) return rc;
+    *bytes = M.total;
+    if (out_capacity) *out_capacity = M.n_out;
+    return FSX_OK;
+}
+
+int fsx_match_batch(int n_img, const int* n_kp, const int* W, const int* H, const uint8_t* desc_dev,
+                    const double* xy_dev, int n_pair, const int* pa, const int* pb, const int* dense,
+                    const double* Hm, double radius, int kc, double ratio, int form, void* ws, size_t ws_bytes,
+                    long long* out_offsets, void* stage_events_v, void* stream) {
+    FSX_REQUIRE(desc_dev && xy_dev && ws && pa && pb, "null pointer argument");
+    FSX_REQUIRE(form == 0 || form == 1, "form must be 0 (plain FP32) or 1 (MFMA)");
+    FSX_REQUIRE(kc >= 0 && kc <= 64, "kc must be in [0, 64]");
+    MatchPlanHost M;
+    int rc = match_plan(n_img, n_kp, W, H, n_pair, pa, pb, dense, Hm, radius, M);
+    if (rc) return rc;
+    FSX_REQUIRE(ws_bytes >= M.total, "match workspace too small: %zu < %zu", ws_bytes, M.total);
+    hipStream_t st = (hipStream_t)stream;
+    char* w = (char*)ws;
+    MatchBatch b = batch_ptrs(M, w, desc_dev, xy_dev, radius, kc, ratio);
+    FSX_HIP(hipMemcpyAsync(w + M.off_img, M.img.data(), sizeof(MImage) * M.img.size(), hipMemcpyHostToDevice, st));
+    FSX_HIP(hipMemcpyAsync(w + M.off_pair, M.pair.data(), sizeof(MPair) * M.pair.size(), hipMemcpyHostToDevice, st));
+    if (out_offsets)
+        for (int p = 0; p < n_pair; p++) out_offsets[p] = M.pair[p].o_off;
+    hipEvent_t* ev = (hipEvent_t*)stage_events_v;
+    if (ev) FSX_HIP(hipEventRecord(ev[0], st));
+    const long long nq = M.n_query;
+    hipLaunchKernelGGL(k_m_reset, dim3(1024), dim3(256), 0, st, b);
+    if (M.n_kp > 0) hipLaunchKernelGGL(k_m_norms_count, dim3(blocks_for(M.n_kp)), dim3(256), 0, st, b);
+    hipLaunchKernelGGL(k_m_grid_scan, dim3(n_img), dim3(256), 0, st, b);
+    if (M.n_kp > 0) hipLaunchKernelGGL(k_m_grid_fill, dim3(blocks_for(M.n_kp)), dim3(256), 0, st, b);
+    if (nq > 0) hipLaunchKernelGGL(k_m_predict, dim3(blocks_for(nq)), dim3(256), 0, st, b);
+    if (nq > 0 && kc > 0) hipLaunchKernelGGL(k_m_knn, dim3(blocks_for(nq)), dim3(256), 0, st, b);
+    FSX_LAUNCH_CHECK();
+    if (ev) FSX_HIP(hipEventRecord(ev[1], st));
+    if (nq > 0) {
+        if (form == 0) {
+            hipLaunchKernelGGL(k_m_plain, dim3(blocks_for(nq)), dim3(256), 0, st, b);
+        } else {
+            hipLaunchKernelGGL(k_m_qkeys, dim3(blocks_for(M.sort_cap)), dim3(256), 0, st, b);
+            hipLaunchKernelGGL(k_keysort_local, dim3((unsigned)(M.sort_cap / 1024)), dim3(256), 0, st, b.qkey,
+                               M.sort_cap);
+            int in_q = 1;
+            for (long long wdt = 1024; wdt < M.sort_cap; wdt <<= 1) {
+                hipLaunchKernelGGL(k_keysort_merge, dim3((unsigned)std::min(M.sort_cap / 256, 2048LL)), dim3(256), 0,
+                                   st, in_q ? b.qkey : b.qkey2, in_q ? b.qkey2 : b.qkey, M.sort_cap, wdt);
+                in_q = !in_q;
+            }
+            hipLaunchKernelGGL(k_m_chunks, dim3(1), dim3(256), 0, st, b, in_q);
+            hipLaunchKernelGGL(k_m_mfma, dim3((unsigned)std::min<long long>(M.max_chunks, 8192)), dim3(64), 0, st, b);
+        }
+        FSX_LAUNCH_CHECK();
+    }
+    if (ev) FSX_HIP(hipEventRecord(ev[2], st));
+    if (nq > 0) hipLaunchKernelGGL(k_m_unique2, dim3(blocks_for(nq)), dim3(256), 0, st, b);
+    hipLaunchKernelGGL(k_m_compact, dim3(n_pair), dim3(256), 0, st, b);
+    FSX_LAUNCH_CHECK();
+    if (ev) FSX_HIP(hipEventRecord(ev[3], st));
+    return FSX_OK;
+}
+
+// Copies the results of the last fsx_match_batch on `ws` to host arrays (synchronises):
+// counts[n_pair], then q/t of all pairs at the offsets fsx_match_batch reported.
+int fsx_match_results(int n_img, const int* n_kp, const int* W, const int* H, int n_pair, const int* pa,
+                      const int* pb, double radius, const void* ws, int* counts, int* q, int* t, int* flags,
+                      void* stream) {
+    MatchPlanHost M;
+    int rc = match_plan(n_img, n_kp, W, H, n_pair, pa, pb, nullptr, nullptr, radius, M);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    const char* w = (const char*)ws;
+    FSX_HIP(hipMemcpyAsync(counts, w + M.off_outn, 4 * (size_t)n_pair, hipMemcpyDeviceToHost, st));
+    if (q) FSX_HIP(hipMemcpyAsync(q, w + M.off_outq, 4 * (size_t)M.n_out, hipMemcpyDeviceToHost, st));
+    if (t) FSX_HIP(hipMemcpyAsync(t, w + M.off_outt, 4 * (size_t)M.n_out, hipMemcpyDeviceToHost, st));
+    if (flags) FSX_HIP(hipMemcpyAsync(flags, w + M.off_cnt, 16, hipMemcpyDeviceToHost, st));
+    FSX_HIP(hipStreamSynchronize(st));
+    return FSX_OK;
+}
+
+// In-bounds check of the 32x32x16 BF16 MFMA operand/result lane layout this file assumes:
+// A[i][k] = i + 32*k... computed for fixed small integer inputs; out: 64 lanes x 16 floats.
+__global__ void k_mfma_layout_probe(float* out) {
+    const int lane = threadIdx.x;
+    bf16x8_t a, b;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const int k = (lane >> 5) * 8 + e;
+        const float av = (float)((lane & 31) + (k & 3));        // A[row=lane%32][k]
+        const float bv = (float)(((lane & 31) * 3 + k) & 7);     // B[k][col=lane%32]
+        a[e] = __builtin_bit_cast(__bf16, (unsigned short)(__float_as_uint(av) >> 16));
+        b[e] = __builtin_bit_cast(__bf16, (unsigned short)(__float_as_uint(bv) >> 16));
+    }
+    f32x16_t acc = {};
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 16; r++) out[lane * 16 + r] = acc[r];
+}
+
+int fsx_mfma_layout_probe(float* out_dev, void* stream) {
+    FSX_REQUIRE(out_dev, "null pointer argument");
+    hipLaunchKernelGGL(k_mfma_layout_probe, dim3(1), dim3(64), 0, (hipStream_t)stream, out_dev);
+    FSX_LAUNCH_CHECK();
+    return FSX_OK;
+}
+
+}  // extern "C"

@@ -8,6 +8,7 @@
 #include "fsx_common.h"
 #include <math.h>
 #include <string.h>
+#include <algorithm>
 #include <vector>
 
 namespace fsx {
@@ -599,7 +600,7 @@ __global__ __launch_bounds__(256) void k_emit(const SiftJob* __restrict__ jobs, 
 
 __global__ void k_reset(const SiftJob* __restrict__ jobs) {
     const SiftJob& J = jobs[blockIdx.x];
-    if (threadIdx.x < 4) J.counters[threadIdx.x] = 0;
+    if (threadIdx.x < 8) J.counters[threadIdx.x] = 0;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -628,20 +629,18 @@ __device__ __forceinline__ float wave_pairwise8(float v_lane_j_partial, int lane
     return ((r0 + r1) + (r2 + r3)) + ((r4 + r5) + (r6 + r7));
 }
 
-__global__ __launch_bounds__(256) void k_describe(const SiftJob* __restrict__ job, DescArgs A) {
-    __shared__ unsigned long long s_h[HIST_SIZE];
-    __shared__ float s_d[128];
-    const SiftJob& J = *job;
-    const int q = blockIdx.x;
-    if (q >= A.m) return;
-    const long long k = A.sel[q];
-    const int* kpi = (const int*)A.kp;
-    const int o = kpi[5 * (long long)A.stride + k], li = kpi[6 * (long long)A.stride + k];
+// one keypoint k of the arrays kp[7][stride], by a 256-thread block; s_h: HIST_SIZE u64,
+// s_d: 128 floats of LDS
+__device__ void describe_kp(const SiftJob& J, const float* __restrict__ kpa, int stride, long long k,
+                            float* __restrict__ desc_row, uint8_t* __restrict__ u8_row,
+                            unsigned long long* s_h, float* s_d) {
+    const int* kpi = (const int*)kpa;
+    const int o = kpi[5 * (long long)stride + k], li = kpi[6 * (long long)stride + k];
     const int R = J.oct[o].R, C = J.oct[o].C;
     const double f = 1.0 / (ldexp(1.0, o) * 0.5);
-    const double px = (double)A.kp[k] * f, py = (double)A.kp[(long long)A.stride + k] * f;
-    const double scl = (double)A.kp[2 * (long long)A.stride + k] * f * 0.5;
-    float ori = 360.0f - A.kp[3 * (long long)A.stride + k];
+    const double px = (double)kpa[k] * f, py = (double)kpa[(long long)stride + k] * f;
+    const double scl = (double)kpa[2 * (long long)stride + k] * f * 0.5;
+    float ori = 360.0f - kpa[3 * (long long)stride + k];
     if (fabsf(ori - 360.0f) < 1e-4f) ori = 0.0f;
     long long rad = (long long)rint(((3.0 * scl) * 1.4142135623730951) * 5 * 0.5);
     const long long rlim = (long long)sqrt((double)((long long)R * R + (long long)C * C));
@@ -719,6 +718,7 @@ __global__ __launch_bounds__(256) void k_describe(const SiftJob* __restrict__ jo
         float d0 = s_d[lane], d1 = s_d[lane + 64];
         d0 = d0 < thr ? d0 : thr;
         d1 = d1 < thr ? d1 : thr;
+        __builtin_amdgcn_wave_barrier();
         s_d[lane] = d0;
         s_d[lane + 64] = d1;
         __builtin_amdgcn_wave_barrier();
@@ -734,9 +734,12 @@ __global__ __launch_bounds__(256) void k_describe(const SiftJob* __restrict__ jo
         const float nrm2 = sqrtf(wave_pairwise8(part, lane));
         const float sc = 512.0f / (nrm2 > 1e-7f ? nrm2 : 1e-7f);
         const float e0 = d0 * sc, e1 = d1 * sc;
-        A.desc[(long long)q * 128 + lane] = e0;
-        A.desc[(long long)q * 128 + lane + 64] = e1;
-        if (A.u8) {
+        if (desc_row) {
+            desc_row[lane] = e0;
+            desc_row[lane + 64] = e1;
+        }
+        if (u8_row) {
+            __builtin_amdgcn_wave_barrier();
             s_d[lane] = e0;
             s_d[lane + 64] = e1;
             __builtin_amdgcn_wave_barrier();
@@ -750,10 +753,152 @@ __global__ __launch_bounds__(256) void k_describe(const SiftJob* __restrict__ jo
             float u0 = sqrtf(e0 / l1) * 512.0f, u1 = sqrtf(e1 / l1) * 512.0f;
             u0 = u0 < 0.f ? 0.f : (u0 > 255.f ? 255.f : u0);
             u1 = u1 < 0.f ? 0.f : (u1 > 255.f ? 255.f : u1);
-            A.u8[(long long)q * 128 + lane] = (uint8_t)u0;
-            A.u8[(long long)q * 128 + lane + 64] = (uint8_t)u1;
+            u8_row[lane] = (uint8_t)u0;
+            u8_row[lane + 64] = (uint8_t)u1;
         }
     }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_describe(const SiftJob* __restrict__ job, DescArgs A) {
+    __shared__ unsigned long long s_h[HIST_SIZE];
+    __shared__ float s_d[128];
+    const int q = blockIdx.x;
+    if (q >= A.m) return;
+    describe_kp(*job, A.kp, A.stride, A.sel[q], A.desc + (long long)q * 128, A.u8 ? A.u8 + (long long)q * 128 : nullptr,
+                s_h, s_d);
+}
+
+// feature batch: descriptor q of image blockIdx.z for its spread selection, in final order
+__global__ __launch_bounds__(256) void k_describe_sel(const SiftJob* __restrict__ jobs) {
+    __shared__ unsigned long long s_h[HIST_SIZE];
+    __shared__ float s_d[128];
+    const SiftJob& J = jobs[blockIdx.z];
+    const int q = blockIdx.x;
+    if (q >= J.counters[4]) return;
+    describe_kp(J, J.kp_out, J.max_kp, J.sel[q], J.desc_out ? J.desc_out + (long long)q * 128 : nullptr,
+                J.u8_out ? J.u8_out + (long long)q * 128 : nullptr, s_h, s_d);
+}
+
+// ------------------------------------------------------------------------------------------
+// Grid spread of the detected keypoints (features.py:38-57 _spread, then the response sort
+// of features.py:126).  One 256-thread block per image.  With N detected (already sorted by
+// response, ties by detection order) and n wanted: cell = 12x9 grid cell of (x, y); the
+// keypoints of a cell are ranked in detection order; ranks < ceil(n/108) are kept in (cell,
+// rank) order, then the best of the rest fill up to n; finally a stable sort by -response.
+
+__device__ int block_exclusive_scan(int v, int* s_tmp, int& total) {
+    // 256 threads; s_tmp: 256 ints
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    int x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        int y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) s_tmp[wv] = x;
+    __syncthreads();
+    int wsum = 0;
+    for (int w2 = 0; w2 < wv; w2++) wsum += s_tmp[w2];
+    total = s_tmp[0] + s_tmp[1] + s_tmp[2] + s_tmp[3];
+    __syncthreads();
+    return wsum + x - v;
+}
+
+__global__ __launch_bounds__(256) void k_spread(const SiftJob* __restrict__ jobs) {
+    __shared__ int s_cnt[108];
+    __shared__ int s_start[109];
+    __shared__ int s_tmp[256];
+    const SiftJob& J = jobs[blockIdx.z];
+    const int N = *J.n_out, n = J.nfeat, stride = J.max_kp;
+    const int tid = threadIdx.x;
+    if (N <= n) {
+        for (int i = tid; i < N; i += 256) J.sel[i] = i;
+        if (tid == 0) J.counters[4] = N;
+        return;
+    }
+    const float* x = J.kp_out;
+    const float* y = J.kp_out + stride;
+    const float* resp = J.kp_out + 4 * (long long)stride;
+    int* cell = J.spread_tmp;
+    int* rank = cell + J.kp_cap;
+    int* spos = rank + J.kp_cap;
+    int* sbefore = spos + J.kp_cap;
+    const float Wf = (float)J.W, Hf = (float)J.H;
+    for (int i = tid; i < N; i += 256) {
+        int cx = (int)((x[i] / Wf) * 12.0f), cy = (int)((y[i] / Hf) * 9.0f);
+        cx = cx < 0 ? 0 : (cx > 11 ? 11 : cx);
+        cy = cy < 0 ? 0 : (cy > 8 ? 8 : cy);
+        cell[i] = cy * 12 + cx;
+    }
+    if (tid < 108) s_cnt[tid] = 0;
+    __syncthreads();
+    if (tid < 64) {
+        const int lane = tid;
+        for (int base = 0; base < N; base += 64) {
+            const int i = base + lane;
+            const int c = i < N ? cell[i] : -1;
+            int before = 0;
+            bool last = true;
+            for (int j = 0; j < 64; j++) {
+                const int cj = __shfl(c, j);
+                if (cj == c && j < lane) before++;
+                if (cj == c && j > lane) last = false;
+            }
+            int r = 0;
+            if (i < N) r = s_cnt[c] + before;
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_s_waitcnt(0);
+            if (i < N) {
+                rank[i] = r;
+                if (last) s_cnt[c] = r + 1;
+            }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_s_waitcnt(0);
+        }
+    }
+    __syncthreads();
+    const int cap = (n + 107) / 108;
+    if (tid == 0) {
+        s_start[0] = 0;
+        for (int c = 0; c < 108; c++) s_start[c + 1] = s_start[c] + min(s_cnt[c], cap);
+    }
+    __syncthreads();
+    const int K = s_start[108];
+    // positions in the spread list: kept ones by (cell, rank), the rest after them in
+    // detection order; selected = position < n
+    int carry = 0;
+    for (int base = 0; base < N; base += 256) {
+        const int i = base + tid;
+        const bool kept = i < N && rank[i] < cap;
+        int tot;
+        const int ex = block_exclusive_scan((i < N && !kept) ? 1 : 0, s_tmp, tot);
+        if (i < N) spos[i] = kept ? s_start[cell[i]] + rank[i] : K + carry + ex;
+        carry += tot;
+    }
+    __syncthreads();
+    carry = 0;
+    for (int base = 0; base < N; base += 256) {
+        const int i = base + tid;
+        const bool selq = i < N && spos[i] < n;
+        int tot;
+        const int ex = block_exclusive_scan(selq ? 1 : 0, s_tmp, tot);
+        if (i < N) sbefore[i] = carry + ex;
+        carry += tot;
+    }
+    __syncthreads();
+    for (int i = tid; i < N; i += 256) {
+        const int p = spos[i];
+        if (p >= n) continue;
+        const float ri = resp[i];
+        int s0 = i, e0 = i + 1;
+        while (s0 > 0 && resp[s0 - 1] == ri) s0--;
+        while (e0 < N && resp[e0] == ri) e0++;
+        int r = 0;
+        for (int j = s0; j < e0; j++)
+            if (spos[j] < n && spos[j] < p) r++;
+        J.sel[sbefore[s0] + r] = i;
+    }
+    if (tid == 0) J.counters[4] = n;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -762,7 +907,7 @@ __global__ __launch_bounds__(256) void k_describe(const SiftJob* __restrict__ jo
 static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct WsLayout {
-    size_t job, counters, DF, G0, G123, G4, cand, refined, refined_src, oriented, keys0, keys1, total;
+    size_t job, counters, DF, G0, G123, G4, cand, refined, refined_src, oriented, keys0, keys1, sel, spread, total;
 };
 
 static WsLayout ws_layout(const SiftPlan& p, int cand_cap, int kp_cap) {
@@ -780,6 +925,8 @@ static WsLayout ws_layout(const SiftPlan& p, int cand_cap, int kp_cap) {
     L.oriented = off; off = align256(off + (size_t)kp_cap * 32);
     L.keys0 = off; off = align256(off + (size_t)kp_cap * 16);
     L.keys1 = off; off = align256(off + (size_t)kp_cap * 16);
+    L.sel = off; off = align256(off + (size_t)kp_cap * 8);
+    L.spread = off; off = align256(off + (size_t)kp_cap * 16);
     L.total = off;
     return L;
 }
@@ -817,6 +964,8 @@ static int fill_job(SiftJob* J, const SiftPlan& p, const uint8_t* img, int ld, d
     J->keys0 = (unsigned long long*)(b + L.keys0);
     J->keys1 = (unsigned long long*)(b + L.keys1);
     J->counters = (int*)(b + L.counters);
+    J->sel = (long long*)(b + L.sel);
+    J->spread_tmp = (int*)(b + L.spread);
     J->cand_cap = cand_cap;
     J->kp_cap = kp_cap;
     // gsift.py:295 prefilter threshold is float64 then compared in float32; contrast and
@@ -855,52 +1004,94 @@ static int launch_layer(const SiftJob* jobs, int n_img, int o, int layer, int R,
     return FSX_OK;
 }
 
-// Enqueue the whole detect pipeline for n_img images of one shape (jobs: device array).
-static int run_detect(const SiftJob* jobs, int n_img, const SiftPlan& p, int kp_cap, hipStream_t st,
-                      hipEvent_t* ev) {
+// A run of same-shape images inside the job table (pyramid and extrema grids depend on shape).
+struct Group {
+    const SiftJob* jobs;
+    int n;
+    SiftPlan plan;
+};
+
+// Enqueue the pipeline for the whole job table: pyramid and extrema per shape group, the
+// count-driven stages over all images at once; with `features`, also the grid spread and the
+// descriptors of the selection.  ev (optional): 8 events bracketing the stages.
+static int run_pipeline(const SiftJob* all, int n_all, const Group* groups, int n_groups, int kp_cap, int nfeat_max,
+                        bool features, hipStream_t st, hipEvent_t* ev) {
     const TapSet& T = taps();
     if (T.r2[0] != 5) {
         set_error("unexpected base blur half-width %d", T.r2[0]);
         return FSX_EINVAL;
     }
     if (ev) FSX_HIP(hipEventRecord(ev[0], st));
-    hipLaunchKernelGGL(k_reset, dim3(n_img), dim3(64), 0, st, jobs);
-    {
-        dim3 grid((p.C[0] + TW - 1) / TW, (p.R[0] + TH - 1) / TH, n_img);
-        hipLaunchKernelGGL((k_blur<5, true>), grid, dim3(256), 0, st, jobs, 0, 0, T.h[0]);
+    hipLaunchKernelGGL(k_reset, dim3(n_all), dim3(64), 0, st, all);
+    for (int gi = 0; gi < n_groups; gi++) {
+        const Group& g = groups[gi];
+        const SiftPlan& p = g.plan;
+        dim3 grid((p.C[0] + TW - 1) / TW, (p.R[0] + TH - 1) / TH, g.n);
+        hipLaunchKernelGGL((k_blur<5, true>), grid, dim3(256), 0, st, g.jobs, 0, 0, T.h[0]);
         FSX_LAUNCH_CHECK();
+        for (int o = 0; o < p.n_oct; o++)
+            for (int layer = 1; layer <= N_LAYERS + 2; layer++) {
+                int rc = launch_layer(g.jobs, g.n, o, layer, p.R[o], p.C[o], st);
+                if (rc) return rc;
+            }
     }
-    for (int o = 0; o < p.n_oct; o++)
-        for (int layer = 1; layer <= N_LAYERS + 2; layer++) {
-            int rc = launch_layer(jobs, n_img, o, layer, p.R[o], p.C[o], st);
-            if (rc) return rc;
-        }
     if (ev) FSX_HIP(hipEventRecord(ev[1], st));
-    for (int o = 0; o < p.n_oct; o++) {
-        dim3 grid((p.C[o] + EX_TW - 1) / EX_TW, (p.R[o] + EX_TH - 1) / EX_TH, n_img);
-        hipLaunchKernelGGL(k_extrema, grid, dim3(256), 0, st, jobs, o);
-        FSX_LAUNCH_CHECK();
+    for (int gi = 0; gi < n_groups; gi++) {
+        const Group& g = groups[gi];
+        const SiftPlan& p = g.plan;
+        for (int o = 0; o < p.n_oct; o++) {
+            dim3 grid((p.C[o] + EX_TW - 1) / EX_TW, (p.R[o] + EX_TH - 1) / EX_TH, g.n);
+            hipLaunchKernelGGL(k_extrema, grid, dim3(256), 0, st, g.jobs, o);
+            FSX_LAUNCH_CHECK();
+        }
     }
     if (ev) FSX_HIP(hipEventRecord(ev[2], st));
-    hipLaunchKernelGGL(k_refine, dim3(256, 1, n_img), dim3(256), 0, st, jobs);
+    hipLaunchKernelGGL(k_refine, dim3(256, 1, n_all), dim3(256), 0, st, all);
     FSX_LAUNCH_CHECK();
     if (ev) FSX_HIP(hipEventRecord(ev[3], st));
-    hipLaunchKernelGGL(k_orient, dim3(4096, 1, n_img), dim3(64), 0, st, jobs);
+    hipLaunchKernelGGL(k_orient, dim3(4096, 1, n_all), dim3(64), 0, st, all);
     FSX_LAUNCH_CHECK();
     if (ev) FSX_HIP(hipEventRecord(ev[4], st));
-    hipLaunchKernelGGL(k_sort_local, dim3(kp_cap / 1024, 1, n_img), dim3(256), 0, st, jobs);
+    hipLaunchKernelGGL(k_sort_local, dim3(kp_cap / 1024, 1, n_all), dim3(256), 0, st, all);
     FSX_LAUNCH_CHECK();
     int in0 = 1;
     for (int w = 1024; w < kp_cap; w <<= 1) {
-        hipLaunchKernelGGL(k_merge, dim3(std::min(kp_cap / 256, 1024), 1, n_img), dim3(256), 0, st, jobs, w, in0);
+        hipLaunchKernelGGL(k_merge, dim3(std::min(kp_cap / 256, 1024), 1, n_all), dim3(256), 0, st, all, w, in0);
         FSX_LAUNCH_CHECK();
         in0 = !in0;
     }
-    hipLaunchKernelGGL(k_emit, dim3(64, 1, n_img), dim3(256), 0, st, jobs, in0);
+    hipLaunchKernelGGL(k_emit, dim3(64, 1, n_all), dim3(256), 0, st, all, in0);
     FSX_LAUNCH_CHECK();
     if (ev) FSX_HIP(hipEventRecord(ev[5], st));
+    if (features) {
+        hipLaunchKernelGGL(k_spread, dim3(1, 1, n_all), dim3(256), 0, st, all);
+        FSX_LAUNCH_CHECK();
+        if (ev) FSX_HIP(hipEventRecord(ev[6], st));
+        if (nfeat_max > 0) {
+            hipLaunchKernelGGL(k_describe_sel, dim3(nfeat_max, 1, n_all), dim3(256), 0, st, all);
+            FSX_LAUNCH_CHECK();
+        }
+        if (ev) FSX_HIP(hipEventRecord(ev[7], st));
+    }
     return FSX_OK;
 }
+
+struct EventSet {
+    hipEvent_t ev[8];
+    int n = 0;
+    ~EventSet() {
+        for (int i = 0; i < n; i++) (void)hipEventDestroy(ev[i]);
+    }
+    int create(int k) {
+        for (; n < k; n++) FSX_HIP(hipEventCreate(&ev[n]));
+        return FSX_OK;
+    }
+    int read(float* ms, int k) {
+        FSX_HIP(hipEventSynchronize(ev[k]));
+        for (int i = 0; i < k; i++) FSX_HIP(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+        return FSX_OK;
+    }
+};
 
 }  // namespace fsx
 
@@ -954,16 +1145,12 @@ int fsx_sift_detect(const uint8_t* img, int H, int W, int ld, double contrast, d
     SiftJob J;
     fill_job(&J, p, img, ld, contrast, edge, max_kp, cand_cap, kp_cap, ws, kp_out, n_out);
     FSX_HIP(hipMemcpyAsync(ws, &J, sizeof(J), hipMemcpyHostToDevice, st));
-    hipEvent_t ev[6];
-    if (stage_ms)
-        for (int i = 0; i < 6; i++) FSX_HIP(hipEventCreate(&ev[i]));
-    rc = run_detect((const SiftJob*)ws, 1, p, kp_cap, st, stage_ms ? ev : nullptr);
+    EventSet E;
+    if (stage_ms && E.create(6)) return FSX_EHIP;
+    Group g{(const SiftJob*)ws, 1, p};
+    rc = run_pipeline((const SiftJob*)ws, 1, &g, 1, kp_cap, 0, false, st, stage_ms ? E.ev : nullptr);
     if (rc) return rc;
-    if (stage_ms) {
-        FSX_HIP(hipEventSynchronize(ev[5]));
-        for (int i = 0; i < 5; i++) FSX_HIP(hipEventElapsedTime(&stage_ms[i], ev[i], ev[i + 1]));
-        for (int i = 0; i < 6; i++) (void)hipEventDestroy(ev[i]);
-    }
+    if (stage_ms && E.read(stage_ms, 5)) return FSX_EHIP;
     return FSX_OK;
 }
 
@@ -988,6 +1175,71 @@ int fsx_sift_describe(const void* ws, const float* kp, int kp_stride, const long
     DescArgs A{kp, kp_stride, sel, m, desc, u8_out};
     hipLaunchKernelGGL(k_describe, dim3(m), dim3(256), 0, st, (const SiftJob*)ws, A);
     FSX_LAUNCH_CHECK();
+    return FSX_OK;
+}
+
+int fsx_sift_table_bytes(int n_img, size_t* bytes) {
+    FSX_REQUIRE(n_img >= 1, "n_img must be >= 1");
+    *bytes = (size_t)n_img * sizeof(SiftJob);
+    return FSX_OK;
+}
+
+int fsx_sift_features_batch(int n_img, const uint8_t* const* imgs, const int* H, const int* W, const int* ld,
+                            double contrast, double edge, int max_kp, int nfeat, int cand_cap, int kp_cap,
+                            void* const* ws, const size_t* ws_bytes, float* const* kp_out, int* n_out_dev,
+                            float* const* desc_out, uint8_t* const* u8_out, void* table_dev, size_t table_bytes,
+                            void* const* stage_events, void* stream) {
+    FSX_REQUIRE(n_img >= 1 && imgs && H && W && ld && ws && ws_bytes && kp_out && n_out_dev && table_dev,
+                "null pointer argument");
+    FSX_REQUIRE(max_kp >= 1 && nfeat >= 0 && max_kp <= kp_cap, "need 1 <= max_kp <= kp_cap (max_kp %d kp_cap %d)",
+                max_kp, kp_cap);
+    FSX_REQUIRE(table_bytes >= (size_t)n_img * sizeof(SiftJob), "job table too small");
+    hipStream_t st = (hipStream_t)stream;
+    // group images by shape; the table holds each group contiguously
+    std::vector<int> order(n_img);
+    for (int i = 0; i < n_img; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return H[a] != H[b] ? H[a] < H[b] : W[a] < W[b];
+    });
+    std::vector<SiftJob> jobs(n_img);
+    std::vector<Group> groups;
+    SiftJob* dev = (SiftJob*)table_dev;
+    for (int k = 0; k < n_img; k++) {
+        const int i = order[k];
+        FSX_REQUIRE(imgs[i] && ws[i] && kp_out[i], "null pointer for image %d", i);
+        FSX_REQUIRE(H[i] >= 7 && W[i] >= 7 && ld[i] >= W[i], "bad geometry for image %d", i);
+        size_t need = 0;
+        int rc = fsx_sift_workspace_bytes(H[i], W[i], cand_cap, kp_cap, &need);
+        if (rc) return rc;
+        FSX_REQUIRE(ws_bytes[i] >= need, "workspace %d too small: %zu < %zu bytes", i, ws_bytes[i], need);
+        SiftPlan p;
+        sift_plan(H[i], W[i], &p);
+        SiftJob& J = jobs[k];
+        fill_job(&J, p, imgs[i], ld[i], contrast, edge, max_kp, cand_cap, kp_cap, ws[i], kp_out[i], n_out_dev + i);
+        J.nfeat = nfeat;
+        J.desc_out = desc_out ? desc_out[i] : nullptr;
+        J.u8_out = u8_out ? u8_out[i] : nullptr;
+        if (groups.empty() || groups.back().plan.H != H[i] || groups.back().plan.W != W[i])
+            groups.push_back(Group{dev + k, 0, p});
+        groups.back().n++;
+    }
+    for (int k = 0; k < n_img; k++)
+        FSX_HIP(hipMemcpyAsync(ws[order[k]], &jobs[k], sizeof(SiftJob), hipMemcpyHostToDevice, st));
+    FSX_HIP(hipMemcpyAsync(table_dev, jobs.data(), (size_t)n_img * sizeof(SiftJob), hipMemcpyHostToDevice, st));
+    hipEvent_t ev[8];
+    if (stage_events)
+        for (int i = 0; i < 8; i++) ev[i] = (hipEvent_t)stage_events[i];
+    return run_pipeline(dev, n_img, groups.data(), (int)groups.size(), kp_cap, nfeat, true, st,
+                        stage_events ? ev : nullptr);
+}
+
+int fsx_sift_selection_offset(int H, int W, int cand_cap, int kp_cap, size_t* off) {
+    size_t total;
+    int rc = fsx_sift_workspace_bytes(H, W, cand_cap, kp_cap, &total);
+    if (rc) return rc;
+    SiftPlan p;
+    sift_plan(H, W, &p);
+    *off = ws_layout(p, cand_cap, kp_cap).sel;
     return FSX_OK;
 }
 

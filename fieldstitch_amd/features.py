@@ -1,0 +1,207 @@
+"""Feature extraction front end (replaces the GPU side of app/planesite/features.py).
+
+* ``sift_gpu(g2, nfeat)`` — drop-in of features.sift_gpu (features.py:115-129): detect
+  ``int(nfeat*2.2)`` keypoints, grid-spread to ``nfeat``, sort by response, describe.
+* ``_spread`` / ``_save`` — the reference's grid spread (features.py:38-57) and RootSIFT npz
+  writer (features.py:74-86), same formats.
+* ``FeatureBatch`` — the whole flight on the GPU without host round trips: one call detects,
+  spreads and describes every image (device-side spread, RootSIFT uint8 epilogue), with the
+  pyramid launched once per (octave, layer) for all images of a shape.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import _lib, gsift
+
+
+class _KP:
+    __slots__ = ("pt", "response")
+
+    def __init__(self, x, y, r):
+        self.pt = (x, y)
+        self.response = r
+
+
+def _spread(kps, W, H, n, gx=12, gy=9):
+    """Keep n keypoints spread over a gx x gy grid, best responses per cell first
+    (features.py:38-57).  Ties in response among the fill-up keypoints keep detection order."""
+    if len(kps) <= n:
+        return list(range(len(kps)))
+    resp = np.array([k.response for k in kps], np.float32)
+    xy = np.array([k.pt for k in kps], np.float32)
+    cx = np.clip((xy[:, 0] / W * gx).astype(int), 0, gx - 1)
+    cy = np.clip((xy[:, 1] / H * gy).astype(int), 0, gy - 1)
+    cell = cy * gx + cx
+    order = np.lexsort((-resp, cell))
+    cs = cell[order]
+    first = np.searchsorted(cs, cs, side="left")
+    rank = np.arange(len(cs)) - first
+    cap = int(np.ceil(n / (gx * gy)))
+    sel = order[rank < cap]
+    if len(sel) < n:
+        rest = np.setdiff1d(np.arange(len(kps)), sel)
+        rest = rest[np.argsort(-resp[rest], kind="stable")][: n - len(sel)]
+        sel = np.concatenate([sel, rest])
+    return sel[:n].tolist()
+
+
+def sift_gpu(g2, nfeat, gpu=True):
+    """Same outputs as features.sift_gpu: (xy f32 (n,2), desc f32 (n,128) or None, resp f32 (n,),
+    describe time)."""
+    h, w = g2.shape[:2]
+    kps, state = gsift.detect(g2, gpu=gpu, max_kp=int(nfeat * 2.2))
+    n = len(kps["x"])
+    if n < 10:
+        return np.zeros((0, 2), np.float32), None, np.zeros(0, np.float32), 0.0
+    t = time.perf_counter()
+    lst = [_KP(float(x), float(y), float(r)) for x, y, r in zip(kps["x"], kps["y"], kps["response"])]
+    sel = np.array(_spread(lst, w, h, nfeat), np.int64)
+    sel = sel[np.argsort(-kps["response"][sel], kind="stable")]
+    desc = gsift.describe(state, kps, sel, budget=4e6)
+    xy = np.c_[kps["x"][sel], kps["y"][sel]].astype(np.float32)
+    return xy, desc, kps["response"][sel].astype(np.float32), time.perf_counter() - t
+
+
+def rootsift_u8(desc):
+    """features.py:80-82 (host numpy): L1-normalise, sqrt, x512, clip, truncate to uint8."""
+    desc = desc / (np.abs(desc).sum(1, keepdims=True) + 1e-7)
+    desc = np.sqrt(desc)
+    return np.clip(desc * 512.0, 0, 255).astype(np.uint8)
+
+
+def _save(out_npz, xy, desc, resp, w, h, desc_u8=None):
+    """RootSIFT-encode descriptors and write the feature file (features.py:74-86 format).
+    ``desc_u8`` (already encoded on the GPU) skips the host encoding."""
+    if (desc is None and desc_u8 is None) or len(xy) < 10:
+        np.savez(out_npz, xy=np.zeros((0, 2), np.float32), desc=np.zeros((0, 128), np.uint8),
+                 resp=np.zeros(0, np.float32), red_wh=np.array([w, h], np.int32))
+        return 0
+    u8 = desc_u8 if desc_u8 is not None else rootsift_u8(desc)
+    np.savez(out_npz, xy=np.asarray(xy, np.float32), desc=u8, resp=np.asarray(resp, np.float32),
+             red_wh=np.array([w, h], np.int32))
+    return len(xy)
+
+
+class FeatureBatch:
+    """Device-resident SIFT features for a list of uint8 images (torch tensors on the GPU).
+
+    ``run()`` enqueues detect -> spread -> describe (+RootSIFT uint8) for every image in one
+    C-ABI call; outputs stay on the device until ``results()``.  Images of equal shape share
+    their pyramid launches."""
+
+    def __init__(self, shapes, nfeat=8000, max_kp=None, contrast=0.04, edge=10.0, device="cuda",
+                 float_desc=False, caps=None):
+        import torch
+        self.torch = torch
+        self.shapes = [tuple(int(v) for v in s) for s in shapes]
+        self.n = len(self.shapes)
+        self.nfeat = int(nfeat)
+        self.max_kp = int(max_kp if max_kp is not None else int(nfeat * 2.2))
+        self.contrast = float(contrast)
+        self.edge = float(edge)
+        Hm = max(s[0] for s in self.shapes)
+        Wm = max(s[1] for s in self.shapes)
+        self.cand_cap, self.kp_cap = caps if caps else gsift.default_caps(Hm, Wm, self.max_kp)
+        self.device = device
+        self.float_desc = float_desc
+        self._alloc()
+
+    def _alloc(self):
+        torch = self.torch
+        dev = self.device
+        self.ws, self.ws_bytes = [], []
+        for (H, W) in self.shapes:
+            nb = C.c_size_t()
+            _lib.call("fsx_sift_workspace_bytes", H, W, self.cand_cap, self.kp_cap, C.byref(nb))
+            self.ws.append(torch.empty(nb.value, dtype=torch.uint8, device=dev))
+            self.ws_bytes.append(nb.value)
+        self.kp = torch.empty((self.n, 7, self.max_kp), dtype=torch.float32, device=dev)
+        self.n_out = torch.zeros(self.n, dtype=torch.int32, device=dev)
+        self.u8 = torch.empty((self.n, max(self.nfeat, 1), 128), dtype=torch.uint8, device=dev)
+        self.desc = (torch.empty((self.n, max(self.nfeat, 1), 128), dtype=torch.float32, device=dev)
+                     if self.float_desc else None)
+        tb = C.c_size_t()
+        _lib.call("fsx_sift_table_bytes", self.n, C.byref(tb))
+        self.table = torch.empty(tb.value, dtype=torch.uint8, device=dev)
+        self.table_bytes = tb.value
+        n = self.n
+        P = C.c_void_p
+        self._c_ws = (P * n)(*[t.data_ptr() for t in self.ws])
+        self._c_wsb = (C.c_size_t * n)(*self.ws_bytes)
+        self._c_kp = (P * n)(*[self.kp[i].data_ptr() for i in range(n)])
+        self._c_u8 = (P * n)(*[self.u8[i].data_ptr() for i in range(n)])
+        self._c_desc = (P * n)(*[self.desc[i].data_ptr() for i in range(n)]) if self.desc is not None else None
+        self._c_H = (C.c_int * n)(*[s[0] for s in self.shapes])
+        self._c_W = (C.c_int * n)(*[s[1] for s in self.shapes])
+
+    STAGES = ("pyramid", "extrema", "refine", "orientation", "select", "spread", "describe")
+
+    def run(self, imgs, events=None, stream=None):
+        """Enqueue the feature pipeline for device images (list of uint8 (H, W) tensors).
+        ``events``: 8 torch.cuda.Event(enable_timing=True) recorded around the stages."""
+        n = self.n
+        if len(imgs) != n:
+            raise ValueError("expected %d images" % n)
+        for t, s in zip(imgs, self.shapes):
+            if tuple(t.shape) != s or t.dtype != self.torch.uint8 or not t.is_cuda:
+                raise ValueError("image shape/dtype/device mismatch")
+        P = C.c_void_p
+        c_img = (P * n)(*[t.data_ptr() for t in imgs])
+        c_ld = (C.c_int * n)(*[t.stride(0) for t in imgs])
+        st = _lib.stream_handle(stream)
+        ev = None
+        if events is not None:
+            for e in events:
+                if not e.cuda_event:     # torch creates the HIP event on first record
+                    e.record()
+            ev = (P * 8)(*[e.cuda_event for e in events])
+        _lib.call("fsx_sift_features_batch", n, c_img, self._c_H, self._c_W, c_ld, self.contrast, self.edge,
+                  self.max_kp, self.nfeat, self.cand_cap, self.kp_cap, self._c_ws, self._c_wsb, self._c_kp,
+                  P(self.n_out.data_ptr()), self._c_desc, self._c_u8, P(self.table.data_ptr()), self.table_bytes,
+                  ev, P(st))
+
+    @classmethod
+    def stage_times(cls, events):
+        """Milliseconds per stage from the events of one run() (synchronises on the last)."""
+        events[-1].synchronize()
+        return {k: events[i].elapsed_time(events[i + 1]) for i, k in enumerate(cls.STAGES)}
+
+    def check_capacity(self, stream=None):
+        """Raise if any image overflowed a workspace capacity (synchronises)."""
+        st = _lib.stream_handle(stream)
+        for i, ws in enumerate(self.ws):
+            cnt = (C.c_int * 4)()
+            _lib.call("fsx_sift_counters", C.c_void_p(ws.data_ptr()), cnt, C.c_void_p(st))
+            if cnt[3]:
+                raise RuntimeError("image %d overflowed capacity (flags %d, counts %d/%d/%d)" %
+                                   (i, cnt[3], cnt[0], cnt[1], cnt[2]))
+
+    def counts(self):
+        """(detected, selected) keypoint counts per image (synchronises)."""
+        n = self.n_out.cpu().numpy().astype(np.int64)
+        return n, np.minimum(n, self.nfeat)
+
+    def results(self, i):
+        """Host arrays of image i in features.sift_gpu's output order: xy (m,2) f32,
+        RootSIFT desc u8 (m,128), resp (m,), plus the float descriptors if kept."""
+        n_det, m = self.counts()
+        m = int(m[i])
+        kp = self.kp[i].cpu().numpy()
+        sel = self._sel(i, m)
+        xy = np.c_[kp[0][sel], kp[1][sel]].astype(np.float32)
+        resp = kp[4][sel].astype(np.float32)
+        u8 = self.u8[i, :m].cpu().numpy()
+        d = self.desc[i, :m].cpu().numpy() if self.desc is not None else None
+        return xy, u8, resp, d
+
+    def _sel(self, i, m):
+        """The spread selection (int64 indices into the detect arrays, final order) of image i."""
+        off = C.c_size_t()
+        H, W = self.shapes[i]
+        _lib.call("fsx_sift_selection_offset", H, W, self.cand_cap, self.kp_cap, C.byref(off))
+        o = off.value
+        return self.ws[i][o:o + 8 * m].cpu().numpy().view(np.int64)

@@ -1,0 +1,171 @@
+"""Descriptor matcher on the GPU (replaces the matcher core of app/planesite/matching.py).
+
+Drop-ins with the reference's signatures and return types:
+
+* ``_ratio_match_bf(da, db, ratio, na=None, nb=None) -> (q int64, t int64)`` (matching.py:86-100)
+* ``_guided(A, B, Hm, radius, kc, ratio) -> (q int64, t int64)`` (matching.py:113-152); A and B
+  are the feature dicts of matching._get (``und`` float64 (n,2), ``desc`` (n,128) holding
+  uint8 values, ``wh``)
+* ``_unique(q, t, dd)`` (matching.py:103-110)
+
+``PairMatcher`` runs many pairs (the pipeline's match_pair pool, pipeline.py:535-548) in one
+batched launch sequence on device-resident descriptors.  Two forms of the contraction:
+``form="plain"`` (FP32 gather + VALU dot products, the baseline) and ``form="mfma"``
+(spatially blocked BF16 MFMA with the prior-window mask and top-2 fused).  Both return the
+FP64-oracle match set for uint8 RootSIFT descriptors (all distances are exact integers).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+FORMS = {"plain": 0, "mfma": 1}
+
+
+def _unique(q, t, dd):
+    """matching.py:103-110 (host): one query per target, smallest dd then earliest; sorted by t."""
+    if len(q) == 0:
+        return q, t
+    order = np.lexsort((dd, t))
+    t_s = t[order]
+    first = np.r_[True, t_s[1:] != t_s[:-1]]
+    keep = order[first]
+    return q[keep], t[keep]
+
+
+def as_u8(desc) -> np.ndarray:
+    """Descriptors as uint8 (the pipeline's RootSIFT format); float input must hold exact
+    integers in 0..255 (matching._get's desc.astype(float32))."""
+    d = np.asarray(desc)
+    if d.dtype == np.uint8:
+        return np.ascontiguousarray(d)
+    u = np.clip(d, 0, 255).astype(np.uint8)
+    if not np.array_equal(u.astype(d.dtype), d):
+        raise ValueError("descriptors must be uint8-valued (RootSIFT u8); float descriptors are not supported")
+    return np.ascontiguousarray(u)
+
+
+class PairMatcher:
+    """Batched guided / brute-force matching of image pairs on one GPU.
+
+    images: list of (desc uint8 (n,128), xy float64 (n,2), (W, H)) — numpy or torch tensors.
+    pairs:  list of (i, j); Hm: list of 3x3 prior mappings i px -> j px (None = brute force).
+    """
+
+    def __init__(self, images, pairs, Hm=None, radius=60.0, kc=0, ratio=0.8, device="cuda"):
+        import torch
+        self.torch = torch
+        self.n_img = len(images)
+        self.pairs = [(int(a), int(b)) for a, b in pairs]
+        self.n_pair = len(self.pairs)
+        self.radius = float(radius)
+        self.kc = int(kc)
+        self.ratio = float(ratio)
+        descs, xys, self.nk, self.W, self.H = [], [], [], [], []
+        for d, xy, (W, H) in images:
+            d = d if isinstance(d, torch.Tensor) else torch.from_numpy(as_u8(d))
+            xy = xy if isinstance(xy, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(xy, np.float64))
+            if d.dtype != torch.uint8 or xy.dtype != torch.float64:
+                raise TypeError("descriptors must be uint8 and positions float64")
+            descs.append(d.reshape(-1, 128).to(device))
+            xys.append(xy.reshape(-1, 2).to(device))
+            self.nk.append(int(d.shape[0]))
+            self.W.append(int(W))
+            self.H.append(int(H))
+        self.desc = torch.cat(descs).contiguous() if descs else torch.empty((0, 128), dtype=torch.uint8, device=device)
+        self.xy = torch.cat(xys).contiguous() if xys else torch.empty((0, 2), dtype=torch.float64, device=device)
+        if self.desc.shape[0] == 0:      # keep a valid device pointer
+            self.desc = torch.zeros((1, 128), dtype=torch.uint8, device=device)
+            self.xy = torch.zeros((1, 2), dtype=torch.float64, device=device)
+        n = self.n_img
+        self._nk = (C.c_int * n)(*self.nk)
+        self._W = (C.c_int * n)(*self.W)
+        self._H = (C.c_int * n)(*self.H)
+        npair = max(self.n_pair, 1)
+        self._pa = (C.c_int * npair)(*[a for a, _ in self.pairs])
+        self._pb = (C.c_int * npair)(*[b for _, b in self.pairs])
+        dense = [1 if (Hm is None or Hm[p] is None) else 0 for p in range(self.n_pair)]
+        self._dense = (C.c_int * npair)(*dense)
+        hm = np.zeros((npair, 9), np.float64)
+        for p in range(self.n_pair):
+            if not dense[p]:
+                hm[p] = np.asarray(Hm[p], np.float64).reshape(9)
+        self._hm = np.ascontiguousarray(hm)
+        nb = C.c_size_t()
+        cap = C.c_longlong()
+        _lib.call("fsx_match_workspace_bytes", n, self._nk, self._W, self._H, self.n_pair, self._pa, self._pb,
+                  self.radius, C.byref(nb), C.byref(cap))
+        self.ws = torch.empty(nb.value, dtype=torch.uint8, device=device)
+        self.ws_bytes = nb.value
+        self.capacity = cap.value
+        self._offsets = (C.c_longlong * npair)()
+
+    def run(self, form="mfma", events=None, stream=None):
+        """Enqueue matching of every pair (asynchronous)."""
+        P = C.c_void_p
+        st = _lib.stream_handle(stream)
+        ev = None
+        if events is not None:
+            for e in events:
+                if not e.cuda_event:
+                    e.record()
+            ev = (P * 4)(*[e.cuda_event for e in events])
+        _lib.call("fsx_match_batch", self.n_img, self._nk, self._W, self._H, P(self.desc.data_ptr()),
+                  P(self.xy.data_ptr()), self.n_pair, self._pa, self._pb, self._dense,
+                  self._hm.ctypes.data_as(C.POINTER(C.c_double)), self.radius, self.kc, self.ratio,
+                  FORMS[form], P(self.ws.data_ptr()), self.ws_bytes, self._offsets, ev, P(st))
+
+    def results(self, stream=None):
+        """List of (q int64, t int64) per pair, sorted by t (synchronises)."""
+        st = _lib.stream_handle(stream)
+        npair = max(self.n_pair, 1)
+        counts = (C.c_int * npair)()
+        q = np.zeros(max(self.capacity, 1), np.int32)
+        t = np.zeros(max(self.capacity, 1), np.int32)
+        flags = (C.c_int * 4)()
+        _lib.call("fsx_match_results", self.n_img, self._nk, self._W, self._H, self.n_pair, self._pa, self._pb,
+                  self.radius, C.c_void_p(self.ws.data_ptr()), counts, q.ctypes.data_as(C.POINTER(C.c_int)),
+                  t.ctypes.data_as(C.POINTER(C.c_int)), flags, C.c_void_p(st))
+        if flags[2]:
+            raise RuntimeError("matcher chunk table overflow (flags %d)" % flags[2])
+        out = []
+        for p in range(self.n_pair):
+            o = self._offsets[p]
+            n = counts[p]
+            out.append((q[o:o + n].astype(np.int64), t[o:o + n].astype(np.int64)))
+        return out
+
+
+def _ratio_match_bf(da, db, ratio, na=None, nb=None, form="mfma"):
+    """matching.py:86-100: exact nearest-neighbour ratio matching of all of da against db."""
+    if len(da) < 2 or len(db) < 2:
+        return np.zeros(0, np.int64), np.zeros(0, np.int64)
+    a, b = as_u8(da), as_u8(db)
+    zero = lambda n: np.zeros((n, 2), np.float64)   # noqa: E731
+    m = PairMatcher([(a, zero(len(a)), (1, 1)), (b, zero(len(b)), (1, 1))], [(0, 1)], None, radius=1.0,
+                    ratio=ratio)
+    m.run(form)
+    return m.results()[0]
+
+
+def _guided(A, B, Hm, radius, kc, ratio, form="mfma"):
+    """matching.py:113-152: match A's features to B's keypoints within `radius` of their
+    predicted position Hm(A.und), the kc spatially nearest (kc >= len(B) or 0: all)."""
+    nb = len(B["und"])
+    kc = 0 if kc >= nb else int(kc)
+    m = PairMatcher([(as_u8(A["desc"]), np.asarray(A["und"], np.float64), A["wh"]),
+                     (as_u8(B["desc"]), np.asarray(B["und"], np.float64), B["wh"])],
+                    [(0, 1)], [np.asarray(Hm, np.float64)], radius=radius, kc=kc, ratio=ratio)
+    m.run(form)
+    return m.results()[0]
+
+
+def mfma_layout_probe():
+    """Run the 32x32x16 BF16 MFMA layout probe; returns (64 lanes, 16) float32."""
+    import torch
+    out = torch.zeros(64 * 16, dtype=torch.float32, device="cuda")
+    _lib.call("fsx_mfma_layout_probe", C.c_void_p(out.data_ptr()), C.c_void_p(_lib.stream_handle()))
+    return out.view(64, 16).cpu().numpy()

@@ -186,15 +186,21 @@ class Flight:
     prior: dict            # (i, j) -> 3x3 prior mapping (true + N(0, sigma) translation)
 
 
+def _render_job(job):
+    mos, M, h, w, noise_seed = job
+    return mos.render(M, h, w, np.random.default_rng(noise_seed))
+
+
 def flight(seed: int = 2, rows: int = 2, cols: int = 4, W: int = 2640, H: int = 1978,
            front: float = 0.75, side: float = 0.65, rot_jitter: float = 3.0, scale_jitter: float = 0.02,
            prior_sigma: float = 3.0, lowres_frac: float = 0.0, poly_density: float = 40.0 / (1368 * 912),
-           min_overlap: float = 0.05) -> Flight:
+           min_overlap: float = 0.05, workers: int = 1) -> Flight:
     """Photos along a rows x cols flight grid.  Flight lines run along x (a line is one grid row),
     so consecutive photos of a line overlap by `front` of the width and neighbouring lines by
     `side` of the height.  A photo is low resolution (W/2 x H/2 covering the same ground) with
     probability `lowres_frac`.  Pairs: every photo pair whose footprints (without jitter)
-    overlap by more than `min_overlap` of the footprint area."""
+    overlap by more than `min_overlap` of the footprint area.  Each photo's noise has its own
+    seed, so `workers` > 1 renders in parallel with identical results."""
     rng = np.random.default_rng(seed)
     step_u = (1.0 - front) * W
     step_v = (1.0 - side) * H
@@ -204,7 +210,7 @@ def flight(seed: int = 2, rows: int = 2, cols: int = 4, W: int = 2640, H: int = 
     mos = Mosaic(seed=int(rng.integers(1 << 30)))
     n_poly = int(round(poly_density * (ext_u + 2 * margin) * (ext_v + 2 * margin)))
     mos.polys = random_polygons(rng, n_poly, -margin, -margin, ext_u + margin, ext_v + margin)
-    images, A, centres = [], [], []
+    jobs, A, centres = [], [], []
     for r in range(rows):
         for c in range(cols):
             theta = rng.uniform(-rot_jitter, rot_jitter)
@@ -213,12 +219,17 @@ def flight(seed: int = 2, rows: int = 2, cols: int = 4, W: int = 2640, H: int = 
             w, h, g = (W // 2, (H + 1) // 2, 2.0) if low else (W, H, 1.0)
             cu = c * step_u + W / 2.0
             cv = r * step_v + H / 2.0
-            # image px (x, y) -> mosaic: centre + s*g*R(x - (w-1)/2, y - (h-1)/2)
+            # image px (x, y) -> mosaic: (cu, cv) + s*g*R(theta) ((x, y) - image centre)
             M = similarity(theta, s * g, cu - (w - 1) / 2.0, cv - (h - 1) / 2.0, (w - 1) / 2.0, (h - 1) / 2.0)
-            # similarity() rotates about the image centre then translates; mosaic centre = (cu, cv)
             A.append(M)
             centres.append((cu, cv))
-            images.append(mos.render(M, h, w, rng))
+            jobs.append((mos, M, h, w, int(rng.integers(1 << 62))))
+    if workers > 1:
+        import multiprocessing as mp
+        with mp.get_context("fork").Pool(min(workers, len(jobs))) as pool:
+            images = pool.map(_render_job, jobs)
+    else:
+        images = [_render_job(j) for j in jobs]
     n = len(images)
     pairs, H_true, prior = [], {}, {}
     for i in range(n):

@@ -70,6 +70,73 @@ int fsx_sift_counters(const void *ws_dev, int *out4, void *stream);
 int fsx_sift_describe(const void *ws_dev, const float *kp_dev, int kp_stride, const long long *sel_dev, int m,
                       float *desc_dev, uint8_t *u8_dev, void *stream);
 
+/* Bytes of the device job table fsx_sift_features_batch needs for n_img images. */
+int fsx_sift_table_bytes(int n_img, size_t *bytes);
+
+/* The GPU feature worker for a batch of photos in one call (features.py:115-129 sift_gpu for
+ * every photo, as gpu_worker_main's loop does one by one, features.py:343-378): detect
+ * int(nfeat*2.2)=max_kp keypoints, grid-spread to nfeat (features.py:38-57 _spread), sort by
+ * response, describe, RootSIFT-encode (features.py:80-82).  Per image i: imgs[i] (device
+ * uint8, H[i] x W[i], pitch ld[i]), its workspace ws[i], detect arrays kp_out[i][7][max_kp],
+ * n_out_dev[i]; descriptors in selection order: desc_out[i] (nfeat x 128 float32, array may be
+ * NULL) and u8_out[i] (nfeat x 128 uint8, array may be NULL).  The selected count is
+ * min(n_out, nfeat); the selection (int64, final order) sits in ws[i] at
+ * fsx_sift_selection_offset.  Images of equal shape share their pyramid launches.  table_dev:
+ * fsx_sift_table_bytes(n_img) bytes of device memory.  stage_events (may be NULL): 8 caller-created
+ * hipEvent_t recorded on `stream` around the stages pyramid / extrema / refine / orientation /
+ * select / spread / describe (asynchronous; the features.py timings of the reference). */
+int fsx_sift_features_batch(int n_img, const uint8_t *const *imgs, const int *H, const int *W, const int *ld,
+                            double contrast, double edge, int max_kp, int nfeat, int cand_cap, int kp_cap,
+                            void *const *ws, const size_t *ws_bytes, float *const *kp_out, int *n_out_dev,
+                            float *const *desc_out, uint8_t *const *u8_out, void *table_dev, size_t table_bytes,
+                            void *const *stage_events, void *stream);
+
+/* Byte offset of the spread selection inside a workspace (no reference counterpart). */
+int fsx_sift_selection_offset(int H, int W, int cand_cap, int kp_cap, size_t *off);
+
+/* ---------------------------------------------------------------------------------------
+ * Matcher (replaces the matcher core of app/planesite/matching.py)
+ * ------------------------------------------------------------------------------------- */
+
+/* Workspace bytes of a batch of n_pair pairs (pa[p] -> pb[p]) over n_img keypoint sets
+ * (n_kp[i] keypoints of an image W[i] x H[i]); out_capacity: total output slots
+ * (sum of min(n_a, n_b)). */
+int fsx_match_workspace_bytes(int n_img, const int *n_kp, const int *W, const int *H, int n_pair, const int *pa,
+                              const int *pb, double radius, size_t *bytes, long long *out_capacity);
+
+/* Batched matching (the match_pair pool of pipeline.py:535-548 over matching.py's core):
+ * for each pair p, queries = keypoints of image pa[p], targets = keypoints of pb[p].
+ *   dense[p] = 1: brute-force ratio matching of all queries (_ratio_match_bf,
+ *     matching.py:86-100);
+ *   dense[p] = 0: guided matching (_guided, matching.py:113-152): prediction
+ *     Hm[9p..9p+8] applied to the query positions with cv2.perspectiveTransform semantics,
+ *     queries predicted strictly inside (-radius, W+radius) x (-radius, H+radius), candidates
+ *     strictly within radius (squared float64 distance, cKDTree's test), the kc spatially nearest
+ *     (kc = 0: all; ties by index).
+ * Then top-2, ratio test sqrt(b1) < ratio*sqrt(b2) in float64 (one candidate: always
+ * accepted) and one query per target (matching.py:103-110, ties to the lower query), output
+ * sorted by target.  desc_dev: all keypoint sets' uint8 descriptors concatenated (n x 128);
+ * xy_dev: their float64 positions (n x 2).  form 0 = plain FP32 gather + VALU dot products,
+ * form 1 = spatially blocked BF16 MFMA with fused window mask and top-2.  Both return the
+ * FP64-oracle set (uint8 distances are exact integers).  out_offsets (host, n_pair): output
+ * slot of each pair.  stage_events (may be NULL): 4 caller hipEvent_t recorded around
+ * setup / contraction / unique.  Asynchronous; read with fsx_match_results. */
+int fsx_match_batch(int n_img, const int *n_kp, const int *W, const int *H, const uint8_t *desc_dev,
+                    const double *xy_dev, int n_pair, const int *pa, const int *pb, const int *dense,
+                    const double *Hm, double radius, int kc, double ratio, int form, void *ws_dev, size_t ws_bytes,
+                    long long *out_offsets, void *stage_events, void *stream);
+
+/* Copies the last batch's results to the host (synchronises): counts[n_pair], q and t at the
+ * pair offsets (int32, as match_pair returns them, matching.py:249), flags[4] (flags[2] != 0:
+ * an internal table overflowed). */
+int fsx_match_results(int n_img, const int *n_kp, const int *W, const int *H, int n_pair, const int *pa,
+                      const int *pb, double radius, const void *ws_dev, int *counts, int *q, int *t, int *flags,
+                      void *stream);
+
+/* Writes the 64 x 16 accumulator of one 32x32x16 BF16 MFMA on fixed small integer operands, to
+ * check the operand/result lane layout the matcher assumes (no reference counterpart). */
+int fsx_mfma_layout_probe(float *out_dev, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

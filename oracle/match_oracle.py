@@ -71,9 +71,25 @@ def ratio_match_bf_f32(da: np.ndarray, db: np.ndarray, ratio: float, na=None, nb
     return unique(r[ok].astype(np.int64), first[ok].astype(np.int64), b1[ok].astype(np.float32))
 
 
+def predict(xy: np.ndarray, Hm: np.ndarray) -> np.ndarray:
+    """cv2.perspectiveTransform of points (matching.py:115) in float64: w = (h6 x + h7 y) + h8;
+    |w| > FLT_EPSILON ? ((h0 x + h1 y) + h2) * (1/w) : 0 (OpenCV's perspectiveTransform_)."""
+    xy = np.asarray(xy, np.float64)
+    Hm = np.asarray(Hm, np.float64).reshape(9)
+    x, y = xy[:, 0], xy[:, 1]
+    w = (x * Hm[6] + y * Hm[7]) + Hm[8]
+    ok = np.abs(w) > np.finfo(np.float32).eps
+    iw = np.where(ok, 1.0 / np.where(ok, w, 1.0), 0.0)
+    px = ((x * Hm[0] + y * Hm[1]) + Hm[2]) * iw
+    py = ((x * Hm[3] + y * Hm[4]) + Hm[5]) * iw
+    return np.where(ok[:, None], np.c_[px, py], 0.0)
+
+
 def guided_candidates(pred: np.ndarray, xy_b: np.ndarray, wh_b, radius: float, kc: int):
     """matching.py:115-131: queries predicted inside B (with a margin of `radius`), each with
     the kc spatially nearest B keypoints strictly within `radius` (kc <= 0: all of them).
+    Spatial distances are squared float64 (dx*dx + dy*dy < radius*radius, cKDTree's p=2
+    comparison); ties in distance go to the lower index.
     Returns (query index array, list of candidate index arrays)."""
     pred = np.asarray(pred, np.float64)
     xy_b = np.asarray(xy_b, np.float64)
@@ -83,9 +99,12 @@ def guided_candidates(pred: np.ndarray, xy_b: np.ndarray, wh_b, radius: float, k
     qs, cands = [], []
     if len(xy_b) == 0:
         return np.zeros(0, np.int64), []
+    r2 = float(radius) * float(radius)
     for q in inside:
-        d = np.sqrt(((xy_b - pred[q]) ** 2).sum(1))
-        idx = np.nonzero(d < radius)[0]
+        dx = xy_b[:, 0] - pred[q, 0]
+        dy = xy_b[:, 1] - pred[q, 1]
+        d = dx * dx + dy * dy
+        idx = np.nonzero(d < r2)[0]
         if len(idx) == 0:
             continue
         if kc > 0 and len(idx) > kc:
@@ -159,3 +178,53 @@ def match_fp64(da, db, ratio: float, qs=None, cands=None, block: int = 1024):
     first = np.r_[True, t_s[1:] != t_s[:-1]]
     keep = order[first]
     return q[keep], t[keep], d[keep]
+
+
+def guided_fp64_fast(da, xy_a, db, xy_b, Hm, wh_b, radius: float, ratio: float):
+    """Vectorised FP64 oracle of guided matching with kc = 0 (all candidates in the window):
+    the same definition as guided_candidates + match_fp64, for large pairs.
+    Returns (q int64, t int64) sorted by t."""
+    from scipy.spatial import cKDTree
+    pred = predict(xy_a, Hm)
+    Wb, Hb = wh_b
+    inside = np.nonzero((pred[:, 0] > -radius) & (pred[:, 0] < Wb + radius)
+                        & (pred[:, 1] > -radius) & (pred[:, 1] < Hb + radius))[0]
+    xy_b = np.asarray(xy_b, np.float64)
+    if len(inside) == 0 or len(xy_b) == 0:
+        return np.zeros(0, np.int64), np.zeros(0, np.int64)
+    lists = cKDTree(xy_b).query_ball_point(pred[inside], radius * (1 + 1e-9) + 1e-9)
+    lens = np.array([len(x) for x in lists])
+    qq = np.repeat(inside, lens)
+    tt = np.concatenate([np.asarray(x, np.int64) for x in lists]) if lens.sum() else np.zeros(0, np.int64)
+    dx = xy_b[tt, 0] - pred[qq, 0]
+    dy = xy_b[tt, 1] - pred[qq, 1]
+    ok = dx * dx + dy * dy < float(radius) * float(radius)
+    qq, tt = qq[ok], tt[ok]
+    a = np.asarray(da, np.float64)
+    b = np.asarray(db, np.float64)
+    d = np.zeros(len(qq))
+    for s0 in range(0, len(qq), 1 << 16):
+        sl = slice(s0, s0 + (1 << 16))
+        acc = np.zeros(len(qq[sl]))
+        A = a[qq[sl]]
+        Bt = b[tt[sl]]
+        for k in range(a.shape[1]):
+            t = A[:, k] - Bt[:, k]
+            acc += t * t
+        d[sl] = acc
+    order = np.lexsort((tt, d, qq))
+    qq, tt, d = qq[order], tt[order], d[order]
+    first = np.r_[True, qq[1:] != qq[:-1]]
+    starts = np.nonzero(first)[0]
+    cnt = np.diff(np.r_[starts, len(qq)])
+    b1 = d[starts]
+    b2 = np.where(cnt > 1, d[np.minimum(starts + 1, len(d) - 1)], np.inf)
+    acc_ = np.sqrt(b1) < float(ratio) * np.sqrt(b2)
+    q = qq[starts][acc_]
+    t = tt[starts][acc_]
+    dd = b1[acc_]
+    o = np.lexsort((q, dd, t))
+    t_s = t[o]
+    f = np.r_[True, t_s[1:] != t_s[:-1]]
+    keep = o[f]
+    return q[keep], t[keep]

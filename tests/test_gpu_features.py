@@ -1,0 +1,55 @@
+"""GPU feature batch (detect -> device grid spread -> describe -> RootSIFT u8 in one call)
+against the drop-in per-image path (host _spread of features.py) and the CPU oracle."""
+import numpy as np
+import pytest
+
+from fieldstitch_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _images():
+    img1, img2, _ = synth.field_pair(1)
+    rng = np.random.default_rng(21)
+    mos = synth.Mosaic(seed=int(rng.integers(1 << 30)))
+    mos.polys = synth.random_polygons(rng, 10, 0, 0, 500, 400)
+    img3 = mos.render(np.eye(3)[:2], 400, 500, rng)
+    return [img1, img2, img3, img1[:600, :900].copy()]
+
+
+@pytest.mark.parametrize("nfeat", [1000, 2500, 20000])
+def test_feature_batch_equals_per_image_path(gpu, nfeat):
+    import torch
+    from fieldstitch_amd import features
+    imgs = _images()
+    print("case nfeat", nfeat, [i.shape for i in imgs], flush=True)
+    fb = features.FeatureBatch([i.shape for i in imgs], nfeat=nfeat, contrast=0.015, edge=12.0, float_desc=True)
+    dev = [torch.from_numpy(i).cuda() for i in imgs]
+    fb.run(dev)
+    fb.check_capacity()
+    for i, img in enumerate(imgs):
+        xy, u8, resp, d = fb.results(i)
+        # drop-in path: detect with max_kp = 2.2 nfeat, host _spread, stable response sort
+        xr, dr, rr, _ = features.sift_gpu(img, nfeat)
+        assert xy.shape == xr.shape, (i, xy.shape, xr.shape)
+        np.testing.assert_array_equal(xy, xr)
+        np.testing.assert_array_equal(resp, rr)
+        np.testing.assert_array_equal(d, dr)
+        np.testing.assert_array_equal(u8, features.rootsift_u8(dr))
+
+
+def test_spread_matches_reference_formula(gpu):
+    """Device spread == host _spread on a case with many keypoints per cell and equal
+    responses (multi-orientation keypoints)."""
+    import torch
+    from fieldstitch_amd import features, gsift
+    img = _images()[0]
+    nfeat = 700
+    kps, st = gsift.detect(img, max_kp=int(nfeat * 2.2), contrast=0.015, edge=12.0)
+    lst = [features._KP(float(x), float(y), float(r)) for x, y, r in zip(kps["x"], kps["y"], kps["response"])]
+    sel = np.array(features._spread(lst, img.shape[1], img.shape[0], nfeat), np.int64)
+    sel = sel[np.argsort(-kps["response"][sel], kind="stable")]
+    fb = features.FeatureBatch([img.shape], nfeat=nfeat, contrast=0.015, edge=12.0)
+    fb.run([torch.from_numpy(img).cuda()])
+    np.testing.assert_array_equal(fb._sel(0, nfeat), sel)
+    assert len(np.unique(kps["response"])) < len(kps["response"])   # ties present

@@ -116,3 +116,26 @@ def test_guided_f32_equals_fp64_on_u8():
     q2, t2, _ = mo.match_fp64(a, b, 0.8, qs, cands)
     np.testing.assert_array_equal(q, q2)
     np.testing.assert_array_equal(t, t2)
+
+
+def test_guided_fp64_fast_equals_reference_form():
+    rng = np.random.default_rng(13)
+    a, b = pair_u8(rng, 500, 600, 250)
+    xy_b = rng.uniform(0, 400, (600, 2))
+    xy_a = rng.uniform(-30, 430, (500, 2))
+    Hm = np.array([[1.0, 0.02, 3.0], [-0.01, 1.0, -4.0], [0, 0, 1.0]])
+    q, t = mo.guided_fp64_fast(a, xy_a, b, xy_b, Hm, (400, 400), 35.0, 0.8)
+    pred = mo.predict(xy_a, Hm)
+    qs, cands = mo.guided_candidates(pred, xy_b, (400, 400), 35.0, 0)
+    q2, t2, _ = mo.match_fp64(a, b, 0.8, qs, cands)
+    np.testing.assert_array_equal(q, q2)
+    np.testing.assert_array_equal(t, t2)
+
+
+def test_predict_matches_projective_formula():
+    rng = np.random.default_rng(2)
+    xy = rng.uniform(0, 100, (50, 2))
+    Hm = np.array([[1.1, 0.1, 5.0], [-0.2, 0.9, 3.0], [1e-4, 2e-4, 1.0]])
+    p = mo.predict(xy, Hm)
+    h = np.c_[xy, np.ones(50)] @ Hm.T
+    np.testing.assert_allclose(p, h[:, :2] / h[:, 2:], rtol=1e-14)
