@@ -41,6 +41,10 @@ SIGNATURES = {
     "fsx_match_results": [_i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), _i, C.POINTER(_i), C.POINTER(_i),
                           C.c_double, _vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), _vp],
     "fsx_mfma_layout_probe": [_vp, _vp],
+    "fsx_match_layout": [_i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), _i, C.POINTER(_i), C.POINTER(_i),
+                         C.c_double, C.POINTER(C.c_longlong)],
+    "fsx_match_query_state": [_i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), _i, C.POINTER(_i), C.POINTER(_i),
+                              C.c_double, _vp, C.POINTER(_i), C.POINTER(C.c_double), C.POINTER(_i), _vp],
 }
 
 _lib = None

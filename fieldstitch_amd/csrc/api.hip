@@ -1,5 +1,6 @@
 // C-ABI plumbing of libfsx: thread-local error text and device queries.
 #include "fsx_common.h"
+#include <mutex>
 #include <string.h>
 
 namespace fsx {
@@ -11,6 +12,44 @@ void set_error(const char* fmt, ...) {
     va_start(ap, fmt);
     vsnprintf(g_err, sizeof(g_err), fmt, ap);
     va_end(ap);
+}
+
+namespace {
+struct Slot {
+    void* buf = nullptr;
+    size_t cap = 0;
+    hipEvent_t ev = nullptr;
+    bool pending = false;
+};
+constexpr int N_SLOTS = 64;
+Slot g_slots[N_SLOTS];
+int g_next = 0;
+std::mutex g_mu;
+}  // namespace
+
+int upload_async(void* dst_dev, const void* src, size_t n, hipStream_t st) {
+    if (n == 0) return FSX_OK;
+    std::lock_guard<std::mutex> lk(g_mu);
+    Slot& s = g_slots[g_next];
+    g_next = (g_next + 1) % N_SLOTS;
+    if (!s.ev) FSX_HIP(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
+    if (s.pending) {
+        FSX_HIP(hipEventSynchronize(s.ev));
+        s.pending = false;
+    }
+    if (s.cap < n) {
+        if (s.buf) FSX_HIP(hipHostFree(s.buf));
+        s.buf = nullptr;
+        size_t cap = 4096;
+        while (cap < n) cap <<= 1;
+        FSX_HIP(hipHostMalloc(&s.buf, cap, hipHostMallocDefault));
+        s.cap = cap;
+    }
+    memcpy(s.buf, src, n);
+    FSX_HIP(hipMemcpyAsync(dst_dev, s.buf, n, hipMemcpyHostToDevice, st));
+    FSX_HIP(hipEventRecord(s.ev, st));
+    s.pending = true;
+    return FSX_OK;
 }
 
 }  // namespace fsx

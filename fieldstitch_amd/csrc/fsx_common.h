@@ -16,6 +16,12 @@ namespace fsx {
 
 void set_error(const char* fmt, ...);
 
+// Host -> device upload of a small table on `st` through library-owned pinned staging memory.
+// hipMemcpyAsync may read pageable host memory after it returns, so callers never pass their
+// own (stack / vector) memory: the bytes are copied into a pinned slot first, and a slot is
+// reused only after the copy that last read it has completed (event wait).
+int upload_async(void* dst_dev, const void* src, size_t n, hipStream_t st);
+
 #define FSX_HIP(call)                                                                \
     do {                                                                             \
         hipError_t e_ = (call);                                                      \

@@ -409,7 +409,8 @@ __device__ __forceinline__ unsigned long long qsort_key_hi(int p, int tilekey) {
 __global__ void k_m_qkeys(MatchBatch B) {
     const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
     if (g >= B.sort_cap) return;
-    unsigned long long k0 = ~0ull, k1 = ~0ull;
+    // every key is unique (k1 = query id, also for the sentinels of invalid queries)
+    unsigned long long k0 = ~0ull, k1 = (unsigned long long)g;
     if (g < B.n_query) {
         const int cell = B.qcell[g];
         if (cell >= 0) {
@@ -487,10 +488,15 @@ __global__ __launch_bounds__(256) void k_keysort_merge(const unsigned long long*
             hi = s + w;
             out = p - w;
         }
+        // stable merge: a left-run element counts right-run keys < x, a right-run element
+        // counts left-run keys <= x, so equal keys keep their order and never collide
+        const bool left = p < s + w;
         long long a = lo, b = hi;
         while (a < b) {
             long long m = (a + b) >> 1;
-            if (key_lt(src[2 * m], src[2 * m + 1], x0, x1)) a = m + 1;
+            const unsigned long long m0 = src[2 * m], m1 = src[2 * m + 1];
+            const bool before = left ? key_lt(m0, m1, x0, x1) : !key_lt(x0, x1, m0, m1);
+            if (before) a = m + 1;
             else b = m;
         }
         out += a - lo;
@@ -905,8 +911,10 @@ int fsx_match_batch(int n_img, const int* n_kp, const int* W, const int* H, cons
     hipStream_t st = (hipStream_t)stream;
     char* w = (char*)ws;
     MatchBatch b = batch_ptrs(M, w, desc_dev, xy_dev, radius, kc, ratio);
-    FSX_HIP(hipMemcpyAsync(w + M.off_img, M.img.data(), sizeof(MImage) * M.img.size(), hipMemcpyHostToDevice, st));
-    FSX_HIP(hipMemcpyAsync(w + M.off_pair, M.pair.data(), sizeof(MPair) * M.pair.size(), hipMemcpyHostToDevice, st));
+    rc = upload_async(w + M.off_img, M.img.data(), sizeof(MImage) * M.img.size(), st);
+    if (rc) return rc;
+    rc = upload_async(w + M.off_pair, M.pair.data(), sizeof(MPair) * M.pair.size(), st);
+    if (rc) return rc;
     if (out_offsets)
         for (int p = 0; p < n_pair; p++) out_offsets[p] = M.pair[p].o_off;
     hipEvent_t* ev = (hipEvent_t*)stage_events_v;
@@ -987,6 +995,42 @@ int fsx_mfma_layout_probe(float* out_dev, void* stream) {
     FSX_REQUIRE(out_dev, "null pointer argument");
     hipLaunchKernelGGL(k_mfma_layout_probe, dim3(1), dim3(64), 0, (hipStream_t)stream, out_dev);
     FSX_LAUNCH_CHECK();
+    return FSX_OK;
+}
+
+// Per-query state of the last batch (synchronises): accepted best target (-1 = none), its
+// distance b1, and the query's grid cell (-1 = predicted outside).  For tests and diagnostics.
+int fsx_match_query_state(int n_img, const int* n_kp, const int* W, const int* H, int n_pair, const int* pa,
+                          const int* pb, double radius, const void* ws, int* best, double* b1, int* qcell,
+                          void* stream) {
+    MatchPlanHost M;
+    int rc = match_plan(n_img, n_kp, W, H, n_pair, pa, pb, nullptr, nullptr, radius, M);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    const char* w = (const char*)ws;
+    if (M.n_query > 0) {
+        if (best) FSX_HIP(hipMemcpyAsync(best, w + M.off_best, 4 * (size_t)M.n_query, hipMemcpyDeviceToHost, st));
+        if (b1) FSX_HIP(hipMemcpyAsync(b1, w + M.off_b1, 8 * (size_t)M.n_query, hipMemcpyDeviceToHost, st));
+        if (qcell) FSX_HIP(hipMemcpyAsync(qcell, w + M.off_qcell, 4 * (size_t)M.n_query, hipMemcpyDeviceToHost, st));
+    }
+    FSX_HIP(hipStreamSynchronize(st));
+    return FSX_OK;
+}
+
+// Workspace layout (byte offsets) for tests and diagnostics: [0] norms [1] cell_start [2] cell_cur
+// [3] cell_idx [4] pred [5] qcell [6] best [7] b1 [8] kth [9] tkey [10] tq [11] qkey [12] qkey2
+// [13] qsorted [14] counters [15] pcount [16] chunk [17] out_q [18] out_t [19] out_n [20] sort_cap
+// [21] max_chunks [22] total
+int fsx_match_layout(int n_img, const int* n_kp, const int* W, const int* H, int n_pair, const int* pa,
+                     const int* pb, double radius, long long* offs) {
+    MatchPlanHost M;
+    int rc = match_plan(n_img, n_kp, W, H, n_pair, pa, pb, nullptr, nullptr, radius, M);
+    if (rc) return rc;
+    const size_t v[23] = {M.off_norms, M.off_cstart, M.off_ccur, M.off_cidx, M.off_pred, M.off_qcell, M.off_best,
+                          M.off_b1, M.off_kth, M.off_tkey, M.off_tq, M.off_qkey, M.off_qkey2, M.off_qsorted,
+                          M.off_cnt, M.off_pcount, M.off_chunk, M.off_outq, M.off_outt, M.off_outn,
+                          (size_t)M.sort_cap, (size_t)M.max_chunks, M.total};
+    for (int i = 0; i < 23; i++) offs[i] = (long long)v[i];
     return FSX_OK;
 }
 

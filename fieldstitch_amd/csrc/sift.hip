@@ -563,11 +563,15 @@ __global__ __launch_bounds__(256) void k_merge(const SiftJob* __restrict__ jobs,
             hi = s + w;
             out = p - w;
         }
-        // count of partner elements less than x
+        // stable merge: a left-run element counts right-run keys < x, a right-run element
+        // counts left-run keys <= x (equal keys keep their order and never collide)
+        const bool left = p < s + w;
         int a = lo, b = hi;
         while (a < b) {
             int m = (a + b) >> 1;
-            if (key_less(src[2 * m], src[2 * m + 1], x0, x1)) a = m + 1;
+            const unsigned long long m0 = src[2 * m], m1 = src[2 * m + 1];
+            const bool before = left ? key_less(m0, m1, x0, x1) : !key_less(x0, x1, m0, m1);
+            if (before) a = m + 1;
             else b = m;
         }
         out += a - lo;
@@ -1144,7 +1148,8 @@ int fsx_sift_detect(const uint8_t* img, int H, int W, int ld, double contrast, d
     sift_plan(H, W, &p);
     SiftJob J;
     fill_job(&J, p, img, ld, contrast, edge, max_kp, cand_cap, kp_cap, ws, kp_out, n_out);
-    FSX_HIP(hipMemcpyAsync(ws, &J, sizeof(J), hipMemcpyHostToDevice, st));
+    rc = upload_async(ws, &J, sizeof(J), st);
+    if (rc) return rc;
     EventSet E;
     if (stage_ms && E.create(6)) return FSX_EHIP;
     Group g{(const SiftJob*)ws, 1, p};
@@ -1223,9 +1228,14 @@ int fsx_sift_features_batch(int n_img, const uint8_t* const* imgs, const int* H,
             groups.push_back(Group{dev + k, 0, p});
         groups.back().n++;
     }
-    for (int k = 0; k < n_img; k++)
-        FSX_HIP(hipMemcpyAsync(ws[order[k]], &jobs[k], sizeof(SiftJob), hipMemcpyHostToDevice, st));
-    FSX_HIP(hipMemcpyAsync(table_dev, jobs.data(), (size_t)n_img * sizeof(SiftJob), hipMemcpyHostToDevice, st));
+    for (int k = 0; k < n_img; k++) {
+        int rc2 = upload_async(ws[order[k]], &jobs[k], sizeof(SiftJob), st);
+        if (rc2) return rc2;
+    }
+    {
+        int rc2 = upload_async(table_dev, jobs.data(), (size_t)n_img * sizeof(SiftJob), st);
+        if (rc2) return rc2;
+    }
     hipEvent_t ev[8];
     if (stage_events)
         for (int i = 0; i < 8; i++) ev[i] = (hipEvent_t)stage_events[i];

@@ -139,6 +139,19 @@ class PairMatcher:
         return out
 
 
+def _query_state(m: "PairMatcher", stream=None):
+    """Per-query (best, b1, cell) arrays of the last run (tests and diagnostics)."""
+    nq = sum(m.nk[a] for a, _ in m.pairs)
+    best = np.zeros(max(nq, 1), np.int32)
+    b1 = np.zeros(max(nq, 1), np.float64)
+    cell = np.zeros(max(nq, 1), np.int32)
+    _lib.call("fsx_match_query_state", m.n_img, m._nk, m._W, m._H, m.n_pair, m._pa, m._pb, m.radius,
+              C.c_void_p(m.ws.data_ptr()), best.ctypes.data_as(C.POINTER(C.c_int)),
+              b1.ctypes.data_as(C.POINTER(C.c_double)), cell.ctypes.data_as(C.POINTER(C.c_int)),
+              C.c_void_p(_lib.stream_handle(stream)))
+    return best[:nq], b1[:nq], cell[:nq]
+
+
 def _ratio_match_bf(da, db, ratio, na=None, nb=None, form="mfma"):
     """matching.py:86-100: exact nearest-neighbour ratio matching of all of da against db."""
     if len(da) < 2 or len(db) < 2:

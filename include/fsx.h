@@ -133,6 +133,18 @@ int fsx_match_results(int n_img, const int *n_kp, const int *W, const int *H, in
                       const int *pb, double radius, const void *ws_dev, int *counts, int *q, int *t, int *flags,
                       void *stream);
 
+/* Per-query state of the last batch (synchronises): accepted best target (-1 = none), its
+ * distance b1 and its grid cell (-1 = predicted outside B); arrays of sum(n_a) entries, may be
+ * NULL.  For tests and diagnostics (no reference counterpart). */
+int fsx_match_query_state(int n_img, const int *n_kp, const int *W, const int *H, int n_pair, const int *pa,
+                          const int *pb, double radius, const void *ws_dev, int *best, double *b1, int *qcell,
+                          void *stream);
+
+/* Byte offsets of the matcher workspace regions (23 values, order documented in match.hip), for
+ * tests and diagnostics (no reference counterpart). */
+int fsx_match_layout(int n_img, const int *n_kp, const int *W, const int *H, int n_pair, const int *pa,
+                     const int *pb, double radius, long long *offs);
+
 /* Writes the 64 x 16 accumulator of one 32x32x16 BF16 MFMA on fixed small integer operands, to
  * check the operand/result lane layout the matcher assumes (no reference counterpart). */
 int fsx_mfma_layout_probe(float *out_dev, void *stream);

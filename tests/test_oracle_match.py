@@ -139,3 +139,16 @@ def test_predict_matches_projective_formula():
     p = mo.predict(xy, Hm)
     h = np.c_[xy, np.ones(50)] @ Hm.T
     np.testing.assert_allclose(p, h[:, :2] / h[:, 2:], rtol=1e-14)
+
+
+def test_device_sort_rank_rule_handles_equal_keys():
+    """The merge rank rule of the device key sorts, modelled on the CPU: with many equal keys
+    (e.g. sentinels) every slot is written exactly once and the output is sorted."""
+    from oracle.sort_model import device_sort
+    rng = np.random.default_rng(0)
+    keys = [(int(a), int(b)) for a, b in zip(rng.integers(0, 5, 300), rng.integers(0, 3, 300))]
+    keys += [(9, 9)] * 60
+    rng.shuffle(keys)
+    out = device_sort(keys, block=16)
+    assert None not in out
+    assert out == sorted(keys)
