@@ -13,7 +13,8 @@
 //
 // Two forms of the candidate contraction:
 //   plain: one thread per query, FP32 gather + dot products on VALU (the baseline form);
-//   mfma:  queries sorted by predicted grid cell, 32 spatial neighbours per wave, the union of
+//   mfma:  32 spatially neighbouring queries per wave (each keypoint set is bucketed by grid
+//          cell in tile-major cell order, which the spatial grid needs anyway), the union of
 //          their candidate cells contracted on MFMA (32x32x16 BF16, FP32 accumulation) with
 //          the prior-window mask and the top-2 selection fused into the epilogue.
 #include "fsx_common.h"
@@ -29,6 +30,8 @@ struct MImage {
     int n;              // keypoints
     int W, H;           // image size (inside test on the B side)
     int gw, gh;         // spatial grid (cells of `radius` px covering [-2r, W+2r) x [-2r, H+2r))
+    int tw4;            // tiles of 4x4 cells per tile row; cells are numbered tile-major
+    int ncell;          // cell slots (whole tiles)
     long long cell_off; // first cell of this image in the cell arrays
 };
 
@@ -40,6 +43,7 @@ struct MPair {
     long long q_off;    // first global query (prefix of n_a)
     long long t_off;    // first per-target slot (prefix of n_b)
     long long o_off;    // first output slot (prefix of min(n_a, n_b))
+    long long c_off;    // first MFMA chunk (prefix of ceil(n_a / 32))
 };
 
 struct MatchBatch {
@@ -64,17 +68,11 @@ struct MatchBatch {
     double* kth;                  // [n_query][2] kc-th nearest (d2, idx) threshold, kc > 0
     unsigned long long* tkey;     // [n_targets] min b1 bits per target
     int* tq;                      // [n_targets] min query among equal b1
-    unsigned long long* qkey;     // [2 * n_query] sort keys of the mfma form
-    unsigned long long* qkey2;    // [2 * n_query] merge pong
-    int* qsorted;                 // [n_query] global query ids in (pair, cell) order
-    int* counters;                // [0] valid queries, [1] chunks, [2] overflow flags
-    int* pcount;                  // [n_pair] valid queries per pair
-    int* chunk;                   // [3 * max_chunks] (pair, start, count)
-    int max_chunks;
+    int* counters;                // [0..3] flags
+    long long n_chunks;           // sum over pairs of ceil(n_a / 32)
     int* out_q;                   // [Σ min(n_a, n_b)]
     int* out_t;
     int* out_n;                   // [n_pair]
-    long long sort_cap;           // power of two >= n_query, >= 1024
 };
 
 // ------------------------------------------------------------------------------------------
@@ -98,6 +96,18 @@ __device__ __forceinline__ int find_image(const MatchBatch& B, long long k) {
     return lo;
 }
 
+// tile-major cell numbering: 4x4-cell tiles row by row, cells row by row inside a tile, so a
+// run of consecutive cells (and of the keypoints bucketed into them) is spatially compact
+__device__ __forceinline__ int cell_id(const MImage& im, int cx, int cy) {
+    return (((cy >> 2) * im.tw4 + (cx >> 2)) << 4) | ((cy & 3) << 2) | (cx & 3);
+}
+
+__device__ __forceinline__ void cell_xy(const MImage& im, int id, int& cx, int& cy) {
+    const int tile = id >> 4;
+    cy = (tile / im.tw4) * 4 + ((id >> 2) & 3);
+    cx = (tile % im.tw4) * 4 + (id & 3);
+}
+
 __device__ __forceinline__ void cell_of(const MatchBatch& B, const MImage& im, double x, double y, int& cx, int& cy) {
     cx = (int)floor((x + 2.0 * B.radius) / B.radius);
     cy = (int)floor((y + 2.0 * B.radius) / B.radius);
@@ -112,10 +122,7 @@ __global__ void k_m_reset(MatchBatch B) {
         B.tq[i] = 0x7fffffff;
     }
     for (long long i = tid; i < B.n_cells; i += stride) B.cell_cur[i] = 0;
-    for (long long i = tid; i < B.n_pair; i += stride) {
-        B.out_n[i] = 0;
-        B.pcount[i] = 0;
-    }
+    for (long long i = tid; i < B.n_pair; i += stride) B.out_n[i] = 0;
     if (tid < 4) B.counters[tid] = 0;
 }
 
@@ -138,14 +145,14 @@ __global__ void k_m_norms_count(MatchBatch B) {
     const MImage& im = B.img[ii];
     int cx, cy;
     cell_of(B, im, B.xy[2 * k], B.xy[2 * k + 1], cx, cy);
-    atomicAdd(&B.cell_cur[im.cell_off + (long long)cy * im.gw + cx], 1);
+    atomicAdd(&B.cell_cur[im.cell_off + cell_id(im, cx, cy)], 1);
 }
 
 // one block per image: exclusive scan of the cell counts -> cell_start; cursors reset
 __global__ __launch_bounds__(256) void k_m_grid_scan(MatchBatch B) {
     __shared__ int s_w[4];
     const MImage& im = B.img[blockIdx.x];
-    const int nc = im.gw * im.gh;
+    const int nc = im.ncell;
     int* start = B.cell_start + im.cell_off + blockIdx.x;   // nc + 1 entries per image
     int* cnt = B.cell_cur + im.cell_off;
     int carry = 0;
@@ -180,7 +187,7 @@ __global__ void k_m_grid_fill(MatchBatch B) {
     const MImage& im = B.img[ii];
     int cx, cy;
     cell_of(B, im, B.xy[2 * k], B.xy[2 * k + 1], cx, cy);
-    const long long c = (long long)cy * im.gw + cx;
+    const long long c = cell_id(im, cx, cy);
     const int* start = B.cell_start + im.cell_off + ii;
     const int slot = start[c] + atomicAdd(&B.cell_cur[im.cell_off + c], 1);
     B.cell_idx[im.off + slot] = (int)(k - im.off);
@@ -220,7 +227,7 @@ __global__ void k_m_predict(MatchBatch B) {
     } else if (px > -B.radius && px < ib.W + B.radius && py > -B.radius && py < ib.H + B.radius && ib.n > 0) {
         int cx, cy;
         cell_of(B, ib, px, py, cx, cy);
-        cell = cy * ib.gw + cx;
+        cell = cell_id(ib, cx, cy);
     }
     B.qcell[g] = cell;
     B.best[g] = -1;
@@ -247,11 +254,12 @@ __global__ void k_m_knn(MatchBatch B) {
         const int kc = B.kc < KMAX ? B.kc : KMAX;
         const MImage& ib = B.img[P.b];
         const double px = B.pred[2 * g], py = B.pred[2 * g + 1];
-        const int cx0 = cell % ib.gw, cy0 = cell / ib.gw;
+        int cx0, cy0;
+        cell_xy(ib, cell, cx0, cy0);
         const int* start = B.cell_start + ib.cell_off + P.b;
         for (int cy = max(cy0 - 1, 0); cy <= min(cy0 + 1, ib.gh - 1); cy++)
             for (int cx = max(cx0 - 1, 0); cx <= min(cx0 + 1, ib.gw - 1); cx++) {
-                const int c = cy * ib.gw + cx;
+                const int c = cell_id(ib, cx, cy);
                 for (int s = start[c]; s < start[c + 1]; s++) {
                     const int t = B.cell_idx[ib.off + s];
                     const double dx = B.xy[2 * (ib.off + t)] - px, dy = B.xy[2 * (ib.off + t) + 1] - py;
@@ -307,8 +315,23 @@ __device__ __forceinline__ void accept_query(const MatchBatch& B, long long g, c
     }
 }
 
-// the plain FP32 GPU form: one thread per query, candidates gathered from global memory,
-// d = na + nb - 2 a.b with FP32 products on VALU
+// the plain FP32 GPU form: one thread per query; the query descriptor stays in registers as
+// float32, each candidate descriptor is gathered from global memory (16-byte loads) and the
+// dot product runs on VALU in FP32: d = na + nb - 2 a.b (exact for uint8 inputs)
+__device__ __forceinline__ float u8dot_f32(const float* qa, const uint4* cb) {
+    float dot = 0.f;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const uint4 v = cb[i];
+        const unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int h = 0; h < 4; h++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) dot += qa[16 * i + 4 * h + e] * (float)((w[h] >> (8 * e)) & 0xffu);
+    }
+    return dot;
+}
+
 __global__ __launch_bounds__(256) void k_m_plain(MatchBatch B) {
     const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
     if (g >= B.n_query) return;
@@ -319,14 +342,24 @@ __global__ __launch_bounds__(256) void k_m_plain(MatchBatch B) {
     const MImage& ia = B.img[P.a];
     const MImage& ib = B.img[P.b];
     const long long ka = ia.off + (g - P.q_off);
-    const uint8_t* da = B.desc + ka * 128;
+    float qa[128];
+    {
+        const uint4* q4 = (const uint4*)(B.desc + ka * 128);
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const uint4 v = q4[i];
+            const unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int h = 0; h < 4; h++)
+#pragma unroll
+                for (int e = 0; e < 4; e++) qa[16 * i + 4 * h + e] = (float)((w[h] >> (8 * e)) & 0xffu);
+        }
+    }
     const float na = (float)B.norms[ka];
     float v1 = INFINITY, v2 = INFINITY;
     int i1 = -1;
     auto eval = [&](int t) {
-        const uint8_t* db = B.desc + (ib.off + t) * 128;
-        float dot = 0.f;
-        for (int k = 0; k < 128; k++) dot += (float)da[k] * (float)db[k];
+        const float dot = u8dot_f32(qa, (const uint4*)(B.desc + (ib.off + t) * 128));
         const float d = fmaxf((na + (float)B.norms[ib.off + t]) - 2.0f * dot, 0.0f);
         top2_push(d, t, v1, i1, v2);
     };
@@ -334,11 +367,12 @@ __global__ __launch_bounds__(256) void k_m_plain(MatchBatch B) {
         for (int t = 0; t < ib.n; t++) eval(t);
     } else {
         const double px = B.pred[2 * g], py = B.pred[2 * g + 1];
-        const int cx0 = cell % ib.gw, cy0 = cell / ib.gw;
+        int cx0, cy0;
+        cell_xy(ib, cell, cx0, cy0);
         const int* start = B.cell_start + ib.cell_off + P.b;
         for (int cy = max(cy0 - 1, 0); cy <= min(cy0 + 1, ib.gh - 1); cy++)
             for (int cx = max(cx0 - 1, 0); cx <= min(cx0 + 1, ib.gw - 1); cx++) {
-                const int c = cy * ib.gw + cx;
+                const int c = cell_id(ib, cx, cy);
                 for (int s = start[c]; s < start[c + 1]; s++) {
                     const int t = B.cell_idx[ib.off + s];
                     const double dx = B.xy[2 * (ib.off + t)] - px, dy = B.xy[2 * (ib.off + t) + 1] - py;
@@ -394,170 +428,18 @@ __global__ __launch_bounds__(256) void k_m_compact(MatchBatch B) {
 }
 
 // ------------------------------------------------------------------------------------------
-// MFMA form.  Queries are sorted by (pair, 4x4-cell tile, cell) so that 32 consecutive ones
-// are spatial neighbours; a wave contracts them against the union of their 3x3 candidate
-// cells, 32 candidates per 32x32x16 BF16 MFMA tile (uint8 -> bf16 is exact).
+// MFMA form.  A chunk = 32 consecutive keypoints of the query image in its cell bucketing
+// (tile-major cells), so the chunk's predictions land in a compact region of B (the prior is a
+// smooth mapping).  One wave per chunk: the union of the queries' 3x3 candidate cells is
+// listed once, its candidates are contracted against the 32 queries on MFMA, 32 candidates per
+// 32x32x16 BF16 tile (uint8 -> bf16 is exact, FP32 sums of uint8 products are exact), and
+// the epilogue applies the prior-window mask and the top-2 update.
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
 
-__device__ __forceinline__ unsigned long long qsort_key_hi(int p, int tilekey) {
-    return ((unsigned long long)(unsigned)p << 32) | (unsigned)tilekey;
-}
-
-// sort keys of the queries: (pair, tile-major cell) ; invalid queries sort last
-__global__ void k_m_qkeys(MatchBatch B) {
-    const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (g >= B.sort_cap) return;
-    // every key is unique (k1 = query id, also for the sentinels of invalid queries)
-    unsigned long long k0 = ~0ull, k1 = (unsigned long long)g;
-    if (g < B.n_query) {
-        const int cell = B.qcell[g];
-        if (cell >= 0) {
-            const int p = find_pair(B, g);
-            const MPair& P = B.pair[p];
-            const MImage& ib = B.img[P.b];
-            int key = 0;
-            if (!P.dense) {
-                const int cx = cell % ib.gw, cy = cell / ib.gw;
-                const int tw = (ib.gw + 3) / 4;
-                key = (((cy >> 2) * tw + (cx >> 2)) << 4) | ((cy & 3) << 2) | (cx & 3);
-            }
-            k0 = qsort_key_hi(p, key);
-            k1 = (unsigned long long)g;
-            atomicAdd(&B.counters[0], 1);
-            atomicAdd(&B.pcount[p], 1);
-        }
-    }
-    B.qkey[2 * g] = k0;
-    B.qkey[2 * g + 1] = k1;
-}
-
-__device__ __forceinline__ bool key_lt(unsigned long long a0, unsigned long long a1, unsigned long long b0,
-                                       unsigned long long b1) {
-    return a0 < b0 || (a0 == b0 && a1 < b1);
-}
-
-// generic key-pair sort: 1024-element bitonic blocks, then rank merges (deterministic)
-__global__ __launch_bounds__(256) void k_keysort_local(unsigned long long* keys, long long n) {
-    __shared__ unsigned long long s0[1024], s1[1024];
-    const long long base = (long long)blockIdx.x * 1024;
-    if (base >= n) return;
-    for (int t = threadIdx.x; t < 1024; t += 256) {
-        long long g = base + t;
-        s0[t] = g < n ? keys[2 * g] : ~0ull;
-        s1[t] = g < n ? keys[2 * g + 1] : ~0ull;
-    }
-    __syncthreads();
-    for (int k = 2; k <= 1024; k <<= 1)
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int t = threadIdx.x; t < 512; t += 256) {
-                int i = 2 * t - (t & (j - 1));
-                int q = i + j;
-                bool up = (i & k) == 0;
-                unsigned long long a0 = s0[i], a1 = s1[i], b0 = s0[q], b1 = s1[q];
-                bool sw = up ? key_lt(b0, b1, a0, a1) : key_lt(a0, a1, b0, b1);
-                if (sw) {
-                    s0[i] = b0; s1[i] = b1;
-                    s0[q] = a0; s1[q] = a1;
-                }
-            }
-            __syncthreads();
-        }
-    for (int t = threadIdx.x; t < 1024; t += 256) {
-        long long g = base + t;
-        if (g < n) {
-            keys[2 * g] = s0[t];
-            keys[2 * g + 1] = s1[t];
-        }
-    }
-}
-
-__global__ __launch_bounds__(256) void k_keysort_merge(const unsigned long long* src, unsigned long long* dst,
-                                                       long long n, long long w) {
-    for (long long p = (long long)blockIdx.x * 256 + threadIdx.x; p < n; p += (long long)gridDim.x * 256) {
-        const unsigned long long x0 = src[2 * p], x1 = src[2 * p + 1];
-        const long long s = (p / (2 * w)) * (2 * w);
-        long long lo, hi, out;
-        if (p < s + w) {
-            lo = s + w;
-            hi = min(s + 2 * w, n);
-            out = p;
-        } else {
-            lo = s;
-            hi = s + w;
-            out = p - w;
-        }
-        // stable merge: a left-run element counts right-run keys < x, a right-run element
-        // counts left-run keys <= x, so equal keys keep their order and never collide
-        const bool left = p < s + w;
-        long long a = lo, b = hi;
-        while (a < b) {
-            long long m = (a + b) >> 1;
-            const unsigned long long m0 = src[2 * m], m1 = src[2 * m + 1];
-            const bool before = left ? key_lt(m0, m1, x0, x1) : !key_lt(x0, x1, m0, m1);
-            if (before) a = m + 1;
-            else b = m;
-        }
-        out += a - lo;
-        dst[2 * out] = x0;
-        dst[2 * out + 1] = x1;
-    }
-}
-
-// chunks of <= 32 sorted queries of one pair (single block): the sorted order is by pair, so
-// pair p's queries start at the prefix sum of the per-pair counts
-__global__ __launch_bounds__(256) void k_m_chunks(MatchBatch B, int sorted_in_qkey) {
-    __shared__ int s_w[8];
-    const unsigned long long* keys = sorted_in_qkey ? B.qkey : B.qkey2;
-    const int nv = B.counters[0];
-    for (int i = threadIdx.x; i < nv; i += 256) B.qsorted[i] = (int)keys[2 * (long long)i + 1];
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    int carry_q = 0, carry_c = 0;
-    for (int base = 0; base < B.n_pair; base += 256) {
-        const int p = base + tid;
-        const int cq = p < B.n_pair ? B.pcount[p] : 0;
-        const int cc = (cq + 31) / 32;
-        int xq = cq, xc = cc;
-        for (int off = 1; off < 64; off <<= 1) {
-            int yq = __shfl_up(xq, off), yc = __shfl_up(xc, off);
-            if (lane >= off) {
-                xq += yq;
-                xc += yc;
-            }
-        }
-        if (lane == 63) {
-            s_w[wv] = xq;
-            s_w[4 + wv] = xc;
-        }
-        __syncthreads();
-        int pq = 0, pc = 0;
-        for (int w2 = 0; w2 < wv; w2++) {
-            pq += s_w[w2];
-            pc += s_w[4 + w2];
-        }
-        const int tq = s_w[0] + s_w[1] + s_w[2] + s_w[3], tc = s_w[4] + s_w[5] + s_w[6] + s_w[7];
-        const int qstart = carry_q + pq + xq - cq, cstart = carry_c + pc + xc - cc;
-        for (int c = 0; c < cc; c++) {
-            const int k = cstart + c;
-            if (k < B.max_chunks) {
-                B.chunk[3 * k] = p;
-                B.chunk[3 * k + 1] = qstart + 32 * c;
-                B.chunk[3 * k + 2] = min(32, cq - 32 * c);
-            }
-        }
-        carry_q += tq;
-        carry_c += tc;
-        __syncthreads();
-    }
-    if (tid == 0) {
-        B.counters[1] = carry_c;
-        if (carry_c > B.max_chunks) B.counters[2] |= 1;
-    }
-}
-
-constexpr int MF_MAXCELLS = 1024;   // bitmap of candidate cells per chunk (32 x 32 cells)
-constexpr int MF_MAXCAND = 2048;    // candidate list per chunk held in LDS
+constexpr int MF_CAP = 512;     // candidates per pass staged in LDS
+constexpr int MF_MAXC = 256;    // listed candidate cells per chunk
 
 __device__ __forceinline__ bf16x8_t u8x8_to_bf16(uint2 v) {
     bf16x8_t r;
@@ -570,67 +452,152 @@ __device__ __forceinline__ bf16x8_t u8x8_to_bf16(uint2 v) {
     return r;
 }
 
+__device__ __forceinline__ int find_chunk_pair(const MatchBatch& B, long long ch) {
+    int lo = 0, hi = B.n_pair - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (B.pair[mid].c_off <= ch) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int wave_excl_scan(int v, int lane, int& total) {
+    int x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    total = __shfl(x, 63);
+    return x - v;
+}
+
+// per-query evaluation of the 3x3 candidate cells (chunks whose cells do not fit the window)
+__device__ void query_direct(const MatchBatch& B, const MPair& P, const MImage& ia, const MImage& ib, int gq, int cell,
+                             double qpx, double qpy, float qn, float& v1, int& i1, float& v2) {
+    v1 = INFINITY;
+    v2 = INFINITY;
+    i1 = -1;
+    const long long ka = ia.off + (gq - P.q_off);
+    const uint8_t* da = B.desc + ka * 128;
+    int cx0, cy0;
+    cell_xy(ib, cell, cx0, cy0);
+    const int* cs = B.cell_start + ib.cell_off + P.b;
+    for (int cy = max(cy0 - 1, 0); cy <= min(cy0 + 1, ib.gh - 1); cy++)
+        for (int cx = max(cx0 - 1, 0); cx <= min(cx0 + 1, ib.gw - 1); cx++) {
+            const int cc = cell_id(ib, cx, cy);
+            for (int s = cs[cc]; s < cs[cc + 1]; s++) {
+                const int t = B.cell_idx[ib.off + s];
+                const double dx = B.xy[2 * (ib.off + t)] - qpx, dy = B.xy[2 * (ib.off + t) + 1] - qpy;
+                if (!in_candidates(B, gq, dx * dx + dy * dy, t)) continue;
+                const unsigned* a4 = (const unsigned*)da;
+                const unsigned* b4 = (const unsigned*)(B.desc + (ib.off + t) * 128);
+                unsigned dot = 0;
+                for (int k = 0; k < 32; k++) dot = __builtin_amdgcn_udot4(a4[k], b4[k], dot, false);
+                const float d = fmaxf((qn + (float)B.norms[ib.off + t]) - 2.0f * (float)dot, 0.0f);
+                top2_push(d, t, v1, i1, v2);
+            }
+        }
+}
+
 __global__ __launch_bounds__(64) void k_m_mfma(MatchBatch B) {
-    __shared__ int s_cand[MF_MAXCAND];
-    __shared__ unsigned s_mark[MF_MAXCELLS / 32];
+    __shared__ int s_cand[MF_CAP];
+    __shared__ double s_cx[MF_CAP], s_cy[MF_CAP];
+    __shared__ float s_cn[MF_CAP];
+    __shared__ unsigned s_mark[32];
+    __shared__ int s_cells[MF_MAXC], s_coff[MF_MAXC + 1];
     __shared__ int s_q[32], s_cell[32];
     __shared__ double s_px[32], s_py[32];
-    __shared__ int s_n;
     __shared__ float s_v1[64], s_v2[64];
     __shared__ int s_i1[64];
     const int lane = threadIdx.x;
-    const int nch = min(B.counters[1], B.max_chunks);
-    for (int ch = blockIdx.x; ch < nch; ch += gridDim.x) {
-        const int p = B.chunk[3 * ch], start = B.chunk[3 * ch + 1], cnt = B.chunk[3 * ch + 2];
+    for (long long ch = blockIdx.x; ch < B.n_chunks; ch += gridDim.x) {
+        const int p = find_chunk_pair(B, ch);
         const MPair& P = B.pair[p];
         const MImage& ia = B.img[P.a];
         const MImage& ib = B.img[P.b];
+        const int k0 = (int)(ch - P.c_off) * 32;
         if (lane < 32) {
-            const int g = lane < cnt ? B.qsorted[start + lane] : -1;
+            int g = -1;
+            const int kk = k0 + lane;
+            if (kk < ia.n) {
+                const long long g0 = P.q_off + B.cell_idx[ia.off + kk];
+                if (B.qcell[g0] >= 0) g = (int)g0;
+            }
             s_q[lane] = g;
             s_cell[lane] = g >= 0 ? B.qcell[g] : -1;
             s_px[lane] = g >= 0 ? B.pred[2 * (long long)g] : 0.0;
             s_py[lane] = g >= 0 ? B.pred[2 * (long long)g + 1] : 0.0;
+            s_mark[lane] = 0u;
         }
-        if (lane < MF_MAXCELLS / 32) s_mark[lane] = 0u;
-        if (lane == 0) s_n = 0;
         __syncthreads();
-        // candidate cells: union of the 3x3 neighbourhoods, inside a 32x32 window of cells
-        int cx_lo = 1 << 30, cy_lo = 1 << 30;
-        for (int i = 0; i < cnt; i++) {
-            const int c = s_cell[i];
-            cx_lo = min(cx_lo, c % ib.gw);
-            cy_lo = min(cy_lo, c / ib.gw);
+        const int qc = lane & 31;
+        const int gq = s_q[qc];
+        if (__ballot(gq >= 0) == 0ull) {       // wave-uniform: nothing to match in this chunk
+            __syncthreads();
+            continue;
+        }
+        // window origin: min cell coordinates of the valid queries, minus one
+        int cxq = 1 << 30, cyq = 1 << 30;
+        if (gq >= 0) cell_xy(ib, s_cell[qc], cxq, cyq);
+        int cx_lo = cxq, cy_lo = cyq;
+        for (int off = 32; off >= 1; off >>= 1) {
+            cx_lo = min(cx_lo, __shfl_xor(cx_lo, off));
+            cy_lo = min(cy_lo, __shfl_xor(cy_lo, off));
         }
         cx_lo = max(cx_lo - 1, 0);
         cy_lo = max(cy_lo - 1, 0);
         bool overflow = false;
-        if (P.dense) {
-            // brute force: every keypoint of b is a candidate (streamed in list-sized passes)
-        } else if (lane < cnt) {
-            const int c = s_cell[lane];
-            const int cx0 = c % ib.gw, cy0 = c / ib.gw;
+        if (!P.dense && lane < 32 && gq >= 0) {
             const int* cs = B.cell_start + ib.cell_off + P.b;
-            for (int cy = max(cy0 - 1, 0); cy <= min(cy0 + 1, ib.gh - 1); cy++)
-                for (int cx = max(cx0 - 1, 0); cx <= min(cx0 + 1, ib.gw - 1); cx++) {
+            for (int cy = max(cyq - 1, 0); cy <= min(cyq + 1, ib.gh - 1); cy++)
+                for (int cx = max(cxq - 1, 0); cx <= min(cxq + 1, ib.gw - 1); cx++) {
                     const int bx = cx - cx_lo, by = cy - cy_lo;
-                    const int cc = cy * ib.gw + cx;
-                    // a cell larger than the LDS list, or a window wider than 32x32 cells:
-                    // the queries of this chunk take the per-query path below
-                    if (bx < 32 && by < 32 && cs[cc + 1] - cs[cc] <= MF_MAXCAND) atomicOr(&s_mark[by], 1u << bx);
+                    const int cc = cell_id(ib, cx, cy);
+                    if (bx < 32 && by < 32 && cs[cc + 1] - cs[cc] <= MF_CAP) atomicOr(&s_mark[by], 1u << bx);
                     else overflow = true;
                 }
         }
-        const bool any_over = __ballot(overflow) != 0ull;
+        bool any_over = __ballot(overflow) != 0ull;
         __syncthreads();
-        // per query lane-pair state: lanes l and l+32 handle query column l%32
+        // list the marked cells (window-row order) and their candidate offsets
+        int total = 0;
+        int ncell = 0;
+        if (P.dense) {
+            total = ib.n;
+        } else if (!any_over) {
+            const unsigned word = lane < 32 ? s_mark[lane] : 0u;
+            const int wo = wave_excl_scan(__popc(word), lane, ncell);
+            if (ncell > MF_MAXC) {
+                any_over = true;
+            } else {
+                int k = wo;
+                for (unsigned w = word; w; w &= w - 1) {
+                    const int bx = __ffs(w) - 1;
+                    s_cells[k++] = cell_id(ib, cx_lo + bx, cy_lo + lane);
+                }
+                __syncthreads();
+                const int* cs = B.cell_start + ib.cell_off + P.b;
+                int carry = 0;
+                for (int c0 = 0; c0 < ncell; c0 += 64) {
+                    const int c = c0 + lane;
+                    const int n = c < ncell ? cs[s_cells[c] + 1] - cs[s_cells[c]] : 0;
+                    int tot;
+                    const int ex = wave_excl_scan(n, lane, tot);
+                    if (c < ncell) s_coff[c] = carry + ex;
+                    carry += tot;
+                }
+                if (lane == 0) s_coff[ncell] = carry;
+                total = carry;
+                __syncthreads();
+            }
+        }
+        // state of this lane's query column over the candidate passes
         float v1 = INFINITY, v2 = INFINITY;
         int i1 = -1;
-        const int qc = lane & 31;
-        const int gq = s_q[qc];
-        // query operand: 8 K-steps of 8 bf16 (k = 16 s + (lane/32)*8 + e)
         bf16x8_t qop[8];
         float qn = 0.f;
+        const double qpx = gq >= 0 ? s_px[qc] : 0.0, qpy = gq >= 0 ? s_py[qc] : 0.0;
         if (gq >= 0) {
             const long long ka = ia.off + (gq - P.q_off);
             const uint2* qa = (const uint2*)(B.desc + ka * 128);
@@ -641,70 +608,61 @@ __global__ __launch_bounds__(64) void k_m_mfma(MatchBatch B) {
 #pragma unroll
             for (int s = 0; s < 8; s++) qop[s] = u8x8_to_bf16(make_uint2(0, 0));
         }
-        const double qpx = gq >= 0 ? s_px[qc] : 0.0, qpy = gq >= 0 ? s_py[qc] : 0.0;
-        // candidate passes
-        int pass_c = 0;           // dense: next b index; guided: next cell (row-major in window)
-        bool more = true;
-        while (more) {
-            // build the candidate list of this pass
-            if (lane == 0) {
-                int n = 0;
-                if (P.dense) {
-                    while (pass_c < ib.n && n < MF_MAXCAND) s_cand[n++] = pass_c++;
-                    more = pass_c < ib.n;
-                } else if (!any_over) {
-                    const int* cs = B.cell_start + ib.cell_off + P.b;
-                    more = false;
-                    while (pass_c < 32 * 32) {
-                        const int by = pass_c >> 5, bx = pass_c & 31;
-                        if (s_mark[by] & (1u << bx)) {
-                            const int c = (cy_lo + by) * ib.gw + cx_lo + bx;
-                            const int a0 = cs[c], a1 = cs[c + 1];
-                            if (n + (a1 - a0) > MF_MAXCAND) {
-                                more = true;
-                                break;
+        if (!any_over) {
+            const int* cs = B.cell_start + ib.cell_off + P.b;
+            for (int base = 0; base < total; base += MF_CAP) {
+                const int np = min(MF_CAP, total - base);
+                for (int j = lane; j < np; j += 64) {
+                    int t;
+                    if (P.dense) {
+                        t = base + j;
+                    } else {
+                        // listed cell holding candidate position base + j: the last cell whose
+                        // offset is <= pos (binary search over s_coff[0..ncell))
+                        const int pos = base + j;
+                        int lo = 0, hi = ncell - 1;
+                        while (lo < hi) {
+                            const int mid = (lo + hi + 1) >> 1;
+                            if (s_coff[mid] <= pos) lo = mid;
+                            else hi = mid - 1;
+                        }
+                        t = B.cell_idx[ib.off + cs[s_cells[lo]] + (pos - s_coff[lo])];
+                        s_cx[j] = B.xy[2 * (ib.off + t)];
+                        s_cy[j] = B.xy[2 * (ib.off + t) + 1];
+                    }
+                    s_cand[j] = t;
+                    s_cn[j] = (float)B.norms[ib.off + t];
+                }
+                __syncthreads();
+                for (int t0 = 0; t0 < np; t0 += 32) {
+                    const int tr = t0 + (lane & 31);
+                    const int tc = tr < np ? s_cand[tr] : -1;
+                    const uint2* cb = (const uint2*)(B.desc + (ib.off + (tc >= 0 ? tc : 0)) * 128);
+                    f32x16_t acc = {};
+#pragma unroll
+                    for (int s = 0; s < 8; s++) {
+                        const bf16x8_t a = u8x8_to_bf16(tc >= 0 ? cb[2 * s + (lane >> 5)] : make_uint2(0, 0));
+                        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, qop[s], acc, 0, 0, 0);
+                    }
+                    if (gq >= 0) {
+#pragma unroll
+                        for (int i = 0; i < 4; i++)
+#pragma unroll
+                            for (int j = 0; j < 4; j++) {
+                                const int r = t0 + i * 8 + (lane >> 5) * 4 + j;
+                                if (r >= np) continue;
+                                const int t = s_cand[r];
+                                if (!P.dense) {
+                                    const double dx = s_cx[r] - qpx, dy = s_cy[r] - qpy;
+                                    if (!in_candidates(B, gq, dx * dx + dy * dy, t)) continue;
+                                }
+                                const float d = fmaxf((qn + s_cn[r]) - 2.0f * acc[i * 4 + j], 0.0f);
+                                top2_push(d, t, v1, i1, v2);
                             }
-                            for (int s = a0; s < a1; s++) s_cand[n++] = B.cell_idx[ib.off + s];
-                        }
-                        pass_c++;
                     }
-                } else {
-                    more = false;
                 }
-                s_n = n;
+                __syncthreads();
             }
-            more = __shfl(more ? 1 : 0, 0) != 0;
-            pass_c = __shfl(pass_c, 0);
-            __syncthreads();
-            const int n = s_n;
-            for (int t0 = 0; t0 < n; t0 += 32) {
-                // candidate operand: row = lane%32 of this tile
-                const int tr = t0 + (lane & 31);
-                const int tc = tr < n ? s_cand[tr] : -1;
-                const uint2* cb = (const uint2*)(B.desc + (ib.off + (tc >= 0 ? tc : 0)) * 128);
-                f32x16_t acc = {};
-#pragma unroll
-                for (int s = 0; s < 8; s++) {
-                    const bf16x8_t a = u8x8_to_bf16(tc >= 0 ? cb[2 * s + (lane >> 5)] : make_uint2(0, 0));
-                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, qop[s], acc, 0, 0, 0);
-                }
-                // epilogue: acc[i*4+j] = dot(cand row i*8 + (lane/32)*4 + j, query col lane%32)
-#pragma unroll
-                for (int i = 0; i < 4; i++)
-#pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        const int r = t0 + i * 8 + (lane >> 5) * 4 + j;
-                        if (r >= n || gq < 0) continue;
-                        const int t = s_cand[r];
-                        if (!P.dense) {
-                            const double dx = B.xy[2 * (ib.off + t)] - qpx, dy = B.xy[2 * (ib.off + t) + 1] - qpy;
-                            if (!in_candidates(B, gq, dx * dx + dy * dy, t)) continue;
-                        }
-                        const float d = fmaxf((qn + (float)B.norms[ib.off + t]) - 2.0f * acc[i * 4 + j], 0.0f);
-                        top2_push(d, t, v1, i1, v2);
-                    }
-            }
-            __syncthreads();
         }
         // merge the two half-waves of each query column
         s_v1[lane] = v1;
@@ -712,33 +670,13 @@ __global__ __launch_bounds__(64) void k_m_mfma(MatchBatch B) {
         s_i1[lane] = i1;
         __syncthreads();
         if (lane < 32 && gq >= 0) {
-            const float w1 = s_v1[lane + 32], w2 = s_v2[lane + 32];
-            const int j1 = s_i1[lane + 32];
-            if (j1 >= 0) top2_push(w1, j1, v1, i1, v2);
-            v2 = fminf(v2, w2);
             if (any_over) {
-                // candidate cells spread beyond the 32x32 window: evaluate this query alone
-                v1 = INFINITY; v2 = INFINITY; i1 = -1;
-                const long long ka = ia.off + (gq - P.q_off);
-                const uint8_t* da = B.desc + ka * 128;
-                const int c = s_cell[lane];
-                const int cx0 = c % ib.gw, cy0 = c / ib.gw;
-                const int* cs = B.cell_start + ib.cell_off + P.b;
-                for (int cy = max(cy0 - 1, 0); cy <= min(cy0 + 1, ib.gh - 1); cy++)
-                    for (int cx = max(cx0 - 1, 0); cx <= min(cx0 + 1, ib.gw - 1); cx++) {
-                        const int cc = cy * ib.gw + cx;
-                        for (int s = cs[cc]; s < cs[cc + 1]; s++) {
-                            const int t = B.cell_idx[ib.off + s];
-                            const double dx = B.xy[2 * (ib.off + t)] - qpx, dy = B.xy[2 * (ib.off + t) + 1] - qpy;
-                            if (!in_candidates(B, gq, dx * dx + dy * dy, t)) continue;
-                            const uint8_t* db = B.desc + (ib.off + t) * 128;
-                            unsigned dot = 0;
-                            for (int k = 0; k < 128; k += 4)
-                                dot = __builtin_amdgcn_udot4(*(const unsigned*)(da + k), *(const unsigned*)(db + k), dot, false);
-                            const float d = fmaxf((qn + (float)B.norms[ib.off + t]) - 2.0f * (float)dot, 0.0f);
-                            top2_push(d, t, v1, i1, v2);
-                        }
-                    }
+                query_direct(B, P, ia, ib, gq, s_cell[lane], qpx, qpy, qn, v1, i1, v2);
+            } else {
+                const float w1 = s_v1[lane + 32], w2 = s_v2[lane + 32];
+                const int j1 = s_i1[lane + 32];
+                if (j1 >= 0) top2_push(w1, j1, v1, i1, v2);
+                v2 = fminf(v2, w2);
             }
             accept_query(B, gq, P, v1, i1, v2);
         }
@@ -754,11 +692,9 @@ static inline size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
 struct MatchPlanHost {
     std::vector<MImage> img;
     std::vector<MPair> pair;
-    long long n_kp = 0, n_query = 0, n_cells = 0, n_targets = 0, n_out = 0, sort_cap = 1024;
-    int max_chunks = 0;
+    long long n_kp = 0, n_query = 0, n_cells = 0, n_targets = 0, n_out = 0, n_chunks = 0;
     size_t off_img, off_pair, off_norms, off_cstart, off_ccur, off_cidx, off_pred, off_qcell, off_best, off_b1,
-        off_kth, off_tkey, off_tq, off_qkey, off_qkey2, off_qsorted, off_cnt, off_pcount, off_chunk, off_outq, off_outt,
-        off_outn, off_batch, total;
+        off_kth, off_tkey, off_tq, off_cnt, off_outq, off_outt, off_outn, off_batch, total;
 };
 
 static int match_plan(int n_img, const int* n_kp, const int* W, const int* H, int n_pair, const int* pa,
@@ -776,9 +712,11 @@ static int match_plan(int n_img, const int* n_kp, const int* W, const int* H, in
         m.H = H[i];
         m.gw = (int)ceil((W[i] + 4.0 * radius) / radius);
         m.gh = (int)ceil((H[i] + 4.0 * radius) / radius);
+        m.tw4 = (m.gw + 3) / 4;
+        m.ncell = m.tw4 * ((m.gh + 3) / 4) * 16;
         m.cell_off = coff;
         off += n_kp[i];
-        coff += (long long)m.gw * m.gh;
+        coff += m.ncell;
     }
     M.n_kp = off;
     M.n_cells = coff;
@@ -795,18 +733,16 @@ static int match_plan(int n_img, const int* n_kp, const int* W, const int* H, in
         P.q_off = qo;
         P.t_off = to;
         P.o_off = oo;
+        P.c_off = chunks;
         qo += M.img[P.a].n;
         to += M.img[P.b].n;
         oo += std::min(M.img[P.a].n, M.img[P.b].n);
-        chunks += (M.img[P.a].n + 31) / 32 + 1;
+        chunks += (M.img[P.a].n + 31) / 32;
     }
     M.n_query = qo;
     M.n_targets = to;
     M.n_out = oo;
-    M.max_chunks = (int)std::max(chunks, 1LL);
-    long long cap = 1024;
-    while (cap < qo) cap <<= 1;
-    M.sort_cap = cap;
+    M.n_chunks = chunks;
     size_t o = 0;
     M.off_batch = o; o = a256(o + sizeof(MatchBatch));
     M.off_img = o; o = a256(o + sizeof(MImage) * n_img);
@@ -822,12 +758,7 @@ static int match_plan(int n_img, const int* n_kp, const int* W, const int* H, in
     M.off_kth = o; o = a256(o + 16 * (size_t)std::max(qo, 1LL));
     M.off_tkey = o; o = a256(o + 8 * (size_t)std::max(to, 1LL));
     M.off_tq = o; o = a256(o + 4 * (size_t)std::max(to, 1LL));
-    M.off_qkey = o; o = a256(o + 16 * (size_t)cap);
-    M.off_qkey2 = o; o = a256(o + 16 * (size_t)cap);
-    M.off_qsorted = o; o = a256(o + 4 * (size_t)std::max(qo, 1LL));
     M.off_cnt = o; o = a256(o + 16 * 4);
-    M.off_pcount = o; o = a256(o + 4 * (size_t)n_pair);
-    M.off_chunk = o; o = a256(o + 12 * (size_t)M.max_chunks);
     M.off_outq = o; o = a256(o + 4 * (size_t)std::max(oo, 1LL));
     M.off_outt = o; o = a256(o + 4 * (size_t)std::max(oo, 1LL));
     M.off_outn = o; o = a256(o + 4 * (size_t)n_pair);
@@ -864,17 +795,11 @@ static MatchBatch batch_ptrs(const MatchPlanHost& M, char* ws, const uint8_t* de
     b.kth = (double*)(ws + M.off_kth);
     b.tkey = (unsigned long long*)(ws + M.off_tkey);
     b.tq = (int*)(ws + M.off_tq);
-    b.qkey = (unsigned long long*)(ws + M.off_qkey);
-    b.qkey2 = (unsigned long long*)(ws + M.off_qkey2);
-    b.qsorted = (int*)(ws + M.off_qsorted);
     b.counters = (int*)(ws + M.off_cnt);
-    b.pcount = (int*)(ws + M.off_pcount);
-    b.chunk = (int*)(ws + M.off_chunk);
-    b.max_chunks = M.max_chunks;
+    b.n_chunks = M.n_chunks;
     b.out_q = (int*)(ws + M.off_outq);
     b.out_t = (int*)(ws + M.off_outt);
     b.out_n = (int*)(ws + M.off_outn);
-    b.sort_cap = M.sort_cap;
     return b;
 }
 
@@ -931,18 +856,8 @@ int fsx_match_batch(int n_img, const int* n_kp, const int* W, const int* H, cons
     if (nq > 0) {
         if (form == 0) {
             hipLaunchKernelGGL(k_m_plain, dim3(blocks_for(nq)), dim3(256), 0, st, b);
-        } else {
-            hipLaunchKernelGGL(k_m_qkeys, dim3(blocks_for(M.sort_cap)), dim3(256), 0, st, b);
-            hipLaunchKernelGGL(k_keysort_local, dim3((unsigned)(M.sort_cap / 1024)), dim3(256), 0, st, b.qkey,
-                               M.sort_cap);
-            int in_q = 1;
-            for (long long wdt = 1024; wdt < M.sort_cap; wdt <<= 1) {
-                hipLaunchKernelGGL(k_keysort_merge, dim3((unsigned)std::min(M.sort_cap / 256, 2048LL)), dim3(256), 0,
-                                   st, in_q ? b.qkey : b.qkey2, in_q ? b.qkey2 : b.qkey, M.sort_cap, wdt);
-                in_q = !in_q;
-            }
-            hipLaunchKernelGGL(k_m_chunks, dim3(1), dim3(256), 0, st, b, in_q);
-            hipLaunchKernelGGL(k_m_mfma, dim3((unsigned)std::min<long long>(M.max_chunks, 8192)), dim3(64), 0, st, b);
+        } else if (M.n_chunks > 0) {
+            hipLaunchKernelGGL(k_m_mfma, dim3((unsigned)std::min<long long>(M.n_chunks, 1 << 20)), dim3(64), 0, st, b);
         }
         FSX_LAUNCH_CHECK();
     }
@@ -1018,19 +933,17 @@ int fsx_match_query_state(int n_img, const int* n_kp, const int* W, const int* H
 }
 
 // Workspace layout (byte offsets) for tests and diagnostics: [0] norms [1] cell_start [2] cell_cur
-// [3] cell_idx [4] pred [5] qcell [6] best [7] b1 [8] kth [9] tkey [10] tq [11] qkey [12] qkey2
-// [13] qsorted [14] counters [15] pcount [16] chunk [17] out_q [18] out_t [19] out_n [20] sort_cap
-// [21] max_chunks [22] total
+// [3] cell_idx [4] pred [5] qcell [6] best [7] b1 [8] kth [9] tkey [10] tq [11] counters
+// [12] out_q [13] out_t [14] out_n [15] n_chunks [16] total
 int fsx_match_layout(int n_img, const int* n_kp, const int* W, const int* H, int n_pair, const int* pa,
                      const int* pb, double radius, long long* offs) {
     MatchPlanHost M;
     int rc = match_plan(n_img, n_kp, W, H, n_pair, pa, pb, nullptr, nullptr, radius, M);
     if (rc) return rc;
-    const size_t v[23] = {M.off_norms, M.off_cstart, M.off_ccur, M.off_cidx, M.off_pred, M.off_qcell, M.off_best,
-                          M.off_b1, M.off_kth, M.off_tkey, M.off_tq, M.off_qkey, M.off_qkey2, M.off_qsorted,
-                          M.off_cnt, M.off_pcount, M.off_chunk, M.off_outq, M.off_outt, M.off_outn,
-                          (size_t)M.sort_cap, (size_t)M.max_chunks, M.total};
-    for (int i = 0; i < 23; i++) offs[i] = (long long)v[i];
+    const size_t v[17] = {M.off_norms, M.off_cstart, M.off_ccur, M.off_cidx, M.off_pred, M.off_qcell, M.off_best,
+                          M.off_b1, M.off_kth, M.off_tkey, M.off_tq, M.off_cnt, M.off_outq, M.off_outt, M.off_outn,
+                          (size_t)M.n_chunks, M.total};
+    for (int i = 0; i < 17; i++) offs[i] = (long long)v[i];
     return FSX_OK;
 }
 

@@ -117,8 +117,6 @@ int sift_plan(int H, int W, SiftPlan* p) {
 // reflect-101.  The epilogue writes the new gauss layer, the DoG (new - previous, float32)
 // and, for layer 3, the next octave's base (every second pixel).
 
-constexpr int TW = 64;
-constexpr int TH = 32;
 
 __device__ __forceinline__ int reflect101(int p, int n) {
     while (p < 0 || p >= n) p = p < 0 ? -p : 2 * n - 2 - p;
@@ -140,16 +138,48 @@ __device__ __forceinline__ float up2_value(const uint8_t* __restrict__ img, int 
     return 0.75f * t_i + 0.25f * t_n;
 }
 
+// Tile: 128 input columns (TWo = 128 - 2*R2 output columns) x 32 output rows, 256 threads.
+// Vertical pass: a thread owns one input column and 16 consecutive output rows; it streams
+// the 16 + 2*R2 input values of its column once from global memory (coalesced across the
+// wave), converts each to float64 once and accumulates from registers.  The float32-rounded
+// result goes to an LDS mid-tile (row pitch 129 floats: conflict-free).  Horizontal pass: a
+// thread owns 8 consecutive outputs of one row, reading 8 + 2*R2 mid values once.
+constexpr int BL_SW = 128;
+constexpr int BL_TH = 32;
+constexpr int BL_RB = 16;
+constexpr int BL_CB = 8;
+
+template <int R2>
+constexpr int blur_tw() { return BL_SW - 2 * R2; }
+
+// Workgroups are dispatched to the 8 XCDs round-robin by linear id; remap so that XCD k gets a
+// contiguous range of row-major tiles (a band of tile rows), keeping vertically adjacent tiles,
+// whose halos overlap, on the same L2.  A bijection for any grid size.
+__device__ __forceinline__ void xcd_tile(int& tx, int& ty, int& tz) {
+    const unsigned nx = gridDim.x, ny = gridDim.y, nz = gridDim.z;
+    const unsigned N = nx * ny * nz;
+    const unsigned L = blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z);
+    const unsigned q = N / 8, r = N % 8, k = L % 8, j = L / 8;
+    const unsigned T = k * q + (k < r ? k : r) + j;
+    tx = (int)(T % nx);
+    ty = (int)((T / nx) % ny);
+    tz = (int)(T / (nx * ny));
+}
+
 template <int R2, bool BASE>
 __global__ __launch_bounds__(256) void k_blur(const SiftJob* __restrict__ jobs, int o, int layer, HalfTaps tp) {
-    constexpr int SW = TW + 2 * R2;
-    constexpr int SH = TH + 2 * R2;
-    __shared__ float s_in[SH * SW];
-    __shared__ float s_mid[TH * SW];
-    const SiftJob& J = jobs[blockIdx.z];
+    constexpr int TWo = BL_SW - 2 * R2;
+    constexpr int NW = BL_RB + 2 * R2;        // vertical window
+    constexpr int HW = BL_CB + 2 * R2;        // horizontal window
+    constexpr int NG = (TWo + BL_CB - 1) / BL_CB;
+    __shared__ float s_mid[BL_TH][BL_SW + 1];
+    __shared__ float s_out[BL_TH][BL_SW + 1];
+    int tx, ty, tz;
+    xcd_tile(tx, ty, tz);
+    const SiftJob& J = jobs[tz];
     if (o >= J.n_oct) return;
     const int R = J.oct[o].R, C = J.oct[o].C;
-    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+    const int x0 = tx * TWo, y0 = ty * BL_TH;
     if (x0 >= C || y0 >= R) return;
     const long long P = (long long)R * C;
     const float* in = nullptr;
@@ -159,24 +189,45 @@ __global__ __launch_bounds__(256) void k_blur(const SiftJob* __restrict__ jobs, 
         else in = J.G4;
     }
     const int tid = threadIdx.x;
-    for (int idx = tid; idx < SH * SW; idx += 256) {
-        int ly = idx / SW, lx = idx - (idx / SW) * SW;
-        int gy = reflect101(y0 + ly - R2, R), gx = reflect101(x0 + lx - R2, C);
-        float v;
-        if (BASE) v = up2_value(J.img, J.H, J.W, J.ld, gy, gx);
-        else v = in[(size_t)gy * C + gx];
-        s_in[idx] = v;
-    }
-    __syncthreads();
-    for (int idx = tid; idx < TH * SW; idx += 256) {
-        int ly = idx / SW, lx = idx - (idx / SW) * SW;
-        const float* col = s_in + (ly + R2) * SW + lx;
-        double s = (double)col[0] * tp.k[0];
+    {
+        const int c = tid & (BL_SW - 1), rg = tid >> 7;
+        const int gx = reflect101(x0 - R2 + c, C);
+        const int ybase = y0 + rg * BL_RB - R2;
+        double w[NW];
 #pragma unroll
-        for (int j = R2; j >= 1; --j) s += ((double)col[-j * SW] + (double)col[j * SW]) * tp.k[j];
-        s_mid[idx] = (float)s;
+        for (int k = 0; k < NW; k++) {
+            const int gy = reflect101(ybase + k, R);
+            float v;
+            if (BASE) v = up2_value(J.img, J.H, J.W, J.ld, gy, gx);
+            else v = in[(size_t)gy * C + gx];
+            w[k] = (double)v;
+        }
+#pragma unroll
+        for (int i = 0; i < BL_RB; i++) {
+            double acc = w[i + R2] * tp.k[0];
+#pragma unroll
+            for (int j = R2; j >= 1; --j) acc += (w[i + R2 - j] + w[i + R2 + j]) * tp.k[j];
+            s_mid[rg * BL_RB + i][c] = (float)acc;
+        }
     }
     __syncthreads();
+    for (int task = tid; task < BL_TH * NG; task += 256) {
+        const int ly = task % BL_TH, cg = task / BL_TH;
+        const int cbase = cg * BL_CB;
+        double v[HW];
+#pragma unroll
+        for (int k = 0; k < HW; k++) v[k] = (cbase + k < BL_SW) ? (double)s_mid[ly][cbase + k] : 0.0;
+#pragma unroll
+        for (int j = 0; j < BL_CB; j++) {
+            if (cbase + j >= TWo) continue;
+            double acc = v[j + R2] * tp.k[0];
+#pragma unroll
+            for (int t = R2; t >= 1; --t) acc += (v[j + R2 - t] + v[j + R2 + t]) * tp.k[t];
+            s_out[ly][cbase + j] = (float)acc;
+        }
+    }
+    __syncthreads();
+    // epilogue with lanes along x: coalesced gauss / DoG / downsample stores
     float* outG = nullptr;
     float* outD = nullptr;
     float* down = nullptr;
@@ -193,23 +244,17 @@ __global__ __launch_bounds__(256) void k_blur(const SiftJob* __restrict__ jobs, 
             C2n = J.oct[o + 1].C;
         }
     }
-    const int lx = tid & (TW - 1);
-    const int gx = x0 + lx;
-    for (int ly = tid / TW; ly < TH; ly += 256 / TW) {
-        const int gy = y0 + ly;
-        const float* row = s_mid + ly * SW + lx + R2;
-        double s = (double)row[0] * tp.k[0];
-#pragma unroll
-        for (int j = R2; j >= 1; --j) s += ((double)row[-j] + (double)row[j]) * tp.k[j];
-        float g = (float)s;
-        if (gy < R && gx < C) {
-            size_t p = (size_t)gy * C + gx;
-            if (outG) outG[p] = g;
-            if (outD) outD[p] = g - s_in[(ly + R2) * SW + lx + R2];
-            if (down && !(gy & 1) && !(gx & 1)) {
-                int ry = gy >> 1, rx = gx >> 1;
-                if (ry < R2n && rx < C2n) down[(size_t)ry * C2n + rx] = g;
-            }
+    for (int idx = tid; idx < BL_TH * TWo; idx += 256) {
+        const int ly = idx / TWo, lx = idx - (idx / TWo) * TWo;
+        const int gy = y0 + ly, gx = x0 + lx;
+        if (gy >= R || gx >= C) continue;
+        const float g = s_out[ly][lx];
+        const size_t p = (size_t)gy * C + gx;
+        if (outG) outG[p] = g;
+        if (outD) outD[p] = g - in[p];
+        if (down && !(gy & 1) && !(gx & 1)) {
+            const int ry = gy >> 1, rx = gx >> 1;
+            if (ry < R2n && rx < C2n) down[(size_t)ry * C2n + rx] = g;
         }
     }
 }
@@ -220,7 +265,8 @@ __global__ __launch_bounds__(256) void k_blur(const SiftJob* __restrict__ jobs, 
 // any order as gauss-atlas indices gbase[o] + (l-1)*R*C + r*C + c (ascending index = the
 // reference's candidate order; the final sort key carries it).
 
-constexpr int EX_TW = 64, EX_TH = 16;
+constexpr int EX_W = 62;     // output columns per wave (lanes 1..62; lanes 0 and 63 are halo)
+constexpr int EX_H = 32;     // output rows per block
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
@@ -235,52 +281,113 @@ __device__ __forceinline__ int wave_append(bool f, int* counter) {
     return base + __popcll(m & ((1ull << lane) - 1ull));
 }
 
+// Streaming-column extrema: each wave owns 62 columns of one octave and walks down EX_H
+// rows; per row it loads the 5 DoG layers (coalesced), takes the horizontal max/min over
+// (x-1, x, x+1) with lane shuffles and keeps a 3-row ring, so the 3x3x3 max/min of a voxel
+// costs a few max/min operations.  v >= max27 is the reference's non-strict test (no NaN).
 __global__ __launch_bounds__(256) void k_extrema(const SiftJob* __restrict__ jobs, int o) {
-    __shared__ float s[5][EX_TH + 2][EX_TW + 2];
-    const SiftJob& J = jobs[blockIdx.z];
+    int tx, ty, tz;
+    xcd_tile(tx, ty, tz);
+    const SiftJob& J = jobs[tz];
     if (o >= J.n_oct) return;
     const int R = J.oct[o].R, C = J.oct[o].C;
-    const int x0 = blockIdx.x * EX_TW, y0 = blockIdx.y * EX_TH;
-    if (x0 >= C || y0 >= R) return;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int xs = (tx * 4 + wv) * EX_W;                   // first output column of this wave
+    const int y0 = ty * EX_H;
+    if (xs >= C || y0 >= R) return;                        // wave-uniform (no block barrier below)
+    const int x = xs - 1 + lane;
+    const int xc = min(max(x, 0), C - 1);
     const long long P = (long long)R * C;
-    const float* D = J.DF + J.oct[o].dbase;
-    const int tid = threadIdx.x;
-    for (int idx = tid; idx < 5 * (EX_TH + 2) * (EX_TW + 2); idx += 256) {
-        int l = idx / ((EX_TH + 2) * (EX_TW + 2));
-        int rem = idx - l * (EX_TH + 2) * (EX_TW + 2);
-        int ly = rem / (EX_TW + 2), lx = rem - ly * (EX_TW + 2);
-        int gy = min(max(y0 + ly - 1, 0), R - 1), gx = min(max(x0 + lx - 1, 0), C - 1);
-        s[l][ly][lx] = D[l * P + (size_t)gy * C + gx];
-    }
-    __syncthreads();
+    const float* D = J.DF + J.oct[o].dbase + xc;
     const float pthr = J.pthr;
-    const int lx = tid & (EX_TW - 1);
-    const int c = x0 + lx;
-    for (int l = 1; l <= N_LAYERS; l++) {
-        for (int ly = tid / EX_TW; ly < EX_TH; ly += 256 / EX_TW) {
-            const int r = y0 + ly;
-            bool f = false;
-            if (r >= BORDER && r < R - BORDER && c >= BORDER && c < C - BORDER) {
-                const float v = s[l][ly + 1][lx + 1];
-                bool mx = v > pthr, mn = v < -pthr;
-                if (mx || mn) {
+    const bool col_ok = lane >= 1 && lane <= EX_W && x >= BORDER && x < C - BORDER;
+    float hmx[3][5], hmn[3][5], mid[3][3];
 #pragma unroll
-                    for (int dl = -1; dl <= 1; dl++)
+    for (int a = 0; a < 3; a++)
 #pragma unroll
-                        for (int dr = -1; dr <= 1; dr++)
+        for (int l = 0; l < 5; l++) {
+            hmx[a][l] = 0.f;
+            hmn[a][l] = 0.f;
+        }
 #pragma unroll
-                            for (int dc = -1; dc <= 1; dc++) {
-                                float u = s[l + dl][ly + 1 + dr][lx + 1 + dc];
-                                mx = mx && !(u > v);
-                                mn = mn && !(u < v);
-                            }
-                    f = mx || mn;
-                }
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int l = 0; l < 3; l++) mid[a][l] = 0.f;
+    const int yend = min(y0 + EX_H, R);
+    // rows are fetched 4 at a time, one block of 4 ahead of the one being processed
+    float nxt[4][5];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int yc = min(max(y0 - 1 + r, 0), R - 1);
+#pragma unroll
+        for (int l = 0; l < 5; l++) nxt[r][l] = D[l * P + (long long)yc * C];
+    }
+    for (int yb = y0 - 1; yb <= yend; yb += 4) {
+        float cur[4][5];
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int l = 0; l < 5; l++) cur[r][l] = nxt[r][l];
+        if (yb + 4 <= yend) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int yc = min(max(yb + 4 + r, 0), R - 1);
+#pragma unroll
+                for (int l = 0; l < 5; l++) nxt[r][l] = D[l * P + (long long)yc * C];
             }
-            int pos = wave_append(f, &J.counters[0]);
-            if (f) {
-                if (pos < J.cand_cap)
-                    J.cand[pos] = (uint32_t)(J.oct[o].gbase + (long long)(l - 1) * P + (long long)r * C + c);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int yy = yb + r;
+            if (yy > yend) break;                           // wave-uniform
+#pragma unroll
+            for (int l = 0; l < 5; l++) {
+                const float v = cur[r][l];
+                const float lf = __shfl_up(v, 1), rt = __shfl_down(v, 1);
+                hmx[0][l] = hmx[1][l];
+                hmx[1][l] = hmx[2][l];
+                hmx[2][l] = fmaxf(fmaxf(lf, v), rt);
+                hmn[0][l] = hmn[1][l];
+                hmn[1][l] = hmn[2][l];
+                hmn[2][l] = fminf(fminf(lf, v), rt);
+            }
+#pragma unroll
+            for (int l = 0; l < 3; l++) {
+                mid[0][l] = mid[1][l];
+                mid[1][l] = mid[2][l];
+                mid[2][l] = cur[r][l + 1];
+            }
+            const int y = yy - 1;                           // the ring's middle row
+            if (yy < y0 + 1) continue;                      // wave-uniform
+            const bool row_ok = y >= BORDER && y < R - BORDER;
+            float vm[5], vn[5];
+#pragma unroll
+            for (int l = 0; l < 5; l++) {
+                vm[l] = fmaxf(fmaxf(hmx[0][l], hmx[1][l]), hmx[2][l]);
+                vn[l] = fminf(fminf(hmn[0][l], hmn[1][l]), hmn[2][l]);
+            }
+            bool f[3];
+#pragma unroll
+            for (int l = 1; l <= N_LAYERS; l++) {
+                const float c = mid[1][l - 1];
+                const float mx = fmaxf(fmaxf(vm[l - 1], vm[l]), vm[l + 1]);
+                const float mn = fminf(fminf(vn[l - 1], vn[l]), vn[l + 1]);
+                f[l - 1] = col_ok && row_ok && ((c >= mx && c > pthr) || (c <= mn && c < -pthr));
+            }
+            const unsigned long long b0 = __ballot(f[0]), b1 = __ballot(f[1]), b2 = __ballot(f[2]);
+            const int n0 = __popcll(b0), n1 = __popcll(b1), n2 = __popcll(b2);
+            if (n0 + n1 + n2 == 0) continue;                // wave-uniform
+            int base = 0;
+            if (lane == 0) base = atomicAdd(&J.counters[0], n0 + n1 + n2);
+            base = __shfl(base, 0);
+            const unsigned long long below = (1ull << lane) - 1ull;
+            const int pos[3] = {base + (int)__popcll(b0 & below), base + n0 + (int)__popcll(b1 & below),
+                                base + n0 + n1 + (int)__popcll(b2 & below)};
+#pragma unroll
+            for (int l = 1; l <= N_LAYERS; l++) {
+                if (!f[l - 1]) continue;
+                if (pos[l - 1] < J.cand_cap)
+                    J.cand[pos[l - 1]] = (uint32_t)(J.oct[o].gbase + (long long)(l - 1) * P + (long long)y * C + x);
                 else
                     atomicOr(&J.counters[3], 1);
             }
@@ -396,6 +503,20 @@ __device__ __forceinline__ void grad_at(const float* __restrict__ G, int C, int 
     mag = sqrtf(xx + yy);
     const float deg = 180.0f / 3.14159265358979323846f;
     float a = (float)atan2((double)dy, (double)dx) * deg;
+    ang = a < 0.f ? a + 360.0f : a;
+}
+
+// The descriptor's per-sample gradient: float32 atan2 (ocml, a few ulp).  The descriptor bar is
+// +-1 unit per bin on the 512 scale, far above these ulps; the orientation histogram keeps the
+// float64-rounded angle of grad_at because its peak decisions must follow the reference.
+__device__ __forceinline__ void grad_at_f(const float* __restrict__ G, int C, int y, int x, float& mag, float& ang) {
+    const size_t k = (size_t)y * C + x;
+    const float dx = G[k + 1] - G[k - 1];
+    const float dy = G[k - C] - G[k + C];
+    const float xx = dx * dx, yy = dy * dy;
+    mag = sqrtf(xx + yy);
+    const float deg = 180.0f / 3.14159265358979323846f;
+    float a = atan2f(dy, dx) * deg;
     ang = a < 0.f ? a + 360.0f : a;
 }
 
@@ -669,8 +790,8 @@ __device__ void describe_kp(const SiftJob& J, const float* __restrict__ kpa, int
         if (!(rbin > -1.0 && rbin < 4.0 && cbin > -1.0 && cbin < 4.0 && yy > 0 && yy < R - 1 && xx > 0 && xx < C - 1))
             continue;
         float mag, ang;
-        grad_at(G, C, (int)yy, (int)xx, mag, ang);
-        const double w = exp((c_rot * c_rot + r_rot * r_rot) * (-0.125));
+        grad_at_f(G, C, (int)yy, (int)xx, mag, ang);
+        const double w = (double)expf((float)((c_rot * c_rot + r_rot * r_rot) * (-0.125)));
         const double mg = (double)mag * w;
         const float obin = (ang - ori) * obs;
         const double r0 = floor(rbin), c0 = floor(cbin);
@@ -834,31 +955,30 @@ __global__ __launch_bounds__(256) void k_spread(const SiftJob* __restrict__ jobs
         cy = cy < 0 ? 0 : (cy > 8 ? 8 : cy);
         cell[i] = cy * 12 + cx;
     }
-    if (tid < 108) s_cnt[tid] = 0;
+    // rank of each keypoint within its cell in detection order: 64 threads each own a
+    // contiguous segment, count per cell, exclusive-scan the counts over the segments, then
+    // rank their own segment
+    __shared__ int s_seg[64][108];
+    const int seg = (N + 63) / 64;
+    if (tid < 64) {
+        for (int c = 0; c < 108; c++) s_seg[tid][c] = 0;
+        const int a = tid * seg, b = min(N, a + seg);
+        for (int i = a; i < b; i++) s_seg[tid][cell[i]]++;
+    }
+    __syncthreads();
+    if (tid < 108) {
+        int run = 0;
+        for (int t = 0; t < 64; t++) {
+            const int v = s_seg[t][tid];
+            s_seg[t][tid] = run;
+            run += v;
+        }
+        s_cnt[tid] = run;
+    }
     __syncthreads();
     if (tid < 64) {
-        const int lane = tid;
-        for (int base = 0; base < N; base += 64) {
-            const int i = base + lane;
-            const int c = i < N ? cell[i] : -1;
-            int before = 0;
-            bool last = true;
-            for (int j = 0; j < 64; j++) {
-                const int cj = __shfl(c, j);
-                if (cj == c && j < lane) before++;
-                if (cj == c && j > lane) last = false;
-            }
-            int r = 0;
-            if (i < N) r = s_cnt[c] + before;
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_s_waitcnt(0);
-            if (i < N) {
-                rank[i] = r;
-                if (last) s_cnt[c] = r + 1;
-            }
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_s_waitcnt(0);
-        }
+        const int a = tid * seg, b = min(N, a + seg);
+        for (int i = a; i < b; i++) rank[i] = s_seg[tid][cell[i]]++;
     }
     __syncthreads();
     const int cap = (n + 107) / 108;
@@ -987,7 +1107,8 @@ static int fill_job(SiftJob* J, const SiftPlan& p, const uint8_t* img, int ld, d
 template <int R2>
 static void launch_blur(const SiftJob* jobs, int n_img, int o, int layer, int R, int C, const HalfTaps& h,
                         hipStream_t st) {
-    dim3 grid((C + TW - 1) / TW, (R + TH - 1) / TH, n_img);
+    constexpr int TWo = blur_tw<R2>();
+    dim3 grid((C + TWo - 1) / TWo, (R + BL_TH - 1) / BL_TH, n_img);
     hipLaunchKernelGGL((k_blur<R2, false>), grid, dim3(256), 0, st, jobs, o, layer, h);
 }
 
@@ -1030,7 +1151,8 @@ static int run_pipeline(const SiftJob* all, int n_all, const Group* groups, int 
     for (int gi = 0; gi < n_groups; gi++) {
         const Group& g = groups[gi];
         const SiftPlan& p = g.plan;
-        dim3 grid((p.C[0] + TW - 1) / TW, (p.R[0] + TH - 1) / TH, g.n);
+        constexpr int TWo = blur_tw<5>();
+        dim3 grid((p.C[0] + TWo - 1) / TWo, (p.R[0] + BL_TH - 1) / BL_TH, g.n);
         hipLaunchKernelGGL((k_blur<5, true>), grid, dim3(256), 0, st, g.jobs, 0, 0, T.h[0]);
         FSX_LAUNCH_CHECK();
         for (int o = 0; o < p.n_oct; o++)
@@ -1044,7 +1166,8 @@ static int run_pipeline(const SiftJob* all, int n_all, const Group* groups, int 
         const Group& g = groups[gi];
         const SiftPlan& p = g.plan;
         for (int o = 0; o < p.n_oct; o++) {
-            dim3 grid((p.C[o] + EX_TW - 1) / EX_TW, (p.R[o] + EX_TH - 1) / EX_TH, g.n);
+            const int strips = (p.C[o] + EX_W - 1) / EX_W;
+            dim3 grid((strips + 3) / 4, (p.R[o] + EX_H - 1) / EX_H, g.n);
             hipLaunchKernelGGL(k_extrema, grid, dim3(256), 0, st, g.jobs, o);
             FSX_LAUNCH_CHECK();
         }
