@@ -34,9 +34,9 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_flight(seed=2):
+def make_flight(seed=2, workers=8):
     from fieldstitch_amd import synth
-    return synth.flight(seed=seed, workers=min(8, os.cpu_count() or 1))
+    return synth.flight(seed=seed, workers=max(1, min(workers, os.cpu_count() or 1)))
 
 
 def pyramid_bytes(H, W):
@@ -141,6 +141,25 @@ def bench_matching(fb, fl, shapes, args):
     return out
 
 
+def copy_probe(mib):
+    """Three device-to-device copies of `mib` MiB: a kernel with known HBM traffic (read mib,
+    write mib per dispatch) whose FETCH_SIZE / WRITE_SIZE readings scale the counters of the
+    SIFT kernels to bytes."""
+    import torch
+    a = torch.ones(mib * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": a.numel() * 4, "dispatches": 3}), flush=True)
+
+
+def pyramid_launches(shapes):
+    """Blur launches of one step: per group of equal shapes, 1 base + 5 per octave."""
+    from fieldstitch_amd import gsift
+    return sum(1 + 5 * len(gsift.plan(H, W)) for H, W in set(shapes))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -149,7 +168,14 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-procs", type=int, default=8)
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_blur_c2.json"))
+    ap.add_argument("--gen-workers", type=int, default=8,
+                    help="processes rendering the synthetic photos (1 under a profiler)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 3 plain device copies of MIB MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding)",
               file=sys.stderr)
@@ -158,7 +184,7 @@ def main():
     from fieldstitch_amd import features
 
     t0 = time.time()
-    fl = make_flight()
+    fl = make_flight(workers=args.gen_workers)
     log("flight: %d photos %s, %d pairs, generated in %.1f s" % (len(fl.images), fl.images[0].shape,
                                                                   len(fl.pairs), time.time() - t0))
     dev = [torch.from_numpy(im).cuda() for im in fl.images]
@@ -167,7 +193,8 @@ def main():
     torch.cuda.synchronize()
     for _ in range(args.warmup):
         fb.run(dev)
-    fb.check_capacity()
+    if args.warmup:
+        fb.check_capacity()
     torch.cuda.synchronize()
 
     # timed region: K steps of the SIFT front end on all photos
@@ -188,11 +215,12 @@ def main():
     pyr_b = sum(pyramid_bytes(*s) for s in shapes)
     sift_b = sum(sift_bytes(*s) for s in shapes)
     pyr_s = stages["pyramid"] / 1e3
+    n_launch = pyramid_launches(shapes)
     achieved = pyr_b / pyr_s
     traffic = None
     if os.path.exists(args.traffic):
         with open(args.traffic) as f:
-            traffic = json.load(f).get("bytes_per_step")
+            traffic = json.load(f).get("bytes_per_launch")
     res = {
         "metric": "images/s (SIFT) + matched pairs/s vs plain FP32 GPU form, match set = FP64 oracle",
         "value": n_img / t_sift,
@@ -216,6 +244,8 @@ def main():
         "roofline": {"kernel": "k_blur (pyramid: base + 5 layers x octaves, all photos)", "bound": "hbm",
                      "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "launches_per_step": n_launch, "algorithmic_bytes_per_launch": pyr_b / n_launch,
+                     "avg_launch_us": stages["pyramid"] * 1e3 / n_launch,
                      "algorithmic_bytes_per_step": pyr_b, "time_ms_per_step": stages["pyramid"]},
     }
     res["matching"] = bench_matching(fb, fl, shapes, args)

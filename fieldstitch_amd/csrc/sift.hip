@@ -118,6 +118,13 @@ int sift_plan(int H, int W, SiftPlan* p) {
 // and, for layer 3, the next octave's base (every second pixel).
 
 
+// global-address-space view of a generic pointer: lets the compiler emit global_load (vector
+// memory counter only) instead of flat_load for data read through pointers held in structs
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* gptr(const T* p) {
+    return (const __attribute__((address_space(1))) T*)p;
+}
+
 __device__ __forceinline__ int reflect101(int p, int n) {
     while (p < 0 || p >= n) p = p < 0 ? -p : 2 * n - 2 - p;
     return p;
@@ -166,6 +173,7 @@ __device__ __forceinline__ void xcd_tile(int& tx, int& ty, int& tz) {
     tz = (int)(T / (nx * ny));
 }
 
+
 template <int R2, bool BASE>
 __global__ __launch_bounds__(256) void k_blur(const SiftJob* __restrict__ jobs, int o, int layer, HalfTaps tp) {
     constexpr int TWo = BL_SW - 2 * R2;
@@ -190,18 +198,31 @@ __global__ __launch_bounds__(256) void k_blur(const SiftJob* __restrict__ jobs, 
     }
     const int tid = threadIdx.x;
     {
-        const int c = tid & (BL_SW - 1), rg = tid >> 7;
+        const int c = tid & (BL_SW - 1);
+        const int rg = __builtin_amdgcn_readfirstlane(tid >> 7);
         const int gx = reflect101(x0 - R2 + c, C);
         const int ybase = y0 + rg * BL_RB - R2;
+        float f[NW];
+        if (ybase >= 0 && ybase + NW <= R) {
+            // interior rows (wave-uniform test): direct addressing, no reflection
+            if (BASE) {
+#pragma unroll
+                for (int k = 0; k < NW; k++) f[k] = up2_value(J.img, J.H, J.W, J.ld, ybase + k, gx);
+            } else {
+                const __attribute__((address_space(1))) float* col = gptr(in) + (size_t)ybase * C + gx;
+#pragma unroll
+                for (int k = 0; k < NW; k++) f[k] = col[(size_t)k * C];
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < NW; k++) {
+                const int gy = reflect101(ybase + k, R);
+                f[k] = BASE ? up2_value(J.img, J.H, J.W, J.ld, gy, gx) : gptr(in)[(size_t)gy * C + gx];
+            }
+        }
         double w[NW];
 #pragma unroll
-        for (int k = 0; k < NW; k++) {
-            const int gy = reflect101(ybase + k, R);
-            float v;
-            if (BASE) v = up2_value(J.img, J.H, J.W, J.ld, gy, gx);
-            else v = in[(size_t)gy * C + gx];
-            w[k] = (double)v;
-        }
+        for (int k = 0; k < NW; k++) w[k] = (double)f[k];
 #pragma unroll
         for (int i = 0; i < BL_RB; i++) {
             double acc = w[i + R2] * tp.k[0];
@@ -214,9 +235,12 @@ __global__ __launch_bounds__(256) void k_blur(const SiftJob* __restrict__ jobs, 
     for (int task = tid; task < BL_TH * NG; task += 256) {
         const int ly = task % BL_TH, cg = task / BL_TH;
         const int cbase = cg * BL_CB;
+        float fv[HW];
+#pragma unroll
+        for (int k = 0; k < HW; k++) fv[k] = (cbase + k < BL_SW) ? s_mid[ly][cbase + k] : 0.0f;
         double v[HW];
 #pragma unroll
-        for (int k = 0; k < HW; k++) v[k] = (cbase + k < BL_SW) ? (double)s_mid[ly][cbase + k] : 0.0;
+        for (int k = 0; k < HW; k++) v[k] = (double)fv[k];
 #pragma unroll
         for (int j = 0; j < BL_CB; j++) {
             if (cbase + j >= TWo) continue;
@@ -244,17 +268,21 @@ __global__ __launch_bounds__(256) void k_blur(const SiftJob* __restrict__ jobs, 
             C2n = J.oct[o + 1].C;
         }
     }
-    for (int idx = tid; idx < BL_TH * TWo; idx += 256) {
-        const int ly = idx / TWo, lx = idx - (idx / TWo) * TWo;
-        const int gy = y0 + ly, gx = x0 + lx;
-        if (gy >= R || gx >= C) continue;
+    const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    for (int ly = wv; ly < BL_TH; ly += 4) {
+        const int gy = y0 + ly;
+        if (gy >= R) break;
+        for (int lx = lane; lx < TWo; lx += 64) {
+        const int gx = x0 + lx;
+        if (gx >= C) break;
         const float g = s_out[ly][lx];
         const size_t p = (size_t)gy * C + gx;
         if (outG) outG[p] = g;
-        if (outD) outD[p] = g - in[p];
+        if (outD) outD[p] = g - gptr(in)[p];
         if (down && !(gy & 1) && !(gx & 1)) {
             const int ry = gy >> 1, rx = gx >> 1;
             if (ry < R2n && rx < C2n) down[(size_t)ry * C2n + rx] = g;
+        }
         }
     }
 }
@@ -298,7 +326,7 @@ __global__ __launch_bounds__(256) void k_extrema(const SiftJob* __restrict__ job
     const int x = xs - 1 + lane;
     const int xc = min(max(x, 0), C - 1);
     const long long P = (long long)R * C;
-    const float* D = J.DF + J.oct[o].dbase + xc;
+    const __attribute__((address_space(1))) float* D = gptr(J.DF + J.oct[o].dbase + xc);
     const float pthr = J.pthr;
     const bool col_ok = lane >= 1 && lane <= EX_W && x >= BORDER && x < C - BORDER;
     float hmx[3][5], hmn[3][5], mid[3][3];
@@ -343,7 +371,10 @@ __global__ __launch_bounds__(256) void k_extrema(const SiftJob* __restrict__ job
 #pragma unroll
             for (int l = 0; l < 5; l++) {
                 const float v = cur[r][l];
-                const float lf = __shfl_up(v, 1), rt = __shfl_down(v, 1);
+                // neighbours x-1 / x+1 by whole-wave DPP shifts (max/min of the three is
+                // symmetric, the halo lanes 0 and 63 never produce an output)
+                const float lf = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x138, 0xf, 0xf, false));
+                const float rt = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x130, 0xf, 0xf, false));
                 hmx[0][l] = hmx[1][l];
                 hmx[1][l] = hmx[2][l];
                 hmx[2][l] = fmaxf(fmaxf(lf, v), rt);
@@ -506,6 +537,15 @@ __device__ __forceinline__ void grad_at(const float* __restrict__ G, int C, int 
     ang = a < 0.f ? a + 360.0f : a;
 }
 
+// Fixed-point (2^50 units) value of a non-negative contribution v < 2^2, rounded to nearest
+// even: x = v * 2^50 < 2^52, so x + 2^52 lands in [2^52, 2^53) where the double's ulp is 1 and
+// its mantissa bits are the rounded integer.  Same result as __double2ull_rn(v * 2^50) in two
+// cheap operations (no float64 -> uint64 conversion sequence).
+__device__ __forceinline__ unsigned long long fx50(double v) {
+    const double y = v * 0x1p50 + 0x1p52;
+    return (unsigned long long)__double_as_longlong(y) - 0x4330000000000000ull;
+}
+
 // The descriptor's per-sample gradient: float32 atan2 (ocml, a few ulp).  The descriptor bar is
 // +-1 unit per bin on the 512 scale, far above these ulps; the orientation histogram keeps the
 // float64-rounded angle of grad_at because its peak decisions must follow the reference.
@@ -528,34 +568,51 @@ __device__ __forceinline__ void grad_at_f(const float* __restrict__ G, int C, in
 __global__ __launch_bounds__(64) void k_orient(const SiftJob* __restrict__ jobs) {
     __shared__ unsigned long long s_h[ORI_BINS];
     __shared__ float s_f[ORI_BINS];
+    __shared__ float s_w[2 * ORI_R_MAX * ORI_R_MAX + 1];   // gaussian weight by i*i + j*j
     const SiftJob& J = jobs[blockIdx.z];
     const int n = min(J.counters[1], J.cand_cap);
     const int lane = threadIdx.x;
-    constexpr int WIN = 2 * ORI_R_MAX + 1;
     for (int k = blockIdx.x; k < n; k += gridDim.x) {
         const float4 a = J.refined[2 * k], b = J.refined[2 * k + 1];
         const int o = __float_as_int(a.x), l = __float_as_int(a.y), r = __float_as_int(a.z), c = __float_as_int(a.w);
         const int R = J.oct[o].R, C = J.oct[o].C;
         const float e = ((float)l + b.z) / 3.0f;
         const float scl = 1.6f * (float)pow(2.0, (double)e);
-        const int rad = (int)rintf(4.5f * scl);
+        const int rad = min((int)rintf(4.5f * scl), ORI_R_MAX);
         const float sw = 1.5f * scl;
         const float den = 2.0f * (sw * sw);
         const float* G = J.G123 + J.oct[o].gbase + (long long)(l - 1) * R * C;
         if (lane < ORI_BINS) s_h[lane] = 0ull;
+        // w = exp(-(i*i + j*j) / den) depends on i*i + j*j only: one float64 exp per value
+        const int dmax = 2 * rad * rad;
+        for (int d = lane; d <= dmax; d += 64) s_w[d] = (float)exp((double)(-(float)d / den));
         __syncthreads();
-        for (int s = lane; s < WIN * WIN; s += 64) {
-            const int i = s / WIN - ORI_R_MAX, j = s - (s / WIN) * WIN - ORI_R_MAX;
+        // samples of the keypoint's own (2 rad + 1)^2 window (gsift.py:396-399 masks the rest)
+        const int W = 2 * rad + 1;
+        for (int s = lane; s < W * W; s += 64) {
+            const int i = s / W - rad, j = s - (s / W) * W - rad;
             const int y = r + i, x = c + j;
-            if (abs(i) <= rad && abs(j) <= rad && y > 0 && y < R - 1 && x > 0 && x < C - 1) {
-                float mag, ang;
-                grad_at(G, C, y, x, mag, ang);
-                const float d2 = (float)(i * i + j * j);
-                const float w = (float)exp((double)(-d2 / den));
-                const float m = mag * w;
-                int bin = (int)rintf(ang * 0.1f);
+            if (y > 0 && y < R - 1 && x > 0 && x < C - 1) {
+                // bin = rint(ANG * 0.1) with ANG from the float64-rounded atan2 (the reference's
+                // value); a float32 atan2 (a few ulp, < 1e-5 in ANG * 0.1) decides it unless
+                // ANG * 0.1 lies within 1e-3 of a rounding boundary, where the exact angle is used
+                const size_t kk = (size_t)y * C + x;
+                const float dx = G[kk + 1] - G[kk - 1];
+                const float dy = G[kk - C] - G[kk + C];
+                const float xx = dx * dx, yy = dy * dy;
+                const float mag = sqrtf(xx + yy);
+                const float deg = 180.0f / 3.14159265358979323846f;
+                float af = atan2f(dy, dx) * deg;
+                af = af < 0.f ? af + 360.0f : af;
+                const float t = af * 0.1f;
+                if (fabsf((t - floorf(t)) - 0.5f) < 1e-3f) {
+                    float ae = (float)atan2((double)dy, (double)dx) * deg;
+                    af = ae < 0.f ? ae + 360.0f : ae;
+                }
+                const float m = mag * s_w[i * i + j * j];
+                int bin = (int)rintf(af * 0.1f);
                 bin = bin >= ORI_BINS ? bin - ORI_BINS : bin;
-                atomicAdd(&s_h[bin], __double2ull_rn((double)m * 0x1p50));
+                atomicAdd(&s_h[bin], fx50((double)m));
             }
         }
         __syncthreads();
@@ -777,11 +834,56 @@ __device__ void describe_kp(const SiftJob& J, const float* __restrict__ kpa, int
     const long long pr = (long long)rint(py), pc = (long long)rint(px);
     const float* G = J.G123 + J.oct[o].gbase + (long long)(li - 1) * R * C;
     for (int b = threadIdx.x; b < HIST_SIZE; b += 256) s_h[b] = 0ull;
+    // per row i of the window, the column span [jl, jh] outside of which |c_rot| or |r_rot|
+    // >= 2.5 for sure (widened by one sample; the exact per-sample test below still decides):
+    // only the rotated 5x5-cell square is visited instead of the whole (2 rad + 1)^2 window
+    __shared__ int s_jl[2 * 38 + 1 + 2], s_pre[2 * 38 + 2 + 2];
+    const int nrow = (int)(2 * rad + 1);
+    const bool spans = nrow <= 2 * 38 + 1;
+    if (spans) {
+        for (int row = threadIdx.x; row < nrow; row += 256) {
+            const double ii = (double)(row - rad);
+            double lo = -(double)rad, hi = (double)rad;
+            // c_rot = j cos_t - ii sin_t in (-2.5, 2.5); r_rot = j sin_t + ii cos_t in (-2.5, 2.5)
+            const double cc[2] = {cos_t, sin_t}, off[2] = {-ii * sin_t, ii * cos_t};
+            for (int q = 0; q < 2; q++) {
+                if (cc[q] != 0.0) {
+                    const double a = (-2.5 - off[q]) / cc[q], b2 = (2.5 - off[q]) / cc[q];
+                    lo = fmax(lo, floor(fmin(a, b2)) - 1.0);
+                    hi = fmin(hi, ceil(fmax(a, b2)) + 1.0);
+                } else if (!(fabs(off[q]) < 2.5)) {
+                    hi = lo - 1.0;
+                }
+            }
+            const int jl = (int)lo, n = hi >= lo ? (int)(hi - lo) + 1 : 0;
+            s_jl[row] = jl;
+            s_pre[row + 1] = n;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            s_pre[0] = 0;
+            for (int row = 0; row < nrow; row++) s_pre[row + 1] += s_pre[row];
+        }
+    }
     __syncthreads();
     const int W = (int)(2 * rad + 1);
     const float obs = (float)(8.0 / 360.0);
-    for (int s = threadIdx.x; s < W * W; s += 256) {
-        const int i = s / W - (int)rad, j = s - (s / W) * W - (int)rad;
+    const int total = spans ? s_pre[nrow] : W * W;
+    for (int s = threadIdx.x; s < total; s += 256) {
+        int i, j;
+        if (spans) {
+            int lo = 0, hi = nrow - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_pre[mid] <= s) lo = mid;
+                else hi = mid - 1;
+            }
+            i = lo - (int)rad;
+            j = s_jl[lo] + (s - s_pre[lo]);
+        } else {
+            i = s / W - (int)rad;
+            j = s - (s / W) * W - (int)rad;
+        }
         const double oi = (double)(float)i, oj = (double)(float)j;
         const double c_rot = oj * cos_t - oi * sin_t;
         const double r_rot = oj * sin_t + oi * cos_t;
@@ -809,14 +911,14 @@ __device__ void describe_kp(const SiftJob& J, const float* __restrict__ kpa, int
         const double v011 = v_rc01 * ob, v010 = v_rc01 - v011;
         const double v001 = v_rc00 * ob, v000 = v_rc00 - v001;
         const int idx = (((int)r0 + 1) * 6 + (int)c0 + 1) * 10 + o0;
-        atomicAdd(&s_h[idx], __double2ull_rn(v000 * 0x1p50));
-        atomicAdd(&s_h[idx + 1], __double2ull_rn(v001 * 0x1p50));
-        atomicAdd(&s_h[idx + 10], __double2ull_rn(v010 * 0x1p50));
-        atomicAdd(&s_h[idx + 11], __double2ull_rn(v011 * 0x1p50));
-        atomicAdd(&s_h[idx + 60], __double2ull_rn(v100 * 0x1p50));
-        atomicAdd(&s_h[idx + 61], __double2ull_rn(v101 * 0x1p50));
-        atomicAdd(&s_h[idx + 70], __double2ull_rn(v110 * 0x1p50));
-        atomicAdd(&s_h[idx + 71], __double2ull_rn(v111 * 0x1p50));
+        atomicAdd(&s_h[idx], fx50(v000));
+        atomicAdd(&s_h[idx + 1], fx50(v001));
+        atomicAdd(&s_h[idx + 10], fx50(v010));
+        atomicAdd(&s_h[idx + 11], fx50(v011));
+        atomicAdd(&s_h[idx + 60], fx50(v100));
+        atomicAdd(&s_h[idx + 61], fx50(v101));
+        atomicAdd(&s_h[idx + 70], fx50(v110));
+        atomicAdd(&s_h[idx + 71], fx50(v111));
     }
     __syncthreads();
     if (threadIdx.x < 128) {
@@ -912,8 +1014,11 @@ __global__ __launch_bounds__(256) void k_describe_sel(const SiftJob* __restrict_
 // keypoints of a cell are ranked in detection order; ranks < ceil(n/108) are kept in (cell,
 // rank) order, then the best of the rest fill up to n; finally a stable sort by -response.
 
+constexpr int SP_THREADS = 1024;
+constexpr int SP_SEGS = 256;
+
 __device__ int block_exclusive_scan(int v, int* s_tmp, int& total) {
-    // 256 threads; s_tmp: 256 ints
+    // SP_THREADS threads; s_tmp: SP_THREADS / 64 ints
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     int x = v;
     for (int off = 1; off < 64; off <<= 1) {
@@ -923,21 +1028,25 @@ __device__ int block_exclusive_scan(int v, int* s_tmp, int& total) {
     if (lane == 63) s_tmp[wv] = x;
     __syncthreads();
     int wsum = 0;
-    for (int w2 = 0; w2 < wv; w2++) wsum += s_tmp[w2];
-    total = s_tmp[0] + s_tmp[1] + s_tmp[2] + s_tmp[3];
+    total = 0;
+    for (int w2 = 0; w2 < SP_THREADS / 64; w2++) {
+        if (w2 < wv) wsum += s_tmp[w2];
+        total += s_tmp[w2];
+    }
     __syncthreads();
     return wsum + x - v;
 }
 
-__global__ __launch_bounds__(256) void k_spread(const SiftJob* __restrict__ jobs) {
+__global__ __launch_bounds__(SP_THREADS) void k_spread(const SiftJob* __restrict__ jobs) {
     __shared__ int s_cnt[108];
     __shared__ int s_start[109];
-    __shared__ int s_tmp[256];
+    __shared__ int s_tmp[SP_THREADS / 64];
+    __shared__ unsigned short s_seg[SP_SEGS][108];
     const SiftJob& J = jobs[blockIdx.z];
     const int N = *J.n_out, n = J.nfeat, stride = J.max_kp;
     const int tid = threadIdx.x;
     if (N <= n) {
-        for (int i = tid; i < N; i += 256) J.sel[i] = i;
+        for (int i = tid; i < N; i += SP_THREADS) J.sel[i] = i;
         if (tid == 0) J.counters[4] = N;
         return;
     }
@@ -949,18 +1058,18 @@ __global__ __launch_bounds__(256) void k_spread(const SiftJob* __restrict__ jobs
     int* spos = rank + J.kp_cap;
     int* sbefore = spos + J.kp_cap;
     const float Wf = (float)J.W, Hf = (float)J.H;
-    for (int i = tid; i < N; i += 256) {
+    for (int i = tid; i < N; i += SP_THREADS) {
         int cx = (int)((x[i] / Wf) * 12.0f), cy = (int)((y[i] / Hf) * 9.0f);
         cx = cx < 0 ? 0 : (cx > 11 ? 11 : cx);
         cy = cy < 0 ? 0 : (cy > 8 ? 8 : cy);
         cell[i] = cy * 12 + cx;
     }
-    // rank of each keypoint within its cell in detection order: 64 threads each own a
-    // contiguous segment, count per cell, exclusive-scan the counts over the segments, then
-    // rank their own segment
-    __shared__ int s_seg[64][108];
-    const int seg = (N + 63) / 64;
-    if (tid < 64) {
+    __syncthreads();
+    // rank of each keypoint within its cell in detection order: SP_SEGS threads each own a
+    // contiguous segment (< 65536 keypoints per segment), count per cell, exclusive-scan the
+    // counts over the segments, then rank their own segment
+    const int seg = (N + SP_SEGS - 1) / SP_SEGS;
+    if (tid < SP_SEGS) {
         for (int c = 0; c < 108; c++) s_seg[tid][c] = 0;
         const int a = tid * seg, b = min(N, a + seg);
         for (int i = a; i < b; i++) s_seg[tid][cell[i]]++;
@@ -968,15 +1077,15 @@ __global__ __launch_bounds__(256) void k_spread(const SiftJob* __restrict__ jobs
     __syncthreads();
     if (tid < 108) {
         int run = 0;
-        for (int t = 0; t < 64; t++) {
+        for (int t = 0; t < SP_SEGS; t++) {
             const int v = s_seg[t][tid];
-            s_seg[t][tid] = run;
+            s_seg[t][tid] = (unsigned short)run;   // < 65536: the host caps max_kp
             run += v;
         }
         s_cnt[tid] = run;
     }
     __syncthreads();
-    if (tid < 64) {
+    if (tid < SP_SEGS) {
         const int a = tid * seg, b = min(N, a + seg);
         for (int i = a; i < b; i++) rank[i] = s_seg[tid][cell[i]]++;
     }
@@ -991,7 +1100,7 @@ __global__ __launch_bounds__(256) void k_spread(const SiftJob* __restrict__ jobs
     // positions in the spread list: kept ones by (cell, rank), the rest after them in
     // detection order; selected = position < n
     int carry = 0;
-    for (int base = 0; base < N; base += 256) {
+    for (int base = 0; base < N; base += SP_THREADS) {
         const int i = base + tid;
         const bool kept = i < N && rank[i] < cap;
         int tot;
@@ -1001,7 +1110,7 @@ __global__ __launch_bounds__(256) void k_spread(const SiftJob* __restrict__ jobs
     }
     __syncthreads();
     carry = 0;
-    for (int base = 0; base < N; base += 256) {
+    for (int base = 0; base < N; base += SP_THREADS) {
         const int i = base + tid;
         const bool selq = i < N && spos[i] < n;
         int tot;
@@ -1010,7 +1119,7 @@ __global__ __launch_bounds__(256) void k_spread(const SiftJob* __restrict__ jobs
         carry += tot;
     }
     __syncthreads();
-    for (int i = tid; i < N; i += 256) {
+    for (int i = tid; i < N; i += SP_THREADS) {
         const int p = spos[i];
         if (p >= n) continue;
         const float ri = resp[i];
@@ -1191,7 +1300,7 @@ static int run_pipeline(const SiftJob* all, int n_all, const Group* groups, int 
     FSX_LAUNCH_CHECK();
     if (ev) FSX_HIP(hipEventRecord(ev[5], st));
     if (features) {
-        hipLaunchKernelGGL(k_spread, dim3(1, 1, n_all), dim3(256), 0, st, all);
+        hipLaunchKernelGGL(k_spread, dim3(1, 1, n_all), dim3(SP_THREADS), 0, st, all);
         FSX_LAUNCH_CHECK();
         if (ev) FSX_HIP(hipEventRecord(ev[6], st));
         if (nfeat_max > 0) {
@@ -1322,6 +1431,7 @@ int fsx_sift_features_batch(int n_img, const uint8_t* const* imgs, const int* H,
     FSX_REQUIRE(max_kp >= 1 && nfeat >= 0 && max_kp <= kp_cap, "need 1 <= max_kp <= kp_cap (max_kp %d kp_cap %d)",
                 max_kp, kp_cap);
     FSX_REQUIRE(table_bytes >= (size_t)n_img * sizeof(SiftJob), "job table too small");
+    FSX_REQUIRE(max_kp < 65536, "max_kp must be < 65536 for the device spread (got %d)", max_kp);
     hipStream_t st = (hipStream_t)stream;
     // group images by shape; the table holds each group contiguously
     std::vector<int> order(n_img);

@@ -108,6 +108,7 @@ class FeatureBatch:
         self.cand_cap, self.kp_cap = caps if caps else gsift.default_caps(Hm, Wm, self.max_kp)
         self.device = device
         self.float_desc = float_desc
+        self._ran = False
         self._alloc()
 
     def _alloc(self):
@@ -163,6 +164,7 @@ class FeatureBatch:
                   self.max_kp, self.nfeat, self.cand_cap, self.kp_cap, self._c_ws, self._c_wsb, self._c_kp,
                   P(self.n_out.data_ptr()), self._c_desc, self._c_u8, P(self.table.data_ptr()), self.table_bytes,
                   ev, P(st))
+        self._ran = True
 
     @classmethod
     def stage_times(cls, events):
@@ -172,6 +174,8 @@ class FeatureBatch:
 
     def check_capacity(self, stream=None):
         """Raise if any image overflowed a workspace capacity (synchronises)."""
+        if not self._ran:
+            raise RuntimeError("FeatureBatch.check_capacity() before any run()")
         st = _lib.stream_handle(stream)
         for i, ws in enumerate(self.ws):
             cnt = (C.c_int * 4)()
