@@ -167,7 +167,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-procs", type=int, default=8)
-    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_blur_c2.json"))
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_k_blur_8x2640x1978.json"))
     ap.add_argument("--gen-workers", type=int, default=8,
                     help="processes rendering the synthetic photos (1 under a profiler)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
