@@ -43,6 +43,12 @@ SIGNATURES = {
     "fsx_mfma_layout_probe": [_vp, _vp],
     "fsx_match_layout": [_i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), _i, C.POINTER(_i), C.POINTER(_i),
                          C.c_double, C.POINTER(C.c_longlong)],
+    "fsx_undistort_table_bytes": [_i, C.POINTER(_sz)],
+    "fsx_undistort_batch": [_i, C.POINTER(_i), C.POINTER(_vp), C.POINTER(C.c_double), C.POINTER(C.c_double), _i,
+                            C.POINTER(_vp), C.POINTER(_vp), _vp, _sz, _vp],
+    "fsx_undistort_f64": [_i, _vp, C.POINTER(C.c_double), _i, _vp, _vp],
+    "fsx_ground_predict": [_i, _vp, C.POINTER(C.c_double), _vp, _vp, _vp],
+    "fsx_pair_select": [_i, _vp, _vp, _i, _i, _vp, _vp],
     "fsx_match_query_state": [_i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), _i, C.POINTER(_i), C.POINTER(_i),
                               C.c_double, _vp, C.POINTER(_i), C.POINTER(C.c_double), C.POINTER(_i), _vp],
 }

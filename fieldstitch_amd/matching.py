@@ -7,6 +7,9 @@ Drop-ins with the reference's signatures and return types:
   are the feature dicts of matching._get (``und`` float64 (n,2), ``desc`` (n,128) holding
   uint8 values, ``wh``)
 * ``_unique(q, t, dd)`` (matching.py:103-110)
+* ``init_worker``, ``load_features``, ``load_xy``, ``_get`` (matching.py:17-64; rescale and
+  undistortion on the GPU) and ``_predict(u, i, j)`` (matching.py:155-172, ground-plane prior
+  on the GPU)
 
 ``PairMatcher`` runs many pairs (the pipeline's match_pair pool, pipeline.py:535-548) in one
 batched launch sequence on device-resident descriptors.  Two forms of the contraction:
@@ -17,12 +20,112 @@ FP64-oracle match set for uint8 RootSIFT descriptors (all distances are exact in
 from __future__ import annotations
 
 import ctypes as C
+import os
+from collections import OrderedDict
 
 import numpy as np
 
 from . import _lib
 
 FORMS = {"plain": 0, "mfma": 1}
+
+# ---- worker context (matching.py:13-26 _CTX / init_worker) ----------------------------------
+_CTX = None
+_CACHE = OrderedDict()
+_CACHE_MAX = 48
+
+
+def init_worker(ctx):
+    """matching.py:17-26: install the matching context (the dict the pipeline pickles: feat_dir,
+    W, H, K, dist, R, C, zg, ...).  ``ctx`` is that dict or the path of its pickle file."""
+    global _CTX
+    if isinstance(ctx, (str, os.PathLike)):
+        import pickle
+        with open(ctx, "rb") as f:
+            ctx = pickle.load(f)
+    _CTX = ctx
+    _CACHE.clear()
+
+
+def _read_npz(feat_dir, i):
+    d = np.load(os.path.join(feat_dir, "%d.npz" % i))
+    return d["xy"].astype(np.float32), d["desc"], d["red_wh"]
+
+
+def load_features(feat_dir, i, W, H):
+    """matching.py:29-35: feature file i -> (xy float64 full-resolution pixels, desc uint8).
+    The rescale (xy + 0.5) * (W/rw, H/rh) - 0.5 runs on the GPU (bit-exact float64)."""
+    xy32, desc, red = _read_npz(feat_dir, i)
+    if len(xy32) == 0:
+        return xy32.astype(np.float64), desc
+    xy, _ = _rescale_undistort(xy32, red, W, H, (1.0, 0.0, 0.0, 0, 0, 0, 0, 0))
+    return xy, desc
+
+
+def load_xy(feat_dir, i, W, H):
+    """matching.py:38-45: positions only."""
+    return load_features(feat_dir, i, W, H)[0]
+
+
+def _rescale_undistort(xy32, red_wh, W, H, k):
+    import torch
+    from . import camera
+    rw, rh = int(red_wh[0]), int(red_wh[1])
+    d = torch.from_numpy(np.ascontiguousarray(xy32, np.float32)).cuda()
+    (xy, und), = camera.undistort_batch([(d, (W / float(rw), H / float(rh)), k)])
+    return xy.cpu().numpy(), und.cpu().numpy()
+
+
+def _get(i):
+    """matching.py:48-64: cached feature dict of image i: xy, und (float64), desc (float32 of the
+    uint8 values), norm, tree (None), wh.  Rescale and undistortion run on the GPU."""
+    if i in _CACHE:
+        _CACHE.move_to_end(i)
+        return _CACHE[i]
+    c = _CTX
+    xy32, desc, red = _read_npz(c["feat_dir"], i)
+    if len(xy32):
+        K, dd = c["K"][i], c["dist"][i]
+        k = (K[0, 0], K[0, 2], K[1, 2], dd[0], dd[1], dd[2], dd[3], dd[4])
+        xy, und = _rescale_undistort(xy32, red, c["W"][i], c["H"][i], k)
+    else:
+        xy = und = xy32.astype(np.float64)
+    d = desc.astype(np.float32)
+    item = {"xy": xy, "und": und, "desc": d, "norm": (d * d).sum(1), "tree": None, "wh": (c["W"][i], c["H"][i])}
+    _CACHE[i] = item
+    while len(_CACHE) > _CACHE_MAX:
+        _CACHE.popitem(last=False)
+    return item
+
+
+def ground_predict(u, Kinv_i, R_i, C_i, zg_i, R_j, K_j, C_j, stream=None):
+    """Ground-plane prior on the GPU: undistorted pixels u (n,2) of camera i -> (pred (n,2)
+    float64 pixels in camera j, ok bool) (matching.py:155-172)."""
+    import torch
+    u = np.ascontiguousarray(np.asarray(u, np.float64).reshape(-1, 2))
+    n = len(u)
+    if n == 0:
+        return np.zeros((0, 2), np.float64), np.zeros(0, bool)
+    cams = np.concatenate([np.asarray(Kinv_i, np.float64).ravel(), np.asarray(R_i, np.float64).ravel(),
+                           np.asarray(C_i, np.float64).ravel(), [float(zg_i)], np.asarray(R_j, np.float64).ravel(),
+                           np.asarray(K_j, np.float64).ravel(), np.asarray(C_j, np.float64).ravel()])
+    if cams.size != 43:
+        raise ValueError("camera parameters must be 3x3 / 3-vectors")
+    d_u = torch.from_numpy(u).cuda()
+    pred = torch.empty((n, 2), dtype=torch.float64, device=d_u.device)
+    ok = torch.empty(n, dtype=torch.uint8, device=d_u.device)
+    c_cams = (C.c_double * 43)(*cams.tolist())
+    _lib.call("fsx_ground_predict", n, C.c_void_p(d_u.data_ptr()), c_cams, C.c_void_p(pred.data_ptr()),
+              C.c_void_p(ok.data_ptr()), C.c_void_p(_lib.stream_handle(stream)))
+    return pred.cpu().numpy(), ok.cpu().numpy().astype(bool)
+
+
+def _predict(u, i, j):
+    """matching.py:155-172: where undistorted pixels u of image i land in image j (ground plane
+    prior from the worker context)."""
+    c = _CTX
+    return ground_predict(u, np.linalg.inv(c["K"][i]), c["R"][i], c["C"][i], c["zg"][i], c["R"][j], c["K"][j],
+                          c["C"][j])
 
 
 def _unique(q, t, dd):

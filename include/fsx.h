@@ -149,6 +149,43 @@ int fsx_match_layout(int n_img, const int *n_kp, const int *W, const int *H, int
  * check the operand/result lane layout the matcher assumes (no reference counterpart). */
 int fsx_mfma_layout_probe(float *out_dev, void *stream);
 
+/* ---------------------------------------------------------------------------------------
+ * Keypoint geometry around the matcher
+ * ------------------------------------------------------------------------------------- */
+
+/* Bytes of the device parameter table fsx_undistort_batch needs for n_img keypoint sets. */
+int fsx_undistort_table_bytes(int n_img, size_t *bytes);
+
+/* Feature positions -> full-resolution undistorted pixels for n_img keypoint sets: the
+ * rescale of matching.load_features (matching.py:29-35, (xy + 0.5) * scale - 0.5) and
+ * camera.undistort_pixels (camera.py:5-30, called by matching._get, matching.py:55-57),
+ * bit-exact float64.  Set i: n_kp[i] float32 (x, y) pairs at xy_dev[i] (reduced-image px),
+ * scale[2i..2i+1] = (W/rw, H/rh), cam8[8i..8i+7] = (f, cx, cy, k1, k2, p1, p2, k3), `iters`
+ * fixed-point iterations (the reference uses 25).  Writes xy_out_dev[i] (n x 2 float64
+ * full-resolution px; the array may be NULL) and und_out_dev[i] (n x 2 float64).  table_dev:
+ * fsx_undistort_table_bytes(n_img) bytes of device memory.  Asynchronous. */
+int fsx_undistort_batch(int n_img, const int *n_kp, const float *const *xy_dev, const double *scale,
+                        const double *cam8, int iters, double *const *xy_out_dev, double *const *und_out_dev,
+                        void *table_dev, size_t table_bytes, void *stream);
+
+/* camera.undistort_pixels (camera.py:28-30) of n float64 pixels uv_dev (n x 2) -> out_dev. */
+int fsx_undistort_f64(int n, const double *uv_dev, const double *cam8, int iters, double *out_dev, void *stream);
+
+/* matching._predict (matching.py:155-172): ground-plane prior of n undistorted pixels u_dev
+ * (n x 2 float64) of camera i into camera j.  cams (host, 43 doubles, row-major):
+ * inv(K_i)[9], R_i[9], C_i[3], zg_i, R_j[9], K_j[9], C_j[3].  Writes pred_dev (n x 2 float64)
+ * and ok_dev (n bytes: t > 0 and z > 0).  Asynchronous. */
+int fsx_ground_predict(int n, const double *u_dev, const double *cams, double *pred_dev, uint8_t *ok_dev,
+                       void *stream);
+
+/* Candidate pairs from GPS positions (pipeline.py:452-476).  xy_dev: n x 2 float64;
+ * half_long_dev: n float64; k = min(n - 1, neighbors) <= 64; has_rel != 0 applies the
+ * 2 * half_long * 0.95 limit.  out_dev (n x (k + 2) x 2 int32): per photo its k neighbour
+ * pairs (min, max) then its 2 temporal pairs (i, i + off); (-1, -1) where none.  The caller
+ * takes the sorted set of the valid entries.  Asynchronous. */
+int fsx_pair_select(int n, const double *xy_dev, const double *half_long_dev, int k, int has_rel, int *out_dev,
+                    void *stream);
+
 #ifdef __cplusplus
 }
 #endif
