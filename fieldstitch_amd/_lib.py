@@ -38,11 +38,14 @@ SIGNATURES = {
     "fsx_match_batch": [_i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), _vp, _vp, _i, C.POINTER(_i), C.POINTER(_i),
                         C.POINTER(_i), C.POINTER(C.c_double), C.c_double, _i, C.c_double, _i, _vp, _sz,
                         C.POINTER(C.c_longlong), _vp, _vp],
+    "fsx_match_prepare": [_i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), _vp, _vp, _i, C.POINTER(_i),
+                          C.POINTER(_i), C.POINTER(_i), C.POINTER(C.c_double), C.c_double, _i, C.c_double, _vp, _sz,
+                          C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), _vp],
+    "fsx_match_launch": [_vp, C.POINTER(C.c_longlong), _i, _vp, _vp],
     "fsx_match_results": [_i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), _i, C.POINTER(_i), C.POINTER(_i),
                           C.c_double, _vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), _vp],
     "fsx_mfma_layout_probe": [_vp, _vp],
-    "fsx_match_layout": [_i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), _i, C.POINTER(_i), C.POINTER(_i),
-                         C.c_double, C.POINTER(C.c_longlong)],
+    "fsx_mfma_i8_layout_probe": [_vp, _vp],
     "fsx_undistort_table_bytes": [_i, C.POINTER(_sz)],
     "fsx_undistort_batch": [_i, C.POINTER(_i), C.POINTER(_vp), C.POINTER(C.c_double), C.POINTER(C.c_double), _i,
                             C.POINTER(_vp), C.POINTER(_vp), _vp, _sz, _vp],
@@ -64,6 +67,8 @@ def load(path: str = LIB_PATH):
     """Load libfsx.so (raises OSError if it has not been built)."""
     global _lib
     if _lib is None:
+        # torch first: its HIP runtime must be the one libfsx binds to (one runtime per process)
+        import torch  # noqa: F401
         if not os.path.exists(path):
             raise OSError("libfsx.so not built (%s); run fieldstitch_amd/build.py or __graft_entry__.build()" % path)
         L = C.CDLL(path)

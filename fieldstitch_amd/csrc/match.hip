@@ -8,15 +8,24 @@
 // B for brute force; d = sum_k (a_k - b_k)^2; best = argmin d (ties: lower index), b2 = the
 // second smallest d (+inf with one candidate); accept iff sqrt(b1) < ratio*sqrt(b2) in
 // float64; per target keep the smallest b1, ties to the lower query; output sorted by target.
-// uint8 RootSIFT descriptors make every d an exact integer < 2^24, so the float32 forms and
-// the BF16-input MFMA form (uint8 is exact in bf16, sums exact in float32) are exact.
+// uint8 RootSIFT descriptors make every d an exact integer < 2^24, so the float32 form and
+// the integer MFMA form are exact.
 //
-// Two forms of the candidate contraction:
-//   plain: one thread per query, FP32 gather + dot products on VALU (the baseline form);
-//   mfma:  32 spatially neighbouring queries per wave (each keypoint set is bucketed by grid
-//          cell in tile-major cell order, which the spatial grid needs anyway), the union of
-//          their candidate cells contracted on MFMA (32x32x16 BF16, FP32 accumulation) with
-//          the prior-window mask and the top-2 selection fused into the epilogue.
+// Three launches per batch:
+//   setup:       one block per image buckets its keypoints into a grid of radius-sized cells
+//                (tile-major numbering) and writes the cell-sorted index and positions, so a
+//                cell's candidates are one contiguous range; the remaining blocks compute the
+//                descriptor norms and reset the per-target keys;
+//   contraction: plain: one thread per query, FP32 gather + dot products on VALU (the
+//                baseline form); mfma: 32 cell-consecutive queries per wave, the union of
+//                their candidate cells contracted on MFMA with the prior-window mask and the
+//                top-2 selection fused into the epilogue.  The MFMA operands are the bytes
+//                shifted by -128 (u8 ^ 0x80 = signed int8): sum_k (a_k - b_k)^2 is unchanged by
+//                a common shift, so d = |a'|^2 + |b'|^2 - 2 a'.b' with a 32x32x32 INT8 MFMA and
+//                exact int32 accumulation.
+//                Both predict inline and publish accepted matches with one 64-bit atomicMin
+//                of (b1 float bits, query) per target, which is the one-query-per-target rule;
+//   compact:     per pair, the surviving targets in target order -> (q, t) lists.
 #include "fsx_common.h"
 #include <math.h>
 #include <string.h>
@@ -26,7 +35,7 @@
 namespace fsx {
 
 struct MImage {
-    long long off;      // first keypoint in the concatenated arrays
+    long long off;      // first keypoint in the concatenated arrays (caller's and cell-sorted)
     int n;              // keypoints
     int W, H;           // image size (inside test on the B side)
     int gw, gh;         // spatial grid (cells of `radius` px covering [-2r, W+2r) x [-2r, H+2r))
@@ -47,55 +56,35 @@ struct MPair {
 };
 
 struct MatchBatch {
-    const uint8_t* desc;          // [Ntot][128] RootSIFT uint8
+    const uint8_t* desc;          // [Ntot][128] RootSIFT uint8 (caller's order)
     const double* xy;             // [Ntot][2] positions (undistorted px)
     const MImage* img;
     const MPair* pair;
     int n_img, n_pair;
-    long long n_query, n_kp, n_cells, n_targets;
+    long long n_query, n_kp, n_cells, n_targets, n_chunks;
     double radius, r2, ratio;
     int kc;
     int pad;
-    // scratch
-    unsigned* norms;              // [Ntot] sum of squares
-    int* cell_start;              // [n_cells + n_img]
-    int* cell_cur;                // [n_cells]
-    int* cell_idx;                // [Ntot] local keypoint indices sorted by cell
-    double* pred;                 // [n_query][2]
-    int* qcell;                   // [n_query] cell of the prediction in b's grid, -1 = not inside
-    int* best;                    // [n_query] accepted best target (local), -1 = none
-    double* b1;                   // [n_query]
-    double* kth;                  // [n_query][2] kc-th nearest (d2, idx) threshold, kc > 0
-    unsigned long long* tkey;     // [n_targets] min b1 bits per target
-    int* tq;                      // [n_targets] min query among equal b1
+    unsigned* norms;              // [Ntot] sum of squares of the descriptor
+    unsigned* snorms;             // [Ntot] sum of squares of the descriptor - 128
+    // cell-sorted index: positions [off, off + n) of each image in grid-cell order
+    double* s_xy;                 // [Ntot][2] positions
+    int* s_orig;                  // [Ntot] the keypoint's index in its image
+    int* cell_start;              // [n_cells + n_img] per image ncell + 1 offsets
+    int* cell_cur;                // [n_cells] cursors of grids too large for LDS
+    // per query, in the caller's order (diagnostics)
+    int* best;                    // accepted best target, -1 = none
+    double* b1;                   // its distance
+    int* qcell;                   // cell of the prediction in b's grid, -1 = not inside
+    double* kth;                  // [n_query][2] kc-th nearest (d2, idx bits) key, kc > 0 only
+    unsigned long long* tkey;     // [n_targets] min over accepted queries of (b1 bits << 32 | q)
     int* counters;                // [0..3] flags
-    long long n_chunks;           // sum over pairs of ceil(n_a / 32)
-    int* out_q;                   // [Σ min(n_a, n_b)]
+    int* out_q;                   // [sum min(n_a, n_b)]
     int* out_t;
     int* out_n;                   // [n_pair]
 };
 
 // ------------------------------------------------------------------------------------------
-__device__ __forceinline__ int find_pair(const MatchBatch& B, long long g) {
-    int lo = 0, hi = B.n_pair - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (B.pair[mid].q_off <= g) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
-__device__ __forceinline__ int find_image(const MatchBatch& B, long long k) {
-    int lo = 0, hi = B.n_img - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (B.img[mid].off <= k) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
 // tile-major cell numbering: 4x4-cell tiles row by row, cells row by row inside a tile, so a
 // run of consecutive cells (and of the keypoints bucketed into them) is spatially compact
 __device__ __forceinline__ int cell_id(const MImage& im, int cx, int cy) {
@@ -115,181 +104,221 @@ __device__ __forceinline__ void cell_of(const MatchBatch& B, const MImage& im, d
     cy = cy < 0 ? 0 : (cy >= im.gh ? im.gh - 1 : cy);
 }
 
-__global__ void k_m_reset(MatchBatch B) {
-    const long long tid = (long long)blockIdx.x * 256 + threadIdx.x, stride = (long long)gridDim.x * 256;
-    for (long long i = tid; i < B.n_targets; i += stride) {
-        B.tkey[i] = ~0ull;
-        B.tq[i] = 0x7fffffff;
-    }
-    for (long long i = tid; i < B.n_cells; i += stride) B.cell_cur[i] = 0;
-    for (long long i = tid; i < B.n_pair; i += stride) B.out_n[i] = 0;
-    if (tid < 4) B.counters[tid] = 0;
-}
+// ------------------------------------------------------------------------------------------
+// Setup
 
-// sum of squares of every descriptor (exact integer for uint8) and its grid cell count
-__global__ void k_m_norms_count(MatchBatch B) {
-    const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (k >= B.n_kp) return;
-    const uint4* d = (const uint4*)(B.desc + k * 128);
-    unsigned s = 0;
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-        uint4 v = d[i];
-        s = __builtin_amdgcn_udot4(v.x, v.x, s, false);
-        s = __builtin_amdgcn_udot4(v.y, v.y, s, false);
-        s = __builtin_amdgcn_udot4(v.z, v.z, s, false);
-        s = __builtin_amdgcn_udot4(v.w, v.w, s, false);
-    }
-    B.norms[k] = s;
-    const int ii = find_image(B, k);
-    const MImage& im = B.img[ii];
-    int cx, cy;
-    cell_of(B, im, B.xy[2 * k], B.xy[2 * k + 1], cx, cy);
-    atomicAdd(&B.cell_cur[im.cell_off + cell_id(im, cx, cy)], 1);
-}
+constexpr int SU_THREADS = 1024;
+constexpr int SU_LDS_CELLS = 8192;      // grids up to this many cells count in LDS
+constexpr int SU_KPT = 8;               // keypoints per thread held in registers
 
-// one block per image: exclusive scan of the cell counts -> cell_start; cursors reset
-__global__ __launch_bounds__(256) void k_m_grid_scan(MatchBatch B) {
-    __shared__ int s_w[4];
-    const MImage& im = B.img[blockIdx.x];
-    const int nc = im.ncell;
-    int* start = B.cell_start + im.cell_off + blockIdx.x;   // nc + 1 entries per image
-    int* cnt = B.cell_cur + im.cell_off;
-    int carry = 0;
+__global__ __launch_bounds__(SU_THREADS) void k_m_setup(const MatchBatch B) {
+    __shared__ int s_cnt[SU_LDS_CELLS];
+    __shared__ int s_w[SU_THREADS / 64];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    for (int base = 0; base < nc; base += 256) {
+    if ((int)blockIdx.x >= B.n_img) {
+        const long long base = (long long)(blockIdx.x - B.n_img) * SU_THREADS + tid;
+        const long long stride = (long long)(gridDim.x - B.n_img) * SU_THREADS;
+        for (long long t = base; t < B.n_targets; t += stride) B.tkey[t] = ~0ull;
+        for (long long k = base; k < B.n_kp; k += stride) {
+            const uint4* d = (const uint4*)(B.desc + k * 128);
+            unsigned s = 0, s8 = 0;
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const uint4 v = d[i];
+                const unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int h = 0; h < 4; h++) {
+                    s = __builtin_amdgcn_udot4(w[h], w[h], s, false);
+                    const int x = (int)(w[h] ^ 0x80808080u);
+                    s8 = (unsigned)__builtin_amdgcn_sdot4(x, x, (int)s8, false);
+                }
+            }
+            B.norms[k] = s;
+            B.snorms[k] = s8;
+        }
+        if (base < 4) B.counters[base] = 0;
+        return;
+    }
+    const int ii = blockIdx.x;
+    const MImage& im = B.img[ii];
+    const int nc = im.ncell;
+    int* cnt = nc <= SU_LDS_CELLS ? s_cnt : B.cell_cur + im.cell_off;   // only this block uses it
+    for (int c = tid; c < nc; c += SU_THREADS) cnt[c] = 0;
+    // keypoints k = tid + SU_THREADS * r, r < SU_KPT, stay in registers between the passes
+    // (all loads in flight at once); larger images take the loop below for the rest
+    double kx[SU_KPT], ky[SU_KPT];
+    int kcell[SU_KPT];
+#pragma unroll
+    for (int r = 0; r < SU_KPT; r++) {
+        const int k = tid + SU_THREADS * r;
+        kx[r] = k < im.n ? B.xy[2 * (im.off + k)] : 0.0;
+        ky[r] = k < im.n ? B.xy[2 * (im.off + k) + 1] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < SU_KPT; r++) {
+        const int k = tid + SU_THREADS * r;
+        int cx, cy;
+        cell_of(B, im, kx[r], ky[r], cx, cy);
+        kcell[r] = cell_id(im, cx, cy);
+        if (k < im.n) atomicAdd(&cnt[kcell[r]], 1);
+    }
+    for (int k = tid + SU_THREADS * SU_KPT; k < im.n; k += SU_THREADS) {
+        int cx, cy;
+        cell_of(B, im, B.xy[2 * (im.off + k)], B.xy[2 * (im.off + k) + 1], cx, cy);
+        atomicAdd(&cnt[cell_id(im, cx, cy)], 1);
+    }
+    __syncthreads();
+    // exclusive scan of the counts -> cell_start; the counts become insertion cursors
+    int* start = B.cell_start + im.cell_off + ii;
+    int carry = 0;
+    for (int base = 0; base < nc; base += SU_THREADS) {
         const int i = base + tid;
         const int v = i < nc ? cnt[i] : 0;
         int x = v;
         for (int off = 1; off < 64; off <<= 1) {
-            int y = __shfl_up(x, off);
+            const int y = __shfl_up(x, off);
             if (lane >= off) x += y;
         }
         if (lane == 63) s_w[wv] = x;
         __syncthreads();
-        int pre = 0;
-        for (int w2 = 0; w2 < wv; w2++) pre += s_w[w2];
-        const int tot = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+        int pre = 0, tot = 0;
+        for (int w2 = 0; w2 < SU_THREADS / 64; w2++) {
+            pre += w2 < wv ? s_w[w2] : 0;
+            tot += s_w[w2];
+        }
         if (i < nc) {
             start[i] = carry + pre + x - v;
-            cnt[i] = 0;
+            cnt[i] = carry + pre + x - v;
         }
         carry += tot;
         __syncthreads();
     }
     if (tid == 0) start[nc] = carry;
-}
-
-__global__ void k_m_grid_fill(MatchBatch B) {
-    const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (k >= B.n_kp) return;
-    const int ii = find_image(B, k);
-    const MImage& im = B.img[ii];
-    int cx, cy;
-    cell_of(B, im, B.xy[2 * k], B.xy[2 * k + 1], cx, cy);
-    const long long c = cell_id(im, cx, cy);
-    const int* start = B.cell_start + im.cell_off + ii;
-    const int slot = start[c] + atomicAdd(&B.cell_cur[im.cell_off + c], 1);
-    B.cell_idx[im.off + slot] = (int)(k - im.off);
-}
-
-// prediction (cv2.perspectiveTransform: w = (h6 x + h7 y) + h8, |w| > FLT_EPSILON ? v * (1/w)
-// : 0) and the strict inside test of matching.py:119-120
-__global__ void k_m_predict(MatchBatch B) {
-    const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (g >= B.n_query) return;
-    const int p = find_pair(B, g);
-    const MPair& P = B.pair[p];
-    const MImage& ia = B.img[P.a];
-    const MImage& ib = B.img[P.b];
-    const long long q = g - P.q_off;
-    const double x = B.xy[2 * (ia.off + q)], y = B.xy[2 * (ia.off + q) + 1];
-    double px, py;
-    if (P.dense) {
-        px = 0.0;
-        py = 0.0;
-    } else {
-        const double w = (x * P.Hm[6] + y * P.Hm[7]) + P.Hm[8];
-        if (fabs(w) > 1.1920928955078125e-07) {
-            const double iw = 1.0 / w;
-            px = ((x * P.Hm[0] + y * P.Hm[1]) + P.Hm[2]) * iw;
-            py = ((x * P.Hm[3] + y * P.Hm[4]) + P.Hm[5]) * iw;
-        } else {
-            px = 0.0;
-            py = 0.0;
+    __syncthreads();
+    // scatter into the cell-sorted index (order inside a cell is arbitrary: every result is
+    // defined by indices, not by visiting order)
+#pragma unroll
+    for (int r = 0; r < SU_KPT; r++) {
+        const int k = tid + SU_THREADS * r;
+        if (k < im.n) {
+            const long long s = im.off + atomicAdd(&cnt[kcell[r]], 1);
+            B.s_orig[s] = k;
+            B.s_xy[2 * s] = kx[r];
+            B.s_xy[2 * s + 1] = ky[r];
         }
     }
-    B.pred[2 * g] = px;
-    B.pred[2 * g + 1] = py;
-    int cell = -1;
-    if (P.dense) {
-        cell = (ib.n >= 2 && ia.n >= 2) ? 0 : -1;
-    } else if (px > -B.radius && px < ib.W + B.radius && py > -B.radius && py < ib.H + B.radius && ib.n > 0) {
+    for (int k = tid + SU_THREADS * SU_KPT; k < im.n; k += SU_THREADS) {
+        const long long kg = im.off + k;
+        const double x = B.xy[2 * kg], y = B.xy[2 * kg + 1];
+        int cx, cy;
+        cell_of(B, im, x, y, cx, cy);
+        const long long s = im.off + atomicAdd(&cnt[cell_id(im, cx, cy)], 1);
+        B.s_orig[s] = k;
+        B.s_xy[2 * s] = x;
+        B.s_xy[2 * s + 1] = y;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Shared pieces of the two contraction forms
+
+// prediction (cv2.perspectiveTransform: w = (h6 x + h7 y) + h8, |w| > FLT_EPSILON ? v * (1/w)
+// : 0) and the strict inside test of matching.py:119-120; returns the cell in b's grid or -1.
+// Brute-force pairs need two keypoints on each side (matching.py:88-89).
+__device__ __forceinline__ int predict_cell(const MatchBatch& B, const MPair& P, const MImage& ia, const MImage& ib,
+                                            double x, double y, double& px, double& py) {
+    px = 0.0;
+    py = 0.0;
+    if (P.dense) return (ib.n >= 2 && ia.n >= 2) ? 0 : -1;
+    const double w = (x * P.Hm[6] + y * P.Hm[7]) + P.Hm[8];
+    if (fabs(w) > 1.1920928955078125e-07) {
+        const double iw = 1.0 / w;
+        px = ((x * P.Hm[0] + y * P.Hm[1]) + P.Hm[2]) * iw;
+        py = ((x * P.Hm[3] + y * P.Hm[4]) + P.Hm[5]) * iw;
+    }
+    if (px > -B.radius && px < ib.W + B.radius && py > -B.radius && py < ib.H + B.radius && ib.n > 0) {
         int cx, cy;
         cell_of(B, ib, px, py, cx, cy);
-        cell = cell_id(ib, cx, cy);
+        return cell_id(ib, cx, cy);
     }
-    B.qcell[g] = cell;
-    B.best[g] = -1;
+    return -1;
 }
 
 // spatial (d2, idx) key comparison for the kc-nearest rule
 __device__ __forceinline__ bool sp_less(double d, int i, double d2, int i2) { return d < d2 || (d == d2 && i < i2); }
 
-// kc-th nearest spatial threshold of every query (kc > 0 only): the candidate set is
-// {in radius and (d2, idx) <= threshold}
-__global__ void k_m_knn(MatchBatch B) {
-    const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (g >= B.n_query) return;
-    const int cell = B.qcell[g];
-    double td = B.r2;
-    int ti = 0x7fffffff;
-    const int p = find_pair(B, g);
-    const MPair& P = B.pair[p];
-    if (cell >= 0 && !P.dense) {
-        constexpr int KMAX = 64;
-        double ld[KMAX];
-        int li[KMAX];
-        int nl = 0;
-        const int kc = B.kc < KMAX ? B.kc : KMAX;
-        const MImage& ib = B.img[P.b];
-        const double px = B.pred[2 * g], py = B.pred[2 * g + 1];
-        int cx0, cy0;
-        cell_xy(ib, cell, cx0, cy0);
-        const int* start = B.cell_start + ib.cell_off + P.b;
-        for (int cy = max(cy0 - 1, 0); cy <= min(cy0 + 1, ib.gh - 1); cy++)
-            for (int cx = max(cx0 - 1, 0); cx <= min(cx0 + 1, ib.gw - 1); cx++) {
-                const int c = cell_id(ib, cx, cy);
-                for (int s = start[c]; s < start[c + 1]; s++) {
-                    const int t = B.cell_idx[ib.off + s];
-                    const double dx = B.xy[2 * (ib.off + t)] - px, dy = B.xy[2 * (ib.off + t) + 1] - py;
-                    const double d = dx * dx + dy * dy;
-                    if (!(d < B.r2)) continue;
-                    if (nl == kc && !sp_less(d, t, ld[nl - 1], li[nl - 1])) continue;
-                    int j = nl < kc ? nl++ : kc - 1;
-                    while (j > 0 && sp_less(d, t, ld[j - 1], li[j - 1])) {
-                        ld[j] = ld[j - 1];
-                        li[j] = li[j - 1];
-                        j--;
-                    }
-                    ld[j] = d;
-                    li[j] = t;
+// kc-th nearest spatial key of a query (kc > 0): the candidate set is {in radius and
+// (d2, idx) <= that key}; (r2, INT_MAX) when fewer than kc keypoints are in the radius
+constexpr int KMAX = 64;
+
+__device__ void knn_threshold(const MatchBatch& B, const MPair& P, const MImage& ib, int cell, double px, double py,
+                              double& td, int& ti) {
+    td = B.r2;
+    ti = 0x7fffffff;
+    if (B.kc <= 0 || cell < 0 || P.dense) return;
+    double ld[KMAX];
+    int li[KMAX];
+    int nl = 0;
+    const int kc = B.kc < KMAX ? B.kc : KMAX;
+    int cx0, cy0;
+    cell_xy(ib, cell, cx0, cy0);
+    const int* cs = B.cell_start + ib.cell_off + P.b;
+    for (int cy = max(cy0 - 1, 0); cy <= min(cy0 + 1, ib.gh - 1); cy++)
+        for (int cx = max(cx0 - 1, 0); cx <= min(cx0 + 1, ib.gw - 1); cx++) {
+            const int c = cell_id(ib, cx, cy);
+            for (int s = cs[c]; s < cs[c + 1]; s++) {
+                const long long sb = ib.off + s;
+                const int t = B.s_orig[sb];
+                const double dx = B.s_xy[2 * sb] - px, dy = B.s_xy[2 * sb + 1] - py;
+                const double d = dx * dx + dy * dy;
+                if (!(d < B.r2)) continue;
+                if (nl == kc && !sp_less(d, t, ld[nl - 1], li[nl - 1])) continue;
+                int j = nl < kc ? nl++ : kc - 1;
+                while (j > 0 && sp_less(d, t, ld[j - 1], li[j - 1])) {
+                    ld[j] = ld[j - 1];
+                    li[j] = li[j - 1];
+                    j--;
                 }
+                ld[j] = d;
+                li[j] = t;
             }
-        if (nl == kc) {
-            td = ld[kc - 1];
-            ti = li[kc - 1];
         }
+    if (nl == kc) {
+        td = ld[kc - 1];
+        ti = li[kc - 1];
     }
+}
+
+// kc > 0: the kc-th nearest key of every query, one thread per query (before the contraction)
+__global__ __launch_bounds__(256) void k_m_knn(const MatchBatch B) {
+    const MPair& P = B.pair[blockIdx.y];
+    const MImage& ia = B.img[P.a];
+    const MImage& ib = B.img[P.b];
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= ia.n) return;
+    const long long g = P.q_off + q;
+    const long long ka = ia.off + q;
+    double px, py, td;
+    int ti;
+    const int cell = predict_cell(B, P, ia, ib, B.xy[2 * ka], B.xy[2 * ka + 1], px, py);
+    knn_threshold(B, P, ib, cell, px, py, td, ti);
     B.kth[2 * g] = td;
     B.kth[2 * g + 1] = __longlong_as_double((long long)ti);
 }
 
-__device__ __forceinline__ bool in_candidates(const MatchBatch& B, long long g, double d, int t) {
+__device__ __forceinline__ void kth_of(const MatchBatch& B, long long g, double& td, int& ti) {
+    td = B.r2;
+    ti = 0x7fffffff;
+    if (B.kc > 0) {
+        td = B.kth[2 * g];
+        ti = (int)__double_as_longlong(B.kth[2 * g + 1]);
+    }
+}
+
+__device__ __forceinline__ bool in_window(const MatchBatch& B, double d, int t, double td, int ti) {
     if (!(d < B.r2)) return false;
     if (B.kc <= 0) return true;
-    const double td = B.kth[2 * g];
-    const int ti = (int)__double_as_longlong(B.kth[2 * g + 1]);
     return d < td || (d == td && t <= ti);
 }
 
@@ -304,20 +333,24 @@ __device__ __forceinline__ void top2_push(float d, int t, float& v1, int& i1, fl
     }
 }
 
-// ratio test in float64 + the first phase of the per-target minimum
-__device__ __forceinline__ void accept_query(const MatchBatch& B, long long g, const MPair& P, float v1, int i1, float v2) {
-    if (i1 < 0) return;
-    const double b1 = (double)v1, b2 = (double)v2;
-    if (sqrt(b1) < B.ratio * sqrt(b2)) {
-        B.best[g] = i1;
-        B.b1[g] = b1;
-        atomicMin(&B.tkey[P.t_off + i1], (unsigned long long)__double_as_longlong(b1));
+// ratio test in float64 (matching.py:145-147) and the one-query-per-target key: the smallest
+// (b1, q) wins, which is _unique's lexsort((dd, t)) first element per target
+__device__ __forceinline__ void accept_query(const MatchBatch& B, long long g, const MPair& P, int q, float v1, int i1,
+                                             float v2) {
+    const bool ok = i1 >= 0 && sqrt((double)v1) < B.ratio * sqrt((double)v2);
+    B.best[g] = ok ? i1 : -1;
+    B.b1[g] = ok ? (double)v1 : 0.0;
+    if (ok) {
+        const unsigned long long key = ((unsigned long long)(__float_as_uint(v1) & 0x7fffffffu) << 32) | (unsigned)q;
+        atomicMin(&B.tkey[P.t_off + i1], key);
     }
 }
 
-// the plain FP32 GPU form: one thread per query; the query descriptor stays in registers as
-// float32, each candidate descriptor is gathered from global memory (16-byte loads) and the
-// dot product runs on VALU in FP32: d = na + nb - 2 a.b (exact for uint8 inputs)
+// ------------------------------------------------------------------------------------------
+// The plain FP32 GPU form: one thread per query; the query descriptor stays in registers as
+// float32, each in-window candidate descriptor is gathered from global memory (16-byte loads)
+// and the dot product runs on VALU in FP32: d = na + nb - 2 a.b (exact for uint8 inputs)
+
 __device__ __forceinline__ float u8dot_f32(const float* qa, const uint4* cb) {
     float dot = 0.f;
 #pragma unroll
@@ -332,16 +365,24 @@ __device__ __forceinline__ float u8dot_f32(const float* qa, const uint4* cb) {
     return dot;
 }
 
-__global__ __launch_bounds__(256) void k_m_plain(MatchBatch B) {
-    const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (g >= B.n_query) return;
-    const int cell = B.qcell[g];
-    if (cell < 0) return;
-    const int p = find_pair(B, g);
-    const MPair& P = B.pair[p];
+__global__ __launch_bounds__(256) void k_m_plain(const MatchBatch B) {
+    const MPair& P = B.pair[blockIdx.y];
     const MImage& ia = B.img[P.a];
     const MImage& ib = B.img[P.b];
-    const long long ka = ia.off + (g - P.q_off);
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= ia.n) return;
+    const long long g = P.q_off + q;
+    const long long ka = ia.off + q;
+    double px, py;
+    const int cell = predict_cell(B, P, ia, ib, B.xy[2 * ka], B.xy[2 * ka + 1], px, py);
+    B.qcell[g] = cell;
+    if (cell < 0) {
+        B.best[g] = -1;
+        return;
+    }
+    double td;
+    int ti;
+    kth_of(B, g, td, ti);
     float qa[128];
     {
         const uint4* q4 = (const uint4*)(B.desc + ka * 128);
@@ -359,107 +400,57 @@ __global__ __launch_bounds__(256) void k_m_plain(MatchBatch B) {
     float v1 = INFINITY, v2 = INFINITY;
     int i1 = -1;
     auto eval = [&](int t) {
-        const float dot = u8dot_f32(qa, (const uint4*)(B.desc + (ib.off + t) * 128));
-        const float d = fmaxf((na + (float)B.norms[ib.off + t]) - 2.0f * dot, 0.0f);
+        const long long kb = ib.off + t;
+        const float dot = u8dot_f32(qa, (const uint4*)(B.desc + kb * 128));
+        const float d = fmaxf((na + (float)B.norms[kb]) - 2.0f * dot, 0.0f);
         top2_push(d, t, v1, i1, v2);
     };
     if (P.dense) {
         for (int t = 0; t < ib.n; t++) eval(t);
     } else {
-        const double px = B.pred[2 * g], py = B.pred[2 * g + 1];
         int cx0, cy0;
         cell_xy(ib, cell, cx0, cy0);
-        const int* start = B.cell_start + ib.cell_off + P.b;
+        const int* cs = B.cell_start + ib.cell_off + P.b;
         for (int cy = max(cy0 - 1, 0); cy <= min(cy0 + 1, ib.gh - 1); cy++)
             for (int cx = max(cx0 - 1, 0); cx <= min(cx0 + 1, ib.gw - 1); cx++) {
                 const int c = cell_id(ib, cx, cy);
-                for (int s = start[c]; s < start[c + 1]; s++) {
-                    const int t = B.cell_idx[ib.off + s];
-                    const double dx = B.xy[2 * (ib.off + t)] - px, dy = B.xy[2 * (ib.off + t) + 1] - py;
-                    if (in_candidates(B, g, dx * dx + dy * dy, t)) eval(t);
+                for (int s = cs[c]; s < cs[c + 1]; s++) {
+                    const long long sb = ib.off + s;
+                    const double dx = B.s_xy[2 * sb] - px, dy = B.s_xy[2 * sb + 1] - py;
+                    const int t = B.s_orig[sb];
+                    if (in_window(B, dx * dx + dy * dy, t, td, ti)) eval(t);
                 }
             }
     }
-    accept_query(B, g, P, v1, i1, v2);
-}
-
-// second phase of the per-target minimum: among queries with the minimal b1, the lowest q
-__global__ void k_m_unique2(MatchBatch B) {
-    const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (g >= B.n_query) return;
-    const int t = B.best[g];
-    if (t < 0) return;
-    const int p = find_pair(B, g);
-    const MPair& P = B.pair[p];
-    if ((unsigned long long)__double_as_longlong(B.b1[g]) == B.tkey[P.t_off + t])
-        atomicMin(&B.tq[P.t_off + t], (int)(g - P.q_off));
-}
-
-// one block per pair: winners in target order -> (q, t) lists
-__global__ __launch_bounds__(256) void k_m_compact(MatchBatch B) {
-    __shared__ int s_w[4];
-    const MPair& P = B.pair[blockIdx.x];
-    const int nb = B.img[P.b].n;
-    const int* tq = B.tq + P.t_off;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    int carry = 0;
-    for (int base = 0; base < nb; base += 256) {
-        const int t = base + tid;
-        const int v = (t < nb && tq[t] != 0x7fffffff) ? 1 : 0;
-        int x = v;
-        for (int off = 1; off < 64; off <<= 1) {
-            int y = __shfl_up(x, off);
-            if (lane >= off) x += y;
-        }
-        if (lane == 63) s_w[wv] = x;
-        __syncthreads();
-        int pre = 0;
-        for (int w2 = 0; w2 < wv; w2++) pre += s_w[w2];
-        const int tot = s_w[0] + s_w[1] + s_w[2] + s_w[3];
-        if (v) {
-            const long long o = P.o_off + carry + pre + x - 1;
-            B.out_q[o] = tq[t];
-            B.out_t[o] = t;
-        }
-        carry += tot;
-        __syncthreads();
-    }
-    if (tid == 0) B.out_n[blockIdx.x] = carry;
+    accept_query(B, g, P, q, v1, i1, v2);
 }
 
 // ------------------------------------------------------------------------------------------
-// MFMA form.  A chunk = 32 consecutive keypoints of the query image in its cell bucketing
-// (tile-major cells), so the chunk's predictions land in a compact region of B (the prior is a
-// smooth mapping).  One wave per chunk: the union of the queries' 3x3 candidate cells is
-// listed once, its candidates are contracted against the 32 queries on MFMA, 32 candidates per
-// 32x32x16 BF16 tile (uint8 -> bf16 is exact, FP32 sums of uint8 products are exact), and
-// the epilogue applies the prior-window mask and the top-2 update.
+// MFMA form.  A chunk = 32 consecutive keypoints of the query image in its cell-sorted order,
+// so the chunk's predictions land in a compact region of B (the prior is a smooth mapping).
+// One wave per chunk: the union of the queries' 3x3 candidate cells is listed once, its
+// candidates are contracted against the 32 queries on MFMA, 32 candidates per 32x32x32 INT8
+// tile (4 instructions cover the 128 bytes), and the epilogue applies the prior-window mask
+// and the top-2 update.  Lane l and lane l+32 hold query l; lane half h = l / 32 supplies
+// bytes [32 s + 16 h, 32 s + 16 h + 16) of its row to instruction s (the same byte positions
+// for both operands, so the instruction's internal k order does not matter).
 
+typedef int i32x4_t __attribute__((ext_vector_type(4)));
+typedef int i32x16_t __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
 
-constexpr int MF_CAP = 512;     // candidates per pass staged in LDS
+constexpr int MF_CAP = 256;     // candidates per pass staged in LDS
 constexpr int MF_MAXC = 256;    // listed candidate cells per chunk
 
-__device__ __forceinline__ bf16x8_t u8x8_to_bf16(uint2 v) {
-    bf16x8_t r;
-#pragma unroll
-    for (int e = 0; e < 8; e++) {
-        const unsigned byte = ((e < 4 ? v.x : v.y) >> (8 * (e & 3))) & 0xffu;
-        const float f = (float)byte;
-        r[e] = __builtin_bit_cast(__bf16, (unsigned short)(__float_as_uint(f) >> 16));
-    }
+// 16 descriptor bytes as signed int8 (byte - 128)
+__device__ __forceinline__ i32x4_t s8x16(uint4 v) {
+    i32x4_t r;
+    r[0] = (int)(v.x ^ 0x80808080u);
+    r[1] = (int)(v.y ^ 0x80808080u);
+    r[2] = (int)(v.z ^ 0x80808080u);
+    r[3] = (int)(v.w ^ 0x80808080u);
     return r;
-}
-
-__device__ __forceinline__ int find_chunk_pair(const MatchBatch& B, long long ch) {
-    int lo = 0, hi = B.n_pair - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (B.pair[mid].c_off <= ch) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
 }
 
 __device__ __forceinline__ int wave_excl_scan(int v, int lane, int& total) {
@@ -473,13 +464,13 @@ __device__ __forceinline__ int wave_excl_scan(int v, int lane, int& total) {
 }
 
 // per-query evaluation of the 3x3 candidate cells (chunks whose cells do not fit the window)
-__device__ void query_direct(const MatchBatch& B, const MPair& P, const MImage& ia, const MImage& ib, int gq, int cell,
-                             double qpx, double qpy, float qn, float& v1, int& i1, float& v2) {
+__device__ void query_direct(const MatchBatch& B, const MPair& P, const MImage& ib, long long ka, int cell,
+                             double qpx, double qpy, double td, int ti, float& v1, int& i1, float& v2) {
     v1 = INFINITY;
     v2 = INFINITY;
     i1 = -1;
-    const long long ka = ia.off + (gq - P.q_off);
-    const uint8_t* da = B.desc + ka * 128;
+    const float qn = (float)B.norms[ka];
+    const unsigned* a4 = (const unsigned*)(B.desc + ka * 128);
     int cx0, cy0;
     cell_xy(ib, cell, cx0, cy0);
     const int* cs = B.cell_start + ib.cell_off + P.b;
@@ -487,10 +478,10 @@ __device__ void query_direct(const MatchBatch& B, const MPair& P, const MImage& 
         for (int cx = max(cx0 - 1, 0); cx <= min(cx0 + 1, ib.gw - 1); cx++) {
             const int cc = cell_id(ib, cx, cy);
             for (int s = cs[cc]; s < cs[cc + 1]; s++) {
-                const int t = B.cell_idx[ib.off + s];
-                const double dx = B.xy[2 * (ib.off + t)] - qpx, dy = B.xy[2 * (ib.off + t) + 1] - qpy;
-                if (!in_candidates(B, gq, dx * dx + dy * dy, t)) continue;
-                const unsigned* a4 = (const unsigned*)da;
+                const long long sb = ib.off + s;
+                const int t = B.s_orig[sb];
+                const double dx = B.s_xy[2 * sb] - qpx, dy = B.s_xy[2 * sb + 1] - qpy;
+                if (!in_window(B, dx * dx + dy * dy, t, td, ti)) continue;
                 const unsigned* b4 = (const unsigned*)(B.desc + (ib.off + t) * 128);
                 unsigned dot = 0;
                 for (int k = 0; k < 32; k++) dot = __builtin_amdgcn_udot4(a4[k], b4[k], dot, false);
@@ -500,46 +491,44 @@ __device__ void query_direct(const MatchBatch& B, const MPair& P, const MImage& 
         }
 }
 
-__global__ __launch_bounds__(64) void k_m_mfma(MatchBatch B) {
-    __shared__ int s_cand[MF_CAP];
+__global__ __launch_bounds__(64) void k_m_mfma(const MatchBatch B) {
+    __shared__ int s_t[MF_CAP];
     __shared__ double s_cx[MF_CAP], s_cy[MF_CAP];
-    __shared__ float s_cn[MF_CAP];
+    __shared__ __attribute__((aligned(16))) int s_cn[MF_CAP];
     __shared__ unsigned s_mark[32];
-    __shared__ int s_cells[MF_MAXC], s_coff[MF_MAXC + 1];
-    __shared__ int s_q[32], s_cell[32];
-    __shared__ double s_px[32], s_py[32];
+    __shared__ int s_cells[MF_MAXC], s_coff[MF_MAXC + 1], s_cst[MF_MAXC];
     __shared__ float s_v1[64], s_v2[64];
     __shared__ int s_i1[64];
     const int lane = threadIdx.x;
-    for (long long ch = blockIdx.x; ch < B.n_chunks; ch += gridDim.x) {
-        const int p = find_chunk_pair(B, ch);
-        const MPair& P = B.pair[p];
+    const int qc = lane & 31;
+    {
+        const MPair& P = B.pair[blockIdx.y];
         const MImage& ia = B.img[P.a];
         const MImage& ib = B.img[P.b];
-        const int k0 = (int)(ch - P.c_off) * 32;
+        if ((int)blockIdx.x * 32 >= ia.n) return;      // block-uniform: this pair has fewer chunks
+        const int kk = (int)blockIdx.x * 32 + qc;
+        const bool valid = kk < ia.n;
+        const long long sa = ia.off + (valid ? kk : 0);
+        const int q = valid ? B.s_orig[sa] : -1;
+        const long long g = P.q_off + (q >= 0 ? q : 0);
+        double qpx = 0.0, qpy = 0.0;
+        const int cell = valid ? predict_cell(B, P, ia, ib, B.s_xy[2 * sa], B.s_xy[2 * sa + 1], qpx, qpy) : -1;
+        const bool act = cell >= 0;
         if (lane < 32) {
-            int g = -1;
-            const int kk = k0 + lane;
-            if (kk < ia.n) {
-                const long long g0 = P.q_off + B.cell_idx[ia.off + kk];
-                if (B.qcell[g0] >= 0) g = (int)g0;
-            }
-            s_q[lane] = g;
-            s_cell[lane] = g >= 0 ? B.qcell[g] : -1;
-            s_px[lane] = g >= 0 ? B.pred[2 * (long long)g] : 0.0;
-            s_py[lane] = g >= 0 ? B.pred[2 * (long long)g + 1] : 0.0;
             s_mark[lane] = 0u;
+            if (valid) B.qcell[g] = cell;
         }
         __syncthreads();
-        const int qc = lane & 31;
-        const int gq = s_q[qc];
-        if (__ballot(gq >= 0) == 0ull) {       // wave-uniform: nothing to match in this chunk
-            __syncthreads();
-            continue;
+        if (__ballot(act) == 0ull) {       // wave-uniform: nothing to match in this chunk
+            if (lane < 32 && valid) B.best[g] = -1;
+            return;
         }
-        // window origin: min cell coordinates of the valid queries, minus one
+        double td = B.r2;
+        int ti = 0x7fffffff;
+        if (act) kth_of(B, g, td, ti);
+        // window origin: min cell coordinates of the active queries, minus one
         int cxq = 1 << 30, cyq = 1 << 30;
-        if (gq >= 0) cell_xy(ib, s_cell[qc], cxq, cyq);
+        if (act) cell_xy(ib, cell, cxq, cyq);
         int cx_lo = cxq, cy_lo = cyq;
         for (int off = 32; off >= 1; off >>= 1) {
             cx_lo = min(cx_lo, __shfl_xor(cx_lo, off));
@@ -548,13 +537,11 @@ __global__ __launch_bounds__(64) void k_m_mfma(MatchBatch B) {
         cx_lo = max(cx_lo - 1, 0);
         cy_lo = max(cy_lo - 1, 0);
         bool overflow = false;
-        if (!P.dense && lane < 32 && gq >= 0) {
-            const int* cs = B.cell_start + ib.cell_off + P.b;
+        if (!P.dense && lane < 32 && act) {
             for (int cy = max(cyq - 1, 0); cy <= min(cyq + 1, ib.gh - 1); cy++)
                 for (int cx = max(cxq - 1, 0); cx <= min(cxq + 1, ib.gw - 1); cx++) {
                     const int bx = cx - cx_lo, by = cy - cy_lo;
-                    const int cc = cell_id(ib, cx, cy);
-                    if (bx < 32 && by < 32 && cs[cc + 1] - cs[cc] <= MF_CAP) atomicOr(&s_mark[by], 1u << bx);
+                    if (bx < 32 && by < 32) atomicOr(&s_mark[by], 1u << bx);
                     else overflow = true;
                 }
         }
@@ -563,6 +550,7 @@ __global__ __launch_bounds__(64) void k_m_mfma(MatchBatch B) {
         // list the marked cells (window-row order) and their candidate offsets
         int total = 0;
         int ncell = 0;
+        const int* cs = B.cell_start + ib.cell_off + P.b;
         if (P.dense) {
             total = ib.n;
         } else if (!any_over) {
@@ -577,14 +565,17 @@ __global__ __launch_bounds__(64) void k_m_mfma(MatchBatch B) {
                     s_cells[k++] = cell_id(ib, cx_lo + bx, cy_lo + lane);
                 }
                 __syncthreads();
-                const int* cs = B.cell_start + ib.cell_off + P.b;
                 int carry = 0;
                 for (int c0 = 0; c0 < ncell; c0 += 64) {
                     const int c = c0 + lane;
-                    const int n = c < ncell ? cs[s_cells[c] + 1] - cs[s_cells[c]] : 0;
+                    const int c_lo = c < ncell ? cs[s_cells[c]] : 0;
+                    const int n = c < ncell ? cs[s_cells[c] + 1] - c_lo : 0;
                     int tot;
                     const int ex = wave_excl_scan(n, lane, tot);
-                    if (c < ncell) s_coff[c] = carry + ex;
+                    if (c < ncell) {
+                        s_coff[c] = carry + ex;
+                        s_cst[c] = c_lo;
+                    }
                     carry += tot;
                 }
                 if (lane == 0) s_coff[ncell] = carry;
@@ -592,74 +583,111 @@ __global__ __launch_bounds__(64) void k_m_mfma(MatchBatch B) {
                 __syncthreads();
             }
         }
-        // state of this lane's query column over the candidate passes
+        // this lane's query column over the candidate passes
         float v1 = INFINITY, v2 = INFINITY;
         int i1 = -1;
-        bf16x8_t qop[8];
-        float qn = 0.f;
-        const double qpx = gq >= 0 ? s_px[qc] : 0.0, qpy = gq >= 0 ? s_py[qc] : 0.0;
-        if (gq >= 0) {
-            const long long ka = ia.off + (gq - P.q_off);
-            const uint2* qa = (const uint2*)(B.desc + ka * 128);
+        const long long ka = ia.off + (q >= 0 ? q : 0);
+        i32x4_t qop[4];
+        const int qn = act ? (int)B.snorms[ka] : 0;
+        {
+            const uint4* qa = (const uint4*)(B.desc + ka * 128);
 #pragma unroll
-            for (int s = 0; s < 8; s++) qop[s] = u8x8_to_bf16(qa[2 * s + (lane >> 5)]);
-            qn = (float)B.norms[ka];
-        } else {
-#pragma unroll
-            for (int s = 0; s < 8; s++) qop[s] = u8x8_to_bf16(make_uint2(0, 0));
+            for (int s = 0; s < 4; s++) qop[s] = s8x16(act ? qa[2 * s + (lane >> 5)] : make_uint4(0, 0, 0, 0));
         }
         if (!any_over) {
-            const int* cs = B.cell_start + ib.cell_off + P.b;
             for (int base = 0; base < total; base += MF_CAP) {
                 const int np = min(MF_CAP, total - base);
-                for (int j = lane; j < np; j += 64) {
-                    int t;
-                    if (P.dense) {
-                        t = base + j;
-                    } else {
-                        // listed cell holding candidate position base + j: the last cell whose
-                        // offset is <= pos (binary search over s_coff[0..ncell))
-                        const int pos = base + j;
-                        int lo = 0, hi = ncell - 1;
-                        while (lo < hi) {
-                            const int mid = (lo + hi + 1) >> 1;
-                            if (s_coff[mid] <= pos) lo = mid;
-                            else hi = mid - 1;
+                // stage the pass's candidates: original index and position (up to MF_CAP / 64
+                // per lane, all loads issued before any LDS store)
+                {
+                    long long sbv[MF_CAP / 64];
+#pragma unroll
+                    for (int u = 0; u < MF_CAP / 64; u++) {
+                        const int j = lane + 64 * u;
+                        sbv[u] = -1;
+                        if (j < np && !P.dense) {
+                            // listed cell holding candidate position base + j: the last cell
+                            // whose offset is <= pos (binary search over s_coff[0..ncell))
+                            const int pos = base + j;
+                            int lo = 0, hi = ncell - 1;
+                            while (lo < hi) {
+                                const int mid = (lo + hi + 1) >> 1;
+                                if (s_coff[mid] <= pos) lo = mid;
+                                else hi = mid - 1;
+                            }
+                            sbv[u] = ib.off + s_cst[lo] + (pos - s_coff[lo]);
                         }
-                        t = B.cell_idx[ib.off + cs[s_cells[lo]] + (pos - s_coff[lo])];
-                        s_cx[j] = B.xy[2 * (ib.off + t)];
-                        s_cy[j] = B.xy[2 * (ib.off + t) + 1];
                     }
-                    s_cand[j] = t;
-                    s_cn[j] = (float)B.norms[ib.off + t];
+                    double cxv[MF_CAP / 64], cyv[MF_CAP / 64];
+                    int tv[MF_CAP / 64];
+#pragma unroll
+                    for (int u = 0; u < MF_CAP / 64; u++) {
+                        const long long sb = sbv[u];
+                        cxv[u] = sb >= 0 ? B.s_xy[2 * sb] : 0.0;
+                        cyv[u] = sb >= 0 ? B.s_xy[2 * sb + 1] : 0.0;
+                        tv[u] = sb >= 0 ? B.s_orig[sb] : base + lane + 64 * u;    // dense: identity
+                    }
+#pragma unroll
+                    for (int u = 0; u < MF_CAP / 64; u++) {
+                        const int j = lane + 64 * u;
+                        if (j < np) {
+                            s_t[j] = tv[u];
+                            s_cx[j] = cxv[u];
+                            s_cy[j] = cyv[u];
+                        }
+                    }
                 }
                 __syncthreads();
-                for (int t0 = 0; t0 < np; t0 += 32) {
+                // tiles of 32 candidates; the next tile's bytes are loaded while this one is
+                // contracted and reduced
+                uint4 cur[4], nxt[4];
+                auto load_tile = [&](int t0, uint4* dst) {
                     const int tr = t0 + (lane & 31);
-                    const int tc = tr < np ? s_cand[tr] : -1;
-                    const uint2* cb = (const uint2*)(B.desc + (ib.off + (tc >= 0 ? tc : 0)) * 128);
-                    f32x16_t acc = {};
+                    const int tc = tr < np ? s_t[tr] : -1;
+                    const uint4* cb = (const uint4*)(B.desc + (ib.off + (tc >= 0 ? tc : 0)) * 128);
 #pragma unroll
-                    for (int s = 0; s < 8; s++) {
-                        const bf16x8_t a = u8x8_to_bf16(tc >= 0 ? cb[2 * s + (lane >> 5)] : make_uint2(0, 0));
-                        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, qop[s], acc, 0, 0, 0);
+                    for (int s = 0; s < 4; s++) dst[s] = tc >= 0 ? cb[2 * s + (lane >> 5)] : make_uint4(0, 0, 0, 0);
+                };
+                load_tile(0, cur);
+                for (int t0 = 0; t0 < np; t0 += 32) {
+                    if (t0 + 32 < np) load_tile(t0 + 32, nxt);
+                    i32x16_t acc = {};
+                    int nrm = 0;     // |b'|^2 of this lane's half row, completed with lane ^ 32
+#pragma unroll
+                    for (int s = 0; s < 4; s++) {
+                        const i32x4_t a = s8x16(cur[s]);
+                        acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, qop[s], acc, 0, 0, 0);
+#pragma unroll
+                        for (int w = 0; w < 4; w++) nrm = __builtin_amdgcn_sdot4(a[w], a[w], nrm, false);
                     }
-                    if (gq >= 0) {
+                    nrm += __shfl_xor(nrm, 32);
+                    if (lane < 32) s_cn[t0 + lane] = nrm;
+                    __syncthreads();
+                    if (act) {
 #pragma unroll
-                        for (int i = 0; i < 4; i++)
+                        for (int i = 0; i < 4; i++) {
+                            const int4 cn4 = *(const int4*)&s_cn[t0 + i * 8 + (lane >> 5) * 4];
+                            const int cnv[4] = {cn4.x, cn4.y, cn4.z, cn4.w};
 #pragma unroll
                             for (int j = 0; j < 4; j++) {
                                 const int r = t0 + i * 8 + (lane >> 5) * 4 + j;
                                 if (r >= np) continue;
-                                const int t = s_cand[r];
+                                // exact: |a'|^2 + |b'|^2 - 2 a'.b' = sum (a - b)^2 < 2^24
+                                const float d = (float)(qn + cnv[j] - 2 * acc[i * 4 + j]);
+                                // a distance above the current second best cannot change the
+                                // top-2, whatever its window membership
+                                if (d > v2) continue;
+                                const int t = s_t[r];
                                 if (!P.dense) {
                                     const double dx = s_cx[r] - qpx, dy = s_cy[r] - qpy;
-                                    if (!in_candidates(B, gq, dx * dx + dy * dy, t)) continue;
+                                    if (!in_window(B, dx * dx + dy * dy, t, td, ti)) continue;
                                 }
-                                const float d = fmaxf((qn + s_cn[r]) - 2.0f * acc[i * 4 + j], 0.0f);
                                 top2_push(d, t, v1, i1, v2);
                             }
+                        }
                     }
+#pragma unroll
+                    for (int s = 0; s < 4; s++) cur[s] = nxt[s];
                 }
                 __syncthreads();
             }
@@ -669,19 +697,74 @@ __global__ __launch_bounds__(64) void k_m_mfma(MatchBatch B) {
         s_v2[lane] = v2;
         s_i1[lane] = i1;
         __syncthreads();
-        if (lane < 32 && gq >= 0) {
-            if (any_over) {
-                query_direct(B, P, ia, ib, gq, s_cell[lane], qpx, qpy, qn, v1, i1, v2);
+        if (lane < 32 && valid) {
+            if (!act) {
+                B.best[g] = -1;
             } else {
-                const float w1 = s_v1[lane + 32], w2 = s_v2[lane + 32];
-                const int j1 = s_i1[lane + 32];
-                if (j1 >= 0) top2_push(w1, j1, v1, i1, v2);
-                v2 = fminf(v2, w2);
+                if (any_over) {
+                    query_direct(B, P, ib, ka, cell, qpx, qpy, td, ti, v1, i1, v2);
+                } else {
+                    const float w1 = s_v1[lane + 32], w2 = s_v2[lane + 32];
+                    const int j1 = s_i1[lane + 32];
+                    if (j1 >= 0) top2_push(w1, j1, v1, i1, v2);
+                    v2 = fminf(v2, w2);
+                }
+                accept_query(B, g, P, q, v1, i1, v2);
             }
-            accept_query(B, gq, P, v1, i1, v2);
         }
         __syncthreads();
     }
+}
+
+// ------------------------------------------------------------------------------------------
+// Compact: one block per pair, winners in target order -> (q, t) lists
+
+constexpr int FSX_MATCH_INFO_LEN = 64;   // launch_info entries (header + the batch record)
+constexpr int CP_THREADS = 1024;
+constexpr int CP_TPT = 8;          // targets per thread
+
+__global__ __launch_bounds__(CP_THREADS) void k_m_compact(const MatchBatch B) {
+    __shared__ int s_w[CP_THREADS / 64];
+    const MPair& P = B.pair[blockIdx.x];
+    const int nb = B.img[P.b].n;
+    const unsigned long long* tk = B.tkey + P.t_off;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    int carry = 0;
+    for (int base0 = 0; base0 < nb; base0 += CP_THREADS * CP_TPT) {
+        // thread tid owns targets base0 + tid * CP_TPT + [0, CP_TPT): all keys loaded at once
+        unsigned long long key[CP_TPT];
+        int cnt = 0;
+#pragma unroll
+        for (int r = 0; r < CP_TPT; r++) {
+            const int t = base0 + tid * CP_TPT + r;
+            key[r] = t < nb ? tk[t] : ~0ull;
+            cnt += key[r] != ~0ull ? 1 : 0;
+        }
+        int x = cnt;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(x, off);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) s_w[wv] = x;
+        __syncthreads();
+        int pre = 0, tot = 0;
+        for (int w2 = 0; w2 < CP_THREADS / 64; w2++) {
+            pre += w2 < wv ? s_w[w2] : 0;
+            tot += s_w[w2];
+        }
+        long long o = P.o_off + carry + pre + x - cnt;
+#pragma unroll
+        for (int r = 0; r < CP_TPT; r++) {
+            if (key[r] != ~0ull) {
+                B.out_q[o] = (int)(unsigned)(key[r] & 0xffffffffull);
+                B.out_t[o] = base0 + tid * CP_TPT + r;
+                o++;
+            }
+        }
+        carry += tot;
+        __syncthreads();
+    }
+    if (tid == 0) B.out_n[blockIdx.x] = carry;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -693,8 +776,8 @@ struct MatchPlanHost {
     std::vector<MImage> img;
     std::vector<MPair> pair;
     long long n_kp = 0, n_query = 0, n_cells = 0, n_targets = 0, n_out = 0, n_chunks = 0;
-    size_t off_img, off_pair, off_norms, off_cstart, off_ccur, off_cidx, off_pred, off_qcell, off_best, off_b1,
-        off_kth, off_tkey, off_tq, off_cnt, off_outq, off_outt, off_outn, off_batch, total;
+    size_t off_batch, off_img, off_pair, off_norms, off_snorms, off_sxy, off_sorig, off_cstart, off_ccur, off_best, off_b1,
+        off_qcell, off_kth, off_tkey, off_cnt, off_outq, off_outt, off_outn, total;
 };
 
 static int match_plan(int n_img, const int* n_kp, const int* W, const int* H, int n_pair, const int* pa,
@@ -710,8 +793,10 @@ static int match_plan(int n_img, const int* n_kp, const int* W, const int* H, in
         m.n = n_kp[i];
         m.W = W[i];
         m.H = H[i];
-        m.gw = (int)ceil((W[i] + 4.0 * radius) / radius);
-        m.gh = (int)ceil((H[i] + 4.0 * radius) / radius);
+        const double gw = ceil((W[i] + 4.0 * radius) / radius), gh = ceil((H[i] + 4.0 * radius) / radius);
+        FSX_REQUIRE(gw * gh < 1e8, "image %d: radius %g gives a grid of %.0f cells", i, radius, gw * gh);
+        m.gw = (int)gw;
+        m.gh = (int)gh;
         m.tw4 = (m.gw + 3) / 4;
         m.ncell = m.tw4 * ((m.gh + 3) / 4) * 16;
         m.cell_off = coff;
@@ -743,21 +828,22 @@ static int match_plan(int n_img, const int* n_kp, const int* W, const int* H, in
     M.n_targets = to;
     M.n_out = oo;
     M.n_chunks = chunks;
+    const size_t nk = (size_t)std::max(M.n_kp, 1LL), nq = (size_t)std::max(qo, 1LL);
     size_t o = 0;
     M.off_batch = o; o = a256(o + sizeof(MatchBatch));
     M.off_img = o; o = a256(o + sizeof(MImage) * n_img);
     M.off_pair = o; o = a256(o + sizeof(MPair) * n_pair);
-    M.off_norms = o; o = a256(o + 4 * (size_t)std::max(M.n_kp, 1LL));
+    M.off_norms = o; o = a256(o + 4 * nk);
+    M.off_snorms = o; o = a256(o + 4 * nk);
+    M.off_sxy = o; o = a256(o + 16 * nk);
+    M.off_sorig = o; o = a256(o + 4 * nk);
     M.off_cstart = o; o = a256(o + 4 * (size_t)(M.n_cells + n_img));
     M.off_ccur = o; o = a256(o + 4 * (size_t)M.n_cells);
-    M.off_cidx = o; o = a256(o + 4 * (size_t)std::max(M.n_kp, 1LL));
-    M.off_pred = o; o = a256(o + 16 * (size_t)std::max(qo, 1LL));
-    M.off_qcell = o; o = a256(o + 4 * (size_t)std::max(qo, 1LL));
-    M.off_best = o; o = a256(o + 4 * (size_t)std::max(qo, 1LL));
-    M.off_b1 = o; o = a256(o + 8 * (size_t)std::max(qo, 1LL));
-    M.off_kth = o; o = a256(o + 16 * (size_t)std::max(qo, 1LL));
+    M.off_best = o; o = a256(o + 4 * nq);
+    M.off_b1 = o; o = a256(o + 8 * nq);
+    M.off_qcell = o; o = a256(o + 4 * nq);
+    M.off_kth = o; o = a256(o + 16 * nq);
     M.off_tkey = o; o = a256(o + 8 * (size_t)std::max(to, 1LL));
-    M.off_tq = o; o = a256(o + 4 * (size_t)std::max(to, 1LL));
     M.off_cnt = o; o = a256(o + 16 * 4);
     M.off_outq = o; o = a256(o + 4 * (size_t)std::max(oo, 1LL));
     M.off_outt = o; o = a256(o + 4 * (size_t)std::max(oo, 1LL));
@@ -780,23 +866,23 @@ static MatchBatch batch_ptrs(const MatchPlanHost& M, char* ws, const uint8_t* de
     b.n_kp = M.n_kp;
     b.n_cells = M.n_cells;
     b.n_targets = M.n_targets;
+    b.n_chunks = M.n_chunks;
     b.radius = radius;
     b.r2 = radius * radius;
     b.ratio = ratio;
     b.kc = kc;
     b.norms = (unsigned*)(ws + M.off_norms);
+    b.snorms = (unsigned*)(ws + M.off_snorms);
+    b.s_xy = (double*)(ws + M.off_sxy);
+    b.s_orig = (int*)(ws + M.off_sorig);
     b.cell_start = (int*)(ws + M.off_cstart);
     b.cell_cur = (int*)(ws + M.off_ccur);
-    b.cell_idx = (int*)(ws + M.off_cidx);
-    b.pred = (double*)(ws + M.off_pred);
-    b.qcell = (int*)(ws + M.off_qcell);
     b.best = (int*)(ws + M.off_best);
     b.b1 = (double*)(ws + M.off_b1);
+    b.qcell = (int*)(ws + M.off_qcell);
     b.kth = (double*)(ws + M.off_kth);
     b.tkey = (unsigned long long*)(ws + M.off_tkey);
-    b.tq = (int*)(ws + M.off_tq);
     b.counters = (int*)(ws + M.off_cnt);
-    b.n_chunks = M.n_chunks;
     b.out_q = (int*)(ws + M.off_outq);
     b.out_t = (int*)(ws + M.off_outt);
     b.out_n = (int*)(ws + M.off_outn);
@@ -822,17 +908,18 @@ int fsx_match_workspace_bytes(int n_img, const int* n_kp, const int* W, const in
     return FSX_OK;
 }
 
-int fsx_match_batch(int n_img, const int* n_kp, const int* W, const int* H, const uint8_t* desc_dev,
-                    const double* xy_dev, int n_pair, const int* pa, const int* pb, const int* dense,
-                    const double* Hm, double radius, int kc, double ratio, int form, void* ws, size_t ws_bytes,
-                    long long* out_offsets, void* stage_events_v, void* stream) {
-    FSX_REQUIRE(desc_dev && xy_dev && ws && pa && pb, "null pointer argument");
-    FSX_REQUIRE(form == 0 || form == 1, "form must be 0 (plain FP32) or 1 (MFMA)");
-    FSX_REQUIRE(kc >= 0 && kc <= 64, "kc must be in [0, 64]");
+int fsx_match_prepare(int n_img, const int* n_kp, const int* W, const int* H, const uint8_t* desc_dev,
+                      const double* xy_dev, int n_pair, const int* pa, const int* pb, const int* dense,
+                      const double* Hm, double radius, int kc, double ratio, void* ws, size_t ws_bytes,
+                      long long* out_offsets, long long* launch_info, void* stream) {
+    FSX_REQUIRE(desc_dev && xy_dev && ws && pa && pb && launch_info, "null pointer argument");
+    FSX_REQUIRE(kc >= 0 && kc <= KMAX, "kc must be in [0, %d]", KMAX);
+    static_assert(sizeof(MatchBatch) <= 8 * (FSX_MATCH_INFO_LEN - 8), "launch_info too small for MatchBatch");
     MatchPlanHost M;
     int rc = match_plan(n_img, n_kp, W, H, n_pair, pa, pb, dense, Hm, radius, M);
     if (rc) return rc;
     FSX_REQUIRE(ws_bytes >= M.total, "match workspace too small: %zu < %zu", ws_bytes, M.total);
+    FSX_REQUIRE(n_img <= (1 << 20) && n_pair <= 65535, "too many images or pairs in one batch");
     hipStream_t st = (hipStream_t)stream;
     char* w = (char*)ws;
     MatchBatch b = batch_ptrs(M, w, desc_dev, xy_dev, radius, kc, ratio);
@@ -842,31 +929,65 @@ int fsx_match_batch(int n_img, const int* n_kp, const int* W, const int* H, cons
     if (rc) return rc;
     if (out_offsets)
         for (int p = 0; p < n_pair; p++) out_offsets[p] = M.pair[p].o_off;
+    int max_na = 0;
+    for (int p = 0; p < n_pair; p++) max_na = std::max(max_na, M.img[M.pair[p].a].n);
+    const long long extra =
+        std::min<long long>(std::max<long long>(1, (std::max(M.n_targets, M.n_kp) + SU_THREADS - 1) / SU_THREADS), 4096);
+    memset(launch_info, 0, 8 * FSX_MATCH_INFO_LEN);
+    launch_info[0] = 0x6673786d;     // tag checked by fsx_match_launch
+    launch_info[1] = n_img;
+    launch_info[2] = n_pair;
+    launch_info[3] = M.n_query;
+    launch_info[4] = max_na;
+    launch_info[5] = n_img + extra;
+    launch_info[6] = kc;
+    launch_info[7] = (long long)(intptr_t)ws;
+    memcpy(launch_info + 8, &b, sizeof(b));
+    return FSX_OK;
+}
+
+int fsx_match_launch(void* ws, const long long* info, int form, void* stage_events_v, void* stream) {
+    FSX_REQUIRE(ws && info, "null pointer argument");
+    FSX_REQUIRE(info[0] == 0x6673786d, "launch_info was not filled by fsx_match_prepare");
+    FSX_REQUIRE(form == 0 || form == 1, "form must be 0 (plain FP32) or 1 (MFMA)");
+    MatchBatch b;
+    memcpy(&b, info + 8, sizeof(b));
+    FSX_REQUIRE(info[7] == (long long)(intptr_t)ws, "launch_info belongs to another workspace");
+    hipStream_t st = (hipStream_t)stream;
+    const long long nq = info[3];
+    const int n_pair = (int)info[2], max_na = (int)info[4];
     hipEvent_t* ev = (hipEvent_t*)stage_events_v;
     if (ev) FSX_HIP(hipEventRecord(ev[0], st));
-    const long long nq = M.n_query;
-    hipLaunchKernelGGL(k_m_reset, dim3(1024), dim3(256), 0, st, b);
-    if (M.n_kp > 0) hipLaunchKernelGGL(k_m_norms_count, dim3(blocks_for(M.n_kp)), dim3(256), 0, st, b);
-    hipLaunchKernelGGL(k_m_grid_scan, dim3(n_img), dim3(256), 0, st, b);
-    if (M.n_kp > 0) hipLaunchKernelGGL(k_m_grid_fill, dim3(blocks_for(M.n_kp)), dim3(256), 0, st, b);
-    if (nq > 0) hipLaunchKernelGGL(k_m_predict, dim3(blocks_for(nq)), dim3(256), 0, st, b);
-    if (nq > 0 && kc > 0) hipLaunchKernelGGL(k_m_knn, dim3(blocks_for(nq)), dim3(256), 0, st, b);
+    hipLaunchKernelGGL(k_m_setup, dim3((unsigned)info[5]), dim3(SU_THREADS), 0, st, b);
+    FSX_LAUNCH_CHECK();
+    if (nq > 0 && info[6] > 0)
+        hipLaunchKernelGGL(k_m_knn, dim3((unsigned)((max_na + 255) / 256), n_pair), dim3(256), 0, st, b);
     FSX_LAUNCH_CHECK();
     if (ev) FSX_HIP(hipEventRecord(ev[1], st));
     if (nq > 0) {
-        if (form == 0) {
-            hipLaunchKernelGGL(k_m_plain, dim3(blocks_for(nq)), dim3(256), 0, st, b);
-        } else if (M.n_chunks > 0) {
-            hipLaunchKernelGGL(k_m_mfma, dim3((unsigned)std::min<long long>(M.n_chunks, 1 << 20)), dim3(64), 0, st, b);
-        }
+        if (form == 0)
+            hipLaunchKernelGGL(k_m_plain, dim3((unsigned)((max_na + 255) / 256), n_pair), dim3(256), 0, st, b);
+        else
+            hipLaunchKernelGGL(k_m_mfma, dim3((unsigned)((max_na + 31) / 32), n_pair), dim3(64), 0, st, b);
         FSX_LAUNCH_CHECK();
     }
     if (ev) FSX_HIP(hipEventRecord(ev[2], st));
-    if (nq > 0) hipLaunchKernelGGL(k_m_unique2, dim3(blocks_for(nq)), dim3(256), 0, st, b);
-    hipLaunchKernelGGL(k_m_compact, dim3(n_pair), dim3(256), 0, st, b);
+    hipLaunchKernelGGL(k_m_compact, dim3((unsigned)n_pair), dim3(CP_THREADS), 0, st, b);
     FSX_LAUNCH_CHECK();
     if (ev) FSX_HIP(hipEventRecord(ev[3], st));
     return FSX_OK;
+}
+
+int fsx_match_batch(int n_img, const int* n_kp, const int* W, const int* H, const uint8_t* desc_dev,
+                    const double* xy_dev, int n_pair, const int* pa, const int* pb, const int* dense,
+                    const double* Hm, double radius, int kc, double ratio, int form, void* ws, size_t ws_bytes,
+                    long long* out_offsets, void* stage_events_v, void* stream) {
+    FSX_REQUIRE(form == 0 || form == 1, "form must be 0 (plain FP32) or 1 (MFMA)");
+    long long info[FSX_MATCH_INFO_LEN];
+    int rc = fsx_match_prepare(n_img, n_kp, W, H, desc_dev, xy_dev, n_pair, pa, pb, dense, Hm, radius, kc, ratio, ws,
+                               ws_bytes, out_offsets, info, stream);
+    if (rc) return rc;
+    return fsx_match_launch(ws, info, form, stage_events_v, stream);
 }
 
 // Copies the results of the last fsx_match_batch on `ws` to host arrays (synchronises):
@@ -906,6 +1027,40 @@ __global__ void k_mfma_layout_probe(float* out) {
     for (int r = 0; r < 16; r++) out[lane * 16 + r] = acc[r];
 }
 
+// The INT8 form's assumption: for v_mfma_i32_32x32x32_i8, lane l supplies row l % 32 of A and
+// column l % 32 of B, both with the same 16 k positions for its half h = l / 32, and the
+// result layout is the one of the 32x32 BF16 instruction.  Operands: A byte e of lane l =
+// ((l%32)*7 + h*3 + e*5) % 11 - 5, B byte = ((l%32)*3 + h*5 + e*7) % 13 - 6; out: 64 x 16 int32.
+__global__ void k_mfma_i8_layout_probe(int* out) {
+    const int lane = threadIdx.x, r = lane & 31, h = lane >> 5;
+    i32x4_t a, b;
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+        unsigned wa = 0, wb = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int e = 4 * w + k;
+            const int av = (r * 7 + h * 3 + e * 5) % 11 - 5;
+            const int bv = (r * 3 + h * 5 + e * 7) % 13 - 6;
+            wa |= ((unsigned)av & 0xffu) << (8 * k);
+            wb |= ((unsigned)bv & 0xffu) << (8 * k);
+        }
+        a[w] = (int)wa;
+        b[w] = (int)wb;
+    }
+    i32x16_t acc = {};
+    acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, acc, 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 16; i++) out[lane * 16 + i] = acc[i];
+}
+
+int fsx_mfma_i8_layout_probe(int* out_dev, void* stream) {
+    FSX_REQUIRE(out_dev, "null pointer argument");
+    hipLaunchKernelGGL(k_mfma_i8_layout_probe, dim3(1), dim3(64), 0, (hipStream_t)stream, out_dev);
+    FSX_LAUNCH_CHECK();
+    return FSX_OK;
+}
+
 int fsx_mfma_layout_probe(float* out_dev, void* stream) {
     FSX_REQUIRE(out_dev, "null pointer argument");
     hipLaunchKernelGGL(k_mfma_layout_probe, dim3(1), dim3(64), 0, (hipStream_t)stream, out_dev);
@@ -929,21 +1084,6 @@ int fsx_match_query_state(int n_img, const int* n_kp, const int* W, const int* H
         if (qcell) FSX_HIP(hipMemcpyAsync(qcell, w + M.off_qcell, 4 * (size_t)M.n_query, hipMemcpyDeviceToHost, st));
     }
     FSX_HIP(hipStreamSynchronize(st));
-    return FSX_OK;
-}
-
-// Workspace layout (byte offsets) for tests and diagnostics: [0] norms [1] cell_start [2] cell_cur
-// [3] cell_idx [4] pred [5] qcell [6] best [7] b1 [8] kth [9] tkey [10] tq [11] counters
-// [12] out_q [13] out_t [14] out_n [15] n_chunks [16] total
-int fsx_match_layout(int n_img, const int* n_kp, const int* W, const int* H, int n_pair, const int* pa,
-                     const int* pb, double radius, long long* offs) {
-    MatchPlanHost M;
-    int rc = match_plan(n_img, n_kp, W, H, n_pair, pa, pb, nullptr, nullptr, radius, M);
-    if (rc) return rc;
-    const size_t v[17] = {M.off_norms, M.off_cstart, M.off_ccur, M.off_cidx, M.off_pred, M.off_qcell, M.off_best,
-                          M.off_b1, M.off_kth, M.off_tkey, M.off_tq, M.off_cnt, M.off_outq, M.off_outt, M.off_outn,
-                          (size_t)M.n_chunks, M.total};
-    for (int i = 0; i < 17; i++) offs[i] = (long long)v[i];
     return FSX_OK;
 }
 

@@ -14,7 +14,8 @@ Drop-ins with the reference's signatures and return types:
 ``PairMatcher`` runs many pairs (the pipeline's match_pair pool, pipeline.py:535-548) in one
 batched launch sequence on device-resident descriptors.  Two forms of the contraction:
 ``form="plain"`` (FP32 gather + VALU dot products, the baseline) and ``form="mfma"``
-(spatially blocked BF16 MFMA with the prior-window mask and top-2 fused).  Both return the
+(spatially blocked INT8 MFMA on the bytes shifted by -128, with the prior-window mask and
+top-2 fused).  Both return the
 FP64-oracle match set for uint8 RootSIFT descriptors (all distances are exact integers).
 """
 from __future__ import annotations
@@ -205,9 +206,27 @@ class PairMatcher:
         self.ws_bytes = nb.value
         self.capacity = cap.value
         self._offsets = (C.c_longlong * npair)()
+        self._info = (C.c_longlong * 64)()
+        self._graphs = {}
+        P = C.c_void_p
+        # plan + table upload once; every run() only enqueues kernels
+        _lib.call("fsx_match_prepare", self.n_img, self._nk, self._W, self._H, P(self.desc.data_ptr()),
+                  P(self.xy.data_ptr()), self.n_pair, self._pa, self._pb, self._dense,
+                  self._hm.ctypes.data_as(C.POINTER(C.c_double)), self.radius, self.kc, self.ratio,
+                  P(self.ws.data_ptr()), self.ws_bytes, self._offsets, self._info,
+                  P(_lib.stream_handle()))
 
-    def run(self, form="mfma", events=None, stream=None):
-        """Enqueue matching of every pair (asynchronous)."""
+    def run(self, form="mfma", events=None, stream=None, graph=False):
+        """Enqueue matching of every pair (asynchronous).  ``graph=True`` replays a HIP graph of
+        the launch sequence (captured on first use per form); events are then not recorded."""
+        if graph:
+            g = self._graphs.get(form) or self.capture(form)
+            if stream is not None:
+                with self.torch.cuda.stream(stream):
+                    g.replay()
+            else:
+                g.replay()
+            return
         P = C.c_void_p
         st = _lib.stream_handle(stream)
         ev = None
@@ -216,10 +235,20 @@ class PairMatcher:
                 if not e.cuda_event:
                     e.record()
             ev = (P * 4)(*[e.cuda_event for e in events])
-        _lib.call("fsx_match_batch", self.n_img, self._nk, self._W, self._H, P(self.desc.data_ptr()),
-                  P(self.xy.data_ptr()), self.n_pair, self._pa, self._pb, self._dense,
-                  self._hm.ctypes.data_as(C.POINTER(C.c_double)), self.radius, self.kc, self.ratio,
-                  FORMS[form], P(self.ws.data_ptr()), self.ws_bytes, self._offsets, ev, P(st))
+        _lib.call("fsx_match_launch", P(self.ws.data_ptr()), self._info, FORMS[form], ev, P(st))
+
+    def capture(self, form="mfma"):
+        """Capture the launch sequence of `form` into a HIP graph (torch.cuda.CUDAGraph)."""
+        torch = self.torch
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=side):
+            _lib.call("fsx_match_launch", C.c_void_p(self.ws.data_ptr()), self._info, FORMS[form], None,
+                      C.c_void_p(_lib.stream_handle()))
+        torch.cuda.current_stream().wait_stream(side)
+        self._graphs[form] = g
+        return g
 
     def results(self, stream=None):
         """List of (q int64, t int64) per pair, sorted by t (synchronises)."""
@@ -277,6 +306,14 @@ def _guided(A, B, Hm, radius, kc, ratio, form="mfma"):
                     [(0, 1)], [np.asarray(Hm, np.float64)], radius=radius, kc=kc, ratio=ratio)
     m.run(form)
     return m.results()[0]
+
+
+def mfma_i8_layout_probe():
+    """Run the 32x32x32 INT8 MFMA layout probe; returns (64 lanes, 16) int32."""
+    import torch
+    out = torch.zeros(64 * 16, dtype=torch.int32, device="cuda")
+    _lib.call("fsx_mfma_i8_layout_probe", C.c_void_p(out.data_ptr()), C.c_void_p(_lib.stream_handle()))
+    return out.view(64, 16).cpu().numpy()
 
 
 def mfma_layout_probe():

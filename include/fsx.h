@@ -120,11 +120,25 @@ int fsx_match_workspace_bytes(int n_img, const int *n_kp, const int *W, const in
  * form 1 = spatially blocked BF16 MFMA with fused window mask and top-2.  Both return the
  * FP64-oracle set (uint8 distances are exact integers).  out_offsets (host, n_pair): output
  * slot of each pair.  stage_events (may be NULL): 4 caller hipEvent_t recorded around
- * setup / contraction / unique.  Asynchronous; read with fsx_match_results. */
+ * setup (grid bucketing) / contraction / compaction.  Asynchronous; read with
+ * fsx_match_results.  Calls on one workspace must not overlap. */
 int fsx_match_batch(int n_img, const int *n_kp, const int *W, const int *H, const uint8_t *desc_dev,
                     const double *xy_dev, int n_pair, const int *pa, const int *pb, const int *dense,
                     const double *Hm, double radius, int kc, double ratio, int form, void *ws_dev, size_t ws_bytes,
                     long long *out_offsets, void *stage_events, void *stream);
+
+/* fsx_match_batch split in two for repeated runs of the same batch (and for graph capture):
+ * fsx_match_prepare plans the batch and uploads its tables into the workspace (asynchronous
+ * on `stream`; the device pointers and parameters are baked in), filling launch_info[64]
+ * (host memory, opaque);
+ * fsx_match_launch then only enqueues the kernels (no host->device copies, no allocation),
+ * so it can be captured into a hipGraph and replayed.  No reference counterpart (the
+ * reference re-plans per pair in a process pool). */
+int fsx_match_prepare(int n_img, const int *n_kp, const int *W, const int *H, const uint8_t *desc_dev,
+                      const double *xy_dev, int n_pair, const int *pa, const int *pb, const int *dense,
+                      const double *Hm, double radius, int kc, double ratio, void *ws_dev, size_t ws_bytes,
+                      long long *out_offsets, long long *launch_info, void *stream);
+int fsx_match_launch(void *ws_dev, const long long *launch_info, int form, void *stage_events, void *stream);
 
 /* Copies the last batch's results to the host (synchronises): counts[n_pair], q and t at the
  * pair offsets (int32, as match_pair returns them, matching.py:249), flags[4] (flags[2] != 0:
@@ -140,14 +154,13 @@ int fsx_match_query_state(int n_img, const int *n_kp, const int *W, const int *H
                           const int *pb, double radius, const void *ws_dev, int *best, double *b1, int *qcell,
                           void *stream);
 
-/* Byte offsets of the matcher workspace regions (23 values, order documented in match.hip), for
- * tests and diagnostics (no reference counterpart). */
-int fsx_match_layout(int n_img, const int *n_kp, const int *W, const int *H, int n_pair, const int *pa,
-                     const int *pb, double radius, long long *offs);
-
 /* Writes the 64 x 16 accumulator of one 32x32x16 BF16 MFMA on fixed small integer operands, to
  * check the operand/result lane layout the matcher assumes (no reference counterpart). */
 int fsx_mfma_layout_probe(float *out_dev, void *stream);
+
+/* Writes the 64 x 16 int32 accumulator of one 32x32x32 INT8 MFMA on fixed small operands, to
+ * check the lane layout the matcher's MFMA form assumes (no reference counterpart). */
+int fsx_mfma_i8_layout_probe(int *out_dev, void *stream);
 
 /* ---------------------------------------------------------------------------------------
  * Keypoint geometry around the matcher

@@ -27,6 +27,19 @@ def test_mfma_layout(gpu):
                 assert out[lane, i * 4 + j] == Cm[i * 8 + (lane >> 5) * 4 + j, lane & 31]
 
 
+def test_mfma_i8_layout(gpu):
+    from fieldstitch_amd import matching
+    out = matching.mfma_i8_layout_probe()
+    # C[i][j] = sum over h, e of A(i, h, e) * B(j, h, e): both operands use the same k positions
+    f = np.array([[[(i * 7 + h * 3 + e * 5) % 11 - 5 for e in range(16)] for h in range(2)] for i in range(32)])
+    g = np.array([[[(j * 3 + h * 5 + e * 7) % 13 - 6 for e in range(16)] for h in range(2)] for j in range(32)])
+    Cm = np.einsum("ihe,jhe->ij", f, g)
+    for lane in range(64):
+        for i in range(4):
+            for j in range(4):
+                assert out[lane, i * 4 + j] == Cm[i * 8 + (lane >> 5) * 4 + j, lane & 31]
+
+
 def rootsift_like(rng, n):
     d = rng.gamma(0.6, 1.0, (n, 128))
     d /= d.sum(1, keepdims=True)
@@ -138,3 +151,29 @@ def test_forms_agree_on_flight(gpu):
     qe, te = oracle_guided(da, xy_a, db, xy_b, Hm, wh, 60.0, 0, 0.8)
     np.testing.assert_array_equal(r1[0][0], qe)
     np.testing.assert_array_equal(r1[0][1], te)
+
+
+@pytest.mark.parametrize("form", ["plain", "mfma"])
+def test_graph_replays_equal_eager(gpu, form):
+    """The launch sequence captured in a HIP graph gives the eager result on every replay, with
+    eager work and synchronisations in between."""
+    import torch
+    from fieldstitch_amd.matching import PairMatcher
+    rng = np.random.default_rng(31)
+    imgs, pairs, Hms = [], [], []
+    for i in range(3):
+        da, xy_a, db, xy_b, Hm, wh = make_pair(rng, 1500, 1700)
+        imgs += [(da, xy_a, wh), (db, xy_b, wh)]
+        pairs.append((2 * i, 2 * i + 1))
+        Hms.append(Hm)
+    m = PairMatcher(imgs, pairs, Hms, radius=40.0, kc=0, ratio=0.8)
+    m.run(form)
+    ref = m.results()
+    for rep in range(4):
+        m.run(form, graph=True)
+        torch.cuda.synchronize()
+        got = m.results()
+        for (q0, t0), (q1, t1) in zip(ref, got):
+            assert np.array_equal(q0, q1) and np.array_equal(t0, t1), "replay %d differs" % rep
+        m.run("plain" if form == "mfma" else "mfma")      # eager work in between
+        torch.cuda.synchronize()
