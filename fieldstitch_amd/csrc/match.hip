@@ -38,10 +38,14 @@ struct MImage {
     long long off;      // first keypoint in the concatenated arrays (caller's and cell-sorted)
     int n;              // keypoints
     int W, H;           // image size (inside test on the B side)
-    int gw, gh;         // spatial grid (cells of `radius` px covering [-2r, W+2r) x [-2r, H+2r))
+    int gw, gh;         // spatial grid (cells of csz = radius * (1 + 2^-20) px covering
+                        // [-2 csz, W + 2 csz) x [-2 csz, H + 2 csz))
     int tw4;            // tiles of 4x4 cells per tile row; cells are numbered tile-major
     int ncell;          // cell slots (whole tiles)
     long long cell_off; // first cell of this image in the cell arrays
+    long long chunk_off;// first slot of this image in the chunk table
+    int chunk_cap;      // chunk slots: ceil(n / 32) + tile rows
+    int pad;
 };
 
 struct MPair {
@@ -63,6 +67,7 @@ struct MatchBatch {
     int n_img, n_pair;
     long long n_query, n_kp, n_cells, n_targets, n_chunks;
     double radius, r2, ratio;
+    double csz;                   // grid cell size
     int kc;
     int pad;
     unsigned* norms;              // [Ntot] sum of squares of the descriptor
@@ -72,6 +77,9 @@ struct MatchBatch {
     int* s_orig;                  // [Ntot] the keypoint's index in its image
     int* cell_start;              // [n_cells + n_img] per image ncell + 1 offsets
     int* cell_cur;                // [n_cells] cursors of grids too large for LDS
+    int2* chunks;                 // [sum chunk_cap] (first sorted position, length) of each MFMA
+                                  // chunk: up to 32 keypoints of one tile row
+    int* n_chunks_img;            // [n_img] chunks of each image
     // per query, in the caller's order (diagnostics)
     int* best;                    // accepted best target, -1 = none
     double* b1;                   // its distance
@@ -98,8 +106,8 @@ __device__ __forceinline__ void cell_xy(const MImage& im, int id, int& cx, int& 
 }
 
 __device__ __forceinline__ void cell_of(const MatchBatch& B, const MImage& im, double x, double y, int& cx, int& cy) {
-    cx = (int)floor((x + 2.0 * B.radius) / B.radius);
-    cy = (int)floor((y + 2.0 * B.radius) / B.radius);
+    cx = (int)floor((x + 2.0 * B.csz) / B.csz);
+    cy = (int)floor((y + 2.0 * B.csz) / B.csz);
     cx = cx < 0 ? 0 : (cx >= im.gw ? im.gw - 1 : cx);
     cy = cy < 0 ? 0 : (cy >= im.gh ? im.gh - 1 : cy);
 }
@@ -114,6 +122,7 @@ constexpr int SU_KPT = 8;               // keypoints per thread held in register
 __global__ __launch_bounds__(SU_THREADS) void k_m_setup(const MatchBatch B) {
     __shared__ int s_cnt[SU_LDS_CELLS];
     __shared__ int s_w[SU_THREADS / 64];
+    __shared__ int s_rowc[SU_THREADS];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     if ((int)blockIdx.x >= B.n_img) {
         const long long base = (long long)(blockIdx.x - B.n_img) * SU_THREADS + tid;
@@ -140,7 +149,7 @@ __global__ __launch_bounds__(SU_THREADS) void k_m_setup(const MatchBatch B) {
         return;
     }
     const int ii = blockIdx.x;
-    const MImage& im = B.img[ii];
+    const MImage im = B.img[ii];      // by value: loaded once, no reloads after stores
     const int nc = im.ncell;
     int* cnt = nc <= SU_LDS_CELLS ? s_cnt : B.cell_cur + im.cell_off;   // only this block uses it
     for (int c = tid; c < nc; c += SU_THREADS) cnt[c] = 0;
@@ -196,6 +205,57 @@ __global__ __launch_bounds__(SU_THREADS) void k_m_setup(const MatchBatch B) {
     }
     if (tid == 0) start[nc] = carry;
     __syncthreads();
+    // MFMA chunk table: each tile row (4 cell rows, tw4 tiles) is cut into runs of <= 32
+    // keypoints, so a chunk never wraps from the right edge of the grid to the left edge.
+    // cnt[] holds the cell starts here (before the scatter advances them).
+    {
+        const int th4 = nc / (16 * im.tw4);
+        int total_chunks = 0;
+        for (int r0 = 0; r0 < th4; r0 += SU_THREADS) {
+            const int tr = r0 + tid;
+            int nch = 0;
+            if (tr < th4) {
+                const int b0 = cnt[tr * 16 * im.tw4];
+                const int b1 = tr + 1 < th4 ? cnt[(tr + 1) * 16 * im.tw4] : im.n;
+                nch = (b1 - b0 + 31) >> 5;
+            }
+            int x = nch;
+            for (int off = 1; off < 64; off <<= 1) {
+                const int y = __shfl_up(x, off);
+                if (lane >= off) x += y;
+            }
+            if (lane == 63) s_w[wv] = x;
+            __syncthreads();
+            int pre = 0, tot = 0;
+            for (int w2 = 0; w2 < SU_THREADS / 64; w2++) {
+                pre += w2 < wv ? s_w[w2] : 0;
+                tot += s_w[w2];
+            }
+            // row tr's chunks start at chunk index first; each thread writes its row's chunks
+            // only when the row is short, otherwise the rows are split below
+            if (tr < th4) s_rowc[tr & (SU_THREADS - 1)] = total_chunks + pre + x - nch;
+            __syncthreads();
+            // all threads: chunk c of this row group -> its row by binary search over s_rowc
+            const int nrow = min(SU_THREADS, th4 - r0);
+            for (int c = tid; c < tot; c += SU_THREADS) {
+                const int cg = total_chunks + c;
+                int lo = 0, hi = nrow - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_rowc[mid] <= cg) lo = mid;
+                    else hi = mid - 1;
+                }
+                const int trr = r0 + lo;
+                const int b0 = cnt[trr * 16 * im.tw4];
+                const int b1 = trr + 1 < th4 ? cnt[(trr + 1) * 16 * im.tw4] : im.n;
+                const int beg = b0 + 32 * (cg - s_rowc[lo]);
+                B.chunks[im.chunk_off + cg] = make_int2(beg, min(32, b1 - beg));
+            }
+            total_chunks += tot;
+            __syncthreads();
+        }
+        if (tid == 0) B.n_chunks_img[ii] = total_chunks;
+    }
     // scatter into the cell-sorted index (order inside a cell is arbitrary: every result is
     // defined by indices, not by visiting order)
 #pragma unroll
@@ -292,9 +352,9 @@ __device__ void knn_threshold(const MatchBatch& B, const MPair& P, const MImage&
 
 // kc > 0: the kc-th nearest key of every query, one thread per query (before the contraction)
 __global__ __launch_bounds__(256) void k_m_knn(const MatchBatch B) {
-    const MPair& P = B.pair[blockIdx.y];
-    const MImage& ia = B.img[P.a];
-    const MImage& ib = B.img[P.b];
+    const MPair P = B.pair[blockIdx.y];      // records by value: loaded once, never reloaded
+    const MImage ia = B.img[P.a];
+    const MImage ib = B.img[P.b];
     const int q = blockIdx.x * 256 + threadIdx.x;
     if (q >= ia.n) return;
     const long long g = P.q_off + q;
@@ -366,9 +426,9 @@ __device__ __forceinline__ float u8dot_f32(const float* qa, const uint4* cb) {
 }
 
 __global__ __launch_bounds__(256) void k_m_plain(const MatchBatch B) {
-    const MPair& P = B.pair[blockIdx.y];
-    const MImage& ia = B.img[P.a];
-    const MImage& ib = B.img[P.b];
+    const MPair P = B.pair[blockIdx.y];      // records by value: loaded once, never reloaded
+    const MImage ia = B.img[P.a];
+    const MImage ib = B.img[P.b];
     const int q = blockIdx.x * 256 + threadIdx.x;
     if (q >= ia.n) return;
     const long long g = P.q_off + q;
@@ -470,7 +530,7 @@ __device__ void query_direct(const MatchBatch& B, const MPair& P, const MImage& 
     v2 = INFINITY;
     i1 = -1;
     const float qn = (float)B.norms[ka];
-    const unsigned* a4 = (const unsigned*)(B.desc + ka * 128);
+    const uint4* a4 = (const uint4*)(B.desc + ka * 128);
     int cx0, cy0;
     cell_xy(ib, cell, cx0, cy0);
     const int* cs = B.cell_start + ib.cell_off + P.b;
@@ -482,9 +542,16 @@ __device__ void query_direct(const MatchBatch& B, const MPair& P, const MImage& 
                 const int t = B.s_orig[sb];
                 const double dx = B.s_xy[2 * sb] - qpx, dy = B.s_xy[2 * sb + 1] - qpy;
                 if (!in_window(B, dx * dx + dy * dy, t, td, ti)) continue;
-                const unsigned* b4 = (const unsigned*)(B.desc + (ib.off + t) * 128);
+                const uint4* b4 = (const uint4*)(B.desc + (ib.off + t) * 128);
                 unsigned dot = 0;
-                for (int k = 0; k < 32; k++) dot = __builtin_amdgcn_udot4(a4[k], b4[k], dot, false);
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const uint4 x = a4[k], y = b4[k];
+                    dot = __builtin_amdgcn_udot4(x.x, y.x, dot, false);
+                    dot = __builtin_amdgcn_udot4(x.y, y.y, dot, false);
+                    dot = __builtin_amdgcn_udot4(x.z, y.z, dot, false);
+                    dot = __builtin_amdgcn_udot4(x.w, y.w, dot, false);
+                }
                 const float d = fmaxf((qn + (float)B.norms[ib.off + t]) - 2.0f * (float)dot, 0.0f);
                 top2_push(d, t, v1, i1, v2);
             }
@@ -502,12 +569,13 @@ __global__ __launch_bounds__(64) void k_m_mfma(const MatchBatch B) {
     const int lane = threadIdx.x;
     const int qc = lane & 31;
     {
-        const MPair& P = B.pair[blockIdx.y];
-        const MImage& ia = B.img[P.a];
-        const MImage& ib = B.img[P.b];
-        if ((int)blockIdx.x * 32 >= ia.n) return;      // block-uniform: this pair has fewer chunks
-        const int kk = (int)blockIdx.x * 32 + qc;
-        const bool valid = kk < ia.n;
+        const MPair P = B.pair[blockIdx.y];      // records by value: loaded once, never reloaded
+        const MImage ia = B.img[P.a];
+        const MImage ib = B.img[P.b];
+        if ((int)blockIdx.x >= B.n_chunks_img[P.a]) return;      // block-uniform
+        const int2 chk = B.chunks[ia.chunk_off + blockIdx.x];
+        const int kk = chk.x + qc;
+        const bool valid = qc < chk.y;
         const long long sa = ia.off + (valid ? kk : 0);
         const int q = valid ? B.s_orig[sa] : -1;
         const long long g = P.q_off + (q >= 0 ? q : 0);
@@ -725,7 +793,7 @@ constexpr int CP_TPT = 8;          // targets per thread
 
 __global__ __launch_bounds__(CP_THREADS) void k_m_compact(const MatchBatch B) {
     __shared__ int s_w[CP_THREADS / 64];
-    const MPair& P = B.pair[blockIdx.x];
+    const MPair P = B.pair[blockIdx.x];
     const int nb = B.img[P.b].n;
     const unsigned long long* tk = B.tkey + P.t_off;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -772,11 +840,17 @@ __global__ __launch_bounds__(CP_THREADS) void k_m_compact(const MatchBatch B) {
 
 static inline size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// Cells slightly larger than the radius: two points closer than the radius then always land in
+// the same or adjacent cells despite the rounding of floor((x + 2 csz) / csz) (a cell
+// difference of 2 needs a computed quotient difference above 1, i.e. a true distance above
+// csz * (1 - 4 ulp) > radius), so the 3x3 cells around a prediction hold its whole window.
+static inline double cell_size(double radius) { return radius * (1.0 + 0x1p-20); }
+
 struct MatchPlanHost {
     std::vector<MImage> img;
     std::vector<MPair> pair;
-    long long n_kp = 0, n_query = 0, n_cells = 0, n_targets = 0, n_out = 0, n_chunks = 0;
-    size_t off_batch, off_img, off_pair, off_norms, off_snorms, off_sxy, off_sorig, off_cstart, off_ccur, off_best, off_b1,
+    long long n_kp = 0, n_query = 0, n_cells = 0, n_targets = 0, n_out = 0, n_chunks = 0, n_chunk_slots = 0;
+    size_t off_chunks, off_nchunks, off_batch, off_img, off_pair, off_norms, off_snorms, off_sxy, off_sorig, off_cstart, off_ccur, off_best, off_b1,
         off_qcell, off_kth, off_tkey, off_cnt, off_outq, off_outt, off_outn, total;
 };
 
@@ -785,7 +859,7 @@ static int match_plan(int n_img, const int* n_kp, const int* W, const int* H, in
     FSX_REQUIRE(n_img >= 1 && n_pair >= 1, "need images and pairs");
     FSX_REQUIRE(radius > 0, "radius must be > 0");
     M.img.resize(n_img);
-    long long off = 0, coff = 0;
+    long long off = 0, coff = 0, choff = 0;
     for (int i = 0; i < n_img; i++) {
         FSX_REQUIRE(n_kp[i] >= 0 && W[i] > 0 && H[i] > 0, "bad image %d", i);
         MImage& m = M.img[i];
@@ -793,18 +867,23 @@ static int match_plan(int n_img, const int* n_kp, const int* W, const int* H, in
         m.n = n_kp[i];
         m.W = W[i];
         m.H = H[i];
-        const double gw = ceil((W[i] + 4.0 * radius) / radius), gh = ceil((H[i] + 4.0 * radius) / radius);
+        const double csz = cell_size(radius);
+        const double gw = ceil((W[i] + 4.0 * csz) / csz), gh = ceil((H[i] + 4.0 * csz) / csz);
         FSX_REQUIRE(gw * gh < 1e8, "image %d: radius %g gives a grid of %.0f cells", i, radius, gw * gh);
         m.gw = (int)gw;
         m.gh = (int)gh;
         m.tw4 = (m.gw + 3) / 4;
         m.ncell = m.tw4 * ((m.gh + 3) / 4) * 16;
         m.cell_off = coff;
+        m.chunk_off = choff;
+        m.chunk_cap = (n_kp[i] + 31) / 32 + (m.gh + 3) / 4;
         off += n_kp[i];
         coff += m.ncell;
+        choff += m.chunk_cap;
     }
     M.n_kp = off;
     M.n_cells = coff;
+    M.n_chunk_slots = choff;
     M.pair.resize(n_pair);
     long long qo = 0, to = 0, oo = 0, chunks = 0;
     for (int p = 0; p < n_pair; p++) {
@@ -839,6 +918,8 @@ static int match_plan(int n_img, const int* n_kp, const int* W, const int* H, in
     M.off_sorig = o; o = a256(o + 4 * nk);
     M.off_cstart = o; o = a256(o + 4 * (size_t)(M.n_cells + n_img));
     M.off_ccur = o; o = a256(o + 4 * (size_t)M.n_cells);
+    M.off_chunks = o; o = a256(o + 8 * (size_t)std::max(M.n_chunk_slots, 1LL));
+    M.off_nchunks = o; o = a256(o + 4 * (size_t)n_img);
     M.off_best = o; o = a256(o + 4 * nq);
     M.off_b1 = o; o = a256(o + 8 * nq);
     M.off_qcell = o; o = a256(o + 4 * nq);
@@ -869,6 +950,7 @@ static MatchBatch batch_ptrs(const MatchPlanHost& M, char* ws, const uint8_t* de
     b.n_chunks = M.n_chunks;
     b.radius = radius;
     b.r2 = radius * radius;
+    b.csz = cell_size(radius);
     b.ratio = ratio;
     b.kc = kc;
     b.norms = (unsigned*)(ws + M.off_norms);
@@ -877,6 +959,8 @@ static MatchBatch batch_ptrs(const MatchPlanHost& M, char* ws, const uint8_t* de
     b.s_orig = (int*)(ws + M.off_sorig);
     b.cell_start = (int*)(ws + M.off_cstart);
     b.cell_cur = (int*)(ws + M.off_ccur);
+    b.chunks = (int2*)(ws + M.off_chunks);
+    b.n_chunks_img = (int*)(ws + M.off_nchunks);
     b.best = (int*)(ws + M.off_best);
     b.b1 = (double*)(ws + M.off_b1);
     b.qcell = (int*)(ws + M.off_qcell);
@@ -914,7 +998,7 @@ int fsx_match_prepare(int n_img, const int* n_kp, const int* W, const int* H, co
                       long long* out_offsets, long long* launch_info, void* stream) {
     FSX_REQUIRE(desc_dev && xy_dev && ws && pa && pb && launch_info, "null pointer argument");
     FSX_REQUIRE(kc >= 0 && kc <= KMAX, "kc must be in [0, %d]", KMAX);
-    static_assert(sizeof(MatchBatch) <= 8 * (FSX_MATCH_INFO_LEN - 8), "launch_info too small for MatchBatch");
+    static_assert(sizeof(MatchBatch) <= 8 * (FSX_MATCH_INFO_LEN - 9), "launch_info too small for MatchBatch");
     MatchPlanHost M;
     int rc = match_plan(n_img, n_kp, W, H, n_pair, pa, pb, dense, Hm, radius, M);
     if (rc) return rc;
@@ -929,8 +1013,11 @@ int fsx_match_prepare(int n_img, const int* n_kp, const int* W, const int* H, co
     if (rc) return rc;
     if (out_offsets)
         for (int p = 0; p < n_pair; p++) out_offsets[p] = M.pair[p].o_off;
-    int max_na = 0;
-    for (int p = 0; p < n_pair; p++) max_na = std::max(max_na, M.img[M.pair[p].a].n);
+    int max_na = 0, max_chunks = 0;
+    for (int p = 0; p < n_pair; p++) {
+        max_na = std::max(max_na, M.img[M.pair[p].a].n);
+        max_chunks = std::max(max_chunks, M.img[M.pair[p].a].chunk_cap);
+    }
     const long long extra =
         std::min<long long>(std::max<long long>(1, (std::max(M.n_targets, M.n_kp) + SU_THREADS - 1) / SU_THREADS), 4096);
     memset(launch_info, 0, 8 * FSX_MATCH_INFO_LEN);
@@ -942,6 +1029,7 @@ int fsx_match_prepare(int n_img, const int* n_kp, const int* W, const int* H, co
     launch_info[5] = n_img + extra;
     launch_info[6] = kc;
     launch_info[7] = (long long)(intptr_t)ws;
+    launch_info[FSX_MATCH_INFO_LEN - 1] = max_chunks;
     memcpy(launch_info + 8, &b, sizeof(b));
     return FSX_OK;
 }
@@ -968,7 +1056,7 @@ int fsx_match_launch(void* ws, const long long* info, int form, void* stage_even
         if (form == 0)
             hipLaunchKernelGGL(k_m_plain, dim3((unsigned)((max_na + 255) / 256), n_pair), dim3(256), 0, st, b);
         else
-            hipLaunchKernelGGL(k_m_mfma, dim3((unsigned)((max_na + 31) / 32), n_pair), dim3(64), 0, st, b);
+            hipLaunchKernelGGL(k_m_mfma, dim3((unsigned)info[FSX_MATCH_INFO_LEN - 1], n_pair), dim3(64), 0, st, b);
         FSX_LAUNCH_CHECK();
     }
     if (ev) FSX_HIP(hipEventRecord(ev[2], st));
