@@ -526,17 +526,6 @@ __global__ __launch_bounds__(256) void k_refine(const SiftJob* __restrict__ jobs
 // dy = G[y-1,x]-G[y+1,x]; magnitude sqrt (float32, correctly rounded); angle =
 // float32(atan2) * (180f/pi_f) in [0, 360).  Callers only ask for interior pixels.
 
-__device__ __forceinline__ void grad_at(const float* __restrict__ G, int C, int y, int x, float& mag, float& ang) {
-    const size_t k = (size_t)y * C + x;
-    const float dx = G[k + 1] - G[k - 1];
-    const float dy = G[k - C] - G[k + C];
-    const float xx = dx * dx, yy = dy * dy;
-    mag = sqrtf(xx + yy);
-    const float deg = 180.0f / 3.14159265358979323846f;
-    float a = (float)atan2((double)dy, (double)dx) * deg;
-    ang = a < 0.f ? a + 360.0f : a;
-}
-
 // Fixed-point (2^50 units) value of a non-negative contribution v < 2^2, rounded to nearest
 // even: x = v * 2^50 < 2^52, so x + 2^52 lands in [2^52, 2^53) where the double's ulp is 1 and
 // its mantissa bits are the rounded integer.  Same result as __double2ull_rn(v * 2^50) in two
@@ -547,9 +536,10 @@ __device__ __forceinline__ unsigned long long fx50(double v) {
 }
 
 // The descriptor's per-sample gradient: float32 atan2 (ocml, a few ulp).  The descriptor bar is
-// +-1 unit per bin on the 512 scale, far above these ulps; the orientation histogram keeps the
-// float64-rounded angle of grad_at because its peak decisions must follow the reference.
-__device__ __forceinline__ void grad_at_f(const float* __restrict__ G, int C, int y, int x, float& mag, float& ang) {
+// +-1 unit per bin on the 512 scale, far above these ulps; the orientation histogram decides
+// its bins with the float64-rounded angle where the float32 one could differ (k_orient).
+__device__ __forceinline__ void grad_at_f(const __attribute__((address_space(1))) float* __restrict__ G, int C, int y,
+                                          int x, float& mag, float& ang) {
     const size_t k = (size_t)y * C + x;
     const float dx = G[k + 1] - G[k - 1];
     const float dy = G[k - C] - G[k + C];
@@ -581,7 +571,7 @@ __global__ __launch_bounds__(64) void k_orient(const SiftJob* __restrict__ jobs)
         const int rad = min((int)rintf(4.5f * scl), ORI_R_MAX);
         const float sw = 1.5f * scl;
         const float den = 2.0f * (sw * sw);
-        const float* G = J.G123 + J.oct[o].gbase + (long long)(l - 1) * R * C;
+        const __attribute__((address_space(1))) float* G = gptr(J.G123 + J.oct[o].gbase + (long long)(l - 1) * R * C);
         if (lane < ORI_BINS) s_h[lane] = 0ull;
         // w = exp(-(i*i + j*j) / den) depends on i*i + j*j only: one float64 exp per value
         const int dmax = 2 * rad * rad;
@@ -813,10 +803,11 @@ __device__ __forceinline__ float wave_pairwise8(float v_lane_j_partial, int lane
 
 // one keypoint k of the arrays kp[7][stride], by a 256-thread block; s_h: HIST_SIZE u64,
 // s_d: 128 floats of LDS
-__device__ void describe_kp(const SiftJob& J, const float* __restrict__ kpa, int stride, long long k,
-                            float* __restrict__ desc_row, uint8_t* __restrict__ u8_row,
-                            unsigned long long* s_h, float* s_d) {
-    const int* kpi = (const int*)kpa;
+__device__ __forceinline__ void describe_kp(const SiftJob& J, const float* __restrict__ kpa_, int stride, long long k,
+                                           float* __restrict__ desc_row, uint8_t* __restrict__ u8_row,
+                                           unsigned long long* s_h, float* s_d) {
+    const __attribute__((address_space(1))) float* kpa = gptr(kpa_);
+    const __attribute__((address_space(1))) int* kpi = gptr((const int*)kpa_);
     const int o = kpi[5 * (long long)stride + k], li = kpi[6 * (long long)stride + k];
     const int R = J.oct[o].R, C = J.oct[o].C;
     const double f = 1.0 / (ldexp(1.0, o) * 0.5);
@@ -832,7 +823,7 @@ __device__ void describe_kp(const SiftJob& J, const float* __restrict__ kpa, int
     const float t = ori * (float)(3.14159265358979323846 / 180.0);
     const double cos_t = (double)(float)cos((double)t) / hw, sin_t = (double)(float)sin((double)t) / hw;
     const long long pr = (long long)rint(py), pc = (long long)rint(px);
-    const float* G = J.G123 + J.oct[o].gbase + (long long)(li - 1) * R * C;
+    const __attribute__((address_space(1))) float* G = gptr(J.G123 + J.oct[o].gbase + (long long)(li - 1) * R * C);
     for (int b = threadIdx.x; b < HIST_SIZE; b += 256) s_h[b] = 0ull;
     // per row i of the window, the column span [jl, jh] outside of which |c_rot| or |r_rot|
     // >= 2.5 for sure (widened by one sample; the exact per-sample test below still decides):
