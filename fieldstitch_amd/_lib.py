@@ -52,6 +52,13 @@ SIGNATURES = {
     "fsx_undistort_f64": [_i, _vp, C.POINTER(C.c_double), _i, _vp, _vp],
     "fsx_ground_predict": [_i, _vp, C.POINTER(C.c_double), _vp, _vp, _vp],
     "fsx_pair_select": [_i, _vp, _vp, _i, _i, _vp, _vp],
+    "fsx_fmatch_workspace_bytes": [_i, C.POINTER(_i), _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_sz),
+                                   C.POINTER(C.c_longlong)],
+    "fsx_fmatch_batch": [_i, C.POINTER(_i), _vp, _i, C.POINTER(_i), C.POINTER(_i), C.c_double, _i, _i, _vp, _sz,
+                         C.POINTER(C.c_longlong), _vp, _vp],
+    "fsx_fmatch_results": [_i, C.POINTER(_i), _i, C.POINTER(_i), C.POINTER(_i), _vp, C.POINTER(_i), C.POINTER(_i),
+                           C.POINTER(_i), C.POINTER(C.c_double), _vp],
+    "fsx_fp8_layout_probe": [_vp, _vp],
     "fsx_match_query_state": [_i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), _i, C.POINTER(_i), C.POINTER(_i),
                               C.c_double, _vp, C.POINTER(_i), C.POINTER(C.c_double), C.POINTER(_i), _vp],
 }

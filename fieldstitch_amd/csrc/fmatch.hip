@@ -1,0 +1,796 @@
+// Brute-force ratio matching of float descriptors (matching.py:86-100 _ratio_match_bf with
+// matching.py:103-110 _unique) on gfx950, exact against the FP64 match-set definition:
+// d(q, t) = sum_k (a_k - b_k)^2 accumulated in float64 in k order; best = argmin d (ties: lower
+// index); b2 = the second smallest d; accept iff sqrt(b1) < ratio * sqrt(b2) in float64; per
+// target the smallest b1 wins, ties to the lower query; output sorted by target.
+//
+// Pipeline (one launch each, all pairs of a batch at once):
+//   quantize: per descriptor its float64 norm, a low-precision copy (BF16, or FP8 e4m3 with a
+//             per-descriptor scale) and the exact norm of its quantization error;
+//   screen:   MFMA over every (query, target) of a pair with FP32 accumulation (BF16
+//             32x32x16, or FP8 32x32x64), keeping the K smallest approximate distances per query;
+//   rescore:  the K candidates in float64 with the definition's arithmetic, then a certificate:
+//             any target outside the K has approximate distance >= D_K (the K-th), hence true
+//             distance >= D_K - eps_q, where eps_q bounds the screening error of query q
+//             (quantization errors via |a^.b^ - a.b| <= |a| |b - b^| + |b^| |a - a^|, plus the
+//             FP32 accumulation and rounding terms).  Accept / reject decisions that hold for
+//             every consistent true value are final; the rest go to
+//   fallback: one block per uncertain query, exact float64 over all targets;
+//   unique / compact: per target the smallest (b1, q), then (q, t) lists in target order.
+#include "fsx_common.h"
+#include <math.h>
+#include <string.h>
+#include <algorithm>
+#include <vector>
+
+namespace fsx {
+
+constexpr int FD = 128;            // descriptor length
+constexpr int FK_MAX = 8;          // candidates kept per query
+
+struct FSet {
+    long long off;     // first descriptor in the concatenated arrays
+    int n;
+    int pad;
+};
+
+struct FPair {
+    int a, b;          // queries from set a, targets from set b
+    long long q_off;   // first global query (prefix of n_a)
+    long long t_off;   // first per-target slot (prefix of n_b)
+    long long o_off;   // first output slot (prefix of min(n_a, n_b))
+};
+
+struct FBatch {
+    const float* desc;            // [Ntot][128] float32
+    const FSet* set;
+    const FPair* pair;
+    int n_set, n_pair;
+    long long n_kp, n_query, n_targets;
+    double ratio;
+    int prec;                     // 1 = BF16, 2 = FP8 e4m3
+    int K;                        // candidates kept (2..8)
+    // per descriptor
+    unsigned short* qbf;          // [Ntot][128] bf16 bits (prec 1)
+    uint8_t* q8;                  // [Ntot][128] e4m3 codes (prec 2)
+    float* scale;                 // [Ntot] fp8 scale (1 for bf16)
+    double* n2;                   // [Ntot] |a|^2 (float64, k order)
+    float* nrm;                   // [Ntot] |a| rounded up
+    float* qerr;                  // [Ntot] |a - dequant(a^)| rounded up
+    unsigned* set_nmax;           // [n_set] max |b| (float bits)
+    unsigned* set_emax;           // [n_set] max quantization error (float bits)
+    // per query
+    float* topd;                  // [n_query][FK_MAX] screened distances, ascending
+    int* topi;                    // [n_query][FK_MAX] their target indices
+    int* best;                    // accepted target, -1 none
+    double* b1;                   // its exact distance
+    unsigned long long* tkey;     // [n_targets] min accepted b1 bits
+    int* tq;                      // [n_targets] min query among equal b1
+    int* fb_list;                 // [n_query] uncertain queries (global index)
+    int* counters;                // [0] fallback count, [1] max |d~ - d| (float bits),
+                                  // [2..3] sum |d~ - d| (double), [4..5] compared count (u64)
+    int* out_q;
+    int* out_t;
+    int* out_n;
+};
+
+__device__ __forceinline__ int fpair_of(const FBatch& B, long long g) {
+    int lo = 0, hi = B.n_pair - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (B.pair[mid].q_off <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int fset_of(const FBatch& B, long long k) {
+    int lo = 0, hi = B.n_set - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (B.set[mid].off <= k) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// float32 -> bfloat16, round to nearest even (finite inputs)
+__device__ __forceinline__ unsigned short f2bf(float x) {
+    const unsigned b = __float_as_uint(x);
+    return (unsigned short)((b + 0x7fffu + ((b >> 16) & 1u)) >> 16);
+}
+
+__device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
+// |v| <= 448 -> OCP FP8 e4m3 code (bias 7, subnormals m * 2^-9), round to nearest even
+__device__ __forceinline__ unsigned f2e4m3(float v) {
+    const unsigned sign = v < 0.0f ? 0x80u : 0u;
+    const float a = fabsf(v);
+    if (a == 0.0f) return sign;
+    const unsigned b = __float_as_uint(a);
+    int e = (int)(b >> 23) - 127;
+    unsigned code;
+    if (e < -6) {
+        code = (unsigned)rintf(a * 512.0f);          // 0..8; 8 is the smallest normal's code
+    } else {
+        const unsigned mant = b & 0x7fffffu;
+        unsigned q = mant >> 20;
+        const unsigned rem = mant & 0xfffffu;
+        if (rem > 0x80000u || (rem == 0x80000u && (q & 1u))) q++;
+        if (q == 8u) {
+            q = 0u;
+            e++;
+        }
+        code = ((unsigned)(e + 7) << 3) | q;
+        code = code > 0x7eu ? 0x7eu : code;          // 448
+    }
+    return sign | code;
+}
+
+__device__ __forceinline__ double e4m3_value(unsigned c) {
+    const double s = (c & 0x80u) ? -1.0 : 1.0;
+    const unsigned ex = (c >> 3) & 15u, m = c & 7u;
+    return s * (ex == 0 ? (double)m * 0x1p-9 : ldexp(1.0 + (double)m / 8.0, (int)ex - 7));
+}
+
+// ------------------------------------------------------------------------------------------
+// Quantize + resets
+
+__global__ __launch_bounds__(256) void k_f_zero_sets(const FBatch B) {
+    for (int s = threadIdx.x; s < B.n_set; s += 256) {
+        B.set_nmax[s] = 0u;
+        B.set_emax[s] = 0u;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_f_quant(const FBatch B) {
+    const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+    const long long stride = (long long)gridDim.x * 256;
+    for (long long t = k; t < B.n_targets; t += stride) {
+        B.tkey[t] = ~0ull;
+        B.tq[t] = 0x7fffffff;
+    }
+    if (k < 8) B.counters[k] = 0;
+    if (k >= B.n_kp) return;
+    const float4* a4 = (const float4*)(B.desc + k * FD);
+    float a[FD];
+#pragma unroll
+    for (int i = 0; i < FD / 4; i++) {
+        const float4 v = a4[i];
+        a[4 * i] = v.x;
+        a[4 * i + 1] = v.y;
+        a[4 * i + 2] = v.z;
+        a[4 * i + 3] = v.w;
+    }
+    double n2 = 0.0, e2 = 0.0;
+    float amax = 0.0f;
+#pragma unroll
+    for (int i = 0; i < FD; i++) {
+        n2 += (double)a[i] * (double)a[i];
+        amax = fmaxf(amax, fabsf(a[i]));
+    }
+    float sc = 1.0f;
+    if (B.prec == 1) {
+        unsigned short* dst = B.qbf + k * FD;
+#pragma unroll
+        for (int i = 0; i < FD; i++) {
+            const unsigned short h = f2bf(a[i]);
+            dst[i] = h;
+            const double d = (double)a[i] - (double)bf2f(h);
+            e2 += d * d;
+        }
+    } else {
+        sc = amax > 0.0f ? amax / 448.0f : 1.0f;
+        uint8_t* dst = B.q8 + k * FD;
+#pragma unroll
+        for (int i = 0; i < FD; i += 4) {
+            unsigned w = 0;
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const unsigned c = f2e4m3(fminf(fmaxf(a[i + u] / sc, -448.0f), 448.0f));
+                w |= c << (8 * u);
+                const double d = (double)a[i + u] - (double)sc * e4m3_value(c);
+                e2 += d * d;
+            }
+            *(unsigned*)(dst + i) = w;
+        }
+    }
+    B.scale[k] = sc;
+    B.n2[k] = n2;
+    // upper bounds as float32 (rounded up by one ulp margin)
+    const float nr = __double2float_ru(sqrt(n2) * (1.0 + 1e-12));
+    const float er = __double2float_ru(sqrt(e2) * (1.0 + 1e-12));
+    B.nrm[k] = nr;
+    B.qerr[k] = er;
+    const int s = fset_of(B, k);
+    atomicMax(&B.set_nmax[s], __float_as_uint(nr));     // non-negative floats order as uints
+    atomicMax(&B.set_emax[s], __float_as_uint(er));
+}
+
+// ------------------------------------------------------------------------------------------
+// Screen: block = 4 waves x 32 queries; targets stream through LDS in tiles of 64.
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+typedef int i32x8_t __attribute__((ext_vector_type(8)));
+
+constexpr int FS_TILE = 64;
+
+template <int K>
+__device__ __forceinline__ void topk_push(float d, int t, float* td, int* ti) {
+    if (!(d < td[K - 1] || (d == td[K - 1] && t < ti[K - 1]))) return;
+    // insert keeping (d, t) ascending
+#pragma unroll
+    for (int j = K - 1; j >= 0; j--) {
+        const bool shift = j > 0 && (d < td[j - 1] || (d == td[j - 1] && t < ti[j - 1]));
+        if (shift) {
+            td[j] = td[j - 1];
+            ti[j] = ti[j - 1];
+        } else {
+            td[j] = d;
+            ti[j] = t;
+            break;
+        }
+    }
+}
+
+template <int PREC, int K>
+__global__ __launch_bounds__(256) void k_f_screen(const FBatch B) {
+    constexpr int ROWB = PREC == 1 ? FD * 2 : FD;                 // bytes per staged row
+    __shared__ __attribute__((aligned(16))) uint8_t s_rows[FS_TILE * ROWB];
+    __shared__ float s_cn2[FS_TILE], s_cs[FS_TILE];
+    __shared__ float s_md[256 * K];
+    __shared__ int s_mi[256 * K];
+    const FPair P = B.pair[blockIdx.y];
+    const FSet sa = B.set[P.a], sb = B.set[P.b];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int q = blockIdx.x * 128 + wv * 32 + (lane & 31);
+    if (blockIdx.x * 128 >= sa.n) return;                           // block-uniform
+    const bool valid = q < sa.n && sa.n >= 2 && sb.n >= 2;
+    const long long ka = sa.off + (q < sa.n ? q : 0);
+    // query operand (this lane's half of its row) and its terms
+    bf16x8_t qb[8];
+    i32x8_t q8[2];
+    if (PREC == 1) {
+        const uint4* src = (const uint4*)(B.qbf + ka * FD);
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            const uint4 v = src[2 * s + (lane >> 5)];
+            qb[s] = __builtin_bit_cast(bf16x8_t, v);
+        }
+    } else {
+        const uint4* src = (const uint4*)(B.q8 + ka * FD);
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            const uint4 v0 = src[4 * s + 2 * (lane >> 5)], v1 = src[4 * s + 2 * (lane >> 5) + 1];
+            q8[s] = i32x8_t{(int)v0.x, (int)v0.y, (int)v0.z, (int)v0.w, (int)v1.x, (int)v1.y, (int)v1.z, (int)v1.w};
+        }
+    }
+    const float qn2 = (float)B.n2[ka];
+    const float qs = B.scale[ka];
+    float td[K];
+    int ti[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        td[j] = INFINITY;
+        ti[j] = 0x7fffffff;
+    }
+    for (int base = 0; base < sb.n; base += FS_TILE) {
+        const int nt = min(FS_TILE, sb.n - base);
+        // stage the tile: 16-byte loads, all threads
+        for (int u = tid; u < FS_TILE * ROWB / 16; u += 256) {
+            const int r = u / (ROWB / 16), c = u - r * (ROWB / 16);
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (r < nt) {
+                const uint8_t* row = PREC == 1 ? (const uint8_t*)(B.qbf + (sb.off + base + r) * FD)
+                                               : B.q8 + (sb.off + base + r) * FD;
+                v = ((const uint4*)row)[c];
+            }
+            ((uint4*)s_rows)[u] = v;
+        }
+        if (tid < FS_TILE) {
+            const bool ok = tid < nt;
+            s_cn2[tid] = ok ? (float)B.n2[sb.off + base + tid] : 0.0f;
+            s_cs[tid] = ok ? B.scale[sb.off + base + tid] : 0.0f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < FS_TILE / 32; h++) {
+            const int rl = h * 32 + (lane & 31);
+            f32x16_t acc = {};
+            if (PREC == 1) {
+                const uint4* rp = (const uint4*)(s_rows + rl * ROWB);
+#pragma unroll
+                for (int s = 0; s < 8; s++) {
+                    const bf16x8_t a = __builtin_bit_cast(bf16x8_t, rp[2 * s + (lane >> 5)]);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, qb[s], acc, 0, 0, 0);
+                }
+            } else {
+                const uint4* rp = (const uint4*)(s_rows + rl * ROWB);
+#pragma unroll
+                for (int s = 0; s < 2; s++) {
+                    const uint4 v0 = rp[4 * s + 2 * (lane >> 5)], v1 = rp[4 * s + 2 * (lane >> 5) + 1];
+                    const i32x8_t a = {(int)v0.x, (int)v0.y, (int)v0.z, (int)v0.w,
+                                       (int)v1.x, (int)v1.y, (int)v1.z, (int)v1.w};
+                    // unit block scales (E8M0 127 = 2^0); both operands e4m3 (format 0)
+                    acc = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, q8[s], acc, 0, 0, 0, 0x7f7f7f7f, 0,
+                                                                          0x7f7f7f7f);
+                }
+            }
+            if (valid) {
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const int r = h * 32 + i * 8 + (lane >> 5) * 4 + j;
+                        if (r >= nt) continue;
+                        const float dot = PREC == 1 ? acc[i * 4 + j] : acc[i * 4 + j] * (qs * s_cs[r]);
+                        const float d = (qn2 + s_cn2[r]) - 2.0f * dot;
+                        topk_push<K>(d, base + r, td, ti);
+                    }
+            }
+        }
+        __syncthreads();
+    }
+    // merge the two half-waves of each query
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        s_md[tid * K + j] = td[j];
+        s_mi[tid * K + j] = ti[j];
+    }
+    __syncthreads();
+    if (lane < 32 && valid) {
+#pragma unroll
+        for (int j = 0; j < K; j++) topk_push<K>(s_md[(tid + 32) * K + j], s_mi[(tid + 32) * K + j], td, ti);
+        const long long g = P.q_off + q;
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            B.topd[g * FK_MAX + j] = td[j];
+            B.topi[g * FK_MAX + j] = ti[j];
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Rescore + certificate
+
+// sum_k (a_k - b_k)^2 in float64, k in order (the definition's arithmetic)
+__device__ __forceinline__ double dist64(const float* __restrict__ a, const float* __restrict__ b) {
+    const float4* a4 = (const float4*)a;
+    const float4* b4 = (const float4*)b;
+    double acc = 0.0;
+#pragma unroll 8
+    for (int i = 0; i < FD / 4; i++) {
+        const float4 x = a4[i], y = b4[i];
+        double t = (double)x.x - (double)y.x;
+        acc += t * t;
+        t = (double)x.y - (double)y.y;
+        acc += t * t;
+        t = (double)x.z - (double)y.z;
+        acc += t * t;
+        t = (double)x.w - (double)y.w;
+        acc += t * t;
+    }
+    return acc;
+}
+
+// bound on |d~ - d| for query k against any target of set s
+__device__ __forceinline__ double screen_eps(const FBatch& B, long long ka, int s) {
+    const double u = 0x1p-24;
+    const double na = (double)B.nrm[ka], ea = (double)B.qerr[ka];
+    const double nb = (double)__uint_as_float(B.set_nmax[s]), eb = (double)__uint_as_float(B.set_emax[s]);
+    const double gam = 2.0 * (FD * u) / (1.0 - FD * u);          // FP32 accumulation, x2 margin
+    const double qa = na + ea, qb = nb + eb;
+    const double dot_err = na * eb + qb * ea + (gam + 4.0 * u) * qa * qb;
+    const double rnd = 4.0 * u * (na * na + nb * nb + 2.0 * qa * qb);
+    return (2.0 * dot_err + rnd) * 1.01 + 1e-30;
+}
+
+__device__ __forceinline__ void f_accept(const FBatch& B, long long g, const FPair& P, int t, double b1) {
+    B.best[g] = t;
+    B.b1[g] = b1;
+    atomicMin(&B.tkey[P.t_off + t], (unsigned long long)__double_as_longlong(b1));
+}
+
+// one 8-lane group per query: lane j rescores candidate j
+__global__ __launch_bounds__(256) void k_f_rescore(const FBatch B) {
+    const FPair P = B.pair[blockIdx.y];
+    const FSet sa = B.set[P.a], sb = B.set[P.b];
+    const int tid = threadIdx.x, sub = tid & 7;
+    const int q = blockIdx.x * 32 + (tid >> 3);
+    if (q >= sa.n) return;
+    const long long g = P.q_off + q;
+    if (sa.n < 2 || sb.n < 2) {
+        if (sub == 0) B.best[g] = -1;
+        return;
+    }
+    const int K = B.K;
+    const long long ka = sa.off + q;
+    if (K == 0) {                           // exact mode: every query takes the float64 pass
+        if (sub == 0) {
+            B.best[g] = -1;
+            B.fb_list[atomicAdd(&B.counters[0], 1)] = (int)g;
+        }
+        return;
+    }
+    const int kk = min(K, sb.n);
+    double dv = INFINITY;
+    int iv = 0x7fffffff;
+    float dt = INFINITY;
+    if (sub < kk) {
+        iv = B.topi[g * FK_MAX + sub];
+        dt = B.topd[g * FK_MAX + sub];
+        dv = dist64(B.desc + ka * FD, B.desc + (sb.off + iv) * FD);
+        const float err = (float)fabs((double)dt - dv);
+        atomicMax((unsigned*)&B.counters[1], __float_as_uint(err));
+        atomicAdd((double*)&B.counters[2], (double)err);
+        atomicAdd((unsigned long long*)&B.counters[4], 1ull);
+    }
+    // gather the group's exact values (lanes of one group are consecutive)
+    double d[FK_MAX];
+    int id[FK_MAX];
+#pragma unroll
+    for (int j = 0; j < FK_MAX; j++) {
+        d[j] = __shfl(dv, (tid & ~7) + j);
+        id[j] = __shfl(iv, (tid & ~7) + j);
+    }
+    const float dK = __shfl(dt, (tid & ~7) + (kk - 1));
+    if (sub != 0) return;
+    // exact top-2 among the candidates
+    double b1 = INFINITY, b2 = INFINITY;
+    int i1 = 0x7fffffff;
+#pragma unroll
+    for (int j = 0; j < FK_MAX; j++) {
+        if (j >= kk) continue;
+        if (d[j] < b1 || (d[j] == b1 && id[j] < i1)) {
+            b2 = b1;
+            b1 = d[j];
+            i1 = id[j];
+        } else {
+            b2 = fmin(b2, d[j]);
+        }
+    }
+    if (sb.n <= kk) {                       // every target was rescored: exact
+        if (sqrt(b1) < B.ratio * sqrt(b2)) f_accept(B, g, P, i1, b1);
+        else B.best[g] = -1;
+        return;
+    }
+    // targets outside the candidates: true distance >= L
+    const double eps = screen_eps(B, ka, P.b);
+    const double L = ((double)dK - eps) * (1.0 - 0x1p-40);
+    const bool best_ok = b1 < L;
+    const double lb2 = fmin(b2, L), lb1 = fmin(b1, L);
+    if (best_ok && sqrt(b1) < B.ratio * sqrt(lb2)) {
+        f_accept(B, g, P, i1, b1);
+    } else if (sqrt(lb1) >= B.ratio * sqrt(b2)) {
+        B.best[g] = -1;
+    } else {
+        B.best[g] = -1;
+        const int slot = atomicAdd(&B.counters[0], 1);
+        B.fb_list[slot] = (int)g;
+    }
+}
+
+// exact float64 over all targets for the uncertain queries, one block each
+__global__ __launch_bounds__(256) void k_f_fallback(const FBatch B) {
+    __shared__ __attribute__((aligned(16))) float s_q[FD];
+    __shared__ double s_b1[256], s_b2[256];
+    __shared__ int s_i1[256];
+    const int nfb = B.counters[0];
+    for (int f = blockIdx.x; f < nfb; f += gridDim.x) {
+        const long long g = B.fb_list[f];
+        const FPair P = B.pair[fpair_of(B, g)];
+        const FSet sa = B.set[P.a], sb = B.set[P.b];
+        const long long ka = sa.off + (g - P.q_off);
+        if (threadIdx.x < FD) s_q[threadIdx.x] = B.desc[ka * FD + threadIdx.x];
+        __syncthreads();
+        double b1 = INFINITY, b2 = INFINITY;
+        int i1 = 0x7fffffff;
+        for (int t = threadIdx.x; t < sb.n; t += 256) {
+            const double d = dist64(s_q, B.desc + (sb.off + t) * FD);
+            if (d < b1 || (d == b1 && t < i1)) {
+                b2 = b1;
+                b1 = d;
+                i1 = t;
+            } else {
+                b2 = fmin(b2, d);
+            }
+        }
+        s_b1[threadIdx.x] = b1;
+        s_b2[threadIdx.x] = b2;
+        s_i1[threadIdx.x] = i1;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int u = 1; u < 256; u++) {
+                const double c1 = s_b1[u], c2 = s_b2[u];
+                const int ci = s_i1[u];
+                if (c1 < b1 || (c1 == b1 && ci < i1)) {
+                    b2 = fmin(b1, c2);
+                    b1 = c1;
+                    i1 = ci;
+                } else {
+                    b2 = fmin(b2, c1);
+                }
+            }
+            if (sqrt(b1) < B.ratio * sqrt(b2)) f_accept(B, g, P, i1, b1);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_f_unique2(const FBatch B) {
+    const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (g >= B.n_query) return;
+    const int t = B.best[g];
+    if (t < 0) return;
+    const FPair P = B.pair[fpair_of(B, g)];
+    if ((unsigned long long)__double_as_longlong(B.b1[g]) == B.tkey[P.t_off + t])
+        atomicMin(&B.tq[P.t_off + t], (int)(g - P.q_off));
+}
+
+__global__ __launch_bounds__(1024) void k_f_compact(const FBatch B) {
+    __shared__ int s_w[16];
+    const FPair P = B.pair[blockIdx.x];
+    const int nb = B.set[P.b].n;
+    const int* tq = B.tq + P.t_off;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    int carry = 0;
+    for (int base = 0; base < nb; base += 1024) {
+        const int t = base + tid;
+        const int v = (t < nb && tq[t] != 0x7fffffff) ? 1 : 0;
+        int x = v;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(x, off);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) s_w[wv] = x;
+        __syncthreads();
+        int pre = 0, tot = 0;
+        for (int w2 = 0; w2 < 16; w2++) {
+            pre += w2 < wv ? s_w[w2] : 0;
+            tot += s_w[w2];
+        }
+        if (v) {
+            const long long o = P.o_off + carry + pre + x - 1;
+            B.out_q[o] = tq[t];
+            B.out_t[o] = t;
+        }
+        carry += tot;
+        __syncthreads();
+    }
+    if (tid == 0) B.out_n[blockIdx.x] = carry;
+}
+
+// ------------------------------------------------------------------------------------------
+// FP8 MFMA probe: A byte e of lane l (row l % 32, half h = l / 32) = e4m3 code of
+// ((l%32)*7 + h*3 + e*5) % 11 - 5, B byte = code of ((l%32)*3 + h*5 + e*7) % 13 - 6 (small
+// integers are exact in e4m3); out: 64 x 16 floats.
+__global__ void k_fp8_probe(float* out) {
+    const int lane = threadIdx.x, r = lane & 31, h = lane >> 5;
+    i32x8_t a, b;
+#pragma unroll
+    for (int w = 0; w < 8; w++) {
+        unsigned wa = 0, wb = 0;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int e = 4 * w + u;
+            wa |= f2e4m3((float)((r * 7 + h * 3 + e * 5) % 11 - 5)) << (8 * u);
+            wb |= f2e4m3((float)((r * 3 + h * 5 + e * 7) % 13 - 6)) << (8 * u);
+        }
+        a[w] = (int)wa;
+        b[w] = (int)wb;
+    }
+    f32x16_t acc = {};
+    acc = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, acc, 0, 0, 0, 0x7f7f7f7f, 0, 0x7f7f7f7f);
+#pragma unroll
+    for (int i = 0; i < 16; i++) out[lane * 16 + i] = acc[i];
+}
+
+// ------------------------------------------------------------------------------------------
+// Host side
+
+static inline size_t fa256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct FPlan {
+    std::vector<FSet> set;
+    std::vector<FPair> pair;
+    long long n_kp = 0, n_query = 0, n_targets = 0, n_out = 0;
+    int max_na = 0;
+    size_t off_set, off_pair, off_qbf, off_q8, off_scale, off_n2, off_nrm, off_qerr, off_nmax, off_emax, off_topd,
+        off_topi, off_best, off_b1, off_tkey, off_tq, off_fb, off_cnt, off_outq, off_outt, off_outn, total;
+};
+
+static int fplan(int n_set, const int* n_kp, int n_pair, const int* pa, const int* pb, FPlan& M) {
+    FSX_REQUIRE(n_set >= 1 && n_pair >= 1 && n_kp && pa && pb, "need sets and pairs");
+    M.set.resize(n_set);
+    long long off = 0;
+    for (int i = 0; i < n_set; i++) {
+        FSX_REQUIRE(n_kp[i] >= 0, "bad set %d", i);
+        M.set[i].off = off;
+        M.set[i].n = n_kp[i];
+        M.set[i].pad = 0;
+        off += n_kp[i];
+    }
+    M.n_kp = off;
+    M.pair.resize(n_pair);
+    long long qo = 0, to = 0, oo = 0;
+    for (int p = 0; p < n_pair; p++) {
+        FSX_REQUIRE(pa[p] >= 0 && pa[p] < n_set && pb[p] >= 0 && pb[p] < n_set, "bad pair %d", p);
+        FPair& P = M.pair[p];
+        P.a = pa[p];
+        P.b = pb[p];
+        P.q_off = qo;
+        P.t_off = to;
+        P.o_off = oo;
+        qo += M.set[P.a].n;
+        to += M.set[P.b].n;
+        oo += std::min(M.set[P.a].n, M.set[P.b].n);
+        M.max_na = std::max(M.max_na, M.set[P.a].n);
+    }
+    M.n_query = qo;
+    M.n_targets = to;
+    M.n_out = oo;
+    const size_t nk = (size_t)std::max(M.n_kp, 1LL), nq = (size_t)std::max(qo, 1LL), nt = (size_t)std::max(to, 1LL);
+    size_t o = 0;
+    M.off_set = o; o = fa256(o + sizeof(FSet) * n_set);
+    M.off_pair = o; o = fa256(o + sizeof(FPair) * n_pair);
+    M.off_qbf = o; o = fa256(o + 2 * FD * nk);
+    M.off_q8 = o; o = fa256(o + FD * nk);
+    M.off_scale = o; o = fa256(o + 4 * nk);
+    M.off_n2 = o; o = fa256(o + 8 * nk);
+    M.off_nrm = o; o = fa256(o + 4 * nk);
+    M.off_qerr = o; o = fa256(o + 4 * nk);
+    M.off_nmax = o; o = fa256(o + 4 * (size_t)n_set);
+    M.off_emax = o; o = fa256(o + 4 * (size_t)n_set);
+    M.off_topd = o; o = fa256(o + 4 * FK_MAX * nq);
+    M.off_topi = o; o = fa256(o + 4 * FK_MAX * nq);
+    M.off_best = o; o = fa256(o + 4 * nq);
+    M.off_b1 = o; o = fa256(o + 8 * nq);
+    M.off_tkey = o; o = fa256(o + 8 * nt);
+    M.off_tq = o; o = fa256(o + 4 * nt);
+    M.off_fb = o; o = fa256(o + 4 * nq);
+    M.off_cnt = o; o = fa256(o + 64);
+    M.off_outq = o; o = fa256(o + 4 * (size_t)std::max(oo, 1LL));
+    M.off_outt = o; o = fa256(o + 4 * (size_t)std::max(oo, 1LL));
+    M.off_outn = o; o = fa256(o + 4 * (size_t)n_pair);
+    M.total = o;
+    return FSX_OK;
+}
+
+}  // namespace fsx
+
+using namespace fsx;
+
+extern "C" {
+
+int fsx_fmatch_workspace_bytes(int n_set, const int* n_kp, int n_pair, const int* pa, const int* pb, size_t* bytes,
+                               long long* out_capacity) {
+    FSX_REQUIRE(bytes, "null pointer argument");
+    FPlan M;
+    int rc = fplan(n_set, n_kp, n_pair, pa, pb, M);
+    if (rc) return rc;
+    *bytes = M.total;
+    if (out_capacity) *out_capacity = M.n_out;
+    return FSX_OK;
+}
+
+int fsx_fmatch_batch(int n_set, const int* n_kp, const float* desc_dev, int n_pair, const int* pa, const int* pb,
+                     double ratio, int precision, int K, void* ws, size_t ws_bytes, long long* out_offsets,
+                     void* stage_events_v, void* stream) {
+    FSX_REQUIRE(desc_dev && ws, "null pointer argument");
+    FSX_REQUIRE(precision == 1 || precision == 2, "precision must be 1 (BF16) or 2 (FP8 e4m3)");
+    FSX_REQUIRE(K == 0 || K == 2 || K == 8, "K must be 0 (exact float64 for every query), 2 or 8");
+    FPlan M;
+    int rc = fplan(n_set, n_kp, n_pair, pa, pb, M);
+    if (rc) return rc;
+    FSX_REQUIRE(ws_bytes >= M.total, "fmatch workspace too small: %zu < %zu", ws_bytes, M.total);
+    FSX_REQUIRE(n_pair <= 65535, "at most 65535 pairs per batch");
+    hipStream_t st = (hipStream_t)stream;
+    char* w = (char*)ws;
+    FBatch b;
+    memset(&b, 0, sizeof(b));
+    b.desc = desc_dev;
+    b.set = (const FSet*)(w + M.off_set);
+    b.pair = (const FPair*)(w + M.off_pair);
+    b.n_set = n_set;
+    b.n_pair = n_pair;
+    b.n_kp = M.n_kp;
+    b.n_query = M.n_query;
+    b.n_targets = M.n_targets;
+    b.ratio = ratio;
+    b.prec = precision;
+    b.K = K;
+    b.qbf = (unsigned short*)(w + M.off_qbf);
+    b.q8 = (uint8_t*)(w + M.off_q8);
+    b.scale = (float*)(w + M.off_scale);
+    b.n2 = (double*)(w + M.off_n2);
+    b.nrm = (float*)(w + M.off_nrm);
+    b.qerr = (float*)(w + M.off_qerr);
+    b.set_nmax = (unsigned*)(w + M.off_nmax);
+    b.set_emax = (unsigned*)(w + M.off_emax);
+    b.topd = (float*)(w + M.off_topd);
+    b.topi = (int*)(w + M.off_topi);
+    b.best = (int*)(w + M.off_best);
+    b.b1 = (double*)(w + M.off_b1);
+    b.tkey = (unsigned long long*)(w + M.off_tkey);
+    b.tq = (int*)(w + M.off_tq);
+    b.fb_list = (int*)(w + M.off_fb);
+    b.counters = (int*)(w + M.off_cnt);
+    b.out_q = (int*)(w + M.off_outq);
+    b.out_t = (int*)(w + M.off_outt);
+    b.out_n = (int*)(w + M.off_outn);
+    rc = upload_async(w + M.off_set, M.set.data(), sizeof(FSet) * M.set.size(), st);
+    if (rc) return rc;
+    rc = upload_async(w + M.off_pair, M.pair.data(), sizeof(FPair) * M.pair.size(), st);
+    if (rc) return rc;
+    if (out_offsets)
+        for (int p = 0; p < n_pair; p++) out_offsets[p] = M.pair[p].o_off;
+    hipEvent_t* ev = (hipEvent_t*)stage_events_v;
+    if (ev) FSX_HIP(hipEventRecord(ev[0], st));
+    // per-set maxima start at 0 (a fill kernel rather than a memset, so the sequence can be captured)
+    hipLaunchKernelGGL(k_f_zero_sets, dim3(1), dim3(256), 0, st, b);
+    const long long nblk = std::max<long long>(std::max(M.n_kp, M.n_targets), 1);
+    hipLaunchKernelGGL(k_f_quant, dim3((unsigned)std::min<long long>((nblk + 255) / 256, 1 << 20)), dim3(256), 0, st, b);
+    FSX_LAUNCH_CHECK();
+    if (ev) FSX_HIP(hipEventRecord(ev[1], st));
+    if (M.n_query > 0 && K > 0) {
+        const dim3 grid((unsigned)((M.max_na + 127) / 128), (unsigned)n_pair);
+        if (precision == 1 && K == 8) hipLaunchKernelGGL((k_f_screen<1, 8>), grid, dim3(256), 0, st, b);
+        if (precision == 1 && K == 2) hipLaunchKernelGGL((k_f_screen<1, 2>), grid, dim3(256), 0, st, b);
+        if (precision == 2 && K == 8) hipLaunchKernelGGL((k_f_screen<2, 8>), grid, dim3(256), 0, st, b);
+        if (precision == 2 && K == 2) hipLaunchKernelGGL((k_f_screen<2, 2>), grid, dim3(256), 0, st, b);
+        FSX_LAUNCH_CHECK();
+    }
+    if (ev) FSX_HIP(hipEventRecord(ev[2], st));
+    if (M.n_query > 0) {
+        hipLaunchKernelGGL(k_f_rescore, dim3((unsigned)((M.max_na + 31) / 32), (unsigned)n_pair), dim3(256), 0, st, b);
+        hipLaunchKernelGGL(k_f_fallback, dim3(1024), dim3(256), 0, st, b);
+        FSX_LAUNCH_CHECK();
+    }
+    if (ev) FSX_HIP(hipEventRecord(ev[3], st));
+    if (M.n_query > 0)
+        hipLaunchKernelGGL(k_f_unique2, dim3((unsigned)((M.n_query + 255) / 256)), dim3(256), 0, st, b);
+    hipLaunchKernelGGL(k_f_compact, dim3((unsigned)n_pair), dim3(1024), 0, st, b);
+    FSX_LAUNCH_CHECK();
+    if (ev) FSX_HIP(hipEventRecord(ev[4], st));
+    return FSX_OK;
+}
+
+// Results of the last fsx_fmatch_batch (synchronises): counts[n_pair], q/t at the pair offsets,
+// stats[4] = fallback queries, max |d~ - d|, mean |d~ - d|, rescored candidates.
+int fsx_fmatch_results(int n_set, const int* n_kp, int n_pair, const int* pa, const int* pb, const void* ws,
+                       int* counts, int* q, int* t, double* stats, void* stream) {
+    FPlan M;
+    int rc = fplan(n_set, n_kp, n_pair, pa, pb, M);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    const char* w = (const char*)ws;
+    int cnt[8];
+    FSX_HIP(hipMemcpyAsync(counts, w + M.off_outn, 4 * (size_t)n_pair, hipMemcpyDeviceToHost, st));
+    if (q) FSX_HIP(hipMemcpyAsync(q, w + M.off_outq, 4 * (size_t)M.n_out, hipMemcpyDeviceToHost, st));
+    if (t) FSX_HIP(hipMemcpyAsync(t, w + M.off_outt, 4 * (size_t)M.n_out, hipMemcpyDeviceToHost, st));
+    FSX_HIP(hipMemcpyAsync(cnt, w + M.off_cnt, sizeof(cnt), hipMemcpyDeviceToHost, st));
+    FSX_HIP(hipStreamSynchronize(st));
+    if (stats) {
+        double sum;
+        unsigned long long n;
+        memcpy(&sum, cnt + 2, 8);
+        memcpy(&n, cnt + 4, 8);
+        float mx;
+        memcpy(&mx, cnt + 1, 4);
+        stats[0] = cnt[0];
+        stats[1] = mx;
+        stats[2] = n ? sum / (double)n : 0.0;
+        stats[3] = (double)n;
+    }
+    return FSX_OK;
+}
+
+int fsx_fp8_layout_probe(float* out_dev, void* stream) {
+    FSX_REQUIRE(out_dev, "null pointer argument");
+    hipLaunchKernelGGL(k_fp8_probe, dim3(1), dim3(64), 0, (hipStream_t)stream, out_dev);
+    FSX_LAUNCH_CHECK();
+    return FSX_OK;
+}
+
+}  // extern "C"

@@ -284,16 +284,121 @@ def _query_state(m: "PairMatcher", stream=None):
     return best[:nq], b1[:nq], cell[:nq]
 
 
+def _is_u8_valued(d) -> bool:
+    d = np.asarray(d)
+    if d.dtype == np.uint8:
+        return True
+    return bool(np.all((d >= 0) & (d <= 255) & (d == np.rint(d))))
+
+
 def _ratio_match_bf(da, db, ratio, na=None, nb=None, form="mfma"):
-    """matching.py:86-100: exact nearest-neighbour ratio matching of all of da against db."""
+    """matching.py:86-100: exact nearest-neighbour ratio matching of all of da against db.
+    uint8-valued descriptors (RootSIFT files) take the integer path (``form`` "mfma" or
+    "plain"); other float descriptors take FloatMatcher (``form`` "bf16", "fp8", or "mfma" =
+    "bf16").  Both return the FP64-oracle match set."""
     if len(da) < 2 or len(db) < 2:
         return np.zeros(0, np.int64), np.zeros(0, np.int64)
-    a, b = as_u8(da), as_u8(db)
-    zero = lambda n: np.zeros((n, 2), np.float64)   # noqa: E731
-    m = PairMatcher([(a, zero(len(a)), (1, 1)), (b, zero(len(b)), (1, 1))], [(0, 1)], None, radius=1.0,
-                    ratio=ratio)
-    m.run(form)
-    return m.results()[0]
+    if form in ("mfma", "plain") and _is_u8_valued(da) and _is_u8_valued(db):
+        a, b = as_u8(da), as_u8(db)
+        zero = lambda n: np.zeros((n, 2), np.float64)   # noqa: E731
+        m = PairMatcher([(a, zero(len(a)), (1, 1)), (b, zero(len(b)), (1, 1))], [(0, 1)], None, radius=1.0,
+                        ratio=ratio)
+        m.run(form)
+        return m.results()[0]
+    fm = FloatMatcher([np.asarray(da, np.float32), np.asarray(db, np.float32)], [(0, 1)], ratio=ratio,
+                      precision="bf16" if form in ("mfma", "plain") else form)
+    fm.run()
+    return fm.results()[0]
+
+
+FLOAT_PRECISION = {"bf16": 1, "fp8": 2}
+
+
+class FloatMatcher:
+    """Batched brute-force ratio matching of float descriptor sets (matching.py:86-110 on float
+    descriptors) with MFMA screening in BF16 or FP8 e4m3 (per-descriptor scales), float64
+    rescoring of the K best screened targets and a certificate; queries the certificate cannot
+    decide are recomputed exactly.  The match set equals the FP64 oracle.
+
+    sets: list of float32 (n, 128) arrays or device tensors; pairs: list of (a, b)."""
+
+    def __init__(self, sets, pairs, ratio=0.8, precision="bf16", K=8, device="cuda"):
+        import torch
+        self.torch = torch
+        if precision not in FLOAT_PRECISION:
+            raise ValueError("precision must be 'bf16' or 'fp8'")
+        self.precision = precision
+        self.K = int(K)
+        self.ratio = float(ratio)
+        self.pairs = [(int(a), int(b)) for a, b in pairs]
+        self.n_pair = len(self.pairs)
+        ds = []
+        for d in sets:
+            d = d if isinstance(d, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(d, np.float32))
+            if d.dtype != torch.float32:
+                raise TypeError("descriptors must be float32")
+            ds.append(d.reshape(-1, 128).to(device))
+        self.nk = [int(d.shape[0]) for d in ds]
+        self.desc = torch.cat(ds).contiguous() if sum(self.nk) else torch.zeros((1, 128), dtype=torch.float32,
+                                                                                 device=device)
+        n = len(ds)
+        npair = max(self.n_pair, 1)
+        self._nk = (C.c_int * n)(*self.nk)
+        self._pa = (C.c_int * npair)(*[a for a, _ in self.pairs])
+        self._pb = (C.c_int * npair)(*[b for _, b in self.pairs])
+        self.n_set = n
+        nb = C.c_size_t()
+        cap = C.c_longlong()
+        _lib.call("fsx_fmatch_workspace_bytes", n, self._nk, self.n_pair, self._pa, self._pb, C.byref(nb),
+                  C.byref(cap))
+        self.ws = torch.empty(nb.value, dtype=torch.uint8, device=device)
+        self.capacity = cap.value
+        self._offsets = (C.c_longlong * npair)()
+
+    def run(self, events=None, stream=None):
+        """Enqueue the batch (asynchronous).  events: 5 timing events (quantize / screen /
+        rescore + fallback / unique + compact)."""
+        P = C.c_void_p
+        ev = None
+        if events is not None:
+            for e in events:
+                if not e.cuda_event:
+                    e.record()
+            ev = (P * 5)(*[e.cuda_event for e in events])
+        _lib.call("fsx_fmatch_batch", self.n_set, self._nk, P(self.desc.data_ptr()), self.n_pair, self._pa,
+                  self._pb, self.ratio, FLOAT_PRECISION[self.precision], self.K, P(self.ws.data_ptr()),
+                  self.ws.numel(), self._offsets, ev, P(_lib.stream_handle(stream)))
+
+    def results(self, stream=None):
+        """List of (q int64, t int64) per pair, sorted by t (synchronises)."""
+        npair = max(self.n_pair, 1)
+        counts = (C.c_int * npair)()
+        q = np.zeros(max(self.capacity, 1), np.int32)
+        t = np.zeros(max(self.capacity, 1), np.int32)
+        st = (C.c_double * 4)()
+        _lib.call("fsx_fmatch_results", self.n_set, self._nk, self.n_pair, self._pa, self._pb,
+                  C.c_void_p(self.ws.data_ptr()), counts, q.ctypes.data_as(C.POINTER(C.c_int)),
+                  t.ctypes.data_as(C.POINTER(C.c_int)), st, C.c_void_p(_lib.stream_handle(stream)))
+        self._stats = {"fallback_queries": int(st[0]), "max_abs_distance_error": float(st[1]),
+                       "mean_abs_distance_error": float(st[2]), "rescored_candidates": int(st[3])}
+        out = []
+        for p in range(self.n_pair):
+            o = self._offsets[p]
+            out.append((q[o:o + counts[p]].astype(np.int64), t[o:o + counts[p]].astype(np.int64)))
+        return out
+
+    def stats(self):
+        """Screening error against float64 over the rescored candidates and the number of
+        queries decided by the exact fallback (of the last results() call)."""
+        return dict(self._stats)
+
+
+def fp8_layout_probe():
+    """Run the 32x32x64 FP8 MFMA layout probe; returns (64 lanes, 16) float32."""
+    import torch
+    out = torch.zeros(64 * 16, dtype=torch.float32, device="cuda")
+    _lib.call("fsx_fp8_layout_probe", C.c_void_p(out.data_ptr()), C.c_void_p(_lib.stream_handle()))
+    return out.view(64, 16).cpu().numpy()
 
 
 def _guided(A, B, Hm, radius, kc, ratio, form="mfma"):

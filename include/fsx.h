@@ -163,6 +163,38 @@ int fsx_mfma_layout_probe(float *out_dev, void *stream);
 int fsx_mfma_i8_layout_probe(int *out_dev, void *stream);
 
 /* ---------------------------------------------------------------------------------------
+ * Float-descriptor brute-force matcher (matching.py:86-110 on float descriptors)
+ * ------------------------------------------------------------------------------------- */
+
+/* Workspace bytes for n_pair brute-force pairs (pa[p] -> pb[p]) over n_set float descriptor
+ * sets of n_kp[i] descriptors; out_capacity: sum of min(n_a, n_b). */
+int fsx_fmatch_workspace_bytes(int n_set, const int *n_kp, int n_pair, const int *pa, const int *pb, size_t *bytes,
+                               long long *out_capacity);
+
+/* _ratio_match_bf + _unique (matching.py:86-110) for every pair, on float32 descriptors desc_dev
+ * (all sets concatenated, n x 128).  Screening on MFMA with FP32 accumulation in precision 1 =
+ * BF16 or 2 = FP8 e4m3 with per-descriptor scales, the K (2 or 8) best screened targets per
+ * query rescored in float64, a certificate from a bound on the screening error, and an exact
+ * float64 pass for the queries it cannot decide (K = 0: the exact pass for every query): the result is the FP64-oracle match set
+ * (distance sum_k (a_k - b_k)^2 in float64, best by index on ties, sqrt(b1) < ratio*sqrt(b2),
+ * one query per target).  out_offsets (host, n_pair): output slot of each pair.
+ * stage_events (may be NULL): 5 caller hipEvent_t.  Asynchronous. */
+int fsx_fmatch_batch(int n_set, const int *n_kp, const float *desc_dev, int n_pair, const int *pa, const int *pb,
+                     double ratio, int precision, int K, void *ws_dev, size_t ws_bytes, long long *out_offsets,
+                     void *stage_events, void *stream);
+
+/* Results of the last fsx_fmatch_batch (synchronises): counts[n_pair], q / t (int32) at the pair
+ * offsets, stats[4] = queries decided by the exact pass, max and mean |screened - float64|
+ * distance over the rescored candidates, number of rescored candidates. */
+int fsx_fmatch_results(int n_set, const int *n_kp, int n_pair, const int *pa, const int *pb, const void *ws_dev,
+                       int *counts, int *q, int *t, double *stats, void *stream);
+
+/* Writes the 64 x 16 accumulator of one 32x32x64 FP8 (e4m3) MFMA with unit scales on fixed
+ * small operands, to check the format and lane layout the float matcher assumes (no
+ * reference counterpart). */
+int fsx_fp8_layout_probe(float *out_dev, void *stream);
+
+/* ---------------------------------------------------------------------------------------
  * Keypoint geometry around the matcher
  * ------------------------------------------------------------------------------------- */
 
