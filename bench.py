@@ -9,8 +9,10 @@ ratio 0.8, one query per target.
 
 Prints ONE JSON line.  ``value`` = SIFT images/s with the photos resident in HBM;
 ``matching`` holds matched pairs/s of the fast and the plain FP32 GPU forms and whether the
-match set equals the FP64 oracle; ``roofline`` is for the dominant kernel (the pyramid's
-LDS strip blur, HBM-bound); ``cpu_baseline`` times the CPU oracle on a bounded sample.
+match set equals the FP64 oracle; ``precision`` holds the float-descriptor brute-force section
+(200 pairs, BF16 and FP8 screening, distance error against float64, match-set equality);
+``roofline`` is for the dominant kernel (the pyramid's LDS strip blur); ``cpu_baseline`` times
+the CPU oracle on a bounded sample.
 """
 from __future__ import annotations
 
@@ -157,6 +159,72 @@ def bench_matching(fb, fl, shapes, args):
     return out
 
 
+def bench_precision(args):
+    """Float-descriptor brute-force matching of 200 pairs, n ~ U[2000, 12000] per set (SIFT-like
+    descriptors, half of the smaller set planted as perturbed copies): BF16 and FP8 screening
+    with the float64 certificate.  Reference: the exact float64 GPU pass (K = 0) on every pair,
+    which the tests pin to the FP64 oracle, plus the CPU FP64 oracle on the two smallest pairs."""
+    import torch
+    from fieldstitch_amd import synth
+    from fieldstitch_amd.matching import FloatMatcher
+    from oracle import match_oracle as mo
+    rng = np.random.default_rng(3)
+    n_pair = args.precision_pairs
+    sizes = rng.integers(2000, 12001, (n_pair, 2))
+    t0 = time.time()
+    sets = []
+    for na, nb in sizes:
+        da, db, _ = synth.descriptor_pair(rng, int(na), int(nb))
+        sets += [da, db]
+    log("precision: %d pairs generated in %.1f s" % (n_pair, time.time() - t0))
+    pairs = [(2 * p, 2 * p + 1) for p in range(n_pair)]
+    flops = 2.0 * 128 * float(sum(int(a) * int(b) for a, b in sizes))
+    out = {"pairs": n_pair, "descriptors": int(sizes.sum()), "algorithmic_tflop": flops / 1e12}
+    ref = FloatMatcher(sets, pairs, ratio=RATIO, precision="bf16", K=0)
+    ref.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ref.run()
+    ref_res = ref.results()
+    out["fp64_exact_ms"] = (time.perf_counter() - t0) * 1e3
+    peaks = {"bf16": 2.5e15, "fp8": 5.0e15}
+    for prec in ("bf16", "fp8"):
+        fm = FloatMatcher(sets, pairs, ratio=RATIO, precision=prec, K=8, device="cuda")
+        fm.desc = ref.desc                       # same device descriptors
+        for _ in range(max(1, args.warmup)):
+            fm.run()
+        torch.cuda.synchronize()
+        steps = max(1, args.steps // 2)
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(steps)]
+        t0 = time.perf_counter()
+        for k in range(steps):
+            fm.run(events=evs[k])
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / steps
+        st = [float(np.mean([evs[k][i].elapsed_time(evs[k][i + 1]) for k in range(steps)])) for i in range(4)]
+        res = fm.results()
+        stats = fm.stats()
+        same = all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) for a, b in zip(res, ref_res))
+        out[prec] = {"ms_per_batch": t * 1e3, "pairs_per_s": n_pair / t,
+                     "stage_ms": {"quantize": st[0], "screen": st[1], "rescore_fallback": st[2], "unique": st[3]},
+                     "screen_tflops": flops / (st[1] / 1e3) / 1e12,
+                     "mfma_roofline_frac": flops / (st[1] / 1e3) / peaks[prec],
+                     "max_abs_distance_error": stats["max_abs_distance_error"],
+                     "mean_abs_distance_error": stats["mean_abs_distance_error"],
+                     "fallback_queries": stats["fallback_queries"],
+                     "match_set_equals_fp64": bool(same)}
+    # CPU FP64 oracle on the two smallest pairs
+    small = np.argsort(sizes[:, 0] * sizes[:, 1])[:2]
+    ok = True
+    for p in small:
+        q, t, _ = mo.match_fp64(sets[2 * p], sets[2 * p + 1], RATIO)
+        ok &= bool(np.array_equal(q, ref_res[p][0]) and np.array_equal(t, ref_res[p][1]))
+    out["cpu_fp64_oracle_pairs_checked"] = [int(p) for p in small]
+    out["cpu_fp64_oracle_equal"] = ok
+    out["matches_total"] = int(sum(len(r[0]) for r in ref_res))
+    return out
+
+
 def copy_probe(mib):
     """Three device-to-device copies of `mib` MiB: a kernel with known HBM traffic (read mib,
     write mib per dispatch) whose FETCH_SIZE / WRITE_SIZE readings scale the counters of the
@@ -186,6 +254,8 @@ def main():
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_k_blur_8x2640x1978.json"))
     ap.add_argument("--gen-workers", type=int, default=8,
                     help="processes rendering the synthetic photos (1 under a profiler)")
+    ap.add_argument("--no-precision", action="store_true", help="skip the float-descriptor precision section")
+    ap.add_argument("--precision-pairs", type=int, default=200)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 3 plain device copies of MIB MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -265,6 +335,8 @@ def main():
                      "algorithmic_bytes_per_step": pyr_b, "time_ms_per_step": stages["pyramid"]},
     }
     res["matching"] = bench_matching(fb, fl, shapes, args)
+    if not args.no_precision:
+        res["precision"] = bench_precision(args)
     if not args.no_cpu_baseline:
         n_proc = min(args.cpu_procs, len(fl.images), os.cpu_count() or 1)
         log("cpu baseline: %d processes" % n_proc)

@@ -69,6 +69,8 @@ struct FBatch {
     int* fb_list;                 // [n_query] uncertain queries (global index)
     int* counters;                // [0] fallback count, [1] max |d~ - d| (float bits),
                                   // [2..3] sum |d~ - d| (double), [4..5] compared count (u64)
+    double* stat_part;            // [n_stat][3] per rescore block: max, sum, count
+    long long n_stat;
     int* out_q;
     int* out_t;
     int* out_n;
@@ -136,75 +138,96 @@ __device__ __forceinline__ double e4m3_value(unsigned c) {
 // ------------------------------------------------------------------------------------------
 // Quantize + resets
 
-__global__ __launch_bounds__(256) void k_f_zero_sets(const FBatch B) {
-    for (int s = threadIdx.x; s < B.n_set; s += 256) {
-        B.set_nmax[s] = 0u;
-        B.set_emax[s] = 0u;
-    }
+// 32 lanes per descriptor: lane j holds elements 4j .. 4j+3 (coalesced 16-byte loads, 8- or
+// 4-byte stores); norms, maxima and error sums reduce across the 32 lanes
+__device__ __forceinline__ double red32_add(double v) {
+#pragma unroll
+    for (int off = 16; off >= 1; off >>= 1) v += __shfl_xor(v, off, 32);
+    return v;
 }
 
 __global__ __launch_bounds__(256) void k_f_quant(const FBatch B) {
-    const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+    const long long tid = (long long)blockIdx.x * 256 + threadIdx.x;
     const long long stride = (long long)gridDim.x * 256;
-    for (long long t = k; t < B.n_targets; t += stride) {
+    for (long long t = tid; t < B.n_targets; t += stride) {
         B.tkey[t] = ~0ull;
         B.tq[t] = 0x7fffffff;
     }
-    if (k < 8) B.counters[k] = 0;
-    if (k >= B.n_kp) return;
-    const float4* a4 = (const float4*)(B.desc + k * FD);
-    float a[FD];
+    if (tid < 8) B.counters[tid] = 0;
+    const int j = threadIdx.x & 31;
+    for (long long k = tid >> 5; k < B.n_kp; k += stride >> 5) {
+        const float4 v = ((const float4*)(B.desc + k * FD))[j];
+        const float x[4] = {v.x, v.y, v.z, v.w};
+        double n2 = 0.0;
+        float amax = 0.0f;
 #pragma unroll
-    for (int i = 0; i < FD / 4; i++) {
-        const float4 v = a4[i];
-        a[4 * i] = v.x;
-        a[4 * i + 1] = v.y;
-        a[4 * i + 2] = v.z;
-        a[4 * i + 3] = v.w;
-    }
-    double n2 = 0.0, e2 = 0.0;
-    float amax = 0.0f;
-#pragma unroll
-    for (int i = 0; i < FD; i++) {
-        n2 += (double)a[i] * (double)a[i];
-        amax = fmaxf(amax, fabsf(a[i]));
-    }
-    float sc = 1.0f;
-    if (B.prec == 1) {
-        unsigned short* dst = B.qbf + k * FD;
-#pragma unroll
-        for (int i = 0; i < FD; i++) {
-            const unsigned short h = f2bf(a[i]);
-            dst[i] = h;
-            const double d = (double)a[i] - (double)bf2f(h);
-            e2 += d * d;
+        for (int u = 0; u < 4; u++) {
+            n2 += (double)x[u] * (double)x[u];
+            amax = fmaxf(amax, fabsf(x[u]));
         }
-    } else {
-        sc = amax > 0.0f ? amax / 448.0f : 1.0f;
-        uint8_t* dst = B.q8 + k * FD;
+        n2 = red32_add(n2);
 #pragma unroll
-        for (int i = 0; i < FD; i += 4) {
+        for (int off = 16; off >= 1; off >>= 1) amax = fmaxf(amax, __shfl_xor(amax, off, 32));
+        double e2 = 0.0;
+        float sc = 1.0f;
+        if (B.prec == 1) {
+            unsigned short h[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                h[u] = f2bf(x[u]);
+                const double d = (double)x[u] - (double)bf2f(h[u]);
+                e2 += d * d;
+            }
+            ((uint2*)(B.qbf + k * FD))[j] = make_uint2((unsigned)h[0] | ((unsigned)h[1] << 16),
+                                                       (unsigned)h[2] | ((unsigned)h[3] << 16));
+        } else {
+            sc = amax > 0.0f ? amax / 448.0f : 1.0f;
             unsigned w = 0;
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                const unsigned c = f2e4m3(fminf(fmaxf(a[i + u] / sc, -448.0f), 448.0f));
+                const unsigned c = f2e4m3(fminf(fmaxf(x[u] / sc, -448.0f), 448.0f));
                 w |= c << (8 * u);
-                const double d = (double)a[i + u] - (double)sc * e4m3_value(c);
+                const double d = (double)x[u] - (double)sc * e4m3_value(c);
                 e2 += d * d;
             }
-            *(unsigned*)(dst + i) = w;
+            ((unsigned*)(B.q8 + k * FD))[j] = w;
+        }
+        e2 = red32_add(e2);
+        if (j == 0) {
+            B.scale[k] = sc;
+            B.n2[k] = n2;
+            // upper bounds as float32 (the float64 sums' rounding is far below the margin)
+            const float nr = __double2float_ru(sqrt(n2) * (1.0 + 1e-12));
+            const float er = __double2float_ru(sqrt(e2) * (1.0 + 1e-12));
+            B.nrm[k] = nr;
+            B.qerr[k] = er;
         }
     }
-    B.scale[k] = sc;
-    B.n2[k] = n2;
-    // upper bounds as float32 (rounded up by one ulp margin)
-    const float nr = __double2float_ru(sqrt(n2) * (1.0 + 1e-12));
-    const float er = __double2float_ru(sqrt(e2) * (1.0 + 1e-12));
-    B.nrm[k] = nr;
-    B.qerr[k] = er;
-    const int s = fset_of(B, k);
-    atomicMax(&B.set_nmax[s], __float_as_uint(nr));     // non-negative floats order as uints
-    atomicMax(&B.set_emax[s], __float_as_uint(er));
+}
+
+// per set: max |b| and max quantization error (one block per set, no contended atomics)
+__global__ __launch_bounds__(256) void k_f_setmax(const FBatch B) {
+    __shared__ float s_n[4], s_e[4];
+    const FSet S = B.set[blockIdx.x];
+    float mn = 0.0f, me = 0.0f;
+    for (int i = threadIdx.x; i < S.n; i += 256) {
+        mn = fmaxf(mn, B.nrm[S.off + i]);
+        me = fmaxf(me, B.qerr[S.off + i]);
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        mn = fmaxf(mn, __shfl_xor(mn, off));
+        me = fmaxf(me, __shfl_xor(me, off));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        s_n[threadIdx.x >> 6] = mn;
+        s_e[threadIdx.x >> 6] = me;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        B.set_nmax[blockIdx.x] = __float_as_uint(fmaxf(fmaxf(s_n[0], s_n[1]), fmaxf(s_n[2], s_n[3])));
+        B.set_emax[blockIdx.x] = __float_as_uint(fmaxf(fmaxf(s_e[0], s_e[1]), fmaxf(s_e[2], s_e[3])));
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -237,10 +260,12 @@ __device__ __forceinline__ void topk_push(float d, int t, float* td, int* ti) {
 template <int PREC, int K>
 __global__ __launch_bounds__(256) void k_f_screen(const FBatch B) {
     constexpr int ROWB = PREC == 1 ? FD * 2 : FD;                 // bytes per staged row
-    __shared__ __attribute__((aligned(16))) uint8_t s_rows[FS_TILE * ROWB];
+    // the tile rows, reused for the final half-wave merge (256 * K (float, int) pairs)
+    constexpr int SBYTES = FS_TILE * ROWB > 256 * K * 8 ? FS_TILE * ROWB : 256 * K * 8;
+    __shared__ __attribute__((aligned(16))) uint8_t s_rows[SBYTES];
     __shared__ float s_cn2[FS_TILE], s_cs[FS_TILE];
-    __shared__ float s_md[256 * K];
-    __shared__ int s_mi[256 * K];
+    float* s_md = (float*)s_rows;
+    int* s_mi = (int*)(s_rows + 256 * K * 4);
     const FPair P = B.pair[blockIdx.y];
     const FSet sa = B.set[P.a], sb = B.set[P.b];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -398,35 +423,56 @@ __global__ __launch_bounds__(256) void k_f_rescore(const FBatch B) {
     const FSet sa = B.set[P.a], sb = B.set[P.b];
     const int tid = threadIdx.x, sub = tid & 7;
     const int q = blockIdx.x * 32 + (tid >> 3);
-    if (q >= sa.n) return;
-    const long long g = P.q_off + q;
-    if (sa.n < 2 || sb.n < 2) {
-        if (sub == 0) B.best[g] = -1;
-        return;
-    }
+    const bool inq = q < sa.n;
+    const long long g = P.q_off + (inq ? q : 0);
+    const bool live = inq && sa.n >= 2 && sb.n >= 2;
+    if (inq && !live && sub == 0) B.best[g] = -1;
     const int K = B.K;
-    const long long ka = sa.off + q;
     if (K == 0) {                           // exact mode: every query takes the float64 pass
-        if (sub == 0) {
+        if (live && sub == 0) {
             B.best[g] = -1;
             B.fb_list[atomicAdd(&B.counters[0], 1)] = (int)g;
         }
         return;
     }
-    const int kk = min(K, sb.n);
+    const long long ka = sa.off + (inq ? q : 0);
+    const int kk = min(K, max(sb.n, 1));
     double dv = INFINITY;
     int iv = 0x7fffffff;
     float dt = INFINITY;
-    if (sub < kk) {
+    float err = 0.0f;
+    if (live && sub < kk) {
         iv = B.topi[g * FK_MAX + sub];
         dt = B.topd[g * FK_MAX + sub];
         dv = dist64(B.desc + ka * FD, B.desc + (sb.off + iv) * FD);
-        const float err = (float)fabs((double)dt - dv);
-        atomicMax((unsigned*)&B.counters[1], __float_as_uint(err));
-        atomicAdd((double*)&B.counters[2], (double)err);
-        atomicAdd((unsigned long long*)&B.counters[4], 1ull);
+        err = (float)fabs((double)dt - dv);
     }
-    // gather the group's exact values (lanes of one group are consecutive)
+    // screening-error statistics: block partials (same-address atomics would serialise)
+    {
+        __shared__ double s_st[4][3];
+        float emax = err;
+        double esum = err;
+        int ecnt = (live && sub < kk) ? 1 : 0;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            emax = fmaxf(emax, __shfl_xor(emax, off));
+            esum += __shfl_xor(esum, off);
+            ecnt += __shfl_xor(ecnt, off);
+        }
+        if ((tid & 63) == 0) {
+            s_st[tid >> 6][0] = emax;
+            s_st[tid >> 6][1] = esum;
+            s_st[tid >> 6][2] = ecnt;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double* o = B.stat_part + 3 * ((long long)blockIdx.y * gridDim.x + blockIdx.x);
+            o[0] = fmax(fmax(s_st[0][0], s_st[1][0]), fmax(s_st[2][0], s_st[3][0]));
+            o[1] = (s_st[0][1] + s_st[1][1]) + (s_st[2][1] + s_st[3][1]);
+            o[2] = (s_st[0][2] + s_st[1][2]) + (s_st[2][2] + s_st[3][2]);
+        }
+    }
+    // gather the group's exact values (the 8 lanes of a group are consecutive)
     double d[FK_MAX];
     int id[FK_MAX];
 #pragma unroll
@@ -435,7 +481,7 @@ __global__ __launch_bounds__(256) void k_f_rescore(const FBatch B) {
         id[j] = __shfl(iv, (tid & ~7) + j);
     }
     const float dK = __shfl(dt, (tid & ~7) + (kk - 1));
-    if (sub != 0) return;
+    if (!live || sub != 0) return;
     // exact top-2 among the candidates
     double b1 = INFINITY, b2 = INFINITY;
     int i1 = 0x7fffffff;
@@ -518,6 +564,40 @@ __global__ __launch_bounds__(256) void k_f_fallback(const FBatch B) {
     }
 }
 
+// statistics partials of the rescore blocks -> counters[1..5]
+__global__ __launch_bounds__(1024) void k_f_stats(const FBatch B) {
+    __shared__ double s_m[16], s_s[16], s_c[16];
+    double m = 0.0, sm = 0.0, c = 0.0;
+    for (long long i = threadIdx.x; i < B.n_stat; i += 1024) {
+        m = fmax(m, B.stat_part[3 * i]);
+        sm += B.stat_part[3 * i + 1];
+        c += B.stat_part[3 * i + 2];
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        m = fmax(m, __shfl_xor(m, off));
+        sm += __shfl_xor(sm, off);
+        c += __shfl_xor(c, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        s_m[threadIdx.x >> 6] = m;
+        s_s[threadIdx.x >> 6] = sm;
+        s_c[threadIdx.x >> 6] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 16; w++) {
+            m = fmax(m, s_m[w]);
+            sm += s_s[w];
+            c += s_c[w];
+        }
+        const float mf = (float)m;
+        B.counters[1] = (int)__float_as_uint(mf);
+        *(double*)&B.counters[2] = sm;
+        *(unsigned long long*)&B.counters[4] = (unsigned long long)c;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_f_unique2(const FBatch B) {
     const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
     if (g >= B.n_query) return;
@@ -594,9 +674,9 @@ static inline size_t fa256(size_t x) { return (x + 255) & ~(size_t)255; }
 struct FPlan {
     std::vector<FSet> set;
     std::vector<FPair> pair;
-    long long n_kp = 0, n_query = 0, n_targets = 0, n_out = 0;
+    long long n_kp = 0, n_query = 0, n_targets = 0, n_out = 0, n_stat = 0;
     int max_na = 0;
-    size_t off_set, off_pair, off_qbf, off_q8, off_scale, off_n2, off_nrm, off_qerr, off_nmax, off_emax, off_topd,
+    size_t off_stat, off_set, off_pair, off_qbf, off_q8, off_scale, off_n2, off_nrm, off_qerr, off_nmax, off_emax, off_topd,
         off_topi, off_best, off_b1, off_tkey, off_tq, off_fb, off_cnt, off_outq, off_outt, off_outn, total;
 };
 
@@ -653,6 +733,8 @@ static int fplan(int n_set, const int* n_kp, int n_pair, const int* pa, const in
     M.off_outq = o; o = fa256(o + 4 * (size_t)std::max(oo, 1LL));
     M.off_outt = o; o = fa256(o + 4 * (size_t)std::max(oo, 1LL));
     M.off_outn = o; o = fa256(o + 4 * (size_t)n_pair);
+    M.n_stat = (long long)((M.max_na + 31) / 32) * n_pair;
+    M.off_stat = o; o = fa256(o + 24 * (size_t)std::max(M.n_stat, 1LL));
     M.total = o;
     return FSX_OK;
 }
@@ -719,6 +801,8 @@ int fsx_fmatch_batch(int n_set, const int* n_kp, const float* desc_dev, int n_pa
     b.out_q = (int*)(w + M.off_outq);
     b.out_t = (int*)(w + M.off_outt);
     b.out_n = (int*)(w + M.off_outn);
+    b.stat_part = (double*)(w + M.off_stat);
+    b.n_stat = M.n_stat;
     rc = upload_async(w + M.off_set, M.set.data(), sizeof(FSet) * M.set.size(), st);
     if (rc) return rc;
     rc = upload_async(w + M.off_pair, M.pair.data(), sizeof(FPair) * M.pair.size(), st);
@@ -727,10 +811,9 @@ int fsx_fmatch_batch(int n_set, const int* n_kp, const float* desc_dev, int n_pa
         for (int p = 0; p < n_pair; p++) out_offsets[p] = M.pair[p].o_off;
     hipEvent_t* ev = (hipEvent_t*)stage_events_v;
     if (ev) FSX_HIP(hipEventRecord(ev[0], st));
-    // per-set maxima start at 0 (a fill kernel rather than a memset, so the sequence can be captured)
-    hipLaunchKernelGGL(k_f_zero_sets, dim3(1), dim3(256), 0, st, b);
-    const long long nblk = std::max<long long>(std::max(M.n_kp, M.n_targets), 1);
-    hipLaunchKernelGGL(k_f_quant, dim3((unsigned)std::min<long long>((nblk + 255) / 256, 1 << 20)), dim3(256), 0, st, b);
+    const long long nthr = std::max<long long>(std::max(32 * M.n_kp, M.n_targets), 1);
+    hipLaunchKernelGGL(k_f_quant, dim3((unsigned)std::min<long long>((nthr + 255) / 256, 1 << 16)), dim3(256), 0, st, b);
+    hipLaunchKernelGGL(k_f_setmax, dim3((unsigned)n_set), dim3(256), 0, st, b);
     FSX_LAUNCH_CHECK();
     if (ev) FSX_HIP(hipEventRecord(ev[1], st));
     if (M.n_query > 0 && K > 0) {
@@ -744,6 +827,7 @@ int fsx_fmatch_batch(int n_set, const int* n_kp, const float* desc_dev, int n_pa
     if (ev) FSX_HIP(hipEventRecord(ev[2], st));
     if (M.n_query > 0) {
         hipLaunchKernelGGL(k_f_rescore, dim3((unsigned)((M.max_na + 31) / 32), (unsigned)n_pair), dim3(256), 0, st, b);
+        if (K > 0) hipLaunchKernelGGL(k_f_stats, dim3(1), dim3(1024), 0, st, b);
         hipLaunchKernelGGL(k_f_fallback, dim3(1024), dim3(256), 0, st, b);
         FSX_LAUNCH_CHECK();
     }
