@@ -32,6 +32,14 @@ SIGNATURES = {
     "fsx_sift_features_batch": [_i, C.POINTER(_vp), C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.c_double,
                                 C.c_double, _i, _i, _i, _i, C.POINTER(_vp), C.POINTER(_sz), C.POINTER(_vp), _vp,
                                 C.POINTER(_vp), C.POINTER(_vp), _vp, _sz, C.POINTER(_vp), _vp],
+    "fsx_sift_features_prepare": [_i, C.POINTER(_vp), C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.c_double,
+                                  C.c_double, _i, _i, _i, _i, C.POINTER(_vp), C.POINTER(_sz), C.POINTER(_vp), _vp,
+                                  C.POINTER(_vp), C.POINTER(_vp), _vp, _sz, _vp],
+    "fsx_sift_features_launch": [_i, C.POINTER(_vp), C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.c_double,
+                                 C.c_double, _i, _i, _i, _i, C.POINTER(_vp), C.POINTER(_sz), C.POINTER(_vp), _vp,
+                                 C.POINTER(_vp), C.POINTER(_vp), _vp, _sz, C.POINTER(_vp), _vp],
+    "fsx_tune_set": [_i, _i],
+    "fsx_tune_get": [_i, C.POINTER(_i)],
     "fsx_sift_selection_offset": [_i, _i, _i, _i, C.POINTER(_sz)],
     "fsx_match_workspace_bytes": [_i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), _i, C.POINTER(_i), C.POINTER(_i),
                                   C.c_double, C.POINTER(_sz), C.POINTER(C.c_longlong)],

@@ -7,6 +7,10 @@ namespace fsx {
 
 static thread_local char g_err[1024];
 
+// tunable launch shapes: orientation blocks / image, refinement blocks / image, float-screen tile
+static int g_tune[TUNE_N_KEYS] = {4096, 256, 64};
+int tune_get(int key) { return g_tune[key]; }
+
 void set_error(const char* fmt, ...) {
     va_list ap;
     va_start(ap, fmt);
@@ -67,6 +71,23 @@ int fsx_last_error(char* buf, size_t len) {
         buf[c] = 0;
     }
     return (int)n;
+}
+
+int fsx_tune_set(int key, int value) {
+    using namespace fsx;
+    FSX_REQUIRE(key >= 0 && key < TUNE_N_KEYS, "unknown tuning key %d", key);
+    if (key == TUNE_ORIENT_BLOCKS || key == TUNE_REFINE_BLOCKS)
+        FSX_REQUIRE(value >= 16 && value <= 65536, "blocks per image must be in [16, 65536] (got %d)", value);
+    if (key == TUNE_FSCREEN_TILE) FSX_REQUIRE(value == 64 || value == 128, "tile must be 64 or 128 (got %d)", value);
+    g_tune[key] = value;
+    return FSX_OK;
+}
+
+int fsx_tune_get(int key, int* value) {
+    using namespace fsx;
+    FSX_REQUIRE(key >= 0 && key < TUNE_N_KEYS && value, "unknown tuning key %d", key);
+    *value = g_tune[key];
+    return FSX_OK;
 }
 
 int fsx_device_info(int device, char* name, size_t len, int* n_cu, long long* hbm_bytes) {

@@ -237,7 +237,6 @@ typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
 typedef int i32x8_t __attribute__((ext_vector_type(8)));
 
-constexpr int FS_TILE = 64;
 
 template <int K>
 __device__ __forceinline__ void topk_push(float d, int t, float* td, int* ti) {
@@ -257,7 +256,7 @@ __device__ __forceinline__ void topk_push(float d, int t, float* td, int* ti) {
     }
 }
 
-template <int PREC, int K>
+template <int PREC, int K, int FS_TILE>
 __global__ __launch_bounds__(256) void k_f_screen(const FBatch B) {
     constexpr int ROWB = PREC == 1 ? FD * 2 : FD;                 // bytes per staged row
     // the tile rows, reused for the final half-wave merge (256 * K (float, int) pairs)
@@ -313,10 +312,10 @@ __global__ __launch_bounds__(256) void k_f_screen(const FBatch B) {
             }
             ((uint4*)s_rows)[u] = v;
         }
-        if (tid < FS_TILE) {
-            const bool ok = tid < nt;
-            s_cn2[tid] = ok ? (float)B.n2[sb.off + base + tid] : 0.0f;
-            s_cs[tid] = ok ? B.scale[sb.off + base + tid] : 0.0f;
+        for (int u = tid; u < FS_TILE; u += 256) {
+            const bool ok = u < nt;
+            s_cn2[u] = ok ? (float)B.n2[sb.off + base + u] : 0.0f;
+            s_cs[u] = ok ? B.scale[sb.off + base + u] : 0.0f;
         }
         __syncthreads();
 #pragma unroll
@@ -818,10 +817,17 @@ int fsx_fmatch_batch(int n_set, const int* n_kp, const float* desc_dev, int n_pa
     if (ev) FSX_HIP(hipEventRecord(ev[1], st));
     if (M.n_query > 0 && K > 0) {
         const dim3 grid((unsigned)((M.max_na + 127) / 128), (unsigned)n_pair);
-        if (precision == 1 && K == 8) hipLaunchKernelGGL((k_f_screen<1, 8>), grid, dim3(256), 0, st, b);
-        if (precision == 1 && K == 2) hipLaunchKernelGGL((k_f_screen<1, 2>), grid, dim3(256), 0, st, b);
-        if (precision == 2 && K == 8) hipLaunchKernelGGL((k_f_screen<2, 8>), grid, dim3(256), 0, st, b);
-        if (precision == 2 && K == 2) hipLaunchKernelGGL((k_f_screen<2, 2>), grid, dim3(256), 0, st, b);
+        const bool t128 = tune_get(TUNE_FSCREEN_TILE) == 128;
+#define FSX_SCREEN(P_, K_)                                                                            \
+    if (precision == P_ && K == K_) {                                                                 \
+        if (t128) hipLaunchKernelGGL((k_f_screen<P_, K_, 128>), grid, dim3(256), 0, st, b);           \
+        else hipLaunchKernelGGL((k_f_screen<P_, K_, 64>), grid, dim3(256), 0, st, b);                 \
+    }
+        FSX_SCREEN(1, 8)
+        FSX_SCREEN(1, 2)
+        FSX_SCREEN(2, 8)
+        FSX_SCREEN(2, 2)
+#undef FSX_SCREEN
         FSX_LAUNCH_CHECK();
     }
     if (ev) FSX_HIP(hipEventRecord(ev[2], st));

@@ -22,6 +22,15 @@ void set_error(const char* fmt, ...);
 // reused only after the copy that last read it has completed (event wait).
 int upload_async(void* dst_dev, const void* src, size_t n, hipStream_t st);
 
+// Launch-shape tunables (process-wide; set by the autotuner through fsx_tune_set)
+enum TuneKey {
+    TUNE_ORIENT_BLOCKS = 0,    // orientation blocks (one wave each) per image
+    TUNE_REFINE_BLOCKS = 1,    // refinement blocks per image
+    TUNE_FSCREEN_TILE = 2,     // float matcher: targets per LDS tile (64 or 128)
+    TUNE_N_KEYS = 3
+};
+int tune_get(int key);
+
 #define FSX_HIP(call)                                                                \
     do {                                                                             \
         hipError_t e_ = (call);                                                      \

@@ -1273,10 +1273,10 @@ static int run_pipeline(const SiftJob* all, int n_all, const Group* groups, int 
         }
     }
     if (ev) FSX_HIP(hipEventRecord(ev[2], st));
-    hipLaunchKernelGGL(k_refine, dim3(256, 1, n_all), dim3(256), 0, st, all);
+    hipLaunchKernelGGL(k_refine, dim3(tune_get(TUNE_REFINE_BLOCKS), 1, n_all), dim3(256), 0, st, all);
     FSX_LAUNCH_CHECK();
     if (ev) FSX_HIP(hipEventRecord(ev[3], st));
-    hipLaunchKernelGGL(k_orient, dim3(4096, 1, n_all), dim3(64), 0, st, all);
+    hipLaunchKernelGGL(k_orient, dim3(tune_get(TUNE_ORIENT_BLOCKS), 1, n_all), dim3(64), 0, st, all);
     FSX_LAUNCH_CHECK();
     if (ev) FSX_HIP(hipEventRecord(ev[4], st));
     hipLaunchKernelGGL(k_sort_local, dim3(kp_cap / 1024, 1, n_all), dim3(256), 0, st, all);
@@ -1412,26 +1412,30 @@ int fsx_sift_table_bytes(int n_img, size_t* bytes) {
     return FSX_OK;
 }
 
-int fsx_sift_features_batch(int n_img, const uint8_t* const* imgs, const int* H, const int* W, const int* ld,
-                            double contrast, double edge, int max_kp, int nfeat, int cand_cap, int kp_cap,
-                            void* const* ws, const size_t* ws_bytes, float* const* kp_out, int* n_out_dev,
-                            float* const* desc_out, uint8_t* const* u8_out, void* table_dev, size_t table_bytes,
-                            void* const* stage_events, void* stream) {
+}  // extern "C"
+
+namespace fsx {
+
+// host-side job records and shape groups of a feature batch (images grouped by shape; the
+// device table holds each group contiguously)
+static int build_feature_jobs(int n_img, const uint8_t* const* imgs, const int* H, const int* W, const int* ld,
+                              double contrast, double edge, int max_kp, int nfeat, int cand_cap, int kp_cap,
+                              void* const* ws, const size_t* ws_bytes, float* const* kp_out, int* n_out_dev,
+                              float* const* desc_out, uint8_t* const* u8_out, void* table_dev, size_t table_bytes,
+                              std::vector<SiftJob>& jobs, std::vector<int>& order, std::vector<Group>& groups) {
     FSX_REQUIRE(n_img >= 1 && imgs && H && W && ld && ws && ws_bytes && kp_out && n_out_dev && table_dev,
                 "null pointer argument");
     FSX_REQUIRE(max_kp >= 1 && nfeat >= 0 && max_kp <= kp_cap, "need 1 <= max_kp <= kp_cap (max_kp %d kp_cap %d)",
                 max_kp, kp_cap);
     FSX_REQUIRE(table_bytes >= (size_t)n_img * sizeof(SiftJob), "job table too small");
     FSX_REQUIRE(max_kp < 65536, "max_kp must be < 65536 for the device spread (got %d)", max_kp);
-    hipStream_t st = (hipStream_t)stream;
-    // group images by shape; the table holds each group contiguously
-    std::vector<int> order(n_img);
+    order.resize(n_img);
     for (int i = 0; i < n_img; i++) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
         return H[a] != H[b] ? H[a] < H[b] : W[a] < W[b];
     });
-    std::vector<SiftJob> jobs(n_img);
-    std::vector<Group> groups;
+    jobs.assign(n_img, SiftJob{});
+    groups.clear();
     SiftJob* dev = (SiftJob*)table_dev;
     for (int k = 0; k < n_img; k++) {
         const int i = order[k];
@@ -1452,19 +1456,61 @@ int fsx_sift_features_batch(int n_img, const uint8_t* const* imgs, const int* H,
             groups.push_back(Group{dev + k, 0, p});
         groups.back().n++;
     }
+    return FSX_OK;
+}
+
+}  // namespace fsx
+
+extern "C" {
+
+int fsx_sift_features_prepare(int n_img, const uint8_t* const* imgs, const int* H, const int* W, const int* ld,
+                              double contrast, double edge, int max_kp, int nfeat, int cand_cap, int kp_cap,
+                              void* const* ws, const size_t* ws_bytes, float* const* kp_out, int* n_out_dev,
+                              float* const* desc_out, uint8_t* const* u8_out, void* table_dev, size_t table_bytes,
+                              void* stream) {
+    std::vector<SiftJob> jobs;
+    std::vector<int> order;
+    std::vector<Group> groups;
+    int rc = build_feature_jobs(n_img, imgs, H, W, ld, contrast, edge, max_kp, nfeat, cand_cap, kp_cap, ws, ws_bytes,
+                                kp_out, n_out_dev, desc_out, u8_out, table_dev, table_bytes, jobs, order, groups);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
     for (int k = 0; k < n_img; k++) {
-        int rc2 = upload_async(ws[order[k]], &jobs[k], sizeof(SiftJob), st);
-        if (rc2) return rc2;
+        rc = upload_async(ws[order[k]], &jobs[k], sizeof(SiftJob), st);
+        if (rc) return rc;
     }
-    {
-        int rc2 = upload_async(table_dev, jobs.data(), (size_t)n_img * sizeof(SiftJob), st);
-        if (rc2) return rc2;
-    }
+    return upload_async(table_dev, jobs.data(), (size_t)n_img * sizeof(SiftJob), st);
+}
+
+int fsx_sift_features_launch(int n_img, const uint8_t* const* imgs, const int* H, const int* W, const int* ld,
+                             double contrast, double edge, int max_kp, int nfeat, int cand_cap, int kp_cap,
+                             void* const* ws, const size_t* ws_bytes, float* const* kp_out, int* n_out_dev,
+                             float* const* desc_out, uint8_t* const* u8_out, void* table_dev, size_t table_bytes,
+                             void* const* stage_events, void* stream) {
+    std::vector<SiftJob> jobs;
+    std::vector<int> order;
+    std::vector<Group> groups;
+    int rc = build_feature_jobs(n_img, imgs, H, W, ld, contrast, edge, max_kp, nfeat, cand_cap, kp_cap, ws, ws_bytes,
+                                kp_out, n_out_dev, desc_out, u8_out, table_dev, table_bytes, jobs, order, groups);
+    if (rc) return rc;
     hipEvent_t ev[8];
     if (stage_events)
         for (int i = 0; i < 8; i++) ev[i] = (hipEvent_t)stage_events[i];
-    return run_pipeline(dev, n_img, groups.data(), (int)groups.size(), kp_cap, nfeat, true, st,
-                        stage_events ? ev : nullptr);
+    return run_pipeline((const SiftJob*)table_dev, n_img, groups.data(), (int)groups.size(), kp_cap, nfeat, true,
+                        (hipStream_t)stream, stage_events ? ev : nullptr);
+}
+
+int fsx_sift_features_batch(int n_img, const uint8_t* const* imgs, const int* H, const int* W, const int* ld,
+                            double contrast, double edge, int max_kp, int nfeat, int cand_cap, int kp_cap,
+                            void* const* ws, const size_t* ws_bytes, float* const* kp_out, int* n_out_dev,
+                            float* const* desc_out, uint8_t* const* u8_out, void* table_dev, size_t table_bytes,
+                            void* const* stage_events, void* stream) {
+    int rc = fsx_sift_features_prepare(n_img, imgs, H, W, ld, contrast, edge, max_kp, nfeat, cand_cap, kp_cap, ws,
+                                       ws_bytes, kp_out, n_out_dev, desc_out, u8_out, table_dev, table_bytes, stream);
+    if (rc) return rc;
+    return fsx_sift_features_launch(n_img, imgs, H, W, ld, contrast, edge, max_kp, nfeat, cand_cap, kp_cap, ws,
+                                    ws_bytes, kp_out, n_out_dev, desc_out, u8_out, table_dev, table_bytes,
+                                    stage_events, stream);
 }
 
 int fsx_sift_selection_offset(int H, int W, int cand_cap, int kp_cap, size_t* off) {

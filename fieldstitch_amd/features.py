@@ -109,6 +109,8 @@ class FeatureBatch:
         self.device = device
         self.float_desc = float_desc
         self._ran = False
+        self._prepared = None
+        self._graph = None
         self._alloc()
 
     def _alloc(self):
@@ -141,9 +143,19 @@ class FeatureBatch:
 
     STAGES = ("pyramid", "extrema", "refine", "orientation", "select", "spread", "describe")
 
-    def run(self, imgs, events=None, stream=None):
+    def _args(self, c_img, c_ld):
+        P = C.c_void_p
+        return (self.n, c_img, self._c_H, self._c_W, c_ld, self.contrast, self.edge, self.max_kp, self.nfeat,
+                self.cand_cap, self.kp_cap, self._c_ws, self._c_wsb, self._c_kp, P(self.n_out.data_ptr()),
+                self._c_desc, self._c_u8, P(self.table.data_ptr()), self.table_bytes)
+
+    def run(self, imgs, events=None, stream=None, graph=False):
         """Enqueue the feature pipeline for device images (list of uint8 (H, W) tensors).
-        ``events``: 8 torch.cuda.Event(enable_timing=True) recorded around the stages."""
+        ``events``: 8 torch.cuda.Event(enable_timing=True) recorded around the stages.
+        The job table is uploaded when the image buffers change; otherwise a run only enqueues
+        kernels.  ``graph=True`` replays a HIP graph of the launch sequence (captured on first
+        use for these buffers; events are then not recorded): refill the same image tensors
+        in place to process new photos."""
         n = self.n
         if len(imgs) != n:
             raise ValueError("expected %d images" % n)
@@ -151,20 +163,41 @@ class FeatureBatch:
             if tuple(t.shape) != s or t.dtype != self.torch.uint8 or not t.is_cuda:
                 raise ValueError("image shape/dtype/device mismatch")
         P = C.c_void_p
+        key = tuple((t.data_ptr(), t.stride(0)) for t in imgs)
         c_img = (P * n)(*[t.data_ptr() for t in imgs])
         c_ld = (C.c_int * n)(*[t.stride(0) for t in imgs])
         st = _lib.stream_handle(stream)
+        if key != self._prepared:
+            _lib.call("fsx_sift_features_prepare", *self._args(c_img, c_ld), P(st))
+            self._prepared = key
+            self._graph = None
+        self._ran = True
+        if graph:
+            if self._graph is None:
+                self._graph = self._capture(c_img, c_ld)
+            if stream is not None:
+                with self.torch.cuda.stream(stream):
+                    self._graph.replay()
+            else:
+                self._graph.replay()
+            return
         ev = None
         if events is not None:
             for e in events:
                 if not e.cuda_event:     # torch creates the HIP event on first record
                     e.record()
             ev = (P * 8)(*[e.cuda_event for e in events])
-        _lib.call("fsx_sift_features_batch", n, c_img, self._c_H, self._c_W, c_ld, self.contrast, self.edge,
-                  self.max_kp, self.nfeat, self.cand_cap, self.kp_cap, self._c_ws, self._c_wsb, self._c_kp,
-                  P(self.n_out.data_ptr()), self._c_desc, self._c_u8, P(self.table.data_ptr()), self.table_bytes,
-                  ev, P(st))
-        self._ran = True
+        _lib.call("fsx_sift_features_launch", *self._args(c_img, c_ld), ev, P(st))
+
+    def _capture(self, c_img, c_ld):
+        torch = self.torch
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=side):
+            _lib.call("fsx_sift_features_launch", *self._args(c_img, c_ld), None, C.c_void_p(_lib.stream_handle()))
+        torch.cuda.current_stream().wait_stream(side)
+        return g
 
     @classmethod
     def stage_times(cls, events):

@@ -27,6 +27,13 @@ int fsx_version(void);
  * exception text (features.py:153-154, matching.py:250-253). */
 int fsx_last_error(char *buf, size_t len);
 
+/* Launch-shape tunables, process-wide (no reference counterpart; set by
+ * fieldstitch_amd.autotune): key 0 = orientation blocks per image (16..65536, default 4096),
+ * 1 = refinement blocks per image (16..65536, default 256), 2 = float matcher targets per LDS
+ * tile (64 or 128, default 64).  Not synchronised with running work: set between batches. */
+int fsx_tune_set(int key, int value);
+int fsx_tune_get(int key, int *value);
+
 /* Device name / CU count / HBM bytes.  Replaces features.gpu_info (features.py:258-267). */
 int fsx_device_info(int device, char *name, size_t len, int *n_cu, long long *hbm_bytes);
 
@@ -90,6 +97,22 @@ int fsx_sift_features_batch(int n_img, const uint8_t *const *imgs, const int *H,
                             void *const *ws, const size_t *ws_bytes, float *const *kp_out, int *n_out_dev,
                             float *const *desc_out, uint8_t *const *u8_out, void *table_dev, size_t table_bytes,
                             void *const *stage_events, void *stream);
+
+/* fsx_sift_features_batch in two halves for repeated runs on the same buffers (and for graph
+ * capture): _prepare uploads the job records (asynchronous on `stream`), _launch only enqueues
+ * the kernels (no host->device copy, no allocation, no memset).  Same arguments as the batch
+ * call; the image / workspace / output pointers are baked in by _prepare.  No reference
+ * counterpart. */
+int fsx_sift_features_prepare(int n_img, const uint8_t *const *imgs, const int *H, const int *W, const int *ld,
+                              double contrast, double edge, int max_kp, int nfeat, int cand_cap, int kp_cap,
+                              void *const *ws, const size_t *ws_bytes, float *const *kp_out, int *n_out_dev,
+                              float *const *desc_out, uint8_t *const *u8_out, void *table_dev, size_t table_bytes,
+                              void *stream);
+int fsx_sift_features_launch(int n_img, const uint8_t *const *imgs, const int *H, const int *W, const int *ld,
+                             double contrast, double edge, int max_kp, int nfeat, int cand_cap, int kp_cap,
+                             void *const *ws, const size_t *ws_bytes, float *const *kp_out, int *n_out_dev,
+                             float *const *desc_out, uint8_t *const *u8_out, void *table_dev, size_t table_bytes,
+                             void *const *stage_events, void *stream);
 
 /* Byte offset of the spread selection inside a workspace (no reference counterpart). */
 int fsx_sift_selection_offset(int H, int W, int cand_cap, int kp_cap, size_t *off);
