@@ -225,6 +225,136 @@ def bench_precision(args):
     return out
 
 
+def bench_flight400(args):
+    """a 400-photo flight (20 x 20 grid, 75 % / 65 % overlap) with resolutions mixed
+    50/50 between 2640x1978 and 1320x989.  Photos resident in HBM; SIFT in chunks of one fixed
+    shape composition through one graph-captured FeatureBatch (launch shapes autotuned first);
+    candidate pairs from the GPS grid on the GPU (16 nearest + temporal); guided matching of all
+    pairs in one batch (MFMA and plain forms, graph replays), a sample checked against the FP64
+    oracle."""
+    import torch
+    from fieldstitch_amd import synth, features, autotune
+    from fieldstitch_amd import pairs as fpairs
+    from fieldstitch_amd.matching import PairMatcher
+    from oracle import match_oracle as mo
+    t0 = time.time()
+    fl = synth.flight(seed=4, rows=20, cols=20, W=2640, H=1978, lowres_frac=0.5,
+                      workers=max(1, min(args.gen_workers, os.cpu_count() or 1)),
+                      progress=lambda d, t: log("flight400: rendered %d / %d photos (%.0f s)" % (d, t, time.time() - t0)))
+    n = len(fl.images)
+    shapes = [im.shape for im in fl.images]
+    log("flight400: %d photos generated in %.1f s" % (n, time.time() - t0))
+    dev = [torch.from_numpy(im).cuda() for im in fl.images]
+    big = [i for i in range(n) if shapes[i] == (1978, 2640)]
+    small = [i for i in range(n) if shapes[i] != (1978, 2640)]
+    CB, CS = 25, 25
+    chunks, bi, si = [], 0, 0
+    while bi < len(big) or si < len(small):
+        chunks.append(big[bi:bi + CB] + small[si:si + CS])
+        bi += CB
+        si += CS
+    batches = {}
+    def batch_for(idx):
+        key = tuple(shapes[i] for i in idx)
+        if key not in batches:
+            fb = features.FeatureBatch(list(key), nfeat=NFEAT, contrast=CONTRAST, edge=EDGE)
+            bufs = [torch.empty(sh, dtype=torch.uint8, device="cuda") for sh in key]
+            batches[key] = (fb, bufs)
+        return batches[key]
+    # autotune the launch shapes on the first chunk
+    fb0, bufs0 = batch_for(chunks[0])
+    for b, i in zip(bufs0, chunks[0]):
+        b.copy_(dev[i])
+    tuned = autotune.tune_features(fb0, bufs0, reps=2)
+    u8_all = torch.empty((n, NFEAT, 128), dtype=torch.uint8, device="cuda")
+    xy_all = torch.empty((n, NFEAT, 2), dtype=torch.float64, device="cuda")
+    cnt_all = torch.empty(n, dtype=torch.int32, device="cuda")
+
+    def sift_pass():
+        for idx in chunks:
+            fb, bufs = batch_for(idx)
+            for b, i in zip(bufs, idx):
+                b.copy_(dev[i])
+            fb.run(bufs, graph=True)
+            it = torch.tensor(idx, device="cuda")
+            u8_all.index_copy_(0, it, fb.u8[:, :NFEAT])
+            cnt_all.index_copy_(0, it, torch.clamp(fb.n_out, max=NFEAT))
+            for s_, i in enumerate(idx):
+                sel = torch.clamp(fb.selection_device(s_)[:NFEAT], 0, fb.max_kp - 1)
+                xy_all[i, :, 0] = fb.kp[s_, 0][sel].double()
+                xy_all[i, :, 1] = fb.kp[s_, 1][sel].double()
+
+    sift_pass()                                   # warm-up (captures the graphs)
+    for fb, _ in batches.values():
+        fb.check_capacity()
+    torch.cuda.synchronize()
+    steps = max(1, args.steps // 5)
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        sift_pass()
+    torch.cuda.synchronize()
+    t_sift = (time.perf_counter() - t1) / steps
+    alg = sum(sift_bytes(*sh) for sh in shapes)
+    counts = cnt_all.cpu().numpy()
+    # candidate pairs from the GPS grid (photo centres in mosaic units, ground half-extent of
+    # the long side = 1320 for both resolutions)
+    centres = np.array([(A @ np.array([(sh[1] - 1) / 2.0, (sh[0] - 1) / 2.0, 1.0]))[:2]
+                        for A, sh in zip(fl.A, shapes)])
+    t2 = time.perf_counter()
+    pl = fpairs.select_pairs(centres, np.full(n, 1320.0), 16, True)
+    t_pairs = time.perf_counter() - t2
+    rng = np.random.default_rng(5)
+    Hm = []
+    for i, j in pl:
+        P = np.linalg.inv(fl.A[j]) @ fl.A[i]
+        d = rng.normal(0.0, 3.0, 2)
+        P[0, 2] += d[0]
+        P[1, 2] += d[1]
+        Hm.append(P)
+    images = [(u8_all[i, :counts[i]], xy_all[i, :counts[i]], (shapes[i][1], shapes[i][0])) for i in range(n)]
+    pm = PairMatcher(images, pl, Hm, radius=RADIUS, kc=0, ratio=RATIO)
+    out_m = {}
+    res = {}
+    for form in ("mfma", "plain"):
+        pm.run(form, graph=True)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        for _ in range(steps):
+            pm.run(form, graph=True)
+        torch.cuda.synchronize()
+        tm = (time.perf_counter() - t3) / steps
+        res[form] = pm.results()
+        out_m[form] = {"ms_per_flight": tm * 1e3, "pairs_per_s": len(pl) / tm}
+    same_forms = all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+                     for a, b in zip(res["mfma"], res["plain"]))
+    sample = rng.choice(len(pl), size=min(24, len(pl)), replace=False)
+    ok = 0
+    for p in sample:
+        i, j = pl[p]
+        q, t = mo.guided_fp64_fast(u8_all[i, :counts[i]].cpu().numpy(), xy_all[i, :counts[i]].cpu().numpy(),
+                                   u8_all[j, :counts[j]].cpu().numpy(), xy_all[j, :counts[j]].cpu().numpy(),
+                                   Hm[p], (shapes[j][1], shapes[j][0]), RADIUS, RATIO)
+        ok += int(np.array_equal(q, res["mfma"][p][0]) and np.array_equal(t, res["mfma"][p][1]))
+    return {
+        "metric": "images/s (SIFT) + matched pairs/s vs plain FP32 GPU form, match set = FP64 oracle",
+        "value": n / t_sift, "unit": "images/s", "n_gpus": 1, "steps": steps, "warmup": 1,
+        "ms_per_step": t_sift * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (f64 blur accumulation)", "data": "synthetic (seeded field mosaic, 400 photos)",
+        "config": {"workload": "400 photos, 20x20 grid 75/65 overlap, 2640x1978 and 1320x989 mixed 50/50; "
+                               "SIFT contrast 0.04 edge 10 nfeat 8000 in graph-captured chunks of %d+%d; "
+                               "guided matching of GPS-selected pairs" % (CB, CS),
+                   "photos": n, "large": len(big), "small": len(small), "pairs": len(pl)},
+        "sift": {"ms_per_flight": t_sift * 1e3, "algorithmic_bytes": alg,
+                 "sift_roofline_frac": alg / t_sift / HBM_PEAK, "keypoints_selected_mean": float(counts.mean()),
+                 "autotune": tuned["chosen"]},
+        "pair_selection_ms": t_pairs * 1e3,
+        "matching": {"pairs": len(pl), **out_m, "speedup_vs_plain_fp32":
+                     out_m["plain"]["ms_per_flight"] / out_m["mfma"]["ms_per_flight"],
+                     "forms_equal": bool(same_forms), "fp64_oracle_sample": int(len(sample)),
+                     "fp64_oracle_equal": int(ok)},
+    }
+
+
 def copy_probe(mib):
     """Three device-to-device copies of `mib` MiB: a kernel with known HBM traffic (read mib,
     write mib per dispatch) whose FETCH_SIZE / WRITE_SIZE readings scale the counters of the
@@ -255,12 +385,17 @@ def main():
     ap.add_argument("--gen-workers", type=int, default=8,
                     help="processes rendering the synthetic photos (1 under a profiler)")
     ap.add_argument("--no-precision", action="store_true", help="skip the float-descriptor precision section")
+    ap.add_argument("--flight400", action="store_true",
+                    help="only run the 400-photo mixed-resolution flight (graph-captured SIFT chunks + guided matching)")
     ap.add_argument("--precision-pairs", type=int, default=200)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 3 plain device copies of MIB MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.flight400:
+        print(json.dumps(bench_flight400(args)), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding)",

@@ -235,6 +235,15 @@ class FeatureBatch:
         d = self.desc[i, :m].cpu().numpy() if self.desc is not None else None
         return xy, u8, resp, d
 
+    def selection_device(self, i):
+        """The spread selection of image i as a device int64 tensor of nfeat entries (entries at
+        and beyond the selected count are stale)."""
+        off = C.c_size_t()
+        H, W = self.shapes[i]
+        _lib.call("fsx_sift_selection_offset", H, W, self.cand_cap, self.kp_cap, C.byref(off))
+        o = off.value
+        return self.ws[i][o:o + 8 * max(self.nfeat, 1)].view(self.torch.int64)
+
     def _sel(self, i, m):
         """The spread selection (int64 indices into the detect arrays, final order) of image i."""
         off = C.c_size_t()

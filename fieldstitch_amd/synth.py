@@ -194,13 +194,14 @@ def _render_job(job):
 def flight(seed: int = 2, rows: int = 2, cols: int = 4, W: int = 2640, H: int = 1978,
            front: float = 0.75, side: float = 0.65, rot_jitter: float = 3.0, scale_jitter: float = 0.02,
            prior_sigma: float = 3.0, lowres_frac: float = 0.0, poly_density: float = 40.0 / (1368 * 912),
-           min_overlap: float = 0.05, workers: int = 1) -> Flight:
+           min_overlap: float = 0.05, workers: int = 1, progress=None) -> Flight:
     """Photos along a rows x cols flight grid.  Flight lines run along x (a line is one grid row),
     so consecutive photos of a line overlap by `front` of the width and neighbouring lines by
     `side` of the height.  A photo is low resolution (W/2 x H/2 covering the same ground) with
     probability `lowres_frac`.  Pairs: every photo pair whose footprints (without jitter)
     overlap by more than `min_overlap` of the footprint area.  Each photo's noise has its own
-    seed, so `workers` > 1 renders in parallel with identical results."""
+    seed, so `workers` > 1 renders in parallel with identical results.  progress(done, total)
+    is called every 25 photos."""
     rng = np.random.default_rng(seed)
     step_u = (1.0 - front) * W
     step_v = (1.0 - side) * H
@@ -226,10 +227,18 @@ def flight(seed: int = 2, rows: int = 2, cols: int = 4, W: int = 2640, H: int = 
             jobs.append((mos, M, h, w, int(rng.integers(1 << 62))))
     if workers > 1:
         import multiprocessing as mp
+        images = []
         with mp.get_context("fork").Pool(min(workers, len(jobs))) as pool:
-            images = pool.map(_render_job, jobs)
+            for im in pool.imap(_render_job, jobs):
+                images.append(im)
+                if progress and len(images) % 25 == 0:
+                    progress(len(images), len(jobs))
     else:
-        images = [_render_job(j) for j in jobs]
+        images = []
+        for j in jobs:
+            images.append(_render_job(j))
+            if progress and len(images) % 25 == 0:
+                progress(len(images), len(jobs))
     n = len(images)
     pairs, H_true, prior = [], {}, {}
     for i in range(n):
