@@ -67,6 +67,12 @@ SIGNATURES = {
     "fsx_fmatch_results": [_i, C.POINTER(_i), _i, C.POINTER(_i), C.POINTER(_i), _vp, C.POINTER(_i), C.POINTER(_i),
                            C.POINTER(_i), C.POINTER(C.c_double), _vp],
     "fsx_fp8_layout_probe": [_vp, _vp],
+    "fsx_mlp_gemm": [_i, _i, _i, _vp, _i, _vp, _i, _i, _i, _i, _vp, _vp, _i, _vp, _vp, _vp],
+    "fsx_mlp_head": [_vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp],
+    "fsx_mlp_colsum_scratch": [_i, _i, C.POINTER(_sz)],
+    "fsx_mlp_colsum": [_vp, _i, _i, _i, _i, _vp, _vp, _vp],
+    "fsx_mlp_adam": [_vp, _vp, _vp, _vp, _vp, _i, C.c_float, C.c_float, C.c_float, C.c_float, _i, _vp],
+    "fsx_mlp_to_bf16": [_vp, _vp, _i, _vp],
     "fsx_match_query_state": [_i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), _i, C.POINTER(_i), C.POINTER(_i),
                               C.c_double, _vp, C.POINTER(_i), C.POINTER(C.c_double), C.POINTER(_i), _vp],
 }

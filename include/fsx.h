@@ -218,6 +218,36 @@ int fsx_fmatch_results(int n_set, const int *n_kp, int n_pair, const int *pa, co
 int fsx_fp8_layout_probe(float *out_dev, void *stream);
 
 /* ---------------------------------------------------------------------------------------
+ * Learned match filter: BF16 MLP kernels (no reference counterpart; trained on the inlier
+ * labels of the plain pipeline, used to prune candidates before the exact ratio test)
+ * ------------------------------------------------------------------------------------- */
+
+/* C = op(A) op(B), bf16 operands (uint16 bit patterns, row-major, leading dimensions in
+ * elements), FP32 accumulation on MFMA.  ta / tb: operand stored transposed (op(A)[m][k] =
+ * A[k*lda + m]; op(B)[k][n] = B[n*ldb + k]).  epi 0: C16 = bf16(relu(acc + bias[n]));
+ * 1: C32 = acc; 2: C16 = bf16(acc where mask[m][n] > 0, else 0).  K % 32 == 0. */
+int fsx_mlp_gemm(int ta, int tb, int epi, const void *A, int lda, const void *B, int ldb, int M, int N, int K,
+                 float *C32, void *C16, int ldc, const float *bias, const void *mask, void *stream);
+
+/* Output layer over nb rows of H2 (bf16, 256 wide): p = sigmoid(H2 w3 + b3) (p_out may be
+ * NULL); with labels y: dZ2 = ((p - y) / nb) w3 masked by H2 > 0 (bf16) and per-4-row partials
+ * part[ceil(nb/4)][258] = (sum dz H2[:, j], sum dz, sum BCE loss). */
+int fsx_mlp_head(const void *H2, const float *w3, const float *b3, const float *y, int nb, float *p_out, void *dZ2,
+                 float *part, void *stream);
+
+/* Column sums in a fixed order (bf16 != 0: bf16 input, else float32); scratch of
+ * fsx_mlp_colsum_scratch bytes. */
+int fsx_mlp_colsum_scratch(int nr, int ncol, size_t *bytes);
+int fsx_mlp_colsum(const void *in, int bf16, int nr, int ld, int ncol, float *out, float *scratch, void *stream);
+
+/* Adam step `step` (>= 1, bias-corrected) on n float32 weights; w16 (may be NULL): bf16 copy. */
+int fsx_mlp_adam(float *w, float *m, float *v, const float *g, void *w16, int n, float lr, float beta1, float beta2,
+                 float eps, int step, void *stream);
+
+/* float32 -> bf16 (round to nearest even). */
+int fsx_mlp_to_bf16(const float *in, void *out, int n, void *stream);
+
+/* ---------------------------------------------------------------------------------------
  * Keypoint geometry around the matcher
  * ------------------------------------------------------------------------------------- */
 
