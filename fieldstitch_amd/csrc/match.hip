@@ -85,6 +85,8 @@ struct MatchBatch {
     double* b1;                   // its distance
     int* qcell;                   // cell of the prediction in b's grid, -1 = not inside
     double* kth;                  // [n_query][2] kc-th nearest (d2, idx bits) key, kc > 0 only
+    int* cbest;                   // [n_query] best candidate before the ratio test, -1 = none
+    float* cb12;                  // [n_query][2] its distance and the second smallest
     unsigned long long* tkey;     // [n_targets] min over accepted queries of (b1 bits << 32 | q)
     int* counters;                // [0..3] flags
     int* out_q;                   // [sum min(n_a, n_b)]
@@ -398,6 +400,9 @@ __device__ __forceinline__ void top2_push(float d, int t, float& v1, int& i1, fl
 __device__ __forceinline__ void accept_query(const MatchBatch& B, long long g, const MPair& P, int q, float v1, int i1,
                                              float v2) {
     const bool ok = i1 >= 0 && sqrt((double)v1) < B.ratio * sqrt((double)v2);
+    B.cbest[g] = i1;
+    B.cb12[2 * g] = v1;
+    B.cb12[2 * g + 1] = v2;
     B.best[g] = ok ? i1 : -1;
     B.b1[g] = ok ? (double)v1 : 0.0;
     if (ok) {
@@ -438,6 +443,7 @@ __global__ __launch_bounds__(256) void k_m_plain(const MatchBatch B) {
     B.qcell[g] = cell;
     if (cell < 0) {
         B.best[g] = -1;
+        B.cbest[g] = -1;
         return;
     }
     double td;
@@ -588,7 +594,10 @@ __global__ __launch_bounds__(64) void k_m_mfma(const MatchBatch B) {
         }
         __syncthreads();
         if (__ballot(act) == 0ull) {       // wave-uniform: nothing to match in this chunk
-            if (lane < 32 && valid) B.best[g] = -1;
+            if (lane < 32 && valid) {
+                B.best[g] = -1;
+                B.cbest[g] = -1;
+            }
             return;
         }
         double td = B.r2;
@@ -768,6 +777,7 @@ __global__ __launch_bounds__(64) void k_m_mfma(const MatchBatch B) {
         if (lane < 32 && valid) {
             if (!act) {
                 B.best[g] = -1;
+                B.cbest[g] = -1;
             } else {
                 if (any_over) {
                     query_direct(B, P, ib, ka, cell, qpx, qpy, td, ti, v1, i1, v2);
@@ -781,6 +791,61 @@ __global__ __launch_bounds__(64) void k_m_mfma(const MatchBatch B) {
             }
         }
         __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Learned-filter inputs: one row of 288 bf16 per query (its best candidate before the ratio
+// test), 64 threads per row: lanes 0..31 write the descriptor terms (4 bins each), lane 32 the
+// geometric terms, the rest the zero padding.
+
+__device__ __forceinline__ unsigned short f_bf16(float x) {
+    const unsigned b = __float_as_uint(x);
+    return (unsigned short)((b + 0x7fffu + ((b >> 16) & 1u)) >> 16);
+}
+
+__global__ __launch_bounds__(256) void k_m_features(const MatchBatch B, const float* __restrict__ size,
+                                                    const float* __restrict__ angle, unsigned short* __restrict__ X) {
+    const MPair P = B.pair[blockIdx.y];
+    const MImage ia = B.img[P.a];
+    const MImage ib = B.img[P.b];
+    const int lane = threadIdx.x & 63;
+    const int q = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (q >= ia.n) return;
+    const long long g = P.q_off + q;
+    unsigned short* row = X + g * 288;
+    const int t = B.cbest[g];
+    if (t < 0) {
+        for (int k = lane; k < 288; k += 64) row[k] = 0;
+        return;
+    }
+    const long long ka = ia.off + q, kb = ib.off + t;
+    if (lane < 32) {
+        const unsigned a4 = ((const unsigned*)(B.desc + ka * 128))[lane];
+        const unsigned b4 = ((const unsigned*)(B.desc + kb * 128))[lane];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const float av = (float)((a4 >> (8 * u)) & 0xffu), bv = (float)((b4 >> (8 * u)) & 0xffu);
+            row[4 * lane + u] = f_bf16(fabsf(av - bv) * (1.0f / 255.0f));
+            row[128 + 4 * lane + u] = f_bf16(av * bv * (1.0f / 65025.0f));
+        }
+    } else if (lane == 32) {
+        const double x = B.xy[2 * ka], y = B.xy[2 * ka + 1];
+        double px, py;
+        predict_cell(B, P, ia, ib, x, y, px, py);
+        const float b1 = B.cb12[2 * g], b2 = B.cb12[2 * g + 1];
+        const float r = (float)B.radius;
+        float dang = angle[kb] - angle[ka];
+        dang = dang >= 180.0f ? dang - 360.0f : (dang < -180.0f ? dang + 360.0f : dang);
+        const float sa = size[ka], sb = size[kb];
+        const float v[8] = {(float)((px - B.xy[2 * kb]) / B.radius), (float)((py - B.xy[2 * kb + 1]) / B.radius),
+                            sqrtf(b1) / 1024.0f, sqrtf(fminf(b2, 4e6f)) / 1024.0f,
+                            isinf(b2) ? 0.0f : sqrtf(b1 / fmaxf(b2, 1e-30f)),
+                            (sa > 0.0f && sb > 0.0f) ? logf(sb / sa) : 0.0f, dang / 180.0f, r / 100.0f};
+#pragma unroll
+        for (int u = 0; u < 8; u++) row[256 + u] = f_bf16(v[u]);
+    } else if (lane < 57) {
+        row[264 + (lane - 33)] = 0;
     }
 }
 
@@ -851,7 +916,7 @@ struct MatchPlanHost {
     std::vector<MPair> pair;
     long long n_kp = 0, n_query = 0, n_cells = 0, n_targets = 0, n_out = 0, n_chunks = 0, n_chunk_slots = 0;
     size_t off_chunks, off_nchunks, off_batch, off_img, off_pair, off_norms, off_snorms, off_sxy, off_sorig, off_cstart, off_ccur, off_best, off_b1,
-        off_qcell, off_kth, off_tkey, off_cnt, off_outq, off_outt, off_outn, total;
+        off_qcell, off_kth, off_cbest, off_cb12, off_tkey, off_cnt, off_outq, off_outt, off_outn, total;
 };
 
 static int match_plan(int n_img, const int* n_kp, const int* W, const int* H, int n_pair, const int* pa,
@@ -924,6 +989,8 @@ static int match_plan(int n_img, const int* n_kp, const int* W, const int* H, in
     M.off_b1 = o; o = a256(o + 8 * nq);
     M.off_qcell = o; o = a256(o + 4 * nq);
     M.off_kth = o; o = a256(o + 16 * nq);
+    M.off_cbest = o; o = a256(o + 4 * nq);
+    M.off_cb12 = o; o = a256(o + 8 * nq);
     M.off_tkey = o; o = a256(o + 8 * (size_t)std::max(to, 1LL));
     M.off_cnt = o; o = a256(o + 16 * 4);
     M.off_outq = o; o = a256(o + 4 * (size_t)std::max(oo, 1LL));
@@ -965,6 +1032,8 @@ static MatchBatch batch_ptrs(const MatchPlanHost& M, char* ws, const uint8_t* de
     b.b1 = (double*)(ws + M.off_b1);
     b.qcell = (int*)(ws + M.off_qcell);
     b.kth = (double*)(ws + M.off_kth);
+    b.cbest = (int*)(ws + M.off_cbest);
+    b.cb12 = (float*)(ws + M.off_cb12);
     b.tkey = (unsigned long long*)(ws + M.off_tkey);
     b.counters = (int*)(ws + M.off_cnt);
     b.out_q = (int*)(ws + M.off_outq);
@@ -1078,6 +1147,20 @@ int fsx_match_batch(int n_img, const int* n_kp, const int* W, const int* H, cons
     return fsx_match_launch(ws, info, form, stage_events_v, stream);
 }
 
+int fsx_match_filter_features(void* ws, const long long* info, const float* size_dev, const float* angle_dev,
+                              void* X_dev, void* stream) {
+    FSX_REQUIRE(ws && info && size_dev && angle_dev && X_dev, "null pointer argument");
+    FSX_REQUIRE(info[0] == 0x6673786d && info[7] == (long long)(intptr_t)ws, "launch_info does not match ws");
+    MatchBatch b;
+    memcpy(&b, info + 8, sizeof(b));
+    const int n_pair = (int)info[2], max_na = (int)info[4];
+    if (info[3] == 0) return FSX_OK;
+    hipLaunchKernelGGL(k_m_features, dim3((unsigned)((max_na + 3) / 4), n_pair), dim3(256), 0, (hipStream_t)stream, b,
+                       size_dev, angle_dev, (unsigned short*)X_dev);
+    FSX_LAUNCH_CHECK();
+    return FSX_OK;
+}
+
 // Copies the results of the last fsx_match_batch on `ws` to host arrays (synchronises):
 // counts[n_pair], then q/t of all pairs at the offsets fsx_match_batch reported.
 int fsx_match_results(int n_img, const int* n_kp, const int* W, const int* H, int n_pair, const int* pa,
@@ -1153,6 +1236,22 @@ int fsx_mfma_layout_probe(float* out_dev, void* stream) {
     FSX_REQUIRE(out_dev, "null pointer argument");
     hipLaunchKernelGGL(k_mfma_layout_probe, dim3(1), dim3(64), 0, (hipStream_t)stream, out_dev);
     FSX_LAUNCH_CHECK();
+    return FSX_OK;
+}
+
+// Byte offsets inside the workspace of the per-query candidate arrays of the last batch (in
+// the caller's query order): [0] cbest (int32, best candidate before the ratio test, -1 = none), [1] cb12 (float32 pairs: its distance, the second smallest), [2] best
+// (int32, accepted target or -1), [3] qcell (int32).  For the learned filter.
+int fsx_match_candidate_offsets(int n_img, const int* n_kp, const int* W, const int* H, int n_pair, const int* pa,
+                                const int* pb, double radius, long long* offs) {
+    FSX_REQUIRE(offs, "null pointer argument");
+    MatchPlanHost M;
+    int rc = match_plan(n_img, n_kp, W, H, n_pair, pa, pb, nullptr, nullptr, radius, M);
+    if (rc) return rc;
+    offs[0] = (long long)M.off_cbest;
+    offs[1] = (long long)M.off_cb12;
+    offs[2] = (long long)M.off_best;
+    offs[3] = (long long)M.off_qcell;
     return FSX_OK;
 }
 

@@ -140,3 +140,101 @@ class MatchFilterMLP:
         torch = self.torch
         if X16.dtype != torch.bfloat16 or X16.dim() != 2 or X16.shape[1] != N_INP or not X16.is_contiguous():
             raise ValueError("X16 must be a contiguous bfloat16 [n, %d] tensor" % N_INP)
+
+
+# ------------------------------------------------------------------------------------------
+# Training data from the plain pipeline and the filter's evaluation
+
+def flight_candidates(seed, workers=8, radius=60.0, ratio=0.8, n_hyp=512, thr=3.0, nfeat=8000, **flight_kw):
+    """Candidates and labels of one synthetic 2 x 4 flight: SIFT on the GPU, guided matching
+    (the plain method), homography RANSAC on each pair's plain matches.  Returns a dict with
+    X (bf16 [n, 288] device rows, one per query with a candidate), y (float32 device labels:
+    1 = the plain pipeline's match and a RANSAC inlier), plain (bool numpy: accepted by the
+    ratio test and the one-query-per-target rule) and counts."""
+    import torch
+    from . import features, synth
+    from .matching import PairMatcher
+    from .verify import ransac_homography
+    fl = synth.flight(seed=seed, workers=workers, **flight_kw)
+    dev = [torch.from_numpy(im).cuda() for im in fl.images]
+    shapes = [im.shape for im in fl.images]
+    fb = features.FeatureBatch(shapes, nfeat=nfeat, contrast=0.04, edge=10.0)
+    fb.run(dev)
+    fb.check_capacity()
+    images, sizes, angles, xys = [], [], [], []
+    for i, (H, W) in enumerate(shapes):
+        xy, u8, _, _ = fb.results(i)
+        sel = fb._sel(i, len(xy))
+        kp = fb.kp[i].cpu().numpy()
+        xy64 = xy.astype(np.float64)
+        images.append((fb.u8[i, :len(xy)].contiguous(), torch.from_numpy(xy64).cuda(), (W, H)))
+        sizes.append(kp[2][sel])
+        angles.append(kp[3][sel])
+        xys.append(xy64)
+    pairs = list(fl.pairs)
+    pm = PairMatcher(images, pairs, [fl.prior[p] for p in pairs], radius=radius, kc=0, ratio=ratio)
+    pm.run("mfma")
+    res = pm.results()
+    Xall = pm.filter_features(sizes, angles)
+    cbest, _, _ = pm.candidates()
+    cb = cbest.cpu().numpy()
+    qoff = pm.query_offsets()
+    # RANSAC labels on each pair's plain matches
+    pa = [xys[i][q] for (i, j), (q, t) in zip(pairs, res)]
+    pb = [xys[j][t] for (i, j), (q, t) in zip(pairs, res)]
+    flags, _, _ = ransac_homography(pa, pb, n_hyp=n_hyp, thr=thr, seed=0x5EED + seed)
+    nq = len(cb)
+    label = np.zeros(nq, np.float32)
+    plain = np.zeros(nq, bool)
+    for p, ((q, t), f) in enumerate(zip(res, flags)):
+        plain[qoff[p] + q] = True
+        label[qoff[p] + q[f]] = 1.0
+    keep = cb >= 0
+    idx = torch.from_numpy(np.nonzero(keep)[0]).cuda()
+    return {"X": Xall.index_select(0, idx).contiguous(), "y": torch.from_numpy(label[keep]).cuda(),
+            "plain": plain[keep], "label": label[keep] > 0.5, "n_query": nq, "n_pairs": len(pairs),
+            "n_plain": int(plain.sum()), "n_inliers": int(label.sum())}
+
+
+def train(net, X, y, epochs=4, batch=8192, seed=0):
+    """Adam epochs over (X, y) in seeded shuffled batches (the tail batch is dropped);
+    returns the mean loss of each epoch."""
+    import torch
+    n = X.shape[0]
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(epochs):
+        perm = torch.from_numpy(rng.permutation(n)).cuda()
+        losses = []
+        for s in range(0, n - batch + 1, batch):
+            ib = perm[s:s + batch]
+            losses.append(net.step(X.index_select(0, ib).contiguous(), y.index_select(0, ib).contiguous()))
+        out.append(float(torch.stack(losses).mean()))
+    return out
+
+
+def threshold_for_recall(p, positive, recall=0.995):
+    """Largest score threshold keeping at least `recall` of the positives."""
+    s = np.sort(np.asarray(p)[np.asarray(positive)])
+    if len(s) == 0:
+        return 0.5
+    k = int(np.floor((1.0 - recall) * len(s)))
+    return float(s[k])
+
+
+def evaluate(p, tau, data):
+    """Recall / precision of the pruned candidate set against the plain pipeline's verified
+    inliers, and what survives into verification with and without the network."""
+    p = np.asarray(p)
+    kept = p >= tau
+    pos = data["label"]
+    plain = data["plain"]
+    n = len(p)
+    both = kept & plain
+    return {"candidates": n, "kept": int(kept.sum()), "pruned_fraction": float(1.0 - kept.mean()),
+            "inlier_recall": float((kept & pos).sum() / max(pos.sum(), 1)),
+            "precision_kept": float((kept & pos).sum() / max(kept.sum(), 1)),
+            "plain_ratio_test": {"accepted": int(plain.sum()), "inlier_fraction": float(pos.sum() / max(plain.sum(), 1))},
+            "filter_then_ratio_test": {"accepted": int(both.sum()),
+                                       "inlier_fraction": float((both & pos).sum() / max(both.sum(), 1)),
+                                       "inlier_recall": float((both & pos).sum() / max(pos.sum(), 1))}}

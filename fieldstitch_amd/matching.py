@@ -237,6 +237,46 @@ class PairMatcher:
             ev = (P * 4)(*[e.cuda_event for e in events])
         _lib.call("fsx_match_launch", P(self.ws.data_ptr()), self._info, FORMS[form], ev, P(st))
 
+    def candidates(self):
+        """Device views of the last run's per-query arrays (caller's query order, pair p's
+        queries at the prefix of n_a): best candidate before the ratio test (int32, -1 none),
+        its distance and the second smallest (float32 (n, 2)), accepted target (int32)."""
+        torch = self.torch
+        offs = (C.c_longlong * 4)()
+        _lib.call("fsx_match_candidate_offsets", self.n_img, self._nk, self._W, self._H, self.n_pair, self._pa,
+                  self._pb, self.radius, offs)
+        nq = sum(self.nk[a] for a, _ in self.pairs)
+        cb = self.ws[offs[0]:offs[0] + 4 * nq].view(torch.int32)
+        c12 = self.ws[offs[1]:offs[1] + 8 * nq].view(torch.float32).view(nq, 2)
+        best = self.ws[offs[2]:offs[2] + 4 * nq].view(torch.int32)
+        return cb, c12, best
+
+    def filter_features(self, sizes, angles):
+        """The learned filter's input rows (bf16 [n_query, 288] device tensor) for the last run:
+        one per query, built from its best candidate before the ratio test (zero rows for
+        queries without one).  sizes / angles: per image float32 arrays of keypoint size and
+        angle (degrees), in the order of the images given to the constructor."""
+        torch = self.torch
+        sz = torch.cat([torch.as_tensor(np.asarray(s_, np.float32)) if not isinstance(s_, torch.Tensor) else s_.float()
+                        for s_ in sizes]).to(self.desc.device).contiguous()
+        an = torch.cat([torch.as_tensor(np.asarray(a_, np.float32)) if not isinstance(a_, torch.Tensor) else a_.float()
+                        for a_ in angles]).to(self.desc.device).contiguous()
+        if sz.numel() != sum(self.nk) or an.numel() != sum(self.nk):
+            raise ValueError("sizes / angles must give one value per keypoint")
+        nq = sum(self.nk[a] for a, _ in self.pairs)
+        X = torch.empty((max(nq, 1), 288), dtype=torch.bfloat16, device=self.desc.device)
+        _lib.call("fsx_match_filter_features", C.c_void_p(self.ws.data_ptr()), self._info, C.c_void_p(sz.data_ptr()),
+                  C.c_void_p(an.data_ptr()), C.c_void_p(X.data_ptr()), C.c_void_p(_lib.stream_handle()))
+        return X[:nq]
+
+    def query_offsets(self):
+        """First global query index of each pair (the row offsets of filter_features)."""
+        out, o = [], 0
+        for a, _ in self.pairs:
+            out.append(o)
+            o += self.nk[a]
+        return out
+
     def capture(self, form="mfma"):
         """Capture the launch sequence of `form` into a HIP graph (torch.cuda.CUDAGraph)."""
         torch = self.torch

@@ -177,6 +177,14 @@ int fsx_match_query_state(int n_img, const int *n_kp, const int *W, const int *H
                           const int *pb, double radius, const void *ws_dev, int *best, double *b1, int *qcell,
                           void *stream);
 
+/* Byte offsets inside a matcher workspace of its per-query arrays (caller's query order,
+ * queries of pair p at the prefix of n_a): offs[0] best candidate before the ratio test
+ * (int32, -1 = none), [1] its distance and the second smallest (float32 pairs), [2] the
+ * accepted target (int32, -1 = none), [3] the prediction's grid cell (int32).  The learned
+ * match filter's inputs (no reference counterpart). */
+int fsx_match_candidate_offsets(int n_img, const int *n_kp, const int *W, const int *H, int n_pair, const int *pa,
+                                const int *pb, double radius, long long *offs);
+
 /* Writes the 64 x 16 accumulator of one 32x32x16 BF16 MFMA on fixed small integer operands, to
  * check the operand/result lane layout the matcher assumes (no reference counterpart). */
 int fsx_mfma_layout_probe(float *out_dev, void *stream);
@@ -246,6 +254,25 @@ int fsx_mlp_adam(float *w, float *m, float *v, const float *g, void *w16, int n,
 
 /* float32 -> bf16 (round to nearest even). */
 int fsx_mlp_to_bf16(const float *in, void *out, int n, void *stream);
+
+/* Deterministic homography RANSAC (the verification step of match_pair, matching.py:226,
+ * with reproducible sampling): pair p's matches are rows off_dev[p] .. off_dev[p+1] (int64
+ * device prefix) of pa_dev / pb_dev (float64 (x, y) in image a / b); n_hyp hypotheses per pair
+ * from splitmix64 counter draws, 4-point DLT in float64, inlier iff reprojection error < thr.
+ * Writes inlier_dev (byte per match), H_dev (9 float64 per pair, zeros without a model) and
+ * n_in_dev. */
+int fsx_ransac_homography(int n_pair, const long long *off_dev, const double *pa_dev, const double *pb_dev, int n_hyp,
+                          double thr, unsigned long long seed, uint8_t *inlier_dev, double *H_dev, int *n_in_dev,
+                          void *stream);
+
+/* The learned filter's inputs for every query of the last matcher batch on ws (launch_info of
+ * fsx_match_prepare): row g (bf16, 288 wide; zero when the query has no candidate) =
+ * |a - b| / 255 (128), a * b / 255^2 (128), prior offset of the candidate / radius (2),
+ * sqrt(b1) / 1024, sqrt(min(b2, 4e6)) / 1024, sqrt(b1 / b2) (0 when b2 is +inf),
+ * log(size_b / size_a), wrapped angle difference / 180, radius / 100, then 24 zeros.
+ * size_dev / angle_dev: float32 per keypoint in the batch's concatenated order. */
+int fsx_match_filter_features(void *ws_dev, const long long *launch_info, const float *size_dev,
+                              const float *angle_dev, void *X_dev, void *stream);
 
 /* ---------------------------------------------------------------------------------------
  * Keypoint geometry around the matcher
