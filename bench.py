@@ -355,6 +355,57 @@ def bench_flight400(args):
     }
 
 
+def bench_filter(args):
+    """configs[4] at reduced scale: the learned match filter trained on the plain pipeline's
+    RANSAC-verified inliers of 4 synthetic 8-photo flights (2640x1978, nfeat 8000), threshold
+    chosen on a 5th for 99.7 % inlier recall, evaluated against the 99.5 % target on a 6th.  Reports the pruned fraction
+    and the inlier fraction of what reaches verification with and without the network."""
+    import torch
+    from fieldstitch_amd import filter_mlp as fm
+    t0 = time.time()
+    data = []
+    for k, seed in enumerate(range(40, 46)):
+        data.append(fm.flight_candidates(seed, workers=max(1, min(args.gen_workers, os.cpu_count() or 1))))
+        log("filter: flight %d: %d candidates, %d plain, %d inliers (%.0f s)" %
+            (k, data[-1]["X"].shape[0], data[-1]["n_plain"], data[-1]["n_inliers"], time.time() - t0))
+    Xtr = torch.cat([d["X"] for d in data[:4]])
+    ytr = torch.cat([d["y"] for d in data[:4]])
+    net = fm.MatchFilterMLP(seed=1, lr=1e-3)
+    batch = 8192
+    epochs = args.filter_epochs
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    losses = fm.train(net, Xtr, ytr, epochs=epochs, batch=batch)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t1
+    steps = epochs * (Xtr.shape[0] // batch)
+    pv = net.predict(data[4]["X"]).cpu().numpy()
+    # the threshold keeps 99.7 % of the validation flight's inliers: margin for the 99.5 % target
+    tau = fm.threshold_for_recall(pv, data[4]["label"], 0.997)
+    Xte = data[5]["X"]
+    net.predict(Xte[:65536])
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    pt = net.predict(Xte)
+    torch.cuda.synchronize()
+    t_inf = time.perf_counter() - t2
+    ev = fm.evaluate(pt.cpu().numpy(), tau, data[5])
+    flops_fwd = 2.0 * (288 * 256 + 256 * 256 + 256)
+    return {
+        "metric": "learned match filter: verified-inlier recall vs the plain set (target >= 99.5 %)",
+        "value": ev["inlier_recall"], "unit": "recall", "n_gpus": 1, "higher_is_better": True,
+        "data": "synthetic (6 seeded 8-photo flights, 2640x1978, RANSAC labels from the plain pipeline)",
+        "config": {"workload": "MLP 264-256-256-1 BF16, Adam lr 1e-3, batch %d, %d epochs; train 4 flights, "
+                               "threshold (99.7 %% recall) on 1, test on 1" % (batch, epochs),
+                   "train_candidates": int(Xtr.shape[0]), "test_candidates": int(Xte.shape[0])},
+        "train": {"seconds": t_train, "steps": steps, "samples_per_s": steps * batch / t_train,
+                  "tflops": 3 * flops_fwd * steps * batch / t_train / 1e12, "epoch_loss": losses},
+        "threshold": tau,
+        "inference": {"candidates_per_s": Xte.shape[0] / t_inf, "tflops": flops_fwd * Xte.shape[0] / t_inf / 1e12},
+        "test": ev,
+    }
+
+
 def copy_probe(mib):
     """Three device-to-device copies of `mib` MiB: a kernel with known HBM traffic (read mib,
     write mib per dispatch) whose FETCH_SIZE / WRITE_SIZE readings scale the counters of the
@@ -385,6 +436,9 @@ def main():
     ap.add_argument("--gen-workers", type=int, default=8,
                     help="processes rendering the synthetic photos (1 under a profiler)")
     ap.add_argument("--no-precision", action="store_true", help="skip the float-descriptor precision section")
+    ap.add_argument("--filter", action="store_true",
+                    help="only train and evaluate the learned match filter (6 synthetic flights)")
+    ap.add_argument("--filter-epochs", type=int, default=8)
     ap.add_argument("--flight400", action="store_true",
                     help="only run the 400-photo mixed-resolution flight (graph-captured SIFT chunks + guided matching)")
     ap.add_argument("--precision-pairs", type=int, default=200)
@@ -396,6 +450,9 @@ def main():
         return
     if args.flight400:
         print(json.dumps(bench_flight400(args)), flush=True)
+        return
+    if args.filter:
+        print(json.dumps(bench_filter(args)), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding)",
