@@ -226,7 +226,7 @@ def bench_precision(args):
 
 
 def bench_flight400(args):
-    """a 400-photo flight (20 x 20 grid, 75 % / 65 % overlap) with resolutions mixed
+    """The 400-photo flight workload: a 400-photo flight (20 x 20 grid, 75 % / 65 % overlap) with resolutions mixed
     50/50 between 2640x1978 and 1320x989.  Photos resident in HBM; SIFT in chunks of one fixed
     shape composition through one graph-captured FeatureBatch (launch shapes autotuned first);
     candidate pairs from the GPS grid on the GPU (16 nearest + temporal); guided matching of all
@@ -356,7 +356,7 @@ def bench_flight400(args):
 
 
 def bench_filter(args):
-    """configs[4] at reduced scale: the learned match filter trained on the plain pipeline's
+    """The learned match filter at reduced scale: trained on the plain pipeline's
     RANSAC-verified inliers of 4 synthetic 8-photo flights (2640x1978, nfeat 8000), threshold
     chosen on a 5th for 99.7 % inlier recall, evaluated against the 99.5 % target on a 6th.  Reports the pruned fraction
     and the inlier fraction of what reaches verification with and without the network."""
@@ -455,7 +455,7 @@ def main():
         print(json.dumps(bench_filter(args)), flush=True)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding)",
+        print("bench.py measures one GPU: this library has no multi-GPU path; photos would shard by flight-grid block",
               file=sys.stderr)
         sys.exit(2)
     import torch

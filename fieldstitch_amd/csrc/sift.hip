@@ -145,16 +145,19 @@ __device__ __forceinline__ float up2_value(const uint8_t* __restrict__ img, int 
     return 0.75f * t_i + 0.25f * t_n;
 }
 
-// Tile: 128 input columns (TWo = 128 - 2*R2 output columns) x 32 output rows, 256 threads.
-// Vertical pass: a thread owns one input column and 16 consecutive output rows; it streams
-// the 16 + 2*R2 input values of its column once from global memory (coalesced across the
-// wave), converts each to float64 once and accumulates from registers.  The float32-rounded
-// result goes to an LDS mid-tile (row pitch 129 floats: conflict-free).  Horizontal pass: a
-// thread owns 8 consecutive outputs of one row, reading 8 + 2*R2 mid values once.
-constexpr int BL_SW = 128;
-constexpr int BL_TH = 32;
-constexpr int BL_RB = 16;
-constexpr int BL_CB = 8;
+// Tile: 256 input columns (TWo = 256 - 2*R2 output columns) x 16 output rows, 256 threads.
+// Vertical pass: a thread owns one input column and the 16 output rows; it streams the
+// 16 + 2*R2 input values of its column once from global memory (coalesced across the wave),
+// converts each to float64 once and accumulates from registers.  The float32-rounded result
+// goes to an LDS mid-tile (row pitch 257 floats).  Horizontal pass: a thread owns 16
+// consecutive outputs of one row, reading 16 + 2*R2 mid values once.  Epilogue: lanes along
+// x; the previous layer's values for the DoG are loaded as one batch before any store (a
+// load after a store to a possibly aliasing address would wait a full memory latency per
+// element), the layer outputs are streamed out with non-temporal stores.
+constexpr int BL_SW = 256;
+constexpr int BL_NT = 256;
+constexpr int BL_TH = 16;
+constexpr int BL_CB = 16;
 
 template <int R2>
 constexpr int blur_tw() { return BL_SW - 2 * R2; }
@@ -173,13 +176,39 @@ __device__ __forceinline__ void xcd_tile(int& tx, int& ty, int& tz) {
     tz = (int)(T / (nx * ny));
 }
 
+// One output of the symmetric correlation, rounded to float32 exactly as the reference's
+// float64 loop rounds it: centre product first, then acc += (w[-j] + w[+j]) * k[j] for
+// j = R2 .. 1, each product and sum rounded (gsift.py:119-147 via scipy's correlate1d).
+// Fast form: the products are fused (acc = fma(s, k, acc)).  Both forms evaluate the same
+// non-negative terms (image data and Gaussian taps are >= 0, the pair sums are identical), so
+// each lies within (2*R2 + 1) * 2^-53 * S of the exact sum S; they differ by < 41 * 2^-53 * S
+// < 2^-47 * acc.  If acc * (1 - 2^-46) and acc * (1 + 2^-46) round to the same float32, the
+// reference's value, which lies between them, rounds to it too (rounding is monotone);
+// otherwise (an f32 rounding boundary within 2^-46 relative, ~1 output in 2^21) the
+// reference's unfused order is evaluated.
+template <int R2>
+__device__ __forceinline__ float blur_point(const double* w, const HalfTaps& tp) {
+    double acc = w[R2] * tp.k[0];
+#pragma unroll
+    for (int j = R2; j >= 1; --j) acc = __builtin_fma(w[R2 - j] + w[R2 + j], tp.k[j], acc);
+    const float lo = (float)(acc * (1.0 - 0x1p-46)), hi = (float)(acc * (1.0 + 0x1p-46));
+    if (__builtin_expect(lo == hi, 1)) return lo;
+    double ex = w[R2] * tp.k[0];
+#pragma unroll
+    for (int j = R2; j >= 1; --j) ex += (w[R2 - j] + w[R2 + j]) * tp.k[j];
+    return (float)ex;
+}
 
 template <int R2, bool BASE>
-__global__ __launch_bounds__(256) void k_blur(const SiftJob* __restrict__ jobs, int o, int layer, HalfTaps tp) {
+__global__ __launch_bounds__(BL_NT) void k_blur(const SiftJob* __restrict__ jobs, int o, int layer, HalfTaps tp) {
     constexpr int TWo = BL_SW - 2 * R2;
-    constexpr int NW = BL_RB + 2 * R2;        // vertical window
+    constexpr int NW = BL_TH + 2 * R2;        // vertical window
     constexpr int HW = BL_CB + 2 * R2;        // horizontal window
     constexpr int NG = (TWo + BL_CB - 1) / BL_CB;
+    constexpr int NWV = BL_NT / 64;           // waves
+    constexpr int EPR = BL_TH / NWV;          // epilogue rows per wave
+    constexpr int EPC = (TWo + 63) / 64;      // epilogue column steps per lane
+    static_assert(BL_SW == BL_NT && BL_TH % NWV == 0, "one input column per thread");
     __shared__ float s_mid[BL_TH][BL_SW + 1];
     __shared__ float s_out[BL_TH][BL_SW + 1];
     int tx, ty, tz;
@@ -197,14 +226,14 @@ __global__ __launch_bounds__(256) void k_blur(const SiftJob* __restrict__ jobs, 
         else in = J.G4;
     }
     const int tid = threadIdx.x;
+    const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     {
-        const int c = tid & (BL_SW - 1);
-        const int rg = __builtin_amdgcn_readfirstlane(tid >> 7);
+        const int c = tid;
         const int gx = reflect101(x0 - R2 + c, C);
-        const int ybase = y0 + rg * BL_RB - R2;
+        const int ybase = y0 - R2;
         float f[NW];
         if (ybase >= 0 && ybase + NW <= R) {
-            // interior rows (wave-uniform test): direct addressing, no reflection
+            // interior rows (block-uniform test): direct addressing, no reflection
             if (BASE) {
 #pragma unroll
                 for (int k = 0; k < NW; k++) f[k] = up2_value(J.img, J.H, J.W, J.ld, ybase + k, gx);
@@ -224,15 +253,10 @@ __global__ __launch_bounds__(256) void k_blur(const SiftJob* __restrict__ jobs, 
 #pragma unroll
         for (int k = 0; k < NW; k++) w[k] = (double)f[k];
 #pragma unroll
-        for (int i = 0; i < BL_RB; i++) {
-            double acc = w[i + R2] * tp.k[0];
-#pragma unroll
-            for (int j = R2; j >= 1; --j) acc += (w[i + R2 - j] + w[i + R2 + j]) * tp.k[j];
-            s_mid[rg * BL_RB + i][c] = (float)acc;
-        }
+        for (int i = 0; i < BL_TH; i++) s_mid[i][c] = blur_point<R2>(w + i, tp);
     }
     __syncthreads();
-    for (int task = tid; task < BL_TH * NG; task += 256) {
+    for (int task = tid; task < BL_TH * NG; task += BL_NT) {
         const int ly = task % BL_TH, cg = task / BL_TH;
         const int cbase = cg * BL_CB;
         float fv[HW];
@@ -244,10 +268,7 @@ __global__ __launch_bounds__(256) void k_blur(const SiftJob* __restrict__ jobs, 
 #pragma unroll
         for (int j = 0; j < BL_CB; j++) {
             if (cbase + j >= TWo) continue;
-            double acc = v[j + R2] * tp.k[0];
-#pragma unroll
-            for (int t = R2; t >= 1; --t) acc += (v[j + R2 - t] + v[j + R2 + t]) * tp.k[t];
-            s_out[ly][cbase + j] = (float)acc;
+            s_out[ly][cbase + j] = blur_point<R2>(v + j, tp);
         }
     }
     __syncthreads();
@@ -268,21 +289,30 @@ __global__ __launch_bounds__(256) void k_blur(const SiftJob* __restrict__ jobs, 
             C2n = J.oct[o + 1].C;
         }
     }
-    const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-    for (int ly = wv; ly < BL_TH; ly += 4) {
-        const int gy = y0 + ly;
-        if (gy >= R) break;
-        for (int lx = lane; lx < TWo; lx += 64) {
-        const int gx = x0 + lx;
-        if (gx >= C) break;
-        const float g = s_out[ly][lx];
-        const size_t p = (size_t)gy * C + gx;
-        if (outG) outG[p] = g;
-        if (outD) outD[p] = g - gptr(in)[p];
-        if (down && !(gy & 1) && !(gx & 1)) {
-            const int ry = gy >> 1, rx = gx >> 1;
-            if (ry < R2n && rx < C2n) down[(size_t)ry * C2n + rx] = g;
+    float iv[EPR][EPC];
+#pragma unroll
+    for (int r = 0; r < EPR; r++)
+#pragma unroll
+        for (int cc = 0; cc < EPC; cc++) {
+            const int gy = y0 + wv + r * NWV, lx = lane + 64 * cc, gx = x0 + lx;
+            iv[r][cc] = (!BASE && gy < R && lx < TWo && gx < C) ? gptr(in)[(size_t)gy * C + gx] : 0.0f;
         }
+#pragma unroll
+    for (int r = 0; r < EPR; r++) {
+        const int ly = wv + r * NWV, gy = y0 + ly;
+#pragma unroll
+        for (int cc = 0; cc < EPC; cc++) {
+            const int lx = lane + 64 * cc, gx = x0 + lx;
+            if (gy < R && lx < TWo && gx < C) {
+                const float g = s_out[ly][lx];
+                const size_t p = (size_t)gy * C + gx;
+                if (outG) __builtin_nontemporal_store(g, &outG[p]);
+                if (outD) __builtin_nontemporal_store(g - iv[r][cc], &outD[p]);
+                if (down && !(gy & 1) && !(gx & 1)) {
+                    const int ry = gy >> 1, rx = gx >> 1;
+                    if (ry < R2n && rx < C2n) down[(size_t)ry * C2n + rx] = g;
+                }
+            }
         }
     }
 }
@@ -1209,7 +1239,7 @@ static void launch_blur(const SiftJob* jobs, int n_img, int o, int layer, int R,
                         hipStream_t st) {
     constexpr int TWo = blur_tw<R2>();
     dim3 grid((C + TWo - 1) / TWo, (R + BL_TH - 1) / BL_TH, n_img);
-    hipLaunchKernelGGL((k_blur<R2, false>), grid, dim3(256), 0, st, jobs, o, layer, h);
+    hipLaunchKernelGGL((k_blur<R2, false>), grid, dim3(BL_NT), 0, st, jobs, o, layer, h);
 }
 
 static int launch_layer(const SiftJob* jobs, int n_img, int o, int layer, int R, int C, hipStream_t st) {
@@ -1253,7 +1283,7 @@ static int run_pipeline(const SiftJob* all, int n_all, const Group* groups, int 
         const SiftPlan& p = g.plan;
         constexpr int TWo = blur_tw<5>();
         dim3 grid((p.C[0] + TWo - 1) / TWo, (p.R[0] + BL_TH - 1) / BL_TH, g.n);
-        hipLaunchKernelGGL((k_blur<5, true>), grid, dim3(256), 0, st, g.jobs, 0, 0, T.h[0]);
+        hipLaunchKernelGGL((k_blur<5, true>), grid, dim3(BL_NT), 0, st, g.jobs, 0, 0, T.h[0]);
         FSX_LAUNCH_CHECK();
         for (int o = 0; o < p.n_oct; o++)
             for (int layer = 1; layer <= N_LAYERS + 2; layer++) {

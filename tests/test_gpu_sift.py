@@ -1,6 +1,6 @@
 """GPU parity of the SIFT front end against the CPU oracle and the reference's golden vectors.
 
-Bars (north star): equal keypoint counts; x, y within 0.01 px; size and angle within 1e-3
+Bars: equal keypoint counts; x, y within 0.01 px; size and angle within 1e-3
 relative (angle compared on the circle); descriptors within 1 unit per bin.  The pyramid,
 extrema and refinement arithmetic is bit-exact, so x, y, response, octave and layer are
 expected to be identical to the oracle's bits.
