@@ -162,6 +162,42 @@ void fso_blur(const float *in, float *out, int R, int C, const float *k, int kle
     free(t);
 }
 
+/* CPU model of the device's fused blur evaluation (fieldstitch_amd/csrc/sift.hip,
+ * blur_point; not part of the reference).  For n windows of 2*r2+1 float32 values and the
+ * half taps kh[0..r2] (centre first), the reference order (as in fso_blur above) is compared
+ * with the fused form acc = fma(w[-j] + w[+j], k[j], acc); with use_check, the fused value is
+ * accepted only when acc*(1-margin) and acc*(1+margin) round to the same float32, else the
+ * reference order is evaluated (counted in *n_fallback).  Returns the number of windows whose
+ * float32 result differs from the reference's. */
+int64_t fso_fused_blur_check(const float *win, int64_t n, int r2, const float *kh, double margin, int use_check,
+                             int64_t *n_fallback) {
+    double k[64];
+    for (int j = 0; j <= r2; j++) k[j] = (double)kh[j];
+    int64_t bad = 0, fb = 0;
+    for (int64_t i = 0; i < n; i++) {
+        const float *w = win + i * (2 * r2 + 1);
+        double ref = (double)w[r2] * k[0];
+        for (int j = r2; j >= 1; j--) ref += ((double)w[r2 - j] + (double)w[r2 + j]) * k[j];
+        double acc = (double)w[r2] * k[0];
+        for (int j = r2; j >= 1; j--) acc = fma((double)w[r2 - j] + (double)w[r2 + j], k[j], acc);
+        float out;
+        if (use_check) {
+            const float lo = (float)(acc * (1.0 - margin)), hi = (float)(acc * (1.0 + margin));
+            if (lo == hi) {
+                out = lo;
+            } else {
+                out = (float)ref;
+                fb++;
+            }
+        } else {
+            out = (float)acc;
+        }
+        bad += out != (float)ref;
+    }
+    if (n_fallback) *n_fallback = fb;
+    return bad;
+}
+
 /* gsift.py:90-102 _up2 on img = u8 * float32(1/255) (gsift.py:160): rows then columns */
 void fso_up2_u8(const uint8_t *img, int H, int W, int ld, float *out) {
     const float s255 = (float)(1.0 / 255.0);

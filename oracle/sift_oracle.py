@@ -71,6 +71,9 @@ def lib():
         L.fso_sift_describe.argtypes = [C.c_void_p, _f32p, _f32p, _f32p, _f32p, _i32p, _i32p,
                                         _i64p, C.c_int64, _f32p]
         L.fso_rootsift_u8.argtypes = [_f32p, C.c_int64, _u8p]
+        L.fso_fused_blur_check.argtypes = [_f32p, C.c_int64, C.c_int, _f32p, C.c_double, C.c_int,
+                                           C.POINTER(C.c_int64)]
+        L.fso_fused_blur_check.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -99,6 +102,19 @@ def blur(img: np.ndarray, k: np.ndarray) -> np.ndarray:
     out = np.empty_like(img)
     lib().fso_blur(img, out, img.shape[0], img.shape[1], np.ascontiguousarray(k, np.float32), len(k))
     return out
+
+
+def fused_blur_check(windows: np.ndarray, half_taps: np.ndarray, margin: float, use_check: bool = True):
+    """Model of the device's fused blur evaluation against the reference order on (n, 2*r2+1)
+    float32 windows: returns (windows whose float32 result differs, fallbacks taken)."""
+    w = np.ascontiguousarray(windows, np.float32)
+    kh = np.ascontiguousarray(half_taps, np.float32)
+    r2 = (w.shape[1] - 1) // 2
+    assert w.shape[1] == 2 * r2 + 1 and len(kh) == r2 + 1
+    fb = C.c_int64(0)
+    bad = lib().fso_fused_blur_check(w.reshape(-1), w.shape[0], r2, kh, float(margin), int(bool(use_check)),
+                                     C.byref(fb))
+    return int(bad), int(fb.value)
 
 
 def up2_u8(gray: np.ndarray) -> np.ndarray:

@@ -187,3 +187,68 @@ def test_rootsift_closed_form():
     ref = d / (np.abs(d).sum(1, keepdims=True) + np.float32(1e-7))
     ref = np.clip(np.sqrt(ref) * np.float32(512.0), 0, 255).astype(np.uint8)
     np.testing.assert_array_equal(u, ref)
+
+
+def _ref_order(W, kh):
+    """The reference's float64 order (centre, then pairs outer -> inner), numpy, unfused."""
+    r2 = len(kh) - 1
+    k = kh.astype(np.float64)
+    acc = W[:, r2].astype(np.float64) * k[0]
+    for j in range(r2, 0, -1):
+        acc = acc + (W[:, r2 - j].astype(np.float64) + W[:, r2 + j].astype(np.float64)) * k[j]
+    return acc
+
+
+def _near_midpoint_windows(rng, n, kh):
+    """Windows whose reference-order value lies within a few float64 ulps of a float32 rounding
+    midpoint: random data, then two knobs (a coarse one at distance 1, a tiny one at the
+    outermost tap) that steer the sum to a target at most 8 ulps from the midpoint above it."""
+    r2 = len(kh) - 1
+    k = kh.astype(np.float64)
+    W = rng.random((n, 2 * r2 + 1), dtype=np.float32)
+    W[:, r2 + 1] = 0.0
+    W[:, 2 * r2] = 0.0
+    S0 = _ref_order(W, kh)
+    f = S0.astype(np.float32)
+    f = np.where(f.astype(np.float64) > S0, np.nextafter(f, np.float32(-1)), f)      # largest f32 <= S0
+    nf = np.nextafter(f, np.float32(2))
+    m = (f.astype(np.float64) + nf.astype(np.float64)) / 2.0
+    m = np.where(m <= S0, (nf.astype(np.float64) + np.nextafter(nf, np.float32(2)).astype(np.float64)) / 2.0, m)
+    target = m + rng.integers(-8, 9, n) * np.spacing(m)
+    c = ((target - S0) / k[1] * (1.0 - 2.0 ** -18)).astype(np.float32)
+    W[:, r2 + 1] = c
+    d1 = target - _ref_order(W, kh)
+    keep = d1 > 0
+    W, d1 = W[keep], d1[keep]
+    W[:, 2 * r2] = (d1 / k[r2]).astype(np.float32)
+    return W, _ref_order(W, kh)
+
+
+@pytest.mark.parametrize("layer", [0, 2, 3, 4, 5])
+def test_fused_blur_rounding_argument(layer):
+    """The device evaluates each blur output with fused multiply-adds and accepts the result only
+    when acc * (1 -+ 2^-46) round to the same float32 (else it re-evaluates in the reference's
+    order).  On windows steered to within 8 float64 ulps of a float32 rounding midpoint the fused form
+    alone disagrees with the reference (so the generator reaches the boundary), the checked
+    form never does, and a margin far below the fused form's error bound (2^-60) does not
+    suffice.  Ordinary image windows: zero disagreements and almost no fallbacks."""
+    full = so.gauss_kernel(so.sigmas()[layer])
+    r2 = len(full) // 2
+    kh = full[r2:].copy()
+    rng = np.random.default_rng(100 + layer)
+    W, S = _near_midpoint_windows(rng, 20000, kh)
+    assert len(W) > 19000
+    assert np.all(np.isfinite(S)) and np.all(S > 0)
+    bad_plain, _ = so.fused_blur_check(W, kh, 0.0, use_check=False)
+    assert bad_plain >= 50, bad_plain
+    bad, fb = so.fused_blur_check(W, kh, 2.0 ** -46, use_check=True)
+    assert bad == 0 and fb > 0.99 * len(W), (bad, fb)
+    bad_small, _ = so.fused_blur_check(W, kh, 2.0 ** -60, use_check=True)
+    assert bad_small > 0
+    # ordinary data: u8 / 255 windows and smooth windows
+    U = (rng.integers(0, 256, (200000, 2 * r2 + 1)).astype(np.float32) * np.float32(1 / 255))
+    bad_u, fb_u = so.fused_blur_check(U, kh, 2.0 ** -46, use_check=True)
+    assert bad_u == 0 and fb_u < 50, fb_u
+    G = rng.random((200000, 2 * r2 + 1), dtype=np.float32) ** 3
+    bad_g, fb_g = so.fused_blur_check(G, kh, 2.0 ** -46, use_check=True)
+    assert bad_g == 0 and fb_g < 50, fb_g
