@@ -136,10 +136,11 @@ __device__ __forceinline__ float up2_value(const uint8_t* __restrict__ img, int 
     int iy = y >> 1, ix = x >> 1;
     int ny = (y & 1) ? min(iy + 1, H - 1) : max(iy - 1, 0);
     int nx = (x & 1) ? min(ix + 1, W - 1) : max(ix - 1, 0);
-    float a_ii = (float)img[(size_t)iy * ld + ix] * s255;
-    float a_ni = (float)img[(size_t)ny * ld + ix] * s255;
-    float a_in = (float)img[(size_t)iy * ld + nx] * s255;
-    float a_nn = (float)img[(size_t)ny * ld + nx] * s255;
+    const __attribute__((address_space(1))) uint8_t* g = gptr(img);
+    float a_ii = (float)g[(size_t)iy * ld + ix] * s255;
+    float a_ni = (float)g[(size_t)ny * ld + ix] * s255;
+    float a_in = (float)g[(size_t)iy * ld + nx] * s255;
+    float a_nn = (float)g[(size_t)ny * ld + nx] * s255;
     float t_i = 0.75f * a_ii + 0.25f * a_ni;
     float t_n = 0.75f * a_in + 0.25f * a_nn;
     return 0.75f * t_i + 0.25f * t_n;
@@ -208,7 +209,7 @@ __global__ __launch_bounds__(BL_NT) void k_blur(const SiftJob* __restrict__ jobs
     constexpr int NWV = BL_NT / 64;           // waves
     constexpr int EPR = BL_TH / NWV;          // epilogue rows per wave
     constexpr int EPC = (TWo + 63) / 64;      // epilogue column steps per lane
-    static_assert(BL_SW == BL_NT && BL_TH % NWV == 0, "one input column per thread");
+    static_assert(BL_SW == BL_NT && BL_TH % NWV == 0 && BL_TH % 2 == 0, "one input column per thread, even rows");
     __shared__ float s_mid[BL_TH][BL_SW + 1];
     __shared__ float s_out[BL_TH][BL_SW + 1];
     int tx, ty, tz;
@@ -232,7 +233,33 @@ __global__ __launch_bounds__(BL_NT) void k_blur(const SiftJob* __restrict__ jobs
         const int gx = reflect101(x0 - R2 + c, C);
         const int ybase = y0 - R2;
         float f[NW];
-        if (ybase >= 0 && ybase + NW <= R) {
+        // base layer: ybase = 16*ty - R2 has the parity of R2, so in the interior the source
+        // rows of each upsampled row are fixed offsets from rb = (ybase >> 1) - 1; the thread
+        // loads its two source columns over those NI rows once (the same float32 operations
+        // as up2_value, no clamping needed inside [rb, rb + NI))
+        constexpr int PAR = R2 & 1;
+        constexpr int NI = ((PAR + NW - 1) >> 1) + 3;
+        const int rb = (ybase >> 1) - 1;
+        if (BASE && ybase >= 0 && rb >= 0 && rb + NI <= J.H) {
+            const float s255 = (float)(1.0 / 255.0);
+            const int ix = gx >> 1;
+            const int nx = (gx & 1) ? min(ix + 1, J.W - 1) : max(ix - 1, 0);
+            const __attribute__((address_space(1))) uint8_t* src = gptr(J.img) + (size_t)rb * J.ld;
+            float bi[NI], bn[NI];
+#pragma unroll
+            for (int r = 0; r < NI; r++) {
+                bi[r] = (float)src[(size_t)r * J.ld + ix] * s255;
+                bn[r] = (float)src[(size_t)r * J.ld + nx] * s255;
+            }
+#pragma unroll
+            for (int k = 0; k < NW; k++) {
+                const int iy = ((PAR + k) >> 1) + 1;
+                const int ny = ((PAR + k) & 1) ? iy + 1 : iy - 1;
+                const float t_i = 0.75f * bi[iy] + 0.25f * bi[ny];
+                const float t_n = 0.75f * bn[iy] + 0.25f * bn[ny];
+                f[k] = 0.75f * t_i + 0.25f * t_n;
+            }
+        } else if (ybase >= 0 && ybase + NW <= R) {
             // interior rows (block-uniform test): direct addressing, no reflection
             if (BASE) {
 #pragma unroll
