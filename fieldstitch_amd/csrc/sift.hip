@@ -146,7 +146,10 @@ __device__ __forceinline__ float up2_value(const uint8_t* __restrict__ img, int 
     return 0.75f * t_i + 0.25f * t_n;
 }
 
-// Tile: 256 input columns (TWo = 256 - 2*R2 output columns) x 16 output rows, 256 threads.
+// Tile: 256 input columns, TWo = 224 output columns (the largest multiple of 32 <= 256 - 2*R2:
+// tiles start on 128-byte boundaries of their row, so where rows are 128-byte aligned, as in
+// octave 0 of most photo widths, each row segment is written as whole lines) x 16 output rows,
+// 256 threads.
 // Vertical pass: a thread owns one input column and the 16 output rows; it streams the
 // 16 + 2*R2 input values of its column once from global memory (coalesced across the wave),
 // converts each to float64 once and accumulates from registers.  The float32-rounded result
@@ -154,14 +157,15 @@ __device__ __forceinline__ float up2_value(const uint8_t* __restrict__ img, int 
 // consecutive outputs of one row, reading 16 + 2*R2 mid values once.  Epilogue: lanes along
 // x; the previous layer's values for the DoG are loaded as one batch before any store (a
 // load after a store to a possibly aliasing address would wait a full memory latency per
-// element), the layer outputs are streamed out with non-temporal stores.
+// element).  The DoG layer, read again only by the extrema pass, is written with non-temporal
+// stores; the Gaussian layer, the next launch's input, with ordinary ones.
 constexpr int BL_SW = 256;
 constexpr int BL_NT = 256;
 constexpr int BL_TH = 16;
 constexpr int BL_CB = 16;
 
 template <int R2>
-constexpr int blur_tw() { return BL_SW - 2 * R2; }
+constexpr int blur_tw() { return (BL_SW - 2 * R2) & ~31; }
 
 // Workgroups are dispatched to the 8 XCDs round-robin by linear id; remap so that XCD k gets a
 // contiguous range of row-major tiles (a band of tile rows), keeping vertically adjacent tiles,
@@ -202,7 +206,7 @@ __device__ __forceinline__ float blur_point(const double* w, const HalfTaps& tp)
 
 template <int R2, bool BASE>
 __global__ __launch_bounds__(BL_NT) void k_blur(const SiftJob* __restrict__ jobs, int o, int layer, HalfTaps tp) {
-    constexpr int TWo = BL_SW - 2 * R2;
+    constexpr int TWo = blur_tw<R2>();
     constexpr int NW = BL_TH + 2 * R2;        // vertical window
     constexpr int HW = BL_CB + 2 * R2;        // horizontal window
     constexpr int NG = (TWo + BL_CB - 1) / BL_CB;
@@ -333,7 +337,7 @@ __global__ __launch_bounds__(BL_NT) void k_blur(const SiftJob* __restrict__ jobs
             if (gy < R && lx < TWo && gx < C) {
                 const float g = s_out[ly][lx];
                 const size_t p = (size_t)gy * C + gx;
-                if (outG) __builtin_nontemporal_store(g, &outG[p]);
+                if (outG) outG[p] = g;
                 if (outD) __builtin_nontemporal_store(g - iv[r][cc], &outD[p]);
                 if (down && !(gy & 1) && !(gx & 1)) {
                     const int ry = gy >> 1, rx = gx >> 1;
