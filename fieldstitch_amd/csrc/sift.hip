@@ -355,7 +355,7 @@ __global__ __launch_bounds__(BL_NT) void k_blur(const SiftJob* __restrict__ jobs
 // reference's candidate order; the final sort key carries it).
 
 constexpr int EX_W = 62;     // output columns per wave (lanes 1..62; lanes 0 and 63 are halo)
-constexpr int EX_H = 32;     // output rows per block
+constexpr int EX_H = 16;     // output rows per block (a sweep of 16..128: 16 is fastest)
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
