@@ -616,6 +616,14 @@ __device__ __forceinline__ void grad_at_f(const __attribute__((address_space(1))
 // (2*ORI_R_MAX+1)^2 window (own radius rint(4.5*scl)), smoothed, peaks >= 0.8 max with
 // parabolic interpolation.  Each peak is appended as an oriented keypoint with its sort key.
 
+constexpr float ORI_GUARD = 1e-4f;
+
+// float32 rounding of the float64 atan2, kept out of line: inlined, its registers would be
+// reserved for the whole orientation kernel although it runs for ~1 sample in 10^4
+__device__ __attribute__((noinline)) float exact_atan2_f32(float dy, float dx) {
+    return (float)atan2((double)dy, (double)dx);
+}
+
 __global__ __launch_bounds__(64) void k_orient(const SiftJob* __restrict__ jobs) {
     __shared__ unsigned long long s_h[ORI_BINS];
     __shared__ float s_f[ORI_BINS];
@@ -639,32 +647,54 @@ __global__ __launch_bounds__(64) void k_orient(const SiftJob* __restrict__ jobs)
         for (int d = lane; d <= dmax; d += 64) s_w[d] = (float)exp((double)(-(float)d / den));
         __syncthreads();
         // samples of the keypoint's own (2 rad + 1)^2 window (gsift.py:396-399 masks the rest)
+        //
+        // bin = rint(ANG * 0.1) with ANG from the float64-rounded atan2 (the reference's value).
+        // atan2f is within 3 ulp of that rounding (fsx_atan2_probe, 2^28 pairs; a test asserts
+        // <= 4): 4 ulp of |theta| <= pi is < 9.6e-7 rad; with the roundings of * deg, + 360 and
+        // * 0.1, ANG * 0.1 then differs from the reference's by < 1.4e-5, so the fast bin is the
+        // reference's unless ANG * 0.1 lies within ORI_GUARD = 1e-4 of a rounding boundary.
+        // Such samples are only marked here (bit `it` of `late`; a lane has <= 18 samples) and
+        // accumulated after the loop with the exact angle, which keeps the float64 atan2 and
+        // its registers out of the hot loop.
         const int W = 2 * rad + 1;
-        for (int s = lane; s < W * W; s += 64) {
+        const float deg = 180.0f / 3.14159265358979323846f;
+        auto sample = [&](int s, float& dx, float& dy, float& m) {
             const int i = s / W - rad, j = s - (s / W) * W - rad;
             const int y = r + i, x = c + j;
-            if (y > 0 && y < R - 1 && x > 0 && x < C - 1) {
-                // bin = rint(ANG * 0.1) with ANG from the float64-rounded atan2 (the reference's
-                // value); a float32 atan2 (a few ulp, < 1e-5 in ANG * 0.1) decides it unless
-                // ANG * 0.1 lies within 1e-3 of a rounding boundary, where the exact angle is used
-                const size_t kk = (size_t)y * C + x;
-                const float dx = G[kk + 1] - G[kk - 1];
-                const float dy = G[kk - C] - G[kk + C];
-                const float xx = dx * dx, yy = dy * dy;
-                const float mag = sqrtf(xx + yy);
-                const float deg = 180.0f / 3.14159265358979323846f;
-                float af = atan2f(dy, dx) * deg;
-                af = af < 0.f ? af + 360.0f : af;
-                const float t = af * 0.1f;
-                if (fabsf((t - floorf(t)) - 0.5f) < 1e-3f) {
-                    float ae = (float)atan2((double)dy, (double)dx) * deg;
-                    af = ae < 0.f ? ae + 360.0f : ae;
-                }
-                const float m = mag * s_w[i * i + j * j];
-                int bin = (int)rintf(af * 0.1f);
-                bin = bin >= ORI_BINS ? bin - ORI_BINS : bin;
-                atomicAdd(&s_h[bin], fx50((double)m));
+            if (!(y > 0 && y < R - 1 && x > 0 && x < C - 1)) return false;
+            const size_t kk = (size_t)y * C + x;
+            dx = G[kk + 1] - G[kk - 1];
+            dy = G[kk - C] - G[kk + C];
+            const float xx = dx * dx, yy = dy * dy;
+            m = sqrtf(xx + yy) * s_w[i * i + j * j];
+            return true;
+        };
+        auto accumulate = [&](float af, float m) {
+            int bin = (int)rintf(af * 0.1f);
+            bin = bin >= ORI_BINS ? bin - ORI_BINS : bin;
+            atomicAdd(&s_h[bin], fx50((double)m));
+        };
+        static_assert((2 * ORI_R_MAX + 1) * (2 * ORI_R_MAX + 1) <= 64 * 32, "sample index fits the mask");
+        unsigned late = 0u;
+        for (int s = lane, it = 0; s < W * W; s += 64, ++it) {
+            float dx, dy, m;
+            if (!sample(s, dx, dy, m)) continue;
+            float af = atan2f(dy, dx) * deg;
+            af = af < 0.f ? af + 360.0f : af;
+            const float t = af * 0.1f;
+            if (fabsf((t - floorf(t)) - 0.5f) < ORI_GUARD) {
+                late |= 1u << it;
+                continue;
             }
+            accumulate(af, m);
+        }
+        while (late) {
+            const int it = __builtin_ctz(late);
+            late &= late - 1u;
+            float dx, dy, m;
+            sample(lane + 64 * it, dx, dy, m);
+            const float ae = exact_atan2_f32(dy, dx) * deg;
+            accumulate(ae < 0.f ? ae + 360.0f : ae, m);
         }
         __syncthreads();
         if (lane < ORI_BINS) s_f[lane] = (float)((double)s_h[lane] * 0x1p-50);
@@ -721,6 +751,57 @@ __global__ __launch_bounds__(64) void k_orient(const SiftJob* __restrict__ jobs)
         }
         __syncthreads();
     }
+}
+
+// ------------------------------------------------------------------------------------------
+// atan2f accuracy probe: the largest difference, in float32 ulps (distance along the ordered
+// float32 line), between atan2f(y, x) and the float32 rounding of the float64 atan2 over n
+// seeded argument pairs, plus the zero / equal-magnitude / sign cases.  Components are
+// gradient-like: |y|, |x| < 1 with exponents 2^-1 .. 2^-40 (and, for 1 pair in 16, down to
+// the float32 denormals).  The orientation kernel's fast bin path relies on this bound.
+
+__device__ __forceinline__ unsigned long long probe_mix(unsigned long long x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+__device__ __forceinline__ float probe_value(unsigned h, bool deep) {
+    const unsigned sign = (h & 1u) << 31;
+    const unsigned span = deep ? 150u : 40u;
+    const int e = 126 - (int)((h >> 1) % span);                 // biased exponent 126 .. -23
+    const unsigned mant = (h >> 9) & 0x7fffffu;
+    const unsigned bits = e > 0 ? sign | ((unsigned)e << 23) | mant : sign | (mant >> (1 - e));
+    return __uint_as_float(bits);
+}
+
+__device__ __forceinline__ int ordered_f32(float f) {
+    const int a = __float_as_int(f);
+    return a < 0 ? (int)(0x80000000u - (unsigned)a) : a;
+}
+
+__global__ __launch_bounds__(256) void k_atan2_probe(long long n, unsigned long long seed, unsigned* __restrict__ out) {
+    unsigned m = 0;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const unsigned long long h = probe_mix(seed ^ probe_mix((unsigned long long)i));
+        const bool deep = (h >> 60) == 0;
+        float y = probe_value((unsigned)h, deep), x = probe_value((unsigned)(h >> 32), deep);
+        switch (i & 63) {                                      // special cases
+            case 0: x = 0.0f; break;
+            case 1: y = 0.0f; break;
+            case 2: y = x; break;
+            case 3: y = -x; break;
+            case 4: x = -0.0f; break;
+            default: break;
+        }
+        const float f = atan2f(y, x);
+        const float r = (float)atan2((double)y, (double)x);
+        const int d = ordered_f32(f) - ordered_f32(r);
+        m = max(m, (unsigned)(d < 0 ? -d : d));
+    }
+    for (int off = 32; off >= 1; off >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, m);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -862,8 +943,10 @@ __device__ __forceinline__ float wave_pairwise8(float v_lane_j_partial, int lane
     return ((r0 + r1) + (r2 + r3)) + ((r4 + r5) + (r6 + r7));
 }
 
-// one keypoint k of the arrays kp[7][stride], by a 256-thread block; s_h: HIST_SIZE u64,
-// s_d: 128 floats of LDS
+constexpr int DS_COPIES = 4;   // interleaved copies of the descriptor histogram (a sweep of 1..16: 4 is fastest)
+
+// one keypoint k of the arrays kp[7][stride], by a 256-thread block; s_h: HIST_SIZE * DS_COPIES
+// u64, s_d: 128 floats of LDS
 __device__ __forceinline__ void describe_kp(const SiftJob& J, const float* __restrict__ kpa_, int stride, long long k,
                                            float* __restrict__ desc_row, uint8_t* __restrict__ u8_row,
                                            unsigned long long* s_h, float* s_d) {
@@ -885,7 +968,7 @@ __device__ __forceinline__ void describe_kp(const SiftJob& J, const float* __res
     const double cos_t = (double)(float)cos((double)t) / hw, sin_t = (double)(float)sin((double)t) / hw;
     const long long pr = (long long)rint(py), pc = (long long)rint(px);
     const __attribute__((address_space(1))) float* G = gptr(J.G123 + J.oct[o].gbase + (long long)(li - 1) * R * C);
-    for (int b = threadIdx.x; b < HIST_SIZE; b += 256) s_h[b] = 0ull;
+    for (int b = threadIdx.x; b < HIST_SIZE * DS_COPIES; b += 256) s_h[b] = 0ull;
     // per row i of the window, the column span [jl, jh] outside of which |c_rot| or |r_rot|
     // >= 2.5 for sure (widened by one sample; the exact per-sample test below still decides):
     // only the rotated 5x5-cell square is visited instead of the whole (2 rad + 1)^2 window
@@ -962,23 +1045,31 @@ __device__ __forceinline__ void describe_kp(const SiftJob& J, const float* __res
         const double v101 = v_rc10 * ob, v100 = v_rc10 - v101;
         const double v011 = v_rc01 * ob, v010 = v_rc01 - v011;
         const double v001 = v_rc00 * ob, v000 = v_rc00 - v001;
-        const int idx = (((int)r0 + 1) * 6 + (int)c0 + 1) * 10 + o0;
+        // copy threadIdx % DS_COPIES of each bin: neighbouring samples, which mostly share
+        // their bins, spread over DS_COPIES addresses instead of contending for one
+        const int idx = ((((int)r0 + 1) * 6 + (int)c0 + 1) * 10 + o0) * DS_COPIES + (threadIdx.x % DS_COPIES);
         atomicAdd(&s_h[idx], fx50(v000));
-        atomicAdd(&s_h[idx + 1], fx50(v001));
-        atomicAdd(&s_h[idx + 10], fx50(v010));
-        atomicAdd(&s_h[idx + 11], fx50(v011));
-        atomicAdd(&s_h[idx + 60], fx50(v100));
-        atomicAdd(&s_h[idx + 61], fx50(v101));
-        atomicAdd(&s_h[idx + 70], fx50(v110));
-        atomicAdd(&s_h[idx + 71], fx50(v111));
+        atomicAdd(&s_h[idx + 1 * DS_COPIES], fx50(v001));
+        atomicAdd(&s_h[idx + 10 * DS_COPIES], fx50(v010));
+        atomicAdd(&s_h[idx + 11 * DS_COPIES], fx50(v011));
+        atomicAdd(&s_h[idx + 60 * DS_COPIES], fx50(v100));
+        atomicAdd(&s_h[idx + 61 * DS_COPIES], fx50(v101));
+        atomicAdd(&s_h[idx + 70 * DS_COPIES], fx50(v110));
+        atomicAdd(&s_h[idx + 71 * DS_COPIES], fx50(v111));
     }
     __syncthreads();
     if (threadIdx.x < 128) {
         const int b = threadIdx.x;
         const int rr = b >> 5, cc = (b >> 3) & 3, ob = b & 7;
         const int cell = ((rr + 1) * 6 + cc + 1) * 10;
-        float v = (float)((double)s_h[cell + ob] * 0x1p-50);
-        if (ob < 2) v = v + (float)((double)s_h[cell + 8 + ob] * 0x1p-50);
+        unsigned long long h0 = 0ull, h1 = 0ull;   // exact integer sums of the copies
+#pragma unroll
+        for (int q = 0; q < DS_COPIES; q++) {
+            h0 += s_h[(cell + ob) * DS_COPIES + q];
+            h1 += s_h[(cell + 8 + ob) * DS_COPIES + q];
+        }
+        float v = (float)((double)h0 * 0x1p-50);
+        if (ob < 2) v = v + (float)((double)h1 * 0x1p-50);
         s_d[b] = v;
     }
     __syncthreads();
@@ -1040,7 +1131,7 @@ __device__ __forceinline__ void describe_kp(const SiftJob& J, const float* __res
 }
 
 __global__ __launch_bounds__(256) void k_describe(const SiftJob* __restrict__ job, DescArgs A) {
-    __shared__ unsigned long long s_h[HIST_SIZE];
+    __shared__ unsigned long long s_h[HIST_SIZE * DS_COPIES];
     __shared__ float s_d[128];
     const int q = blockIdx.x;
     if (q >= A.m) return;
@@ -1050,7 +1141,7 @@ __global__ __launch_bounds__(256) void k_describe(const SiftJob* __restrict__ jo
 
 // feature batch: descriptor q of image blockIdx.z for its spread selection, in final order
 __global__ __launch_bounds__(256) void k_describe_sel(const SiftJob* __restrict__ jobs) {
-    __shared__ unsigned long long s_h[HIST_SIZE];
+    __shared__ unsigned long long s_h[HIST_SIZE * DS_COPIES];
     __shared__ float s_d[128];
     const SiftJob& J = jobs[blockIdx.z];
     const int q = blockIdx.x;
@@ -1572,6 +1663,15 @@ int fsx_sift_features_batch(int n_img, const uint8_t* const* imgs, const int* H,
     return fsx_sift_features_launch(n_img, imgs, H, W, ld, contrast, edge, max_kp, nfeat, cand_cap, kp_cap, ws,
                                     ws_bytes, kp_out, n_out_dev, desc_out, u8_out, table_dev, table_bytes,
                                     stage_events, stream);
+}
+
+// Largest atan2f error in float32 ulps over n seeded pairs (k_atan2_probe); *max_ulp_dev must
+// be zero on entry.
+int fsx_atan2_probe(long long n, unsigned long long seed, unsigned* max_ulp_dev, void* stream) {
+    FSX_REQUIRE(n >= 1 && max_ulp_dev, "bad atan2 probe arguments");
+    hipLaunchKernelGGL(k_atan2_probe, dim3(1024), dim3(256), 0, (hipStream_t)stream, n, seed, max_ulp_dev);
+    FSX_LAUNCH_CHECK();
+    return FSX_OK;
 }
 
 int fsx_sift_selection_offset(int H, int W, int cand_cap, int kp_cap, size_t* off) {

@@ -164,3 +164,14 @@ def describe(state: State, kps, sel, budget=2.5e6, timings=None, rootsift_u8=Fal
     if rootsift_u8:
         return d, u8.cpu().numpy()
     return d
+
+
+def atan2_probe(n: int = 1 << 26, seed: int = 0x5EED) -> int:
+    """Largest difference in float32 ulps between the device's atan2f and the float32 rounding
+    of the float64 atan2 over n seeded gradient-like argument pairs (no reference
+    counterpart: it bounds the orientation kernel's fast bin path)."""
+    torch = _torch()
+    out = torch.zeros(1, dtype=torch.int32, device="cuda")
+    _lib.call("fsx_atan2_probe", C.c_longlong(int(n)), C.c_ulonglong(int(seed)), C.c_void_p(out.data_ptr()),
+              C.c_void_p(_lib.stream_handle()))
+    return int(out.item())

@@ -193,6 +193,12 @@ int fsx_mfma_layout_probe(float *out_dev, void *stream);
  * check the lane layout the matcher's MFMA form assumes (no reference counterpart). */
 int fsx_mfma_i8_layout_probe(int *out_dev, void *stream);
 
+/* Largest difference in float32 ulps between the device atan2f and the float32 rounding of
+ * the float64 atan2 over n seeded gradient-like argument pairs and the zero / sign / equal
+ * magnitude cases; *max_ulp_dev must be zero on entry.  Bounds the orientation kernel's fast
+ * bin path (no reference counterpart). */
+int fsx_atan2_probe(long long n, unsigned long long seed, unsigned *max_ulp_dev, void *stream);
+
 /* ---------------------------------------------------------------------------------------
  * Float-descriptor brute-force matcher (matching.py:86-110 on float descriptors)
  * ------------------------------------------------------------------------------------- */
