@@ -374,13 +374,40 @@ __device__ __forceinline__ int wave_append(bool f, int* counter) {
 // rows; per row it loads the 5 DoG layers (coalesced), takes the horizontal max/min over
 // (x-1, x, x+1) with lane shuffles and keeps a 3-row ring, so the 3x3x3 max/min of a voxel
 // costs a few max/min operations.  v >= max27 is the reference's non-strict test (no NaN).
+// Candidates are staged per wave in LDS and appended EX_BUF at a time with one atomic on the
+// image's counter (an atomic per row with candidates, all on one address, cost a third of the
+// kernel).
+constexpr int EX_BUF = 256;
+static_assert(EX_BUF >= 3 * 64, "one row's candidates always fit an empty buffer");
+
 __global__ __launch_bounds__(256) void k_extrema(const SiftJob* __restrict__ jobs, int o) {
+    __shared__ uint32_t s_buf[4][EX_BUF];
     int tx, ty, tz;
     xcd_tile(tx, ty, tz);
     const SiftJob& J = jobs[tz];
     if (o >= J.n_oct) return;
     const int R = J.oct[o].R, C = J.oct[o].C;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint32_t* buf = s_buf[wv];
+    int nbuf = 0;   // staged candidates of this wave (wave-uniform)
+    // waves of a block may leave early, so the buffer is ordered with wave-level barriers
+    auto flush = [&]() {
+        if (nbuf == 0) return;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        int base = 0;
+        if (lane == 0) base = atomicAdd(&J.counters[0], nbuf);
+        base = __shfl(base, 0);
+        for (int i = lane; i < nbuf; i += 64) {
+            if (base + i < J.cand_cap) J.cand[base + i] = buf[i];
+            else atomicOr(&J.counters[3], 1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        nbuf = 0;
+    };
     const int xs = (tx * 4 + wv) * EX_W;                   // first output column of this wave
     const int y0 = ty * EX_H;
     if (xs >= C || y0 >= R) return;                        // wave-uniform (no block barrier below)
@@ -469,22 +496,17 @@ __global__ __launch_bounds__(256) void k_extrema(const SiftJob* __restrict__ job
             const unsigned long long b0 = __ballot(f[0]), b1 = __ballot(f[1]), b2 = __ballot(f[2]);
             const int n0 = __popcll(b0), n1 = __popcll(b1), n2 = __popcll(b2);
             if (n0 + n1 + n2 == 0) continue;                // wave-uniform
-            int base = 0;
-            if (lane == 0) base = atomicAdd(&J.counters[0], n0 + n1 + n2);
-            base = __shfl(base, 0);
+            if (nbuf + n0 + n1 + n2 > EX_BUF) flush();
             const unsigned long long below = (1ull << lane) - 1ull;
-            const int pos[3] = {base + (int)__popcll(b0 & below), base + n0 + (int)__popcll(b1 & below),
-                                base + n0 + n1 + (int)__popcll(b2 & below)};
+            const int pos[3] = {nbuf + (int)__popcll(b0 & below), nbuf + n0 + (int)__popcll(b1 & below),
+                                nbuf + n0 + n1 + (int)__popcll(b2 & below)};
 #pragma unroll
-            for (int l = 1; l <= N_LAYERS; l++) {
-                if (!f[l - 1]) continue;
-                if (pos[l - 1] < J.cand_cap)
-                    J.cand[pos[l - 1]] = (uint32_t)(J.oct[o].gbase + (long long)(l - 1) * P + (long long)y * C + x);
-                else
-                    atomicOr(&J.counters[3], 1);
-            }
+            for (int l = 1; l <= N_LAYERS; l++)
+                if (f[l - 1]) buf[pos[l - 1]] = (uint32_t)(J.oct[o].gbase + (long long)(l - 1) * P + (long long)y * C + x);
+            nbuf += n0 + n1 + n2;
         }
     }
+    flush();
 }
 
 // ------------------------------------------------------------------------------------------
@@ -624,13 +646,41 @@ __device__ __attribute__((noinline)) float exact_atan2_f32(float dy, float dx) {
     return (float)atan2((double)dy, (double)dx);
 }
 
+// Oriented keypoints are staged in LDS and appended ORI_BUF at a time with one atomic on the
+// image's counter (an atomic per keypoint, all on one address, cost more than the histograms).
+constexpr int ORI_BUF = 64;
+static_assert(ORI_BUF >= ORI_BINS, "one keypoint's peaks always fit an empty buffer");
+
 __global__ __launch_bounds__(64) void k_orient(const SiftJob* __restrict__ jobs) {
     __shared__ unsigned long long s_h[ORI_BINS];
+    __shared__ float4 s_rec[2 * ORI_BUF];
+    __shared__ unsigned long long s_k0[ORI_BUF];
+    __shared__ int s_rank[ORI_BUF];
     __shared__ float s_f[ORI_BINS];
     __shared__ float s_w[2 * ORI_R_MAX * ORI_R_MAX + 1];   // gaussian weight by i*i + j*j
     const SiftJob& J = jobs[blockIdx.z];
     const int n = min(J.counters[1], J.cand_cap);
     const int lane = threadIdx.x;
+    int nbuf = 0;   // staged keypoints (block = one wave: uniform)
+    auto flush = [&]() {
+        if (nbuf == 0) return;
+        int base = 0;
+        if (lane == 0) base = atomicAdd(&J.counters[2], nbuf);
+        base = __shfl(base, 0);
+        for (int i = lane; i < nbuf; i += 64) {
+            const int pos = base + i;
+            if (pos < J.kp_cap) {
+                J.oriented[2 * pos] = s_rec[2 * i];
+                J.oriented[2 * pos + 1] = s_rec[2 * i + 1];
+                J.keys0[2 * pos] = s_k0[i];
+                J.keys0[2 * pos + 1] = ((unsigned long long)s_rank[i] << 32) | (unsigned)pos;
+            } else {
+                atomicOr(&J.counters[3], 4);
+            }
+        }
+        nbuf = 0;
+        __syncthreads();   // the buffer is read before it is refilled
+    };
     for (int k = blockIdx.x; k < n; k += gridDim.x) {
         const float4 a = J.refined[2 * k], b = J.refined[2 * k + 1];
         const int o = __float_as_int(a.x), l = __float_as_int(a.y), r = __float_as_int(a.z), c = __float_as_int(a.w);
@@ -729,28 +779,26 @@ __global__ __launch_bounds__(64) void k_orient(const SiftJob* __restrict__ jobs)
         }
         const unsigned long long pm = __ballot(peak);
         const int rank = __popcll(pm & ((1ull << lane) - 1ull));
-        int pos = wave_append(peak, &J.counters[2]);
+        const int npk = __popcll(pm);
+        if (nbuf + npk > ORI_BUF) flush();
         if (peak) {
-            if (pos < J.kp_cap) {
-                const float f = (float)ldexp(1.0, o) * 0.5f;
-                const float x = ((float)c + b.x) * f;
-                const float y = ((float)r + b.y) * f;
-                const float size = (scl * 2.0f) * f;
-                const uint32_t src = J.refined_src[k];
-                J.oriented[2 * pos] = make_float4(x, y, size, angle);
-                J.oriented[2 * pos + 1] = make_float4(b.w, __int_as_float(o), __int_as_float(l), __uint_as_float(src));
-                // stable argsort of -response (gsift.py:471): response descending, then
-                // candidate order, then peak order within the keypoint
-                const unsigned long long k0 = ((unsigned long long)(~__float_as_uint(b.w)) << 32) | src;
-                const unsigned long long k1 = ((unsigned long long)rank << 32) | (unsigned)pos;
-                J.keys0[2 * pos] = k0;
-                J.keys0[2 * pos + 1] = k1;
-            } else {
-                atomicOr(&J.counters[3], 4);
-            }
+            const int i = nbuf + rank;
+            const float f = (float)ldexp(1.0, o) * 0.5f;
+            const float x = ((float)c + b.x) * f;
+            const float y = ((float)r + b.y) * f;
+            const float size = (scl * 2.0f) * f;
+            const uint32_t src = J.refined_src[k];
+            s_rec[2 * i] = make_float4(x, y, size, angle);
+            s_rec[2 * i + 1] = make_float4(b.w, __int_as_float(o), __int_as_float(l), __uint_as_float(src));
+            // stable argsort of -response (gsift.py:471): response descending, then
+            // candidate order, then peak order within the keypoint (k1 gets the slot at flush)
+            s_k0[i] = ((unsigned long long)(~__float_as_uint(b.w)) << 32) | src;
+            s_rank[i] = rank;
         }
+        nbuf += npk;
         __syncthreads();
     }
+    flush();
 }
 
 // ------------------------------------------------------------------------------------------
