@@ -1020,7 +1020,8 @@ __device__ __forceinline__ void describe_kp(const SiftJob& J, const float* __res
     // per row i of the window, the column span [jl, jh] outside of which |c_rot| or |r_rot|
     // >= 2.5 for sure (widened by one sample; the exact per-sample test below still decides):
     // only the rotated 5x5-cell square is visited instead of the whole (2 rad + 1)^2 window
-    __shared__ int s_jl[2 * 38 + 1 + 2], s_pre[2 * 38 + 2 + 2];
+    __shared__ int s_jl[2 * 38 + 1 + 2], s_pre[2 * 38 + 2 + 2], s_wtot;
+    __shared__ uint8_t s_row[(2 * 38 + 1) * (2 * 38 + 1)];   // window row of each span sample
     const int nrow = (int)(2 * rad + 1);
     const bool spans = nrow <= 2 * 38 + 1;
     if (spans) {
@@ -1043,10 +1044,27 @@ __device__ __forceinline__ void describe_kp(const SiftJob& J, const float* __res
             s_pre[row + 1] = n;
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            s_pre[0] = 0;
-            for (int row = 0; row < nrow; row++) s_pre[row + 1] += s_pre[row];
+        // inclusive scan of the per-row counts by the first two waves (nrow <= 77 < 128)
+        int v = 0;
+        if (threadIdx.x < 128) {
+            v = (int)threadIdx.x < nrow ? s_pre[threadIdx.x + 1] : 0;
+            const int ln = threadIdx.x & 63;
+            for (int off = 1; off < 64; off <<= 1) {
+                const int u = __shfl_up(v, off);
+                if (ln >= off) v += u;
+            }
+            if (threadIdx.x == 63) s_wtot = v;
         }
+        __syncthreads();
+        if (threadIdx.x < 128) {
+            if (threadIdx.x >= 64) v += s_wtot;
+            if ((int)threadIdx.x < nrow) s_pre[threadIdx.x + 1] = v;
+        }
+        if (threadIdx.x == 0) s_pre[0] = 0;
+        __syncthreads();
+        // the window row of every span sample (replaces a binary search per sample)
+        for (int row = threadIdx.x; row < nrow; row += 256)
+            for (int q = s_pre[row]; q < s_pre[row + 1]; q++) s_row[q] = (uint8_t)row;
     }
     __syncthreads();
     const int W = (int)(2 * rad + 1);
@@ -1055,12 +1073,7 @@ __device__ __forceinline__ void describe_kp(const SiftJob& J, const float* __res
     for (int s = threadIdx.x; s < total; s += 256) {
         int i, j;
         if (spans) {
-            int lo = 0, hi = nrow - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (s_pre[mid] <= s) lo = mid;
-                else hi = mid - 1;
-            }
+            const int lo = s_row[s];
             i = lo - (int)rad;
             j = s_jl[lo] + (s - s_pre[lo]);
         } else {
