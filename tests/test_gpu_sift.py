@@ -162,3 +162,16 @@ def test_orientation_bin_boundaries(gpu):
     assert_kps_match(k, kr)
     sel = np.arange(len(k["x"]))
     assert np.abs(gsift.describe(st, k, sel) - so.describe(sr, kr, sel)).max() <= 1.0
+
+
+def test_capacity_overflow_retry(gpu):
+    """Capacities far below the image's candidate and keypoint counts: the staged appends of the
+    extrema, refinement and orientation kernels flag the overflow, detect() grows the
+    capacities and runs again, and the result equals a run with ample capacities."""
+    from fieldstitch_amd import gsift
+    img, _, _ = synth.field_pair(1)
+    k_big, _ = gsift.detect(img, max_kp=4000, contrast=0.015, edge=12.0)
+    k_small, _ = gsift.detect(img, max_kp=4000, contrast=0.015, edge=12.0, caps=(256, 4096))
+    assert len(k_big["x"]) > 1000
+    for key in gsift.KEYS:
+        np.testing.assert_array_equal(k_small[key], k_big[key])
