@@ -77,6 +77,8 @@ SIGNATURES = {
     "fsx_match_candidate_offsets": [_i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), _i, C.POINTER(_i),
                                     C.POINTER(_i), C.c_double, C.POINTER(C.c_longlong)],
     "fsx_ransac_homography": [_i, _vp, _vp, _vp, _i, C.c_double, C.c_ulonglong, _vp, _vp, _vp, _vp],
+    "fsx_ransac_fundamental": [_i, _vp, _vp, _vp, _i, C.c_double, C.c_ulonglong, _vp, _vp, _vp, _vp],
+    "fsx_ransac_similarity": [_i, _vp, _vp, _vp, _i, C.c_double, C.c_ulonglong, _vp, _vp, _vp, _vp],
     "fsx_match_filter_features": [_vp, C.POINTER(C.c_longlong), _vp, _vp, _vp, _vp],
     "fsx_match_query_state": [_i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), _i, C.POINTER(_i), C.POINTER(_i),
                               C.c_double, _vp, C.POINTER(_i), C.POINTER(C.c_double), C.POINTER(_i), _vp],

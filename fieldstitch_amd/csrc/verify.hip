@@ -78,47 +78,208 @@ __device__ __forceinline__ bool is_inlier(const double* h, double x, double y, d
     return dx * dx + dy * dy < thr2;
 }
 
+// ---- models: K points per sample, solve(K points) -> 9 parameters, inlier test -------------
+
+// Homography (the role of cv2.findHomography in match_pair, matching.py:226): 4-point DLT,
+// h33 = 1, reprojection error in image b.
+struct ModelH {
+    static constexpr int K = 4;
+    __device__ static bool solve(const double* xa, const double* ya, const double* xb, const double* yb, double* m) {
+        return solve_h4(xa, ya, xb, yb, m);
+    }
+    __device__ static bool inlier(const double* m, double x, double y, double u, double v, double thr2) {
+        return is_inlier(m, x, y, u, v, thr2);
+    }
+};
+
+// Fundamental matrix (the role of cv2.findFundamentalMat in _verify, matching.py:175-191):
+// normalized 8-point algorithm.  Each image's 8 points are shifted to their centroid and scaled
+// to mean distance sqrt(2); the 8 x 9 system [u x, u y, u, v x, v y, v, x, y, 1] f = 0 is
+// reduced by Gaussian elimination with full pivoting (largest remaining |entry|, rows then
+// columns), the one column left unpivoted is set to 1 and the rest back-substituted, so no
+// entry of F is assumed non-zero; F = Tb^T F^ Ta.  No rank-2 projection.  Error: cv2's
+// FM_RANSAC one, the larger of the two squared point-to-epipolar-line distances.
+struct ModelF {
+    static constexpr int K = 8;
+    __device__ static bool solve(const double* xa, const double* ya, const double* xb, const double* yb, double* m) {
+        double ca[2] = {0.0, 0.0}, cb[2] = {0.0, 0.0};
+        for (int k = 0; k < 8; k++) {
+            ca[0] += xa[k];
+            ca[1] += ya[k];
+            cb[0] += xb[k];
+            cb[1] += yb[k];
+        }
+        for (int q = 0; q < 2; q++) {
+            ca[q] /= 8.0;
+            cb[q] /= 8.0;
+        }
+        double ra = 0.0, rb = 0.0;
+        for (int k = 0; k < 8; k++) {
+            ra += sqrt((xa[k] - ca[0]) * (xa[k] - ca[0]) + (ya[k] - ca[1]) * (ya[k] - ca[1]));
+            rb += sqrt((xb[k] - cb[0]) * (xb[k] - cb[0]) + (yb[k] - cb[1]) * (yb[k] - cb[1]));
+        }
+        if (!(ra > 1e-12 && rb > 1e-12)) return false;
+        const double sa = 1.4142135623730951 * 8.0 / ra, sb = 1.4142135623730951 * 8.0 / rb;
+        double A[8][9];
+        for (int k = 0; k < 8; k++) {
+            const double x = (xa[k] - ca[0]) * sa, y = (ya[k] - ca[1]) * sa;
+            const double u = (xb[k] - cb[0]) * sb, v = (yb[k] - cb[1]) * sb;
+            double* r = A[k];
+            r[0] = u * x; r[1] = u * y; r[2] = u;
+            r[3] = v * x; r[4] = v * y; r[5] = v;
+            r[6] = x; r[7] = y; r[8] = 1.0;
+        }
+        int perm[9];
+        for (int k = 0; k < 9; k++) perm[k] = k;
+        for (int c = 0; c < 8; c++) {
+            int pr = c, pc = c;
+            double best = -1.0;
+            for (int r = c; r < 8; r++)
+                for (int k = c; k < 9; k++) {
+                    const double a = fabs(A[r][k]);
+                    if (a > best) {
+                        best = a;
+                        pr = r;
+                        pc = k;
+                    }
+                }
+            if (best < 1e-12) return false;
+            if (pr != c)
+                for (int k = 0; k < 9; k++) {
+                    const double t = A[c][k];
+                    A[c][k] = A[pr][k];
+                    A[pr][k] = t;
+                }
+            if (pc != c) {
+                for (int r = 0; r < 8; r++) {
+                    const double t = A[r][c];
+                    A[r][c] = A[r][pc];
+                    A[r][pc] = t;
+                }
+                const int t = perm[c];
+                perm[c] = perm[pc];
+                perm[pc] = t;
+            }
+            for (int r = c + 1; r < 8; r++) {
+                const double f = A[r][c] / A[c][c];
+                for (int k = c; k < 9; k++) A[r][k] = A[r][k] - f * A[c][k];
+            }
+        }
+        double fz[9];   // solution in pivot order; the free (last) one is 1
+        fz[8] = 1.0;
+        for (int c = 7; c >= 0; c--) {
+            double s = 0.0;
+            for (int k = c + 1; k < 9; k++) s = s + A[c][k] * fz[k];
+            fz[c] = -s / A[c][c];
+        }
+        double fh[9];
+        for (int k = 0; k < 9; k++) fh[perm[k]] = fz[k];
+        // F = Tb^T F^ Ta with T = [[s, 0, -s cx], [0, s, -s cy], [0, 0, 1]]
+        const double Ta[3][3] = {{sa, 0.0, -sa * ca[0]}, {0.0, sa, -sa * ca[1]}, {0.0, 0.0, 1.0}};
+        const double Tb[3][3] = {{sb, 0.0, -sb * cb[0]}, {0.0, sb, -sb * cb[1]}, {0.0, 0.0, 1.0}};
+        double G[3][3];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                G[i][j] = (fh[3 * i + 0] * Ta[0][j] + fh[3 * i + 1] * Ta[1][j]) + fh[3 * i + 2] * Ta[2][j];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) m[3 * i + j] = (Tb[0][i] * G[0][j] + Tb[1][i] * G[1][j]) + Tb[2][i] * G[2][j];
+        return true;
+    }
+    __device__ static bool inlier(const double* F, double x, double y, double u, double v, double thr2) {
+        const double a2 = (F[0] * x + F[1] * y) + F[2], b2 = (F[3] * x + F[4] * y) + F[5],
+                     c2 = (F[6] * x + F[7] * y) + F[8];
+        const double a1 = (F[0] * u + F[3] * v) + F[6], b1 = (F[1] * u + F[4] * v) + F[7],
+                     c1 = (F[2] * u + F[5] * v) + F[8];
+        const double n2 = a2 * a2 + b2 * b2, n1 = a1 * a1 + b1 * b1;
+        if (!(n1 > 1e-30 && n2 > 1e-30)) return false;
+        const double d2 = (u * a2 + v * b2) + c2, d1 = (x * a1 + y * b1) + c1;
+        const double e2 = d2 * d2 / n2, e1 = d1 * d1 / n1;
+        return (e1 > e2 ? e1 : e2) < thr2;
+    }
+};
+
+// Similarity (the role of cv2.estimateAffinePartial2D in _similarity, matching.py:194-202):
+// from 2 points, M = [[a, -b, tx], [b, a, ty]]; stored as m[0..5] = a, -b, tx, b, a, ty.
+struct ModelS {
+    static constexpr int K = 2;
+    __device__ static bool solve(const double* xa, const double* ya, const double* xb, const double* yb, double* m) {
+        const double dx = xa[1] - xa[0], dy = ya[1] - ya[0];
+        const double ex = xb[1] - xb[0], ey = yb[1] - yb[0];
+        const double den = dx * dx + dy * dy;
+        if (!(den > 1e-12)) return false;
+        const double a = (dx * ex + dy * ey) / den, b = (dx * ey - dy * ex) / den;
+        m[0] = a;
+        m[1] = -b;
+        m[2] = xb[0] - (a * xa[0] - b * ya[0]);
+        m[3] = b;
+        m[4] = a;
+        m[5] = yb[0] - (b * xa[0] + a * ya[0]);
+        m[6] = 0.0;
+        m[7] = 0.0;
+        m[8] = 1.0;
+        return true;
+    }
+    __device__ static bool inlier(const double* m, double x, double y, double u, double v, double thr2) {
+        const double px = (m[0] * x + m[1] * y) + m[2], py = (m[3] * x + m[4] * y) + m[5];
+        const double dx = px - u, dy = py - v;
+        return dx * dx + dy * dy < thr2;
+    }
+};
+
+// draw the K distinct matches of hypothesis hy of pair p (16 attempts per slot)
+template <int K>
+__device__ bool draw_sample(unsigned long long seed, int p, int hy, int n, int* idx) {
+    for (int j = 0; j < K; j++) {
+        int cand = -1;
+        for (int attempt = 0; attempt < 16; attempt++) {
+            const int c = (int)(draw(seed, p, hy, j * 16 + attempt) % (unsigned long long)n);
+            bool dup = false;
+            for (int u = 0; u < j; u++) dup |= idx[u] == c;
+            if (!dup) {
+                cand = c;
+                break;
+            }
+        }
+        if (cand < 0) return false;
+        idx[j] = cand;
+    }
+    return true;
+}
+
+template <class M>
+__device__ bool hypothesis(const double* A, const double* Bq, unsigned long long seed, int p, int hy, int n, double* m) {
+    int idx[M::K];
+    if (!draw_sample<M::K>(seed, p, hy, n, idx)) return false;
+    double xa[M::K], ya[M::K], xb[M::K], yb[M::K];
+    for (int j = 0; j < M::K; j++) {
+        xa[j] = A[2 * idx[j]];
+        ya[j] = A[2 * idx[j] + 1];
+        xb[j] = Bq[2 * idx[j]];
+        yb[j] = Bq[2 * idx[j] + 1];
+    }
+    return M::solve(xa, ya, xb, yb, m);
+}
+
+template <class M>
 __global__ __launch_bounds__(256) void k_ransac(const long long* __restrict__ off, const double* __restrict__ pa,
                                                 const double* __restrict__ pb, int n_hyp, double thr2,
                                                 unsigned long long seed, uint8_t* __restrict__ inlier,
                                                 double* __restrict__ H_out, int* __restrict__ n_in) {
     __shared__ int s_cnt[256], s_hyp[256];
     __shared__ double s_h[9];
+    __shared__ int s_have;
     const int p = blockIdx.x, tid = threadIdx.x;
     const long long o = off[p];
     const int n = (int)(off[p + 1] - o);
     const double* A = pa + 2 * o;
     const double* Bq = pb + 2 * o;
     int best = -1, best_h = 0x7fffffff;
-    if (n >= 4) {
+    if (n >= M::K) {
         for (int hy = tid; hy < n_hyp; hy += 256) {
-            int idx[4];
-            bool ok = true;
-            for (int j = 0; j < 4 && ok; j++) {
-                int cand = -1;
-                for (int attempt = 0; attempt < 16; attempt++) {
-                    const int c = (int)(draw(seed, p, hy, j * 16 + attempt) % (unsigned long long)n);
-                    bool dup = false;
-                    for (int u = 0; u < j; u++) dup |= idx[u] == c;
-                    if (!dup) {
-                        cand = c;
-                        break;
-                    }
-                }
-                if (cand < 0) ok = false;
-                else idx[j] = cand;
-            }
-            if (!ok) continue;
-            double xa[4], ya[4], xb[4], yb[4], h[9];
-            for (int j = 0; j < 4; j++) {
-                xa[j] = A[2 * idx[j]];
-                ya[j] = A[2 * idx[j] + 1];
-                xb[j] = Bq[2 * idx[j]];
-                yb[j] = Bq[2 * idx[j] + 1];
-            }
-            if (!solve_h4(xa, ya, xb, yb, h)) continue;
+            double h[9];
+            if (!hypothesis<M>(A, Bq, seed, p, hy, n, h)) continue;
             int cnt = 0;
-            for (int i = 0; i < n; i++) cnt += is_inlier(h, A[2 * i], A[2 * i + 1], Bq[2 * i], Bq[2 * i + 1], thr2);
+            for (int i = 0; i < n; i++) cnt += M::inlier(h, A[2 * i], A[2 * i + 1], Bq[2 * i], Bq[2 * i + 1], thr2);
             if (cnt > best || (cnt == best && hy < best_h)) {
                 best = cnt;
                 best_h = hy;
@@ -136,67 +297,64 @@ __global__ __launch_bounds__(256) void k_ransac(const long long* __restrict__ of
                 bh = s_hyp[u];
             }
         // rebuild the winning model (the same draws and solve as above)
-        bool have = false;
-        if (bc >= 0) {
-            int idx[4];
-            bool ok = true;
-            for (int j = 0; j < 4 && ok; j++) {
-                int cand = -1;
-                for (int attempt = 0; attempt < 16; attempt++) {
-                    const int c = (int)(draw(seed, p, bh, j * 16 + attempt) % (unsigned long long)n);
-                    bool dup = false;
-                    for (int u = 0; u < j; u++) dup |= idx[u] == c;
-                    if (!dup) {
-                        cand = c;
-                        break;
-                    }
-                }
-                if (cand < 0) ok = false;
-                else idx[j] = cand;
-            }
-            double xa[4], ya[4], xb[4], yb[4], h[9];
-            for (int j = 0; ok && j < 4; j++) {
-                xa[j] = A[2 * idx[j]];
-                ya[j] = A[2 * idx[j] + 1];
-                xb[j] = Bq[2 * idx[j]];
-                yb[j] = Bq[2 * idx[j] + 1];
-            }
-            have = ok && solve_h4(xa, ya, xb, yb, h);
-            for (int k = 0; k < 9; k++) s_h[k] = have ? h[k] : 0.0;
-        }
-        if (!have)
-            for (int k = 0; k < 9; k++) s_h[k] = 0.0;
+        double h[9];
+        const bool have = bc >= 0 && hypothesis<M>(A, Bq, seed, p, bh, n, h);
+        for (int k = 0; k < 9; k++) s_h[k] = have ? h[k] : 0.0;
+        s_have = have;
         n_in[p] = have ? bc : 0;
         for (int k = 0; k < 9; k++) H_out[9 * p + k] = s_h[k];
     }
     __syncthreads();
-    const bool have = s_h[8] == 1.0;
+    const bool have = s_have != 0;
     double h[9];
     for (int k = 0; k < 9; k++) h[k] = s_h[k];
     for (int i = tid; i < n; i += 256)
-        inlier[o + i] = (uint8_t)(have && is_inlier(h, A[2 * i], A[2 * i + 1], Bq[2 * i], Bq[2 * i + 1], thr2));
+        inlier[o + i] = (uint8_t)(have && M::inlier(h, A[2 * i], A[2 * i + 1], Bq[2 * i], Bq[2 * i + 1], thr2));
 }
 
 }  // namespace fsx
 
 using namespace fsx;
 
-extern "C" {
-
-// Deterministic homography RANSAC for n_pair sets of matches: pair p's matches are rows
-// off_dev[p] .. off_dev[p+1] of pa_dev / pb_dev (float64 (x, y) in image a / image b).  n_hyp
-// hypotheses per pair, inlier iff reprojection error < thr px.  Writes inlier_dev (one byte per
-// match), H_dev (9 float64 per pair, all zero when no model) and n_in_dev (per pair).
-int fsx_ransac_homography(int n_pair, const long long* off_dev, const double* pa_dev, const double* pb_dev, int n_hyp,
-                          double thr, unsigned long long seed, uint8_t* inlier_dev, double* H_dev, int* n_in_dev,
-                          void* stream) {
+template <class M>
+static int launch_ransac(int n_pair, const long long* off_dev, const double* pa_dev, const double* pb_dev, int n_hyp,
+                         double thr, unsigned long long seed, uint8_t* inlier_dev, double* H_dev, int* n_in_dev,
+                         void* stream) {
     FSX_REQUIRE(n_pair >= 0 && n_hyp >= 1 && n_hyp <= (1 << 20) && thr > 0, "bad RANSAC arguments");
     if (n_pair == 0) return FSX_OK;
     FSX_REQUIRE(off_dev && pa_dev && pb_dev && inlier_dev && H_dev && n_in_dev, "null pointer argument");
-    hipLaunchKernelGGL(k_ransac, dim3(n_pair), dim3(256), 0, (hipStream_t)stream, off_dev, pa_dev, pb_dev, n_hyp,
+    hipLaunchKernelGGL(k_ransac<M>, dim3(n_pair), dim3(256), 0, (hipStream_t)stream, off_dev, pa_dev, pb_dev, n_hyp,
                        thr * thr, seed, inlier_dev, H_dev, n_in_dev);
     FSX_LAUNCH_CHECK();
     return FSX_OK;
+}
+
+extern "C" {
+
+// Deterministic RANSAC for n_pair sets of matches: pair p's matches are rows off_dev[p] ..
+// off_dev[p+1] of pa_dev / pb_dev (float64 (x, y) in image a / image b).  n_hyp hypotheses
+// per pair (hypothesis h draws its sample with a counter hash of (seed, p, h)); the one with
+// the most inliers wins (ties: lowest h).  Writes inlier_dev (one byte per match), H_dev (9
+// float64 per pair, all zero when no model) and n_in_dev (per pair).
+int fsx_ransac_homography(int n_pair, const long long* off_dev, const double* pa_dev, const double* pb_dev, int n_hyp,
+                          double thr, unsigned long long seed, uint8_t* inlier_dev, double* H_dev, int* n_in_dev,
+                          void* stream) {
+    return launch_ransac<ModelH>(n_pair, off_dev, pa_dev, pb_dev, n_hyp, thr, seed, inlier_dev, H_dev, n_in_dev,
+                                 stream);
+}
+
+int fsx_ransac_fundamental(int n_pair, const long long* off_dev, const double* pa_dev, const double* pb_dev,
+                           int n_hyp, double thr, unsigned long long seed, uint8_t* inlier_dev, double* F_dev,
+                           int* n_in_dev, void* stream) {
+    return launch_ransac<ModelF>(n_pair, off_dev, pa_dev, pb_dev, n_hyp, thr, seed, inlier_dev, F_dev, n_in_dev,
+                                 stream);
+}
+
+int fsx_ransac_similarity(int n_pair, const long long* off_dev, const double* pa_dev, const double* pb_dev,
+                          int n_hyp, double thr, unsigned long long seed, uint8_t* inlier_dev, double* M_dev,
+                          int* n_in_dev, void* stream) {
+    return launch_ransac<ModelS>(n_pair, off_dev, pa_dev, pb_dev, n_hyp, thr, seed, inlier_dev, M_dev, n_in_dev,
+                                 stream);
 }
 
 }  // extern "C"

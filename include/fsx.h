@@ -271,6 +271,22 @@ int fsx_ransac_homography(int n_pair, const long long *off_dev, const double *pa
                           double thr, unsigned long long seed, uint8_t *inlier_dev, double *H_dev, int *n_in_dev,
                           void *stream);
 
+/* Deterministic fundamental-matrix RANSAC (the role of cv2.findFundamentalMat in _verify,
+ * matching.py:175-191): same layout and sampling as fsx_ransac_homography; normalized 8-point
+ * algorithm (full-pivot elimination of the 8 x 9 system, no entry of F fixed), inlier iff the
+ * larger squared point-to-epipolar-line distance < thr^2.  F_dev: 9 float64 per pair (row-major,
+ * x_b^T F x_a = 0). */
+int fsx_ransac_fundamental(int n_pair, const long long *off_dev, const double *pa_dev, const double *pb_dev,
+                           int n_hyp, double thr, unsigned long long seed, uint8_t *inlier_dev, double *F_dev,
+                           int *n_in_dev, void *stream);
+
+/* Deterministic similarity RANSAC (the role of cv2.estimateAffinePartial2D in _similarity,
+ * matching.py:194-202): 2-point samples, M = [[a, -b, tx], [b, a, ty]] in M_dev[0..5] of each
+ * pair's 9 float64, inlier iff |M x_a - x_b| < thr. */
+int fsx_ransac_similarity(int n_pair, const long long *off_dev, const double *pa_dev, const double *pb_dev,
+                          int n_hyp, double thr, unsigned long long seed, uint8_t *inlier_dev, double *M_dev,
+                          int *n_in_dev, void *stream);
+
 /* The learned filter's inputs for every query of the last matcher batch on ws (launch_info of
  * fsx_match_prepare): row g (bf16, 288 wide; zero when the query has no candidate) =
  * |a - b| / 255 (128), a * b / 255^2 (128), prior offset of the candidate / radius (2),

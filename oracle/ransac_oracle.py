@@ -1,7 +1,8 @@
-"""CPU restatement of the deterministic homography RANSAC (test infrastructure only): the same
-counter-hash sampling, 4-point DLT solve (Gaussian elimination with partial pivoting, float64,
-no contraction) and inlier test as fieldstitch_amd/csrc/verify.hip, vectorised over the
-hypotheses, so every count and label can be compared exactly."""
+"""CPU restatement of the deterministic RANSAC (test infrastructure only): the same
+counter-hash sampling, model solves (4-point homography DLT with partial pivoting; normalized
+8-point fundamental matrix with full pivoting; 2-point similarity; float64, no contraction,
+sums in the device's order) and inlier tests as fieldstitch_amd/csrc/verify.hip, vectorised
+over the hypotheses, so every count and label can be compared exactly."""
 from __future__ import annotations
 
 import numpy as np
@@ -22,12 +23,12 @@ def draw(seed, p, h, j):
     return splitmix64(np.uint64(seed) ^ splitmix64(inner))
 
 
-def samples(seed, p, n, hyps):
-    """(n_hyp, 4) indices and a validity mask."""
+def samples(seed, p, n, hyps, K=4):
+    """(n_hyp, K) indices and a validity mask."""
     hyps = np.asarray(hyps, np.int64)
-    idx = np.full((len(hyps), 4), -1, np.int64)
+    idx = np.full((len(hyps), K), -1, np.int64)
     ok = np.ones(len(hyps), bool)
-    for j in range(4):
+    for j in range(K):
         cand = np.full(len(hyps), -1, np.int64)
         for attempt in range(16):
             c = (draw(seed, p, hyps, j * 16 + attempt) % np.uint64(n)).astype(np.int64)
@@ -90,20 +91,139 @@ def inliers(h, a, b, thr2):
     return good & (dx * dx + dy * dy < thr2)
 
 
-def ransac_pair(p, a, b, n_hyp, thr, seed):
-    """Returns (inlier flags (n,), H (9,), count) for pair index p."""
+def solve_f8(xa, ya, xb, yb):
+    """Batched normalized 8-point fundamental matrix: arrays (m, 8) -> (F (m, 9), valid (m,)).
+    Centroid / mean-distance normalization per image, full-pivot elimination of the 8 x 9
+    system, free unknown = 1, F = Tb^T F^ Ta (verify.hip ModelF)."""
+    m = xa.shape[0]
+    r_ = np.arange(m)
+    ca = [np.zeros(m), np.zeros(m)]
+    cb = [np.zeros(m), np.zeros(m)]
+    for k in range(8):
+        ca[0] = ca[0] + xa[:, k]
+        ca[1] = ca[1] + ya[:, k]
+        cb[0] = cb[0] + xb[:, k]
+        cb[1] = cb[1] + yb[:, k]
+    ca = [c / 8.0 for c in ca]
+    cb = [c / 8.0 for c in cb]
+    ra = np.zeros(m)
+    rb = np.zeros(m)
+    for k in range(8):
+        ra = ra + np.sqrt((xa[:, k] - ca[0]) * (xa[:, k] - ca[0]) + (ya[:, k] - ca[1]) * (ya[:, k] - ca[1]))
+        rb = rb + np.sqrt((xb[:, k] - cb[0]) * (xb[:, k] - cb[0]) + (yb[:, k] - cb[1]) * (yb[:, k] - cb[1]))
+    valid = (ra > 1e-12) & (rb > 1e-12)
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        sa = 1.4142135623730951 * 8.0 / ra
+        sb = 1.4142135623730951 * 8.0 / rb
+        A = np.zeros((m, 8, 9))
+        for k in range(8):
+            x = (xa[:, k] - ca[0]) * sa
+            y = (ya[:, k] - ca[1]) * sa
+            u = (xb[:, k] - cb[0]) * sb
+            v = (yb[:, k] - cb[1]) * sb
+            A[:, k] = np.stack([u * x, u * y, u, v * x, v * y, v, x, y, np.ones(m)], axis=1)
+        perm = np.tile(np.arange(9), (m, 1))
+        for c in range(8):
+            blk = np.abs(A[:, c:, c:]).reshape(m, -1)
+            flat = np.argmax(blk, axis=1)
+            best = blk[r_, flat]
+            valid &= best >= 1e-12
+            pr = c + flat // (9 - c)
+            pc = c + flat % (9 - c)
+            rc, rp = A[r_, c].copy(), A[r_, pr].copy()
+            A[r_, c], A[r_, pr] = rp, rc
+            cc, cp = A[r_, :, c].copy(), A[r_, :, pc].copy()
+            A[r_, :, c], A[r_, :, pc] = cp, cc
+            qc, qp = perm[r_, c].copy(), perm[r_, pc].copy()
+            perm[r_, c], perm[r_, pc] = qp, qc
+            for r in range(c + 1, 8):
+                f = A[:, r, c] / A[:, c, c]
+                for k in range(c, 9):
+                    A[:, r, k] = A[:, r, k] - f * A[:, c, k]
+        fz = np.zeros((m, 9))
+        fz[:, 8] = 1.0
+        for c in range(7, -1, -1):
+            s_ = np.zeros(m)
+            for k in range(c + 1, 9):
+                s_ = s_ + A[:, c, k] * fz[:, k]
+            fz[:, c] = -s_ / A[:, c, c]
+        fh = np.zeros((m, 9))
+        fh[r_[:, None], perm] = fz
+        z = np.zeros(m)
+        o = np.ones(m)
+        Ta = [[sa, z, -sa * ca[0]], [z, sa, -sa * ca[1]], [z, z, o]]
+        Tb = [[sb, z, -sb * cb[0]], [z, sb, -sb * cb[1]], [z, z, o]]
+        G = [[(fh[:, 3 * i] * Ta[0][j] + fh[:, 3 * i + 1] * Ta[1][j]) + fh[:, 3 * i + 2] * Ta[2][j] for j in range(3)]
+             for i in range(3)]
+        F = np.zeros((m, 9))
+        for i in range(3):
+            for j in range(3):
+                F[:, 3 * i + j] = (Tb[0][i] * G[0][j] + Tb[1][i] * G[1][j]) + Tb[2][i] * G[2][j]
+    return F, valid
+
+
+def inliers_f(F, a, b, thr2):
+    """F (m, 9); a, b (n, 2) -> (m, n) bool: max squared point-to-epipolar-line distance < thr2."""
+    x, y, u, v = a[None, :, 0], a[None, :, 1], b[None, :, 0], b[None, :, 1]
+    f = [F[:, k:k + 1] for k in range(9)]
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        a2, b2, c2 = (f[0] * x + f[1] * y) + f[2], (f[3] * x + f[4] * y) + f[5], (f[6] * x + f[7] * y) + f[8]
+        a1, b1, c1 = (f[0] * u + f[3] * v) + f[6], (f[1] * u + f[4] * v) + f[7], (f[2] * u + f[5] * v) + f[8]
+        n2 = a2 * a2 + b2 * b2
+        n1 = a1 * a1 + b1 * b1
+        d2 = (u * a2 + v * b2) + c2
+        d1 = (x * a1 + y * b1) + c1
+        e2 = d2 * d2 / n2
+        e1 = d1 * d1 / n1
+        e = np.where(e1 > e2, e1, e2)
+    return (n1 > 1e-30) & (n2 > 1e-30) & (e < thr2)
+
+
+def solve_s2(xa, ya, xb, yb):
+    """Batched 2-point similarity: arrays (m, 2) -> (M (m, 9) = a, -b, tx, b, a, ty, 0, 0, 1,
+    valid)."""
+    dx, dy = xa[:, 1] - xa[:, 0], ya[:, 1] - ya[:, 0]
+    ex, ey = xb[:, 1] - xb[:, 0], yb[:, 1] - yb[:, 0]
+    den = dx * dx + dy * dy
+    valid = den > 1e-12
+    with np.errstate(divide="ignore", invalid="ignore"):
+        a = (dx * ex + dy * ey) / den
+        b = (dx * ey - dy * ex) / den
+    M = np.zeros((len(a), 9))
+    M[:, 0], M[:, 1], M[:, 3], M[:, 4], M[:, 8] = a, -b, b, a, 1.0
+    M[:, 2] = xb[:, 0] - (a * xa[:, 0] - b * ya[:, 0])
+    M[:, 5] = yb[:, 0] - (b * xa[:, 0] + a * ya[:, 0])
+    return M, valid
+
+
+def inliers_s(M, a, b, thr2):
+    x, y, u, v = a[None, :, 0], a[None, :, 1], b[None, :, 0], b[None, :, 1]
+    m = [M[:, k:k + 1] for k in range(6)]
+    with np.errstate(invalid="ignore", over="ignore"):
+        dx = ((m[0] * x + m[1] * y) + m[2]) - u
+        dy = ((m[3] * x + m[4] * y) + m[5]) - v
+    return dx * dx + dy * dy < thr2
+
+
+MODELS = {"homography": (4, solve_h4, inliers), "fundamental": (8, solve_f8, inliers_f),
+          "similarity": (2, solve_s2, inliers_s)}
+
+
+def ransac_pair(p, a, b, n_hyp, thr, seed, model="homography"):
+    """Returns (inlier flags (n,), model (9,), count) for pair index p."""
+    K, solve, test = MODELS[model]
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     n = len(a)
-    if n < 4:
+    if n < K:
         return np.zeros(n, bool), np.zeros(9), 0
     hyps = np.arange(n_hyp)
-    idx, ok = samples(seed, p, n, hyps)
+    idx, ok = samples(seed, p, n, hyps, K)
     safe = np.where(ok[:, None], idx, 0)
-    h, valid = solve_h4(a[safe, 0], a[safe, 1], b[safe, 0], b[safe, 1])
+    h, valid = solve(a[safe, 0], a[safe, 1], b[safe, 0], b[safe, 1])
     valid &= ok
-    cnt = np.where(valid, inliers(h, a, b, thr * thr).sum(1), -1)
+    cnt = np.where(valid, test(h, a, b, thr * thr).sum(1), -1)
     if cnt.max() < 0:
         return np.zeros(n, bool), np.zeros(9), 0
     best = int(np.argmax(cnt))          # first maximum = lowest hypothesis index
-    return inliers(h[best:best + 1], a, b, thr * thr)[0], h[best], int(cnt[best])
+    return test(h[best:best + 1], a, b, thr * thr)[0], h[best], int(cnt[best])

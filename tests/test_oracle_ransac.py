@@ -1,5 +1,6 @@
-"""The RANSAC restatement pinned by planted homographies (CPU only)."""
+"""The RANSAC restatement pinned by planted homographies, two-view geometry and similarities (CPU only)."""
 import numpy as np
+import pytest
 
 from oracle import ransac_oracle as ro
 
@@ -37,3 +38,87 @@ def test_draws_distinct():
     idx, ok = ro.samples(5, 2, 6, np.arange(500))
     assert ok.all()
     assert all(len(set(r)) == 4 for r in idx)
+
+
+def _two_views(rng, n, t, rot_deg=(2.0, -1.5, 4.0), f=1500.0, c=(1300.0, 980.0)):
+    """Points on a bumpy ground 100 m below camera a, camera b displaced by t and rotated:
+    pixel correspondences and the true F (x_b^T F x_a = 0)."""
+    K = np.array([[f, 0.0, c[0]], [0.0, f, c[1]], [0.0, 0.0, 1.0]])
+    ax, ay, az = np.radians(rot_deg)
+    Rx = np.array([[1, 0, 0], [0, np.cos(ax), -np.sin(ax)], [0, np.sin(ax), np.cos(ax)]])
+    Ry = np.array([[np.cos(ay), 0, np.sin(ay)], [0, 1, 0], [-np.sin(ay), 0, np.cos(ay)]])
+    Rz = np.array([[np.cos(az), -np.sin(az), 0], [np.sin(az), np.cos(az), 0], [0, 0, 1]])
+    R = Rz @ Ry @ Rx
+    X = np.c_[rng.uniform(-60, 60, n), rng.uniform(-45, 45, n), 100.0 + rng.uniform(-8, 8, n)]
+    xa = (K @ X.T).T
+    xa = xa[:, :2] / xa[:, 2:]
+    Xb = (R @ (X - np.asarray(t, float)).T).T
+    xb = (K @ Xb.T).T
+    xb = xb[:, :2] / xb[:, 2:]
+    tx = np.array([[0, -t[2], t[1]], [t[2], 0, -t[0]], [-t[1], t[0], 0]], float)
+    # x_b ~ K R (X - t), x_a ~ K X  =>  E = R [t]x with X_b = R (X - t)
+    E = R @ tx
+    Kinv = np.linalg.inv(K)
+    F = Kinv.T @ E @ Kinv
+    return xa, xb, F / np.linalg.norm(F)
+
+
+def _epi(F, a, b):
+    F = F.reshape(3, 3)
+    ha = np.c_[a, np.ones(len(a))]
+    hb = np.c_[b, np.ones(len(b))]
+    l2 = ha @ F.T
+    return np.abs(np.sum(hb * l2, 1)) / np.hypot(l2[:, 0], l2[:, 1])
+
+
+@pytest.mark.parametrize("t", [(3.0, 1.0, 0.5), (20.0, 2.0, 0.0)])
+def test_fundamental_8point_exact(t):
+    """Exact correspondences: the 8-point solve reproduces the true F up to scale, also for the
+    nadir sideways translation whose normalized F has a near-zero (3, 3) entry."""
+    rng = np.random.default_rng(5)
+    a, b, Ft = _two_views(rng, 40, t)
+    F, ok = ro.solve_f8(a[None, :8, 0], a[None, :8, 1], b[None, :8, 0], b[None, :8, 1])
+    assert ok[0]
+    Fn = F[0] / np.linalg.norm(F[0])
+    Fn = Fn * np.sign(Fn @ Ft.ravel())
+    assert np.abs(Fn - Ft.ravel()).max() < 1e-6
+    assert _epi(F[0], a, b).max() < 1e-6                          # every other point too
+    # a transposed or sign-flipped model does not fit
+    assert _epi(F[0].reshape(3, 3).T.ravel(), a, b).max() > 1e-2
+
+
+def test_fundamental_ransac_with_outliers():
+    rng = np.random.default_rng(6)
+    a, b, Ft = _two_views(rng, 300, (12.0, 3.0, 1.0))
+    b = b + rng.normal(0.0, 0.2, b.shape)
+    out = rng.random(300) < 0.3
+    b[out] = rng.uniform(0, 2600, (int(out.sum()), 2))
+    flags, F, cnt = ro.ransac_pair(0, a, b, 1024, 1.0, 77, model="fundamental")
+    true_in = ~out
+    assert cnt == flags.sum()
+    assert (flags & true_in).sum() >= 0.95 * true_in.sum()
+    assert (flags & out).sum() <= 0.05 * out.sum()
+    assert np.median(_epi(F, a[true_in], b[true_in])) < 0.5
+
+
+def test_similarity_ransac_planted():
+    rng = np.random.default_rng(7)
+    n = 200
+    a = rng.uniform(0, 2000, (n, 2))
+    ang, sc, tx, ty = 0.3, 1.07, 40.0, -25.0
+    A = sc * np.array([[np.cos(ang), -np.sin(ang)], [np.sin(ang), np.cos(ang)]])
+    b = a @ A.T + [tx, ty] + rng.normal(0.0, 0.5, (n, 2))
+    out = rng.random(n) < 0.25
+    b[out] = rng.uniform(0, 2000, (int(out.sum()), 2))
+    flags, M, cnt = ro.ransac_pair(3, a, b, 256, 3.0, 9, model="similarity")
+    assert (flags == ~out).mean() > 0.97
+    assert abs(np.arctan2(M[3], M[0]) - ang) < 0.01 and abs(np.hypot(M[0], M[3]) - sc) < 0.01
+    assert M[1] == -M[3] and M[4] == M[0]
+
+
+def test_fundamental_and_similarity_degenerate():
+    a = np.tile([[100.0, 200.0]], (20, 1))
+    b = a + 5.0
+    for model in ("fundamental", "similarity"):
+        flags, M, cnt = ro.ransac_pair(0, a, b, 64, 1.0, 1, model=model)
+        assert cnt == 0 and not flags.any() and not M.any()
