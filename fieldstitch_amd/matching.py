@@ -467,3 +467,128 @@ def mfma_layout_probe():
     out = torch.zeros(64 * 16, dtype=torch.float32, device="cuda")
     _lib.call("fsx_mfma_layout_probe", C.c_void_p(out.data_ptr()), C.c_void_p(_lib.stream_handle()))
     return out.view(64, 16).cpu().numpy()
+
+
+# ---- match_pair (matching.py:175-253): coarse -> homography -> guided -> verification -------
+
+RANSAC_H_ITERS = 2000    # cv2.findHomography's default maxIters
+RANSAC_F_ITERS = 4096    # fixed count (cv2: up to 10000 with early exit at 0.999 confidence)
+RANSAC_S_ITERS = 2000    # cv2.estimateAffinePartial2D's default maxIters
+
+
+def _pair_seed(i, j, salt):
+    return (0x5EED0000 ^ (int(i) * 0x9E3779B1) ^ (int(j) << 24) ^ salt) & 0xFFFFFFFFFFFFFFFF
+
+
+def _find_homography(pa, pb, thr, seed):
+    """matching.py:226 cv2.findHomography(..., RANSAC, thr) -> (H 3x3 or None, inlier mask)."""
+    from . import verify
+    flags, M, cnt = verify.ransac("homography", [pa], [pb], n_hyp=RANSAC_H_ITERS, thr=thr, seed=seed)
+    if cnt[0] == 0:
+        return None, None
+    return M[0].reshape(3, 3), flags[0]
+
+
+def _verify(c, i, j, ui, uj, q, t):
+    """matching.py:175-191: fundamental-matrix RANSAC at c["ransac_thr"] on the GPU
+    (deterministic normalized 8-point, cv2's FM_RANSAC error, fsx_ransac_fundamental) in the
+    place of cv2's USAC_MAGSAC -> (q, t of the inliers, count) or (None, None, 0)."""
+    from . import verify
+    if len(q) < 8:
+        return None, None, 0
+    flags, _, cnt = verify.ransac("fundamental", [ui[q]], [uj[t]], n_hyp=int(c.get("ransac_iters", RANSAC_F_ITERS)),
+                                  thr=float(c["ransac_thr"]), seed=_pair_seed(i, j, 0xF))
+    if cnt[0] == 0:
+        return None, None, 0
+    m = flags[0]
+    return q[m], t[m], int(m.sum())
+
+
+def _similarity(c, i, ui, uj, q, t):
+    """matching.py:194-202: (angle rad, scale, tx, ty) of the similarity from image i to image
+    j, or None.  Similarity RANSAC on the GPU (fsx_ransac_similarity, threshold 0.02 max(W, H))
+    in the place of cv2.estimateAffinePartial2D, then the least-squares fit on its inliers (the
+    optimum cv2's refinement converges to for this linear model)."""
+    import math
+    from . import verify
+    try:
+        if len(q) < 2:
+            return None
+        flags, _, cnt = verify.ransac("similarity", [ui[q]], [uj[t]], n_hyp=RANSAC_S_ITERS,
+                                      thr=0.02 * max(c["W"][i], c["H"][i]), seed=_pair_seed(i, len(q), 0x5))
+        if cnt[0] < 2:
+            return None
+        a = np.asarray(ui[q][flags[0]], np.float64)
+        b = np.asarray(uj[t][flags[0]], np.float64)
+        A = np.zeros((2 * len(a), 4))
+        A[0::2, 0], A[0::2, 1], A[0::2, 2] = a[:, 0], -a[:, 1], 1.0
+        A[1::2, 0], A[1::2, 1], A[1::2, 3] = a[:, 1], a[:, 0], 1.0
+        (sa, sb, tx, ty), *_ = np.linalg.lstsq(A, b.reshape(-1), rcond=None)
+        return (math.atan2(sb, sa), math.hypot(sa, sb), float(tx), float(ty))
+    except Exception:  # noqa: BLE001 - the reference returns None on any failure here
+        return None
+
+
+def _ratio_match(da, db, ratio, checks=48):
+    """matching.py:73-83 (FLANN kd-tree, `checks` leaves): the exhaustive GPU ratio matcher
+    instead, whose match set is the exact one FLANN approximates."""
+    return _ratio_match_bf(da, db, ratio)
+
+
+def match_pair(task):
+    """matching.py:205-253 for the pair task = (i, j) of the installed context: coarse ratio
+    matching of the n_coarse strongest features (+ the ground-plane prior), homography RANSAC,
+    guided matching under that homography, fundamental-matrix verification; exhaustive ratio
+    matching as the fallback; similarity of the verified set.  Returns (i, j, q int32, t int32,
+    sim, n_inliers), (i, j, None, None, None, n) on failure, or ("error", msg) in the sim slot
+    on an exception.  The matching runs on the GPU kernels; cv2's estimators are replaced by the
+    deterministic GPU RANSAC of verify.py."""
+    i, j = task
+    c = _CTX
+    try:
+        A = _get(i)
+        B = _get(j)
+        ui, uj = A["und"], B["und"]
+        if len(ui) < 30 or len(uj) < 30:
+            return i, j, None, None, None, 0
+        use_prior = c["has_rot"][i] and c["has_rot"][j] and c.get("use_prior", True)
+        tol = c["prior_tol"] * max(c["W"][j], c["H"][j])
+        result = None
+        # stage 1: strongest features -> coarse homography -> guided matching
+        n1 = c.get("n_coarse", 1500)
+        if c.get("guided", True) and len(ui) > n1 // 2:
+            q1, t1 = _ratio_match_bf(A["desc"][:n1], B["desc"][:n1], 0.85, A["norm"][:n1], B["norm"][:n1])
+            if use_prior and len(q1):
+                pred, ok = _predict(ui[q1], i, j)
+                m = ok & (np.linalg.norm(pred - uj[t1], axis=1) < tol)
+                q1, t1 = q1[m], t1[m]
+            if len(q1) >= 12:
+                Hm, hm = _find_homography(ui[q1], uj[t1], 0.012 * max(c["W"][i], c["H"][i]), _pair_seed(i, j, 0x1))
+                if Hm is not None and hm is not None and hm.sum() >= 10:
+                    radius = c["guided_radius"] * max(c["W"][j], c["H"][j])
+                    q, t = _guided(A, B, Hm, radius, c.get("guided_k", 24), 0.85)
+                    if len(q) >= c["min_inliers"]:
+                        q, t, n_in = _verify(c, i, j, ui, uj, q, t)
+                        if q is not None and n_in >= max(c["min_inliers"], 40):
+                            result = (q, t, n_in)
+        # fallback: exhaustive ratio matching
+        if result is None:
+            q, t = _ratio_match(A["desc"], B["desc"], c.get("ratio", 0.8), c.get("checks", 48))
+            if use_prior and len(q):
+                pred, ok = _predict(ui[q], i, j)
+                m = ok & (np.linalg.norm(pred - uj[t], axis=1) < tol)
+                q, t = q[m], t[m]
+            if len(q) < 20:
+                return i, j, None, None, None, 0
+            q, t, n_in = _verify(c, i, j, ui, uj, q, t)
+            if q is None or n_in < c["min_inliers"]:
+                return i, j, None, None, None, n_in
+            result = (q, t, n_in)
+        q, t, n_in = result
+        sim = _similarity(c, i, ui, uj, q, t)
+        return i, j, np.asarray(q).astype(np.int32), np.asarray(t).astype(np.int32), sim, n_in
+    except Exception as e:  # noqa: BLE001 - matching.py:249-253 reports every failure in the tuple
+        import traceback
+        tb = traceback.extract_tb(e.__traceback__)[-1]
+        return i, j, None, None, ("error", "%s: %s (%s line %d)" % (type(e).__name__, e, os.path.basename(tb.filename),
+                                                                     tb.lineno)), 0
