@@ -251,3 +251,135 @@ class FeatureBatch:
         _lib.call("fsx_sift_selection_offset", H, W, self.cand_cap, self.kp_cap, C.byref(off))
         o = off.value
         return self.ws[i][o:o + 8 * m].cpu().numpy().view(np.int64)
+
+
+# ---- GPU feature worker (features.py:270-381) ---------------------------------------------------
+
+def read_gray(path):
+    """Decode one photo for the worker: ``.npy`` files holding a 2-D uint8 gray image (already
+    reduced) are read directly.  JPEG decoding, reduction and CLAHE (features.py:60-71) are
+    outside this library; other paths raise, and the worker reports that photo as failed."""
+    p = str(path)
+    if not p.endswith(".npy"):
+        raise ValueError("no decoder for %s (the worker reads .npy gray images)" % p.rsplit("/", 1)[-1])
+    g = np.load(p, allow_pickle=False)
+    if g.dtype != np.uint8 or g.ndim != 2:
+        raise ValueError("%s: expected a 2-D uint8 array, got %s %s" % (p.rsplit("/", 1)[-1], g.dtype, g.shape))
+    return np.ascontiguousarray(g)
+
+
+def _gpu_ready():
+    """Device name, memory and a short self-check of the library (features.py:179-210's role)."""
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("no ROCm GPU visible")
+    _lib.load()
+    prop = torch.cuda.get_device_properties(0)
+    g = (np.add.outer(np.arange(64), np.arange(80)) % 7 * 30).astype(np.uint8)
+    gsift.detect(g, max_kp=100)
+    return "%s (%.1f GB)" % (prop.name, prop.total_memory / 2 ** 30)
+
+
+def gpu_worker_main(task_q, res_q, n_threads, wid, batch=16, decode=read_gray):
+    """features.py:270-381 with the one-photo GPU loop replaced by batches.  ``n_threads``
+    decoder threads take tasks (idx, path, out_npz, thumb_path, reduce, nfeat) from task_q
+    (None ends a thread) and decode them; the GPU loop gathers what is ready (at least one,
+    at most ``batch`` photos), runs them through one FeatureBatch per nfeat (the reference's
+    sift_gpu settings: max_kp = 2.2 nfeat, contrast 0.015, edge 12), writes each feature file
+    with _save and posts ("ok", (idx, n, err, timings)).  Also posted: ("gpu", wid, info) when
+    ready, ("gpu_off", wid, msg) if the GPU cannot be used (there is no CPU path here: every
+    photo is then reported failed), and ("exit", "gpu%d" % wid) at the end.  thumb_path and
+    reduce belong to the decoder the reference uses and are not interpreted."""
+    import queue
+    import threading
+    try:
+        t0 = time.perf_counter()
+        info = _gpu_ready()
+        res_q.put(("gpu", wid, "%s, ready in %.1f s" % (info, time.perf_counter() - t0)))
+        gpu_err = None
+    except Exception as e:  # noqa: BLE001 - reported in the protocol, as features.py:299
+        gpu_err = "%s: %s" % (type(e).__name__, str(e)[:300])
+        res_q.put(("gpu_off", wid, gpu_err))
+
+    ready = queue.Queue(maxsize=max(4, n_threads, batch))
+
+    def decoder():
+        while True:
+            task = task_q.get()
+            if task is None:
+                ready.put(None)
+                break
+            tt = {"read": 0.0, "decode": 0.0, "bytes": 0, "photos_gpu": 1}
+            try:
+                t1 = time.perf_counter()
+                g2 = decode(task[1])
+                tt["decode"] = time.perf_counter() - t1
+                tt["bytes"] = int(g2.nbytes)
+                ready.put((task, g2, tt, None))
+            except Exception as e:  # noqa: BLE001 - per-photo error, as features.py:335-336
+                ready.put((task, None, tt, "%s: %s" % (type(e).__name__, e)))
+
+    ths = [threading.Thread(target=decoder, daemon=True) for _ in range(n_threads)]
+    for t in ths:
+        t.start()
+    cache = {}
+
+    def run_group(group):
+        import torch
+        nfeat = int(group[0][0][5])
+        shapes = tuple(g2.shape for _, g2, _, _ in group)
+        key = (shapes, nfeat)
+        fb = cache.pop(key, None)
+        if fb is None:
+            fb = FeatureBatch(list(shapes), nfeat=nfeat, contrast=0.015, edge=12.0)
+        cache[key] = fb
+        while len(cache) > 4:
+            cache.pop(next(iter(cache)))
+        t2 = time.perf_counter()
+        fb.run([torch.from_numpy(g2).cuda() for _, g2, _, _ in group])
+        fb.check_capacity()
+        for k, (task, g2, tt, _) in enumerate(group):
+            xy, u8, resp, _ = fb.results(k)
+            n = _save(task[2], xy, None, resp, g2.shape[1], g2.shape[0], desc_u8=u8)
+            tt["gpu_detect"] = (time.perf_counter() - t2) / len(group)
+            tt["gpu_describe"] = 0.0
+            res_q.put(("ok", (task[0], n, None, tt)))
+
+    def process(items):
+        good = []
+        for task, g2, tt, err in items:
+            if err or gpu_err:
+                res_q.put(("ok", (task[0], 0, err or ("GPU unavailable: " + gpu_err), tt)))
+            else:
+                good.append((task, g2, tt, None))
+        by_nfeat = {}
+        for it in good:
+            by_nfeat.setdefault(int(it[0][5]), []).append(it)
+        for group in by_nfeat.values():
+            try:
+                run_group(group)
+            except Exception as e:  # noqa: BLE001 - the batch's photos are reported failed
+                msg = "%s: %s" % (type(e).__name__, e)
+                for task, _, tt, _ in group:
+                    res_q.put(("ok", (task[0], 0, msg, tt)))
+
+    finished = 0
+    while finished < n_threads:
+        items = []
+        item = ready.get()
+        while True:
+            if item is None:
+                finished += 1
+            else:
+                items.append(item)
+            if len(items) >= batch or finished >= n_threads:
+                break
+            try:
+                item = ready.get_nowait()
+            except queue.Empty:
+                break
+        if items:
+            process(items)
+    for t in ths:
+        t.join()
+    res_q.put(("exit", "gpu%d" % wid))

@@ -1,0 +1,76 @@
+"""The batched GPU feature worker (features.py:270-381 protocol): tasks in, ("gpu" | "gpu_off",
+wid, msg), ("ok", (idx, n, err, timings)) per photo and ("exit", "gpuN") out."""
+import queue
+
+import numpy as np
+import pytest
+
+
+def _tasks(tmp_path, shapes, nfeat=1500):
+    from fieldstitch_amd import synth
+    rng = np.random.default_rng(8)
+    tasks = []
+    for k, (h, w) in enumerate(shapes):
+        mos = synth.Mosaic(seed=100 + k)
+        mos.polys = synth.random_polygons(rng, 10, 0, 0, w, h)
+        g = mos.render(np.eye(3)[:2], h, w, rng)
+        p = tmp_path / ("img%d.npy" % k)
+        np.save(p, g)
+        tasks.append((k, str(p), str(tmp_path / ("%d.npz" % k)), None, 1, nfeat))
+    return tasks
+
+
+def _run(tasks, n_threads=2, batch=4):
+    from fieldstitch_amd import features
+    tq, rq = queue.Queue(), queue.Queue()
+    for t in tasks:
+        tq.put(t)
+    for _ in range(n_threads):
+        tq.put(None)
+    features.gpu_worker_main(tq, rq, n_threads, 0, batch=batch)
+    out = []
+    while not rq.empty():
+        out.append(rq.get())
+    return out
+
+
+def test_worker_without_gpu_reports_every_photo(tmp_path, monkeypatch):
+    from fieldstitch_amd import features
+
+    def no_gpu():
+        raise RuntimeError("no ROCm GPU visible")
+    monkeypatch.setattr(features, "_gpu_ready", no_gpu)
+    tasks = _tasks(tmp_path, [(60, 80), (64, 96)])
+    tasks.append((7, str(tmp_path / "missing.jpg"), str(tmp_path / "7.npz"), None, 1, 1500))
+    msgs = _run(tasks)
+    assert msgs[0][0] == "gpu_off" and "no ROCm GPU" in msgs[0][2]
+    assert msgs[-1] == ("exit", "gpu0")
+    ok = {m[1][0]: m[1] for m in msgs if m[0] == "ok"}
+    assert sorted(ok) == [0, 1, 7]
+    assert all(r[1] == 0 and r[2] for r in ok.values())
+    assert "no decoder" in ok[7][2] and "GPU unavailable" in ok[0][2]
+
+
+@pytest.mark.gpu
+def test_worker_matches_sift_gpu(tmp_path, gpu):
+    from fieldstitch_amd import features
+    shapes = [(240, 320), (240, 320), (300, 410), (240, 320), (180, 200)]
+    tasks = _tasks(tmp_path, shapes)
+    tasks[3] = tasks[3][:5] + (800,)                        # a second nfeat group
+    tasks.append((9, str(tmp_path / "missing.npy"), str(tmp_path / "9.npz"), None, 1, 1500))
+    msgs = _run(tasks, n_threads=2, batch=3)
+    assert msgs[0][0] == "gpu" and msgs[-1] == ("exit", "gpu0")
+    ok = {m[1][0]: m[1] for m in msgs if m[0] == "ok"}
+    assert sorted(ok) == [0, 1, 2, 3, 4, 9]
+    assert ok[9][1] == 0 and "FileNotFoundError" in ok[9][2]
+    for idx, path, out, _, _, nfeat in tasks[:5]:
+        _, n, err, tt = ok[idx]
+        assert err is None and n > 10 and tt["photos_gpu"] == 1
+        g = np.load(path)
+        xy, desc, resp, _ = features.sift_gpu(g, nfeat)
+        ref = str(tmp_path / ("ref%d.npz" % idx))
+        assert features._save(ref, xy, desc, resp, g.shape[1], g.shape[0]) == n
+        a, b = np.load(out), np.load(ref)
+        for key in ("xy", "resp", "red_wh"):
+            np.testing.assert_array_equal(a[key], b[key])
+        assert np.abs(a["desc"].astype(int) - b["desc"].astype(int)).max() <= 1
