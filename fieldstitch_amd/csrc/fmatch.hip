@@ -259,10 +259,21 @@ __device__ __forceinline__ void topk_push(float d, int t, float* td, int* ti) {
 template <int PREC, int K, int FS_TILE>
 __global__ __launch_bounds__(256) void k_f_screen(const FBatch B) {
     constexpr int ROWB = PREC == 1 ? FD * 2 : FD;                 // bytes per staged row
-    // the tile rows, reused for the final half-wave merge (256 * K (float, int) pairs)
-    constexpr int SBYTES = FS_TILE * ROWB > 256 * K * 8 ? FS_TILE * ROWB : 256 * K * 8;
+    // LDS row pitch: 16 bytes more than a row, so the 16-byte operand reads of 16 consecutive
+    // lanes (consecutive rows) fall on 16 disjoint 4-bank groups (a 256- or 128-byte pitch put
+    // them all on the same banks)
+    constexpr int PITCH = ROWB + 16;
+    constexpr int TBYTES = FS_TILE * PITCH;                       // one staged tile
+    constexpr int NV = FS_TILE * ROWB / 16 / 256;                 // 16-byte pieces per thread per tile
+    static_assert(NV * 16 * 256 == FS_TILE * ROWB && FS_TILE <= 256, "tile staging layout");
+    // BF16: two tile buffers, the next tile fetched into registers while the current one is
+    // used, then stored into the other buffer; FP8 (half the bytes per row, and the extra
+    // registers would cost a wave per SIMD): one buffer.  Reused for the final merge.
+    constexpr bool DB = PREC == 1;
+    constexpr int NBUF = DB ? 2 : 1;
+    constexpr int SBYTES = NBUF * TBYTES > 256 * K * 8 ? NBUF * TBYTES : 256 * K * 8;
     __shared__ __attribute__((aligned(16))) uint8_t s_rows[SBYTES];
-    __shared__ float s_cn2[FS_TILE], s_cs[FS_TILE];
+    __shared__ float s_cn2[NBUF][FS_TILE], s_cs[NBUF][FS_TILE];
     float* s_md = (float*)s_rows;
     int* s_mi = (int*)(s_rows + 256 * K * 4);
     const FPair P = B.pair[blockIdx.y];
@@ -299,38 +310,71 @@ __global__ __launch_bounds__(256) void k_f_screen(const FBatch B) {
         td[j] = INFINITY;
         ti[j] = 0x7fffffff;
     }
-    for (int base = 0; base < sb.n; base += FS_TILE) {
+    uint4 pv[NV];
+    float pcn = 0.0f, pcs = 0.0f;
+    auto fetch = [&](int base) {   // 16-byte loads of tile `base` into registers, all threads
         const int nt = min(FS_TILE, sb.n - base);
-        // stage the tile: 16-byte loads, all threads
-        for (int u = tid; u < FS_TILE * ROWB / 16; u += 256) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            const int u = tid + 256 * k;
             const int r = u / (ROWB / 16), c = u - r * (ROWB / 16);
-            uint4 v = make_uint4(0, 0, 0, 0);
+            pv[k] = make_uint4(0, 0, 0, 0);
             if (r < nt) {
                 const uint8_t* row = PREC == 1 ? (const uint8_t*)(B.qbf + (sb.off + base + r) * FD)
                                                : B.q8 + (sb.off + base + r) * FD;
-                v = ((const uint4*)row)[c];
+                pv[k] = ((const uint4*)row)[c];
             }
-            ((uint4*)s_rows)[u] = v;
         }
-        for (int u = tid; u < FS_TILE; u += 256) {
-            const bool ok = u < nt;
-            s_cn2[u] = ok ? (float)B.n2[sb.off + base + u] : 0.0f;
-            s_cs[u] = ok ? B.scale[sb.off + base + u] : 0.0f;
+        if (tid < FS_TILE) {
+            const bool ok = tid < nt;
+            pcn = ok ? (float)B.n2[sb.off + base + tid] : 0.0f;
+            pcs = ok ? B.scale[sb.off + base + tid] : 0.0f;
         }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            const int u = tid + 256 * k;
+            const int r = u / (ROWB / 16), c = u - r * (ROWB / 16);
+            *(uint4*)(s_rows + buf * TBYTES + r * PITCH + 16 * c) = pv[k];
+        }
+        if (tid < FS_TILE) {
+            s_cn2[buf][tid] = pcn;
+            s_cs[buf][tid] = pcs;
+        }
+    };
+    if (DB) {
+        fetch(0);
+        store(0);
         __syncthreads();
+    }
+    int buf = 0;
+    for (int base = 0; base < sb.n; base += FS_TILE) {
+        const int nt = min(FS_TILE, sb.n - base);
+        const bool more = base + FS_TILE < sb.n;   // block-uniform
+        if (!DB) {
+            fetch(base);
+            store(0);
+            __syncthreads();
+        } else if (more) {
+            fetch(base + FS_TILE);                 // in flight during this tile's MFMA work
+        }
+        const uint8_t* tile = s_rows + buf * TBYTES;
+        const float* t_cn2 = s_cn2[buf];
+        const float* t_cs = s_cs[buf];
 #pragma unroll
         for (int h = 0; h < FS_TILE / 32; h++) {
             const int rl = h * 32 + (lane & 31);
             f32x16_t acc = {};
             if (PREC == 1) {
-                const uint4* rp = (const uint4*)(s_rows + rl * ROWB);
+                const uint4* rp = (const uint4*)(tile + rl * PITCH);
 #pragma unroll
                 for (int s = 0; s < 8; s++) {
                     const bf16x8_t a = __builtin_bit_cast(bf16x8_t, rp[2 * s + (lane >> 5)]);
                     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, qb[s], acc, 0, 0, 0);
                 }
             } else {
-                const uint4* rp = (const uint4*)(s_rows + rl * ROWB);
+                const uint4* rp = (const uint4*)(tile + rl * PITCH);
 #pragma unroll
                 for (int s = 0; s < 2; s++) {
                     const uint4 v0 = rp[4 * s + 2 * (lane >> 5)], v1 = rp[4 * s + 2 * (lane >> 5) + 1];
@@ -348,13 +392,15 @@ __global__ __launch_bounds__(256) void k_f_screen(const FBatch B) {
                     for (int j = 0; j < 4; j++) {
                         const int r = h * 32 + i * 8 + (lane >> 5) * 4 + j;
                         if (r >= nt) continue;
-                        const float dot = PREC == 1 ? acc[i * 4 + j] : acc[i * 4 + j] * (qs * s_cs[r]);
-                        const float d = (qn2 + s_cn2[r]) - 2.0f * dot;
+                        const float dot = PREC == 1 ? acc[i * 4 + j] : acc[i * 4 + j] * (qs * t_cs[r]);
+                        const float d = (qn2 + t_cn2[r]) - 2.0f * dot;
                         topk_push<K>(d, base + r, td, ti);
                     }
             }
         }
+        if (DB && more) store(buf ^ 1);   // the other buffer was last read before the previous barrier
         __syncthreads();
+        if (DB) buf ^= 1;
     }
     // merge the two half-waves of each query
 #pragma unroll
@@ -443,8 +489,12 @@ __global__ __launch_bounds__(256) void k_f_rescore(const FBatch B) {
     if (live && sub < kk) {
         iv = B.topi[g * FK_MAX + sub];
         dt = B.topd[g * FK_MAX + sub];
-        dv = dist64(B.desc + ka * FD, B.desc + (sb.off + iv) * FD);
-        err = (float)fabs((double)dt - dv);
+        if (iv >= 0 && iv < sb.n) {   // a screened slot left empty (no finite distance) stays out
+            dv = dist64(B.desc + ka * FD, B.desc + (sb.off + iv) * FD);
+            err = (float)fabs((double)dt - dv);
+        } else {
+            iv = 0x7fffffff;
+        }
     }
     // screening-error statistics: block partials (same-address atomics would serialise)
     {

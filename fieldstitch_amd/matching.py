@@ -381,6 +381,8 @@ class FloatMatcher:
         self.nk = [int(d.shape[0]) for d in ds]
         self.desc = torch.cat(ds).contiguous() if sum(self.nk) else torch.zeros((1, 128), dtype=torch.float32,
                                                                                  device=device)
+        if sum(self.nk) and not bool(torch.isfinite(self.desc).all()):
+            raise ValueError("descriptors must be finite")
         n = len(ds)
         npair = max(self.n_pair, 1)
         self._nk = (C.c_int * n)(*self.nk)

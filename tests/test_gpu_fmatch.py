@@ -83,3 +83,13 @@ def test_ratio_match_bf_dispatch(gpu):
     q, t = _ratio_match_bf(ua, ub, 0.8)
     rq, rt = _oracle(ua, ub, 0.8)
     assert np.array_equal(q, rq) and np.array_equal(t, rt)
+
+
+def test_non_finite_descriptors_rejected(gpu):
+    from fieldstitch_amd.matching import FloatMatcher
+    rng = np.random.default_rng(1)
+    a = rng.random((50, 128)).astype(np.float32)
+    b = rng.random((60, 128)).astype(np.float32)
+    b[7, 3] = np.nan
+    with pytest.raises(ValueError):
+        FloatMatcher([a, b], [(0, 1)])
