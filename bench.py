@@ -259,10 +259,11 @@ def bench_flight400(args):
         if key not in batches:
             fb = features.FeatureBatch(list(key), nfeat=NFEAT, contrast=CONTRAST, edge=EDGE)
             bufs = [torch.empty(sh, dtype=torch.uint8, device="cuda") for sh in key]
-            batches[key] = (fb, bufs)
+            sels = [fb.selection_device(s_)[:NFEAT] for s_ in range(len(key))]   # views, fixed addresses
+            batches[key] = (fb, bufs, sels)
         return batches[key]
     # autotune the launch shapes on the first chunk
-    fb0, bufs0 = batch_for(chunks[0])
+    fb0, bufs0, _ = batch_for(chunks[0])
     for b, i in zip(bufs0, chunks[0]):
         b.copy_(dev[i])
     tuned = autotune.tune_features(fb0, bufs0, reps=2)
@@ -272,20 +273,21 @@ def bench_flight400(args):
 
     def sift_pass():
         for idx in chunks:
-            fb, bufs = batch_for(idx)
+            fb, bufs, sels = batch_for(idx)
             for b, i in zip(bufs, idx):
                 b.copy_(dev[i])
             fb.run(bufs, graph=True)
             it = torch.tensor(idx, device="cuda")
             u8_all.index_copy_(0, it, fb.u8[:, :NFEAT])
             cnt_all.index_copy_(0, it, torch.clamp(fb.n_out, max=NFEAT))
-            for s_, i in enumerate(idx):
-                sel = torch.clamp(fb.selection_device(s_)[:NFEAT], 0, fb.max_kp - 1)
-                xy_all[i, :, 0] = fb.kp[s_, 0][sel].double()
-                xy_all[i, :, 1] = fb.kp[s_, 1][sel].double()
+            # selected (x, y) of every photo of the chunk in one gather (entries past a
+            # photo's count are stale and clamped, as the matcher only reads its count)
+            sel = torch.stack(sels).clamp_(0, fb.max_kp - 1)
+            g = torch.gather(fb.kp[:, :2, :], 2, sel[:, None, :].expand(-1, 2, -1))
+            xy_all.index_copy_(0, it, g.transpose(1, 2).double())
 
     sift_pass()                                   # warm-up (captures the graphs)
-    for fb, _ in batches.values():
+    for fb, _, _ in batches.values():
         fb.check_capacity()
     torch.cuda.synchronize()
     steps = max(1, args.steps // 5)
