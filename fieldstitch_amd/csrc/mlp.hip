@@ -37,14 +37,18 @@ template <bool TA, bool TB, int EPI>
 __global__ __launch_bounds__(256) void k_gemm(const unsigned short* __restrict__ A, int lda,
                                               const unsigned short* __restrict__ Bm, int ldb, int M, int N, int K,
                                               float* __restrict__ C32, unsigned short* __restrict__ C16, int ldc,
-                                              const float* __restrict__ bias, const unsigned short* __restrict__ mask) {
+                                              const float* __restrict__ bias, const unsigned short* __restrict__ mask,
+                                              int kchunk) {
     __shared__ __attribute__((aligned(16))) unsigned short sA[GT][GKP];   // [m][k]
     __shared__ __attribute__((aligned(16))) unsigned short sB[GT][GKP];   // [n][k]
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
     const int wm = (wv >> 1) * 32, wn = (wv & 1) * 32;
+    // split K: block z sums k in [z * kchunk, (z + 1) * kchunk) into the z-th M x ldc slice of C32
+    const int kb = blockIdx.z * kchunk, ke = min(K, kb + kchunk);
+    if (EPI == EPI_F32) C32 += (size_t)blockIdx.z * M * ldc;
     mf32x16_t acc = {};
-    for (int k0 = 0; k0 < K; k0 += GK) {
+    for (int k0 = kb; k0 < ke; k0 += GK) {
         // stage A tile: 64 (m) x 32 (k)
         for (int e = tid; e < GT * GK; e += 256) {
             int m, k;
@@ -57,7 +61,7 @@ __global__ __launch_bounds__(256) void k_gemm(const unsigned short* __restrict__
             }
             const int gm = m0 + m, gk = k0 + k;
             unsigned short v = 0;
-            if (gm < M && gk < K) v = TA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk];
+            if (gm < M && gk < ke) v = TA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk];
             sA[m][k] = v;
         }
         // stage B tile: 32 (k) x 64 (n), stored [n][k]
@@ -72,7 +76,7 @@ __global__ __launch_bounds__(256) void k_gemm(const unsigned short* __restrict__
             }
             const int gn = n0 + n, gk = k0 + k;
             unsigned short v = 0;
-            if (gn < N && gk < K) v = TB ? Bm[(size_t)gn * ldb + gk] : Bm[(size_t)gk * ldb + gn];
+            if (gn < N && gk < ke) v = TB ? Bm[(size_t)gn * ldb + gk] : Bm[(size_t)gk * ldb + gn];
             sB[n][k] = v;
         }
         __syncthreads();
@@ -98,6 +102,17 @@ __global__ __launch_bounds__(256) void k_gemm(const unsigned short* __restrict__
             else if (EPI == EPI_F32) C32[o] = a;
             else C16[o] = m_f2bf(m_bf2f(mask[o]) > 0.0f ? a : 0.0f);
         }
+}
+
+// C[m][n] = sum over the split-K partials part[z][m][n], z = 0 .. splits-1 in order
+__global__ __launch_bounds__(256) void k_splitk_sum(const float* __restrict__ part, int splits, int M, int N, int ldc,
+                                                    float* __restrict__ C, int ldo) {
+    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (long long)M * N) return;
+    const int m = (int)(e / N), n = (int)(e - (long long)m * N);
+    float a = 0.0f;
+    for (int z = 0; z < splits; z++) a += part[((size_t)z * M + m) * ldc + n];
+    C[(size_t)m * ldo + n] = a;
 }
 
 // Head: z = H2[b] . w3 + b3, p = sigmoid(z); with labels: loss = BCE, dz = (p - y) / nb,
@@ -218,12 +233,46 @@ int fsx_mlp_gemm(int ta, int tb, int epi, const void* A, int lda, const void* Bm
 #define FSX_G(TA_, TB_, E_)                                                                          \
     if (!!ta == TA_ && !!tb == TB_ && epi == E_)                                                     \
         hipLaunchKernelGGL((k_gemm<TA_, TB_, E_>), grid, dim3(256), 0, st, a, lda, b, ldb, M, N, K, C32, c16, ldc, \
-                           bias, mk);
+                           bias, mk, K);
     FSX_G(false, false, 0) FSX_G(false, false, 1) FSX_G(false, false, 2)
     FSX_G(true, false, 0) FSX_G(true, false, 1) FSX_G(true, false, 2)
     FSX_G(false, true, 0) FSX_G(false, true, 1) FSX_G(false, true, 2)
     FSX_G(true, true, 0) FSX_G(true, true, 1) FSX_G(true, true, 2)
 #undef FSX_G
+    FSX_LAUNCH_CHECK();
+    return FSX_OK;
+}
+
+// Scratch bytes of fsx_mlp_gemm_splitk for an M x N product in `splits` K chunks.
+int fsx_mlp_splitk_scratch(int M, int N, int splits, size_t* bytes) {
+    FSX_REQUIRE(M > 0 && N > 0 && splits >= 1 && splits <= 256 && bytes, "bad split-K arguments");
+    *bytes = (size_t)splits * M * N * sizeof(float);
+    return FSX_OK;
+}
+
+// C32 (M x N, leading dimension ldc) = op(A) op(B) with K split into `splits` chunks of whole
+// K steps computed by separate blocks into scratch, then summed chunk by chunk in order
+// (deterministic).  For products with a small M x N and a long K (weight gradients over a
+// batch), which a single pass would give to a handful of blocks.
+int fsx_mlp_gemm_splitk(int ta, int tb, const void* A, int lda, const void* Bm, int ldb, int M, int N, int K, float* C32,
+                        int ldc, float* scratch, int splits, void* stream) {
+    FSX_REQUIRE(A && Bm && C32 && scratch && M > 0 && N > 0 && K > 0 && (K % GK) == 0,
+                "bad split-K GEMM shape M=%d N=%d K=%d", M, N, K);
+    FSX_REQUIRE(splits >= 1 && splits <= 256 && lda > 0 && ldb > 0 && ldc >= N, "bad split-K arguments");
+    const int kchunk = ((K / GK + splits - 1) / splits) * GK;
+    const int nz = (K + kchunk - 1) / kchunk;
+    const dim3 grid((N + GT - 1) / GT, (M + GT - 1) / GT, nz);
+    hipStream_t st = (hipStream_t)stream;
+    const unsigned short* a = (const unsigned short*)A;
+    const unsigned short* b = (const unsigned short*)Bm;
+#define FSX_G(TA_, TB_)                                                                                        \
+    if (!!ta == TA_ && !!tb == TB_)                                                                            \
+        hipLaunchKernelGGL((k_gemm<TA_, TB_, EPI_F32>), grid, dim3(256), 0, st, a, lda, b, ldb, M, N, K, scratch, \
+                           nullptr, N, nullptr, nullptr, kchunk);
+    FSX_G(false, false) FSX_G(true, false) FSX_G(false, true) FSX_G(true, true)
+#undef FSX_G
+    hipLaunchKernelGGL(k_splitk_sum, dim3((unsigned)(((long long)M * N + 255) / 256)), dim3(256), 0, st, scratch, nz,
+                       M, N, N, C32, ldc);
     FSX_LAUNCH_CHECK();
     return FSX_OK;
 }

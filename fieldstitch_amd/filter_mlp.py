@@ -17,6 +17,7 @@ import numpy as np
 from . import _lib
 
 N_IN, N_INP, N_HID = 264, 288, 256
+WGRAD_SPLITS = 16          # K chunks of the weight-gradient products (K = batch rows)
 
 
 class MatchFilterMLP:
@@ -48,6 +49,8 @@ class MatchFilterMLP:
             bf = torch.bfloat16
             sc = C.c_size_t()
             _lib.call("fsx_mlp_colsum_scratch", nb, N_HID + 2, C.byref(sc))
+            sk = C.c_size_t()
+            _lib.call("fsx_mlp_splitk_scratch", N_INP, N_HID, WGRAD_SPLITS, C.byref(sk))
             b = {"H1": torch.empty((nb, N_HID), dtype=bf, device=self.dev),
                  "H2": torch.empty((nb, N_HID), dtype=bf, device=self.dev),
                  "dZ2": torch.empty((nb, N_HID), dtype=bf, device=self.dev),
@@ -56,6 +59,7 @@ class MatchFilterMLP:
                  "part": torch.empty(((nb + 3) // 4, N_HID + 2), dtype=torch.float32, device=self.dev),
                  "hp": torch.empty(N_HID + 2, dtype=torch.float32, device=self.dev),
                  "scratch": torch.empty(max(sc.value // 4, 1), dtype=torch.float32, device=self.dev),
+                 "splitk": torch.empty(max(sk.value // 4, 1), dtype=torch.float32, device=self.dev),
                  "g": {"W1": torch.empty((N_INP, N_HID), dtype=torch.float32, device=self.dev),
                        "W2": torch.empty((N_HID, N_HID), dtype=torch.float32, device=self.dev),
                        "b1": torch.empty(N_HID, dtype=torch.float32, device=self.dev),
@@ -103,14 +107,14 @@ class MatchFilterMLP:
                   P(b["scratch"]), self._st())
         g = b["g"]
         # dW2 = H1^T dZ2 ; db2 = colsum(dZ2) ; dZ1 = (dZ2 W2^T) masked by H1 > 0
-        _lib.call("fsx_mlp_gemm", 1, 0, 1, P(b["H1"]), N_HID, P(b["dZ2"]), N_HID, N_HID, N_HID, nb, P(g["W2"]),
-                  None, N_HID, None, None, self._st())
+        _lib.call("fsx_mlp_gemm_splitk", 1, 0, P(b["H1"]), N_HID, P(b["dZ2"]), N_HID, N_HID, N_HID, nb, P(g["W2"]),
+                  N_HID, P(b["splitk"]), WGRAD_SPLITS, self._st())
         _lib.call("fsx_mlp_colsum", P(b["dZ2"]), 1, nb, N_HID, N_HID, P(g["b2"]), P(b["scratch"]), self._st())
         _lib.call("fsx_mlp_gemm", 0, 1, 2, P(b["dZ2"]), N_HID, P(self.W2_16), N_HID, nb, N_HID, N_HID, None,
                   P(b["dZ1"]), N_HID, None, P(b["H1"]), self._st())
         # dW1 = X^T dZ1 ; db1 = colsum(dZ1)
-        _lib.call("fsx_mlp_gemm", 1, 0, 1, P(X16), N_INP, P(b["dZ1"]), N_HID, N_INP, N_HID, nb, P(g["W1"]),
-                  None, N_HID, None, None, self._st())
+        _lib.call("fsx_mlp_gemm_splitk", 1, 0, P(X16), N_INP, P(b["dZ1"]), N_HID, N_INP, N_HID, nb, P(g["W1"]),
+                  N_HID, P(b["splitk"]), WGRAD_SPLITS, self._st())
         _lib.call("fsx_mlp_colsum", P(b["dZ1"]), 1, nb, N_HID, N_HID, P(g["b1"]), P(b["scratch"]), self._st())
         hp = b["hp"]
         grads = {"W1": g["W1"], "b1": g["b1"], "W2": g["W2"], "b2": g["b2"], "w3": hp[:N_HID], "b3": hp[N_HID:N_HID + 1]}

@@ -243,6 +243,15 @@ int fsx_fp8_layout_probe(float *out_dev, void *stream);
 int fsx_mlp_gemm(int ta, int tb, int epi, const void *A, int lda, const void *B, int ldb, int M, int N, int K,
                  float *C32, void *C16, int ldc, const float *bias, const void *mask, void *stream);
 
+/* Scratch bytes of fsx_mlp_gemm_splitk for an M x N product in `splits` K chunks. */
+int fsx_mlp_splitk_scratch(int M, int N, int splits, size_t *bytes);
+
+/* C32 (M x N, leading dimension ldc) = op(A) op(B) as fsx_mlp_gemm's epilogue 1, with K split
+ * into `splits` chunks computed by separate blocks into scratch and summed in chunk order
+ * (deterministic): the weight gradients of a batch (small M x N, long K). */
+int fsx_mlp_gemm_splitk(int ta, int tb, const void *A, int lda, const void *B, int ldb, int M, int N, int K,
+                        float *C32, int ldc, float *scratch, int splits, void *stream);
+
 /* Output layer over nb rows of H2 (bf16, 256 wide): p = sigmoid(H2 w3 + b3) (p_out may be
  * NULL); with labels y: dZ2 = ((p - y) / nb) w3 masked by H2 > 0 (bf16) and per-4-row partials
  * part[ceil(nb/4)][258] = (sum dz H2[:, j], sum dz, sum BCE loss). */

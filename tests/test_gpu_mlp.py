@@ -53,6 +53,33 @@ def _sync_params(net):
     return {k: v.cpu().numpy().astype(np.float64) for k, v in net.p.items()}
 
 
+@pytest.mark.parametrize("ta,splits", [(1, 16), (1, 5), (0, 3)])
+def test_gemm_splitk(gpu, ta, splits):
+    """Split-K product = float64 product of the bf16 operands, deterministic (two runs equal),
+    with chunks that do not divide K evenly."""
+    import torch
+    from fieldstitch_amd import _lib
+    rng = np.random.default_rng(40 + splits)
+    M, N, K = 288, 256, 8192 - 96
+    A, a = _bf(torch, rng, (K, M) if ta else (M, K))
+    Bm, b = _bf(torch, rng, (K, N))
+    ref = (a.T if ta else a) @ b
+    P = lambda t: C.c_void_p(t.data_ptr())   # noqa: E731
+    st = C.c_void_p(_lib.stream_handle())
+    sz = C.c_size_t()
+    _lib.call("fsx_mlp_splitk_scratch", M, N, splits, C.byref(sz))
+    scratch = torch.empty(sz.value // 4, dtype=torch.float32, device="cuda")
+    outs = []
+    for _ in range(2):
+        c32 = torch.full((M, N + 3), -7.0, dtype=torch.float32, device="cuda")
+        _lib.call("fsx_mlp_gemm_splitk", ta, 0, P(A), M if ta else K, P(Bm), N, M, N, K, P(c32), N + 3, P(scratch),
+                  splits, st)
+        outs.append(c32.cpu().numpy())
+    np.testing.assert_array_equal(outs[0], outs[1])
+    assert (outs[0][:, N:] == -7.0).all()                       # ldc padding untouched
+    np.testing.assert_allclose(outs[0][:, :N], ref, rtol=1e-4, atol=1e-3)
+
+
 def test_gradients_vs_oracle(gpu):
     import torch
     from fieldstitch_amd.filter_mlp import MatchFilterMLP, N_IN
