@@ -1220,6 +1220,7 @@ __global__ __launch_bounds__(256) void k_describe_sel(const SiftJob* __restrict_
 
 constexpr int SP_THREADS = 1024;
 constexpr int SP_SEGS = 256;
+constexpr int SP_LDS_N = 32768;
 
 __device__ int block_exclusive_scan(int v, int* s_tmp, int& total) {
     // SP_THREADS threads; s_tmp: SP_THREADS / 64 ints
@@ -1246,6 +1247,7 @@ __global__ __launch_bounds__(SP_THREADS) void k_spread(const SiftJob* __restrict
     __shared__ int s_start[109];
     __shared__ int s_tmp[SP_THREADS / 64];
     __shared__ unsigned short s_seg[SP_SEGS][108];
+    __shared__ uint8_t s_cell[SP_LDS_N];   // cell ids for the serial segment passes (N <= SP_LDS_N)
     const SiftJob& J = jobs[blockIdx.z];
     const int N = *J.n_out, n = J.nfeat, stride = J.max_kp;
     const int tid = threadIdx.x;
@@ -1262,11 +1264,13 @@ __global__ __launch_bounds__(SP_THREADS) void k_spread(const SiftJob* __restrict
     int* spos = rank + J.kp_cap;
     int* sbefore = spos + J.kp_cap;
     const float Wf = (float)J.W, Hf = (float)J.H;
+    const bool lds = N <= SP_LDS_N;
     for (int i = tid; i < N; i += SP_THREADS) {
         int cx = (int)((x[i] / Wf) * 12.0f), cy = (int)((y[i] / Hf) * 9.0f);
         cx = cx < 0 ? 0 : (cx > 11 ? 11 : cx);
         cy = cy < 0 ? 0 : (cy > 8 ? 8 : cy);
         cell[i] = cy * 12 + cx;
+        if (lds) s_cell[i] = (uint8_t)(cy * 12 + cx);
     }
     __syncthreads();
     // rank of each keypoint within its cell in detection order: SP_SEGS threads each own a
@@ -1276,7 +1280,10 @@ __global__ __launch_bounds__(SP_THREADS) void k_spread(const SiftJob* __restrict
     if (tid < SP_SEGS) {
         for (int c = 0; c < 108; c++) s_seg[tid][c] = 0;
         const int a = tid * seg, b = min(N, a + seg);
-        for (int i = a; i < b; i++) s_seg[tid][cell[i]]++;
+        if (lds)
+            for (int i = a; i < b; i++) s_seg[tid][s_cell[i]]++;
+        else
+            for (int i = a; i < b; i++) s_seg[tid][cell[i]]++;
     }
     __syncthreads();
     if (tid < 108) {
@@ -1291,7 +1298,10 @@ __global__ __launch_bounds__(SP_THREADS) void k_spread(const SiftJob* __restrict
     __syncthreads();
     if (tid < SP_SEGS) {
         const int a = tid * seg, b = min(N, a + seg);
-        for (int i = a; i < b; i++) rank[i] = s_seg[tid][cell[i]]++;
+        if (lds)
+            for (int i = a; i < b; i++) rank[i] = s_seg[tid][s_cell[i]]++;
+        else
+            for (int i = a; i < b; i++) rank[i] = s_seg[tid][cell[i]]++;
     }
     __syncthreads();
     const int cap = (n + 107) / 108;
