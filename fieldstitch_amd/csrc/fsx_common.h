@@ -87,6 +87,7 @@ struct SiftJob {
     float* G4;           // P0 (scratch for layer 4)
     // keypoint pipeline buffers
     uint32_t* cand;      // candidate atlas indices (gauss-atlas numbering, layers 1..3)
+    float* cand_der;     // [cand_cap][10] DoG derivatives at each candidate (v, dx, dy, ds, dxx, dyy, dss, dxy, dxs, dys)
     float4* refined;     // 2 x float4 per refined keypoint: (o,l,r,c as int bits), (X0,X1,X2,resp)
     uint32_t* refined_src;   // candidate index of each refined keypoint
     float4* oriented;    // 2 x float4 per oriented keypoint: (x,y,size,angle), (resp, o, l bits, src)

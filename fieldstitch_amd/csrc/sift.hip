@@ -370,18 +370,53 @@ __device__ __forceinline__ int wave_append(bool f, int* counter) {
     return base + __popcll(m & ((1ull << lane) - 1ull));
 }
 
+// Finite differences of gsift.py:274-290 at one DoG voxel.  derivs_from takes the 19 samples
+// of the 3x3x3 cube (x / y / s = column / row / layer, p / m = +1 / -1); the extrema kernel
+// feeds it from registers and derivs_at from memory, so both give the same bits.
+struct Derivs {
+    float v, dx, dy, ds, dxx, dyy, dss, dxy, dxs, dys;
+};
+constexpr int DER_N = 10;   // floats of a Derivs record in the candidate side array
+
+__device__ __forceinline__ Derivs derivs_from(float v, float xp, float xm, float yp, float ym, float sp, float sm,
+                                              float ypxp, float ypxm, float ymxp, float ymxm, float spxp,
+                                              float spxm, float smxp, float smxm, float spyp, float spym,
+                                              float smyp, float smym) {
+    Derivs d;
+    d.v = v;
+    d.dx = (xp - xm) * 0.5f;
+    d.dy = (yp - ym) * 0.5f;
+    d.ds = (sp - sm) * 0.5f;
+    d.dxx = (xp + xm) - 2.0f * v;
+    d.dyy = (yp + ym) - 2.0f * v;
+    d.dss = (sp + sm) - 2.0f * v;
+    d.dxy = (((ypxp - ypxm) - ymxp) + ymxm) * 0.25f;
+    d.dxs = (((spxp - spxm) - smxp) + smxm) * 0.25f;
+    d.dys = (((spyp - spym) - smyp) + smym) * 0.25f;
+    return d;
+}
+
+__device__ __forceinline__ Derivs derivs_at(const float* __restrict__ D, long long k, long long C, long long RC) {
+    return derivs_from(D[k], D[k + 1], D[k - 1], D[k + C], D[k - C], D[k + RC], D[k - RC], D[k + C + 1],
+                       D[k + C - 1], D[k - C + 1], D[k - C - 1], D[k + RC + 1], D[k + RC - 1], D[k - RC + 1],
+                       D[k - RC - 1], D[k + RC + C], D[k + RC - C], D[k - RC + C], D[k - RC - C]);
+}
+
 // Streaming-column extrema: each wave owns 62 columns of one octave and walks down EX_H
 // rows; per row it loads the 5 DoG layers (coalesced), takes the horizontal max/min over
 // (x-1, x, x+1) with lane shuffles and keeps a 3-row ring, so the 3x3x3 max/min of a voxel
 // costs a few max/min operations.  v >= max27 is the reference's non-strict test (no NaN).
 // Candidates are staged per wave in LDS and appended EX_BUF at a time with one atomic on the
 // image's counter (an atomic per row with candidates, all on one address, cost a third of the
-// kernel).
-constexpr int EX_BUF = 256;
-static_assert(EX_BUF >= 3 * 64, "one row's candidates always fit an empty buffer");
+// kernel).  The kernel also keeps the raw 3-row ring, so it hands each candidate's first
+// refinement derivatives to k_refine (three of four candidates converge without moving, and
+// then refinement gathers nothing from the DoG atlas).
+constexpr int EX_BUF = 64;
+static_assert(EX_BUF >= EX_W, "one layer's candidates of a row always fit an empty buffer");
 
 __global__ __launch_bounds__(256) void k_extrema(const SiftJob* __restrict__ jobs, int o) {
     __shared__ uint32_t s_buf[4][EX_BUF];
+    __shared__ float s_der[4][EX_BUF * DER_N];
     int tx, ty, tz;
     xcd_tile(tx, ty, tz);
     const SiftJob& J = jobs[tz];
@@ -389,6 +424,7 @@ __global__ __launch_bounds__(256) void k_extrema(const SiftJob* __restrict__ job
     const int R = J.oct[o].R, C = J.oct[o].C;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     uint32_t* buf = s_buf[wv];
+    float* der = s_der[wv];
     int nbuf = 0;   // staged candidates of this wave (wave-uniform)
     // waves of a block may leave early, so the buffer is ordered with wave-level barriers
     auto flush = [&]() {
@@ -403,6 +439,8 @@ __global__ __launch_bounds__(256) void k_extrema(const SiftJob* __restrict__ job
             if (base + i < J.cand_cap) J.cand[base + i] = buf[i];
             else atomicOr(&J.counters[3], 1);
         }
+        const int nder = min(nbuf, J.cand_cap - base) * DER_N;   // <= 0 once the array is full
+        for (int i = lane; i < nder; i += 64) J.cand_der[(size_t)base * DER_N + i] = der[i];
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -417,7 +455,8 @@ __global__ __launch_bounds__(256) void k_extrema(const SiftJob* __restrict__ job
     const __attribute__((address_space(1))) float* D = gptr(J.DF + J.oct[o].dbase + xc);
     const float pthr = J.pthr;
     const bool col_ok = lane >= 1 && lane <= EX_W && x >= BORDER && x < C - BORDER;
-    float hmx[3][5], hmn[3][5], mid[3][3];
+    float hmx[3][5], hmn[3][5];
+    float prev[2][5];   // the last two rows of the previous 4-row block
 #pragma unroll
     for (int a = 0; a < 3; a++)
 #pragma unroll
@@ -426,9 +465,9 @@ __global__ __launch_bounds__(256) void k_extrema(const SiftJob* __restrict__ job
             hmn[a][l] = 0.f;
         }
 #pragma unroll
-    for (int a = 0; a < 3; a++)
+    for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int l = 0; l < 3; l++) mid[a][l] = 0.f;
+        for (int l = 0; l < 5; l++) prev[a][l] = 0.f;
     const int yend = min(y0 + EX_H, R);
     // rows are fetched 4 at a time, one block of 4 ahead of the one being processed
     float nxt[4][5];
@@ -470,12 +509,12 @@ __global__ __launch_bounds__(256) void k_extrema(const SiftJob* __restrict__ job
                 hmn[1][l] = hmn[2][l];
                 hmn[2][l] = fminf(fminf(lf, v), rt);
             }
+            // raw[a][l]: layer l of rows y-1, y, y+1 (= yy-2 .. yy) of this column
+            float raw[3][5];
 #pragma unroll
-            for (int l = 0; l < 3; l++) {
-                mid[0][l] = mid[1][l];
-                mid[1][l] = mid[2][l];
-                mid[2][l] = cur[r][l + 1];
-            }
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int l = 0; l < 5; l++) raw[a][l] = r - 2 + a >= 0 ? cur[(r - 2 + a) & 3][l] : prev[(r + a) & 1][l];
             const int y = yy - 1;                           // the ring's middle row
             if (yy < y0 + 1) continue;                      // wave-uniform
             const bool row_ok = y >= BORDER && y < R - BORDER;
@@ -488,7 +527,7 @@ __global__ __launch_bounds__(256) void k_extrema(const SiftJob* __restrict__ job
             bool f[3];
 #pragma unroll
             for (int l = 1; l <= N_LAYERS; l++) {
-                const float c = mid[1][l - 1];
+                const float c = raw[1][l];
                 const float mx = fmaxf(fmaxf(vm[l - 1], vm[l]), vm[l + 1]);
                 const float mn = fminf(fminf(vn[l - 1], vn[l]), vn[l + 1]);
                 f[l - 1] = col_ok && row_ok && ((c >= mx && c > pthr) || (c <= mn && c < -pthr));
@@ -496,14 +535,40 @@ __global__ __launch_bounds__(256) void k_extrema(const SiftJob* __restrict__ job
             const unsigned long long b0 = __ballot(f[0]), b1 = __ballot(f[1]), b2 = __ballot(f[2]);
             const int n0 = __popcll(b0), n1 = __popcll(b1), n2 = __popcll(b2);
             if (n0 + n1 + n2 == 0) continue;                // wave-uniform
-            if (nbuf + n0 + n1 + n2 > EX_BUF) flush();
             const unsigned long long below = (1ull << lane) - 1ull;
-            const int pos[3] = {nbuf + (int)__popcll(b0 & below), nbuf + n0 + (int)__popcll(b1 & below),
-                                nbuf + n0 + n1 + (int)__popcll(b2 & below)};
 #pragma unroll
-            for (int l = 1; l <= N_LAYERS; l++)
-                if (f[l - 1]) buf[pos[l - 1]] = (uint32_t)(J.oct[o].gbase + (long long)(l - 1) * P + (long long)y * C + x);
-            nbuf += n0 + n1 + n2;
+            for (int l = 1; l <= N_LAYERS; l++) {
+                const unsigned long long bl = l == 1 ? b0 : (l == 2 ? b1 : b2);
+                if (bl == 0) continue;                      // wave-uniform
+                const int nl = __popcll(bl);
+                if (nbuf + nl > EX_BUF) flush();
+                // the 3x3x3 cube around (l, y, x): rows from the ring, columns x -/+ 1 by DPP
+                // (all lanes take part; lanes 0 and 63 never hold a candidate)
+                auto xm = [](float v) {
+                    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x138, 0xf, 0xf, false));
+                };
+                auto xp = [](float v) {
+                    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x130, 0xf, 0xf, false));
+                };
+                const Derivs d = derivs_from(raw[1][l], xp(raw[1][l]), xm(raw[1][l]), raw[2][l], raw[0][l],
+                                             raw[1][l + 1], raw[1][l - 1], xp(raw[2][l]), xm(raw[2][l]),
+                                             xp(raw[0][l]), xm(raw[0][l]), xp(raw[1][l + 1]), xm(raw[1][l + 1]),
+                                             xp(raw[1][l - 1]), xm(raw[1][l - 1]), raw[2][l + 1], raw[0][l + 1],
+                                             raw[2][l - 1], raw[0][l - 1]);
+                if (f[l - 1]) {
+                    const int p = nbuf + (int)__popcll(bl & below);
+                    buf[p] = (uint32_t)(J.oct[o].gbase + (long long)(l - 1) * P + (long long)y * C + x);
+                    float* q = der + p * DER_N;
+                    q[0] = d.v; q[1] = d.dx; q[2] = d.dy; q[3] = d.ds; q[4] = d.dxx;
+                    q[5] = d.dyy; q[6] = d.dss; q[7] = d.dxy; q[8] = d.dxs; q[9] = d.dys;
+                }
+                nbuf += nl;
+            }
+        }
+#pragma unroll
+        for (int l = 0; l < 5; l++) {
+            prev[0][l] = cur[2][l];
+            prev[1][l] = cur[3][l];
         }
     }
     flush();
@@ -512,26 +577,6 @@ __global__ __launch_bounds__(256) void k_extrema(const SiftJob* __restrict__ job
 // ------------------------------------------------------------------------------------------
 // Refinement (gsift.py:274-290 derivatives, 182-198 adjugate solve, 322-369 iteration and
 // the contrast / edge tests).  One thread per candidate.
-
-struct Derivs {
-    float v, dx, dy, ds, dxx, dyy, dss, dxy, dxs, dys;
-};
-
-__device__ __forceinline__ Derivs derivs_at(const float* __restrict__ D, long long k, long long C, long long RC) {
-    Derivs d;
-    d.v = D[k];
-    float xp1 = D[k + 1], xm1 = D[k - 1], yp1 = D[k + C], ym1 = D[k - C], sp1 = D[k + RC], sm1 = D[k - RC];
-    d.dx = (xp1 - xm1) * 0.5f;
-    d.dy = (yp1 - ym1) * 0.5f;
-    d.ds = (sp1 - sm1) * 0.5f;
-    d.dxx = (xp1 + xm1) - 2.0f * d.v;
-    d.dyy = (yp1 + ym1) - 2.0f * d.v;
-    d.dss = (sp1 + sm1) - 2.0f * d.v;
-    d.dxy = (((D[k + C + 1] - D[k + C - 1]) - D[k - C + 1]) + D[k - C - 1]) * 0.25f;
-    d.dxs = (((D[k + RC + 1] - D[k + RC - 1]) - D[k - RC + 1]) + D[k - RC - 1]) * 0.25f;
-    d.dys = (((D[k + RC + C] - D[k + RC - C]) - D[k - RC + C]) + D[k - RC - C]) * 0.25f;
-    return d;
-}
 
 __global__ __launch_bounds__(256) void k_refine(const SiftJob* __restrict__ jobs) {
     const SiftJob& J = jobs[blockIdx.z];
@@ -555,8 +600,14 @@ __global__ __launch_bounds__(256) void k_refine(const SiftJob* __restrict__ jobs
             c = (int)(rem - (long long)r * C);
             const float* D = J.DF + J.oct[o].dbase;
             bool good = false;
+            // derivatives at the candidate itself come from the extrema kernel
+            const float2* q = reinterpret_cast<const float2*>(J.cand_der) + (size_t)i * (DER_N / 2);
+            const float2 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4];
+            Derivs d;
+            d.v = q0.x; d.dx = q0.y; d.dy = q1.x; d.ds = q1.y; d.dxx = q2.x;
+            d.dyy = q2.y; d.dss = q3.x; d.dxy = q3.y; d.dxs = q4.x; d.dys = q4.y;
             for (int it = 0; it < MAX_INTERP; it++) {
-                Derivs d = derivs_at(D, (long long)l * RC + (long long)r * C + c, C, RC);
+                if (it > 0) d = derivs_at(D, (long long)l * RC + (long long)r * C + c, C, RC);
                 const float a = d.dxx, bb = d.dxy, cc = d.dxs, dd = d.dxy, e = d.dyy, ff = d.dys, gg = d.dxs,
                             h = d.dys, ii = d.dss;
                 const float A0 = e * ii - ff * h, B0 = -(dd * ii - ff * gg), C0 = dd * h - e * gg;
@@ -580,7 +631,7 @@ __global__ __launch_bounds__(256) void k_refine(const SiftJob* __restrict__ jobs
                 if (l < 1 || l > N_LAYERS || c < BORDER || c >= C - BORDER || r < BORDER || r >= R - BORDER) break;
             }
             if (good) {
-                Derivs d = derivs_at(D, (long long)l * RC + (long long)r * C + c, C, RC);
+                // d is already at the final voxel: a converged step does not move
                 const float contr = d.v + 0.5f * ((d.dx * X0 + d.dy * X1) + d.ds * X2);
                 const float tr = d.dxx + d.dyy;
                 const float dt = d.dxx * d.dyy - d.dxy * d.dxy;
@@ -1354,7 +1405,7 @@ __global__ __launch_bounds__(SP_THREADS) void k_spread(const SiftJob* __restrict
 static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct WsLayout {
-    size_t job, counters, DF, G0, G123, G4, cand, refined, refined_src, oriented, keys0, keys1, sel, spread, total;
+    size_t job, counters, DF, G0, G123, G4, cand, cand_der, refined, refined_src, oriented, keys0, keys1, sel, spread, total;
 };
 
 static WsLayout ws_layout(const SiftPlan& p, int cand_cap, int kp_cap) {
@@ -1367,6 +1418,7 @@ static WsLayout ws_layout(const SiftPlan& p, int cand_cap, int kp_cap) {
     L.G123 = off; off = align256(off + 3 * (size_t)p.sumP * 4);
     L.G4 = off; off = align256(off + (size_t)p.P0 * 4);
     L.cand = off; off = align256(off + (size_t)cand_cap * 4);
+    L.cand_der = off; off = align256(off + (size_t)cand_cap * DER_N * 4);
     L.refined = off; off = align256(off + (size_t)cand_cap * 32);
     L.refined_src = off; off = align256(off + (size_t)cand_cap * 4);
     L.oriented = off; off = align256(off + (size_t)kp_cap * 32);
@@ -1405,6 +1457,7 @@ static int fill_job(SiftJob* J, const SiftPlan& p, const uint8_t* img, int ld, d
     J->G123 = (float*)(b + L.G123);
     J->G4 = (float*)(b + L.G4);
     J->cand = (uint32_t*)(b + L.cand);
+    J->cand_der = (float*)(b + L.cand_der);
     J->refined = (float4*)(b + L.refined);
     J->refined_src = (uint32_t*)(b + L.refined_src);
     J->oriented = (float4*)(b + L.oriented);
