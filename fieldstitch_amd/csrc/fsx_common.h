@@ -70,7 +70,7 @@ constexpr int HIST_SIZE = 6 * 6 * 10;    // (d+2) x (d+2) x (bins+2) descriptor 
 // Octave geometry inside one image's workspace.  Offsets are in float elements.
 struct OctGeom {
     int R, C;
-    long long dbase;     // DoG atlas: 5 layers (R, C) per octave
+    long long g45base;   // gauss layers 4..5: (2, R, C) per octave
     long long gbase;     // gauss layers 1..3: (3, R, C) per octave
     long long g0base;    // gauss layer 0 (octave base): (R, C) per octave
 };
@@ -81,10 +81,9 @@ struct SiftJob {
     int H, W, ld;
     int n_oct;
     OctGeom oct[MAXO];
-    float* DF;           // 5 * sumP
-    float* G123;         // 3 * sumP
-    float* G0;           // sumP
-    float* G4;           // P0 (scratch for layer 4)
+    float* G0;           // sumP: gauss layer 0 of every octave
+    float* G123;         // 3 * sumP: gauss layers 1..3
+    float* G45;          // 2 * sumP: gauss layers 4..5 (DoG layer l = G[l+1] - G[l], never stored)
     // keypoint pipeline buffers
     uint32_t* cand;      // candidate atlas indices (gauss-atlas numbering, layers 1..3)
     float* cand_der;     // [cand_cap][10] DoG derivatives at each candidate (v, dx, dy, ds, dxx, dyy, dss, dxy, dxs, dys)

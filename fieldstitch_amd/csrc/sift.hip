@@ -114,8 +114,10 @@ int sift_plan(int H, int W, SiftPlan* p) {
 // Vertical pass (axis 0) then horizontal pass (axis 1), each rounded to float32 like
 // scipy.ndimage.correlate1d; float64 accumulation: centre tap first, then symmetric pairs
 // from the outermost inwards (scipy ni_filters.c, symmetric branch).  Mirror border =
-// reflect-101.  The epilogue writes the new gauss layer, the DoG (new - previous, float32)
-// and, for layer 3, the next octave's base (every second pixel).
+// reflect-101.  The epilogue writes the new gauss layer and, for layer 3, the next octave's
+// base (every second pixel).  The DoG atlas is never stored: its readers form
+// G[l+1] - G[l] in float32 (the reference's DF values bit for bit), which saves a 4-byte
+// write and a re-read of the previous layer per pixel and layer.
 
 
 // global-address-space view of a generic pointer: lets the compiler emit global_load (vector
@@ -155,10 +157,7 @@ __device__ __forceinline__ float up2_value(const uint8_t* __restrict__ img, int 
 // converts each to float64 once and accumulates from registers.  The float32-rounded result
 // goes to an LDS mid-tile (row pitch 257 floats).  Horizontal pass: a thread owns 16
 // consecutive outputs of one row, reading 16 + 2*R2 mid values once.  Epilogue: lanes along
-// x; the previous layer's values for the DoG are loaded as one batch before any store (a
-// load after a store to a possibly aliasing address would wait a full memory latency per
-// element).  The DoG layer, read again only by the extrema pass, is written with non-temporal
-// stores; the Gaussian layer, the next launch's input, with ordinary ones.
+// x, coalesced stores of the gauss layer (the next launch's input).
 constexpr int BL_SW = 256;
 constexpr int BL_NT = 256;
 constexpr int BL_TH = 16;
@@ -228,7 +227,7 @@ __global__ __launch_bounds__(BL_NT) void k_blur(const SiftJob* __restrict__ jobs
     if (!BASE) {
         if (layer == 1) in = J.G0 + J.oct[o].g0base;
         else if (layer <= 4) in = J.G123 + J.oct[o].gbase + (long long)(layer - 2) * P;
-        else in = J.G4;
+        else in = J.G45 + J.oct[o].g45base;
     }
     const int tid = threadIdx.x;
     const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -303,31 +302,22 @@ __global__ __launch_bounds__(BL_NT) void k_blur(const SiftJob* __restrict__ jobs
         }
     }
     __syncthreads();
-    // epilogue with lanes along x: coalesced gauss / DoG / downsample stores
+    // epilogue with lanes along x: coalesced gauss / downsample stores (the DoG is never
+    // stored: its readers form G[l+1] - G[l], the same float32 subtraction)
     float* outG = nullptr;
-    float* outD = nullptr;
     float* down = nullptr;
     int R2n = 0, C2n = 0;
     if (BASE) {
         outG = J.G0 + J.oct[0].g0base;
     } else {
         if (layer <= 3) outG = J.G123 + J.oct[o].gbase + (long long)(layer - 1) * P;
-        else if (layer == 4) outG = J.G4;
-        outD = J.DF + J.oct[o].dbase + (long long)(layer - 1) * P;
+        else outG = J.G45 + J.oct[o].g45base + (long long)(layer - 4) * P;
         if (layer == 3 && o + 1 < J.n_oct) {
             down = J.G0 + J.oct[o + 1].g0base;
             R2n = J.oct[o + 1].R;
             C2n = J.oct[o + 1].C;
         }
     }
-    float iv[EPR][EPC];
-#pragma unroll
-    for (int r = 0; r < EPR; r++)
-#pragma unroll
-        for (int cc = 0; cc < EPC; cc++) {
-            const int gy = y0 + wv + r * NWV, lx = lane + 64 * cc, gx = x0 + lx;
-            iv[r][cc] = (!BASE && gy < R && lx < TWo && gx < C) ? gptr(in)[(size_t)gy * C + gx] : 0.0f;
-        }
 #pragma unroll
     for (int r = 0; r < EPR; r++) {
         const int ly = wv + r * NWV, gy = y0 + ly;
@@ -337,8 +327,7 @@ __global__ __launch_bounds__(BL_NT) void k_blur(const SiftJob* __restrict__ jobs
             if (gy < R && lx < TWo && gx < C) {
                 const float g = s_out[ly][lx];
                 const size_t p = (size_t)gy * C + gx;
-                if (outG) outG[p] = g;
-                if (outD) __builtin_nontemporal_store(g - iv[r][cc], &outD[p]);
+                outG[p] = g;
                 if (down && !(gy & 1) && !(gx & 1)) {
                     const int ry = gy >> 1, rx = gx >> 1;
                     if (ry < R2n && rx < C2n) down[(size_t)ry * C2n + rx] = g;
@@ -372,7 +361,7 @@ __device__ __forceinline__ int wave_append(bool f, int* counter) {
 
 // Finite differences of gsift.py:274-290 at one DoG voxel.  derivs_from takes the 19 samples
 // of the 3x3x3 cube (x / y / s = column / row / layer, p / m = +1 / -1); the extrema kernel
-// feeds it from registers and derivs_at from memory, so both give the same bits.
+// feeds it from registers and derivs_at from the gauss planes, so both give the same bits.
 struct Derivs {
     float v, dx, dy, ds, dxx, dyy, dss, dxy, dxs, dys;
 };
@@ -396,21 +385,38 @@ __device__ __forceinline__ Derivs derivs_from(float v, float xp, float xm, float
     return d;
 }
 
-__device__ __forceinline__ Derivs derivs_at(const float* __restrict__ D, long long k, long long C, long long RC) {
-    return derivs_from(D[k], D[k + 1], D[k - 1], D[k + C], D[k - C], D[k + RC], D[k - RC], D[k + C + 1],
-                       D[k + C - 1], D[k - C + 1], D[k - C - 1], D[k + RC + 1], D[k + RC - 1], D[k - RC + 1],
-                       D[k - RC - 1], D[k + RC + C], D[k + RC - C], D[k - RC + C], D[k - RC - C]);
+// gauss plane l (0..5) of octave o
+__device__ __forceinline__ const float* gauss_plane(const SiftJob& J, int o, int l, long long P) {
+    return l == 0 ? J.G0 + J.oct[o].g0base
+                  : (l <= 3 ? J.G123 + J.oct[o].gbase + (long long)(l - 1) * P
+                            : J.G45 + J.oct[o].g45base + (long long)(l - 4) * P);
+}
+
+// Derivatives at DoG layer l (1..3), plane offset k = r*C + c, from gauss planes l-1 .. l+2
+// (DoG layer m = G[m+1] - G[m]).
+__device__ __forceinline__ Derivs derivs_at(const SiftJob& J, int o, int l, long long k, long long C, long long P) {
+    const float* __restrict__ g0 = gauss_plane(J, o, l - 1, P);
+    const float* __restrict__ g1 = gauss_plane(J, o, l, P);
+    const float* __restrict__ g2 = gauss_plane(J, o, l + 1, P);
+    const float* __restrict__ g3 = gauss_plane(J, o, l + 2, P);
+    auto dm = [&](long long q) { return g1[q] - g0[q]; };   // DoG layer l-1
+    auto d0 = [&](long long q) { return g2[q] - g1[q]; };   // DoG layer l
+    auto dp = [&](long long q) { return g3[q] - g2[q]; };   // DoG layer l+1
+    return derivs_from(d0(k), d0(k + 1), d0(k - 1), d0(k + C), d0(k - C), dp(k), dm(k), d0(k + C + 1), d0(k + C - 1),
+                       d0(k - C + 1), d0(k - C - 1), dp(k + 1), dp(k - 1), dm(k + 1), dm(k - 1), dp(k + C), dp(k - C),
+                       dm(k + C), dm(k - C));
 }
 
 // Streaming-column extrema: each wave owns 62 columns of one octave and walks down EX_H
-// rows; per row it loads the 5 DoG layers (coalesced), takes the horizontal max/min over
+// rows; per row it loads the 6 gauss layers (coalesced), forms the 5 DoG layers, takes the
+// horizontal max/min over
 // (x-1, x, x+1) with lane shuffles and keeps a 3-row ring, so the 3x3x3 max/min of a voxel
 // costs a few max/min operations.  v >= max27 is the reference's non-strict test (no NaN).
 // Candidates are staged per wave in LDS and appended EX_BUF at a time with one atomic on the
 // image's counter (an atomic per row with candidates, all on one address, cost a third of the
 // kernel).  The kernel also keeps the raw 3-row ring, so it hands each candidate's first
 // refinement derivatives to k_refine (three of four candidates converge without moving, and
-// then refinement gathers nothing from the DoG atlas).
+// then refinement gathers nothing).
 constexpr int EX_BUF = 64;
 static_assert(EX_BUF >= EX_W, "one layer's candidates of a row always fit an empty buffer");
 
@@ -452,7 +458,12 @@ __global__ __launch_bounds__(256) void k_extrema(const SiftJob* __restrict__ job
     const int x = xs - 1 + lane;
     const int xc = min(max(x, 0), C - 1);
     const long long P = (long long)R * C;
-    const __attribute__((address_space(1))) float* D = gptr(J.DF + J.oct[o].dbase + xc);
+    // the six gauss planes of this octave; DoG layer l = G[l+1] - G[l] (float32, as the
+    // reference's DF atlas holds it)
+    const __attribute__((address_space(1))) float* Gp[6] = {
+        gptr(J.G0 + J.oct[o].g0base + xc), gptr(J.G123 + J.oct[o].gbase + xc),
+        gptr(J.G123 + J.oct[o].gbase + P + xc), gptr(J.G123 + J.oct[o].gbase + 2 * P + xc),
+        gptr(J.G45 + J.oct[o].g45base + xc), gptr(J.G45 + J.oct[o].g45base + P + xc)};
     const float pthr = J.pthr;
     const bool col_ok = lane >= 1 && lane <= EX_W && x >= BORDER && x < C - BORDER;
     float hmx[3][5], hmn[3][5];
@@ -470,27 +481,33 @@ __global__ __launch_bounds__(256) void k_extrema(const SiftJob* __restrict__ job
         for (int l = 0; l < 5; l++) prev[a][l] = 0.f;
     const int yend = min(y0 + EX_H, R);
     // rows are fetched 4 at a time, one block of 4 ahead of the one being processed
-    float nxt[4][5];
+    float nxt[4][6];
 #pragma unroll
     for (int r = 0; r < 4; r++) {
         const int yc = min(max(y0 - 1 + r, 0), R - 1);
 #pragma unroll
-        for (int l = 0; l < 5; l++) nxt[r][l] = D[l * P + (long long)yc * C];
+        for (int l = 0; l < 6; l++) nxt[r][l] = Gp[l][(long long)yc * C];
     }
     for (int yb = y0 - 1; yb <= yend; yb += 4) {
-        float cur[4][5];
+        float cg[4][6];
 #pragma unroll
         for (int r = 0; r < 4; r++)
 #pragma unroll
-            for (int l = 0; l < 5; l++) cur[r][l] = nxt[r][l];
+            for (int l = 0; l < 6; l++) cg[r][l] = nxt[r][l];
         if (yb + 4 <= yend) {
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int yc = min(max(yb + 4 + r, 0), R - 1);
 #pragma unroll
-                for (int l = 0; l < 5; l++) nxt[r][l] = D[l * P + (long long)yc * C];
+                for (int l = 0; l < 6; l++) nxt[r][l] = Gp[l][(long long)yc * C];
             }
         }
+        // DoG of this block's rows, formed after the next block's loads are in flight
+        float cur[4][5];
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int l = 0; l < 5; l++) cur[r][l] = cg[r][l + 1] - cg[r][l];
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const int yy = yb + r;
@@ -598,7 +615,6 @@ __global__ __launch_bounds__(256) void k_refine(const SiftJob* __restrict__ jobs
             long long rem = loc - (long long)(l - 1) * RC;
             r = (int)(rem / C);
             c = (int)(rem - (long long)r * C);
-            const float* D = J.DF + J.oct[o].dbase;
             bool good = false;
             // derivatives at the candidate itself come from the extrema kernel
             const float2* q = reinterpret_cast<const float2*>(J.cand_der) + (size_t)i * (DER_N / 2);
@@ -607,7 +623,7 @@ __global__ __launch_bounds__(256) void k_refine(const SiftJob* __restrict__ jobs
             d.v = q0.x; d.dx = q0.y; d.dy = q1.x; d.ds = q1.y; d.dxx = q2.x;
             d.dyy = q2.y; d.dss = q3.x; d.dxy = q3.y; d.dxs = q4.x; d.dys = q4.y;
             for (int it = 0; it < MAX_INTERP; it++) {
-                if (it > 0) d = derivs_at(D, (long long)l * RC + (long long)r * C + c, C, RC);
+                if (it > 0) d = derivs_at(J, o, l, (long long)r * C + c, C, RC);
                 const float a = d.dxx, bb = d.dxy, cc = d.dxs, dd = d.dxy, e = d.dyy, ff = d.dys, gg = d.dxs,
                             h = d.dys, ii = d.dss;
                 const float A0 = e * ii - ff * h, B0 = -(dd * ii - ff * gg), C0 = dd * h - e * gg;
@@ -1405,7 +1421,7 @@ __global__ __launch_bounds__(SP_THREADS) void k_spread(const SiftJob* __restrict
 static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct WsLayout {
-    size_t job, counters, DF, G0, G123, G4, cand, cand_der, refined, refined_src, oriented, keys0, keys1, sel, spread, total;
+    size_t job, counters, G0, G123, G45, cand, cand_der, refined, refined_src, oriented, keys0, keys1, sel, spread, total;
 };
 
 static WsLayout ws_layout(const SiftPlan& p, int cand_cap, int kp_cap) {
@@ -1413,10 +1429,9 @@ static WsLayout ws_layout(const SiftPlan& p, int cand_cap, int kp_cap) {
     size_t off = 0;
     L.job = off; off = align256(off + sizeof(SiftJob));
     L.counters = off; off = align256(off + 16 * sizeof(int));
-    L.DF = off; off = align256(off + 5 * (size_t)p.sumP * 4);
     L.G0 = off; off = align256(off + (size_t)p.sumP * 4);
     L.G123 = off; off = align256(off + 3 * (size_t)p.sumP * 4);
-    L.G4 = off; off = align256(off + (size_t)p.P0 * 4);
+    L.G45 = off; off = align256(off + 2 * (size_t)p.sumP * 4);
     L.cand = off; off = align256(off + (size_t)cand_cap * 4);
     L.cand_der = off; off = align256(off + (size_t)cand_cap * DER_N * 4);
     L.refined = off; off = align256(off + (size_t)cand_cap * 32);
@@ -1440,22 +1455,21 @@ static int fill_job(SiftJob* J, const SiftPlan& p, const uint8_t* img, int ld, d
     J->W = p.W;
     J->ld = ld;
     J->n_oct = p.n_oct;
-    long long db = 0, gb = 0, g0 = 0;
+    long long g45 = 0, gb = 0, g0 = 0;
     for (int o = 0; o < p.n_oct; o++) {
         long long P = (long long)p.R[o] * p.C[o];
         J->oct[o].R = p.R[o];
         J->oct[o].C = p.C[o];
-        J->oct[o].dbase = db;
+        J->oct[o].g45base = g45;
         J->oct[o].gbase = gb;
         J->oct[o].g0base = g0;
-        db += 5 * P;
+        g45 += 2 * P;
         gb += 3 * P;
         g0 += P;
     }
-    J->DF = (float*)(b + L.DF);
     J->G0 = (float*)(b + L.G0);
     J->G123 = (float*)(b + L.G123);
-    J->G4 = (float*)(b + L.G4);
+    J->G45 = (float*)(b + L.G45);
     J->cand = (uint32_t*)(b + L.cand);
     J->cand_der = (float*)(b + L.cand_der);
     J->refined = (float4*)(b + L.refined);
