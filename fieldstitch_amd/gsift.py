@@ -66,8 +66,9 @@ def default_caps(H: int, W: int, max_kp: int):
 
 
 class State:
-    """Device workspace of one detected image: the DoG atlas and gauss layers 1..3 that
-    describe() samples, plus the detect() output arrays (the reference's ``(xp, _Atlas)``)."""
+    """Device workspace of one detected image: the Gaussian planes (layers 1..3 are what
+    describe() samples; the DoG is their difference, never stored), plus the detect() output
+    arrays (the reference's ``(xp, _Atlas)``)."""
 
     def __init__(self, shape, ws, kp_dev, n, max_kp, stream):
         self.shape = shape
