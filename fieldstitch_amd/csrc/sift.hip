@@ -214,7 +214,8 @@ __global__ __launch_bounds__(BL_NT) void k_blur(const SiftJob* __restrict__ jobs
     constexpr int EPC = (TWo + 63) / 64;      // epilogue column steps per lane
     static_assert(BL_SW == BL_NT && BL_TH % NWV == 0 && BL_TH % 2 == 0, "one input column per thread, even rows");
     __shared__ float s_mid[BL_TH][BL_SW + 1];
-    __shared__ float s_out[BL_TH][BL_SW + 1];
+    __shared__ float s_out[BL_TH][TWo + 1];   // odd pitch: the horizontal pass writes conflict-free;
+                                              // 30.8 KB in all: 5 blocks per CU where VGPRs allow
     int tx, ty, tz;
     xcd_tile(tx, ty, tz);
     const SiftJob& J = jobs[tz];
