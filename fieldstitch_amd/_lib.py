@@ -55,6 +55,7 @@ SIGNATURES = {
     "fsx_mfma_layout_probe": [_vp, _vp],
     "fsx_mfma_i8_layout_probe": [_vp, _vp],
     "fsx_atan2_probe": [C.c_longlong, C.c_ulonglong, _vp, _vp],
+    "fsx_desc_angle_probe": [C.c_longlong, C.c_ulonglong, _vp, _vp],
     "fsx_undistort_table_bytes": [_i, C.POINTER(_sz)],
     "fsx_undistort_batch": [_i, C.POINTER(_i), C.POINTER(_vp), C.POINTER(C.c_double), C.POINTER(C.c_double), _i,
                             C.POINTER(_vp), C.POINTER(_vp), _vp, _sz, _vp],

@@ -686,19 +686,35 @@ __device__ __forceinline__ unsigned long long fx50(double v) {
     return (unsigned long long)__double_as_longlong(y) - 0x4330000000000000ull;
 }
 
-// The descriptor's per-sample gradient: float32 atan2 (ocml, a few ulp).  The descriptor bar is
-// +-1 unit per bin on the 512 scale, far above these ulps; the orientation histogram decides
-// its bins with the float64-rounded angle where the float32 one could differ (k_orient).
-__device__ __forceinline__ void grad_at_f(const __attribute__((address_space(1))) float* __restrict__ G, int C, int y,
-                                          int x, float& mag, float& ang) {
-    const size_t k = (size_t)y * C + x;
-    const float dx = G[k + 1] - G[k - 1];
-    const float dy = G[k - C] - G[k + C];
-    const float xx = dx * dx, yy = dy * dy;
-    mag = sqrtf(xx + yy);
-    const float deg = 180.0f / 3.14159265358979323846f;
-    float a = atan2f(dy, dx) * deg;
-    ang = a < 0.f ? a + 360.0f : a;
+// Gradient angle in degrees [0, 360) for the descriptor's samples: t = min / max of |dx|, |dy|
+// (one hardware reciprocal, 1 ulp), atan(t) = t * P(t^2) with an odd degree-17 polynomial
+// fitted to atan on [0, 1] by iterated weighted least squares (max error 1.0e-7 rad in
+// float32 Horner form), then the octant folds.  1e-7 rad moves a sample by 5e-7 of an
+// orientation bin; the descriptor's bar is +-1 unit per bin on the 512 scale.
+__device__ __forceinline__ float desc_angle_deg(float dy, float dx) {
+    const float ax = fabsf(dx), ay = fabsf(dy);
+    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+    const float t = mx > 1e-30f ? mn * __builtin_amdgcn_rcpf(mx) : 0.0f;
+    const float z = t * t;
+    float p = 0.00245657f;
+    p = fmaf(p, z, -0.0144007f);
+    p = fmaf(p, z, 0.03978004f);
+    p = fmaf(p, z, -0.07234744f);
+    p = fmaf(p, z, 0.10498885f);
+    p = fmaf(p, z, -0.1416121f);
+    p = fmaf(p, z, 0.19985904f);
+    p = fmaf(p, z, -0.33332597f);
+    p = fmaf(p, z, 0.99999989f);
+    float r = t * p;   // (Horner steps fused)
+    if (ay > ax) r = 1.57079632679489662f - r;
+    if (dx < 0.0f) r = 3.14159265358979324f - r;
+    const float d = r * (180.0f / 3.14159265358979324f);
+    return dy < 0.0f ? 360.0f - d : d;
+}
+
+// fixed-point 2^-50 bits of a value already scaled by 2^50 (see fx50)
+__device__ __forceinline__ unsigned long long fx50_scaled(double v) {
+    return (unsigned long long)__double_as_longlong(v + 0x1p52) - 0x4330000000000000ull;
 }
 
 // Orientation (gsift.py:372-432 with the detect epilogue :462-469).  One wave per refined
@@ -920,6 +936,45 @@ __global__ __launch_bounds__(256) void k_atan2_probe(long long n, unsigned long 
     if ((threadIdx.x & 63) == 0) atomicMax(out, m);
 }
 
+// The descriptor's angle (desc_angle_deg) against the float64 atan2 in degrees on [0, 360):
+// out[0] = largest circular difference in units of 1e-9 degree over the same argument pairs
+// (only normal-range components: the descriptor's gradients are differences of Gaussian
+// values in [0, 1]), out[1] = the same for the atan2f form, for comparison.
+__global__ __launch_bounds__(256) void k_desc_angle_probe(long long n, unsigned long long seed,
+                                                          unsigned* __restrict__ out) {
+    unsigned m0 = 0, m1 = 0;
+    const float deg = 180.0f / 3.14159265358979323846f;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const unsigned long long h = probe_mix(seed ^ probe_mix((unsigned long long)i));
+        float y = probe_value((unsigned)h, false), x = probe_value((unsigned)(h >> 32), false);
+        switch (i & 63) {
+            case 0: x = 0.0f; break;
+            case 1: y = 0.0f; break;
+            case 2: y = x; break;
+            case 3: y = -x; break;
+            default: break;
+        }
+        double r = atan2((double)y, (double)x) * (180.0 / 3.14159265358979323846);
+        if (r < 0.0) r += 360.0;
+        const float a0 = desc_angle_deg(y, x);
+        float a1 = atan2f(y, x) * deg;
+        a1 = a1 < 0.0f ? a1 + 360.0f : a1;
+        double d0 = fabs((double)a0 - r), d1 = fabs((double)a1 - r);
+        d0 = fmin(d0, 360.0 - d0);
+        d1 = fmin(d1, 360.0 - d1);
+        m0 = max(m0, (unsigned)ceil(d0 * 1e9));
+        m1 = max(m1, (unsigned)ceil(d1 * 1e9));
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        m0 = max(m0, (unsigned)__shfl_xor((int)m0, off));
+        m1 = max(m1, (unsigned)__shfl_xor((int)m1, off));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&out[0], m0);
+        atomicMax(&out[1], m1);
+    }
+}
+
 // ------------------------------------------------------------------------------------------
 // Deterministic sort of (k0, k1) key pairs: 1024-element bitonic blocks in LDS, then merge
 // passes where every element finds its output slot by binary search in the partner run.
@@ -1083,6 +1138,7 @@ __device__ __forceinline__ void describe_kp(const SiftJob& J, const float* __res
     const float t = ori * (float)(3.14159265358979323846 / 180.0);
     const double cos_t = (double)(float)cos((double)t) / hw, sin_t = (double)(float)sin((double)t) / hw;
     const long long pr = (long long)rint(py), pc = (long long)rint(px);
+    const int pr32 = (int)pr, pc32 = (int)pc;   // keypoints lie inside the octave
     const __attribute__((address_space(1))) float* G = gptr(J.G123 + J.oct[o].gbase + (long long)(li - 1) * R * C);
     for (int b = threadIdx.x; b < HIST_SIZE * DS_COPIES; b += 256) s_h[b] = 0ull;
     // per row i of the window, the column span [jl, jh] outside of which |c_rot| or |r_rot|
@@ -1152,13 +1208,17 @@ __device__ __forceinline__ void describe_kp(const SiftJob& J, const float* __res
         const double c_rot = oj * cos_t - oi * sin_t;
         const double r_rot = oj * sin_t + oi * cos_t;
         const double rbin = (r_rot + 2.0) - 0.5, cbin = (c_rot + 2.0) - 0.5;
-        const long long yy = pr + i, xx = pc + j;
+        const int yy = pr32 + i, xx = pc32 + j;
         if (!(rbin > -1.0 && rbin < 4.0 && cbin > -1.0 && cbin < 4.0 && yy > 0 && yy < R - 1 && xx > 0 && xx < C - 1))
             continue;
-        float mag, ang;
-        grad_at_f(G, C, (int)yy, (int)xx, mag, ang);
-        const double w = (double)expf((float)((c_rot * c_rot + r_rot * r_rot) * (-0.125)));
-        const double mg = (double)mag * w;
+        // gradient as gsift.py:256-267 (magnitude correctly rounded, angle by desc_angle_deg)
+        const size_t kk = (size_t)yy * C + xx;
+        const float gdx = G[kk + 1] - G[kk - 1];
+        const float gdy = G[kk - C] - G[kk + C];
+        const float mag = sqrtf(gdx * gdx + gdy * gdy);
+        const float ang = desc_angle_deg(gdy, gdx);
+        const double w = (double)__expf((float)((c_rot * c_rot + r_rot * r_rot) * (-0.125)));
+        const double mg = ((double)mag * w) * 0x1p50;   // exact scaling: fixed point below
         const float obin = (ang - ori) * obs;
         const double r0 = floor(rbin), c0 = floor(cbin);
         const float o0f = floorf(obin);
@@ -1177,14 +1237,14 @@ __device__ __forceinline__ void describe_kp(const SiftJob& J, const float* __res
         // copy threadIdx % DS_COPIES of each bin: neighbouring samples, which mostly share
         // their bins, spread over DS_COPIES addresses instead of contending for one
         const int idx = ((((int)r0 + 1) * 6 + (int)c0 + 1) * 10 + o0) * DS_COPIES + (threadIdx.x % DS_COPIES);
-        atomicAdd(&s_h[idx], fx50(v000));
-        atomicAdd(&s_h[idx + 1 * DS_COPIES], fx50(v001));
-        atomicAdd(&s_h[idx + 10 * DS_COPIES], fx50(v010));
-        atomicAdd(&s_h[idx + 11 * DS_COPIES], fx50(v011));
-        atomicAdd(&s_h[idx + 60 * DS_COPIES], fx50(v100));
-        atomicAdd(&s_h[idx + 61 * DS_COPIES], fx50(v101));
-        atomicAdd(&s_h[idx + 70 * DS_COPIES], fx50(v110));
-        atomicAdd(&s_h[idx + 71 * DS_COPIES], fx50(v111));
+        atomicAdd(&s_h[idx], fx50_scaled(v000));
+        atomicAdd(&s_h[idx + 1 * DS_COPIES], fx50_scaled(v001));
+        atomicAdd(&s_h[idx + 10 * DS_COPIES], fx50_scaled(v010));
+        atomicAdd(&s_h[idx + 11 * DS_COPIES], fx50_scaled(v011));
+        atomicAdd(&s_h[idx + 60 * DS_COPIES], fx50_scaled(v100));
+        atomicAdd(&s_h[idx + 61 * DS_COPIES], fx50_scaled(v101));
+        atomicAdd(&s_h[idx + 70 * DS_COPIES], fx50_scaled(v110));
+        atomicAdd(&s_h[idx + 71 * DS_COPIES], fx50_scaled(v111));
     }
     __syncthreads();
     if (threadIdx.x < 128) {
@@ -1809,6 +1869,15 @@ int fsx_sift_features_batch(int n_img, const uint8_t* const* imgs, const int* H,
 int fsx_atan2_probe(long long n, unsigned long long seed, unsigned* max_ulp_dev, void* stream) {
     FSX_REQUIRE(n >= 1 && max_ulp_dev, "bad atan2 probe arguments");
     hipLaunchKernelGGL(k_atan2_probe, dim3(1024), dim3(256), 0, (hipStream_t)stream, n, seed, max_ulp_dev);
+    FSX_LAUNCH_CHECK();
+    return FSX_OK;
+}
+
+// Largest error of the descriptor's gradient angle and of the atan2f form, in 1e-9 degree,
+// over n seeded pairs (k_desc_angle_probe); out[0..1] must hold zeros on entry.
+int fsx_desc_angle_probe(long long n, unsigned long long seed, unsigned* out, void* stream) {
+    FSX_REQUIRE(n >= 1 && out, "bad descriptor angle probe arguments");
+    hipLaunchKernelGGL(k_desc_angle_probe, dim3(1024), dim3(256), 0, (hipStream_t)stream, n, seed, out);
     FSX_LAUNCH_CHECK();
     return FSX_OK;
 }

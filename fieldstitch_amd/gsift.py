@@ -176,3 +176,15 @@ def atan2_probe(n: int = 1 << 26, seed: int = 0x5EED) -> int:
     _lib.call("fsx_atan2_probe", C.c_longlong(int(n)), C.c_ulonglong(int(seed)), C.c_void_p(out.data_ptr()),
               C.c_void_p(_lib.stream_handle()))
     return int(out.item())
+
+
+def desc_angle_probe(n: int = 1 << 26, seed: int = 0x5EED):
+    """(descriptor angle error, atan2f-form error), both the largest circular difference in
+    degrees from the float64 atan2 over n seeded gradient-like argument pairs (no reference
+    counterpart: it bounds the descriptor's polynomial angle)."""
+    torch = _torch()
+    out = torch.zeros(2, dtype=torch.int32, device="cuda")
+    _lib.call("fsx_desc_angle_probe", C.c_longlong(int(n)), C.c_ulonglong(int(seed)), C.c_void_p(out.data_ptr()),
+              C.c_void_p(_lib.stream_handle()))
+    e = out.cpu().numpy().view(np.uint32).astype(np.float64) * 1e-9
+    return float(e[0]), float(e[1])

@@ -199,6 +199,13 @@ int fsx_mfma_i8_layout_probe(int *out_dev, void *stream);
  * bin path (no reference counterpart). */
 int fsx_atan2_probe(long long n, unsigned long long seed, unsigned *max_ulp_dev, void *stream);
 
+/* Largest circular difference, in units of 1e-9 degree, between the descriptor's gradient
+ * angle (a fitted odd polynomial for atan, sift.hip desc_angle_deg) and the float64 atan2 in
+ * degrees on [0, 360), over n seeded gradient-like argument pairs: out[0]; out[1] = the same
+ * for the atan2f form.  out[0..1] must be zero on entry.  Bounds the descriptor's angle error
+ * (no reference counterpart). */
+int fsx_desc_angle_probe(long long n, unsigned long long seed, unsigned *out, void *stream);
+
 /* ---------------------------------------------------------------------------------------
  * Float-descriptor brute-force matcher (matching.py:86-110 on float descriptors)
  * ------------------------------------------------------------------------------------- */

@@ -130,6 +130,19 @@ def test_atan2f_error_bound(gpu):
     assert worst <= 4
 
 
+def test_descriptor_angle_error_bound(gpu):
+    """The descriptor's gradient angle is an odd polynomial fitted to atan (1e-7 rad in float32
+    Horner form) plus octant folds.  Its error in degrees, circular, against the float64
+    atan2 must stay within 2 float32 ulps at 360 degrees (the representation limit of the
+    angle itself, 6.1e-5) and near the atan2f form's own error."""
+    from fieldstitch_amd import gsift
+    errs = [gsift.desc_angle_probe(1 << 24, seed) for seed in (21, 22)]
+    poly, ref = max(e[0] for e in errs), max(e[1] for e in errs)
+    print("descriptor angle max error (deg)", poly, "atan2f form", ref, flush=True)
+    assert poly <= 6.2e-5
+    assert poly <= ref + 1e-5
+
+
 def _diamonds(H=160, W=200, seed=0):
     """Flat background with diamond blobs u8(60 + A (1 - (|x - cx| + |y - cy|) / r)) at integer
     centres: inside a quadrant the image, and so every Gaussian layer, is a function of x +- y
