@@ -688,8 +688,8 @@ __device__ __forceinline__ unsigned long long fx50(double v) {
 
 // Gradient angle in degrees [0, 360) for the descriptor's samples: t = min / max of |dx|, |dy|
 // (one hardware reciprocal, 1 ulp), atan(t) = t * P(t^2) with an odd degree-17 polynomial
-// fitted to atan on [0, 1] by iterated weighted least squares (max error 1.0e-7 rad in
-// float32 Horner form), then the octant folds.  1e-7 rad moves a sample by 5e-7 of an
+// fitted to atan on [0, 1] by iterated weighted least squares (atan_poly_fit.py in this
+// directory; max error 1.0e-7 rad in float32 Horner form), then the octant folds.  1e-7 rad moves a sample by 5e-7 of an
 // orientation bin; the descriptor's bar is +-1 unit per bin on the 512 scale.
 __device__ __forceinline__ float desc_angle_deg(float dy, float dx) {
     const float ax = fabsf(dx), ay = fabsf(dy);
