@@ -53,3 +53,25 @@ def test_spread_matches_reference_formula(gpu):
     fb.run([torch.from_numpy(img).cuda()])
     np.testing.assert_array_equal(fb._sel(0, nfeat), sel)
     assert len(np.unique(kps["response"])) < len(kps["response"])   # ties present
+
+
+def test_feature_batch_full_size(gpu):
+    """Two photos of the benchmark's size (2640x1978, nfeat 8000) through one FeatureBatch
+    equal the per-image drop-in path (detect with its default contrast 0.015 and edge 12,
+    host _spread, sort, describe, RootSIFT) bit for bit."""
+    import torch
+    from fieldstitch_amd import features
+    fl = synth.flight(seed=2, rows=1, cols=2, workers=8)
+    imgs = fl.images
+    print("case full size", [i.shape for i in imgs], flush=True)
+    fb = features.FeatureBatch([i.shape for i in imgs], nfeat=8000, contrast=0.015, edge=12.0, float_desc=True)
+    fb.run([torch.from_numpy(i).cuda() for i in imgs])
+    fb.check_capacity()
+    for i, img in enumerate(imgs):
+        xy, u8, resp, d = fb.results(i)
+        xr, dr, rr, _ = features.sift_gpu(img, 8000)
+        assert len(xy) == 8000
+        np.testing.assert_array_equal(xy, xr)
+        np.testing.assert_array_equal(resp, rr)
+        np.testing.assert_array_equal(d, dr)
+        np.testing.assert_array_equal(u8, features.rootsift_u8(dr))

@@ -83,6 +83,25 @@ def test_detect_vs_oracle(gpu, seed, shape, contrast, edge):
     assert np.abs(d - dr).mean() <= 1e-3
 
 
+def test_detect_full_size_vs_oracle(gpu):
+    """One photo of the benchmark's size, 2640x1978 (9 octaves, 5280x3956 down to 21x16,
+    ~17.5 k keypoints): keypoints identical to the oracle's bits, and 2000 descriptors spread
+    over the response order within the descriptor bar."""
+    from fieldstitch_amd import gsift
+    from oracle import sift_oracle as so
+    img, _, _ = synth.field_pair(5, W=2640, H=1978)
+    print("case full size", img.shape, flush=True)
+    k, st = gsift.detect(img, max_kp=20000, contrast=0.04, edge=10.0)
+    kr, sr = so.detect(img, 20000, 0.04, 10.0)
+    assert len(kr["x"]) > 10000
+    assert_kps_match(k, kr)
+    sel = np.linspace(0, len(k["x"]) - 1, 2000).astype(np.int64)
+    d = gsift.describe(st, k, sel)
+    dr = so.describe(sr, kr, sel)
+    assert np.abs(d - dr).max() <= 1.0
+    assert np.abs(d - dr).mean() <= 1e-3
+
+
 def test_rootsift_u8_epilogue(gpu):
     from fieldstitch_amd import gsift
     from oracle import sift_oracle as so
