@@ -1123,7 +1123,17 @@ __device__ __forceinline__ void describe_kp(const SiftJob& J, const float* __res
                                            unsigned long long* s_h, float* s_d) {
     const __attribute__((address_space(1))) float* kpa = gptr(kpa_);
     const __attribute__((address_space(1))) int* kpi = gptr((const int*)kpa_);
-    const int o = kpi[5 * (long long)stride + k], li = kpi[6 * (long long)stride + k];
+    // a selection outside the arrays, or an octave / layer the workspace does not hold, gives
+    // a zero row (block-uniform: k, o and li are the same for every thread)
+    const bool k_ok = k >= 0 && k < stride;
+    const int o = k_ok ? kpi[5 * (long long)stride + k] : -1, li = k_ok ? kpi[6 * (long long)stride + k] : 0;
+    if (!(o >= 0 && o < J.n_oct && li >= 1 && li <= N_LAYERS)) {
+        if (threadIdx.x < 128) {
+            if (desc_row) desc_row[threadIdx.x] = 0.0f;
+            if (u8_row) u8_row[threadIdx.x] = 0;
+        }
+        return;
+    }
     const int R = J.oct[o].R, C = J.oct[o].C;
     const double f = 1.0 / (ldexp(1.0, o) * 0.5);
     const double px = (double)kpa[k] * f, py = (double)kpa[(long long)stride + k] * f;

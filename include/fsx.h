@@ -73,7 +73,9 @@ int fsx_sift_counters(const void *ws_dev, int *out4, void *stream);
 /* gsift.describe (gsift.py:480-525, _descr_flat :800-871): 128-d descriptors (L2 norm 512)
  * of keypoints sel_dev[0..m) of the arrays kp_dev[7][kp_stride] (layout of fsx_sift_detect's
  * output), sampled from the workspace of the detect call.  desc_dev: m x 128 float32.
- * u8_dev (may be NULL): RootSIFT uint8 encoding of the same rows (features.py:80-82 _save). */
+ * u8_dev (may be NULL): RootSIFT uint8 encoding of the same rows (features.py:80-82 _save).
+ * A selection outside [0, kp_stride), or a keypoint whose octave / layer the workspace does
+ * not hold, yields a zero row (the device cannot report it; gsift.describe checks both). */
 int fsx_sift_describe(const void *ws_dev, const float *kp_dev, int kp_stride, const long long *sel_dev, int m,
                       float *desc_dev, uint8_t *u8_dev, void *stream);
 

@@ -207,3 +207,38 @@ def test_capacity_overflow_retry(gpu):
     assert len(k_big["x"]) > 1000
     for key in gsift.KEYS:
         np.testing.assert_array_equal(k_small[key], k_big[key])
+
+
+def test_describe_abi_rejects_bad_rows(gpu):
+    """fsx_sift_describe called directly (gsift.describe checks these on the host): selection
+    indices outside the keypoint arrays and a keypoint whose layer the workspace does not hold
+    give zero rows; the other rows equal the ordinary descriptors."""
+    import ctypes as C
+    import torch
+    from fieldstitch_amd import _lib, gsift
+    g = _golden("c1_crop_cv")
+    k, st = gsift.detect(g["img"], max_kp=12000, contrast=0.04, edge=10.0)
+    n = len(k["x"])
+    assert n > 20
+    good = gsift.describe(st, k, np.arange(10))
+    kp = np.empty((7, n), np.float32)
+    for i, key in enumerate(gsift.KEYS[:5]):
+        kp[i] = k[key]
+    kp[5] = k["octave"].view(np.float32)
+    layer = k["layer"].copy()
+    layer[3] = 7                                   # not a layer of the workspace
+    kp[6] = layer.view(np.float32)
+    sel = np.array([0, 1, -1, 3, n, 5, 1 << 40, 9], np.int64)
+    kp_d = torch.from_numpy(kp).cuda()
+    sel_d = torch.from_numpy(sel).cuda()
+    out = torch.full((len(sel), 128), 7.0, dtype=torch.float32, device="cuda")
+    u8 = torch.full((len(sel), 128), 7, dtype=torch.uint8, device="cuda")
+    _lib.call("fsx_sift_describe", C.c_void_p(st.ws.data_ptr()), C.c_void_p(kp_d.data_ptr()), n,
+              C.c_void_p(sel_d.data_ptr()), len(sel), C.c_void_p(out.data_ptr()), C.c_void_p(u8.data_ptr()),
+              C.c_void_p(st.stream))
+    d, b = out.cpu().numpy(), u8.cpu().numpy()
+    for r, s in enumerate(sel):
+        if s in (-1, n, 1 << 40, 3):
+            assert not d[r].any() and not b[r].any(), r
+        else:
+            np.testing.assert_array_equal(d[r], good[s])
