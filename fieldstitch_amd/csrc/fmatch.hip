@@ -900,6 +900,7 @@ int fsx_fmatch_batch(int n_set, const int* n_kp, const float* desc_dev, int n_pa
 // stats[4] = fallback queries, max |d~ - d|, mean |d~ - d|, rescored candidates.
 int fsx_fmatch_results(int n_set, const int* n_kp, int n_pair, const int* pa, const int* pb, const void* ws,
                        int* counts, int* q, int* t, double* stats, void* stream) {
+    FSX_REQUIRE(ws && (counts || n_pair <= 0), "null pointer argument");
     FPlan M;
     int rc = fplan(n_set, n_kp, n_pair, pa, pb, M);
     if (rc) return rc;

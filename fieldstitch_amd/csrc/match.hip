@@ -1166,6 +1166,7 @@ int fsx_match_filter_features(void* ws, const long long* info, const float* size
 int fsx_match_results(int n_img, const int* n_kp, const int* W, const int* H, int n_pair, const int* pa,
                       const int* pb, double radius, const void* ws, int* counts, int* q, int* t, int* flags,
                       void* stream) {
+    FSX_REQUIRE(ws && (counts || n_pair <= 0), "null pointer argument");
     MatchPlanHost M;
     int rc = match_plan(n_img, n_kp, W, H, n_pair, pa, pb, nullptr, nullptr, radius, M);
     if (rc) return rc;
@@ -1260,6 +1261,7 @@ int fsx_match_candidate_offsets(int n_img, const int* n_kp, const int* W, const 
 int fsx_match_query_state(int n_img, const int* n_kp, const int* W, const int* H, int n_pair, const int* pa,
                           const int* pb, double radius, const void* ws, int* best, double* b1, int* qcell,
                           void* stream) {
+    FSX_REQUIRE(ws, "null pointer argument");
     MatchPlanHost M;
     int rc = match_plan(n_img, n_kp, W, H, n_pair, pa, pb, nullptr, nullptr, radius, M);
     if (rc) return rc;
