@@ -358,22 +358,30 @@ def bench_flight400(args):
 
 
 def bench_filter(args):
-    """The learned match filter at reduced scale: trained on the plain pipeline's
-    RANSAC-verified inliers of 4 synthetic 8-photo flights (2640x1978, nfeat 8000), threshold
-    chosen on a 5th for 99.7 % inlier recall, evaluated against the 99.5 % target on a 6th.  Reports the pruned fraction
-    and the inlier fraction of what reaches verification with and without the network."""
+    """The learned match filter on 400-photo flights (20 x 20 grid, 2640x1978 and 1320x989
+    mixed 50/50, nfeat 8000, every overlapping pair guided-matched): trained with Adam (lr 1e-3,
+    batch 65536) on the plain pipeline's RANSAC-verified inliers of the first n - 2 flights,
+    threshold chosen on the next for 99.7 % inlier recall, evaluated against the 99.5 % target
+    on the last.  Reports the pruned fraction and the inlier fraction of what reaches
+    verification with and without the network."""
     import torch
     from fieldstitch_amd import filter_mlp as fm
     t0 = time.time()
+    n_fl = max(3, args.filter_flights)
+    workers = max(1, min(args.gen_workers, os.cpu_count() or 1))
     data = []
-    for k, seed in enumerate(range(40, 46)):
-        data.append(fm.flight_candidates(seed, workers=max(1, min(args.gen_workers, os.cpu_count() or 1))))
-        log("filter: flight %d: %d candidates, %d plain, %d inliers (%.0f s)" %
-            (k, data[-1]["X"].shape[0], data[-1]["n_plain"], data[-1]["n_inliers"], time.time() - t0))
-    Xtr = torch.cat([d["X"] for d in data[:4]])
-    ytr = torch.cat([d["y"] for d in data[:4]])
+    for k, seed in enumerate(range(40, 40 + n_fl)):
+        data.append(fm.flight_candidates(seed, workers=workers, rows=20, cols=20, lowres_frac=0.5))
+        log("filter: flight %d: %d pairs, %d candidates, %d plain, %d inliers (%.0f s)" %
+            (k, data[-1]["n_pairs"], data[-1]["X"].shape[0], data[-1]["n_plain"], data[-1]["n_inliers"],
+             time.time() - t0))
+    n_tr = n_fl - 2
+    Xtr = torch.cat([d["X"] for d in data[:n_tr]])
+    ytr = torch.cat([d["y"] for d in data[:n_tr]])
+    for d in data[:n_tr]:
+        d["X"] = d["y"] = None            # the training copy is all that is needed
     net = fm.MatchFilterMLP(seed=1, lr=1e-3)
-    batch = 8192
+    batch = args.filter_batch
     epochs = args.filter_epochs
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -381,24 +389,26 @@ def bench_filter(args):
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t1
     steps = epochs * (Xtr.shape[0] // batch)
-    pv = net.predict(data[4]["X"]).cpu().numpy()
+    pv = net.predict(data[n_tr]["X"]).cpu().numpy()
     # the threshold keeps 99.7 % of the validation flight's inliers: margin for the 99.5 % target
-    tau = fm.threshold_for_recall(pv, data[4]["label"], 0.997)
-    Xte = data[5]["X"]
+    tau = fm.threshold_for_recall(pv, data[n_tr]["label"], 0.997)
+    Xte = data[n_tr + 1]["X"]
     net.predict(Xte[:65536])
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     pt = net.predict(Xte)
     torch.cuda.synchronize()
     t_inf = time.perf_counter() - t2
-    ev = fm.evaluate(pt.cpu().numpy(), tau, data[5])
+    ev = fm.evaluate(pt.cpu().numpy(), tau, data[n_tr + 1])
     flops_fwd = 2.0 * (288 * 256 + 256 * 256 + 256)
     return {
         "metric": "learned match filter: verified-inlier recall vs the plain set (target >= 99.5 %)",
         "value": ev["inlier_recall"], "unit": "recall", "n_gpus": 1, "higher_is_better": True,
-        "data": "synthetic (6 seeded 8-photo flights, 2640x1978, RANSAC labels from the plain pipeline)",
-        "config": {"workload": "MLP 264-256-256-1 BF16, Adam lr 1e-3, batch %d, %d epochs; train 4 flights, "
-                               "threshold (99.7 %% recall) on 1, test on 1" % (batch, epochs),
+        "data": "synthetic (%d seeded 400-photo flights, 2640x1978 / 1320x989, RANSAC labels from the plain "
+                "pipeline)" % n_fl,
+        "config": {"workload": "MLP 264-256-256-1 BF16, Adam lr 1e-3, batch %d, %d epochs; train %d flights, "
+                               "threshold (99.7 %% recall) on 1, test on 1" % (batch, epochs, n_tr),
+                   "pairs_per_flight": [d["n_pairs"] for d in data],
                    "train_candidates": int(Xtr.shape[0]), "test_candidates": int(Xte.shape[0])},
         "train": {"seconds": t_train, "steps": steps, "samples_per_s": steps * batch / t_train,
                   "tflops": 3 * flops_fwd * steps * batch / t_train / 1e12, "epoch_loss": losses},
@@ -439,8 +449,10 @@ def main():
                     help="processes rendering the synthetic photos (1 under a profiler)")
     ap.add_argument("--no-precision", action="store_true", help="skip the float-descriptor precision section")
     ap.add_argument("--filter", action="store_true",
-                    help="only train and evaluate the learned match filter (6 synthetic flights)")
+                    help="only train and evaluate the learned match filter (400-photo synthetic flights)")
     ap.add_argument("--filter-epochs", type=int, default=8)
+    ap.add_argument("--filter-flights", type=int, default=7, help="400-photo flights: n-2 train, 1 threshold, 1 test")
+    ap.add_argument("--filter-batch", type=int, default=65536)
     ap.add_argument("--flight400", action="store_true",
                     help="only run the 400-photo mixed-resolution flight (graph-captured SIFT chunks + guided matching)")
     ap.add_argument("--precision-pairs", type=int, default=200)

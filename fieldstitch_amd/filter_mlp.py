@@ -149,32 +149,39 @@ class MatchFilterMLP:
 # ------------------------------------------------------------------------------------------
 # Training data from the plain pipeline and the filter's evaluation
 
-def flight_candidates(seed, workers=8, radius=60.0, ratio=0.8, n_hyp=512, thr=3.0, nfeat=8000, **flight_kw):
-    """Candidates and labels of one synthetic 2 x 4 flight: SIFT on the GPU, guided matching
-    (the plain method), homography RANSAC on each pair's plain matches.  Returns a dict with
-    X (bf16 [n, 288] device rows, one per query with a candidate), y (float32 device labels:
-    1 = the plain pipeline's match and a RANSAC inlier), plain (bool numpy: accepted by the
-    ratio test and the one-query-per-target rule) and counts."""
+def flight_candidates(seed, workers=8, radius=60.0, ratio=0.8, n_hyp=512, thr=3.0, nfeat=8000, chunk=32,
+                      **flight_kw):
+    """Candidates and labels of one synthetic flight (synth.flight with flight_kw: a 2 x 4 grid
+    by default, 20 x 20 for the 400-photo flights): SIFT on the GPU in chunks of `chunk` photos,
+    guided matching of every overlapping pair (the plain method), homography RANSAC on each
+    pair's plain matches.  Returns a dict with X (bf16 [n, 288] device rows, one per query with
+    a candidate), y (float32 device labels: 1 = the plain pipeline's match and a RANSAC
+    inlier), plain (bool numpy: accepted by the ratio test and the one-query-per-target rule)
+    and counts."""
     import torch
     from . import features, synth
     from .matching import PairMatcher
     from .verify import ransac_homography
     fl = synth.flight(seed=seed, workers=workers, **flight_kw)
-    dev = [torch.from_numpy(im).cuda() for im in fl.images]
     shapes = [im.shape for im in fl.images]
-    fb = features.FeatureBatch(shapes, nfeat=nfeat, contrast=0.04, edge=10.0)
-    fb.run(dev)
-    fb.check_capacity()
-    images, sizes, angles, xys = [], [], [], []
-    for i, (H, W) in enumerate(shapes):
-        xy, u8, _, _ = fb.results(i)
-        sel = fb._sel(i, len(xy))
-        kp = fb.kp[i].cpu().numpy()
-        xy64 = xy.astype(np.float64)
-        images.append((fb.u8[i, :len(xy)].contiguous(), torch.from_numpy(xy64).cuda(), (W, H)))
-        sizes.append(kp[2][sel])
-        angles.append(kp[3][sel])
-        xys.append(xy64)
+    n_img = len(shapes)
+    images, sizes, angles, xys = [None] * n_img, [None] * n_img, [None] * n_img, [None] * n_img
+    for c0 in range(0, n_img, chunk):
+        idx = list(range(c0, min(n_img, c0 + chunk)))
+        fb = features.FeatureBatch([shapes[i] for i in idx], nfeat=nfeat, contrast=0.04, edge=10.0)
+        fb.run([torch.from_numpy(fl.images[i]).cuda() for i in idx])
+        fb.check_capacity()
+        for k, i in enumerate(idx):
+            H, W = shapes[i]
+            xy, u8, _, _ = fb.results(k)
+            sel = fb._sel(k, len(xy))
+            kp = fb.kp[k].cpu().numpy()
+            xy64 = xy.astype(np.float64)
+            images[i] = (fb.u8[k, :len(xy)].clone(), torch.from_numpy(xy64).cuda(), (W, H))
+            sizes[i] = kp[2][sel]
+            angles[i] = kp[3][sel]
+            xys[i] = xy64
+        del fb
     pairs = list(fl.pairs)
     pm = PairMatcher(images, pairs, [fl.prior[p] for p in pairs], radius=radius, kc=0, ratio=ratio)
     pm.run("mfma")
