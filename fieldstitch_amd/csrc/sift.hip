@@ -741,7 +741,7 @@ __global__ __launch_bounds__(64) void k_orient(const SiftJob* __restrict__ jobs)
     __shared__ unsigned long long s_k0[ORI_BUF];
     __shared__ int s_rank[ORI_BUF];
     __shared__ float s_f[ORI_BINS];
-    __shared__ float s_w[2 * ORI_R_MAX * ORI_R_MAX + 1];   // gaussian weight by i*i + j*j
+    __shared__ float s_w[(ORI_R_MAX + 1) * (ORI_R_MAX + 2) / 2];   // gaussian weight by (max, min) of |i|, |j|
     const SiftJob& J = jobs[blockIdx.z];
     const int n = min(J.counters[1], J.cand_cap);
     const int lane = threadIdx.x;
@@ -776,9 +776,16 @@ __global__ __launch_bounds__(64) void k_orient(const SiftJob* __restrict__ jobs)
         const float den = 2.0f * (sw * sw);
         const __attribute__((address_space(1))) float* G = gptr(J.G123 + J.oct[o].gbase + (long long)(l - 1) * R * C);
         if (lane < ORI_BINS) s_h[lane] = 0ull;
-        // w = exp(-(i*i + j*j) / den) depends on i*i + j*j only: one float64 exp per value
-        const int dmax = 2 * rad * rad;
-        for (int d = lane; d <= dmax; d += 64) s_w[d] = (float)exp((double)(-(float)d / den));
+        // w = exp(-(i*i + j*j) / den), tabulated by (a, b) = (max(|i|, |j|), min(|i|, |j|)) at
+        // a*(a+1)/2 + b: one float64 exp per pair that occurs, from the same d = a*a + b*b
+        const int npair = (rad + 1) * (rad + 2) / 2;
+        for (int t = lane; t < npair; t += 64) {
+            int a = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+            while ((a + 1) * (a + 2) / 2 <= t) a++;
+            while (a * (a + 1) / 2 > t) a--;
+            const int b = t - a * (a + 1) / 2;
+            s_w[t] = (float)exp((double)(-(float)(a * a + b * b) / den));
+        }
         __syncthreads();
         // samples of the keypoint's own (2 rad + 1)^2 window (gsift.py:396-399 masks the rest)
         //
@@ -800,7 +807,8 @@ __global__ __launch_bounds__(64) void k_orient(const SiftJob* __restrict__ jobs)
             dx = G[kk + 1] - G[kk - 1];
             dy = G[kk - C] - G[kk + C];
             const float xx = dx * dx, yy = dy * dy;
-            m = sqrtf(xx + yy) * s_w[i * i + j * j];
+            const int ai = abs(i), aj = abs(j), wa = max(ai, aj), wb = min(ai, aj);
+            m = sqrtf(xx + yy) * s_w[wa * (wa + 1) / 2 + wb];
             return true;
         };
         auto accumulate = [&](float af, float m) {
