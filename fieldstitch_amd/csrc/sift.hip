@@ -818,9 +818,31 @@ __global__ __launch_bounds__(64) void k_orient(const SiftJob* __restrict__ jobs)
         };
         static_assert((2 * ORI_R_MAX + 1) * (2 * ORI_R_MAX + 1) <= 64 * 32, "sample index fits the mask");
         unsigned late = 0u;
+        // one iteration of software pipelining: the four gradient taps of the next sample are
+        // loaded before this one is binned (the loads of an iteration otherwise wait alone)
+        auto taps4 = [&](int s, float4& g, int& wi) {
+            const int i = s / W - rad, j = s - (s / W) * W - rad;
+            const int y = r + i, x = c + j;
+            const bool ok = s < W * W && y > 0 && y < R - 1 && x > 0 && x < C - 1;
+            if (ok) {
+                const size_t kk = (size_t)y * C + x;
+                g = make_float4(G[kk + 1], G[kk - 1], G[kk - C], G[kk + C]);
+                const int ai = abs(i), aj = abs(j), wa = max(ai, aj), wb = min(ai, aj);
+                wi = wa * (wa + 1) / 2 + wb;
+            } else {
+                wi = -1;
+            }
+        };
+        float4 gn = make_float4(0.f, 0.f, 0.f, 0.f);
+        int wn = -1;
+        taps4(lane, gn, wn);
         for (int s = lane, it = 0; s < W * W; s += 64, ++it) {
-            float dx, dy, m;
-            if (!sample(s, dx, dy, m)) continue;
+            const float4 gc = gn;
+            const int wc = wn;
+            taps4(s + 64, gn, wn);
+            if (wc < 0) continue;
+            const float dx = gc.x - gc.y, dy = gc.z - gc.w;
+            const float m = sqrtf(dx * dx + dy * dy) * s_w[wc];
             float af = atan2f(dy, dx) * deg;
             af = af < 0.f ? af + 360.0f : af;
             const float t = af * 0.1f;
