@@ -765,8 +765,18 @@ __global__ __launch_bounds__(64) void k_orient(const SiftJob* __restrict__ jobs)
         nbuf = 0;
         __syncthreads();   // the buffer is read before it is refilled
     };
+    // the next keypoint's record is fetched while this one is processed
+    float4 na = make_float4(0.f, 0.f, 0.f, 0.f), nb = na;
+    if ((int)blockIdx.x < n) {
+        na = J.refined[2 * blockIdx.x];
+        nb = J.refined[2 * blockIdx.x + 1];
+    }
     for (int k = blockIdx.x; k < n; k += gridDim.x) {
-        const float4 a = J.refined[2 * k], b = J.refined[2 * k + 1];
+        const float4 a = na, b = nb;
+        if (k + (int)gridDim.x < n) {
+            na = J.refined[2 * (k + gridDim.x)];
+            nb = J.refined[2 * (k + gridDim.x) + 1];
+        }
         const int o = __float_as_int(a.x), l = __float_as_int(a.y), r = __float_as_int(a.z), c = __float_as_int(a.w);
         const int R = J.oct[o].R, C = J.oct[o].C;
         const float e = ((float)l + b.z) / 3.0f;
