@@ -280,7 +280,7 @@ __global__ __launch_bounds__(256) void k_f_screen(const FBatch B) {
     const FSet sa = B.set[P.a], sb = B.set[P.b];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int q = blockIdx.x * 128 + wv * 32 + (lane & 31);
-    if (blockIdx.x * 128 >= sa.n) return;                           // block-uniform
+    if ((int)blockIdx.x * 128 >= sa.n) return;                      // block-uniform
     const bool valid = q < sa.n && sa.n >= 2 && sb.n >= 2;
     const long long ka = sa.off + (q < sa.n ? q : 0);
     // query operand (this lane's half of its row) and its terms

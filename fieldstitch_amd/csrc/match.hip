@@ -1042,8 +1042,6 @@ static MatchBatch batch_ptrs(const MatchPlanHost& M, char* ws, const uint8_t* de
     return b;
 }
 
-static inline unsigned blocks_for(long long n) { return (unsigned)std::max(1LL, (n + 255) / 256); }
-
 }  // namespace fsx
 
 using namespace fsx;

@@ -100,7 +100,7 @@ __global__ __launch_bounds__(256) void k_gemm(const unsigned short* __restrict__
             const size_t o = (size_t)m * ldc + n;
             if (EPI == EPI_BIAS_RELU) C16[o] = m_f2bf(fmaxf(a + bias[n], 0.0f));
             else if (EPI == EPI_F32) C32[o] = a;
-            else C16[o] = m_f2bf(m_bf2f(mask[o]) > 0.0f ? a : 0.0f);
+            else if (EPI == EPI_MASK) C16[o] = m_f2bf(m_bf2f(mask[o]) > 0.0f ? a : 0.0f);
         }
 }
 

@@ -1145,7 +1145,7 @@ struct DescArgs {
     uint8_t* u8;          // (m, 128) or null
 };
 
-__device__ __forceinline__ float wave_pairwise8(float v_lane_j_partial, int lane) {
+__device__ __forceinline__ float wave_pairwise8(float v_lane_j_partial) {
     // lanes 0..7 hold numpy's 8 running accumulators; combine in numpy's fixed tree
     float r0 = __shfl(v_lane_j_partial, 0), r1 = __shfl(v_lane_j_partial, 1);
     float r2 = __shfl(v_lane_j_partial, 2), r3 = __shfl(v_lane_j_partial, 3);
@@ -1322,7 +1322,7 @@ __device__ __forceinline__ void describe_kp(const SiftJob& J, const float* __res
                 part += v * v;
             }
         }
-        const float nrm = sqrtf(wave_pairwise8(part, lane));
+        const float nrm = sqrtf(wave_pairwise8(part));
         const float thr = nrm * 0.2f;
         float d0 = s_d[lane], d1 = s_d[lane + 64];
         d0 = d0 < thr ? d0 : thr;
@@ -1340,7 +1340,7 @@ __device__ __forceinline__ void describe_kp(const SiftJob& J, const float* __res
                 part += v * v;
             }
         }
-        const float nrm2 = sqrtf(wave_pairwise8(part, lane));
+        const float nrm2 = sqrtf(wave_pairwise8(part));
         const float sc = 512.0f / (nrm2 > 1e-7f ? nrm2 : 1e-7f);
         const float e0 = d0 * sc, e1 = d1 * sc;
         if (desc_row) {
@@ -1358,7 +1358,7 @@ __device__ __forceinline__ void describe_kp(const SiftJob& J, const float* __res
                 part = fabsf(s_d[lane]);
                 for (int t2 = 1; t2 < 16; t2++) part += fabsf(s_d[8 * t2 + lane]);
             }
-            const float l1 = wave_pairwise8(part, lane) + 1e-7f;
+            const float l1 = wave_pairwise8(part) + 1e-7f;
             float u0 = sqrtf(e0 / l1) * 512.0f, u1 = sqrtf(e1 / l1) * 512.0f;
             u0 = u0 < 0.f ? 0.f : (u0 > 255.f ? 255.f : u0);
             u1 = u1 < 0.f ? 0.f : (u1 > 255.f ? 255.f : u1);
