@@ -7,7 +7,9 @@ detect int(8000*2.2) keypoints, grid-spread to 8000, describe, RootSIFT uint8.  
 matching: every overlapping pair, prior = true mapping + N(0, 3 px), 60 px window, top-2,
 ratio 0.8, one query per target.
 
-Prints ONE JSON line.  ``value`` = SIFT images/s with the photos resident in HBM;
+Prints ONE JSON line.  ``value`` = SIFT images/s with the photos resident in HBM, consecutive
+steps rotating over --inflight feature batches (workspaces) on as many streams, so that one
+step's latency-bound stages overlap the next step's pyramid (each step does the whole pipeline);
 ``matching`` holds matched pairs/s of the fast and the plain FP32 GPU forms and whether the
 match set equals the FP64 oracle; ``precision`` holds the float-descriptor brute-force section
 (200 pairs, BF16 and FP8 screening, distance error against float64, match-set equality);
@@ -246,7 +248,8 @@ def bench_precision(args):
 def bench_flight400(args):
     """The 400-photo flight workload: a 400-photo flight (20 x 20 grid, 75 % / 65 % overlap) with resolutions mixed
     50/50 between 2640x1978 and 1320x989.  Photos resident in HBM; SIFT in chunks of one fixed
-    shape composition through one graph-captured FeatureBatch (launch shapes autotuned first);
+    shape composition through graph-captured FeatureBatches, --inflight chunks in flight on as
+    many streams (launch shapes autotuned first);
     candidate pairs from the GPS grid on the GPU (16 nearest + temporal); guided matching of all
     pairs in one batch (MFMA and plain forms, graph replays), a sample checked against the FP64
     oracle."""
@@ -272,12 +275,15 @@ def bench_flight400(args):
         bi += CB
         si += CS
     batches = {}
-    def batch_for(idx):
-        key = tuple(shapes[i] for i in idx)
+    # batch sets in flight: chunk c runs on set c % args.inflight and its stream (inputs copied,
+    # run and results gathered there, so a set's buffers are reused in stream order)
+    streams = [torch.cuda.Stream() for _ in range(args.inflight)]
+    def batch_for(idx, slot=0):
+        key = (tuple(shapes[i] for i in idx), slot)
         if key not in batches:
-            fb = features.FeatureBatch(list(key), nfeat=NFEAT, contrast=CONTRAST, edge=EDGE)
-            bufs = [torch.empty(sh, dtype=torch.uint8, device="cuda") for sh in key]
-            sels = [fb.selection_device(s_)[:NFEAT] for s_ in range(len(key))]   # views, fixed addresses
+            fb = features.FeatureBatch(list(key[0]), nfeat=NFEAT, contrast=CONTRAST, edge=EDGE)
+            bufs = [torch.empty(sh, dtype=torch.uint8, device="cuda") for sh in key[0]]
+            sels = [fb.selection_device(s_)[:NFEAT] for s_ in range(len(key[0]))]   # views, fixed addresses
             batches[key] = (fb, bufs, sels)
         return batches[key]
     # autotune the launch shapes on the first chunk
@@ -289,20 +295,29 @@ def bench_flight400(args):
     xy_all = torch.empty((n, NFEAT, 2), dtype=torch.float64, device="cuda")
     cnt_all = torch.empty(n, dtype=torch.int32, device="cuda")
 
+    chunk_idx = [torch.tensor(idx, device="cuda") for idx in chunks]
+    main = torch.cuda.current_stream()
+
     def sift_pass():
-        for idx in chunks:
-            fb, bufs, sels = batch_for(idx)
-            for b, i in zip(bufs, idx):
-                b.copy_(dev[i])
-            fb.run(bufs, graph=True)
-            it = torch.tensor(idx, device="cuda")
-            u8_all.index_copy_(0, it, fb.u8[:, :NFEAT])
-            cnt_all.index_copy_(0, it, torch.clamp(fb.n_out, max=NFEAT))
-            # selected (x, y) of every photo of the chunk in one gather (entries past a
-            # photo's count are stale and clamped, as the matcher only reads its count)
-            sel = torch.stack(sels).clamp_(0, fb.max_kp - 1)
-            g = torch.gather(fb.kp[:, :2, :], 2, sel[:, None, :].expand(-1, 2, -1))
-            xy_all.index_copy_(0, it, g.transpose(1, 2).double())
+        for s_ in streams:
+            s_.wait_stream(main)
+        for c, idx in enumerate(chunks):
+            slot = c % args.inflight
+            fb, bufs, sels = batch_for(idx, slot)
+            with torch.cuda.stream(streams[slot]):
+                for b, i in zip(bufs, idx):
+                    b.copy_(dev[i])
+                fb.run(bufs, graph=True, stream=streams[slot])
+                it = chunk_idx[c]
+                u8_all.index_copy_(0, it, fb.u8[:, :NFEAT])
+                cnt_all.index_copy_(0, it, torch.clamp(fb.n_out, max=NFEAT))
+                # selected (x, y) of every photo of the chunk in one gather (entries past a
+                # photo's count are stale and clamped, as the matcher only reads its count)
+                sel = torch.stack(sels).clamp_(0, fb.max_kp - 1)
+                g = torch.gather(fb.kp[:, :2, :], 2, sel[:, None, :].expand(-1, 2, -1))
+                xy_all.index_copy_(0, it, g.transpose(1, 2).double())
+        for s_ in streams:
+            main.wait_stream(s_)
 
     sift_pass()                                   # warm-up (captures the graphs)
     for fb, _, _ in batches.values():
@@ -361,9 +376,10 @@ def bench_flight400(args):
         "ms_per_step": t_sift * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 (f64 blur accumulation)", "data": "synthetic (seeded field mosaic, 400 photos)",
         "config": {"workload": "400 photos, 20x20 grid 75/65 overlap, 2640x1978 and 1320x989 mixed 50/50; "
-                               "SIFT contrast 0.04 edge 10 nfeat 8000 in graph-captured chunks of %d+%d; "
-                               "guided matching of GPS-selected pairs" % (CB, CS),
-                   "photos": n, "large": len(big), "small": len(small), "pairs": len(pl)},
+                               "SIFT contrast 0.04 edge 10 nfeat 8000 in graph-captured chunks of %d+%d, %d chunks "
+                               "in flight; guided matching of GPS-selected pairs" % (CB, CS, args.inflight),
+                   "photos": n, "large": len(big), "small": len(small), "pairs": len(pl),
+                   "batches_in_flight": args.inflight},
         "sift": {"ms_per_flight": t_sift * 1e3, "algorithmic_bytes": alg,
                  "sift_roofline_frac": alg / t_sift / HBM_PEAK, "keypoints_selected_mean": float(counts.mean()),
                  "autotune": tuned["chosen"]},
@@ -474,6 +490,8 @@ def main():
     ap.add_argument("--flight400", action="store_true",
                     help="only run the 400-photo mixed-resolution flight (graph-captured SIFT chunks + guided matching)")
     ap.add_argument("--precision-pairs", type=int, default=200)
+    ap.add_argument("--inflight", type=int, default=3,
+                    help="feature batches in flight (workspaces and streams) in the timed SIFT steps")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 3 plain device copies of MIB MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -499,23 +517,42 @@ def main():
                                                                   len(fl.pairs), time.time() - t0))
     dev = [torch.from_numpy(im).cuda() for im in fl.images]
     shapes = [im.shape for im in fl.images]
-    fb = features.FeatureBatch(shapes, nfeat=NFEAT, contrast=CONTRAST, edge=EDGE)
+    # batches in flight (--inflight, 3 by default): consecutive steps rotate over that many
+    # workspaces on as many (default-priority) streams, so one step's latency-bound stages
+    # (small octaves, sort, spread, descriptors) overlap the next step's memory-bound pyramid
+    fbs = [features.FeatureBatch(shapes, nfeat=NFEAT, contrast=CONTRAST, edge=EDGE) for _ in range(args.inflight)]
+    fb = fbs[0]
+    streams = [torch.cuda.Stream() for _ in range(args.inflight)]
+    main = torch.cuda.current_stream()
+
+    def sift_steps(k_steps):
+        for s_ in streams:
+            s_.wait_stream(main)
+        for k in range(k_steps):
+            fbs[k % args.inflight].run(dev, stream=streams[k % args.inflight])
+        for s_ in streams:
+            main.wait_stream(s_)
+
     torch.cuda.synchronize()
-    for _ in range(args.warmup):
-        fb.run(dev)
-    if args.warmup:
-        fb.check_capacity()
+    sift_steps(max(args.warmup, args.inflight))
+    for f_ in fbs:
+        f_.check_capacity()
     torch.cuda.synchronize()
 
     # timed region: K steps of the SIFT front end on all photos, as a caller runs it (no stage
     # events)
     torch.cuda.synchronize()
     t_start = time.perf_counter()
-    for k in range(args.steps):
-        fb.run(dev)
+    sift_steps(args.steps)
     torch.cuda.synchronize()
     t_sift = (time.perf_counter() - t_start) / args.steps
-    fb.check_capacity()
+    for f_ in fbs:
+        f_.check_capacity()
+    # every workspace holds the same features (identical inputs): the matcher reads fbs[0]
+    for i in range(len(shapes)):
+        a_, b_ = fbs[0].results(i), fbs[-1].results(i)
+        if not (np.array_equal(a_[0], b_[0]) and np.array_equal(a_[1], b_[1])):
+            raise RuntimeError("the in-flight batches disagree on photo %d" % i)
     # stage breakdown (and the pyramid time of the roofline) from K more steps with per-stage
     # events
     evs = [[torch.cuda.Event(enable_timing=True) for _ in range(8)] for _ in range(args.steps)]
@@ -552,9 +589,10 @@ def main():
         "data": "synthetic (seeded field mosaic, 8 photos)",
         "config": {"workload": "8 photos 2640x1978 u8, 2x4 flight grid 75/65 overlap; SIFT contrast 0.04 edge 10, "
                                "nfeat 8000 (detect 17600), RootSIFT u8",
-                   "photos": n_img, "nfeat": NFEAT, "pairs": len(fl.pairs)},
+                   "photos": n_img, "nfeat": NFEAT, "pairs": len(fl.pairs), "batches_in_flight": args.inflight},
         "sift": {"stage_ms": {k: round(v, 4) for k, v in stages.items()},
-                 "stage_ms_note": "separate runs with per-stage events",
+                 "stage_ms_note": "separate runs with per-stage events, one batch in flight (their sum is "
+                                  "serial_ms_per_step; ms_per_step has batches_in_flight of them)",
                  "serial_ms_per_step": round(sum(stages.values()), 4),
                  "keypoints_detected_mean": float(n_det.mean()), "keypoints_selected_mean": float(n_sel.mean()),
                  "algorithmic_bytes_per_step": sift_b,
