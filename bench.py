@@ -268,7 +268,7 @@ def bench_flight400(args):
     dev = [torch.from_numpy(im).cuda() for im in fl.images]
     big = [i for i in range(n) if shapes[i] == (1978, 2640)]
     small = [i for i in range(n) if shapes[i] != (1978, 2640)]
-    CB, CS = 25, 25
+    CB = CS = args.flight_chunk
     chunks, bi, si = [], 0, 0
     while bi < len(big) or si < len(small):
         chunks.append(big[bi:bi + CB] + small[si:si + CS])
@@ -490,6 +490,8 @@ def main():
     ap.add_argument("--flight400", action="store_true",
                     help="only run the 400-photo mixed-resolution flight (graph-captured SIFT chunks + guided matching)")
     ap.add_argument("--precision-pairs", type=int, default=200)
+    ap.add_argument("--flight-chunk", type=int, default=25,
+                    help="--flight400: photos of each resolution per feature batch (chunk)")
     ap.add_argument("--inflight", type=int, default=3,
                     help="feature batches in flight (workspaces and streams) in the timed SIFT steps")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
