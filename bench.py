@@ -497,6 +497,8 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 3 plain device copies of MIB MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.inflight < 1 or args.flight_chunk < 1:
+        ap.error("--inflight and --flight-chunk must be >= 1")
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
