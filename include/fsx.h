@@ -49,9 +49,10 @@ int fsx_sift_plan(int H, int W, int *n_oct, int *rows, int *cols);
  * _Kernels).  out: >= 64 floats. */
 int fsx_sift_taps(int layer, float *out, int *len);
 
-/* Bytes of the per-image device workspace (the reference's _Atlas, gsift.py:227-271, plus
- * keypoint buffers).  cand_cap: extrema candidate capacity; kp_cap: oriented keypoint
- * capacity, a power of two >= 1024. */
+/* Bytes of the per-image device workspace (the reference's _Atlas, gsift.py:227-271, as the
+ * six Gaussian layers of every octave: its DoG layers are their differences, formed by the
+ * kernels that read them; plus keypoint buffers).  cand_cap: extrema candidate capacity;
+ * kp_cap: oriented keypoint capacity, a power of two >= 1024. */
 int fsx_sift_workspace_bytes(int H, int W, int cand_cap, int kp_cap, size_t *bytes);
 
 /* gsift.detect (gsift.py:435-477): pyramid, extrema, refinement, orientation and the
