@@ -90,7 +90,20 @@ def cpu_baseline(images, n_proc):
     wall = time.perf_counter() - t0
     return {"value": len(sample) / wall, "unit": "images/s", "cores": len(sample), "kind": "port",
             "sample": "%d of the 2640x1978 photos, one per process (C oracle SIFT + spread + describe + "
-                      "RootSIFT), wall %.1f s, per-photo CPU %.1f s" % (len(sample), wall, float(np.mean(per)))}
+                      "RootSIFT), wall %.1f s, per-photo CPU %.1f s" % (len(sample), wall, float(np.mean(per))),
+            "host_cpus_usable": len(os.sched_getaffinity(0)), "cpu_model": cpu_model()}
+
+
+def cpu_model():
+    """The host CPU's model name (first 'model name' of /proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 RADIUS, RATIO = 60.0, 0.8
@@ -147,12 +160,14 @@ def bench_matching(fb, fl, shapes, args):
     equal_forms = all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
                       for a, b in zip(results["mfma"], results["plain"]))
     n_equal, n_matches, cand = 0, 0, 0
+    t_cpu = time.perf_counter()
     for p, (i, j) in enumerate(pairs):
         q, t = mo.guided_fp64_fast(host[i][0], host[i][1], host[j][0], host[j][1], Hm[p], (shapes[j][1], shapes[j][0]),
                                    RADIUS, RATIO)
         got = results["mfma"][p]
         n_equal += int(np.array_equal(got[0], q) and np.array_equal(got[1], t))
         n_matches += len(q)
+    t_cpu = time.perf_counter() - t_cpu
     # algorithmic contraction work: every query against the targets inside its prior window
     # (128 multiply-adds each); the MFMA form computes dense 32 x 32 blocks over whole grid
     # cells, so the MFMA work it executes is larger than this
@@ -170,6 +185,15 @@ def bench_matching(fb, fl, shapes, args):
     out["mfma_roofline_frac"] = ops / t_m / INT8_PEAK
     out["mfma_roofline_note"] = ("algorithmic ops = 2 * 128 * (targets inside each query's window) over the "
                                  "MFMA form's graph-replay time, against the dense INT8 peak")
+    # compulsory bytes: every descriptor of the batch read once (u8 rows)
+    comp = 128.0 * sum(len(h[0]) for h in host)
+    out["compulsory_bytes"] = comp
+    out["compulsory_hbm_frac"] = comp / t_m / HBM_PEAK
+    # the CPU oracle's FP64 brute-force guided matching of the same pairs, one process (it also
+    # decides the exactness flag above)
+    out["cpu_baseline"] = {"value": len(pairs) / t_cpu, "unit": "pairs/s", "cores": 1, "kind": "port",
+                           "sample": "the %d pairs, FP64 numpy guided matching (oracle.match_oracle."
+                                     "guided_fp64_fast), %.2f s" % (len(pairs), t_cpu)}
     out["match_set_equals_fp64_oracle"] = bool(n_equal == len(pairs) and equal_forms)
     out["pairs_equal_fp64"] = n_equal
     out["matches_total"] = n_matches
@@ -387,7 +411,10 @@ def bench_flight400(args):
         "matching": {"pairs": len(pl), **out_m, "speedup_vs_plain_fp32":
                      out_m["plain"]["ms_per_flight"] / out_m["mfma"]["ms_per_flight"],
                      "forms_equal": bool(same_forms), "fp64_oracle_sample": int(len(sample)),
-                     "fp64_oracle_equal": int(ok)},
+                     "fp64_oracle_equal": int(ok),
+                     "compulsory_bytes": 128.0 * float(counts.sum()),
+                     "compulsory_hbm_frac": 128.0 * float(counts.sum()) / (out_m["mfma"]["ms_per_flight"] / 1e3)
+                                            / HBM_PEAK},
     }
 
 
