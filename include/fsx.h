@@ -352,6 +352,24 @@ int fsx_ground_predict(int n, const double *u_dev, const double *cams, double *p
 int fsx_pair_select(int n, const double *xy_dev, const double *half_long_dev, int k, int has_rel, int *out_dev,
                     void *stream);
 
+/* Feature tracks from verified match lists (recon.py:14-56, build_tracks(pairs, nfeat, n_img)).
+ * Workspace bytes and the observation capacity min(n_nodes, 2 * n_edges) for n_nodes features
+ * in all (n_nodes < 2^31) and n_edges matches. */
+int fsx_tracks_workspace_bytes(long long n_nodes, long long n_edges, size_t *bytes, long long *obs_capacity);
+
+/* offsets_dev: n_img + 1 int64, the exclusive cumulative feature counts (offsets[n_img] =
+ * n_nodes).  Pair p matches image pair_ij_dev[2p] to pair_ij_dev[2p+1]; its matches are
+ * q_dev[k], t_dev[k] (int32 feature indices) for pair_start_dev[p] <= k < pair_start_dev[p+1]
+ * (int64, n_pair + 1 entries, last = n_edges).  Writes the observations sorted by track, then
+ * node: tid_dev / img_dev / feat_dev (int64, out_capacity >= the workspace call's
+ * obs_capacity), and result_dev (4 int64): [n_observations, n_tracks, index error, capacity
+ * error].  An image or feature index out of range sets the index error; the outputs are then
+ * undefined.  Asynchronous; deterministic. */
+int fsx_build_tracks(int n_img, const long long *offsets_dev, long long n_nodes, int n_pair, const int *pair_ij_dev,
+                     const long long *pair_start_dev, const int *q_dev, const int *t_dev, long long n_edges,
+                     void *ws_dev, size_t ws_bytes, long long out_capacity, long long *tid_dev, long long *img_dev,
+                     long long *feat_dev, long long *result_dev, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
