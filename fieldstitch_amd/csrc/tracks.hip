@@ -3,24 +3,25 @@
 // Nodes are the features of all photos, numbered image by image (node = offsets[img] + feature,
 // offsets = exclusive cumulative sum of the per-photo feature counts).  Each verified match
 // (i, j, q, t) is an undirected edge offsets[i] + q -- offsets[j] + t.  A track is a connected
-// component of the used nodes with at least 2 members and no two members in the same photo
+// component of the matched nodes with at least 2 members and no two members in the same photo
 // (recon.py:37-48); tracks are numbered in increasing order of their smallest node, and the
 // observations are listed track by track, nodes in increasing order (recon.py:49-56; scipy's
 // connected_components labels components in order of their smallest node, and build_tracks
 // keeps the node order inside a component).
 //
-// Kernels, one pass each over the nodes or the edges:
-//   k_tr_init      parent[x] = x, counters and flags cleared, the (root, image) table emptied
-//   k_tr_union     one block per pair: lock-free union-find, the larger root is linked under
-//                  the smaller by compare-and-swap, so every root is its component's minimum
-//   k_tr_compress  parent[x] = root of x
-//   k_tr_mark      per used node: component size += 1; (root, image) inserted into an open-
-//                  addressing table, an insert that finds its key already present marks the
-//                  component bad (two features of one photo)
-//   k_tr_scan_*    exclusive scans of (kept root ? 1 : 0) and (kept root ? size : 0): the track
-//                  number and the first output slot of each kept component
-//   k_tr_scatter   used node -> a slot of its track (atomic cursor per track)
-//   k_tr_emit      per kept track: insertion sort of its slots by node, then tid / img / feat
+// Kernels:
+//   k_tr_init         parent[x] = x, counters cleared
+//   k_tr_union        one block per pair: lock-free union-find, the larger root is linked under
+//                     the smaller by compare-and-swap, so every root is its component's minimum
+//   k_tr_compress     parent[x] = root of x; the root counts its matched members
+//   k_tr_scan_*<0>    over nodes: components with >= 2 matched members ("candidates") get a
+//                     number and a run of slots
+//   k_tr_scatter      matched node -> a slot of its candidate (atomic cursor per candidate)
+//   k_tr_rank         one thread per slot: the member's rank (members below it) places it in
+//                     node order; a second member in its photo's node range, or more members
+//                     than photos, clears the candidate
+//   k_tr_scan_*<1>    over candidates: tracks (no repeated photo) get a number and an output run
+//   k_tr_emit         one thread per slot: tid / img / feat of the kept ones, in slot order
 #include "fsx_common.h"
 
 namespace {
@@ -29,55 +30,57 @@ constexpr int TR_THREADS = 256;
 constexpr int TR_SCAN_PER_THREAD = 8;
 constexpr int TR_SCAN_CHUNK = TR_THREADS * TR_SCAN_PER_THREAD;
 constexpr int TR_TOP_THREADS = 1024;
-constexpr unsigned long long TR_EMPTY = ~0ull;
+constexpr int TR_OFF_LDS = 2048;      // photo offsets staged in LDS up to this many entries
 
 // result slots
 constexpr int TR_N_OBS = 0, TR_N_TRACKS = 1, TR_ERR_INDEX = 2, TR_ERR_CAP = 3;
+// internal counters
+constexpr int TR_N_CAND = 0, TR_N_SLOTS = 1;
 
 struct TrWs {
     int* parent;        // [n]
-    int* size;          // [n] used members of the component (at its root)
-    int* cursor;        // [n] scatter cursor per track (at its root)
-    int* track;         // [n] track number (at kept roots)
-    int* first;         // [n] first output slot (at kept roots)
-    uint8_t* used;      // [n]
-    uint8_t* bad;       // [n] component holds two features of one photo (at its root)
-    int* slots;         // [obs_cap] node per output slot
-    unsigned long long* table;   // [table_n] (root << 32 | image), TR_EMPTY when free
-    int2* block_sums;   // [n_blocks] per scan chunk
-    long long table_n, obs_cap, n_blocks;
+    int* size;          // [n] matched members of the component (at its root)
+    int* cursor;        // [n] scatter cursor (at candidate roots)
+    int* comp;          // [n] candidate number (at candidate roots)
+    int* first;         // [n] first slot (at candidate roots)
+    uint8_t* used;      // [n] node has a match
+    int* slots;         // [obs_cap] member nodes, candidate by candidate
+    int* tmp;           // [obs_cap] members in node order, candidate by candidate
+    int* cand_root;     // [cand_cap]
+    int* cand_ok;       // [cand_cap] 1 = no photo twice
+    int* cand_track;    // [cand_cap] track number (kept candidates)
+    int* cand_out;      // [cand_cap] first output position (kept candidates)
+    int2* block_sums;   // [max(nb_nodes, nb_cand)]
+    unsigned long long* counts;  // [4]
+    long long obs_cap, cand_cap, nb_nodes, nb_cand;
 };
 
 inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
-long long pow2_at_least(long long x) {
-    long long p = 64;
-    while (p < x) p <<= 1;
-    return p;
-}
-
-// Byte offsets of the arrays inside the workspace for n nodes and `obs_cap` possible
-// observations (min(n, 2 * edges)).
+// Layout of the workspace for n nodes and obs_cap = min(n, 2 * edges) possible observations
+// (each candidate has >= 2 of them, so at most obs_cap / 2 candidates).
 size_t tr_layout(long long n, long long obs_cap, TrWs* w, char* base) {
-    const long long table_n = pow2_at_least(2 * (obs_cap > 0 ? obs_cap : 1));
-    const long long n_blocks = (n + TR_SCAN_CHUNK - 1) / TR_SCAN_CHUNK;
+    const long long oc = obs_cap > 0 ? obs_cap : 1;
+    const long long cc = obs_cap / 2 > 0 ? obs_cap / 2 : 1;
+    const long long nb_nodes = (n + TR_SCAN_CHUNK - 1) / TR_SCAN_CHUNK;
+    const long long nb_cand = (cc + TR_SCAN_CHUNK - 1) / TR_SCAN_CHUNK;
+    const long long nb = nb_nodes > nb_cand ? nb_nodes : nb_cand;
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += align256(bytes); return base ? base + o : nullptr; };
-    char* p_parent = take(sizeof(int) * n);
-    char* p_size = take(sizeof(int) * n);
-    char* p_cursor = take(sizeof(int) * n);
-    char* p_track = take(sizeof(int) * n);
-    char* p_first = take(sizeof(int) * n);
-    char* p_used = take(n);
-    char* p_bad = take(n);
-    char* p_slots = take(sizeof(int) * (obs_cap > 0 ? obs_cap : 1));
-    char* p_table = take(sizeof(unsigned long long) * table_n);
-    char* p_bsum = take(sizeof(int2) * (n_blocks > 0 ? n_blocks : 1));
+    char* p[14];
+    for (int k = 0; k < 5; ++k) p[k] = take(sizeof(int) * n);
+    p[5] = take(n);
+    p[6] = take(sizeof(int) * oc);
+    p[7] = take(sizeof(int) * oc);
+    for (int k = 8; k < 12; ++k) p[k] = take(sizeof(int) * cc);
+    p[12] = take(sizeof(int2) * (nb > 0 ? nb : 1));
+    p[13] = take(sizeof(unsigned long long) * 4);
     if (w) {
-        w->parent = (int*)p_parent; w->size = (int*)p_size; w->cursor = (int*)p_cursor;
-        w->track = (int*)p_track; w->first = (int*)p_first; w->used = (uint8_t*)p_used;
-        w->bad = (uint8_t*)p_bad; w->slots = (int*)p_slots; w->table = (unsigned long long*)p_table;
-        w->block_sums = (int2*)p_bsum; w->table_n = table_n; w->obs_cap = obs_cap; w->n_blocks = n_blocks;
+        w->parent = (int*)p[0]; w->size = (int*)p[1]; w->cursor = (int*)p[2]; w->comp = (int*)p[3];
+        w->first = (int*)p[4]; w->used = (uint8_t*)p[5]; w->slots = (int*)p[6]; w->tmp = (int*)p[7];
+        w->cand_root = (int*)p[8]; w->cand_ok = (int*)p[9]; w->cand_track = (int*)p[10]; w->cand_out = (int*)p[11];
+        w->block_sums = (int2*)p[12]; w->counts = (unsigned long long*)p[13];
+        w->obs_cap = oc; w->cand_cap = cc; w->nb_nodes = nb_nodes; w->nb_cand = nb_cand;
     }
     return off;
 }
@@ -116,11 +119,11 @@ __global__ void k_tr_init(TrWs w, long long n, long long* result) {
         w.size[x] = 0;
         w.cursor[x] = 0;
         w.used[x] = 0;
-        w.bad[x] = 0;
     }
-    for (long long h = (long long)blockIdx.x * blockDim.x + threadIdx.x; h < w.table_n; h += stride)
-        w.table[h] = TR_EMPTY;
-    if (blockIdx.x == 0 && threadIdx.x < 4) result[threadIdx.x] = 0;
+    if (blockIdx.x == 0 && threadIdx.x < 4) {
+        result[threadIdx.x] = 0;
+        w.counts[threadIdx.x] = 0;
+    }
 }
 
 __global__ void __launch_bounds__(TR_THREADS) k_tr_union(TrWs w, const long long* offsets, int n_img,
@@ -165,42 +168,34 @@ __global__ void __launch_bounds__(TR_THREADS) k_tr_union(TrWs w, const long long
 
 // After the unions: every thread stores only roots, so no parent is left pointing below its
 // root when the kernel ends (path halving here could store a non-root ancestor after the
-// node's own thread stored its root).
+// node's own thread stored its root).  The root counts the matched members.
 __global__ void k_tr_compress(TrWs w, long long n) {
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += stride) {
         int r = (int)x, p = ld_parent(w.parent, r);
         while (p != r) { r = p; p = ld_parent(w.parent, r); }
         __hip_atomic_store(w.parent + x, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (w.used[x]) atomicAdd(w.size + r, 1);
     }
 }
 
-__device__ __forceinline__ unsigned long long tr_hash(unsigned long long key) {
-    return (key * 0x9E3779B97F4A7C15ull) ^ (key >> 29);
+// Candidates to scan in the second pass: none when a capacity was exceeded (the sort and
+// scatter kernels then skipped their work and the host reports the capacity error).
+__device__ __forceinline__ long long tr_live_candidates(const TrWs& w) {
+    const unsigned long long nc = w.counts[TR_N_CAND], ns = w.counts[TR_N_SLOTS];
+    return (nc <= (unsigned long long)w.cand_cap && ns <= (unsigned long long)w.obs_cap) ? (long long)nc : 0;
 }
 
-__global__ void k_tr_mark(TrWs w, long long n, const long long* offsets, int n_img) {
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    const unsigned long long mask = (unsigned long long)w.table_n - 1;
-    for (long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += stride) {
-        if (!w.used[x]) continue;
-        const int r = w.parent[x];
-        atomicAdd(w.size + r, 1);
-        const unsigned long long key = ((unsigned long long)r << 32) | (unsigned)tr_image(offsets, n_img, x);
-        unsigned long long h = tr_hash(key) & mask;
-        // load factor <= 1/2 and each node inserts once: the probe ends at a free slot or its key
-        while (true) {
-            const unsigned long long old = atomicCAS(w.table + h, TR_EMPTY, key);
-            if (old == TR_EMPTY) break;
-            if (old == key) { w.bad[r] = 1; break; }
-            h = (h + 1) & mask;
-        }
+// Scan inputs: MODE 0 over nodes (candidate roots: 1, members), MODE 1 over candidates
+// (kept tracks: 1, members).
+template <int MODE>
+__device__ __forceinline__ int2 tr_flags(const TrWs& w, long long x, long long n_cand) {
+    if (MODE == 0) {
+        const int s = w.size[x];
+        return (w.parent[x] == (int)x && s >= 2) ? make_int2(1, s) : make_int2(0, 0);
     }
-}
-
-__device__ __forceinline__ int2 tr_flags(const TrWs& w, long long x) {
-    const bool kept = w.parent[x] == (int)x && w.size[x] >= 2 && !w.bad[x];
-    return kept ? make_int2(1, w.size[x]) : make_int2(0, 0);
+    if (x >= n_cand || !w.cand_ok[x]) return make_int2(0, 0);
+    return make_int2(1, w.size[w.cand_root[x]]);
 }
 
 // Block-wide exclusive scan of one int2 per thread; returns the prefix, `total` gets the sum.
@@ -224,44 +219,54 @@ __device__ int2 tr_block_scan(int2 v, int2* s_wave, int2& total) {
     return make_int2(base.x + inc.x - v.x, base.y + inc.y - v.y);
 }
 
+template <int MODE>
 __global__ void __launch_bounds__(TR_THREADS) k_tr_scan_blocks(TrWs w, long long n) {
     __shared__ int2 s_wave[TR_THREADS / 64];
+    const long long n_cand = MODE ? tr_live_candidates(w) : 0;
     const long long b0 = (long long)blockIdx.x * TR_SCAN_CHUNK + (long long)threadIdx.x * TR_SCAN_PER_THREAD;
     int2 sum = make_int2(0, 0);
     for (int k = 0; k < TR_SCAN_PER_THREAD; ++k) {
-        if (b0 + k < n) { const int2 f = tr_flags(w, b0 + k); sum.x += f.x; sum.y += f.y; }
+        if (b0 + k < n) { const int2 f = tr_flags<MODE>(w, b0 + k, n_cand); sum.x += f.x; sum.y += f.y; }
     }
     int2 total;
     tr_block_scan(sum, s_wave, total);
     if (threadIdx.x == 0) w.block_sums[blockIdx.x] = total;
 }
 
-__global__ void __launch_bounds__(TR_TOP_THREADS) k_tr_scan_top(TrWs w, long long* result, long long obs_cap) {
+template <int MODE>
+__global__ void __launch_bounds__(TR_TOP_THREADS) k_tr_scan_top(TrWs w, long long nb, long long* result) {
     __shared__ int2 s_wave[TR_TOP_THREADS / 64];
     int2 carry = make_int2(0, 0);
-    for (long long c = 0; c < w.n_blocks; c += TR_TOP_THREADS) {
+    for (long long c = 0; c < nb; c += TR_TOP_THREADS) {
         const long long b = c + threadIdx.x;
-        const int2 v = b < w.n_blocks ? w.block_sums[b] : make_int2(0, 0);
+        const int2 v = b < nb ? w.block_sums[b] : make_int2(0, 0);
         int2 total;
         const int2 ex = tr_block_scan(v, s_wave, total);
-        if (b < w.n_blocks) w.block_sums[b] = make_int2(carry.x + ex.x, carry.y + ex.y);
+        if (b < nb) w.block_sums[b] = make_int2(carry.x + ex.x, carry.y + ex.y);
         carry.x += total.x;
         carry.y += total.y;
     }
     if (threadIdx.x == 0) {
-        result[TR_N_OBS] = carry.y;
-        result[TR_N_TRACKS] = carry.x;
-        if ((long long)carry.y > obs_cap) result[TR_ERR_CAP] = 1;
+        if (MODE == 0) {
+            w.counts[TR_N_CAND] = (unsigned long long)carry.x;
+            w.counts[TR_N_SLOTS] = (unsigned long long)carry.y;
+            if ((long long)carry.y > w.obs_cap || (long long)carry.x > w.cand_cap) result[TR_ERR_CAP] = 1;
+        } else {
+            result[TR_N_OBS] = carry.y;
+            result[TR_N_TRACKS] = carry.x;
+        }
     }
 }
 
+template <int MODE>
 __global__ void __launch_bounds__(TR_THREADS) k_tr_scan_apply(TrWs w, long long n) {
     __shared__ int2 s_wave[TR_THREADS / 64];
+    const long long n_cand = MODE ? tr_live_candidates(w) : 0;
     const long long b0 = (long long)blockIdx.x * TR_SCAN_CHUNK + (long long)threadIdx.x * TR_SCAN_PER_THREAD;
     int2 f[TR_SCAN_PER_THREAD];
     int2 sum = make_int2(0, 0);
     for (int k = 0; k < TR_SCAN_PER_THREAD; ++k) {
-        f[k] = b0 + k < n ? tr_flags(w, b0 + k) : make_int2(0, 0);
+        f[k] = b0 + k < n ? tr_flags<MODE>(w, b0 + k, n_cand) : make_int2(0, 0);
         sum.x += f[k].x;
         sum.y += f[k].y;
     }
@@ -271,46 +276,93 @@ __global__ void __launch_bounds__(TR_THREADS) k_tr_scan_apply(TrWs w, long long 
     ex.x += base.x;
     ex.y += base.y;
     for (int k = 0; k < TR_SCAN_PER_THREAD; ++k) {
-        if (f[k].x) { w.track[b0 + k] = ex.x; w.first[b0 + k] = ex.y; }
+        if (f[k].x) {
+            if (MODE == 0) {
+                if (ex.x < w.cand_cap) {
+                    w.comp[b0 + k] = ex.x;
+                    w.first[b0 + k] = ex.y;
+                    w.cand_root[ex.x] = (int)(b0 + k);
+                    w.cand_ok[ex.x] = 1;
+                }
+            } else {
+                w.cand_track[b0 + k] = ex.x;
+                w.cand_out[b0 + k] = ex.y;
+            }
+        }
         ex.x += f[k].x;
         ex.y += f[k].y;
     }
 }
 
 __global__ void k_tr_scatter(TrWs w, long long n) {
+    if (w.counts[TR_N_SLOTS] > (unsigned long long)w.obs_cap || w.counts[TR_N_CAND] > (unsigned long long)w.cand_cap)
+        return;
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += stride) {
         if (!w.used[x]) continue;
         const int r = w.parent[x];
-        if (!tr_flags(w, r).x) continue;
-        const long long slot = (long long)w.first[r] + atomicAdd(w.cursor + r, 1);
-        if (slot < w.obs_cap) w.slots[slot] = (int)x;
+        if (w.size[r] < 2) continue;
+        w.slots[w.first[r] + atomicAdd(w.cursor + r, 1)] = (int)x;
     }
 }
 
-__global__ void k_tr_emit(TrWs w, long long n, const long long* offsets, int n_img, long long* tid,
-                          long long* img, long long* feat) {
+// Photo offsets in LDS when they fit, else the global table.
+__device__ __forceinline__ const long long* tr_stage_offsets(long long* s_off, const long long* offsets, int n_img) {
+    if (n_img + 1 > TR_OFF_LDS) return offsets;
+    for (int k = threadIdx.x; k <= n_img; k += blockDim.x) s_off[k] = offsets[k];
+    __syncthreads();
+    return s_off;
+}
+
+// One thread per slot.  Its member x gets rank = members below x and counts the members in
+// x's photo's node range; the sorted run goes to tmp, and a second member in one photo clears
+// the candidate.  A candidate with more members than photos repeats a photo and is skipped.
+__global__ void __launch_bounds__(TR_THREADS) k_tr_rank(TrWs w, const long long* offsets, int n_img) {
+    __shared__ long long s_off[TR_OFF_LDS];
+    const long long* off = tr_stage_offsets(s_off, offsets, n_img);
+    const long long n_slots = (long long)w.counts[TR_N_SLOTS];
+    if (n_slots > w.obs_cap || w.counts[TR_N_CAND] > (unsigned long long)w.cand_cap) return;
     const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
-        const int2 f = tr_flags(w, r);
-        if (!f.x) continue;
-        const long long s0 = w.first[r], len = f.y;
-        if (s0 + len > w.obs_cap) continue;
-        int* s = w.slots + s0;
-        for (long long a = 1; a < len; ++a) {      // tracks hold at most one feature per photo
-            const int v = s[a];
-            long long b = a - 1;
-            while (b >= 0 && s[b] > v) { s[b + 1] = s[b]; --b; }
-            s[b + 1] = v;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n_slots; k += stride) {
+        const int x = w.slots[k];
+        const int r = w.parent[x];
+        const int len = w.size[r];
+        const int c = w.comp[r];
+        if (len > n_img) {
+            w.cand_ok[c] = 0;
+            continue;
         }
-        const long long tr = w.track[r];
-        for (long long a = 0; a < len; ++a) {
-            const long long x = s[a];
-            const int im = tr_image(offsets, n_img, x);
-            tid[s0 + a] = tr;
-            img[s0 + a] = im;
-            feat[s0 + a] = x - offsets[im];
+        const int* seg = w.slots + w.first[r];
+        const int im = tr_image(off, n_img, x);
+        const long long lo = off[im], hi = off[im + 1];
+        int below = 0, same = 0;
+        for (int b = 0; b < len; ++b) {
+            const int u = seg[b];
+            below += u < x;
+            same += (u >= lo) & (u < hi);
         }
+        w.tmp[w.first[r] + below] = x;      // node ids are distinct: ranks are a permutation
+        if (same > 1) w.cand_ok[c] = 0;
+    }
+}
+
+__global__ void __launch_bounds__(TR_THREADS) k_tr_emit(TrWs w, const long long* offsets, int n_img, long long* tid,
+                                                        long long* img, long long* feat) {
+    __shared__ long long s_off[TR_OFF_LDS];
+    const long long* off = tr_stage_offsets(s_off, offsets, n_img);
+    const long long n_slots = (long long)w.counts[TR_N_SLOTS];
+    if (n_slots > w.obs_cap || w.counts[TR_N_CAND] > (unsigned long long)w.cand_cap) return;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n_slots; k += stride) {
+        const int r = w.parent[w.slots[k]];
+        const int c = w.comp[r];
+        if (!w.cand_ok[c]) continue;
+        const int x = w.tmp[k];              // k's place in the sorted run of candidate c
+        const long long pos = (long long)w.cand_out[c] + (k - w.first[r]);
+        const int im = tr_image(off, n_img, x);
+        tid[pos] = w.cand_track[c];
+        img[pos] = im;
+        feat[pos] = x - off[im];
     }
 }
 
@@ -348,8 +400,7 @@ int fsx_build_tracks(int n_img, const long long* offsets_dev, long long n_nodes,
     const size_t need = tr_layout(n_nodes, cap, &w, (char*)ws_dev);
     FSX_REQUIRE(ws_bytes >= need, "fsx_build_tracks: workspace %zu < %zu bytes", ws_bytes, need);
     hipStream_t st = (hipStream_t)stream;
-    const long long init_n = n_nodes > w.table_n ? n_nodes : w.table_n;
-    hipLaunchKernelGGL(k_tr_init, dim3(grid_for(init_n)), dim3(TR_THREADS), 0, st, w, n_nodes, result_dev);
+    hipLaunchKernelGGL(k_tr_init, dim3(grid_for(n_nodes)), dim3(TR_THREADS), 0, st, w, n_nodes, result_dev);
     FSX_LAUNCH_CHECK();
     if (n_pair > 0) {
         hipLaunchKernelGGL(k_tr_union, dim3(n_pair), dim3(TR_THREADS), 0, st, w, offsets_dev, n_img, pair_ij_dev,
@@ -359,18 +410,24 @@ int fsx_build_tracks(int n_img, const long long* offsets_dev, long long n_nodes,
     if (n_nodes == 0) return FSX_OK;
     hipLaunchKernelGGL(k_tr_compress, dim3(grid_for(n_nodes)), dim3(TR_THREADS), 0, st, w, n_nodes);
     FSX_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_tr_mark, dim3(grid_for(n_nodes)), dim3(TR_THREADS), 0, st, w, n_nodes, offsets_dev, n_img);
+    hipLaunchKernelGGL(k_tr_scan_blocks<0>, dim3((unsigned)w.nb_nodes), dim3(TR_THREADS), 0, st, w, n_nodes);
     FSX_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_tr_scan_blocks, dim3((unsigned)w.n_blocks), dim3(TR_THREADS), 0, st, w, n_nodes);
+    hipLaunchKernelGGL(k_tr_scan_top<0>, dim3(1), dim3(TR_TOP_THREADS), 0, st, w, w.nb_nodes, result_dev);
     FSX_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_tr_scan_top, dim3(1), dim3(TR_TOP_THREADS), 0, st, w, result_dev, cap);
-    FSX_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_tr_scan_apply, dim3((unsigned)w.n_blocks), dim3(TR_THREADS), 0, st, w, n_nodes);
+    hipLaunchKernelGGL(k_tr_scan_apply<0>, dim3((unsigned)w.nb_nodes), dim3(TR_THREADS), 0, st, w, n_nodes);
     FSX_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_tr_scatter, dim3(grid_for(n_nodes)), dim3(TR_THREADS), 0, st, w, n_nodes);
     FSX_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_tr_emit, dim3(grid_for(n_nodes)), dim3(TR_THREADS), 0, st, w, n_nodes, offsets_dev, n_img,
-                       tid_dev, img_dev, feat_dev);
+    hipLaunchKernelGGL(k_tr_rank, dim3(grid_for(w.obs_cap)), dim3(TR_THREADS), 0, st, w, offsets_dev, n_img);
+    FSX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_tr_scan_blocks<1>, dim3((unsigned)w.nb_cand), dim3(TR_THREADS), 0, st, w, w.cand_cap);
+    FSX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_tr_scan_top<1>, dim3(1), dim3(TR_TOP_THREADS), 0, st, w, w.nb_cand, result_dev);
+    FSX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_tr_scan_apply<1>, dim3((unsigned)w.nb_cand), dim3(TR_THREADS), 0, st, w, w.cand_cap);
+    FSX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_tr_emit, dim3(grid_for(w.obs_cap)), dim3(TR_THREADS), 0, st, w, offsets_dev, n_img, tid_dev,
+                       img_dev, feat_dev);
     FSX_LAUNCH_CHECK();
     return FSX_OK;
 }

@@ -30,7 +30,7 @@ def _random_case(seed, n_img, max_feat, n_pairs, n_match):
 @pytest.mark.parametrize("seed", range(12))
 def test_random_cases_match_oracle(gpu, seed):
     from fieldstitch_amd import tracks
-    shape = [(6, 12, 10, 8), (30, 200, 120, 150), (80, 3000, 400, 2000)][seed % 3]
+    shape = [(6, 12, 10, 8), (30, 200, 120, 150), (80, 3000, 400, 2000), (50, 500, 200, 30)][seed % 4]
     pairs, nfeat = _random_case(seed, *shape)
     print("tracks random seed=%d shape=%s" % (seed, shape), flush=True)
     g = tracks.build_tracks(pairs, nfeat, len(nfeat))
@@ -77,3 +77,32 @@ def test_index_errors_raise(gpu):
     # the builder is usable after an error
     r = tracks.build_tracks([(0, 1, np.array([0]), np.array([2]))], [3, 3], 2)
     assert r[3] == 1 and r[2].tolist() == [0, 2]
+
+
+def test_long_tracks_and_many_photos(gpu):
+    """Long tracks (a 150-photo chain kept, a 120-photo chain with a repeated photo dropped),
+    components with more members than photos, and more photos than the kernels stage in LDS
+    (3000 photos)."""
+    from fieldstitch_amd import tracks
+    rng = np.random.default_rng(5)
+    n_img = 200
+    nfeat = np.full(n_img, 10, np.int64)
+    order = rng.permutation(n_img)
+    pairs = []
+    for a, b in zip(order[:149], order[1:150]):          # one feature per photo, 150 photos
+        pairs.append((int(a), int(b), np.array([3]), np.array([3])))
+    chain = rng.permutation(n_img)[:120]
+    for a, b in zip(chain[:-1], chain[1:]):
+        pairs.append((int(a), int(b), np.array([7]), np.array([7])))
+    pairs.append((int(chain[0]), int(chain[0]), np.array([7]), np.array([8])))   # photo repeats
+    pairs.append((0, 1, np.array([0, 1]), np.array([0, 1])))
+    print("tracks long chains", flush=True)
+    g = tracks.build_tracks(pairs, nfeat, n_img)
+    ref = O.build_tracks(pairs, nfeat, n_img)
+    assert _eq(g, ref) and np.bincount(g[0]).max() == 150
+    print("tracks 3000 photos", flush=True)
+    pairs, nfeat = _random_case(9, 3000, 6, 6000, 4)
+    assert _eq(tracks.build_tracks(pairs, nfeat, 3000), O.build_tracks(pairs, nfeat, 3000))
+    # one dense cluster: more members than photos
+    pairs = [(i % 4, (i + 1) % 4, np.arange(6), np.roll(np.arange(6), 1)) for i in range(4)]
+    assert _eq(tracks.build_tracks(pairs, [6] * 4, 4), O.build_tracks(pairs, [6] * 4, 4))
