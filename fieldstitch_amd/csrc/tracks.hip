@@ -11,12 +11,13 @@
 //
 // Kernels:
 //   k_tr_init         parent[x] = x, counters cleared
-//   k_tr_union        one block per pair: lock-free union-find, the larger root is linked under
+//   k_tr_union        1024 matches per block: lock-free union-find, the larger root is linked under
 //                     the smaller by compare-and-swap, so every root is its component's minimum
-//   k_tr_compress     parent[x] = root of x; the root counts its matched members
-//   k_tr_scan_*<0>    over nodes: components with >= 2 matched members ("candidates") get a
+//   k_tr_compress     parent[x] = root of x; the root counts its members, each member keeps
+//                     the count it found as its index
+//   k_tr_scan_*<0>    over nodes: components with >= 2 members ("candidates") get a
 //                     number and a run of slots
-//   k_tr_scatter      matched node -> a slot of its candidate (atomic cursor per candidate)
+//   k_tr_scatter      member of a candidate -> its slot (index from the compress kernel)
 //   k_tr_rank         one thread per slot: the member's rank (members below it) places it in
 //                     node order; a second member in its photo's node range, or more members
 //                     than photos, clears the candidate
@@ -30,6 +31,7 @@ constexpr int TR_THREADS = 256;
 constexpr int TR_SCAN_PER_THREAD = 8;
 constexpr int TR_SCAN_CHUNK = TR_THREADS * TR_SCAN_PER_THREAD;
 constexpr int TR_TOP_THREADS = 1024;
+constexpr int TR_UCHUNK = 1024;      // matches per union block
 constexpr int TR_OFF_LDS = 2048;      // photo offsets staged in LDS up to this many entries
 
 // result slots
@@ -39,11 +41,10 @@ constexpr int TR_N_CAND = 0, TR_N_SLOTS = 1;
 
 struct TrWs {
     int* parent;        // [n]
-    int* size;          // [n] matched members of the component (at its root)
-    int* cursor;        // [n] scatter cursor (at candidate roots)
+    int* size;          // [n] members of the component (at its root; 1 for a feature without matches)
+    int* local;         // [n] index of the node among its component's members (the root: 0)
     int* comp;          // [n] candidate number (at candidate roots)
     int* first;         // [n] first slot (at candidate roots)
-    uint8_t* used;      // [n] node has a match
     int* slots;         // [obs_cap] member nodes, candidate by candidate
     int* tmp;           // [obs_cap] members in node order, candidate by candidate
     int* cand_root;     // [cand_cap]
@@ -69,15 +70,14 @@ size_t tr_layout(long long n, long long obs_cap, TrWs* w, char* base) {
     auto take = [&](size_t bytes) { size_t o = off; off += align256(bytes); return base ? base + o : nullptr; };
     char* p[14];
     for (int k = 0; k < 5; ++k) p[k] = take(sizeof(int) * n);
-    p[5] = take(n);
     p[6] = take(sizeof(int) * oc);
     p[7] = take(sizeof(int) * oc);
     for (int k = 8; k < 12; ++k) p[k] = take(sizeof(int) * cc);
     p[12] = take(sizeof(int2) * (nb > 0 ? nb : 1));
     p[13] = take(sizeof(unsigned long long) * 4);
     if (w) {
-        w->parent = (int*)p[0]; w->size = (int*)p[1]; w->cursor = (int*)p[2]; w->comp = (int*)p[3];
-        w->first = (int*)p[4]; w->used = (uint8_t*)p[5]; w->slots = (int*)p[6]; w->tmp = (int*)p[7];
+        w->parent = (int*)p[0]; w->size = (int*)p[1]; w->local = (int*)p[2]; w->comp = (int*)p[3];
+        w->first = (int*)p[4]; w->slots = (int*)p[6]; w->tmp = (int*)p[7];
         w->cand_root = (int*)p[8]; w->cand_ok = (int*)p[9]; w->cand_track = (int*)p[10]; w->cand_out = (int*)p[11];
         w->block_sums = (int2*)p[12]; w->counts = (unsigned long long*)p[13];
         w->obs_cap = oc; w->cand_cap = cc; w->nb_nodes = nb_nodes; w->nb_cand = nb_cand;
@@ -89,17 +89,25 @@ __device__ __forceinline__ int ld_parent(const int* p, int x) {
     return __hip_atomic_load(p + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Root of x with path halving.  parent[x] <= x always holds and parents only move to
-// ancestors, so every value read is an ancestor of x in the current forest.
-__device__ int tr_find(int* parent, int x) {
-    int p = ld_parent(parent, x);
-    while (p != x) {
-        const int gp = ld_parent(parent, p);
-        if (gp != p) __hip_atomic_store(parent + x, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        x = gp;
-        p = ld_parent(parent, x);
+// Roots of a and b by path halving.  parent[x] <= x always holds and parents only move to
+// ancestors, so every value read is an ancestor in the current forest.  The two walks advance
+// in lockstep, so their loads overlap instead of running one chain after the other.
+__device__ void tr_find2(int* parent, int& a, int& b) {
+    int pa = ld_parent(parent, a), pb = ld_parent(parent, b);
+    while (pa != a || pb != b) {
+        const int ga = pa != a ? ld_parent(parent, pa) : pa;
+        const int gb = pb != b ? ld_parent(parent, pb) : pb;
+        if (pa != a) {
+            if (ga != pa) __hip_atomic_store(parent + a, ga, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            a = ga;
+            pa = ld_parent(parent, a);
+        }
+        if (pb != b) {
+            if (gb != pb) __hip_atomic_store(parent + b, gb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            b = gb;
+            pb = ld_parent(parent, b);
+        }
     }
-    return x;
 }
 
 // Image of node x: the last image whose first node is <= x (offsets has n_img + 1 entries).
@@ -116,9 +124,7 @@ __global__ void k_tr_init(TrWs w, long long n, long long* result) {
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += stride) {
         w.parent[x] = (int)x;
-        w.size[x] = 0;
-        w.cursor[x] = 0;
-        w.used[x] = 0;
+        w.size[x] = 1;
     }
     if (blockIdx.x == 0 && threadIdx.x < 4) {
         result[threadIdx.x] = 0;
@@ -126,24 +132,39 @@ __global__ void k_tr_init(TrWs w, long long n, long long* result) {
     }
 }
 
+// Last pair whose first match is <= e among pairs [lo, hi) (pair_start rises; a pair
+// without matches shares its start with the next, so the last such pair is the one holding e).
+__device__ __forceinline__ int tr_pair_of(const long long* pair_start, int lo, int hi, long long e) {
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (pair_start[mid] <= e) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// TR_UCHUNK consecutive matches per block, whatever pairs they belong to (one block per pair
+// left the largest pairs on the critical path).
 __global__ void __launch_bounds__(TR_THREADS) k_tr_union(TrWs w, const long long* offsets, int n_img,
                                                          const int* pair_ij, const long long* pair_start,
-                                                         const int* q, const int* t, long long n_edges,
+                                                         int n_pair, const int* q, const int* t, long long n_edges,
                                                          long long n_nodes, long long* result) {
-    const int p = blockIdx.x;
-    const int i = pair_ij[2 * p], j = pair_ij[2 * p + 1];
-    if (i < 0 || i >= n_img || j < 0 || j >= n_img) {
-        if (threadIdx.x == 0) result[TR_ERR_INDEX] = 1;
-        return;
-    }
-    const long long oi = offsets[i], ni = offsets[i + 1] - oi;
-    const long long oj = offsets[j], nj = offsets[j + 1] - oj;
-    const long long s0 = pair_start[p], s1 = pair_start[p + 1];
-    if (s0 < 0 || s1 < s0 || s1 > n_edges) {
-        if (threadIdx.x == 0) result[TR_ERR_INDEX] = 1;
-        return;
-    }
-    for (long long k = s0 + threadIdx.x; k < s1; k += TR_THREADS) {
+    __shared__ int s_range[2];
+    const long long base = (long long)blockIdx.x * TR_UCHUNK;
+    const long long last = (base + TR_UCHUNK < n_edges ? base + TR_UCHUNK : n_edges) - 1;
+    if (threadIdx.x < 2) s_range[threadIdx.x] = tr_pair_of(pair_start, 0, n_pair, threadIdx.x ? last : base);
+    __syncthreads();
+    const int plo = s_range[0], phi = s_range[1];
+    for (int m = 0; m < TR_UCHUNK / TR_THREADS; ++m) {
+        const long long k = base + m * TR_THREADS + threadIdx.x;
+        if (k > last) break;
+        const int p = tr_pair_of(pair_start, plo, phi + 1, k);
+        const int i = pair_ij[2 * p], j = pair_ij[2 * p + 1];
+        if (pair_start[p] > k || pair_start[p + 1] <= k || i < 0 || i >= n_img || j < 0 || j >= n_img) {
+            result[TR_ERR_INDEX] = 1;
+            continue;
+        }
+        const long long oi = offsets[i], ni = offsets[i + 1] - oi;
+        const long long oj = offsets[j], nj = offsets[j + 1] - oj;
         const int a = q[k], b = t[k];
         if (a < 0 || a >= ni || b < 0 || b >= nj || oi + a < 0 || oi + a >= n_nodes || oj + b < 0 ||
             oj + b >= n_nodes) {
@@ -151,11 +172,8 @@ __global__ void __launch_bounds__(TR_THREADS) k_tr_union(TrWs w, const long long
             continue;
         }
         int u = (int)(oi + a), v = (int)(oj + b);
-        w.used[u] = 1;
-        w.used[v] = 1;
         while (true) {
-            u = tr_find(w.parent, u);
-            v = tr_find(w.parent, v);
+            tr_find2(w.parent, u, v);
             if (u == v) break;
             if (u < v) { const int s = u; u = v; v = s; }
             // link the larger root u under v; a failed swap returns u's new parent
@@ -168,14 +186,16 @@ __global__ void __launch_bounds__(TR_THREADS) k_tr_union(TrWs w, const long long
 
 // After the unions: every thread stores only roots, so no parent is left pointing below its
 // root when the kernel ends (path halving here could store a non-root ancestor after the
-// node's own thread stored its root).  The root counts the matched members.
+// node's own thread stored its root).  Sizes start at 1 and every non-root adds itself to its
+// root; the count it finds there is its index among the members.  A feature without matches is never linked: it stays a root of size 1, so the
+// components of >= 2 members are exactly those of matched features.
 __global__ void k_tr_compress(TrWs w, long long n) {
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += stride) {
         int r = (int)x, p = ld_parent(w.parent, r);
         while (p != r) { r = p; p = ld_parent(w.parent, r); }
         __hip_atomic_store(w.parent + x, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (w.used[x]) atomicAdd(w.size + r, 1);
+        w.local[x] = r != (int)x ? atomicAdd(w.size + r, 1) : 0;
     }
 }
 
@@ -299,10 +319,9 @@ __global__ void k_tr_scatter(TrWs w, long long n) {
         return;
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += stride) {
-        if (!w.used[x]) continue;
         const int r = w.parent[x];
         if (w.size[r] < 2) continue;
-        w.slots[w.first[r] + atomicAdd(w.cursor + r, 1)] = (int)x;
+        w.slots[w.first[r] + w.local[x]] = (int)x;
     }
 }
 
@@ -402,9 +421,10 @@ int fsx_build_tracks(int n_img, const long long* offsets_dev, long long n_nodes,
     hipStream_t st = (hipStream_t)stream;
     hipLaunchKernelGGL(k_tr_init, dim3(grid_for(n_nodes)), dim3(TR_THREADS), 0, st, w, n_nodes, result_dev);
     FSX_LAUNCH_CHECK();
-    if (n_pair > 0) {
-        hipLaunchKernelGGL(k_tr_union, dim3(n_pair), dim3(TR_THREADS), 0, st, w, offsets_dev, n_img, pair_ij_dev,
-                           pair_start_dev, q_dev, t_dev, n_edges, n_nodes, result_dev);
+    if (n_pair > 0 && n_edges > 0) {
+        hipLaunchKernelGGL(k_tr_union, dim3((unsigned)((n_edges + TR_UCHUNK - 1) / TR_UCHUNK)), dim3(TR_THREADS), 0, st,
+                           w, offsets_dev, n_img, pair_ij_dev, pair_start_dev, n_pair, q_dev, t_dev, n_edges, n_nodes,
+                           result_dev);
         FSX_LAUNCH_CHECK();
     }
     if (n_nodes == 0) return FSX_OK;
