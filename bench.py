@@ -569,6 +569,50 @@ def bench_tracks(args):
     }
 
 
+def bench_clahe(args):
+    """CLAHE (clip 2.0, 8 x 8 tiles; features.py:60-71) of the 8-photo 2640 x 1978 batch,
+    photos resident on the GPU: one LUT launch and one apply launch per batch.  Algorithmic
+    bytes 3 per pixel (read for the histograms, read + write for the mapping).  Checked equal to
+    the CPU restatement of OpenCV's algorithm on one photo, which is also the CPU baseline."""
+    import torch
+    from fieldstitch_amd import clahe
+    from oracle import clahe_oracle as co
+    rng = np.random.default_rng(0)
+    n, H, W = 8, 1978, 2640
+    base = (rng.normal(110, 35, (H, W)) + 40 * np.sin(np.arange(W) / 53.0)[None, :]).clip(0, 255).astype(np.uint8)
+    imgs = np.stack([np.roll(base, 97 * k, axis=1) for k in range(n)])
+    d = torch.from_numpy(imgs).cuda()
+    out = torch.empty_like(d)
+    c = clahe.createCLAHE(2.0, (8, 8))
+    for _ in range(args.warmup):
+        c.apply(d, out=out)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        c.apply(d, out=out)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    t0 = time.time()
+    ref = co.apply(imgs[0])
+    t_cpu = time.time() - t0
+    equal = bool(np.array_equal(out[0].cpu().numpy(), ref))
+    alg = 3.0 * n * H * W
+    return {
+        "metric": "photos/s through CLAHE (clip 2.0, 8 x 8 tiles)", "value": n / (ms / 1e3), "unit": "photos/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u8 / f32", "data": "synthetic",
+        "config": {"workload": "CLAHE, 8 photos 2640x1978 uint8"},
+        "equal_to_oracle_photo0": equal,
+        "roofline": {"bound": "hbm", "achieved": alg / (ms / 1e3) / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": alg / (ms / 1e3) / HBM_PEAK, "traffic": None,
+                     "note": "3 B/px: histogram read, mapping read + write; both launches in the time"},
+        "cpu_baseline": {"value": 1.0 / t_cpu, "unit": "photos/s", "cores": 1, "kind": "port",
+                         "sample": "one photo, oracle.clahe_oracle.apply (numpy), %.2f s" % t_cpu},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -585,6 +629,7 @@ def main():
     ap.add_argument("--filter-epochs", type=int, default=8)
     ap.add_argument("--filter-flights", type=int, default=7, help="400-photo flights: n-2 train, 1 threshold, 1 test")
     ap.add_argument("--filter-batch", type=int, default=65536)
+    ap.add_argument("--clahe", action="store_true", help="CLAHE of the 8-photo batch instead of the headline")
     ap.add_argument("--tracks", action="store_true",
                     help="track building from a 400-photo survey's match lists instead of the headline")
     ap.add_argument("--flight400", action="store_true",
@@ -610,6 +655,9 @@ def main():
         return
     if args.tracks:
         print(json.dumps(bench_tracks(args)), flush=True)
+        return
+    if args.clahe:
+        print(json.dumps(bench_clahe(args)), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this library has no multi-GPU path; photos would shard by flight-grid block",

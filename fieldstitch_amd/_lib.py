@@ -62,6 +62,8 @@ SIGNATURES = {
     "fsx_undistort_f64": [_i, _vp, C.POINTER(C.c_double), _i, _vp, _vp],
     "fsx_ground_predict": [_i, _vp, C.POINTER(C.c_double), _vp, _vp, _vp],
     "fsx_pair_select": [_i, _vp, _vp, _i, _i, _vp, _vp],
+    "fsx_clahe_workspace_bytes": [_i, _i, _i, C.POINTER(_sz)],
+    "fsx_clahe": [_i, _vp, _i, _i, C.c_double, _i, _i, _vp, _vp, _sz, _vp],
     "fsx_tracks_workspace_bytes": [C.c_longlong, C.c_longlong, C.POINTER(_sz), C.POINTER(C.c_longlong)],
     "fsx_build_tracks": [_i, _vp, C.c_longlong, _i, _vp, _vp, _vp, _vp, C.c_longlong, _vp, _sz, C.c_longlong, _vp,
                          _vp, _vp, _vp, _vp],

@@ -8,7 +8,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfsx.so")
-SOURCES = ["api.hip", "sift.hip", "match.hip", "geom.hip", "fmatch.hip", "mlp.hip", "verify.hip", "tracks.hip"]
+SOURCES = ["api.hip", "sift.hip", "match.hip", "geom.hip", "fmatch.hip", "mlp.hip", "verify.hip", "tracks.hip", "clahe.hip"]
 
 # Strict IEEE arithmetic (the kernels reproduce numpy's float32/float64 rounding step by
 # step): no contraction into FMA, correctly rounded float32 divide/sqrt, denormals kept.

@@ -352,6 +352,15 @@ int fsx_ground_predict(int n, const double *u_dev, const double *cams, double *p
 int fsx_pair_select(int n, const double *xy_dev, const double *half_long_dev, int k, int has_rel, int *out_dev,
                     void *stream);
 
+/* Contrast-limited adaptive histogram equalisation of n_img uint8 photos of H x W (contiguous,
+ * n x H x W), the conditioning of features.py:60-71 (cv2.createCLAHE(clip_limit, (tiles_x,
+ * tiles_y)).apply, OpenCV's 8-bit algorithm).  Workspace: the per-tile LUTs,
+ * n_img * tiles_x * tiles_y * 256 bytes.  dst may equal src.  Requires H > tiles_y,
+ * W > tiles_x, H <= 65535, n_img <= 65535.  Asynchronous. */
+int fsx_clahe_workspace_bytes(int n_img, int tiles_x, int tiles_y, size_t *bytes);
+int fsx_clahe(int n_img, const uint8_t *src_dev, int H, int W, double clip_limit, int tiles_x, int tiles_y,
+              uint8_t *dst_dev, void *ws_dev, size_t ws_bytes, void *stream);
+
 /* Feature tracks from verified match lists (recon.py:14-56, build_tracks(pairs, nfeat, n_img)).
  * Workspace bytes and the observation capacity min(n_nodes, 2 * n_edges) for n_nodes features
  * in all (n_nodes < 2^31) and n_edges matches. */
