@@ -354,8 +354,8 @@ int fsx_pair_select(int n, const double *xy_dev, const double *half_long_dev, in
 
 /* Contrast-limited adaptive histogram equalisation of n_img uint8 photos of H x W (contiguous,
  * n x H x W), the conditioning of features.py:60-71 (cv2.createCLAHE(clip_limit, (tiles_x,
- * tiles_y)).apply, OpenCV's 8-bit algorithm).  Workspace: the per-tile LUTs,
- * n_img * tiles_x * tiles_y * 256 bytes.  dst may equal src.  Requires H > tiles_y,
+ * tiles_y)).apply, OpenCV's 8-bit algorithm).  Workspace: per-tile LUTs and histograms,
+ * n_img * tiles_x * tiles_y * 256 * 5 bytes.  dst may equal src.  Requires H > tiles_y,
  * W > tiles_x, H <= 65535, n_img <= 65535.  Asynchronous. */
 int fsx_clahe_workspace_bytes(int n_img, int tiles_x, int tiles_y, size_t *bytes);
 int fsx_clahe(int n_img, const uint8_t *src_dev, int H, int W, double clip_limit, int tiles_x, int tiles_y,
