@@ -12,6 +12,7 @@ Modules (each a drop-in for the reference module named):
 * ``camera``     undistortion (app/planesite/camera.py); ``pairs`` the pipeline's GPS pair
                  selection (app/planesite/pipeline.py)
 * ``verify``     deterministic GPU RANSAC (homography, fundamental matrix, similarity)
+* ``clahe``      createCLAHE(...).apply, the photos' input conditioning (app/planesite/features.py)
 * ``tracks``     build_tracks: feature tracks from verified match lists (app/planesite/recon.py)
 * ``filter_mlp`` the learned BF16 match filter; ``autotune`` launch-shape tuning; ``synth``
                  seeded synthetic flights; ``build`` compiles libfsx.so in-tree
