@@ -280,7 +280,18 @@ def _gpu_ready():
     return "%s (%.1f GB)" % (prop.name, prop.total_memory / 2 ** 30)
 
 
-def gpu_worker_main(task_q, res_q, n_threads, wid, batch=16, decode=read_gray):
+_CLAHE = []
+
+
+def _clahe():
+    """The worker's CLAHE object (clip 2.0, 8 x 8 tiles, features.py:70), created on first use."""
+    if not _CLAHE:
+        from .clahe import createCLAHE
+        _CLAHE.append(createCLAHE(2.0, (8, 8)))
+    return _CLAHE[0]
+
+
+def gpu_worker_main(task_q, res_q, n_threads, wid, batch=16, decode=read_gray, clahe=False):
     """features.py:270-381 with the one-photo GPU loop replaced by batches.  ``n_threads``
     decoder threads take tasks (idx, path, out_npz, thumb_path, reduce, nfeat) from task_q
     (None ends a thread) and decode them; the GPU loop gathers what is ready (at least one,
@@ -289,7 +300,9 @@ def gpu_worker_main(task_q, res_q, n_threads, wid, batch=16, decode=read_gray):
     with _save and posts ("ok", (idx, n, err, timings)).  Also posted: ("gpu", wid, info) when
     ready, ("gpu_off", wid, msg) if the GPU cannot be used (there is no CPU path here: every
     photo is then reported failed), and ("exit", "gpu%d" % wid) at the end.  thumb_path and
-    reduce belong to the decoder the reference uses and are not interpreted."""
+    reduce belong to the decoder the reference uses and are not interpreted.  With
+    ``clahe=True`` the decoder returns the plain gray image and the contrast equalisation of
+    features.py:70-71 (clip 2.0, 8 x 8 tiles) runs on the GPU on each photo of the batch."""
     import queue
     import threading
     try:
@@ -336,7 +349,10 @@ def gpu_worker_main(task_q, res_q, n_threads, wid, batch=16, decode=read_gray):
         while len(cache) > 4:
             cache.pop(next(iter(cache)))
         t2 = time.perf_counter()
-        fb.run([torch.from_numpy(g2).cuda() for _, g2, _, _ in group])
+        dev = [torch.from_numpy(g2).cuda() for _, g2, _, _ in group]
+        if clahe:
+            dev = [_clahe().apply(d) for d in dev]
+        fb.run(dev)
         fb.check_capacity()
         for k, (task, g2, tt, _) in enumerate(group):
             xy, u8, resp, _ = fb.results(k)

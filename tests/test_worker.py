@@ -20,14 +20,14 @@ def _tasks(tmp_path, shapes, nfeat=1500):
     return tasks
 
 
-def _run(tasks, n_threads=2, batch=4):
+def _run(tasks, n_threads=2, batch=4, **kw):
     from fieldstitch_amd import features
     tq, rq = queue.Queue(), queue.Queue()
     for t in tasks:
         tq.put(t)
     for _ in range(n_threads):
         tq.put(None)
-    features.gpu_worker_main(tq, rq, n_threads, 0, batch=batch)
+    features.gpu_worker_main(tq, rq, n_threads, 0, batch=batch, **kw)
     out = []
     while not rq.empty():
         out.append(rq.get())
@@ -67,6 +67,30 @@ def test_worker_matches_sift_gpu(tmp_path, gpu):
         _, n, err, tt = ok[idx]
         assert err is None and n > 10 and tt["photos_gpu"] == 1
         g = np.load(path)
+        xy, desc, resp, _ = features.sift_gpu(g, nfeat)
+        ref = str(tmp_path / ("ref%d.npz" % idx))
+        assert features._save(ref, xy, desc, resp, g.shape[1], g.shape[0]) == n
+        a, b = np.load(out), np.load(ref)
+        for key in ("xy", "resp", "red_wh"):
+            np.testing.assert_array_equal(a[key], b[key])
+        assert np.abs(a["desc"].astype(int) - b["desc"].astype(int)).max() <= 1
+
+
+@pytest.mark.gpu
+def test_worker_gpu_clahe(tmp_path, gpu):
+    """clahe=True: the GPU equalises the plain decoded photos; the feature files equal
+    sift_gpu on the CPU restatement's CLAHE output."""
+    from fieldstitch_amd import features
+    from oracle import clahe_oracle
+    shapes = [(240, 320), (300, 410), (240, 320)]
+    tasks = _tasks(tmp_path, shapes)
+    msgs = _run(tasks, n_threads=2, batch=2, clahe=True)
+    ok = {m[1][0]: m[1] for m in msgs if m[0] == "ok"}
+    assert sorted(ok) == [0, 1, 2]
+    for idx, path, out, _, _, nfeat in tasks:
+        _, n, err, _ = ok[idx]
+        assert err is None and n > 10
+        g = clahe_oracle.apply(np.load(path))
         xy, desc, resp, _ = features.sift_gpu(g, nfeat)
         ref = str(tmp_path / ("ref%d.npz" % idx))
         assert features._save(ref, xy, desc, resp, g.shape[1], g.shape[0]) == n
