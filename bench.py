@@ -394,6 +394,25 @@ def bench_flight400(args):
                                    u8_all[j, :counts[j]].cpu().numpy(), xy_all[j, :counts[j]].cpu().numpy(),
                                    Hm[p], (shapes[j][1], shapes[j][0]), RADIUS, RATIO)
         ok += int(np.array_equal(q, res["mfma"][p][0]) and np.array_equal(t, res["mfma"][p][1]))
+    # tracks (recon.py:14-56) from the flight's guided matches (the pipeline feeds RANSAC-
+    # verified ones; the guided matches stand in for them here), device pass timed
+    from fieldstitch_amd import tracks as trk
+    from oracle import tracks_oracle as to
+    tpairs = [(i, j, q, t) for (i, j), (q, t) in zip(pl, res["mfma"])]
+    toff, tij, tstart, tq, tt = trk.pack_pairs(tpairs, counts)
+    dtr = [torch.from_numpy(np.ascontiguousarray(a_)).cuda() for a_ in (toff, tij, tstart, tq, tt)]
+    tb = trk.TrackBuilder("cuda")
+    tb.run(*dtr)
+    torch.cuda.synchronize()
+    t4 = time.perf_counter()
+    for _ in range(steps):
+        g_tid, g_img, g_feat, g_ntr = tb.run(*dtr)
+    t_tr = (time.perf_counter() - t4) / steps
+    ref_tr = to.build_tracks(tpairs, counts, n)
+    tracks_equal = (np.array_equal(g_tid.cpu().numpy(), ref_tr[0]) and np.array_equal(g_img.cpu().numpy(), ref_tr[1])
+                    and np.array_equal(g_feat.cpu().numpy(), ref_tr[2]) and g_ntr == ref_tr[3])
+    out_tracks = {"matches": int(len(tq)), "observations": int(len(g_tid)), "tracks": int(g_ntr),
+                  "ms_per_flight_incl_count_readback": t_tr * 1e3, "equal_to_oracle": bool(tracks_equal)}
     return {
         "metric": "images/s (SIFT) + matched pairs/s vs plain FP32 GPU form, match set = FP64 oracle",
         "value": n / t_sift, "unit": "images/s", "n_gpus": 1, "steps": steps, "warmup": 1,
@@ -411,7 +430,7 @@ def bench_flight400(args):
         "matching": {"pairs": len(pl), **out_m, "speedup_vs_plain_fp32":
                      out_m["plain"]["ms_per_flight"] / out_m["mfma"]["ms_per_flight"],
                      "forms_equal": bool(same_forms), "fp64_oracle_sample": int(len(sample)),
-                     "fp64_oracle_equal": int(ok),
+                     "fp64_oracle_equal": int(ok), "tracks_from_these_matches": out_tracks,
                      "compulsory_bytes": 128.0 * float(counts.sum()),
                      "compulsory_hbm_frac": 128.0 * float(counts.sum()) / (out_m["mfma"]["ms_per_flight"] / 1e3)
                                             / HBM_PEAK},
