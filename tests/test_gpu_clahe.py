@@ -34,5 +34,11 @@ def test_batch_and_in_place(gpu):
         assert np.array_equal(out[k], O.apply(imgs[k]))
     c.apply(d, out=d)                        # in place
     assert np.array_equal(d.cpu().numpy(), out)
+    # width divisible by 4 and tiles >= 16 rows: the LDS-staged 4-pixel form, in place too
+    imgs = rng.integers(0, 256, (2, 200, 256)).astype(np.uint8)
+    d = torch.from_numpy(imgs).cuda()
+    c.apply(d, out=d)
+    for k in range(2):
+        assert np.array_equal(d[k].cpu().numpy(), O.apply(imgs[k]))
     with pytest.raises(TypeError):
         c.apply(imgs[0].astype(np.float32))
