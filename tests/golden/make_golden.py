@@ -10,13 +10,16 @@ synthetic images of ``fieldstitch_amd.synth`` (identified by their SHA-256 so th
 the generator is caught), outputs are the reference's keypoint dictionaries, descriptors and
 per-stage intermediates.  Nothing from the reference is stored except these numbers.
 """
-import hashlib
-import os
 import sys
-import time
 
-import numpy as np
-import scipy
+sys.dont_write_bytecode = True      # nothing may be written into the reference checkout
+
+import hashlib  # noqa: E402
+import os  # noqa: E402
+import time  # noqa: E402
+
+import numpy as np  # noqa: E402
+import scipy  # noqa: E402
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
