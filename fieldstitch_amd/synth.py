@@ -194,14 +194,16 @@ def _render_job(job):
 def flight(seed: int = 2, rows: int = 2, cols: int = 4, W: int = 2640, H: int = 1978,
            front: float = 0.75, side: float = 0.65, rot_jitter: float = 3.0, scale_jitter: float = 0.02,
            prior_sigma: float = 3.0, lowres_frac: float = 0.0, poly_density: float = 40.0 / (1368 * 912),
-           min_overlap: float = 0.05, workers: int = 1, progress=None) -> Flight:
+           min_overlap: float = 0.05, workers: int = 1, progress=None, lowres_pattern=None) -> Flight:
     """Photos along a rows x cols flight grid.  Flight lines run along x (a line is one grid row),
     so consecutive photos of a line overlap by `front` of the width and neighbouring lines by
     `side` of the height.  A photo is low resolution (W/2 x H/2 covering the same ground) with
     probability `lowres_frac`.  Pairs: every photo pair whose footprints (without jitter)
     overlap by more than `min_overlap` of the footprint area.  Each photo's noise has its own
     seed, so `workers` > 1 renders in parallel with identical results.  progress(done, total)
-    is called every 25 photos."""
+    is called every 25 photos.  ``lowres_pattern`` (one bool per photo, row-major) fixes which
+    photos are low resolution instead of the draw (the draw still happens, so the rest of the
+    flight is the same)."""
     rng = np.random.default_rng(seed)
     step_u = (1.0 - front) * W
     step_v = (1.0 - side) * H
@@ -217,6 +219,8 @@ def flight(seed: int = 2, rows: int = 2, cols: int = 4, W: int = 2640, H: int = 
             theta = rng.uniform(-rot_jitter, rot_jitter)
             s = 1.0 + rng.uniform(-scale_jitter, scale_jitter)
             low = rng.random() < lowres_frac
+            if lowres_pattern is not None:
+                low = bool(lowres_pattern[r * cols + c])
             w, h, g = (W // 2, (H + 1) // 2, 2.0) if low else (W, H, 1.0)
             cu = c * step_u + W / 2.0
             cv = r * step_v + H / 2.0

@@ -180,6 +180,52 @@ def match_fp64(da, db, ratio: float, qs=None, cands=None, block: int = 1024):
     return q[keep], t[keep], d[keep]
 
 
+def match_fp64_fast(da, db, ratio: float, block: int = 2048):
+    """match_fp64 (brute force) for large sets, same definition and result.  A float64 BLAS
+    product gives every distance by the expansion |a|^2 + |b|^2 - 2 a.b, whose error e is at
+    most ~128 * 2^-53 * (|a|^2 + |b|^2 + 2|a||b|); per query, every target whose expanded
+    distance is within tol >= 2e of the second smallest expanded distance is rescored exactly
+    (sum_k (a_k - b_k)^2 in float64, k in order).  The exact two smallest distances always lie in
+    that set: a target with exact distance <= the exact second smallest s2 has an expanded
+    distance <= s2 + e, and the second smallest expanded distance is >= s2 - e.  Selection,
+    ties and the one-query-per-target rule then follow match_fp64.
+    Returns (q int64, t int64, b1 float64) sorted by t."""
+    da = np.asarray(da, np.float64)
+    db = np.asarray(db, np.float64)
+    if len(da) < 2 or len(db) < 2:
+        return np.zeros(0, np.int64), np.zeros(0, np.int64), np.zeros(0)
+    na = (da * da).sum(1)
+    nb = (db * db).sum(1)
+    tol = 1e-9 * (na.max() + nb.max()) + 1e-9
+    out_q, out_t, out_d = [], [], []
+    for s0 in range(0, len(da), block):
+        A = da[s0:s0 + block]
+        approx = na[s0:s0 + block, None] + nb[None, :] - 2.0 * (A @ db.T)
+        second = np.partition(approx, 1, axis=1)[:, 1]
+        qq, tt = np.nonzero(approx <= (second + tol)[:, None])
+        acc = np.zeros(len(qq))
+        for k in range(da.shape[1]):
+            t_ = A[qq, k] - db[tt, k]
+            acc += t_ * t_
+        order = np.lexsort((tt, acc, qq))          # per query: distance, then index
+        qq, tt, acc = qq[order], tt[order], acc[order]
+        starts = np.nonzero(np.r_[True, qq[1:] != qq[:-1]])[0]
+        b1 = acc[starts]
+        b2 = acc[starts + 1]                        # >= 2 candidates: the second smallest is in the set
+        ok = np.sqrt(b1) < float(ratio) * np.sqrt(b2)
+        out_q.append(qq[starts][ok] + s0)
+        out_t.append(tt[starts][ok])
+        out_d.append(b1[ok])
+    q = np.concatenate(out_q).astype(np.int64)
+    t = np.concatenate(out_t).astype(np.int64)
+    d = np.concatenate(out_d)
+    order = np.lexsort((q, d, t))
+    t_s = t[order]
+    first = np.r_[True, t_s[1:] != t_s[:-1]]
+    keep = order[first]
+    return q[keep], t[keep], d[keep]
+
+
 def guided_fp64_fast(da, xy_a, db, xy_b, Hm, wh_b, radius: float, ratio: float):
     """Vectorised FP64 oracle of guided matching with kc = 0 (all candidates in the window):
     the same definition as guided_candidates + match_fp64, for large pairs.

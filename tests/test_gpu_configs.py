@@ -1,0 +1,207 @@
+"""GPU parity on the three workload shapes the headline metric names besides the 8-photo
+benchmark itself:
+
+* the 1368 x 912 field pair with OpenCV-default thresholds: keypoints equal to the reference's
+  golden vectors, then brute-force ratio-0.8 matching of the two photos' descriptors (float SIFT
+  through the BF16 and FP8 screens, RootSIFT uint8 through both integer forms) equal to the CPU
+  FP64 oracle;
+* float descriptor pairs at the precision workload's extremes (2000 x 12000, 12000 x 2000 and
+  one near 7000): BF16 and FP8 screening with K = 8 equal to the CPU FP64 oracle;
+* a mixed-resolution flight (two 2640 x 1978 and two 1320 x 989 photos): one graph-replayed
+  feature batch equal to the per-photo path, a 1320 x 989 photo bit-exact to the CPU oracle,
+  and the uneven-count pair batch equal to the FP64 guided oracle;
+
+plus the kc-nearest rule on a keypoint lattice full of exact spatial ties.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from fieldstitch_amd import synth
+from oracle import match_oracle as mo
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+RATIO = 0.8
+
+
+def _same(got, want):
+    return np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+
+
+# ---- 1368 x 912 pair --------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def c1_pair(gpu):
+    """Both golden photos detected and fully described on the GPU, with the FP64 oracle's match
+    lists of their float and RootSIFT descriptors."""
+    from fieldstitch_amd import gsift
+    img1, img2, _ = synth.field_pair(1)
+    feats = []
+    for name, img in (("c1_full_cv", img1), ("c1_full2_cv", img2)):
+        g = np.load(os.path.join(GOLD, "gsift_%s.npz" % name))
+        k, st = gsift.detect(img, max_kp=int(g["max_kp"]), contrast=float(g["contrast"]), edge=float(g["edge"]))
+        for key in ("x", "y", "response", "octave", "layer"):
+            np.testing.assert_array_equal(k[key], g["kp_" + key], err_msg=key)
+        d, u8 = gsift.describe(st, k, np.arange(len(k["x"])), rootsift_u8=True)
+        feats.append((d, u8))
+    (d1, u1), (d2, u2) = feats
+    print("case 1368x912 pair", d1.shape, d2.shape, flush=True)
+    ref_f = mo.match_fp64_fast(d1, d2, RATIO)[:2]
+    ref_u = mo.match_fp64_fast(u1, u2, RATIO)[:2]
+    return d1, d2, u1, u2, ref_f, ref_u
+
+
+@pytest.mark.parametrize("form", ["bf16", "fp8"])
+def test_c1_pair_float_descriptors_vs_fp64(c1_pair, form):
+    from fieldstitch_amd import matching
+    d1, d2, _, _, ref_f, _ = c1_pair
+    assert len(ref_f[0]) > 500
+    got = matching._ratio_match_bf(d1, d2, RATIO, form=form)
+    assert _same(got, ref_f)
+
+
+@pytest.mark.parametrize("form", ["mfma", "plain"])
+def test_c1_pair_rootsift_vs_fp64_and_f32_restatement(c1_pair, form):
+    from fieldstitch_amd import matching
+    _, _, u1, u2, _, ref_u = c1_pair
+    assert len(ref_u[0]) > 500
+    got = matching._ratio_match_bf(u1.astype(np.float32), u2.astype(np.float32), RATIO, form=form)
+    assert _same(got, ref_u)
+    # the repository's own float32 matcher (matching.py:86-110) gives the same list on these
+    assert _same(mo.ratio_match_bf_f32(u1, u2, RATIO), ref_u)
+
+
+# ---- precision pairs at the workload's extremes ---------------------------------------------
+
+C3_SIZES = [(2000, 12000), (12000, 2000), (7000, 6500)]
+
+
+@pytest.fixture(scope="module")
+def c3_sets():
+    rng = np.random.default_rng(303)
+    sets, refs = [], []
+    for na, nb in C3_SIZES:
+        da, db, _ = synth.descriptor_pair(rng, na, nb)
+        sets += [da, db]
+        refs.append(mo.match_fp64_fast(da, db, RATIO)[:2])
+    return sets, refs
+
+
+@pytest.mark.parametrize("precision", ["bf16", "fp8"])
+def test_precision_pairs_at_extremes_vs_cpu_fp64(gpu, c3_sets, precision):
+    from fieldstitch_amd.matching import FloatMatcher
+    sets, refs = c3_sets
+    print("case precision", precision, C3_SIZES, flush=True)
+    fm = FloatMatcher(sets, [(2 * p, 2 * p + 1) for p in range(len(C3_SIZES))], ratio=RATIO, precision=precision,
+                      K=8)
+    fm.run()
+    got = fm.results()
+    for p, (g, r) in enumerate(zip(got, refs)):
+        assert len(r[0]) > 500
+        assert _same(g, r), C3_SIZES[p]
+    st = fm.stats()
+    assert st["rescored_candidates"] > 0 and st["fallback_queries"] < 0.01 * sum(a for a, _ in C3_SIZES)
+
+
+# ---- mixed-resolution flight ----------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def mixed_flight():
+    fl = synth.flight(seed=31, rows=2, cols=2, lowres_pattern=[False, True, True, False], workers=1)
+    assert sorted(im.shape for im in fl.images) == [(989, 1320), (989, 1320), (1978, 2640), (1978, 2640)]
+    return fl
+
+
+def test_mixed_batch_graph_replay_equals_per_image_path(gpu, mixed_flight):
+    import torch
+    from fieldstitch_amd import features
+    imgs = mixed_flight.images
+    print("case mixed batch", [i.shape for i in imgs], flush=True)
+    fb = features.FeatureBatch([i.shape for i in imgs], nfeat=8000, contrast=0.015, edge=12.0, float_desc=True)
+    dev = [torch.from_numpy(i).cuda() for i in imgs]
+    fb.run(dev)                      # eager first, then replays of the captured graph
+    fb.check_capacity()
+    for rep in range(2):
+        fb.run(dev, graph=True)
+        torch.cuda.synchronize()
+        for i, img in enumerate(imgs):
+            xy, u8, resp, d = fb.results(i)
+            xr, dr, rr, _ = features.sift_gpu(img, 8000)
+            np.testing.assert_array_equal(xy, xr)
+            np.testing.assert_array_equal(resp, rr)
+            np.testing.assert_array_equal(d, dr)
+            np.testing.assert_array_equal(u8, features.rootsift_u8(dr))
+
+
+def test_mixed_small_photo_bit_exact_to_oracle(gpu, mixed_flight):
+    from fieldstitch_amd import gsift
+    from oracle import sift_oracle as so
+    img = mixed_flight.images[1]
+    assert img.shape == (989, 1320)
+    k, st = gsift.detect(img, max_kp=20000, contrast=0.04, edge=10.0)
+    kr, sr = so.detect(img, 20000, 0.04, 10.0)
+    assert len(k["x"]) == len(kr["x"]) > 2000
+    for key in gsift.KEYS:
+        if key == "angle" or key == "size":
+            rel = np.abs(k[key].astype(np.float64) - kr[key]) / np.maximum(np.abs(kr[key].astype(np.float64)), 1e-6)
+            assert rel.max() <= 1e-3, key
+        else:
+            np.testing.assert_array_equal(k[key], kr[key], err_msg=key)
+    sel = np.linspace(0, len(k["x"]) - 1, 1000).astype(np.int64)
+    assert np.abs(gsift.describe(st, k, sel) - so.describe(sr, kr, sel)).max() <= 1.0
+
+
+@pytest.mark.parametrize("form", ["mfma", "plain"])
+def test_mixed_uneven_pair_batch_vs_fp64(gpu, mixed_flight, form):
+    import torch
+    from fieldstitch_amd import features
+    from fieldstitch_amd.matching import PairMatcher
+    fl = mixed_flight
+    shapes = [i.shape for i in fl.images]
+    fb = features.FeatureBatch(shapes, nfeat=8000, contrast=0.04, edge=10.0)
+    fb.run([torch.from_numpy(i).cuda() for i in fl.images])
+    fb.check_capacity()
+    host = [fb.results(i) for i in range(len(shapes))]
+    counts = [len(h[0]) for h in host]
+    assert len(set(counts)) > 1, counts          # uneven keypoint counts
+    images = [(h[1], h[0].astype(np.float64), (s[1], s[0])) for h, s in zip(host, shapes)]
+    pairs = list(fl.pairs)
+    Hm = [fl.prior[p] for p in pairs]
+    pm = PairMatcher(images, pairs, Hm, radius=60.0, kc=0, ratio=RATIO)
+    pm.run(form)
+    got = pm.results()
+    for p, (i, j) in enumerate(pairs):
+        want = mo.guided_fp64_fast(host[i][1], host[i][0].astype(np.float64), host[j][1], host[j][0].astype(np.float64),
+                                   Hm[p], (shapes[j][1], shapes[j][0]), 60.0, RATIO)
+        assert len(want[0]) > 50, (i, j)
+        assert _same(got[p], want), (i, j)
+
+
+# ---- kc nearest with exact spatial ties -------------------------------------------------------
+
+@pytest.mark.parametrize("kc", [1, 3, 5, 8])
+def test_kc_nearest_on_lattice_ties_vs_oracle(gpu, kc):
+    """B keypoints on a 3 px lattice and predictions on lattice / half-lattice points: the kc
+    nearest within the radius include exact distance ties at the kc boundary, which both forms
+    resolve to the lower index, as the oracle does.  Descriptors: every query has a perturbed
+    copy somewhere among the B keypoints of its window."""
+    from fieldstitch_amd.matching import PairMatcher
+    rng = np.random.default_rng(90 + kc)
+    gx, gy = np.meshgrid(np.arange(0, 300, 3.0), np.arange(0, 240, 3.0))
+    xy_b = np.c_[gx.ravel(), gy.ravel()][rng.permutation(gx.size)]
+    d = rng.gamma(0.6, 1.0, (len(xy_b), 128))
+    db = np.clip(np.sqrt(d / d.sum(1, keepdims=True)) * 512, 0, 255).astype(np.uint8)
+    na = 3000
+    xy_a = np.c_[rng.integers(0, 100, na) * 3.0, rng.integers(0, 80, na) * 3.0] + rng.choice([0.0, 1.5], (na, 2))
+    near = np.array([np.argmin(((xy_b - p) ** 2).sum(1) + rng.random(len(xy_b))) for p in xy_a])
+    da = np.clip(db[near].astype(int) + rng.integers(-12, 13, (na, 128)), 0, 255).astype(np.uint8)
+    qs, cands = mo.guided_candidates(xy_a, xy_b, (300, 240), 4.6, kc)
+    want = mo.match_fp64(da, db, RATIO, qs, cands)[:2]
+    assert len(want[0]) > 200
+    for form in ("mfma", "plain"):
+        pm = PairMatcher([(da, xy_a, (300, 240)), (db, xy_b, (300, 240))], [(0, 1)], [np.eye(3)], radius=4.6, kc=kc,
+                         ratio=RATIO)
+        pm.run(form)
+        assert _same(pm.results()[0], want), form

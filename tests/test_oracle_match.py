@@ -97,6 +97,36 @@ def test_guided_candidates_vs_ckdtree():
             np.testing.assert_array_equal(c, np.sort(row[row < len(xy_b)]))
 
 
+def test_guided_candidates_vs_ckdtree_with_exact_ties():
+    """Keypoints on an integer lattice, predictions on lattice and half-lattice points, so many
+    candidates lie at exactly the same distance.  The oracle's kc nearest within the radius
+    (ties to the lower index) and cKDTree.query(k=kc, distance_upper_bound) (matching.py:123-131)
+    agree on the strict bound and on the kc distances of every query.  Their candidate sets are
+    equal wherever the kc-th and the (kc+1)-th distances differ.  On an exact tie at that
+    boundary cKDTree's choice follows its tree layout: a layout detail, not an index rule."""
+    rng = np.random.default_rng(8)
+    gx, gy = np.meshgrid(np.arange(0, 120, 3.0), np.arange(0, 90, 3.0))
+    xy_b = np.c_[gx.ravel(), gy.ravel()][rng.permutation(gx.size)]
+    pred = np.c_[rng.integers(0, 40, 600) * 3.0, rng.integers(0, 30, 600) * 3.0] + rng.choice([0.0, 1.5], (600, 2))
+    tree = cKDTree(xy_b)
+    n_tie_boundary = 0
+    for kc in (1, 3, 5, 8):
+        qs, cands = mo.guided_candidates(pred, xy_b, (120, 90), 4.6, kc)
+        d, idx = tree.query(pred[qs], k=kc, distance_upper_bound=4.6)
+        if kc == 1:
+            idx, d = idx[:, None], d[:, None]
+        for q, c, row in zip(qs, cands, idx):
+            row = row[row < len(xy_b)]
+            dq = ((xy_b - pred[q]) ** 2).sum(1)
+            np.testing.assert_array_equal(np.sort(dq[c]), np.sort(dq[row]))
+            within = np.sort(dq[dq < 4.6 * 4.6])
+            if len(within) > kc and within[kc - 1] == within[kc]:
+                n_tie_boundary += 1
+            else:
+                np.testing.assert_array_equal(c, np.sort(row))
+    assert n_tie_boundary > 100
+
+
 def test_guided_single_candidate_always_passes():
     a = np.zeros((1, 128), np.float32)
     b = np.full((1, 128), 50, np.float32)
@@ -152,3 +182,23 @@ def test_device_sort_rank_rule_handles_equal_keys():
     out = device_sort(keys, block=16)
     assert None not in out
     assert out == sorted(keys)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_fp64_fast_equals_fp64_loop(seed):
+    """The BLAS-screened FP64 oracle returns match_fp64's list exactly, on SIFT-like float
+    descriptors with exact duplicates (equal distances), exact copies of queries (distance 0)
+    and on uint8 RootSIFT-like rows (many equal integer distances)."""
+    from fieldstitch_amd import synth
+    rng = np.random.default_rng(seed)
+    da, db, _ = synth.descriptor_pair(rng, 700, 900)
+    db[100] = db[200]
+    db[300] = da[5]
+    db[301] = da[6]
+    db[302] = da[6]
+    da[7] = da[8]
+    for a, b in ((da, db), (np.rint(da / 4), np.rint(db / 4)), (da[:3], db), (da, db[:2])):
+        slow = mo.match_fp64(a, b, 0.8)
+        fast = mo.match_fp64_fast(a, b, 0.8, block=256)
+        for x, y in zip(slow, fast):
+            np.testing.assert_array_equal(x, y)
