@@ -50,6 +50,9 @@ SIGNATURES = {
                           C.POINTER(_i), C.POINTER(_i), C.POINTER(C.c_double), C.c_double, _i, C.c_double, _vp, _sz,
                           C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), _vp],
     "fsx_match_launch": [_vp, C.POINTER(C.c_longlong), _i, _vp, _vp],
+    "fsx_match_batch_noalloc": [_i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), _vp, _vp, _i, C.POINTER(_i),
+                                C.POINTER(_i), C.POINTER(_i), C.POINTER(C.c_double), C.c_double, _i, C.c_double, _i,
+                                _vp, _vp, _vp, _vp, C.POINTER(C.c_longlong), _vp],
     "fsx_match_results": [_i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), _i, C.POINTER(_i), C.POINTER(_i),
                           C.c_double, _vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), _vp],
     "fsx_mfma_layout_probe": [_vp, _vp],

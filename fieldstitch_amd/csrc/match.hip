@@ -30,6 +30,7 @@
 #include <math.h>
 #include <string.h>
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 namespace fsx {
@@ -911,9 +912,31 @@ static inline size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
 // csz * (1 - 4 ulp) > radius), so the 3x3 cells around a prediction hold its whole window.
 static inline double cell_size(double radius) { return radius * (1.0 + 0x1p-20); }
 
+// A host table held in a std::vector, or in fixed storage the caller provides (the
+// no-allocation entry point plans into arrays on its stack).
+template <class T>
+struct HostTab {
+    std::vector<T> store;
+    T* ext = nullptr;
+    size_t cap = 0, n = 0;
+    bool resize(size_t k) {
+        if (ext) {
+            if (k > cap) return false;
+        } else {
+            store.resize(k);
+        }
+        n = k;
+        return true;
+    }
+    T& operator[](size_t i) { return ext ? ext[i] : store[i]; }
+    const T& operator[](size_t i) const { return ext ? ext[i] : store[i]; }
+    size_t size() const { return n; }
+    T* data() { return ext ? ext : store.data(); }
+};
+
 struct MatchPlanHost {
-    std::vector<MImage> img;
-    std::vector<MPair> pair;
+    HostTab<MImage> img;
+    HostTab<MPair> pair;
     long long n_kp = 0, n_query = 0, n_cells = 0, n_targets = 0, n_out = 0, n_chunks = 0, n_chunk_slots = 0;
     size_t off_chunks, off_nchunks, off_batch, off_img, off_pair, off_norms, off_snorms, off_sxy, off_sorig, off_cstart, off_ccur, off_best, off_b1,
         off_qcell, off_kth, off_cbest, off_cb12, off_tkey, off_cnt, off_outq, off_outt, off_outn, total;
@@ -923,7 +946,7 @@ static int match_plan(int n_img, const int* n_kp, const int* W, const int* H, in
                       const int* pb, const int* dense, const double* Hm, double radius, MatchPlanHost& M) {
     FSX_REQUIRE(n_img >= 1 && n_pair >= 1, "need images and pairs");
     FSX_REQUIRE(radius > 0, "radius must be > 0");
-    M.img.resize(n_img);
+    FSX_REQUIRE(M.img.resize(n_img), "too many images (%d) for this entry point", n_img);
     long long off = 0, coff = 0, choff = 0;
     for (int i = 0; i < n_img; i++) {
         FSX_REQUIRE(n_kp[i] >= 0 && W[i] > 0 && H[i] > 0, "bad image %d", i);
@@ -949,7 +972,7 @@ static int match_plan(int n_img, const int* n_kp, const int* W, const int* H, in
     M.n_kp = off;
     M.n_cells = coff;
     M.n_chunk_slots = choff;
-    M.pair.resize(n_pair);
+    FSX_REQUIRE(M.pair.resize(n_pair), "too many pairs (%d) for this entry point", n_pair);
     long long qo = 0, to = 0, oo = 0, chunks = 0;
     for (int p = 0; p < n_pair; p++) {
         FSX_REQUIRE(pa[p] >= 0 && pa[p] < n_img && pb[p] >= 0 && pb[p] < n_img, "bad pair %d", p);
@@ -1042,6 +1065,115 @@ static MatchBatch batch_ptrs(const MatchPlanHost& M, char* ws, const uint8_t* de
     return b;
 }
 
+
+// launch_info: [0] tag, [1] n_img, [2] n_pair, [3] queries, [4] largest query set, [5] setup
+// blocks, [6] kc, [7] workspace address, [8..] the MatchBatch record, [last] MFMA chunk rows.
+static void fill_launch_info(const MatchPlanHost& M, const MatchBatch& b, int kc, const void* ws, long long* info) {
+    const int n_pair = (int)M.pair.size();
+    int max_na = 0, max_chunks = 0;
+    for (int p = 0; p < n_pair; p++) {
+        max_na = std::max(max_na, M.img[M.pair[p].a].n);
+        max_chunks = std::max(max_chunks, M.img[M.pair[p].a].chunk_cap);
+    }
+    const long long extra =
+        std::min<long long>(std::max<long long>(1, (std::max(M.n_targets, M.n_kp) + SU_THREADS - 1) / SU_THREADS), 4096);
+    memset(info, 0, 8 * FSX_MATCH_INFO_LEN);
+    info[0] = 0x6673786d;     // tag checked by fsx_match_launch
+    info[1] = (long long)M.img.size();
+    info[2] = n_pair;
+    info[3] = M.n_query;
+    info[4] = max_na;
+    info[5] = (long long)M.img.size() + extra;
+    info[6] = kc;
+    info[7] = (long long)(intptr_t)ws;
+    info[FSX_MATCH_INFO_LEN - 1] = max_chunks;
+    memcpy(info + 8, &b, sizeof(b));
+}
+
+// The matcher's launches (setup, kc-nearest prefilter, contraction, compaction) on `st`.
+static int launch_sequence(const long long* info, int form, hipEvent_t* ev, hipStream_t st) {
+    MatchBatch b;
+    memcpy(&b, info + 8, sizeof(b));
+    const long long nq = info[3];
+    const int n_pair = (int)info[2], max_na = (int)info[4];
+    if (ev) FSX_HIP(hipEventRecord(ev[0], st));
+    hipLaunchKernelGGL(k_m_setup, dim3((unsigned)info[5]), dim3(SU_THREADS), 0, st, b);
+    FSX_LAUNCH_CHECK();
+    if (nq > 0 && info[6] > 0)
+        hipLaunchKernelGGL(k_m_knn, dim3((unsigned)((max_na + 255) / 256), n_pair), dim3(256), 0, st, b);
+    FSX_LAUNCH_CHECK();
+    if (ev) FSX_HIP(hipEventRecord(ev[1], st));
+    if (nq > 0) {
+        if (form == 0)
+            hipLaunchKernelGGL(k_m_plain, dim3((unsigned)((max_na + 255) / 256), n_pair), dim3(256), 0, st, b);
+        else
+            hipLaunchKernelGGL(k_m_mfma, dim3((unsigned)info[FSX_MATCH_INFO_LEN - 1], n_pair), dim3(64), 0, st, b);
+        FSX_LAUNCH_CHECK();
+    }
+    if (ev) FSX_HIP(hipEventRecord(ev[2], st));
+    hipLaunchKernelGGL(k_m_compact, dim3((unsigned)n_pair), dim3(CP_THREADS), 0, st, b);
+    FSX_LAUNCH_CHECK();
+    if (ev) FSX_HIP(hipEventRecord(ev[3], st));
+    return FSX_OK;
+}
+
+
+// ---- matcher without allocation ---------------------------------------------------------
+// fsx_match_batch_noalloc keeps its workspace in this file-scope array (one copy per GPU,
+// held until the process ends) and writes its results straight into the caller's buffers.
+// The host never uses the array's name as an address: scratch_base() looks the device copy
+// up once per device with hipGetSymbolAddress and keeps that pointer.
+constexpr size_t MATCH_SCRATCH_BYTES = (size_t)32 << 20;
+constexpr int NOALLOC_MAX_IMG = 64;
+constexpr int NOALLOC_MAX_PAIR = 256;
+constexpr int MAX_DEVICES = 64;
+static __device__ __align__(256) unsigned char scratch[MATCH_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[MAX_DEVICES];
+
+static int scratch_base(void** out) {
+    int dev = -1;
+    FSX_HIP(hipGetDevice(&dev));
+    FSX_REQUIRE(dev >= 0 && dev < MAX_DEVICES, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        FSX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        FSX_REQUIRE(p != nullptr, "no device address for the matcher scratch");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return FSX_OK;
+}
+
+// Host tables reach the device as kernel arguments (no staging buffer, no host wait): each
+// launch of k_put_bytes stores up to PUT_CHUNK bytes of its argument at base + off.
+constexpr int PUT_CHUNK = 2016;
+struct PutBytes {
+    unsigned long long off;
+    int n, pad;
+    unsigned char b[PUT_CHUNK];
+};
+
+__global__ __launch_bounds__(256) void k_put_bytes(unsigned char* base, const PutBytes P) {
+    for (int i = threadIdx.x; i < P.n; i += blockDim.x) base[P.off + i] = P.b[i];
+}
+
+static int put_bytes(unsigned char* base, size_t off, const void* src, size_t n, hipStream_t st) {
+    PutBytes P;
+    memset(&P, 0, sizeof(P));
+    for (size_t o = 0; o < n; o += PUT_CHUNK) {
+        const size_t k = std::min(n - o, (size_t)PUT_CHUNK);
+        P.off = off + o;
+        P.n = (int)k;
+        memcpy(P.b, (const char*)src + o, k);
+        hipLaunchKernelGGL(k_put_bytes, dim3(1), dim3(256), 0, st, base, P);
+        FSX_LAUNCH_CHECK();
+    }
+    return FSX_OK;
+}
+
 }  // namespace fsx
 
 using namespace fsx;
@@ -1080,24 +1212,7 @@ int fsx_match_prepare(int n_img, const int* n_kp, const int* W, const int* H, co
     if (rc) return rc;
     if (out_offsets)
         for (int p = 0; p < n_pair; p++) out_offsets[p] = M.pair[p].o_off;
-    int max_na = 0, max_chunks = 0;
-    for (int p = 0; p < n_pair; p++) {
-        max_na = std::max(max_na, M.img[M.pair[p].a].n);
-        max_chunks = std::max(max_chunks, M.img[M.pair[p].a].chunk_cap);
-    }
-    const long long extra =
-        std::min<long long>(std::max<long long>(1, (std::max(M.n_targets, M.n_kp) + SU_THREADS - 1) / SU_THREADS), 4096);
-    memset(launch_info, 0, 8 * FSX_MATCH_INFO_LEN);
-    launch_info[0] = 0x6673786d;     // tag checked by fsx_match_launch
-    launch_info[1] = n_img;
-    launch_info[2] = n_pair;
-    launch_info[3] = M.n_query;
-    launch_info[4] = max_na;
-    launch_info[5] = n_img + extra;
-    launch_info[6] = kc;
-    launch_info[7] = (long long)(intptr_t)ws;
-    launch_info[FSX_MATCH_INFO_LEN - 1] = max_chunks;
-    memcpy(launch_info + 8, &b, sizeof(b));
+    fill_launch_info(M, b, kc, ws, launch_info);
     return FSX_OK;
 }
 
@@ -1105,32 +1220,8 @@ int fsx_match_launch(void* ws, const long long* info, int form, void* stage_even
     FSX_REQUIRE(ws && info, "null pointer argument");
     FSX_REQUIRE(info[0] == 0x6673786d, "launch_info was not filled by fsx_match_prepare");
     FSX_REQUIRE(form == 0 || form == 1, "form must be 0 (plain FP32) or 1 (MFMA)");
-    MatchBatch b;
-    memcpy(&b, info + 8, sizeof(b));
     FSX_REQUIRE(info[7] == (long long)(intptr_t)ws, "launch_info belongs to another workspace");
-    hipStream_t st = (hipStream_t)stream;
-    const long long nq = info[3];
-    const int n_pair = (int)info[2], max_na = (int)info[4];
-    hipEvent_t* ev = (hipEvent_t*)stage_events_v;
-    if (ev) FSX_HIP(hipEventRecord(ev[0], st));
-    hipLaunchKernelGGL(k_m_setup, dim3((unsigned)info[5]), dim3(SU_THREADS), 0, st, b);
-    FSX_LAUNCH_CHECK();
-    if (nq > 0 && info[6] > 0)
-        hipLaunchKernelGGL(k_m_knn, dim3((unsigned)((max_na + 255) / 256), n_pair), dim3(256), 0, st, b);
-    FSX_LAUNCH_CHECK();
-    if (ev) FSX_HIP(hipEventRecord(ev[1], st));
-    if (nq > 0) {
-        if (form == 0)
-            hipLaunchKernelGGL(k_m_plain, dim3((unsigned)((max_na + 255) / 256), n_pair), dim3(256), 0, st, b);
-        else
-            hipLaunchKernelGGL(k_m_mfma, dim3((unsigned)info[FSX_MATCH_INFO_LEN - 1], n_pair), dim3(64), 0, st, b);
-        FSX_LAUNCH_CHECK();
-    }
-    if (ev) FSX_HIP(hipEventRecord(ev[2], st));
-    hipLaunchKernelGGL(k_m_compact, dim3((unsigned)n_pair), dim3(CP_THREADS), 0, st, b);
-    FSX_LAUNCH_CHECK();
-    if (ev) FSX_HIP(hipEventRecord(ev[3], st));
-    return FSX_OK;
+    return launch_sequence(info, form, (hipEvent_t*)stage_events_v, (hipStream_t)stream);
 }
 
 int fsx_match_batch(int n_img, const int* n_kp, const int* W, const int* H, const uint8_t* desc_dev,
@@ -1272,6 +1363,54 @@ int fsx_match_query_state(int n_img, const int* n_kp, const int* W, const int* H
     }
     FSX_HIP(hipStreamSynchronize(st));
     return FSX_OK;
+}
+
+
+// fsx_match_batch with no allocation of any kind (see fsx.h): plans into stack tables, keeps
+// its workspace in the per-device scratch array, and points the compaction's outputs and the
+// flag words at the caller's buffers.
+int fsx_match_batch_noalloc(int n_img, const int* n_kp, const int* W, const int* H, const uint8_t* desc_dev,
+                            const double* xy_dev, int n_pair, const int* pa, const int* pb, const int* dense,
+                            const double* Hm, double radius, int kc, double ratio, int form, int* q_dev, int* t_dev,
+                            int* counts_dev, int* flags_dev, long long* out_offsets, void* stream) {
+    FSX_REQUIRE(desc_dev && xy_dev && n_kp && W && H && pa && pb && q_dev && t_dev && counts_dev && flags_dev,
+                "null pointer argument");
+    FSX_REQUIRE(form == 0 || form == 1, "form must be 0 (plain FP32) or 1 (MFMA)");
+    FSX_REQUIRE(kc >= 0 && kc <= KMAX, "kc must be in [0, %d]", KMAX);
+    FSX_REQUIRE(n_img >= 1 && n_img <= NOALLOC_MAX_IMG, "n_img must be in [1, %d] here", NOALLOC_MAX_IMG);
+    FSX_REQUIRE(n_pair >= 1 && n_pair <= NOALLOC_MAX_PAIR, "n_pair must be in [1, %d] here", NOALLOC_MAX_PAIR);
+    MImage img_tab[NOALLOC_MAX_IMG];
+    MPair pair_tab[NOALLOC_MAX_PAIR];
+    MatchPlanHost M;
+    M.img.ext = img_tab;
+    M.img.cap = NOALLOC_MAX_IMG;
+    M.pair.ext = pair_tab;
+    M.pair.cap = NOALLOC_MAX_PAIR;
+    int rc = match_plan(n_img, n_kp, W, H, n_pair, pa, pb, dense, Hm, radius, M);
+    if (rc) return rc;
+    // everything below the flag words lives in scratch; flags and outputs are the caller's
+    FSX_REQUIRE(M.off_cnt <= MATCH_SCRATCH_BYTES,
+                "batch needs %zu bytes of matcher scratch, above the %zu this entry point holds "
+                "(fsx_match_batch takes a workspace of any size)", M.off_cnt, MATCH_SCRATCH_BYTES);
+    void* base = nullptr;
+    rc = scratch_base(&base);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    unsigned char* w = (unsigned char*)base;
+    MatchBatch b = batch_ptrs(M, (char*)w, desc_dev, xy_dev, radius, kc, ratio);
+    b.counters = flags_dev;
+    b.out_q = q_dev;
+    b.out_t = t_dev;
+    b.out_n = counts_dev;
+    rc = put_bytes(w, M.off_img, img_tab, sizeof(MImage) * (size_t)n_img, st);
+    if (rc) return rc;
+    rc = put_bytes(w, M.off_pair, pair_tab, sizeof(MPair) * (size_t)n_pair, st);
+    if (rc) return rc;
+    if (out_offsets)
+        for (int p = 0; p < n_pair; p++) out_offsets[p] = M.pair[p].o_off;
+    long long info[FSX_MATCH_INFO_LEN];
+    fill_launch_info(M, b, kc, base, info);
+    return launch_sequence(info, form, nullptr, st);
 }
 
 }  // extern "C"

@@ -173,6 +173,23 @@ int fsx_match_results(int n_img, const int *n_kp, const int *W, const int *H, in
                       const int *pb, double radius, const void *ws_dev, int *counts, int *q, int *t, int *flags,
                       void *stream);
 
+/* fsx_match_batch for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  No counterpart in the reference.  Same
+ * match set and arguments as fsx_match_batch, but the call allocates nothing (no device,
+ * pinned or heap memory, no hipMalloc / hipFree): the workspace is a 32 MiB array in the
+ * library's device data (one copy per GPU, held until the process ends), and the results go
+ * straight to the caller's device buffers: q_dev / t_dev (int32, sum over pairs of
+ * min(n_a, n_b) entries, pair p's list at out_offsets[p]), counts_dev (int32, n_pair) and
+ * flags_dev (int32, 4; flags_dev[2] != 0: an internal table overflowed).  Limits: n_img <= 64,
+ * n_pair <= 256, and the batch's workspace (fsx_match_workspace_bytes minus its output
+ * region) within 32 MiB; beyond them the call fails with FSX_EINVAL and a message.
+ * Asynchronous on `stream` (no host wait).  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (other streams or threads) race; serialise them. */
+int fsx_match_batch_noalloc(int n_img, const int *n_kp, const int *W, const int *H, const uint8_t *desc_dev,
+                            const double *xy_dev, int n_pair, const int *pa, const int *pb, const int *dense,
+                            const double *Hm, double radius, int kc, double ratio, int form, int *q_dev,
+                            int *t_dev, int *counts_dev, int *flags_dev, long long *out_offsets, void *stream);
+
 /* Per-query state of the last batch (synchronises): accepted best target (-1 = none), its
  * distance b1 and its grid cell (-1 = predicted outside B); arrays of sum(n_a) entries, may be
  * NULL.  For tests and diagnostics (no reference counterpart). */

@@ -65,3 +65,29 @@ def test_workspace_bytes_rejects_bad_caps():
         _lib.call("fsx_sift_workspace_bytes", 100, 100, 1000, 1000, C.byref(n))
     _lib.call("fsx_sift_workspace_bytes", 100, 100, 1000, 1024, C.byref(n))
     assert n.value > 4 * 200 * 200 * 9
+
+
+def _noalloc_args(nk, pairs, wh=(1000, 800)):
+    n = len(nk)
+    ci = lambda v: (C.c_int * max(len(v), 1))(*v)   # noqa: E731
+    hm = (C.c_double * (9 * max(len(pairs), 1)))()
+    fake = C.c_void_p(256)      # never dereferenced: every rejection below happens on the host
+    return (n, ci(nk), ci([wh[0]] * n), ci([wh[1]] * n), fake, fake, len(pairs), ci([a for a, _ in pairs]),
+            ci([b for _, b in pairs]), ci([1] * len(pairs)), hm, 40.0, 0, 0.8, 1, fake, fake, fake, fake,
+            (C.c_longlong * max(len(pairs), 1))(), None)
+
+
+def test_noalloc_matcher_rejects_what_exceeds_its_limits():
+    """fsx_match_batch_noalloc refuses too many pairs / images and batches whose workspace
+    exceeds its 32 MiB scratch, with a message, before any device work."""
+    from fieldstitch_amd import _lib
+    with pytest.raises(_lib.FsxError, match="n_pair must be in"):
+        _lib.call("fsx_match_batch_noalloc", *_noalloc_args([100, 100], [(0, 1)] * 257))
+    with pytest.raises(_lib.FsxError, match="n_img must be in"):
+        _lib.call("fsx_match_batch_noalloc", *_noalloc_args([10] * 65, [(0, 1)]))
+    with pytest.raises(_lib.FsxError, match="bytes of matcher scratch"):
+        _lib.call("fsx_match_batch_noalloc", *_noalloc_args([600000, 600000], [(0, 1)]))
+    with pytest.raises(_lib.FsxError, match="null pointer"):
+        a = list(_noalloc_args([100, 100], [(0, 1)]))
+        a[15] = None
+        _lib.call("fsx_match_batch_noalloc", *a)
