@@ -91,6 +91,12 @@ SIGNATURES = {
     "fsx_ransac_fundamental": [_i, _vp, _vp, _vp, _i, C.c_double, C.c_ulonglong, _vp, _vp, _vp, _vp],
     "fsx_ransac_similarity": [_i, _vp, _vp, _vp, _i, C.c_double, C.c_ulonglong, _vp, _vp, _vp, _vp],
     "fsx_match_filter_features": [_vp, C.POINTER(C.c_longlong), _vp, _vp, _vp, _vp],
+    "fsx_match_contract": [_vp, C.POINTER(C.c_longlong), _i, _vp],
+    "fsx_match_accepted": [_vp, C.POINTER(C.c_longlong), _vp, _vp, _vp],
+    "fsx_match_filter_rows": [_vp, C.POINTER(C.c_longlong), _vp, _vp, _vp, _vp, C.c_longlong, _vp, _vp],
+    "fsx_match_publish": [_vp, C.POINTER(C.c_longlong), _vp, _vp, C.c_longlong, _vp, C.c_float, _vp],
+    "fsx_ransac_batch": [_i, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp],
+    "fsx_prior_keep_batch": [_i, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp],
     "fsx_match_query_state": [_i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), _i, C.POINTER(_i), C.POINTER(_i),
                               C.c_double, _vp, C.POINTER(_i), C.POINTER(C.c_double), C.POINTER(_i), _vp],
 }

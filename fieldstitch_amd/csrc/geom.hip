@@ -101,11 +101,11 @@ struct GroundCams {
     double C_j[3];
 };
 
-__global__ __launch_bounds__(256) void k_ground_predict(int n, const double* __restrict__ u, GroundCams P,
-                                                        double* __restrict__ pred, uint8_t* __restrict__ ok) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const double u0 = u[2 * i], u1 = u[2 * i + 1];
+static_assert(sizeof(GroundCams) == 43 * sizeof(double), "GroundCams is 43 packed doubles");
+
+// matching.py:155-172 for one undistorted pixel (u0, u1) of camera i: returns ok (t > 0 and
+// z > 0) and the predicted pixel in camera j
+__device__ __forceinline__ bool ground_pred(const GroundCams& P, double u0, double u1, double& px, double& py) {
     // rays = [u, 1] @ inv(K_i)^T ; d = rays @ R_i
     double ray[3];
 #pragma unroll
@@ -127,10 +127,39 @@ __global__ __launch_bounds__(256) void k_ground_predict(int n, const double* __r
 #pragma unroll
     for (int b = 0; b < 3; b++) q[b] = (w[0] * P.K_j[3 * b] + w[1] * P.K_j[3 * b + 1]) + w[2] * P.K_j[3 * b + 2];
     double z = q[2];
-    ok[i] = (uint8_t)((t > 0.0) && (z > 0.0));
+    const bool ok = (t > 0.0) && (z > 0.0);
     z = fabs(z) < 1e-9 ? 1e-9 : z;
-    pred[2 * i] = q[0] / z;
-    pred[2 * i + 1] = q[1] / z;
+    px = q[0] / z;
+    py = q[1] / z;
+    return ok;
+}
+
+__global__ __launch_bounds__(256) void k_ground_predict(int n, const double* __restrict__ u, GroundCams P,
+                                                        double* __restrict__ pred, uint8_t* __restrict__ ok) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    ok[i] = (uint8_t)ground_pred(P, u[2 * i], u[2 * i + 1], pred[2 * i], pred[2 * i + 1]);
+}
+
+// The prior filter of match_pair (matching.py:222-225, :238-241) for many pairs: row r of pair
+// p (rows off[p] .. off[p+1]) keeps its match iff the ground-plane prediction of ua[r] through
+// cams[p] is valid and lands within tol[p] of ub[r] (Euclidean norm, float64).  tol[p] < 0:
+// the pair uses no prior and keeps every row.
+__global__ __launch_bounds__(256) void k_prior_keep(const long long* __restrict__ off, const double* __restrict__ ua,
+                                                    const double* __restrict__ ub, const GroundCams* __restrict__ cams,
+                                                    const double* __restrict__ tol, uint8_t* __restrict__ keep) {
+    const int p = blockIdx.y;
+    const long long r = off[p] + (long long)blockIdx.x * 256 + threadIdx.x;
+    if (r >= off[p + 1]) return;
+    const double tp = tol[p];
+    if (tp < 0.0) {
+        keep[r] = 1;
+        return;
+    }
+    double px, py;
+    const bool ok = ground_pred(cams[p], ua[2 * r], ua[2 * r + 1], px, py);
+    const double dx = px - ub[2 * r], dy = py - ub[2 * r + 1];
+    keep[r] = (uint8_t)(ok && sqrt(dx * dx + dy * dy) < tp);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -306,6 +335,19 @@ int fsx_pair_select(int n, const double* xy_dev, const double* half_long_dev, in
     if (n == 0) return FSX_OK;
     hipLaunchKernelGGL(k_pair_select, dim3(n), dim3(PS_THREADS), 0, (hipStream_t)stream, n, xy_dev, half_long_dev, k,
                        has_rel, out_dev);
+    FSX_LAUNCH_CHECK();
+    return FSX_OK;
+}
+
+
+int fsx_prior_keep_batch(int n_pair, const long long* off_dev, const double* ua_dev, const double* ub_dev,
+                         const double* cams_dev, const double* tol_dev, int max_rows, uint8_t* keep_dev,
+                         void* stream) {
+    FSX_REQUIRE(n_pair >= 0 && n_pair <= 65535 && max_rows >= 0, "fsx_prior_keep_batch: bad sizes");
+    if (n_pair == 0 || max_rows == 0) return FSX_OK;
+    FSX_REQUIRE(off_dev && ua_dev && ub_dev && cams_dev && tol_dev && keep_dev, "fsx_prior_keep_batch: null pointer");
+    hipLaunchKernelGGL(k_prior_keep, dim3((unsigned)((max_rows + 255) / 256), (unsigned)n_pair), dim3(256), 0,
+                       (hipStream_t)stream, off_dev, ua_dev, ub_dev, (const GroundCams*)cams_dev, tol_dev, keep_dev);
     FSX_LAUNCH_CHECK();
     return FSX_OK;
 }

@@ -70,7 +70,7 @@ struct MatchBatch {
     double radius, r2, ratio;
     double csz;                   // grid cell size
     int kc;
-    int pad;
+    int defer;                    // 1: the contraction leaves publication to k_m_publish
     unsigned* norms;              // [Ntot] sum of squares of the descriptor
     unsigned* snorms;             // [Ntot] sum of squares of the descriptor - 128
     // cell-sorted index: positions [off, off + n) of each image in grid-cell order
@@ -406,7 +406,7 @@ __device__ __forceinline__ void accept_query(const MatchBatch& B, long long g, c
     B.cb12[2 * g + 1] = v2;
     B.best[g] = ok ? i1 : -1;
     B.b1[g] = ok ? (double)v1 : 0.0;
-    if (ok) {
+    if (ok && !B.defer) {
         const unsigned long long key = ((unsigned long long)(__float_as_uint(v1) & 0x7fffffffu) << 32) | (unsigned)q;
         atomicMin(&B.tkey[P.t_off + i1], key);
     }
@@ -805,16 +805,10 @@ __device__ __forceinline__ unsigned short f_bf16(float x) {
     return (unsigned short)((b + 0x7fffu + ((b >> 16) & 1u)) >> 16);
 }
 
-__global__ __launch_bounds__(256) void k_m_features(const MatchBatch B, const float* __restrict__ size,
-                                                    const float* __restrict__ angle, unsigned short* __restrict__ X) {
-    const MPair P = B.pair[blockIdx.y];
-    const MImage ia = B.img[P.a];
-    const MImage ib = B.img[P.b];
-    const int lane = threadIdx.x & 63;
-    const int q = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (q >= ia.n) return;
-    const long long g = P.q_off + q;
-    unsigned short* row = X + g * 288;
+// one 288-wide row per query, written by the 64 lanes of a wave
+__device__ void write_filter_row(const MatchBatch& B, const MPair& P, const MImage& ia, const MImage& ib, int q,
+                                 long long g, const float* __restrict__ size, const float* __restrict__ angle,
+                                 unsigned short* __restrict__ row, int lane) {
     const int t = B.cbest[g];
     if (t < 0) {
         for (int k = lane; k < 288; k += 64) row[k] = 0;
@@ -848,6 +842,68 @@ __global__ __launch_bounds__(256) void k_m_features(const MatchBatch B, const fl
     } else if (lane < 57) {
         row[264 + (lane - 33)] = 0;
     }
+}
+
+__global__ __launch_bounds__(256) void k_m_features(const MatchBatch B, const float* __restrict__ size,
+                                                    const float* __restrict__ angle, unsigned short* __restrict__ X) {
+    const MPair P = B.pair[blockIdx.y];
+    const MImage ia = B.img[P.a];
+    const int q = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (q >= ia.n) return;
+    const long long g = P.q_off + q;
+    write_filter_row(B, P, ia, B.img[P.b], q, g, size, angle, X + g * 288, threadIdx.x & 63);
+}
+
+// ------------------------------------------------------------------------------------------
+// Learned-filter stage (contraction with defer = 1, then these, then compaction):
+//   k_m_accepted: the queries that passed the ratio test, as (global query, pair) entries of a
+//                 list in arbitrary order (one atomic slot each);
+//   k_m_features_list: the filter's input row of each list entry (the row k_m_features writes);
+//   k_m_publish: list entries whose score is >= tau publish their (b1, query) key into the
+//                per-target minimum (the one-query-per-target rule among the kept queries);
+//                the others are marked rejected (best = -1).  score = null keeps every entry.
+
+__global__ __launch_bounds__(256) void k_m_accepted(const MatchBatch B, int2* __restrict__ list,
+                                                    int* __restrict__ count) {
+    const MPair P = B.pair[blockIdx.y];
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= B.img[P.a].n) return;
+    const long long g = P.q_off + q;
+    if (B.best[g] >= 0) {
+        const int slot = atomicAdd(count, 1);
+        list[slot] = make_int2((int)g, (int)blockIdx.y);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_m_features_list(const MatchBatch B, const float* __restrict__ size,
+                                                         const float* __restrict__ angle, const int2* __restrict__ list,
+                                                         const int* __restrict__ count, unsigned short* __restrict__ X) {
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= *count) return;
+    const int2 e = list[r];
+    const MPair P = B.pair[e.y];
+    write_filter_row(B, P, B.img[P.a], B.img[P.b], (int)(e.x - P.q_off), e.x, size, angle, X + (long long)r * 288,
+                     threadIdx.x & 63);
+}
+
+__global__ __launch_bounds__(256) void k_m_publish(const MatchBatch B, const int2* __restrict__ list,
+                                                   const int* __restrict__ count, const float* __restrict__ score,
+                                                   float tau) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= *count) return;
+    const int2 e = list[r];
+    const MPair P = B.pair[e.y];
+    const long long g = e.x;
+    const int t = B.best[g];
+    if (score && !(score[r] >= tau)) {
+        B.best[g] = -1;
+        B.b1[g] = 0.0;
+        return;
+    }
+    const float v1 = (float)B.b1[g];
+    const unsigned long long key =
+        ((unsigned long long)(__float_as_uint(v1) & 0x7fffffffu) << 32) | (unsigned)(g - P.q_off);
+    atomicMin(&B.tkey[P.t_off + t], key);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1091,9 +1147,13 @@ static void fill_launch_info(const MatchPlanHost& M, const MatchBatch& b, int kc
 }
 
 // The matcher's launches (setup, kc-nearest prefilter, contraction, compaction) on `st`.
-static int launch_sequence(const long long* info, int form, hipEvent_t* ev, hipStream_t st) {
+// contract_only: stop after the contraction, which then publishes nothing (the learned
+// filter's stage runs k_m_publish and the compaction afterwards).
+static int launch_sequence(const long long* info, int form, hipEvent_t* ev, hipStream_t st,
+                           bool contract_only = false) {
     MatchBatch b;
     memcpy(&b, info + 8, sizeof(b));
+    b.defer = contract_only ? 1 : 0;
     const long long nq = info[3];
     const int n_pair = (int)info[2], max_na = (int)info[4];
     if (ev) FSX_HIP(hipEventRecord(ev[0], st));
@@ -1111,6 +1171,7 @@ static int launch_sequence(const long long* info, int form, hipEvent_t* ev, hipS
         FSX_LAUNCH_CHECK();
     }
     if (ev) FSX_HIP(hipEventRecord(ev[2], st));
+    if (contract_only) return FSX_OK;
     hipLaunchKernelGGL(k_m_compact, dim3((unsigned)n_pair), dim3(CP_THREADS), 0, st, b);
     FSX_LAUNCH_CHECK();
     if (ev) FSX_HIP(hipEventRecord(ev[3], st));
@@ -1144,6 +1205,15 @@ static int scratch_base(void** out) {
         g_scratch_ptr[dev] = p;
     }
     *out = g_scratch_ptr[dev];
+    return FSX_OK;
+}
+
+__global__ void k_m_zero_int(int* p) { *p = 0; }
+
+static int check_info(const void* ws, const long long* info) {
+    FSX_REQUIRE(ws && info, "null pointer argument");
+    FSX_REQUIRE(info[0] == 0x6673786d, "launch_info was not filled by fsx_match_prepare");
+    FSX_REQUIRE(info[7] == (long long)(intptr_t)ws, "launch_info belongs to another workspace");
     return FSX_OK;
 }
 
@@ -1411,6 +1481,67 @@ int fsx_match_batch_noalloc(int n_img, const int* n_kp, const int* W, const int*
     long long info[FSX_MATCH_INFO_LEN];
     fill_launch_info(M, b, kc, base, info);
     return launch_sequence(info, form, nullptr, st);
+}
+
+
+// ---- learned-filter stage: contraction, accepted list, filter rows, filtered publication ----
+
+int fsx_match_contract(void* ws, const long long* info, int form, void* stream) {
+    int rc = check_info(ws, info);
+    if (rc) return rc;
+    FSX_REQUIRE(form == 0 || form == 1, "form must be 0 (plain FP32) or 1 (MFMA)");
+    return launch_sequence(info, form, nullptr, (hipStream_t)stream, true);
+}
+
+int fsx_match_accepted(void* ws, const long long* info, int* list_dev, int* count_dev, void* stream) {
+    int rc = check_info(ws, info);
+    if (rc) return rc;
+    FSX_REQUIRE(list_dev && count_dev, "null pointer argument");
+    MatchBatch b;
+    memcpy(&b, info + 8, sizeof(b));
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_m_zero_int, dim3(1), dim3(1), 0, st, count_dev);
+    FSX_LAUNCH_CHECK();
+    const int n_pair = (int)info[2], max_na = (int)info[4];
+    if (info[3] == 0 || max_na == 0) return FSX_OK;
+    hipLaunchKernelGGL(k_m_accepted, dim3((unsigned)((max_na + 255) / 256), n_pair), dim3(256), 0, st, b,
+                       (int2*)list_dev, count_dev);
+    FSX_LAUNCH_CHECK();
+    return FSX_OK;
+}
+
+int fsx_match_filter_rows(void* ws, const long long* info, const float* size_dev, const float* angle_dev,
+                          const int* list_dev, const int* count_dev, long long cap, void* X_dev, void* stream) {
+    int rc = check_info(ws, info);
+    if (rc) return rc;
+    FSX_REQUIRE(size_dev && angle_dev && list_dev && count_dev && X_dev, "null pointer argument");
+    FSX_REQUIRE(cap >= 0 && cap <= info[3], "row capacity %lld outside [0, %lld]", cap, info[3]);
+    if (cap == 0) return FSX_OK;
+    MatchBatch b;
+    memcpy(&b, info + 8, sizeof(b));
+    hipLaunchKernelGGL(k_m_features_list, dim3((unsigned)((cap + 3) / 4)), dim3(256), 0, (hipStream_t)stream, b,
+                       size_dev, angle_dev, (const int2*)list_dev, count_dev, (unsigned short*)X_dev);
+    FSX_LAUNCH_CHECK();
+    return FSX_OK;
+}
+
+int fsx_match_publish(void* ws, const long long* info, const int* list_dev, const int* count_dev, long long cap,
+                      const float* score_dev, float tau, void* stream) {
+    int rc = check_info(ws, info);
+    if (rc) return rc;
+    FSX_REQUIRE(list_dev && count_dev, "null pointer argument");
+    FSX_REQUIRE(cap >= 0 && cap <= info[3], "list capacity %lld outside [0, %lld]", cap, info[3]);
+    MatchBatch b;
+    memcpy(&b, info + 8, sizeof(b));
+    hipStream_t st = (hipStream_t)stream;
+    if (cap > 0) {
+        hipLaunchKernelGGL(k_m_publish, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, st, b,
+                           (const int2*)list_dev, count_dev, score_dev, tau);
+        FSX_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_m_compact, dim3((unsigned)info[2]), dim3(CP_THREADS), 0, st, b);
+    FSX_LAUNCH_CHECK();
+    return FSX_OK;
 }
 
 }  // extern "C"

@@ -260,17 +260,25 @@ __device__ bool hypothesis(const double* A, const double* Bq, unsigned long long
     return M::solve(xa, ya, xb, yb, m);
 }
 
+// seeds / thrs (device, per pair; may be null): pair p draws with (seeds[p], 0, h) instead of
+// (seed, p, h) and tests against thrs[p]^2 instead of thr2, so its result depends on its own
+// seed only, not on its position in the batch.
 template <class M>
 __global__ __launch_bounds__(256) void k_ransac(const long long* __restrict__ off, const double* __restrict__ pa,
-                                                const double* __restrict__ pb, int n_hyp, double thr2,
-                                                unsigned long long seed, uint8_t* __restrict__ inlier,
+                                                const double* __restrict__ pb, int n_hyp, double thr2_all,
+                                                unsigned long long seed_all, const unsigned long long* __restrict__ seeds,
+                                                const double* __restrict__ thrs, uint8_t* __restrict__ inlier,
                                                 double* __restrict__ H_out, int* __restrict__ n_in) {
     __shared__ int s_cnt[256], s_hyp[256];
     __shared__ double s_h[9];
     __shared__ int s_have;
-    const int p = blockIdx.x, tid = threadIdx.x;
-    const long long o = off[p];
-    const int n = (int)(off[p + 1] - o);
+    const int tid = threadIdx.x;
+    const int pb_ = blockIdx.x;                                  // slot in the batch
+    const int p = seeds ? 0 : pb_;                               // pair index of the draws
+    const unsigned long long seed = seeds ? seeds[pb_] : seed_all;
+    const double thr2 = thrs ? thrs[pb_] * thrs[pb_] : thr2_all;
+    const long long o = off[pb_];
+    const int n = (int)(off[pb_ + 1] - o);
     const double* A = pa + 2 * o;
     const double* Bq = pb + 2 * o;
     int best = -1, best_h = 0x7fffffff;
@@ -301,8 +309,8 @@ __global__ __launch_bounds__(256) void k_ransac(const long long* __restrict__ of
         const bool have = bc >= 0 && hypothesis<M>(A, Bq, seed, p, bh, n, h);
         for (int k = 0; k < 9; k++) s_h[k] = have ? h[k] : 0.0;
         s_have = have;
-        n_in[p] = have ? bc : 0;
-        for (int k = 0; k < 9; k++) H_out[9 * p + k] = s_h[k];
+        n_in[pb_] = have ? bc : 0;
+        for (int k = 0; k < 9; k++) H_out[9 * pb_ + k] = s_h[k];
     }
     __syncthreads();
     const bool have = s_have != 0;
@@ -324,7 +332,21 @@ static int launch_ransac(int n_pair, const long long* off_dev, const double* pa_
     if (n_pair == 0) return FSX_OK;
     FSX_REQUIRE(off_dev && pa_dev && pb_dev && inlier_dev && H_dev && n_in_dev, "null pointer argument");
     hipLaunchKernelGGL(k_ransac<M>, dim3(n_pair), dim3(256), 0, (hipStream_t)stream, off_dev, pa_dev, pb_dev, n_hyp,
-                       thr * thr, seed, inlier_dev, H_dev, n_in_dev);
+                       thr * thr, seed, nullptr, nullptr, inlier_dev, H_dev, n_in_dev);
+    FSX_LAUNCH_CHECK();
+    return FSX_OK;
+}
+
+template <class M>
+static int launch_ransac_per_pair(int n_pair, const long long* off_dev, const double* pa_dev, const double* pb_dev,
+                                  int n_hyp, const double* thr_dev, const unsigned long long* seed_dev,
+                                  uint8_t* inlier_dev, double* M_dev, int* n_in_dev, void* stream) {
+    FSX_REQUIRE(n_pair >= 0 && n_hyp >= 1 && n_hyp <= (1 << 20), "bad RANSAC arguments");
+    if (n_pair == 0) return FSX_OK;
+    FSX_REQUIRE(off_dev && pa_dev && pb_dev && thr_dev && seed_dev && inlier_dev && M_dev && n_in_dev,
+                "null pointer argument");
+    hipLaunchKernelGGL(k_ransac<M>, dim3(n_pair), dim3(256), 0, (hipStream_t)stream, off_dev, pa_dev, pb_dev, n_hyp,
+                       0.0, 0ull, seed_dev, thr_dev, inlier_dev, M_dev, n_in_dev);
     FSX_LAUNCH_CHECK();
     return FSX_OK;
 }
@@ -355,6 +377,29 @@ int fsx_ransac_similarity(int n_pair, const long long* off_dev, const double* pa
                           int* n_in_dev, void* stream) {
     return launch_ransac<ModelS>(n_pair, off_dev, pa_dev, pb_dev, n_hyp, thr, seed, inlier_dev, M_dev, n_in_dev,
                                  stream);
+}
+
+
+// One RANSAC launch for pairs that each carry their own threshold and seed (device arrays
+// thr_dev, seed_dev of n_pair entries): model 0 = homography, 1 = fundamental matrix,
+// 2 = similarity.  Pair p draws hypothesis h from the counter hash of (seed_dev[p], 0, h), so
+// its result is the one a single-pair call with that seed gives, wherever p sits in the batch.
+int fsx_ransac_batch(int model, int n_pair, const long long* off_dev, const double* pa_dev, const double* pb_dev,
+                     int n_hyp, const double* thr_dev, const unsigned long long* seed_dev, uint8_t* inlier_dev,
+                     double* M_dev, int* n_in_dev, void* stream) {
+    switch (model) {
+        case 0:
+            return launch_ransac_per_pair<ModelH>(n_pair, off_dev, pa_dev, pb_dev, n_hyp, thr_dev, seed_dev,
+                                                  inlier_dev, M_dev, n_in_dev, stream);
+        case 1:
+            return launch_ransac_per_pair<ModelF>(n_pair, off_dev, pa_dev, pb_dev, n_hyp, thr_dev, seed_dev,
+                                                  inlier_dev, M_dev, n_in_dev, stream);
+        case 2:
+            return launch_ransac_per_pair<ModelS>(n_pair, off_dev, pa_dev, pb_dev, n_hyp, thr_dev, seed_dev,
+                                                  inlier_dev, M_dev, n_in_dev, stream);
+        default:
+            FSX_REQUIRE(false, "model must be 0 (homography), 1 (fundamental) or 2 (similarity)");
+    }
 }
 
 }  // extern "C"

@@ -1,9 +1,11 @@
 """Feature extraction front end (replaces the GPU side of app/planesite/features.py).
 
-* ``sift_gpu(g2, nfeat)`` — drop-in of features.sift_gpu (features.py:115-129): detect
-  ``int(nfeat*2.2)`` keypoints, grid-spread to ``nfeat``, sort by response, describe.
-* ``_spread`` / ``_save`` — the reference's grid spread (features.py:38-57) and RootSIFT npz
-  writer (features.py:74-86), same formats.
+* ``sift_gpu(g2, nfeat)`` — drop-in of features.sift_gpu (features.py:115-129) on a one-photo
+  FeatureBatch: detect ``int(nfeat*2.2)`` keypoints, grid-spread to ``nfeat`` on the device,
+  sort by response, describe.
+* ``spread_select`` / ``_spread`` — the grid spread (features.py:38-57) on arrays / keypoint
+  objects; ``rootsift_u8`` / ``_save`` — the RootSIFT encoding and the feature file
+  (features.py:74-86, same keys, plus optional keypoint size / angle).
 * ``FeatureBatch`` — the whole flight on the GPU without host round trips: one call detects,
   spreads and describes every image (device-side spread, RootSIFT uint8 epilogue), with the
   pyramid launched once per (octave, layer) for all images of a shape.
@@ -12,78 +14,119 @@ from __future__ import annotations
 
 import ctypes as C
 import time
+from collections import OrderedDict
 
 import numpy as np
 
 from . import _lib, gsift
 
 
-class _KP:
-    __slots__ = ("pt", "response")
-
-    def __init__(self, x, y, r):
-        self.pt = (x, y)
-        self.response = r
+def spread_select(x, y, resp, W, H, n, gx=12, gy=9):
+    """The grid spread of features.py:38-57 on arrays: indices of at most n keypoints.  Each
+    of the gx x gy cells contributes its ceil(n / (gx gy)) strongest keypoints, listed cell by
+    cell (row-major cells, strongest first inside a cell); when that is short of n, the
+    strongest of the rest follow.  Equal responses keep detection order."""
+    m = len(resp)
+    if m <= n:
+        return np.arange(m, dtype=np.int64)
+    resp = np.asarray(resp, np.float32)
+    x = np.asarray(x, np.float32)
+    y = np.asarray(y, np.float32)
+    # cell of each keypoint, in float32 as the reference computes it (x / W * gx, truncated)
+    cell = (np.clip((y / H * gy).astype(np.int64), 0, gy - 1) * gx +
+            np.clip((x / W * gx).astype(np.int64), 0, gx - 1))
+    strongest = np.argsort(-resp, kind="stable")                       # response order, ties by index
+    by_cell = strongest[np.argsort(cell[strongest], kind="stable")]    # cell, then response order
+    per_cell = np.bincount(cell, minlength=gx * gy)
+    first_in_cell = np.cumsum(per_cell) - per_cell
+    rank = np.arange(m) - first_in_cell[cell[by_cell]]
+    quota = -(-n // (gx * gy))
+    chosen = by_cell[rank < quota]
+    if len(chosen) < n:
+        taken = np.zeros(m, bool)
+        taken[chosen] = True
+        chosen = np.concatenate([chosen, strongest[~taken[strongest]][:n - len(chosen)]])
+    return chosen[:n].astype(np.int64)
 
 
 def _spread(kps, W, H, n, gx=12, gy=9):
-    """Keep n keypoints spread over a gx x gy grid, best responses per cell first
-    (features.py:38-57).  Ties in response among the fill-up keypoints keep detection order."""
-    if len(kps) <= n:
-        return list(range(len(kps)))
+    """features.py:38-57 drop-in: the spread of keypoint objects with .pt and .response
+    (cv2.KeyPoint or alike), as a list of indices."""
+    pts = np.array([k.pt for k in kps], np.float32).reshape(-1, 2)
     resp = np.array([k.response for k in kps], np.float32)
-    xy = np.array([k.pt for k in kps], np.float32)
-    cx = np.clip((xy[:, 0] / W * gx).astype(int), 0, gx - 1)
-    cy = np.clip((xy[:, 1] / H * gy).astype(int), 0, gy - 1)
-    cell = cy * gx + cx
-    order = np.lexsort((-resp, cell))
-    cs = cell[order]
-    first = np.searchsorted(cs, cs, side="left")
-    rank = np.arange(len(cs)) - first
-    cap = int(np.ceil(n / (gx * gy)))
-    sel = order[rank < cap]
-    if len(sel) < n:
-        rest = np.setdiff1d(np.arange(len(kps)), sel)
-        rest = rest[np.argsort(-resp[rest], kind="stable")][: n - len(sel)]
-        sel = np.concatenate([sel, rest])
-    return sel[:n].tolist()
+    return spread_select(pts[:, 0], pts[:, 1], resp, W, H, n, gx, gy).tolist()
+
+
+_SIFT_BATCHES = OrderedDict()
+
+
+def _single_batch(shape, nfeat, caps=None):
+    """A one-photo FeatureBatch for sift_gpu (gsift.detect's thresholds, float descriptors
+    kept), cached per (shape, nfeat, caps)."""
+    key = (tuple(shape), int(nfeat), caps)
+    fb = _SIFT_BATCHES.pop(key, None)
+    if fb is None:
+        fb = FeatureBatch([shape], nfeat=nfeat, contrast=0.015, edge=12.0, float_desc=True, caps=caps)
+    _SIFT_BATCHES[key] = fb
+    while len(_SIFT_BATCHES) > 2:
+        _SIFT_BATCHES.popitem(last=False)
+    return fb
 
 
 def sift_gpu(g2, nfeat, gpu=True):
-    """Same outputs as features.sift_gpu: (xy f32 (n,2), desc f32 (n,128) or None, resp f32 (n,),
-    describe time)."""
-    h, w = g2.shape[:2]
-    kps, state = gsift.detect(g2, gpu=gpu, max_kp=int(nfeat * 2.2))
-    n = len(kps["x"])
-    if n < 10:
+    """features.py:115-129 drop-in on the batched pipeline: one FeatureBatch run detects
+    int(2.2 nfeat) keypoints (contrast 0.015, edge 12), spreads them to nfeat on the device,
+    orders them by response and describes them.  Returns (xy float32 (n, 2), desc float32
+    (n, 128) or None, resp float32 (n,), seconds spent in spread + describe)."""
+    if not gpu:
+        raise ValueError("fieldstitch_amd.features.sift_gpu runs on the GPU only")
+    import torch
+    img = g2 if isinstance(g2, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(g2, np.uint8))
+    img = img.cuda() if not img.is_cuda else img
+    caps = None
+    while True:
+        fb = _single_batch(tuple(img.shape), nfeat, caps)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(8)]
+        fb.run([img], events=ev)
+        try:
+            fb.check_capacity()
+            break
+        except RuntimeError:
+            # a capacity overflowed: double both capacities and run again
+            caps = tuple(2 * c for c in (caps or (fb.cand_cap, fb.kp_cap)))
+    t = FeatureBatch.stage_times(ev)
+    n_det, _ = fb.counts()
+    if n_det[0] < 10:
         return np.zeros((0, 2), np.float32), None, np.zeros(0, np.float32), 0.0
-    t = time.perf_counter()
-    lst = [_KP(float(x), float(y), float(r)) for x, y, r in zip(kps["x"], kps["y"], kps["response"])]
-    sel = np.array(_spread(lst, w, h, nfeat), np.int64)
-    sel = sel[np.argsort(-kps["response"][sel], kind="stable")]
-    desc = gsift.describe(state, kps, sel, budget=4e6)
-    xy = np.c_[kps["x"][sel], kps["y"][sel]].astype(np.float32)
-    return xy, desc, kps["response"][sel].astype(np.float32), time.perf_counter() - t
+    xy, _, resp, desc = fb.results(0)
+    return xy, desc, resp, (t["spread"] + t["describe"]) / 1e3
 
 
 def rootsift_u8(desc):
-    """features.py:80-82 (host numpy): L1-normalise, sqrt, x512, clip, truncate to uint8."""
-    desc = desc / (np.abs(desc).sum(1, keepdims=True) + 1e-7)
-    desc = np.sqrt(desc)
-    return np.clip(desc * 512.0, 0, 255).astype(np.uint8)
+    """The RootSIFT encoding of features.py:80-82 in float32: each row over its L1 norm (+1e-7),
+    square root, times 512, clipped to [0, 255], truncated to uint8."""
+    d = np.asarray(desc, np.float32)
+    l1 = np.add.reduce(np.abs(d), axis=1, keepdims=True) + np.float32(1e-7)
+    return np.clip(np.sqrt(d / l1) * np.float32(512.0), 0, 255).astype(np.uint8)
 
 
-def _save(out_npz, xy, desc, resp, w, h, desc_u8=None):
-    """RootSIFT-encode descriptors and write the feature file (features.py:74-86 format).
-    ``desc_u8`` (already encoded on the GPU) skips the host encoding."""
-    if (desc is None and desc_u8 is None) or len(xy) < 10:
-        np.savez(out_npz, xy=np.zeros((0, 2), np.float32), desc=np.zeros((0, 128), np.uint8),
-                 resp=np.zeros(0, np.float32), red_wh=np.array([w, h], np.int32))
-        return 0
-    u8 = desc_u8 if desc_u8 is not None else rootsift_u8(desc)
-    np.savez(out_npz, xy=np.asarray(xy, np.float32), desc=u8, resp=np.asarray(resp, np.float32),
-             red_wh=np.array([w, h], np.int32))
-    return len(xy)
+def _save(out_npz, xy, desc, resp, w, h, desc_u8=None, size=None, angle=None):
+    """Write one photo's feature file in the format of features.py:74-86: xy float32 (n, 2)
+    reduced-image pixels, desc uint8 RootSIFT (n, 128), resp float32 (n,), red_wh int32 [w, h];
+    fewer than 10 features (or no descriptors) give empty arrays.  desc_u8: rows already
+    encoded on the GPU.  size / angle (optional, float32 (n,)) are stored too for the learned
+    match filter; readers of the reference format ignore them.  Returns the feature count."""
+    have = (desc is not None or desc_u8 is not None) and len(xy) >= 10
+    n = len(xy) if have else 0
+    fields = {"xy": np.asarray(xy, np.float32).reshape(-1, 2)[:n],
+              "desc": (desc_u8 if desc_u8 is not None else rootsift_u8(desc)) if have else np.zeros((0, 128), np.uint8),
+              "resp": np.asarray(resp, np.float32)[:n] if have else np.zeros(0, np.float32),
+              "red_wh": np.array([w, h], np.int32)}
+    if have and size is not None and angle is not None:
+        fields["size"] = np.asarray(size, np.float32)[:n]
+        fields["angle"] = np.asarray(angle, np.float32)[:n]
+    np.savez(out_npz, **fields)
+    return n
 
 
 class FeatureBatch:
@@ -235,6 +278,15 @@ class FeatureBatch:
         d = self.desc[i, :m].cpu().numpy() if self.desc is not None else None
         return xy, u8, resp, d
 
+    def size_angle(self, i):
+        """Keypoint size and angle (float32 host arrays) of image i's selected features, in the
+        order of results(i)."""
+        _, m = self.counts()
+        m = int(m[i])
+        kp = self.kp[i, 2:4].cpu().numpy()
+        sel = self._sel(i, m)
+        return kp[0][sel].astype(np.float32), kp[1][sel].astype(np.float32)
+
     def selection_device(self, i):
         """The spread selection of image i as a device int64 tensor of nfeat entries (entries at
         and beyond the selected count are stale)."""
@@ -356,7 +408,8 @@ def gpu_worker_main(task_q, res_q, n_threads, wid, batch=16, decode=read_gray, c
         fb.check_capacity()
         for k, (task, g2, tt, _) in enumerate(group):
             xy, u8, resp, _ = fb.results(k)
-            n = _save(task[2], xy, None, resp, g2.shape[1], g2.shape[0], desc_u8=u8)
+            size, angle = fb.size_angle(k)
+            n = _save(task[2], xy, None, resp, g2.shape[1], g2.shape[0], desc_u8=u8, size=size, angle=angle)
             tt["gpu_detect"] = (time.perf_counter() - t2) / len(group)
             tt["gpu_describe"] = 0.0
             res_q.put(("ok", (task[0], n, None, tt)))

@@ -1,0 +1,417 @@
+"""match_pair for many pairs at once (the per-pair process pool of pipeline.py:535-548 over
+matching.py:205-253, as one batched launch sequence per stage).
+
+Every pair goes through the reference's decision chain; what changes is that each stage runs
+for all the pairs that reach it in one batch:
+
+  1. coarse      the n_coarse strongest features of both photos, exhaustive ratio matching at
+                 0.85 (matching.py:219-220): one dense PairMatcher batch;
+  2. prior       the ground-plane prior filter (matching.py:221-225): one fsx_prior_keep_batch;
+  3. homography  RANSAC on pairs with >= 12 coarse matches, threshold 0.012 max(W_i, H_i)
+                 (matching.py:226): one fsx_ransac_batch, a seed per pair;
+  4. guided      pairs whose homography has >= 10 inliers: guided matching in a window of
+                 guided_radius max(W_j, H_j) around the homography's prediction, the guided_k
+                 nearest, ratio 0.85 (matching.py:227-229): one PairMatcher batch per window
+                 size, optionally pruned by the learned match filter;
+  5. verify      pairs with >= min_inliers guided matches: fundamental-matrix RANSAC at
+                 ransac_thr; success needs max(min_inliers, 40) inliers (matching.py:230-233);
+  6. fallback    the rest: exhaustive ratio matching at `ratio` (the exact set FLANN
+                 approximates, matching.py:236), prior filter, >= 20 survivors, verification
+                 with success at min_inliers (matching.py:236-246);
+  7. similarity  RANSAC of a 2-point similarity at 0.02 max(W_i, H_i), then its least-squares
+                 fit on the inliers (matching.py:194-202).
+
+Results are the reference's tuples: (i, j, q int32, t int32, sim, n_inliers), or
+(i, j, None, None, None, n) when matching fails.  A stage that raises sends its pairs through
+the chain one by one, so a failing pair reports ("error", message) in the sim slot
+(matching.py:249-253) without taking the others down.  Each pair's RANSAC draws come from a
+seed of its own (i, j), so a pair's tuple does not depend on the batch it ran in.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+import traceback
+
+import numpy as np
+
+from . import _lib
+
+H_ITERS = 2000        # cv2.findHomography's default iteration cap
+F_ITERS = 4096        # fixed count (cv2 stops at 0.999 confidence, at most 10000)
+S_ITERS = 2000        # cv2.estimateAffinePartial2D's default iteration cap
+_MODEL = {"homography": 0, "fundamental": 1, "similarity": 2}
+
+
+def pair_seed(i, j, salt):
+    """RANSAC seed of the image pair (i, j) and stage `salt`."""
+    return (0x5EED0000 ^ (int(i) * 0x9E3779B1) ^ (int(j) << 24) ^ salt) & 0xFFFFFFFFFFFFFFFF
+
+
+class FlowImage:
+    """One photo's device-resident features: RootSIFT uint8 descriptors (n, 128) in response
+    order, undistorted full-resolution positions (n, 2) float64, the photo size and, for the
+    learned filter, keypoint size and angle (float32, may be None)."""
+
+    def __init__(self, desc, und, W, H, size=None, angle=None):
+        self.desc = desc
+        self.und = und
+        self.W = int(W)
+        self.H = int(H)
+        self.size = size
+        self.angle = angle
+        self.n = int(desc.shape[0])
+        self._und_host = None
+
+    def und_host(self):
+        if self._und_host is None:
+            self._und_host = self.und.cpu().numpy()
+        return self._und_host
+
+
+class FlowConfig:
+    """The matching context's parameters (pipeline.py:478-482): per-image W / H and attitude
+    flags, the prior's cameras, and the thresholds."""
+
+    def __init__(self, ctx):
+        self.W = list(ctx["W"])
+        self.H = list(ctx["H"])
+        self.has_rot = list(ctx.get("has_rot", [False] * len(self.W)))
+        self.use_prior = bool(ctx.get("use_prior", True))
+        self.prior_tol = float(ctx.get("prior_tol", 0.18))
+        self.n_coarse = int(ctx.get("n_coarse", 1500))
+        self.guided = bool(ctx.get("guided", True))
+        self.guided_radius = float(ctx.get("guided_radius", 0.03))
+        self.guided_k = int(ctx.get("guided_k", 24))
+        self.min_inliers = int(ctx["min_inliers"])
+        self.ransac_thr = float(ctx["ransac_thr"])
+        self.f_iters = int(ctx.get("ransac_iters", F_ITERS))
+        self.ratio = float(ctx.get("ratio", 0.8))
+        self._cams = None
+        if all(k in ctx for k in ("K", "R", "C", "zg")):
+            self._cams = (ctx["K"], ctx["R"], ctx["C"], ctx["zg"])
+
+    def prior_on(self, i, j):
+        return bool(self.has_rot[i] and self.has_rot[j] and self.use_prior and self._cams is not None)
+
+    def cams(self, i, j):
+        """The 43 float64 of the ground-plane prior from camera i into camera j."""
+        K, R, Cc, zg = self._cams
+        return np.concatenate([np.linalg.inv(np.asarray(K[i], np.float64)).ravel(), np.asarray(R[i], np.float64).ravel(),
+                               np.asarray(Cc[i], np.float64).ravel(), [float(zg[i])],
+                               np.asarray(R[j], np.float64).ravel(), np.asarray(K[j], np.float64).ravel(),
+                               np.asarray(Cc[j], np.float64).ravel()])
+
+    def side(self, i):
+        return max(self.W[i], self.H[i])
+
+
+# ---- batched primitives ---------------------------------------------------------------------
+
+def _torch():
+    import torch
+    return torch
+
+
+def _dense_matches(images, pairs, ratio, prefix=None, form="mfma"):
+    """Exhaustive ratio matching of every (i, j) in `pairs` (the first `prefix` features of
+    each photo when given): one PairMatcher batch.  Returns [(q, t)]."""
+    from .matching import PairMatcher
+    if not pairs:
+        return []
+    torch = _torch()
+    used = sorted({i for p in pairs for i in p})
+    slot = {i: k for k, i in enumerate(used)}
+    zeros = {}
+    sets = []
+    for i in used:
+        d = images[i].desc if prefix is None else images[i].desc[:prefix]
+        n = int(d.shape[0])
+        if n not in zeros:
+            zeros[n] = torch.zeros((n, 2), dtype=torch.float64, device=d.device)
+        sets.append((d, zeros[n], (1, 1)))
+    pm = PairMatcher(sets, [(slot[i], slot[j]) for i, j in pairs], None, radius=1.0, ratio=ratio)
+    pm.run(form)
+    out = pm.results()
+    # the reference returns nothing when either side has fewer than 2 features (matching.py:88)
+    for k, (i, j) in enumerate(pairs):
+        if min(sets[slot[i]][0].shape[0], sets[slot[j]][0].shape[0]) < 2:
+            out[k] = (np.zeros(0, np.int64), np.zeros(0, np.int64))
+    return out
+
+
+def _guided_matches(images, pairs, Hms, radii, kc, ratio, form="mfma", filt=None):
+    """Guided matching of each pair under its homography, window radius radii[k]: one
+    PairMatcher batch per distinct radius.  filt = (net, tau): the learned filter prunes the
+    ratio-accepted queries scoring below tau before the one-query-per-target rule."""
+    from .matching import PairMatcher
+    out = [None] * len(pairs)
+    groups = {}
+    for k, r in enumerate(radii):
+        groups.setdefault(float(r), []).append(k)
+    for r, ks in groups.items():
+        used = sorted({i for k in ks for i in pairs[k]})
+        slot = {i: s for s, i in enumerate(used)}
+        sets = [(images[i].desc, images[i].und, (images[i].W, images[i].H)) for i in used]
+        sub = [(slot[pairs[k][0]], slot[pairs[k][1]]) for k in ks]
+        kcs = [0 if kc >= images[pairs[k][1]].n else kc for k in ks]
+        # kc >= n_b means every candidate in the window (cKDTree's k clipped to n_b)
+        for kc_val in sorted(set(kcs)):
+            ks2 = [k for k, v in zip(ks, kcs) if v == kc_val]
+            sub2 = [(slot[pairs[k][0]], slot[pairs[k][1]]) for k in ks2]
+            pm = PairMatcher(sets, sub2, [Hms[k] for k in ks2], radius=r, kc=kc_val, ratio=ratio)
+            if filt is None:
+                pm.run(form)
+            else:
+                net, tau = filt
+                pm.run_filtered(form, net, tau, [images[i].size for i in used], [images[i].angle for i in used])
+            for k, res in zip(ks2, pm.results()):
+                out[k] = res
+        del sub
+    return out
+
+
+def _concat_rows(rows):
+    lens = np.array([len(r) for r in rows], np.int64)
+    off = np.r_[0, np.cumsum(lens)].astype(np.int64)
+    data = np.concatenate(rows) if off[-1] else np.zeros((0, 2))
+    return off, np.ascontiguousarray(data, np.float64).reshape(-1, 2)
+
+
+def _prior_keep(cfg, items):
+    """items: [(i, j, ua rows, ub rows)] -> per item a bool keep mask (matching.py:221-225):
+    one fsx_prior_keep_batch for all of them."""
+    if not items:
+        return []
+    torch = _torch()
+    off, ua = _concat_rows([it[2] for it in items])
+    _, ub = _concat_rows([it[3] for it in items])
+    if off[-1] == 0:
+        return [np.zeros(0, bool) for _ in items]
+    cams = np.zeros((len(items), 43), np.float64)
+    tol = np.full(len(items), -1.0)
+    for k, (i, j, _, _) in enumerate(items):
+        if cfg.prior_on(i, j):
+            cams[k] = cfg.cams(i, j)
+            tol[k] = cfg.prior_tol * cfg.side(j)
+    dev = "cuda"
+    d_off = torch.from_numpy(off).to(dev)
+    d_ua = torch.from_numpy(ua).to(dev)
+    d_ub = torch.from_numpy(ub).to(dev)
+    d_cams = torch.from_numpy(cams).to(dev)
+    d_tol = torch.from_numpy(tol).to(dev)
+    keep = torch.empty(int(off[-1]), dtype=torch.uint8, device=dev)
+    P = C.c_void_p
+    _lib.call("fsx_prior_keep_batch", len(items), P(d_off.data_ptr()), P(d_ua.data_ptr()), P(d_ub.data_ptr()),
+              P(d_cams.data_ptr()), P(d_tol.data_ptr()), int(np.diff(off).max()), P(keep.data_ptr()),
+              P(_lib.stream_handle()))
+    k = keep.cpu().numpy().astype(bool)
+    return [k[off[n]:off[n + 1]] for n in range(len(items))]
+
+
+def ransac_batch(model, pa_list, pb_list, thr, seeds, n_hyp):
+    """RANSAC of `model` for every pair, each with its own threshold and seed (one launch).
+    Returns (inlier masks, models (n, 9), counts)."""
+    torch = _torch()
+    n = len(pa_list)
+    if n == 0:
+        return [], np.zeros((0, 9)), np.zeros(0, np.int64)
+    off, A = _concat_rows(pa_list)
+    _, B = _concat_rows(pb_list)
+    dev = "cuda"
+    tot = max(int(off[-1]), 1)
+    d_off = torch.from_numpy(off).to(dev)
+    d_a = torch.from_numpy(A if off[-1] else np.zeros((1, 2))).to(dev)
+    d_b = torch.from_numpy(B if off[-1] else np.zeros((1, 2))).to(dev)
+    d_thr = torch.from_numpy(np.asarray(thr, np.float64).reshape(n)).to(dev)
+    d_seed = torch.from_numpy(np.asarray([s & 0x7FFFFFFFFFFFFFFF if s < (1 << 63) else s - (1 << 64) for s in seeds],
+                                         np.int64)).to(dev)
+    inl = torch.zeros(tot, dtype=torch.uint8, device=dev)
+    M = torch.zeros((n, 9), dtype=torch.float64, device=dev)
+    cnt = torch.zeros(n, dtype=torch.int32, device=dev)
+    P = C.c_void_p
+    _lib.call("fsx_ransac_batch", _MODEL[model], n, P(d_off.data_ptr()), P(d_a.data_ptr()), P(d_b.data_ptr()),
+              int(n_hyp), P(d_thr.data_ptr()), P(d_seed.data_ptr()), P(inl.data_ptr()), P(M.data_ptr()),
+              P(cnt.data_ptr()), P(_lib.stream_handle()))
+    flags = inl.cpu().numpy().astype(bool)
+    return [flags[off[k]:off[k + 1]] for k in range(n)], M.cpu().numpy(), cnt.cpu().numpy().astype(np.int64)
+
+
+def similarity_fit(a, b):
+    """Least-squares similarity b ~ s R(theta) a + t of matched points (closed form on the
+    centred sets): (theta rad, s, tx, ty), or None for fewer than 2 points or points that
+    coincide."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    if len(a) < 2:
+        return None
+    ma, mb = a.mean(0), b.mean(0)
+    ac, bc = a - ma, b - mb
+    den = float((ac * ac).sum())
+    if not den > 1e-12 * len(a):        # all points within ~1e-6 px of their mean: no fit
+        return None
+    c = float((ac[:, 0] * bc[:, 0] + ac[:, 1] * bc[:, 1]).sum()) / den
+    s = float((ac[:, 0] * bc[:, 1] - ac[:, 1] * bc[:, 0]).sum()) / den
+    tx = mb[0] - (c * ma[0] - s * ma[1])
+    ty = mb[1] - (s * ma[0] + c * ma[1])
+    return (math.atan2(s, c), math.hypot(c, s), float(tx), float(ty))
+
+
+# ---- the chain ------------------------------------------------------------------------------
+
+def _error_tuple(i, j, e):
+    tb = traceback.extract_tb(e.__traceback__)
+    where = tb[-1] if tb else None
+    loc = " (%s line %d)" % (os.path.basename(where.filename), where.lineno) if where else ""
+    return (i, j, None, None, ("error", "%s: %s%s" % (type(e).__name__, e, loc)), 0)
+
+
+def run_pairs(tasks, images, cfg, form="mfma", filt=None, stats=None):
+    """The match_pair chain for every (i, j) of `tasks` over `images` (index -> FlowImage).
+    Returns the tuples in task order.  filt = (net, tau) prunes the guided stage's candidates
+    with the learned filter.  stats (dict, optional) receives per-stage pair counts."""
+    tasks = [(int(i), int(j)) for i, j in tasks]
+    try:
+        return _chain(tasks, images, cfg, form, filt, stats)
+    except Exception as e:  # noqa: BLE001 - isolate the failing pair(s), as the per-pair pool does
+        if len(tasks) == 1:
+            return [_error_tuple(tasks[0][0], tasks[0][1], e)]
+        return [run_pairs([t], images, cfg, form, filt)[0] for t in tasks]
+
+
+def _chain(tasks, images, cfg, form, filt, stats):
+    n_task = len(tasks)
+    out = [None] * n_task
+    live = []
+    for k, (i, j) in enumerate(tasks):
+        if images[i].n < 30 or images[j].n < 30:
+            out[k] = (i, j, None, None, None, 0)
+        else:
+            live.append(k)
+    und = {i: images[i].und_host() for k in live for i in tasks[k]}
+    result = {}            # task index -> (q, t, n_in)
+
+    # 1-2. coarse matches of the strongest features, prior filter
+    n1 = cfg.n_coarse
+    coarse = [k for k in live if cfg.guided and images[tasks[k][0]].n > n1 // 2]
+    cm = _dense_matches(images, [tasks[k] for k in coarse], 0.85, prefix=n1, form=form)
+    cm = dict(zip(coarse, cm))
+    pri = [k for k in coarse if cfg.prior_on(*tasks[k]) and len(cm[k][0])]
+    keeps = _prior_keep(cfg, [(tasks[k][0], tasks[k][1], und[tasks[k][0]][cm[k][0]], und[tasks[k][1]][cm[k][1]])
+                              for k in pri])
+    for k, m in zip(pri, keeps):
+        cm[k] = (cm[k][0][m], cm[k][1][m])
+
+    # 3. homography RANSAC on the coarse set
+    hk = [k for k in coarse if len(cm[k][0]) >= 12]
+    hflags, hM, _ = ransac_batch("homography", [und[tasks[k][0]][cm[k][0]] for k in hk],
+                                 [und[tasks[k][1]][cm[k][1]] for k in hk],
+                                 [0.012 * cfg.side(tasks[k][0]) for k in hk],
+                                 [pair_seed(*tasks[k], 0x1) for k in hk], H_ITERS)
+    gk = [k for k, f in zip(hk, hflags) if f.sum() >= 10]
+    Hm = {k: hM[n].reshape(3, 3) for n, k in enumerate(hk)}
+
+    # 4. guided matching under the homography
+    gm = _guided_matches(images, [tasks[k] for k in gk], [Hm[k] for k in gk],
+                         [cfg.guided_radius * cfg.side(tasks[k][1]) for k in gk], cfg.guided_k, 0.85, form, filt)
+    vk = [k for k, (q, _) in zip(gk, gm) if len(q) >= cfg.min_inliers]
+    gm = dict(zip(gk, gm))
+
+    # 5. fundamental-matrix verification of the guided matches
+    for k, (q, t, n_in) in zip(vk, _verify(cfg, tasks, und, [(k, gm[k]) for k in vk])):
+        if q is not None and n_in >= max(cfg.min_inliers, 40):
+            result[k] = (q, t, n_in)
+
+    # 6. fallback: exhaustive matching, prior filter, verification
+    fb = [k for k in live if k not in result]
+    fm = dict(zip(fb, _dense_matches(images, [tasks[k] for k in fb], cfg.ratio, form=form)))
+    pri = [k for k in fb if cfg.prior_on(*tasks[k]) and len(fm[k][0])]
+    keeps = _prior_keep(cfg, [(tasks[k][0], tasks[k][1], und[tasks[k][0]][fm[k][0]], und[tasks[k][1]][fm[k][1]])
+                              for k in pri])
+    for k, m in zip(pri, keeps):
+        fm[k] = (fm[k][0][m], fm[k][1][m])
+    enough = []
+    for k in fb:
+        if len(fm[k][0]) < 20:
+            out[k] = (tasks[k][0], tasks[k][1], None, None, None, 0)
+        else:
+            enough.append(k)
+    for k, (q, t, n_in) in zip(enough, _verify(cfg, tasks, und, [(k, fm[k]) for k in enough])):
+        if q is None or n_in < cfg.min_inliers:
+            out[k] = (tasks[k][0], tasks[k][1], None, None, None, n_in)
+        else:
+            result[k] = (q, t, n_in)
+
+    # 7. similarity of every verified set
+    rk = sorted(result)
+    sims = _similarities(cfg, tasks, und, [(k, result[k]) for k in rk])
+    for k, sim in zip(rk, sims):
+        q, t, n_in = result[k]
+        out[k] = (tasks[k][0], tasks[k][1], np.asarray(q).astype(np.int32), np.asarray(t).astype(np.int32), sim, n_in)
+    if stats is not None:
+        stats.update({"pairs": n_task, "coarse": len(coarse), "homography": len(hk), "guided": len(gk),
+                      "guided_verified": len(vk), "fallback": len(fb), "matched": len(rk)})
+    return out
+
+
+def _verify(cfg, tasks, und, items):
+    """matching.py:175-191 for items [(task index, (q, t))]: fundamental-matrix RANSAC at
+    ransac_thr (fewer than 8 matches: no model) -> [(q, t of the inliers, count) or
+    (None, None, 0)]."""
+    res = [(None, None, 0)] * len(items)
+    ok = [n for n, (_, (q, _)) in enumerate(items) if len(q) >= 8]
+    flags, _, cnt = ransac_batch("fundamental",
+                                 [und[tasks[items[n][0]][0]][items[n][1][0]] for n in ok],
+                                 [und[tasks[items[n][0]][1]][items[n][1][1]] for n in ok],
+                                 [cfg.ransac_thr] * len(ok), [pair_seed(*tasks[items[n][0]], 0xF) for n in ok],
+                                 cfg.f_iters)
+    for n, f, c in zip(ok, flags, cnt):
+        if c > 0:
+            q, t = items[n][1]
+            res[n] = (q[f], t[f], int(f.sum()))
+    return res
+
+
+def _similarities(cfg, tasks, und, items):
+    """matching.py:194-202 for items [(task index, (q, t, n_in))]: similarity RANSAC at
+    0.02 max(W_i, H_i), then the least-squares fit on its inliers; None when it fails."""
+    res = [None] * len(items)
+    ok = [n for n, (_, (q, _, _)) in enumerate(items) if len(q) >= 2]
+    pa = [und[tasks[items[n][0]][0]][items[n][1][0]] for n in ok]
+    pb = [und[tasks[items[n][0]][1]][items[n][1][1]] for n in ok]
+    flags, _, cnt = ransac_batch("similarity", pa, pb, [0.02 * cfg.side(tasks[items[n][0]][0]) for n in ok],
+                                 [pair_seed(*tasks[items[n][0]], 0x5) for n in ok], S_ITERS)
+    for n, a, b, f, c in zip(ok, pa, pb, flags, cnt):
+        if c >= 2:
+            res[n] = similarity_fit(a[f], b[f])
+    return res
+
+
+# ---- images from feature files --------------------------------------------------------------
+
+def load_images(ctx, idx):
+    """FlowImages of photos `idx` from the context's feature files (matching.py:29-64): one
+    batched rescale + undistortion launch for all of them.  Keypoint size / angle are read when
+    the file holds them (the learned filter's inputs)."""
+    torch = _torch()
+    from . import camera
+    items, meta = [], []
+    for i in idx:
+        with np.load(os.path.join(ctx["feat_dir"], "%d.npz" % i)) as d:
+            xy = np.ascontiguousarray(d["xy"], np.float32).reshape(-1, 2)
+            desc = np.ascontiguousarray(d["desc"], np.uint8).reshape(-1, 128)
+            rw, rh = (int(v) for v in d["red_wh"])
+            size = d["size"].astype(np.float32) if "size" in d.files else None
+            angle = d["angle"].astype(np.float32) if "angle" in d.files else None
+        W, H = ctx["W"][i], ctx["H"][i]
+        K, dd = ctx["K"][i], ctx["dist"][i]
+        k8 = (K[0, 0], K[0, 2], K[1, 2], dd[0], dd[1], dd[2], dd[3], dd[4])
+        items.append((torch.from_numpy(xy).cuda(), (W / float(max(rw, 1)), H / float(max(rh, 1))), k8))
+        meta.append((torch.from_numpy(desc).cuda(), W, H, size, angle))
+    outs = camera.undistort_batch(items, want_xy=False) if items else []
+    res = {}
+    for i, (_, und), (desc, W, H, size, angle) in zip(idx, outs, meta):
+        t = lambda a: torch.from_numpy(a).cuda() if a is not None else None   # noqa: E731
+        res[i] = FlowImage(desc, und, W, H, t(size), t(angle))
+    return res

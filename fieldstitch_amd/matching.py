@@ -34,6 +34,7 @@ FORMS = {"plain": 0, "mfma": 1}
 _CTX = None
 _CACHE = OrderedDict()
 _CACHE_MAX = 48
+_FLOW_CACHE = OrderedDict()       # photo -> match_flow.FlowImage (device-resident features)
 
 
 def init_worker(ctx):
@@ -46,6 +47,7 @@ def init_worker(ctx):
             ctx = pickle.load(f)
     _CTX = ctx
     _CACHE.clear()
+    _FLOW_CACHE.clear()
 
 
 def _read_npz(feat_dir, i):
@@ -130,14 +132,15 @@ def _predict(u, i, j):
 
 
 def _unique(q, t, dd):
-    """matching.py:103-110 (host): one query per target, smallest dd then earliest; sorted by t."""
+    """matching.py:103-110 (host): one query per target.  Visiting the matches by distance
+    (a stable order, so equal distances keep their input position), the first visit of each
+    target wins; np.unique returns those first visits by ascending target."""
     if len(q) == 0:
         return q, t
-    order = np.lexsort((dd, t))
-    t_s = t[order]
-    first = np.r_[True, t_s[1:] != t_s[:-1]]
-    keep = order[first]
-    return q[keep], t[keep]
+    by_dist = np.argsort(np.asarray(dd), kind="stable")
+    _, first = np.unique(np.asarray(t)[by_dist], return_index=True)
+    win = by_dist[first]
+    return q[win], t[win]
 
 
 def as_u8(desc) -> np.ndarray:
@@ -237,6 +240,51 @@ class PairMatcher:
             ev = (P * 4)(*[e.cuda_event for e in events])
         _lib.call("fsx_match_launch", P(self.ws.data_ptr()), self._info, FORMS[form], ev, P(st))
 
+    def run_filtered(self, form, net, tau, sizes, angles, stream=None):
+        """run() with the learned match filter as a pruning stage (asynchronous up to one count
+        read): contraction without publication, then the filter scores every query that passed
+        the ratio test (its row built from the query and its best candidate), queries scoring
+        below tau are dropped, and the one-query-per-target rule and compaction run on the
+        rest.  net: filter_mlp.MatchFilterMLP; sizes / angles: per image float32 keypoint size
+        and angle (degrees), as filter_features takes them."""
+        torch = self.torch
+        P = C.c_void_p
+        st = P(_lib.stream_handle(stream))
+        sz, an = self._size_angle(sizes, angles)
+        nq = sum(self.nk[a] for a, _ in self.pairs)
+        lst = torch.empty((max(nq, 1), 2), dtype=torch.int32, device=self.desc.device)
+        cnt = torch.empty(1, dtype=torch.int32, device=self.desc.device)
+        _lib.call("fsx_match_contract", P(self.ws.data_ptr()), self._info, FORMS[form], st)
+        _lib.call("fsx_match_accepted", P(self.ws.data_ptr()), self._info, P(lst.data_ptr()), P(cnt.data_ptr()), st)
+        n_acc = int(cnt.item())
+        score = None
+        if n_acc:
+            X = torch.empty((n_acc, 288), dtype=torch.bfloat16, device=self.desc.device)
+            _lib.call("fsx_match_filter_rows", P(self.ws.data_ptr()), self._info, P(sz.data_ptr()), P(an.data_ptr()),
+                      P(lst.data_ptr()), P(cnt.data_ptr()), n_acc, P(X.data_ptr()), st)
+            score = net.predict(X)
+        self._filter_kept = None if score is None else int((score >= tau).sum())
+        self._filter_scored = n_acc
+        _lib.call("fsx_match_publish", P(self.ws.data_ptr()), self._info, P(lst.data_ptr()), P(cnt.data_ptr()),
+                  n_acc, P(score.data_ptr()) if score is not None else None, float(tau), st)
+
+    def filter_counts(self):
+        """(queries scored, queries kept) of the last run_filtered."""
+        return self._filter_scored, self._filter_kept
+
+    def _size_angle(self, sizes, angles):
+        torch = self.torch
+
+        def cat(vals):
+            if any(v is None for v in vals):
+                raise ValueError("the learned filter needs every image's keypoint sizes and angles")
+            return torch.cat([v.float() if isinstance(v, torch.Tensor) else torch.as_tensor(np.asarray(v, np.float32))
+                              for v in vals]).to(self.desc.device).contiguous()
+        sz, an = cat(sizes), cat(angles)
+        if sz.numel() != sum(self.nk) or an.numel() != sum(self.nk):
+            raise ValueError("sizes / angles must give one value per keypoint")
+        return sz, an
+
     def candidates(self):
         """Device views of the last run's per-query arrays (caller's query order, pair p's
         queries at the prefix of n_a): best candidate before the ratio test (int32, -1 none),
@@ -257,12 +305,7 @@ class PairMatcher:
         queries without one).  sizes / angles: per image float32 arrays of keypoint size and
         angle (degrees), in the order of the images given to the constructor."""
         torch = self.torch
-        sz = torch.cat([torch.as_tensor(np.asarray(s_, np.float32)) if not isinstance(s_, torch.Tensor) else s_.float()
-                        for s_ in sizes]).to(self.desc.device).contiguous()
-        an = torch.cat([torch.as_tensor(np.asarray(a_, np.float32)) if not isinstance(a_, torch.Tensor) else a_.float()
-                        for a_ in angles]).to(self.desc.device).contiguous()
-        if sz.numel() != sum(self.nk) or an.numel() != sum(self.nk):
-            raise ValueError("sizes / angles must give one value per keypoint")
+        sz, an = self._size_angle(sizes, angles)
         nq = sum(self.nk[a] for a, _ in self.pairs)
         X = torch.empty((max(nq, 1), 288), dtype=torch.bfloat16, device=self.desc.device)
         _lib.call("fsx_match_filter_features", C.c_void_p(self.ws.data_ptr()), self._info, C.c_void_p(sz.data_ptr()),
@@ -471,64 +514,58 @@ def mfma_layout_probe():
     return out.view(64, 16).cpu().numpy()
 
 
-# ---- match_pair (matching.py:175-253): coarse -> homography -> guided -> verification -------
+# ---- match_pair (matching.py:175-253) on the batched stage pipeline of match_flow -----------
 
-RANSAC_H_ITERS = 2000    # cv2.findHomography's default maxIters
-RANSAC_F_ITERS = 4096    # fixed count (cv2: up to 10000 with early exit at 0.999 confidence)
-RANSAC_S_ITERS = 2000    # cv2.estimateAffinePartial2D's default maxIters
-
-
-def _pair_seed(i, j, salt):
-    return (0x5EED0000 ^ (int(i) * 0x9E3779B1) ^ (int(j) << 24) ^ salt) & 0xFFFFFFFFFFFFFFFF
-
-
-def _find_homography(pa, pb, thr, seed):
-    """matching.py:226 cv2.findHomography(..., RANSAC, thr) -> (H 3x3 or None, inlier mask)."""
-    from . import verify
-    flags, M, cnt = verify.ransac("homography", [pa], [pb], n_hyp=RANSAC_H_ITERS, thr=thr, seed=seed)
-    if cnt[0] == 0:
-        return None, None
-    return M[0].reshape(3, 3), flags[0]
+def _flow_images(idx):
+    """FlowImages of photos idx from the worker context's feature files, cached (48 photos, as
+    matching.py:13-14)."""
+    from . import match_flow
+    need = [i for i in dict.fromkeys(idx) if i not in _FLOW_CACHE]
+    if need:
+        _FLOW_CACHE.update(match_flow.load_images(_CTX, need))
+    out = {}
+    for i in idx:
+        _FLOW_CACHE.move_to_end(i)
+        out[i] = _FLOW_CACHE[i]
+    while len(_FLOW_CACHE) > max(_CACHE_MAX, len(set(idx))):
+        _FLOW_CACHE.popitem(last=False)
+    return out
 
 
-def _verify(c, i, j, ui, uj, q, t):
-    """matching.py:175-191: fundamental-matrix RANSAC at c["ransac_thr"] on the GPU
-    (deterministic normalized 8-point, cv2's FM_RANSAC error, fsx_ransac_fundamental) in the
-    place of cv2's USAC_MAGSAC -> (q, t of the inliers, count) or (None, None, 0)."""
-    from . import verify
-    if len(q) < 8:
-        return None, None, 0
-    flags, _, cnt = verify.ransac("fundamental", [ui[q]], [uj[t]], n_hyp=int(c.get("ransac_iters", RANSAC_F_ITERS)),
-                                  thr=float(c["ransac_thr"]), seed=_pair_seed(i, j, 0xF))
-    if cnt[0] == 0:
-        return None, None, 0
-    m = flags[0]
-    return q[m], t[m], int(m.sum())
-
-
-def _similarity(c, i, ui, uj, q, t):
-    """matching.py:194-202: (angle rad, scale, tx, ty) of the similarity from image i to image
-    j, or None.  Similarity RANSAC on the GPU (fsx_ransac_similarity, threshold 0.02 max(W, H))
-    in the place of cv2.estimateAffinePartial2D, then the least-squares fit on its inliers (the
-    optimum cv2's refinement converges to for this linear model)."""
-    import math
-    from . import verify
+def match_pairs(tasks, form="mfma", filt=None, stats=None):
+    """match_pair for a list of pairs of the installed context, each stage batched over all the
+    pairs that reach it (fieldstitch_amd.match_flow).  Returns the tuples in task order."""
+    from . import match_flow
+    tasks = [(int(i), int(j)) for i, j in tasks]
+    idx = list(dict.fromkeys(i for t in tasks for i in t))
+    bad = {}
     try:
-        if len(q) < 2:
-            return None
-        flags, _, cnt = verify.ransac("similarity", [ui[q]], [uj[t]], n_hyp=RANSAC_S_ITERS,
-                                      thr=0.02 * max(c["W"][i], c["H"][i]), seed=_pair_seed(i, len(q), 0x5))
-        if cnt[0] < 2:
-            return None
-        a = np.asarray(ui[q][flags[0]], np.float64)
-        b = np.asarray(uj[t][flags[0]], np.float64)
-        A = np.zeros((2 * len(a), 4))
-        A[0::2, 0], A[0::2, 1], A[0::2, 2] = a[:, 0], -a[:, 1], 1.0
-        A[1::2, 0], A[1::2, 1], A[1::2, 3] = a[:, 1], a[:, 0], 1.0
-        (sa, sb, tx, ty), *_ = np.linalg.lstsq(A, b.reshape(-1), rcond=None)
-        return (math.atan2(sb, sa), math.hypot(sa, sb), float(tx), float(ty))
-    except Exception:  # noqa: BLE001 - the reference returns None on any failure here
-        return None
+        images = _flow_images(idx)
+    except Exception:  # noqa: BLE001 - find the photos that fail to load, report their pairs
+        images = {}
+        for i in idx:
+            try:
+                images.update(_flow_images([i]))
+            except Exception as e:  # noqa: BLE001
+                bad[i] = e
+    good = [k for k, (i, j) in enumerate(tasks) if i not in bad and j not in bad]
+    res = match_flow.run_pairs([tasks[k] for k in good], images, match_flow.FlowConfig(_CTX), form=form, filt=filt,
+                               stats=stats)
+    out = [None] * len(tasks)
+    for k, r in zip(good, res):
+        out[k] = r
+    for k, (i, j) in enumerate(tasks):
+        if out[k] is None:
+            out[k] = match_flow._error_tuple(i, j, bad[i] if i in bad else bad[j])
+    return out
+
+
+def match_pair(task):
+    """matching.py:205-253 for the pair task = (i, j) of the installed context: (i, j, q int32,
+    t int32, sim, n_inliers), (i, j, None, None, None, n) on failure, or ("error", msg) in the
+    sim slot.  One pair through match_pairs; cv2's estimators are replaced by the deterministic
+    GPU RANSAC of match_flow, FLANN by the exact brute force."""
+    return match_pairs([task])[0]
 
 
 def _ratio_match(da, db, ratio, checks=48):
@@ -537,60 +574,22 @@ def _ratio_match(da, db, ratio, checks=48):
     return _ratio_match_bf(da, db, ratio)
 
 
-def match_pair(task):
-    """matching.py:205-253 for the pair task = (i, j) of the installed context: coarse ratio
-    matching of the n_coarse strongest features (+ the ground-plane prior), homography RANSAC,
-    guided matching under that homography, fundamental-matrix verification; exhaustive ratio
-    matching as the fallback; similarity of the verified set.  Returns (i, j, q int32, t int32,
-    sim, n_inliers), (i, j, None, None, None, n) on failure, or ("error", msg) in the sim slot
-    on an exception.  The matching runs on the GPU kernels; cv2's estimators are replaced by the
-    deterministic GPU RANSAC of verify.py."""
-    i, j = task
-    c = _CTX
+def _verify(c, i, j, ui, uj, q, t):
+    """matching.py:175-191 for one pair: fundamental-matrix RANSAC at c["ransac_thr"] on the
+    GPU -> (q, t of the inliers, count) or (None, None, 0)."""
+    from . import match_flow
+    cfg = match_flow.FlowConfig(c)
+    und = {i: np.asarray(ui, np.float64), j: np.asarray(uj, np.float64)}
+    return match_flow._verify(cfg, [(i, j)], und, [(0, (np.asarray(q), np.asarray(t)))])[0]
+
+
+def _similarity(c, i, j, ui, uj, q, t):
+    """matching.py:194-202 for one pair: (angle rad, scale, tx, ty) of the similarity from
+    image i to image j, or None (similarity RANSAC on the GPU, then its least-squares fit)."""
+    from . import match_flow
     try:
-        A = _get(i)
-        B = _get(j)
-        ui, uj = A["und"], B["und"]
-        if len(ui) < 30 or len(uj) < 30:
-            return i, j, None, None, None, 0
-        use_prior = c["has_rot"][i] and c["has_rot"][j] and c.get("use_prior", True)
-        tol = c["prior_tol"] * max(c["W"][j], c["H"][j])
-        result = None
-        # stage 1: strongest features -> coarse homography -> guided matching
-        n1 = c.get("n_coarse", 1500)
-        if c.get("guided", True) and len(ui) > n1 // 2:
-            q1, t1 = _ratio_match_bf(A["desc"][:n1], B["desc"][:n1], 0.85, A["norm"][:n1], B["norm"][:n1])
-            if use_prior and len(q1):
-                pred, ok = _predict(ui[q1], i, j)
-                m = ok & (np.linalg.norm(pred - uj[t1], axis=1) < tol)
-                q1, t1 = q1[m], t1[m]
-            if len(q1) >= 12:
-                Hm, hm = _find_homography(ui[q1], uj[t1], 0.012 * max(c["W"][i], c["H"][i]), _pair_seed(i, j, 0x1))
-                if Hm is not None and hm is not None and hm.sum() >= 10:
-                    radius = c["guided_radius"] * max(c["W"][j], c["H"][j])
-                    q, t = _guided(A, B, Hm, radius, c.get("guided_k", 24), 0.85)
-                    if len(q) >= c["min_inliers"]:
-                        q, t, n_in = _verify(c, i, j, ui, uj, q, t)
-                        if q is not None and n_in >= max(c["min_inliers"], 40):
-                            result = (q, t, n_in)
-        # fallback: exhaustive ratio matching
-        if result is None:
-            q, t = _ratio_match(A["desc"], B["desc"], c.get("ratio", 0.8), c.get("checks", 48))
-            if use_prior and len(q):
-                pred, ok = _predict(ui[q], i, j)
-                m = ok & (np.linalg.norm(pred - uj[t], axis=1) < tol)
-                q, t = q[m], t[m]
-            if len(q) < 20:
-                return i, j, None, None, None, 0
-            q, t, n_in = _verify(c, i, j, ui, uj, q, t)
-            if q is None or n_in < c["min_inliers"]:
-                return i, j, None, None, None, n_in
-            result = (q, t, n_in)
-        q, t, n_in = result
-        sim = _similarity(c, i, ui, uj, q, t)
-        return i, j, np.asarray(q).astype(np.int32), np.asarray(t).astype(np.int32), sim, n_in
-    except Exception as e:  # noqa: BLE001 - matching.py:249-253 reports every failure in the tuple
-        import traceback
-        tb = traceback.extract_tb(e.__traceback__)[-1]
-        return i, j, None, None, ("error", "%s: %s (%s line %d)" % (type(e).__name__, e, os.path.basename(tb.filename),
-                                                                     tb.lineno)), 0
+        cfg = match_flow.FlowConfig(c)
+        und = {i: np.asarray(ui, np.float64), j: np.asarray(uj, np.float64)}
+        return match_flow._similarities(cfg, [(i, j)], und, [(0, (np.asarray(q), np.asarray(t), 0))])[0]
+    except Exception:  # noqa: BLE001 - the reference returns None on any failure here
+        return None

@@ -282,3 +282,31 @@ def descriptor_pair(rng: np.random.Generator, na: int, nb: int, n_true: int | No
     qa = rng.permutation(na)[:n_true]
     db[tb] = np.maximum(da[qa] + rng.normal(0.0, noise, (n_true, 128)).astype(np.float32), 0.0)
     return da, db, (qa, tb)
+
+
+def flight_cameras(fl: Flight, f: float = 3000.0, gnss_sigma_px: float = 3.0 / np.sqrt(2.0), seed: int = 0):
+    """Pinhole nadir cameras that see the flight's photos exactly (for the ground-plane prior
+    of matching.py:155-172): the mosaic plane is the ground (z = 0, z pointing down), photo i
+    looks straight down from height f * s_i (s_i = mosaic units per pixel) with its image axes
+    along the photo's rotation, no lens distortion.  The camera centres carry GNSS noise of
+    gnss_sigma_px pixels per axis (so a pair's prediction errs by ~gnss_sigma_px * sqrt(2)).
+    Returns the matching context's camera entries: K, dist, R, C, zg, has_rot, W, H."""
+    rng = np.random.default_rng(seed)
+    K, dist, R, Cc, zg, W, H = [], [], [], [], [], [], []
+    for A, img in zip(fl.A, fl.images):
+        h, w = img.shape
+        a, b = float(A[0, 0]), float(A[1, 0])
+        s = float(np.hypot(a, b))
+        c, sn = a / s, b / s
+        cx, cy = (w - 1) / 2.0, (h - 1) / 2.0
+        centre = np.asarray(A, np.float64)[:2] @ np.array([cx, cy, 1.0])
+        K.append(np.array([[f, 0.0, cx], [0.0, f, cy], [0.0, 0.0, 1.0]]))
+        dist.append(np.zeros(5))
+        R.append(np.array([[c, sn, 0.0], [-sn, c, 0.0], [0.0, 0.0, 1.0]]))
+        noise = rng.normal(0.0, gnss_sigma_px * s, 2)
+        Cc.append(np.array([centre[0] + noise[0], centre[1] + noise[1], -f * s]))
+        zg.append(0.0)
+        W.append(w)
+        H.append(h)
+    n = len(fl.images)
+    return {"K": K, "dist": dist, "R": R, "C": Cc, "zg": zg, "has_rot": [True] * n, "W": W, "H": H}

@@ -332,6 +332,37 @@ int fsx_ransac_similarity(int n_pair, const long long *off_dev, const double *pa
 int fsx_match_filter_features(void *ws_dev, const long long *launch_info, const float *size_dev,
                               const float *angle_dev, void *X_dev, void *stream);
 
+/* The matcher in stages, so that the learned match filter can prune between the contraction
+ * and the one-query-per-target rule (no reference counterpart: the north star's pruning stage
+ * ahead of matching.py:146-152's acceptance and the verification of :226-229).  On a workspace
+ * prepared by fsx_match_prepare:
+ *   fsx_match_contract: setup, kc-nearest prefilter and contraction (`form` as
+ *     fsx_match_launch); every query's best candidate, distances and ratio-test outcome are
+ *     stored, nothing is published yet;
+ *   fsx_match_accepted: list_dev (2 int32 per entry, n_query entries of capacity) receives the
+ *     queries that passed the ratio test as (global query, pair), in arbitrary order, and
+ *     *count_dev their number;
+ *   fsx_match_filter_rows: row r of X_dev (bf16, 288 wide, cap rows) = the filter input of list
+ *     entry r (the layout of fsx_match_filter_features), for r < *count_dev;
+ *   fsx_match_publish: entries with score_dev[r] >= tau (score_dev NULL: all) take part in the
+ *     one-query-per-target rule, the rest are rejected; then the per-pair (q, t) lists are
+ *     compacted for fsx_match_results.  cap >= *count_dev (n_query always is).
+ * With score_dev NULL the result equals fsx_match_launch's.  Asynchronous. */
+int fsx_match_contract(void *ws_dev, const long long *launch_info, int form, void *stream);
+int fsx_match_accepted(void *ws_dev, const long long *launch_info, int *list_dev, int *count_dev, void *stream);
+int fsx_match_filter_rows(void *ws_dev, const long long *launch_info, const float *size_dev, const float *angle_dev,
+                          const int *list_dev, const int *count_dev, long long cap, void *X_dev, void *stream);
+int fsx_match_publish(void *ws_dev, const long long *launch_info, const int *list_dev, const int *count_dev,
+                      long long cap, const float *score_dev, float tau, void *stream);
+
+/* RANSAC for pairs that each carry their own threshold and seed (device arrays of n_pair):
+ * model 0 = homography (as fsx_ransac_homography), 1 = fundamental matrix, 2 = similarity.
+ * Pair p draws hypothesis h from (seed_dev[p], 0, h), so its result does not depend on which
+ * other pairs share the launch (match_pair per pair, matching.py:226,180,196). */
+int fsx_ransac_batch(int model, int n_pair, const long long *off_dev, const double *pa_dev, const double *pb_dev,
+                     int n_hyp, const double *thr_dev, const unsigned long long *seed_dev, uint8_t *inlier_dev,
+                     double *M_dev, int *n_in_dev, void *stream);
+
 /* ---------------------------------------------------------------------------------------
  * Keypoint geometry around the matcher
  * ------------------------------------------------------------------------------------- */
@@ -360,6 +391,16 @@ int fsx_undistort_f64(int n, const double *uv_dev, const double *cam8, int iters
  * and ok_dev (n bytes: t > 0 and z > 0).  Asynchronous. */
 int fsx_ground_predict(int n, const double *u_dev, const double *cams, double *pred_dev, uint8_t *ok_dev,
                        void *stream);
+
+/* match_pair's prior filter (matching.py:222-225, 238-241) for n_pair pairs at once: rows
+ * off_dev[p] .. off_dev[p+1] (int64 prefix) of ua_dev / ub_dev (float64 undistorted pixels of
+ * the matches in images i and j); cams_dev: 43 float64 per pair in fsx_ground_predict's layout;
+ * tol_dev[p] (float64): keep_dev[r] = prediction valid and |pred - ub| < tol; tol < 0 keeps every
+ * row (the pair has no attitude prior).  max_rows: the largest pair's row count.
+ * Asynchronous. */
+int fsx_prior_keep_batch(int n_pair, const long long *off_dev, const double *ua_dev, const double *ub_dev,
+                         const double *cams_dev, const double *tol_dev, int max_rows, uint8_t *keep_dev,
+                         void *stream);
 
 /* Candidate pairs from GPS positions (pipeline.py:452-476).  xy_dev: n x 2 float64;
  * half_long_dev: n float64; k = min(n - 1, neighbors) <= 64; has_rel != 0 applies the

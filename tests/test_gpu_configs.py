@@ -20,6 +20,7 @@ import pytest
 
 from fieldstitch_amd import synth
 from oracle import match_oracle as mo
+from test_gpu_features import per_image_path
 
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
@@ -128,7 +129,7 @@ def test_mixed_batch_graph_replay_equals_per_image_path(gpu, mixed_flight):
         torch.cuda.synchronize()
         for i, img in enumerate(imgs):
             xy, u8, resp, d = fb.results(i)
-            xr, dr, rr, _ = features.sift_gpu(img, 8000)
+            xr, dr, rr = per_image_path(img, 8000)
             np.testing.assert_array_equal(xy, xr)
             np.testing.assert_array_equal(resp, rr)
             np.testing.assert_array_equal(d, dr)

@@ -5,6 +5,8 @@ import queue
 import numpy as np
 import pytest
 
+from test_gpu_features import per_image_path
+
 
 def _tasks(tmp_path, shapes, nfeat=1500):
     from fieldstitch_amd import synth
@@ -67,7 +69,7 @@ def test_worker_matches_sift_gpu(tmp_path, gpu):
         _, n, err, tt = ok[idx]
         assert err is None and n > 10 and tt["photos_gpu"] == 1
         g = np.load(path)
-        xy, desc, resp, _ = features.sift_gpu(g, nfeat)
+        xy, desc, resp = per_image_path(g, nfeat)
         ref = str(tmp_path / ("ref%d.npz" % idx))
         assert features._save(ref, xy, desc, resp, g.shape[1], g.shape[0]) == n
         a, b = np.load(out), np.load(ref)
@@ -91,7 +93,7 @@ def test_worker_gpu_clahe(tmp_path, gpu):
         _, n, err, _ = ok[idx]
         assert err is None and n > 10
         g = clahe_oracle.apply(np.load(path))
-        xy, desc, resp, _ = features.sift_gpu(g, nfeat)
+        xy, desc, resp = per_image_path(g, nfeat)
         ref = str(tmp_path / ("ref%d.npz" % idx))
         assert features._save(ref, xy, desc, resp, g.shape[1], g.shape[0]) == n
         a, b = np.load(out), np.load(ref)
