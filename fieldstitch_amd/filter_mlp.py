@@ -150,14 +150,15 @@ class MatchFilterMLP:
 # Training data from the plain pipeline and the filter's evaluation
 
 def flight_candidates(seed, workers=8, radius=60.0, ratio=0.8, n_hyp=512, thr=3.0, nfeat=8000, chunk=32,
-                      **flight_kw):
+                      keep_matcher=False, **flight_kw):
     """Candidates and labels of one synthetic flight (synth.flight with flight_kw: a 2 x 4 grid
     by default, 20 x 20 for the 400-photo flights): SIFT on the GPU in chunks of `chunk` photos,
     guided matching of every overlapping pair (the plain method), homography RANSAC on each
     pair's plain matches.  Returns a dict with X (bf16 [n, 288] device rows, one per query with
     a candidate), y (float32 device labels: 1 = the plain pipeline's match and a RANSAC
     inlier), plain (bool numpy: accepted by the ratio test and the one-query-per-target rule)
-    and counts."""
+    and counts.  keep_matcher=True adds the flight's PairMatcher ("matcher", after its plain run),
+    the per-photo keypoint sizes / angles and per-query arrays for checking the pruning stage."""
     import torch
     from . import features, synth
     from .matching import PairMatcher
@@ -202,9 +203,13 @@ def flight_candidates(seed, workers=8, radius=60.0, ratio=0.8, n_hyp=512, thr=3.
         label[qoff[p] + q[f]] = 1.0
     keep = cb >= 0
     idx = torch.from_numpy(np.nonzero(keep)[0]).cuda()
-    return {"X": Xall.index_select(0, idx).contiguous(), "y": torch.from_numpy(label[keep]).cuda(),
-            "plain": plain[keep], "label": label[keep] > 0.5, "n_query": nq, "n_pairs": len(pairs),
-            "n_plain": int(plain.sum()), "n_inliers": int(label.sum())}
+    out = {"X": Xall.index_select(0, idx).contiguous(), "y": torch.from_numpy(label[keep]).cuda(),
+           "plain": plain[keep], "label": label[keep] > 0.5, "n_query": nq, "n_pairs": len(pairs),
+           "n_plain": int(plain.sum()), "n_inliers": int(label.sum())}
+    if keep_matcher:
+        out.update({"matcher": pm, "sizes": sizes, "angles": angles, "query_offsets": qoff, "plain_results": res,
+                    "query_label": label > 0.5})
+    return out
 
 
 def train(net, X, y, epochs=4, batch=8192, seed=0):

@@ -246,11 +246,12 @@ class PairMatcher:
         the ratio test (its row built from the query and its best candidate), queries scoring
         below tau are dropped, and the one-query-per-target rule and compaction run on the
         rest.  net: filter_mlp.MatchFilterMLP; sizes / angles: per image float32 keypoint size
-        and angle (degrees), as filter_features takes them."""
+        and angle (degrees), as filter_features takes them.  net = None keeps every query (the
+        result is run()'s)."""
         torch = self.torch
         P = C.c_void_p
         st = P(_lib.stream_handle(stream))
-        sz, an = self._size_angle(sizes, angles)
+        sz, an = self._size_angle(sizes, angles) if net is not None else (None, None)
         nq = sum(self.nk[a] for a, _ in self.pairs)
         lst = torch.empty((max(nq, 1), 2), dtype=torch.int32, device=self.desc.device)
         cnt = torch.empty(1, dtype=torch.int32, device=self.desc.device)
@@ -258,7 +259,7 @@ class PairMatcher:
         _lib.call("fsx_match_accepted", P(self.ws.data_ptr()), self._info, P(lst.data_ptr()), P(cnt.data_ptr()), st)
         n_acc = int(cnt.item())
         score = None
-        if n_acc:
+        if n_acc and net is not None:
             X = torch.empty((n_acc, 288), dtype=torch.bfloat16, device=self.desc.device)
             _lib.call("fsx_match_filter_rows", P(self.ws.data_ptr()), self._info, P(sz.data_ptr()), P(an.data_ptr()),
                       P(lst.data_ptr()), P(cnt.data_ptr()), n_acc, P(X.data_ptr()), st)
