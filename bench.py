@@ -13,8 +13,12 @@ step's latency-bound stages overlap the next step's pyramid (each step does the 
 ``matching`` holds matched pairs/s of the fast and the plain FP32 GPU forms and whether the
 match set equals the FP64 oracle; ``precision`` holds the float-descriptor brute-force section
 (200 pairs, BF16 and FP8 screening, distance error against float64, match-set equality);
-``roofline`` is for the dominant kernel (the pyramid's LDS strip blur); ``cpu_baseline`` times
-the CPU oracle on a bounded sample.
+``roofline`` is for the dominant kernel (the pyramid's LDS strip blur); ``flight400`` holds the
+400-photo mixed-resolution flight (SIFT in graph-captured chunks against the HBM roofline, guided
+matching of its ~3400 pairs in both forms, a sample against the FP64 oracle, and match_pair's
+whole chain through match_pairs); ``cpu_baseline`` times the reference's CPU pool shape
+(os.cpu_count() - 1 single-threaded workers) on a bounded sample: SIFT with the C restatement of
+gsift's CPU path and guided matching with the float32 numpy matcher.
 """
 from __future__ import annotations
 
@@ -64,34 +68,96 @@ def sift_bytes(H, W):
     return H * W + 4 * P0 + 118 * sumP
 
 
-def cpu_sift_worker(args):
-    img, max_kp, nfeat = args
-    os.environ["OMP_NUM_THREADS"] = "1"
+# ---- CPU baseline: the reference's CPU worker pool (BASELINE.md plan) -----------------------
+# A spawn pool of os.cpu_count() - 1 single-threaded workers (sysinfo.py:28-35, pipeline.py:14-18);
+# photos and features reach the workers through files they memory-map, so the timed maps carry
+# only indices.
+
+_W = {}
+
+
+def _cpu_pool_init(photo_file, feat_file):
+    os.environ["OMP_NUM_THREADS"] = os.environ["OPENBLAS_NUM_THREADS"] = os.environ["MKL_NUM_THREADS"] = "1"
+    _W["photos"] = np.load(photo_file, mmap_mode="r")
+    with np.load(feat_file) as f:
+        _W["feat"] = {k: f[k] for k in f.files}
+
+
+def _cpu_noop(k):
+    return k
+
+
+def cpu_sift_photo(k):
+    """One photo through the CPU SIFT front end: the C restatement of gsift's CPU path (detect
+    int(2.2 nfeat), spread to nfeat, response order, describe, RootSIFT)."""
     from oracle import sift_oracle as so
     from fieldstitch_amd import features
+    img = np.ascontiguousarray(_W["photos"][k % len(_W["photos"])])
     t = time.perf_counter()
-    k, st = so.detect(img, max_kp, CONTRAST, EDGE)
-    lst = [features._KP(float(x), float(y), float(r)) for x, y, r in zip(k["x"], k["y"], k["response"])]
-    sel = np.array(features._spread(lst, img.shape[1], img.shape[0], nfeat), np.int64)
-    sel = sel[np.argsort(-k["response"][sel], kind="stable")]
-    d = so.describe(st, k, sel)
-    so.rootsift_u8(d)
+    kp, st = so.detect(img, int(NFEAT * 2.2), CONTRAST, EDGE)
+    sel = features.spread_select(kp["x"], kp["y"], kp["response"], img.shape[1], img.shape[0], NFEAT)
+    sel = sel[np.argsort(-kp["response"][sel], kind="stable")]
+    so.rootsift_u8(so.describe(st, kp, sel))
     return time.perf_counter() - t
 
 
-def cpu_baseline(images, n_proc):
-    """The CPU oracle (C restatement of gsift's CPU path) on one photo per process."""
+def cpu_match_pair(p):
+    """One guided pair with the reference's float32 numpy matcher (vectorised restatement of
+    matching.py:113-152, every candidate in the 60 px window)."""
+    from oracle import match_oracle as mo
+    f = _W["feat"]
+    i, j = int(f["pairs"][p, 0]), int(f["pairs"][p, 1])
+    t = time.perf_counter()
+    mo.guided_f32_fast(f["desc%d" % i], f["xy%d" % i], f["desc%d" % j], f["xy%d" % j], f["Hm"][p],
+                       tuple(f["wh%d" % j]), RADIUS, RATIO)
+    return time.perf_counter() - t
+
+
+def cpu_baseline(images, host, pairs, Hm, shapes, match_reps=16):
+    """SIFT images/s and guided-matching pairs/s of the CPU pool on the host cores: 2 photos
+    per worker (the benchmark's photos cycled), then the benchmark's pairs match_reps times in
+    chunks of 8 (pipeline.py:536)."""
     import multiprocessing as mp
-    sample = images[:n_proc]
-    ctx = mp.get_context("spawn")
-    t0 = time.perf_counter()
-    with ctx.Pool(len(sample)) as pool:
-        per = pool.map(cpu_sift_worker, [(im, int(NFEAT * 2.2), NFEAT) for im in sample])
-    wall = time.perf_counter() - t0
-    return {"value": len(sample) / wall, "unit": "images/s", "cores": len(sample), "kind": "port",
-            "sample": "%d of the 2640x1978 photos, one per process (C oracle SIFT + spread + describe + "
-                      "RootSIFT), wall %.1f s, per-photo CPU %.1f s" % (len(sample), wall, float(np.mean(per))),
-            "host_cpus_usable": len(os.sched_getaffinity(0)), "cpu_model": cpu_model()}
+    import tempfile
+    n_w = max(1, (os.cpu_count() or 1) - 1)
+    try:
+        import cv2  # noqa: F401
+        have_cv2 = True
+    except Exception:  # noqa: BLE001 - OpenCV is optional on the box
+        have_cv2 = False
+    with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
+        photo_file = os.path.join(tmp, "photos.npy")
+        np.save(photo_file, np.stack(images))
+        feat = {"pairs": np.asarray(pairs, np.int64), "Hm": np.stack([np.asarray(h, np.float64) for h in Hm])}
+        for i, (u8, xy) in enumerate(host):
+            feat["desc%d" % i], feat["xy%d" % i] = u8, xy
+            feat["wh%d" % i] = np.array([shapes[i][1], shapes[i][0]])
+        feat_file = os.path.join(tmp, "feat.npz")
+        np.savez(feat_file, **feat)
+        ctx = mp.get_context("spawn")
+        with ctx.Pool(n_w, initializer=_cpu_pool_init, initargs=(photo_file, feat_file)) as pool:
+            pool.map(_cpu_noop, range(4 * n_w))                 # every worker started and importing done
+            pool.map(cpu_match_pair, range(min(n_w, len(pairs))))   # imports of the matching leg
+            n_photo = 2 * n_w
+            t0 = time.perf_counter()
+            per_photo = pool.map(cpu_sift_photo, range(n_photo), chunksize=1)
+            t_sift = time.perf_counter() - t0
+            tasks = list(range(len(pairs))) * match_reps
+            t0 = time.perf_counter()
+            per_pair = pool.map(cpu_match_pair, tasks, chunksize=8)
+            t_match = time.perf_counter() - t0
+    return {"value": n_photo / t_sift, "unit": "images/s", "cores": n_w, "kind": "port",
+            "pool": "spawn pool of os.cpu_count() - 1 = %d single-threaded workers" % n_w,
+            "sample": "%d photos (the %d benchmark photos cycled, 2 per worker) through the C restatement of "
+                      "gsift's CPU path (detect, spread, describe, RootSIFT): wall %.1f s, %.1f s CPU per photo; "
+                      "OpenCV %s" % (n_photo, len(images), t_sift, float(np.mean(per_photo)),
+                                      "importable but not timed" if have_cv2 else "not installed"),
+            "matching": {"value": len(tasks) / t_match, "unit": "pairs/s", "cores": n_w, "kind": "port",
+                         "sample": "the %d guided pairs x %d through the float32 numpy guided matcher "
+                                   "(oracle.match_oracle.guided_f32_fast) in the same pool, chunksize 8: wall %.2f s, "
+                                   "%.3f s CPU per pair" % (len(pairs), match_reps, t_match, float(np.mean(per_pair)))},
+            "host_cpu_count": os.cpu_count(), "host_cpus_usable": len(os.sched_getaffinity(0)),
+            "cpu_model": cpu_model()}
 
 
 def cpu_model():
@@ -200,7 +266,37 @@ def bench_matching(fb, fl, shapes, args):
     out["speedup_vs_plain_fp32"] = out["plain"]["ms_per_step"] / out["mfma"]["ms_per_step"]
     out["value"] = out["mfma"]["pairs_per_s"]
     out["unit"] = "matched pairs/s"
-    return out
+    return out, host, pairs, Hm
+
+
+CHAIN_CTX = {"prior_tol": 0.18, "ransac_thr": 1.0, "min_inliers": 20, "ratio": 0.8, "use_prior": True,
+             "guided": True, "n_coarse": 1000, "guided_radius": 0.03, "guided_k": 16}
+
+
+def bench_chain(fl, desc_dev, xy_dev, counts, shapes, pairs, reps=2):
+    """match_pair's whole chain (coarse, prior, homography RANSAC, guided, F verification,
+    fallback, similarity) for every pair in one match_pairs call, on device-resident features,
+    with nadir cameras that see the synthetic flight (ground-plane prior on; the pipeline's
+    thresholds, pipeline.py:478-482).  Timed end to end (host glue and the count reads between
+    stages included)."""
+    import torch
+    from fieldstitch_amd import match_flow, synth
+    ctx = dict(CHAIN_CTX, **synth.flight_cameras(fl, seed=7))
+    cfg = match_flow.FlowConfig(ctx)
+    images = {i: match_flow.FlowImage(desc_dev[i][:counts[i]], xy_dev[i][:counts[i]], shapes[i][1], shapes[i][0])
+              for i in range(len(shapes))}
+    stats = {}
+    res = match_flow.run_pairs(pairs, images, cfg, stats=stats)          # warm-up (allocations, kernels)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        res = match_flow.run_pairs(pairs, images, cfg)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / reps
+    matched = [r for r in res if r[2] is not None]
+    return {"pairs": len(pairs), "ms": t * 1e3, "pairs_per_s": len(pairs) / t, "stage_pairs": stats,
+            "matched_pairs": len(matched), "verified_matches_mean": float(np.mean([r[5] for r in matched])) if matched
+            else 0.0, "errors": int(sum(1 for r in res if isinstance(r[4], tuple) and r[4] and r[4][0] == "error"))}
 
 
 def bench_precision(args):
@@ -269,7 +365,7 @@ def bench_precision(args):
     return out
 
 
-def bench_flight400(args):
+def bench_flight400(args, tracks=True):
     """The 400-photo flight workload: a 400-photo flight (20 x 20 grid, 75 % / 65 % overlap) with resolutions mixed
     50/50 between 2640x1978 and 1320x989.  Photos resident in HBM; SIFT in chunks of one fixed
     shape composition through graph-captured FeatureBatches, --inflight chunks in flight on as
@@ -394,6 +490,12 @@ def bench_flight400(args):
                                    u8_all[j, :counts[j]].cpu().numpy(), xy_all[j, :counts[j]].cpu().numpy(),
                                    Hm[p], (shapes[j][1], shapes[j][0]), RADIUS, RATIO)
         ok += int(np.array_equal(q, res["mfma"][p][0]) and np.array_equal(t, res["mfma"][p][1]))
+    log("flight400: matching timed (%d pairs), chain next" % len(pl))
+    chain = bench_chain(fl, u8_all, xy_all, counts, shapes, pl, reps=1)
+    log("flight400: chain %.1f ms" % chain["ms"])
+    if not tracks:
+        return _flight400_record(args, n, t_sift, alg, counts, tuned, t_pairs, pl, out_m, same_forms, sample, ok,
+                                 None, big, small, CB, CS, chain)
     # tracks (recon.py:14-56) from the flight's guided matches (the pipeline feeds RANSAC-
     # verified ones; the guided matches stand in for them here), device pass timed
     from fieldstitch_amd import tracks as trk
@@ -413,7 +515,14 @@ def bench_flight400(args):
                     and np.array_equal(g_feat.cpu().numpy(), ref_tr[2]) and g_ntr == ref_tr[3])
     out_tracks = {"matches": int(len(tq)), "observations": int(len(g_tid)), "tracks": int(g_ntr),
                   "ms_per_flight_incl_count_readback": t_tr * 1e3, "equal_to_oracle": bool(tracks_equal)}
-    return {
+    return _flight400_record(args, n, t_sift, alg, counts, tuned, t_pairs, pl, out_m, same_forms, sample, ok,
+                             out_tracks, big, small, CB, CS, chain)
+
+
+def _flight400_record(args, n, t_sift, alg, counts, tuned, t_pairs, pl, out_m, same_forms, sample, ok, out_tracks,
+                      big, small, CB, CS, chain):
+    steps = max(1, args.steps // 5)
+    rec = {
         "metric": "images/s (SIFT) + matched pairs/s vs plain FP32 GPU form, match set = FP64 oracle",
         "value": n / t_sift, "unit": "images/s", "n_gpus": 1, "steps": steps, "warmup": 1,
         "ms_per_step": t_sift * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -430,11 +539,15 @@ def bench_flight400(args):
         "matching": {"pairs": len(pl), **out_m, "speedup_vs_plain_fp32":
                      out_m["plain"]["ms_per_flight"] / out_m["mfma"]["ms_per_flight"],
                      "forms_equal": bool(same_forms), "fp64_oracle_sample": int(len(sample)),
-                     "fp64_oracle_equal": int(ok), "tracks_from_these_matches": out_tracks,
+                     "fp64_oracle_equal": int(ok),
                      "compulsory_bytes": 128.0 * float(counts.sum()),
                      "compulsory_hbm_frac": 128.0 * float(counts.sum()) / (out_m["mfma"]["ms_per_flight"] / 1e3)
                                             / HBM_PEAK},
+        "match_pairs_chain": chain,
     }
+    if out_tracks is not None:
+        rec["matching"]["tracks_from_these_matches"] = out_tracks
+    return rec
 
 
 def bench_filter(args):
@@ -638,9 +751,9 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-procs", type=int, default=8)
+    ap.add_argument("--no-flight400", action="store_true", help="skip the 400-photo flight section")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_k_blur_8x2640x1978.json"))
-    ap.add_argument("--gen-workers", type=int, default=8,
+    ap.add_argument("--gen-workers", type=int, default=16,
                     help="processes rendering the synthetic photos (1 under a profiler)")
     ap.add_argument("--no-precision", action="store_true", help="skip the float-descriptor precision section")
     ap.add_argument("--filter", action="store_true",
@@ -778,14 +891,24 @@ def main():
                      "avg_launch_us": stages["pyramid"] * 1e3 / n_launch,
                      "algorithmic_bytes_per_step": pyr_b, "time_ms_per_step": stages["pyramid"]},
     }
-    res["matching"] = bench_matching(fb, fl, shapes, args)
+    log("sift timed: %.3f ms per step; matching next" % (t_sift * 1e3))
+    res["matching"], host, pairs, Hm = bench_matching(fb, fl, shapes, args)
+    # match_pair's whole chain on the same photos' device-resident features
+    xy_dev = [torch.from_numpy(h[1]).cuda() for h in host]
+    res["matching"]["match_pairs_chain"] = bench_chain(fl, [fb.u8[i, :len(host[i][0])] for i in range(n_img)],
+                                                        xy_dev, [len(h[0]) for h in host], shapes, pairs)
     if not args.no_precision:
+        log("precision section")
         res["precision"] = bench_precision(args)
+    if not args.no_flight400:
+        log("flight400 section")
+        del fbs, fb
+        torch.cuda.empty_cache()
+        res["flight400"] = bench_flight400(args, tracks=False)
+        res["config"]["flight400_workload"] = res["flight400"]["config"]["workload"]
     if not args.no_cpu_baseline:
-        n_proc = min(args.cpu_procs, len(fl.images), os.cpu_count() or 1)
-        log("cpu baseline: %d processes" % n_proc)
-        res["cpu_baseline"] = cpu_baseline(fl.images, n_proc)
-        res["cpu_baseline"]["host_cpu_count"] = os.cpu_count()
+        log("cpu baseline: pool of %d workers" % max(1, (os.cpu_count() or 1) - 1))
+        res["cpu_baseline"] = cpu_baseline(fl.images, host, pairs, Hm, shapes)
     print(json.dumps(res), flush=True)
 
 

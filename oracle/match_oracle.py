@@ -274,3 +274,55 @@ def guided_fp64_fast(da, xy_a, db, xy_b, Hm, wh_b, radius: float, ratio: float):
     f = np.r_[True, t_s[1:] != t_s[:-1]]
     keep = o[f]
     return q[keep], t[keep]
+
+
+def guided_f32_fast(da, xy_a, db, xy_b, Hm, wh_b, radius: float, ratio: float, chunk: int = 1024):
+    """The reference's float32 guided matcher (matching.py:113-152) vectorised the way it is
+    written there, with every candidate inside the window (kc = all, through
+    cKDTree.query_ball_point on the strict radius): chunks of `chunk` queries, candidate lists
+    padded to the chunk's longest, d = |a|^2 + |b|^2 - 2 b.a in float32 by one batched product,
+    padding at +inf, the two smallest by argpartition, the float32 ratio test, then the
+    one-query-per-target rule.  The CPU baseline of the guided matcher (numpy, one thread).
+    Returns (q int64, t int64)."""
+    from scipy.spatial import cKDTree
+    pred = predict(xy_a, Hm)
+    Wb, Hb = wh_b
+    inside = np.nonzero((pred[:, 0] > -radius) & (pred[:, 0] < Wb + radius)
+                        & (pred[:, 1] > -radius) & (pred[:, 1] < Hb + radius))[0]
+    xy_b = np.asarray(xy_b, np.float64)
+    if len(inside) == 0 or len(xy_b) == 0:
+        return np.zeros(0, np.int64), np.zeros(0, np.int64)
+    lists = cKDTree(xy_b).query_ball_point(pred[inside], radius * (1 + 1e-9) + 1e-9)
+    a = np.asarray(da, np.float32)
+    b = np.asarray(db, np.float32)
+    na = (a * a).sum(1)
+    nb = (b * b).sum(1)
+    r2 = float(radius) * float(radius)
+    qs, ts, ds = [], [], []
+    for s0 in range(0, len(inside), chunk):
+        qi = inside[s0:s0 + chunk]
+        cand = []
+        for q, lst in zip(qi, lists[s0:s0 + chunk]):
+            c = np.asarray(lst, np.int64)
+            d = xy_b[c] - pred[q]
+            cand.append(c[(d * d).sum(1) < r2])
+        width = max(1, max(len(c) for c in cand))
+        idx = np.zeros((len(qi), width), np.int64)
+        valid = np.zeros((len(qi), width), bool)
+        for r, c in enumerate(cand):
+            idx[r, :len(c)] = c
+            valid[r, :len(c)] = True
+        d2 = na[qi, None] + nb[idx] - np.float32(2.0) * np.matmul(b[idx], a[qi][:, :, None])[:, :, 0]
+        d2 = np.where(valid, np.maximum(d2, np.float32(0)), np.float32(np.inf))
+        two = np.argpartition(d2, 1, axis=1)[:, :2] if width > 1 else np.zeros((len(qi), 1), np.int64)
+        rr = np.arange(len(qi))
+        v0 = d2[rr, two[:, 0]]
+        v1 = d2[rr, two[:, 1]] if width > 1 else np.full(len(qi), np.inf, np.float32)
+        best = np.where(v0 <= v1, two[:, 0], two[:, 1] if width > 1 else two[:, 0])
+        b1 = np.minimum(v0, v1)
+        b2 = np.maximum(v0, v1)
+        ok = np.isfinite(b1) & (np.sqrt(b1) < np.float32(ratio) * np.sqrt(b2))
+        qs.append(qi[ok])
+        ts.append(idx[rr[ok], best[ok]])
+        ds.append(b1[ok])
+    return unique(np.concatenate(qs), np.concatenate(ts), np.concatenate(ds))

@@ -202,3 +202,22 @@ def test_fp64_fast_equals_fp64_loop(seed):
         fast = mo.match_fp64_fast(a, b, 0.8, block=256)
         for x, y in zip(slow, fast):
             np.testing.assert_array_equal(x, y)
+
+
+def test_guided_f32_fast_equals_fp64_on_u8():
+    """The vectorised float32 guided matcher (the CPU baseline) returns the FP64 oracle's list
+    on uint8 RootSIFT-like descriptors with planted matches (exact integer distances)."""
+    rng = np.random.default_rng(12)
+    nb, na = 3000, 2500
+    xy_b = rng.uniform(0, 800, (nb, 2))
+    d = rng.gamma(0.6, 1.0, (nb, 128))
+    db = np.clip(np.sqrt(d / d.sum(1, keepdims=True)) * 512, 0, 255).astype(np.uint8)
+    src = rng.integers(0, nb, na)
+    xy_a = xy_b[src] - (20.0, 10.0) + rng.normal(0, 2.0, (na, 2))
+    da = np.clip(db[src].astype(int) + rng.integers(-9, 10, (na, 128)), 0, 255).astype(np.uint8)
+    Hm = np.array([[1.0, 0, 20.0], [0, 1.0, 10.0], [0, 0, 1.0]])
+    q, t = mo.guided_f32_fast(da, xy_a, db, xy_b, Hm, (800, 800), 25.0, 0.8, chunk=300)
+    rq, rt = mo.guided_fp64_fast(da, xy_a, db, xy_b, Hm, (800, 800), 25.0, 0.8)
+    assert len(rq) > 500
+    np.testing.assert_array_equal(q, rq)
+    np.testing.assert_array_equal(t, rt)
