@@ -1,7 +1,7 @@
 """Autotuner of libfsx's launch shapes (no counterpart in the reference).
 
 Tunables (fsx_tune_set keys): orientation blocks per image, refinement blocks per image, and the
-float matcher's targets per LDS tile.  Every choice gives identical results (they only change
+float matcher's targets per LDS tile and blocks of 32 queries per wave.  Every choice gives identical results (they only change
 how work is split over workgroups), so the tuner times the candidates on the caller's own
 workload with HIP events and keeps the fastest.  Choices are cached per (device, key,
 workload signature) in memory, and in the JSON file named by FSX_TUNE_CACHE if set.
@@ -16,9 +16,9 @@ import numpy as np
 
 from . import _lib
 
-ORIENT_BLOCKS, REFINE_BLOCKS, FSCREEN_TILE = 0, 1, 2
+ORIENT_BLOCKS, REFINE_BLOCKS, FSCREEN_TILE, FSCREEN_QB = 0, 1, 2, 3
 CANDIDATES = {ORIENT_BLOCKS: (1024, 2048, 4096, 8192), REFINE_BLOCKS: (64, 128, 256, 512),
-              FSCREEN_TILE: (64, 128)}
+              FSCREEN_TILE: (64, 128), FSCREEN_QB: (1, 2)}
 _cache = {}
 
 
@@ -94,8 +94,11 @@ def tune_features(fb, imgs, reps=3):
 
 
 def tune_float_matcher(fm, reps=2):
-    """Targets per LDS tile of the float matcher's screening kernel."""
+    """Blocks of queries per wave, then targets per LDS tile, of the float matcher's screening
+    kernel."""
     sig = ("fmatch", fm.precision, fm.K, tuple(fm.nk[:8]), len(fm.nk))
     run = lambda ev: fm.run(events=ev)           # noqa: E731
+    qb = tune(FSCREEN_QB, run, (1, 2, 5), sig, reps)
     times = tune(FSCREEN_TILE, run, (1, 2, 5), sig, reps)
-    return {"tile": times, "chosen": get(FSCREEN_TILE)}
+    return {"query_blocks": qb, "tile": times, "chosen": get(FSCREEN_TILE),
+            "chosen_query_blocks": get(FSCREEN_QB)}

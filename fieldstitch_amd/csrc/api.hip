@@ -8,7 +8,7 @@ namespace fsx {
 static thread_local char g_err[1024];
 
 // tunable launch shapes: orientation blocks / image, refinement blocks / image, float-screen tile
-static int g_tune[TUNE_N_KEYS] = {4096, 256, 64};
+static int g_tune[TUNE_N_KEYS] = {4096, 256, 64, 1};
 int tune_get(int key) { return g_tune[key]; }
 
 void set_error(const char* fmt, ...) {
@@ -79,6 +79,7 @@ int fsx_tune_set(int key, int value) {
     if (key == TUNE_ORIENT_BLOCKS || key == TUNE_REFINE_BLOCKS)
         FSX_REQUIRE(value >= 16 && value <= 65536, "blocks per image must be in [16, 65536] (got %d)", value);
     if (key == TUNE_FSCREEN_TILE) FSX_REQUIRE(value == 64 || value == 128, "tile must be 64 or 128 (got %d)", value);
+    if (key == TUNE_FSCREEN_QB) FSX_REQUIRE(value == 1 || value == 2, "query blocks must be 1 or 2 (got %d)", value);
     g_tune[key] = value;
     return FSX_OK;
 }

@@ -53,7 +53,8 @@ struct FBatch {
     // per descriptor
     unsigned short* qbf;          // [Ntot][128] bf16 bits (prec 1)
     uint8_t* q8;                  // [Ntot][128] e4m3 codes (prec 2)
-    float* scale;                 // [Ntot] fp8 scale (1 for bf16)
+    float* scale;                 // [Ntot] fp8 scale, a power of two (1 for bf16)
+    uint8_t* sexp;                // [Ntot] its E8M0 code (exponent + 127)
     double* n2;                   // [Ntot] |a|^2 (float64, k order)
     float* nrm;                   // [Ntot] |a| rounded up
     float* qerr;                  // [Ntot] |a - dequant(a^)| rounded up
@@ -181,7 +182,15 @@ __global__ __launch_bounds__(256) void k_f_quant(const FBatch B) {
             ((uint2*)(B.qbf + k * FD))[j] = make_uint2((unsigned)h[0] | ((unsigned)h[1] << 16),
                                                        (unsigned)h[2] | ((unsigned)h[3] << 16));
         } else {
-            sc = amax > 0.0f ? amax / 448.0f : 1.0f;
+            // the smallest power of two with amax / sc <= 448 (exact divisions below), kept
+            // within E8M0's range
+            int ex = 0;
+            if (amax > 0.0f) {
+                const float m = frexpf(amax / 448.0f, &ex);     // amax / 448 = m 2^ex, m in [0.5, 1)
+                if (m == 0.5f) ex -= 1;
+                ex = ex < -127 ? -127 : (ex > 127 ? 127 : ex);
+            }
+            sc = ldexpf(1.0f, ex);
             unsigned w = 0;
 #pragma unroll
             for (int u = 0; u < 4; u++) {
@@ -195,6 +204,7 @@ __global__ __launch_bounds__(256) void k_f_quant(const FBatch B) {
         e2 = red32_add(e2);
         if (j == 0) {
             B.scale[k] = sc;
+            B.sexp[k] = (uint8_t)(B.prec == 2 ? (int)ilogbf(sc) + 127 : 127);
             B.n2[k] = n2;
             // upper bounds as float32 (the float64 sums' rounding is far below the margin)
             const float nr = __double2float_ru(sqrt(n2) * (1.0 + 1e-12));
@@ -238,80 +248,82 @@ typedef float f32x16_t __attribute__((ext_vector_type(16)));
 typedef int i32x8_t __attribute__((ext_vector_type(8)));
 
 
+// Branch-free insertion into an ascending list: the smaller of (d, td[j]) stays at j, the
+// larger moves on.  Equal screened distances keep their arrival order; the certificate only
+// needs the K-th value as a bound, so the order among ties never changes a result.
 template <int K>
-__device__ __forceinline__ void topk_push(float d, int t, float* td, int* ti) {
-    if (!(d < td[K - 1] || (d == td[K - 1] && t < ti[K - 1]))) return;
-    // insert keeping (d, t) ascending
+__device__ __forceinline__ void topk_insert(float d, int t, float* td, int* ti) {
 #pragma unroll
-    for (int j = K - 1; j >= 0; j--) {
-        const bool shift = j > 0 && (d < td[j - 1] || (d == td[j - 1] && t < ti[j - 1]));
-        if (shift) {
-            td[j] = td[j - 1];
-            ti[j] = ti[j - 1];
-        } else {
-            td[j] = d;
-            ti[j] = t;
-            break;
-        }
+    for (int j = 0; j < K; j++) {
+        const bool c = d < td[j];
+        const float od = td[j];
+        const int ot = ti[j];
+        td[j] = c ? d : od;
+        ti[j] = c ? t : ot;
+        d = c ? od : d;
+        t = c ? ot : t;
     }
 }
 
-template <int PREC, int K, int FS_TILE>
+template <int PREC, int K, int FS_TILE, int QB>                   // QB: query blocks per wave
 __global__ __launch_bounds__(256) void k_f_screen(const FBatch B) {
     constexpr int ROWB = PREC == 1 ? FD * 2 : FD;                 // bytes per staged row
     // LDS row pitch: 16 bytes more than a row, so the 16-byte operand reads of 16 consecutive
-    // lanes (consecutive rows) fall on 16 disjoint 4-bank groups (a 256- or 128-byte pitch put
-    // them all on the same banks)
+    // lanes (consecutive rows) fall on 16 disjoint 4-bank groups
     constexpr int PITCH = ROWB + 16;
     constexpr int TBYTES = FS_TILE * PITCH;                       // one staged tile
     constexpr int NV = FS_TILE * ROWB / 16 / 256;                 // 16-byte pieces per thread per tile
     static_assert(NV * 16 * 256 == FS_TILE * ROWB && FS_TILE <= 256, "tile staging layout");
-    // BF16: two tile buffers, the next tile fetched into registers while the current one is
-    // used, then stored into the other buffer; FP8 (half the bytes per row, and the extra
-    // registers would cost a wave per SIMD): one buffer.  Reused for the final merge.
-    constexpr bool DB = PREC == 1;
-    constexpr int NBUF = DB ? 2 : 1;
-    constexpr int SBYTES = NBUF * TBYTES > 256 * K * 8 ? NBUF * TBYTES : 256 * K * 8;
+    constexpr int NBUF = 2;                                       // next tile fetched during this one
+    constexpr int SBYTES = NBUF * TBYTES > 256 * QB * K * 8 ? NBUF * TBYTES : 256 * QB * K * 8;
     __shared__ __attribute__((aligned(16))) uint8_t s_rows[SBYTES];
-    __shared__ float s_cn2[NBUF][FS_TILE], s_cs[NBUF][FS_TILE];
+    __shared__ __attribute__((aligned(16))) float s_c0[NBUF][FS_TILE];   // -|b|^2 / 2, -inf past the end
+    __shared__ int s_sx[NBUF][FS_TILE];                                   // FP8: E8M0 scale of the row
+    __shared__ __attribute__((aligned(16))) float s_acc[256 * 16];        // per lane: one fragment's values
     float* s_md = (float*)s_rows;
-    int* s_mi = (int*)(s_rows + 256 * K * 4);
+    int* s_mi = (int*)(s_rows + 256 * QB * K * 4);
     const FPair P = B.pair[blockIdx.y];
     const FSet sa = B.set[P.a], sb = B.set[P.b];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int q = blockIdx.x * 128 + wv * 32 + (lane & 31);
-    if ((int)blockIdx.x * 128 >= sa.n) return;                      // block-uniform
-    const bool valid = q < sa.n && sa.n >= 2 && sb.n >= 2;
-    const long long ka = sa.off + (q < sa.n ? q : 0);
-    // query operand (this lane's half of its row) and its terms
-    bf16x8_t qb[8];
-    i32x8_t q8[2];
-    if (PREC == 1) {
-        const uint4* src = (const uint4*)(B.qbf + ka * FD);
+    if ((int)blockIdx.x * 128 * QB >= sa.n) return;                 // block-uniform
+    const bool pair_ok = sa.n >= 2 && sb.n >= 2;
+    // query operands, norms and top-K lists of the QB blocks
+    bf16x8_t qb[QB][8];
+    i32x8_t q8[QB][2];
+    int qsx[QB];
+    float qn2[QB], hthr[QB];
+    bool valid[QB];
+    float td[QB][K];
+    int ti[QB][K];
 #pragma unroll
-        for (int s = 0; s < 8; s++) {
-            const uint4 v = src[2 * s + (lane >> 5)];
-            qb[s] = __builtin_bit_cast(bf16x8_t, v);
+    for (int u = 0; u < QB; u++) {
+        const int q = blockIdx.x * 128 * QB + (wv * QB + u) * 32 + (lane & 31);
+        valid[u] = q < sa.n && pair_ok;
+        const long long ka = sa.off + (q < sa.n ? q : 0);
+        if (PREC == 1) {
+            const uint4* src = (const uint4*)(B.qbf + ka * FD);
+#pragma unroll
+            for (int s = 0; s < 8; s++) qb[u][s] = __builtin_bit_cast(bf16x8_t, src[2 * s + (lane >> 5)]);
+        } else {
+            const uint4* src = (const uint4*)(B.q8 + ka * FD);
+#pragma unroll
+            for (int s = 0; s < 2; s++) {
+                const uint4 v0 = src[4 * s + 2 * (lane >> 5)], v1 = src[4 * s + 2 * (lane >> 5) + 1];
+                q8[u][s] = i32x8_t{(int)v0.x, (int)v0.y, (int)v0.z, (int)v0.w, (int)v1.x, (int)v1.y, (int)v1.z, (int)v1.w};
+            }
+            qsx[u] = (int)B.sexp[ka];
         }
-    } else {
-        const uint4* src = (const uint4*)(B.q8 + ka * FD);
+        qn2[u] = (float)B.n2[ka];
+        hthr[u] = valid[u] ? -INFINITY : INFINITY;       // empty list: every real target enters
 #pragma unroll
-        for (int s = 0; s < 2; s++) {
-            const uint4 v0 = src[4 * s + 2 * (lane >> 5)], v1 = src[4 * s + 2 * (lane >> 5) + 1];
-            q8[s] = i32x8_t{(int)v0.x, (int)v0.y, (int)v0.z, (int)v0.w, (int)v1.x, (int)v1.y, (int)v1.z, (int)v1.w};
+        for (int j = 0; j < K; j++) {
+            td[u][j] = INFINITY;
+            ti[u][j] = 0x7fffffff;
         }
-    }
-    const float qn2 = (float)B.n2[ka];
-    const float qs = B.scale[ka];
-    float td[K];
-    int ti[K];
-#pragma unroll
-    for (int j = 0; j < K; j++) {
-        td[j] = INFINITY;
-        ti[j] = 0x7fffffff;
     }
     uint4 pv[NV];
-    float pcn = 0.0f, pcs = 0.0f;
+    float pc0 = 0.0f;
+    int psx = 127;
     auto fetch = [&](int base) {   // 16-byte loads of tile `base` into registers, all threads
         const int nt = min(FS_TILE, sb.n - base);
 #pragma unroll
@@ -327,8 +339,8 @@ __global__ __launch_bounds__(256) void k_f_screen(const FBatch B) {
         }
         if (tid < FS_TILE) {
             const bool ok = tid < nt;
-            pcn = ok ? (float)B.n2[sb.off + base + tid] : 0.0f;
-            pcs = ok ? B.scale[sb.off + base + tid] : 0.0f;
+            pc0 = ok ? -0.5f * (float)B.n2[sb.off + base + tid] : -INFINITY;
+            psx = ok && PREC == 2 ? (int)B.sexp[sb.off + base + tid] : 127;
         }
     };
     auto store = [&](int buf) {
@@ -339,84 +351,126 @@ __global__ __launch_bounds__(256) void k_f_screen(const FBatch B) {
             *(uint4*)(s_rows + buf * TBYTES + r * PITCH + 16 * c) = pv[k];
         }
         if (tid < FS_TILE) {
-            s_cn2[buf][tid] = pcn;
-            s_cs[buf][tid] = pcs;
+            s_c0[buf][tid] = pc0;
+            s_sx[buf][tid] = psx;
         }
     };
-    if (DB) {
-        fetch(0);
-        store(0);
-        __syncthreads();
-    }
+    fetch(0);
+    store(0);
+    __syncthreads();
     int buf = 0;
     for (int base = 0; base < sb.n; base += FS_TILE) {
-        const int nt = min(FS_TILE, sb.n - base);
         const bool more = base + FS_TILE < sb.n;   // block-uniform
-        if (!DB) {
-            fetch(base);
-            store(0);
-            __syncthreads();
-        } else if (more) {
-            fetch(base + FS_TILE);                 // in flight during this tile's MFMA work
-        }
+        if (more) fetch(base + FS_TILE);           // in flight during this tile's MFMA work
         const uint8_t* tile = s_rows + buf * TBYTES;
-        const float* t_cn2 = s_cn2[buf];
-        const float* t_cs = s_cs[buf];
 #pragma unroll
         for (int h = 0; h < FS_TILE / 32; h++) {
+            // C operand: -|b|^2 / 2 of the lane's 16 accumulator rows i*8 + (lane/32)*4 + j
+            f32x16_t c0;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const float4 v = *(const float4*)&s_c0[buf][h * 32 + i * 8 + (lane >> 5) * 4];
+                c0[4 * i] = v.x;
+                c0[4 * i + 1] = v.y;
+                c0[4 * i + 2] = v.z;
+                c0[4 * i + 3] = v.w;
+            }
+            f32x16_t acc[QB];
+#pragma unroll
+            for (int u = 0; u < QB; u++) acc[u] = c0;
             const int rl = h * 32 + (lane & 31);
-            f32x16_t acc = {};
+            const uint4* rp = (const uint4*)(tile + rl * PITCH);
             if (PREC == 1) {
-                const uint4* rp = (const uint4*)(tile + rl * PITCH);
 #pragma unroll
                 for (int s = 0; s < 8; s++) {
                     const bf16x8_t a = __builtin_bit_cast(bf16x8_t, rp[2 * s + (lane >> 5)]);
-                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, qb[s], acc, 0, 0, 0);
+#pragma unroll
+                    for (int u = 0; u < QB; u++) acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, qb[u][s], acc[u], 0, 0, 0);
                 }
             } else {
-                const uint4* rp = (const uint4*)(tile + rl * PITCH);
+                const int asx = s_sx[buf][rl];
 #pragma unroll
                 for (int s = 0; s < 2; s++) {
                     const uint4 v0 = rp[4 * s + 2 * (lane >> 5)], v1 = rp[4 * s + 2 * (lane >> 5) + 1];
                     const i32x8_t a = {(int)v0.x, (int)v0.y, (int)v0.z, (int)v0.w,
                                        (int)v1.x, (int)v1.y, (int)v1.z, (int)v1.w};
-                    // unit block scales (E8M0 127 = 2^0); both operands e4m3 (format 0)
-                    acc = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, q8[s], acc, 0, 0, 0, 0x7f7f7f7f, 0,
-                                                                          0x7f7f7f7f);
+                    // e4m3 operands (format 0) with the rows' / queries' E8M0 scales (byte 0)
+#pragma unroll
+                    for (int u = 0; u < QB; u++)
+                        acc[u] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, q8[u][s], acc[u], 0, 0, 0, asx, 0,
+                                                                                 qsx[u]);
                 }
             }
-            if (valid) {
 #pragma unroll
-                for (int i = 0; i < 4; i++)
+            for (int u = 0; u < QB; u++) {
+                unsigned m = 0;
 #pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        const int r = h * 32 + i * 8 + (lane >> 5) * 4 + j;
-                        if (r >= nt) continue;
-                        const float dot = PREC == 1 ? acc[i * 4 + j] : acc[i * 4 + j] * (qs * t_cs[r]);
-                        const float d = (qn2 + t_cn2[r]) - 2.0f * dot;
-                        topk_push<K>(d, base + r, td, ti);
+                for (int e = 0; e < 16; e++) m |= acc[u][e] > hthr[u] ? (1u << e) : 0u;
+                if (__any(m != 0)) {
+                    // the lane's 16 values to its LDS slot (float4 chunks swizzled by lane & 3
+                    // against bank conflicts), then one iteration per passing element
+                    float* my = s_acc + ((wv * 64 + lane) << 4);
+                    const int sw = lane & 3;
+                    if (m) {
+#pragma unroll
+                        for (int c = 0; c < 4; c++)
+                            *(float4*)(my + 4 * (c ^ sw)) =
+                                make_float4(acc[u][4 * c], acc[u][4 * c + 1], acc[u][4 * c + 2], acc[u][4 * c + 3]);
                     }
+                    if (m) {
+                        // software-pipelined: the next set element's value is read while the
+                        // current one is inserted
+                        int e = __builtin_ctz(m);
+                        m &= m - 1;
+                        float v = my[4 * ((e >> 2) ^ sw) + (e & 3)];
+                        while (true) {
+                            const bool more = m != 0;
+                            const int en = more ? __builtin_ctz(m) : e;
+                            const float vn = my[4 * ((en >> 2) ^ sw) + (en & 3)];
+                            m = more ? (m & (m - 1)) : 0u;
+                            if (v > hthr[u]) {             // the threshold may have moved
+                                const int r = h * 32 + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3);
+                                topk_insert<K>(qn2[u] - 2.0f * v, base + r, td[u], ti[u]);
+                                const float tau = td[u][K - 1];
+                                hthr[u] = 0.5f * (qn2[u] - tau) - 0x1p-20f * (fabsf(qn2[u]) + fabsf(tau)) - 1e-30f;
+                            }
+                            if (!more) break;
+                            e = en;
+                            v = vn;
+                        }
+                    }
+                    // both half-waves of a query gate with the tighter threshold: an element of
+                    // the union's top-K lies below both halves' K-th values
+                    hthr[u] = fmaxf(hthr[u], __shfl_xor(hthr[u], 32));
+                }
             }
         }
-        if (DB && more) store(buf ^ 1);   // the other buffer was last read before the previous barrier
+        if (more) store(buf ^ 1);   // the other buffer was last read before the previous barrier
         __syncthreads();
-        if (DB) buf ^= 1;
+        buf ^= 1;
     }
     // merge the two half-waves of each query
 #pragma unroll
-    for (int j = 0; j < K; j++) {
-        s_md[tid * K + j] = td[j];
-        s_mi[tid * K + j] = ti[j];
-    }
-    __syncthreads();
-    if (lane < 32 && valid) {
-#pragma unroll
-        for (int j = 0; j < K; j++) topk_push<K>(s_md[(tid + 32) * K + j], s_mi[(tid + 32) * K + j], td, ti);
-        const long long g = P.q_off + q;
+    for (int u = 0; u < QB; u++)
 #pragma unroll
         for (int j = 0; j < K; j++) {
-            B.topd[g * FK_MAX + j] = td[j];
-            B.topi[g * FK_MAX + j] = ti[j];
+            s_md[(u * 256 + tid) * K + j] = td[u][j];
+            s_mi[(u * 256 + tid) * K + j] = ti[u][j];
+        }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < QB; u++) {
+        if (lane < 32 && valid[u]) {
+#pragma unroll
+            for (int j = 0; j < K; j++)
+                topk_insert<K>(s_md[(u * 256 + tid + 32) * K + j], s_mi[(u * 256 + tid + 32) * K + j], td[u], ti[u]);
+            const int q = blockIdx.x * 128 * QB + (wv * QB + u) * 32 + lane;
+            const long long g = P.q_off + q;
+#pragma unroll
+            for (int j = 0; j < K; j++) {
+                B.topd[g * FK_MAX + j] = td[u][j];
+                B.topi[g * FK_MAX + j] = ti[u][j];
+            }
         }
     }
 }
@@ -449,11 +503,16 @@ __device__ __forceinline__ double screen_eps(const FBatch& B, long long ka, int 
     const double u = 0x1p-24;
     const double na = (double)B.nrm[ka], ea = (double)B.qerr[ka];
     const double nb = (double)__uint_as_float(B.set_nmax[s]), eb = (double)__uint_as_float(B.set_emax[s]);
-    const double gam = 2.0 * (FD * u) / (1.0 - FD * u);          // FP32 accumulation, x2 margin
+    // FP32 accumulation of FD products plus the -|b|^2 / 2 start value, x2 margin
+    const double gam = 2.0 * ((FD + 1) * u) / (1.0 - (FD + 1) * u);
     const double qa = na + ea, qb = nb + eb;
-    const double dot_err = na * eb + qb * ea + (gam + 4.0 * u) * qa * qb;
+    // quantization: |a^.b^ - a.b| <= |a| |b - b^| + |b^| |a - a^|
+    const double quant = na * eb + qb * ea;
+    // accumulation over sum |a^_k b^_k| + |b|^2 / 2 <= qa qb + nb^2 / 2, then the float32
+    // roundings of |a|^2, |b|^2 and of d~ = |a|^2 - 2 acc
+    const double accum = gam * (qa * qb + 0.5 * nb * nb);
     const double rnd = 4.0 * u * (na * na + nb * nb + 2.0 * qa * qb);
-    return (2.0 * dot_err + rnd) * 1.01 + 1e-30;
+    return (2.0 * (quant + accum) + rnd) * 1.01 + 1e-30;
 }
 
 __device__ __forceinline__ void f_accept(const FBatch& B, long long g, const FPair& P, int t, double b1) {
@@ -715,6 +774,31 @@ __global__ void k_fp8_probe(float* out) {
     for (int i = 0; i < 16; i++) out[lane * 16 + i] = acc[i];
 }
 
+// The same operands with E8M0 scales per lane: lane l scales A row l % 32 by 2^((l%32) % 5 - 2)
+// and B column l % 32 by 2^((l%32) % 3 - 1) (code = exponent + 127, byte 0), the form the
+// screen uses for its power-of-two per-descriptor scales; out: 64 x 16 floats.
+__global__ void k_fp8_scale_probe(float* out) {
+    const int lane = threadIdx.x, r = lane & 31, h = lane >> 5;
+    i32x8_t a, b;
+#pragma unroll
+    for (int w = 0; w < 8; w++) {
+        unsigned wa = 0, wb = 0;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int e = 4 * w + u;
+            wa |= f2e4m3((float)((r * 7 + h * 3 + e * 5) % 11 - 5)) << (8 * u);
+            wb |= f2e4m3((float)((r * 3 + h * 5 + e * 7) % 13 - 6)) << (8 * u);
+        }
+        a[w] = (int)wa;
+        b[w] = (int)wb;
+    }
+    const int sa = r % 5 - 2 + 127, sb = r % 3 - 1 + 127;
+    f32x16_t acc = {};
+    acc = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, acc, 0, 0, 0, sa, 0, sb);
+#pragma unroll
+    for (int i = 0; i < 16; i++) out[lane * 16 + i] = acc[i];
+}
+
 // ------------------------------------------------------------------------------------------
 // Host side
 
@@ -725,7 +809,7 @@ struct FPlan {
     std::vector<FPair> pair;
     long long n_kp = 0, n_query = 0, n_targets = 0, n_out = 0, n_stat = 0;
     int max_na = 0;
-    size_t off_stat, off_set, off_pair, off_qbf, off_q8, off_scale, off_n2, off_nrm, off_qerr, off_nmax, off_emax, off_topd,
+    size_t off_stat, off_set, off_pair, off_qbf, off_q8, off_scale, off_sexp, off_n2, off_nrm, off_qerr, off_nmax, off_emax, off_topd,
         off_topi, off_best, off_b1, off_tkey, off_tq, off_fb, off_cnt, off_outq, off_outt, off_outn, total;
 };
 
@@ -766,6 +850,7 @@ static int fplan(int n_set, const int* n_kp, int n_pair, const int* pa, const in
     M.off_qbf = o; o = fa256(o + 2 * FD * nk);
     M.off_q8 = o; o = fa256(o + FD * nk);
     M.off_scale = o; o = fa256(o + 4 * nk);
+    M.off_sexp = o; o = fa256(o + nk);
     M.off_n2 = o; o = fa256(o + 8 * nk);
     M.off_nrm = o; o = fa256(o + 4 * nk);
     M.off_qerr = o; o = fa256(o + 4 * nk);
@@ -834,6 +919,7 @@ int fsx_fmatch_batch(int n_set, const int* n_kp, const float* desc_dev, int n_pa
     b.qbf = (unsigned short*)(w + M.off_qbf);
     b.q8 = (uint8_t*)(w + M.off_q8);
     b.scale = (float*)(w + M.off_scale);
+    b.sexp = (uint8_t*)(w + M.off_sexp);
     b.n2 = (double*)(w + M.off_n2);
     b.nrm = (float*)(w + M.off_nrm);
     b.qerr = (float*)(w + M.off_qerr);
@@ -866,12 +952,18 @@ int fsx_fmatch_batch(int n_set, const int* n_kp, const float* desc_dev, int n_pa
     FSX_LAUNCH_CHECK();
     if (ev) FSX_HIP(hipEventRecord(ev[1], st));
     if (M.n_query > 0 && K > 0) {
-        const dim3 grid((unsigned)((M.max_na + 127) / 128), (unsigned)n_pair);
         const bool t128 = tune_get(TUNE_FSCREEN_TILE) == 128;
+        const int qb = tune_get(TUNE_FSCREEN_QB);
+        const dim3 grid((unsigned)((M.max_na + 128 * qb - 1) / (128 * qb)), (unsigned)n_pair);
 #define FSX_SCREEN(P_, K_)                                                                            \
     if (precision == P_ && K == K_) {                                                                 \
-        if (t128) hipLaunchKernelGGL((k_f_screen<P_, K_, 128>), grid, dim3(256), 0, st, b);           \
-        else hipLaunchKernelGGL((k_f_screen<P_, K_, 64>), grid, dim3(256), 0, st, b);                 \
+        if (qb == 2) {                                                                                \
+            if (t128) hipLaunchKernelGGL((k_f_screen<P_, K_, 128, 2>), grid, dim3(256), 0, st, b);    \
+            else hipLaunchKernelGGL((k_f_screen<P_, K_, 64, 2>), grid, dim3(256), 0, st, b);          \
+        } else {                                                                                      \
+            if (t128) hipLaunchKernelGGL((k_f_screen<P_, K_, 128, 1>), grid, dim3(256), 0, st, b);    \
+            else hipLaunchKernelGGL((k_f_screen<P_, K_, 64, 1>), grid, dim3(256), 0, st, b);          \
+        }                                                                                             \
     }
         FSX_SCREEN(1, 8)
         FSX_SCREEN(1, 2)
@@ -924,6 +1016,13 @@ int fsx_fmatch_results(int n_set, const int* n_kp, int n_pair, const int* pa, co
         stats[2] = n ? sum / (double)n : 0.0;
         stats[3] = (double)n;
     }
+    return FSX_OK;
+}
+
+int fsx_fp8_scale_probe(float* out_dev, void* stream) {
+    FSX_REQUIRE(out_dev, "null pointer argument");
+    hipLaunchKernelGGL(k_fp8_scale_probe, dim3(1), dim3(64), 0, (hipStream_t)stream, out_dev);
+    FSX_LAUNCH_CHECK();
     return FSX_OK;
 }
 

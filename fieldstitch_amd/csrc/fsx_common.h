@@ -27,7 +27,8 @@ enum TuneKey {
     TUNE_ORIENT_BLOCKS = 0,    // orientation blocks (one wave each) per image
     TUNE_REFINE_BLOCKS = 1,    // refinement blocks per image
     TUNE_FSCREEN_TILE = 2,     // float matcher: targets per LDS tile (64 or 128)
-    TUNE_N_KEYS = 3
+    TUNE_FSCREEN_QB = 3,       // float matcher: blocks of 32 queries per wave (1 or 2)
+    TUNE_N_KEYS = 4
 };
 int tune_get(int key);
 

@@ -487,6 +487,14 @@ def fp8_layout_probe():
     return out.view(64, 16).cpu().numpy()
 
 
+def fp8_scale_probe():
+    """Run the FP8 MFMA probe with per-lane E8M0 scales; returns (64 lanes, 16) float32."""
+    import torch
+    out = torch.zeros(64 * 16, dtype=torch.float32, device="cuda")
+    _lib.call("fsx_fp8_scale_probe", C.c_void_p(out.data_ptr()), C.c_void_p(_lib.stream_handle()))
+    return out.view(64, 16).cpu().numpy()
+
+
 def _guided(A, B, Hm, radius, kc, ratio, form="mfma"):
     """matching.py:113-152: match A's features to B's keypoints within `radius` of their
     predicted position Hm(A.und), the kc spatially nearest (kc >= len(B) or 0: all)."""

@@ -30,7 +30,8 @@ int fsx_last_error(char *buf, size_t len);
 /* Launch-shape tunables, process-wide (no reference counterpart; set by
  * fieldstitch_amd.autotune): key 0 = orientation blocks per image (16..65536, default 4096),
  * 1 = refinement blocks per image (16..65536, default 256), 2 = float matcher targets per LDS
- * tile (64 or 128, default 64).  Not synchronised with running work: set between batches. */
+ * tile (64 or 128, default 64), 3 = float matcher blocks of 32 queries per wave (1 or 2,
+ * default 1).  Not synchronised with running work: set between batches. */
 int fsx_tune_set(int key, int value);
 int fsx_tune_get(int key, int *value);
 
@@ -257,6 +258,11 @@ int fsx_fmatch_results(int n_set, const int *n_kp, int n_pair, const int *pa, co
  * small operands, to check the format and lane layout the float matcher assumes (no
  * reference counterpart). */
 int fsx_fp8_layout_probe(float *out_dev, void *stream);
+
+/* fsx_fp8_layout_probe's operands with E8M0 scales per lane (A row l % 32 scaled by
+ * 2^((l%32) % 5 - 2), B column l % 32 by 2^((l%32) % 3 - 1)): checks how the float matcher's
+ * power-of-two per-descriptor scales enter the MFMA (no reference counterpart). */
+int fsx_fp8_scale_probe(float *out_dev, void *stream);
 
 /* ---------------------------------------------------------------------------------------
  * Learned match filter: BF16 MLP kernels (no reference counterpart; trained on the inlier

@@ -97,10 +97,13 @@ def test_autotune_features_and_float_matcher(gpu):
     da, db, _ = synth.descriptor_pair(rng, 900, 1100)
     fm = FloatMatcher([da, db], [(0, 1)], ratio=0.8, precision="bf16")
     res = {}
-    for tile in (64, 128):
-        autotune.set(autotune.FSCREEN_TILE, tile)
-        fm.run()
-        res[tile] = fm.results()[0]
-    assert np.array_equal(res[64][0], res[128][0]) and np.array_equal(res[64][1], res[128][1])
+    for qb in (1, 2):
+        for tile in (64, 128):
+            autotune.set(autotune.FSCREEN_QB, qb)
+            autotune.set(autotune.FSCREEN_TILE, tile)
+            fm.run()
+            res[(qb, tile)] = fm.results()[0]
+    ref = res[(1, 64)]
+    assert all(np.array_equal(r[0], ref[0]) and np.array_equal(r[1], ref[1]) for r in res.values())
     t = autotune.tune_float_matcher(fm, reps=2)
-    assert t["chosen"] in (64, 128)
+    assert t["chosen"] in (64, 128) and t["chosen_query_blocks"] in (1, 2)

@@ -21,6 +21,22 @@ def test_fp8_layout(gpu):
                 assert out[lane, i * 4 + j] == Cm[i * 8 + (lane >> 5) * 4 + j, lane & 31]
 
 
+def test_fp8_scale_layout(gpu):
+    """E8M0 scales: lane l's scale applies to A row l % 32 (its K block) and to B column l % 32."""
+    from fieldstitch_amd import matching
+    out = matching.fp8_scale_probe()
+    f = np.array([[[(i * 7 + h * 3 + e * 5) % 11 - 5 for e in range(32)] for h in range(2)] for i in range(32)])
+    g = np.array([[[(j * 3 + h * 5 + e * 7) % 13 - 6 for e in range(32)] for h in range(2)] for j in range(32)])
+    Cm = np.einsum("ihe,jhe->ij", f, g).astype(np.float64)
+    sa = 2.0 ** (np.arange(32) % 5 - 2)
+    sb = 2.0 ** (np.arange(32) % 3 - 1)
+    Cm = Cm * sa[:, None] * sb[None, :]
+    for lane in range(64):
+        for i in range(4):
+            for j in range(4):
+                assert out[lane, i * 4 + j] == Cm[i * 8 + (lane >> 5) * 4 + j, lane & 31]
+
+
 def _oracle(da, db, ratio):
     q, t, _ = mo.match_fp64(da, db, ratio)
     return q, t
