@@ -186,17 +186,23 @@ __device__ __forceinline__ void xcd_tile(int& tx, int& ty, int& tz) {
 // Fast form: the products are fused (acc = fma(s, k, acc)).  Both forms evaluate the same
 // non-negative terms (image data and Gaussian taps are >= 0, the pair sums are identical), so
 // each lies within (2*R2 + 1) * 2^-53 * S of the exact sum S; they differ by < 41 * 2^-53 * S
-// < 2^-47 * acc.  If acc * (1 - 2^-46) and acc * (1 + 2^-46) round to the same float32, the
-// reference's value, which lies between them, rounds to it too (rounding is monotone);
-// otherwise (an f32 rounding boundary within 2^-46 relative, ~1 output in 2^21) the
-// reference's unfused order is evaluated.
+// < 2^-47 * acc, i.e. by less than 64 float64 ulps of acc.  Round-to-nearest float32 only
+// jumps at the midpoints between float32 values, which are the float64 values whose 29
+// mantissa bits below float32 precision read 0x10000000; if acc's low 29 bits are more than
+// 256 ulps away from that pattern, no midpoint separates acc from the reference's value and
+// both round to the same float32 (rounding is monotone).  Otherwise (~1 output in 2^20), and
+// for 0 < acc < 2^-120 where float32 is subnormal, the reference's unfused order is evaluated.
 template <int R2>
 __device__ __forceinline__ float blur_point(const double* w, const HalfTaps& tp) {
     double acc = w[R2] * tp.k[0];
 #pragma unroll
     for (int j = R2; j >= 1; --j) acc = __builtin_fma(w[R2 - j] + w[R2 + j], tp.k[j], acc);
-    const float lo = (float)(acc * (1.0 - 0x1p-46)), hi = (float)(acc * (1.0 + 0x1p-46));
-    if (__builtin_expect(lo == hi, 1)) return lo;
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(acc);
+    const unsigned low29 = (unsigned)bits & 0x1fffffffu;
+    const bool near_mid = low29 - (0x10000000u - 256u) <= 512u;
+    // acc >= 0: the biased exponent; 0 itself is exact (all terms are zero)
+    const bool tiny = bits != 0ull && (unsigned)(bits >> 52) < 1023u - 120u;
+    if (__builtin_expect(!(near_mid || tiny), 1)) return (float)acc;
     double ex = w[R2] * tp.k[0];
 #pragma unroll
     for (int j = R2; j >= 1; --j) ex += (w[R2 - j] + w[R2 + j]) * tp.k[j];

@@ -165,8 +165,10 @@ void fso_blur(const float *in, float *out, int R, int C, const float *k, int kle
 /* CPU model of the device's fused blur evaluation (fieldstitch_amd/csrc/sift.hip,
  * blur_point; not part of the reference).  For n windows of 2*r2+1 float32 values and the
  * half taps kh[0..r2] (centre first), the reference order (as in fso_blur above) is compared
- * with the fused form acc = fma(w[-j] + w[+j], k[j], acc); with use_check, the fused value is
- * accepted only when acc*(1-margin) and acc*(1+margin) round to the same float32, else the
+ * with the fused form acc = fma(w[-j] + w[+j], k[j], acc).  use_check 1: the fused value is
+ * accepted only when acc*(1-margin) and acc*(1+margin) round to the same float32; use_check 2
+ * (the device's test): accepted unless acc's 29 mantissa bits below float32 precision lie within
+ * `margin` float64 ulps of the midpoint pattern 0x10000000, or 0 < acc < 2^-120.  Otherwise the
  * reference order is evaluated (counted in *n_fallback).  Returns the number of windows whose
  * float32 result differs from the reference's. */
 int64_t fso_fused_blur_check(const float *win, int64_t n, int r2, const float *kh, double margin, int use_check,
@@ -181,7 +183,19 @@ int64_t fso_fused_blur_check(const float *win, int64_t n, int r2, const float *k
         double acc = (double)w[r2] * k[0];
         for (int j = r2; j >= 1; j--) acc = fma((double)w[r2 - j] + (double)w[r2 + j], k[j], acc);
         float out;
-        if (use_check) {
+        if (use_check == 2) {
+            uint64_t bits;
+            memcpy(&bits, &acc, 8);
+            const uint32_t low29 = (uint32_t)bits & 0x1fffffffu, m = (uint32_t)margin;
+            const int near_mid = (uint32_t)(low29 - (0x10000000u - m)) <= 2u * m;
+            const int tiny = bits != 0 && (uint32_t)(bits >> 52) < 1023u - 120u;
+            if (!(near_mid || tiny)) {
+                out = (float)acc;
+            } else {
+                out = (float)ref;
+                fb++;
+            }
+        } else if (use_check) {
             const float lo = (float)(acc * (1.0 - margin)), hi = (float)(acc * (1.0 + margin));
             if (lo == hi) {
                 out = lo;

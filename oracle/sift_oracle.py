@@ -104,15 +104,17 @@ def blur(img: np.ndarray, k: np.ndarray) -> np.ndarray:
     return out
 
 
-def fused_blur_check(windows: np.ndarray, half_taps: np.ndarray, margin: float, use_check: bool = True):
+def fused_blur_check(windows: np.ndarray, half_taps: np.ndarray, margin: float, use_check=True):
     """Model of the device's fused blur evaluation against the reference order on (n, 2*r2+1)
-    float32 windows: returns (windows whose float32 result differs, fallbacks taken)."""
+    float32 windows: returns (windows whose float32 result differs, fallbacks taken).
+    use_check: False = the fused value as is; True / 1 = accept when acc * (1 -+ margin) round
+    alike; 2 = the device's test (margin = float64 ulps from a float32 rounding midpoint)."""
     w = np.ascontiguousarray(windows, np.float32)
     kh = np.ascontiguousarray(half_taps, np.float32)
     r2 = (w.shape[1] - 1) // 2
     assert w.shape[1] == 2 * r2 + 1 and len(kh) == r2 + 1
     fb = C.c_int64(0)
-    bad = lib().fso_fused_blur_check(w.reshape(-1), w.shape[0], r2, kh, float(margin), int(bool(use_check)),
+    bad = lib().fso_fused_blur_check(w.reshape(-1), w.shape[0], r2, kh, float(margin), int(use_check),
                                      C.byref(fb))
     return int(bad), int(fb.value)
 

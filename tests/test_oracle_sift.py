@@ -252,3 +252,30 @@ def test_fused_blur_rounding_argument(layer):
     G = rng.random((200000, 2 * r2 + 1), dtype=np.float32) ** 3
     bad_g, fb_g = so.fused_blur_check(G, kh, 2.0 ** -46, use_check=True)
     assert bad_g == 0 and fb_g < 50, fb_g
+
+
+@pytest.mark.parametrize("layer", [0, 2, 3, 4, 5])
+def test_fused_blur_midpoint_bits_test(layer):
+    """The device's cheaper test: the fused value is taken unless its low 29 float64 mantissa
+    bits lie within 256 ulps of a float32 rounding midpoint (or it is below 2^-120).  Near
+    midpoints it never disagrees with the reference order and falls back almost always; with
+    no margin (only the exact midpoint pattern caught) it does disagree, so the steered windows
+    reach the boundary; ordinary image windows and exact zeros: no disagreement and almost no
+    fallbacks."""
+    full = so.gauss_kernel(so.sigmas()[layer])
+    r2 = len(full) // 2
+    kh = full[r2:].copy()
+    rng = np.random.default_rng(200 + layer)
+    W, S = _near_midpoint_windows(rng, 20000, kh)
+    bad, fb = so.fused_blur_check(W, kh, 256, use_check=2)
+    assert bad == 0 and fb > 0.99 * len(W), (bad, fb)
+    bad_small, _ = so.fused_blur_check(W, kh, 0, use_check=2)
+    assert bad_small > 0
+    U = (rng.integers(0, 256, (200000, 2 * r2 + 1)).astype(np.float32) * np.float32(1 / 255))
+    bad_u, fb_u = so.fused_blur_check(U, kh, 256, use_check=2)
+    assert bad_u == 0 and fb_u < 50, fb_u
+    G = rng.random((200000, 2 * r2 + 1), dtype=np.float32) ** 3
+    bad_g, fb_g = so.fused_blur_check(G, kh, 256, use_check=2)
+    assert bad_g == 0 and fb_g < 50, fb_g
+    Z = np.zeros((1000, 2 * r2 + 1), np.float32)
+    assert so.fused_blur_check(Z, kh, 256, use_check=2) == (0, 0)
