@@ -29,11 +29,22 @@ class CLAHE:
         return (self.tiles_x, self.tiles_y)
 
     def apply(self, gray, out=None, stream=None):
+        """Equalise gray (see the module docstring).  out (optional, device input only): a
+        contiguous uint8 CUDA tensor of the input's shape on its device, written in place.
+        stream: all of the call's work (buffers, kernels, copies) is ordered on it."""
+        import torch
+        from contextlib import nullcontext
+        with torch.cuda.stream(stream) if stream is not None else nullcontext():
+            return self._apply(gray, out, stream)
+
+    def _apply(self, gray, out, stream):
         import torch
         host = isinstance(gray, np.ndarray)
         if host:
             if gray.dtype != np.uint8 or gray.ndim != 2:
                 raise TypeError("CLAHE.apply: expected a uint8 (H, W) image")
+            if out is not None:
+                raise TypeError("CLAHE.apply: out is only accepted with a device input")
             src = torch.from_numpy(np.ascontiguousarray(gray)).cuda()
         else:
             if gray.dtype != torch.uint8 or not gray.is_cuda or gray.ndim not in (2, 3):
@@ -45,7 +56,12 @@ class CLAHE:
         _lib.call("fsx_clahe_workspace_bytes", n, self.tiles_x, self.tiles_y, C.byref(nb))
         if self._ws is None or self._ws.numel() < nb.value or self._ws.device != batch.device:
             self._ws = torch.empty(max(1, nb.value), dtype=torch.uint8, device=batch.device)
-        dst = torch.empty_like(batch) if out is None else out.reshape(batch.shape)
+        if out is not None:
+            if (not isinstance(out, torch.Tensor) or out.dtype != torch.uint8 or not out.is_cuda
+                    or out.device != batch.device or tuple(out.shape) != tuple(src.shape) or not out.is_contiguous()):
+                raise ValueError("CLAHE.apply: out must be a contiguous uint8 CUDA tensor of the input's shape "
+                                 "on its device")
+        dst = torch.empty_like(batch) if out is None else out.view(batch.shape)
         _lib.call("fsx_clahe", n, C.c_void_p(batch.data_ptr()), H, W, C.c_double(self.clip), self.tiles_x,
                   self.tiles_y, C.c_void_p(dst.data_ptr()), C.c_void_p(self._ws.data_ptr()), nb.value,
                   C.c_void_p(_lib.stream_handle(stream)))

@@ -333,6 +333,7 @@ def _gpu_ready():
 
 
 _CLAHE = []
+_CPU = object()          # marks a photo the decoder left to the CPU fallback
 
 
 def _clahe():
@@ -343,18 +344,26 @@ def _clahe():
     return _CLAHE[0]
 
 
-def gpu_worker_main(task_q, res_q, n_threads, wid, batch=16, decode=read_gray, clahe=False):
+def gpu_worker_main(task_q, res_q, n_threads, wid, batch=16, decode=read_gray, clahe=False, decode_task=None,
+                    cpu_fallback=None):
     """features.py:270-381 with the one-photo GPU loop replaced by batches.  ``n_threads``
     decoder threads take tasks (idx, path, out_npz, thumb_path, reduce, nfeat) from task_q
     (None ends a thread) and decode them; the GPU loop gathers what is ready (at least one,
     at most ``batch`` photos), runs them through one FeatureBatch per nfeat (the reference's
     sift_gpu settings: max_kp = 2.2 nfeat, contrast 0.015, edge 12), writes each feature file
     with _save and posts ("ok", (idx, n, err, timings)).  Also posted: ("gpu", wid, info) when
-    ready, ("gpu_off", wid, msg) if the GPU cannot be used (there is no CPU path here: every
-    photo is then reported failed), and ("exit", "gpu%d" % wid) at the end.  thumb_path and
+    ready, ("gpu_off", wid, msg) if the GPU cannot be used (without cpu_fallback every photo is
+    then reported failed), and ("exit", "gpu%d" % wid) at the end.  thumb_path and
     reduce belong to the decoder the reference uses and are not interpreted.  With
     ``clahe=True`` the decoder returns the plain gray image and the contrast equalisation of
-    features.py:70-71 (clip 2.0, 8 x 8 tiles) runs on the GPU on each photo of the batch."""
+    features.py:70-71 (clip 2.0, 8 x 8 tiles) runs on the GPU on each photo of the batch.
+    decode_task (optional): called with the whole task tuple instead of decode(path), for a
+    decoder that needs reduce / thumb_path (the reference's prepare, features.py:60-71).
+    cpu_fallback (optional): the CPU path of features.py:355-364 - called with a task tuple, it
+    returns (idx, n, err, timings) (the reference's features.extract); when given, the photos go
+    to it while the GPU is unavailable ("gpu_off") or after a GPU batch fails (then "gpu_off" is
+    posted once and the rest of the photos stay on the CPU).  Both must be picklable when the
+    worker is a spawned process (module-level functions)."""
     import queue
     import threading
     try:
@@ -367,6 +376,7 @@ def gpu_worker_main(task_q, res_q, n_threads, wid, batch=16, decode=read_gray, c
         res_q.put(("gpu_off", wid, gpu_err))
 
     ready = queue.Queue(maxsize=max(4, n_threads, batch))
+    gpu_state = {"off": gpu_err is not None}
 
     def decoder():
         while True:
@@ -375,9 +385,12 @@ def gpu_worker_main(task_q, res_q, n_threads, wid, batch=16, decode=read_gray, c
                 ready.put(None)
                 break
             tt = {"read": 0.0, "decode": 0.0, "bytes": 0, "photos_gpu": 1}
+            if gpu_state["off"] and cpu_fallback is not None:
+                ready.put((task, None, tt, _CPU))       # decoded by the CPU path itself
+                continue
             try:
                 t1 = time.perf_counter()
-                g2 = decode(task[1])
+                g2 = decode_task(task) if decode_task is not None else decode(task[1])
                 tt["decode"] = time.perf_counter() - t1
                 tt["bytes"] = int(g2.nbytes)
                 ready.put((task, g2, tt, None))
@@ -414,23 +427,38 @@ def gpu_worker_main(task_q, res_q, n_threads, wid, batch=16, decode=read_gray, c
             tt["gpu_describe"] = 0.0
             res_q.put(("ok", (task[0], n, None, tt)))
 
+    def on_cpu(task):
+        res_q.put(("ok", cpu_fallback(task)))
+
     def process(items):
         good = []
         for task, g2, tt, err in items:
-            if err or gpu_err:
-                res_q.put(("ok", (task[0], 0, err or ("GPU unavailable: " + gpu_err), tt)))
+            if err is _CPU or (gpu_state["off"] and cpu_fallback is not None and not err):
+                on_cpu(task)
+            elif err or gpu_state["off"]:
+                res_q.put(("ok", (task[0], 0, err or ("GPU unavailable: " + str(gpu_err)), tt)))
             else:
                 good.append((task, g2, tt, None))
         by_nfeat = {}
         for it in good:
             by_nfeat.setdefault(int(it[0][5]), []).append(it)
         for group in by_nfeat.values():
+            if gpu_state["off"] and cpu_fallback is not None:
+                for task, _, _, _ in group:
+                    on_cpu(task)
+                continue
             try:
                 run_group(group)
-            except Exception as e:  # noqa: BLE001 - the batch's photos are reported failed
+            except Exception as e:  # noqa: BLE001 - the GPU failed on this batch
                 msg = "%s: %s" % (type(e).__name__, e)
-                for task, _, tt, _ in group:
-                    res_q.put(("ok", (task[0], 0, msg, tt)))
+                if cpu_fallback is None:
+                    for task, _, tt, _ in group:
+                        res_q.put(("ok", (task[0], 0, msg, tt)))
+                else:
+                    gpu_state["off"] = True
+                    res_q.put(("gpu_off", wid, "GPU failed during processing, continuing on the CPU: %s" % msg[:300]))
+                    for task, _, _, _ in group:
+                        on_cpu(task)
 
     finished = 0
     while finished < n_threads:

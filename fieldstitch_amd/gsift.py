@@ -26,7 +26,12 @@ N_LAYERS = 3
 SIGMA = 1.6
 BORDER = 5
 KEYS = ("x", "y", "size", "angle", "response", "octave", "layer")
-USE_CUDA_KERNELS = [True]     # kept for callers that toggle it (features.gpu_selftest); always on
+# Module state the reference's GPU self-test touches (features.py:179-187 sets
+# USE_CUDA_KERNELS[0] = False and clears _KERNELS when its CUDA kernels misbehave).  Here the
+# kernels are the compiled libfsx, always used; the flag is ignored and the dict stays empty
+# (there is no runtime-compiled kernel cache to clear).
+USE_CUDA_KERNELS = [True]
+_KERNELS = {}
 
 
 def _torch():
@@ -70,20 +75,35 @@ class State:
     describe() samples; the DoG is their difference, never stored), plus the detect() output
     arrays (the reference's ``(xp, _Atlas)``)."""
 
-    def __init__(self, shape, ws, kp_dev, n, max_kp, stream):
+    def __init__(self, shape, ws, kp_dev, n, max_kp, stream, stream_given=False):
         self.shape = shape
         self.ws = ws
         self.kp_dev = kp_dev
         self.n = n
         self.max_kp = max_kp
         self.stream = stream
+        self.stream_given = stream_given
+
+
+def _on_stream(stream):
+    """Context that makes `stream` torch's current stream for the call (all allocations,
+    launches and copies of the call are then ordered on it)."""
+    from contextlib import nullcontext
+    return _torch().cuda.stream(stream) if stream is not None else nullcontext()
 
 
 def detect(gray, gpu=True, max_kp=12000, contrast=0.015, edge=12.0, timings=None, stream=None,
            caps=None):
-    """Detect oriented keypoints (gsift.py:435-477).  Returns (dict of host arrays, state)."""
+    """Detect oriented keypoints (gsift.py:435-477).  Returns (dict of host arrays, state).
+    gpu=False (the reference's CPU path) raises ValueError: this module has no CPU path.
+    stream: every buffer, launch and copy of the call is ordered on it."""
     if not gpu:
         raise ValueError("fieldstitch_amd.gsift runs on the GPU only (gpu=False is the test oracle's job)")
+    with _on_stream(stream):
+        return _detect(gray, max_kp, contrast, edge, timings, stream, caps)
+
+
+def _detect(gray, max_kp, contrast, edge, timings, stream, caps):
     torch = _torch()
     t0 = time.perf_counter()
     if isinstance(gray, torch.Tensor):
@@ -125,13 +145,19 @@ def detect(gray, gpu=True, max_kp=12000, contrast=0.015, edge=12.0, timings=None
         for name, v in zip(("pyramid", "extrema", "refine", "orientation", "select"), stage):
             timings[name] = timings.get(name, 0.0) + v / 1000.0
         timings["detect_wall"] = timings.get("detect_wall", 0.0) + time.perf_counter() - t0
-    return kps, State((H, W), ws, kp, n, max_kp, st)
+    return kps, State((H, W), ws, kp, n, max_kp, st, stream is not None)
 
 
 def describe(state: State, kps, sel, budget=2.5e6, timings=None, rootsift_u8=False):
     """128-d SIFT descriptors (L2 norm 512) for kps[sel] (gsift.py:480-525).  ``budget`` is
     accepted for signature compatibility (the GPU kernel needs no chunking).  With
-    ``rootsift_u8=True`` returns (float32 desc, uint8 RootSIFT desc) (features.py:80-82)."""
+    ``rootsift_u8=True`` returns (float32 desc, uint8 RootSIFT desc) (features.py:80-82).
+    Runs on the stream the state was detected on."""
+    with _on_stream(_torch().cuda.ExternalStream(state.stream) if state.stream_given else None):
+        return _describe(state, kps, sel, timings, rootsift_u8)
+
+
+def _describe(state, kps, sel, timings, rootsift_u8):
     torch = _torch()
     t0 = time.perf_counter()
     sel = np.asarray(sel, np.int64).reshape(-1)

@@ -44,7 +44,14 @@ class TrackBuilder:
         """Device form.  offsets_d: (n_img + 1,) int64; pair_ij_d: (n_pair, 2) int32;
         pair_start_d: (n_pair + 1,) int64; q_d, t_d: (n_edges,) int32, all on the GPU.
         Returns (tid, img, feat) int64 device views of length n_observations, and n_tracks;
-        synchronises the stream once to read the two counts."""
+        synchronises the stream once to read the two counts.  With `stream`, every buffer,
+        launch and copy of the call is ordered on it."""
+        import torch
+        from contextlib import nullcontext
+        with torch.cuda.stream(stream) if stream is not None else nullcontext():
+            return self._run(offsets_d, pair_ij_d, pair_start_d, q_d, t_d, stream)
+
+    def _run(self, offsets_d, pair_ij_d, pair_start_d, q_d, t_d, stream):
         import torch
         n_img = int(offsets_d.shape[0]) - 1
         n_pair = int(pair_ij_d.shape[0])

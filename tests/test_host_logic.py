@@ -119,3 +119,17 @@ def test_pair_seeds_differ_by_pair_and_stage():
     from fieldstitch_amd import match_flow
     seeds = {match_flow.pair_seed(i, j, s) for i in range(30) for j in range(30) if i != j for s in (0x1, 0x5, 0xF)}
     assert len(seeds) == 30 * 29 * 3
+
+
+def test_clahe_rejects_host_out_and_bad_images():
+    """CLAHE.apply argument checks that run before any device work."""
+    from fieldstitch_amd import clahe
+    c = clahe.createCLAHE(2.0, (8, 8))
+    assert c.getClipLimit() == 2.0 and c.getTilesGridSize() == (8, 8)
+    img = np.zeros((32, 48), np.uint8)
+    with pytest.raises(TypeError):
+        c.apply(img, out=np.zeros_like(img))
+    with pytest.raises(TypeError):
+        c.apply(img.astype(np.float32))
+    with pytest.raises(TypeError):
+        c.apply(np.zeros((2, 32, 48), np.uint8))

@@ -100,3 +100,62 @@ def test_worker_gpu_clahe(tmp_path, gpu):
         for key in ("xy", "resp", "red_wh"):
             np.testing.assert_array_equal(a[key], b[key])
         assert np.abs(a["desc"].astype(int) - b["desc"].astype(int)).max() <= 1
+
+
+def _cpu_extract(task):
+    """A stand-in for the reference's CPU extract (features.py:132-154): (idx, n, err, timings)."""
+    return task[0], 1000 + task[0], None, {"photos_cpu": 1}
+
+
+def test_worker_falls_back_to_cpu_without_gpu(tmp_path, monkeypatch):
+    """No GPU and a cpu_fallback: every task of the reference's tuple form goes to the CPU
+    path (the photo that no decoder reads included), as features.py:355-364 does."""
+    from fieldstitch_amd import features
+
+    def no_gpu():
+        raise RuntimeError("no ROCm GPU visible")
+    monkeypatch.setattr(features, "_gpu_ready", no_gpu)
+    tasks = _tasks(tmp_path, [(60, 80), (64, 96), (60, 80)])
+    tasks.append((7, str(tmp_path / "missing.jpg"), str(tmp_path / "7.npz"), str(tmp_path / "t7.jpg"), 4, 1500))
+    msgs = _run(tasks, cpu_fallback=_cpu_extract)
+    assert msgs[0][0] == "gpu_off" and msgs[-1] == ("exit", "gpu0")
+    ok = {m[1][0]: m[1] for m in msgs if m[0] == "ok"}
+    assert sorted(ok) == [0, 1, 2, 7]
+    assert all(r == (k, 1000 + k, None, {"photos_cpu": 1}) for k, r in ok.items())
+
+
+def test_worker_gpu_failure_mid_run_continues_on_cpu(tmp_path, monkeypatch):
+    """The GPU passes its check but fails on a batch: one "gpu_off" message, and that batch's
+    photos and all later ones go to the CPU path."""
+    from fieldstitch_amd import features
+    monkeypatch.setattr(features, "_gpu_ready", lambda: "test device")
+
+    class Broken:
+        def __init__(self, *a, **k):
+            raise RuntimeError("device lost")
+    monkeypatch.setattr(features, "FeatureBatch", Broken)
+    tasks = _tasks(tmp_path, [(60, 80), (64, 96), (60, 80), (64, 96), (60, 80)])
+    msgs = _run(tasks, n_threads=1, batch=2, cpu_fallback=_cpu_extract)
+    assert msgs[0][0] == "gpu"
+    offs = [m for m in msgs if m[0] == "gpu_off"]
+    assert len(offs) == 1 and "device lost" in offs[0][2]
+    ok = {m[1][0]: m[1] for m in msgs if m[0] == "ok"}
+    assert sorted(ok) == [0, 1, 2, 3, 4] and all(r[1] == 1000 + k for k, r in ok.items())
+
+
+def test_worker_decode_task_gets_the_task_tuple(tmp_path, monkeypatch):
+    from fieldstitch_amd import features
+
+    def no_gpu():
+        raise RuntimeError("no ROCm GPU visible")
+    monkeypatch.setattr(features, "_gpu_ready", no_gpu)
+    seen = []
+
+    def dec(task):
+        seen.append((task[0], task[3], task[4]))
+        raise IOError("cannot decode image")
+    tasks = _tasks(tmp_path, [(60, 80)])
+    tasks[0] = tasks[0][:3] + ("thumb.jpg", 2, tasks[0][5])
+    msgs = _run(tasks, decode_task=dec)
+    ok = [m[1] for m in msgs if m[0] == "ok"]
+    assert seen == [(0, "thumb.jpg", 2)] and ok[0][1] == 0 and "cannot decode image" in ok[0][2]
