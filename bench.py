@@ -564,7 +564,8 @@ def bench_filter(args):
     workers = max(1, min(args.gen_workers, os.cpu_count() or 1))
     data = []
     for k, seed in enumerate(range(40, 40 + n_fl)):
-        data.append(fm.flight_candidates(seed, workers=workers, rows=20, cols=20, lowres_frac=0.5))
+        data.append(fm.flight_candidates(seed, workers=workers, rows=20, cols=20, lowres_frac=0.5,
+                                         keep_matcher=k == n_fl - 1))
         log("filter: flight %d: %d pairs, %d candidates, %d plain, %d inliers (%.0f s)" %
             (k, data[-1]["n_pairs"], data[-1]["X"].shape[0], data[-1]["n_plain"], data[-1]["n_inliers"],
              time.time() - t0))
@@ -593,6 +594,7 @@ def bench_filter(args):
     torch.cuda.synchronize()
     t_inf = time.perf_counter() - t2
     ev = fm.evaluate(pt.cpu().numpy(), tau, data[n_tr + 1])
+    stage = filter_stage_times(data[n_tr + 1], net, tau)
     flops_fwd = 2.0 * (288 * 256 + 256 * 256 + 256)
     return {
         "metric": "learned match filter: verified-inlier recall vs the plain set (target >= 99.5 %)",
@@ -608,7 +610,49 @@ def bench_filter(args):
         "threshold": tau,
         "inference": {"candidates_per_s": Xte.shape[0] / t_inf, "tflops": flops_fwd * Xte.shape[0] / t_inf / 1e12},
         "test": ev,
+        "match_and_verify": stage,
     }
+
+
+def filter_stage_times(d, net, tau, reps=5):
+    """Matching + verification of the test flight's pairs with and without the pruning stage:
+    guided matching (PairMatcher.run, or run_filtered with the network and tau) through the host
+    match lists, then homography RANSAC over every pair's list with the labelling settings.
+    Median of `reps` runs each; also the matches reaching verification and the inliers found."""
+    import torch
+    from fieldstitch_amd.verify import ransac_homography
+    pm, xy = d["matcher"], d["xy"]
+    n_hyp, thr, seed = d["ransac"]
+    sz = [torch.as_tensor(np.asarray(v, np.float32)).cuda() for v in d["sizes"]]
+    an = [torch.as_tensor(np.asarray(v, np.float32)).cuda() for v in d["angles"]]
+
+    def once(filtered):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if filtered:
+            pm.run_filtered("mfma", net, tau, sz, an)
+        else:
+            pm.run("mfma")
+        res = pm.results()
+        t1 = time.perf_counter()
+        pa = [xy[i][q] for (i, j), (q, _) in zip(pm.pairs, res)]
+        pb = [xy[j][t] for (i, j), (_, t) in zip(pm.pairs, res)]
+        _, _, cnt = ransac_homography(pa, pb, n_hyp=n_hyp, thr=thr, seed=seed)
+        t2 = time.perf_counter()
+        return t1 - t0, t2 - t1, sum(len(q) for q, _ in res), int(np.sum(cnt))
+
+    out = {}
+    for name, filtered in (("plain", False), ("filtered", True)):
+        once(filtered)
+        runs = [once(filtered) for _ in range(reps)]
+        m = float(np.median([r[0] for r in runs]))
+        v = float(np.median([r[1] for r in runs]))
+        out[name] = {"match_ms": 1e3 * m, "verify_ms": 1e3 * v, "total_ms": 1e3 * (m + v),
+                     "matches_to_verify": runs[-1][2], "inliers": runs[-1][3]}
+    out["pairs"] = len(pm.pairs)
+    out["queries_scored"], out["queries_kept"] = pm.filter_counts()
+    out["speedup"] = out["plain"]["total_ms"] / out["filtered"]["total_ms"]
+    return out
 
 
 def copy_probe(mib):

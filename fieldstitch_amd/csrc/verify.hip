@@ -7,8 +7,10 @@
 // (splitmix64 of (seed, pair, h, draw)), solves the 8x8 DLT system (h33 = 1) by Gaussian
 // elimination with partial pivoting in float64 and counts matches whose reprojection error is
 // below thr; the hypothesis with the most inliers wins (ties: lowest h), and its inliers are
-// the labels.  Plain float64 arithmetic without contraction, in a fixed order, so the CPU
-// restatement reproduces every count.
+// the labels; the winning homography is then refit on them.  The same scheme serves the
+// fundamental matrix (8 points, rank 2) and the similarity (2 points).  Plain float64
+// arithmetic without contraction, in a fixed order, so the CPU restatement reproduces every
+// count and model.
 #include "fsx_common.h"
 #include <math.h>
 
@@ -84,6 +86,7 @@ __device__ __forceinline__ bool is_inlier(const double* h, double x, double y, d
 // h33 = 1, reprojection error in image b.
 struct ModelH {
     static constexpr int K = 4;
+    static constexpr bool refit = true;
     __device__ static bool solve(const double* xa, const double* ya, const double* xb, const double* yb, double* m) {
         return solve_h4(xa, ya, xb, yb, m);
     }
@@ -92,15 +95,71 @@ struct ModelH {
     }
 };
 
+// The closest rank-2 matrix (Frobenius norm) to the 3 x 3 row-major f, in place: f is scaled
+// by its largest |entry|, then replaced by f (I - v v^T), v the eigenvector of f^T f with the
+// smallest eigenvalue (f v v^T = sigma3 u3 v3^T, the last term of f's SVD).  v comes from six
+// cyclic Jacobi sweeps over f^T f (rotations (0,1), (0,2), (1,2); tan of the angle from
+// theta = (a_qq - a_pp) / (2 a_pq), the smaller root), ties of the smallest diagonal to the
+// lower index.  Only +, -, *, / and sqrt in a fixed order, so a CPU restatement gives the same
+// bits.
+__device__ void rank2_project(double* f) {
+    double mx = 0.0;
+    for (int k = 0; k < 9; k++) mx = fmax(mx, fabs(f[k]));
+    if (!(mx > 0.0)) return;
+    for (int k = 0; k < 9; k++) f[k] = f[k] / mx;
+    double S[3][3], V[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            S[i][j] = (f[i] * f[j] + f[3 + i] * f[3 + j]) + f[6 + i] * f[6 + j];
+            V[i][j] = i == j ? 1.0 : 0.0;
+        }
+    for (int sweep = 0; sweep < 6; sweep++)
+        for (int r = 0; r < 3; r++) {
+            const int p = r == 2 ? 1 : 0, q = r == 0 ? 1 : 2;
+            const double apq = S[p][q];
+            if (apq == 0.0) continue;
+            const double theta = (S[q][q] - S[p][p]) / (2.0 * apq);
+            const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+            const double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
+            // S <- R^T S R and V <- V R, R = I but R[p][p] = R[q][q] = c, R[p][q] = sn, R[q][p] = -sn
+            for (int k = 0; k < 3; k++) {
+                const double skp = S[k][p], skq = S[k][q];
+                S[k][p] = c * skp - sn * skq;
+                S[k][q] = sn * skp + c * skq;
+                const double vkp = V[k][p], vkq = V[k][q];
+                V[k][p] = c * vkp - sn * vkq;
+                V[k][q] = sn * vkp + c * vkq;
+            }
+            for (int k = 0; k < 3; k++) {
+                const double spk = S[p][k], sqk = S[q][k];
+                S[p][k] = c * spk - sn * sqk;
+                S[q][k] = sn * spk + c * sqk;
+            }
+        }
+    int kmin = 0;
+    for (int k = 1; k < 3; k++)
+        if (S[k][k] < S[kmin][kmin]) kmin = k;
+    const double v0 = V[0][kmin], v1 = V[1][kmin], v2 = V[2][kmin];
+    for (int i = 0; i < 3; i++) {
+        const double w = (f[3 * i] * v0 + f[3 * i + 1] * v1) + f[3 * i + 2] * v2;
+        f[3 * i] = f[3 * i] - w * v0;
+        f[3 * i + 1] = f[3 * i + 1] - w * v1;
+        f[3 * i + 2] = f[3 * i + 2] - w * v2;
+    }
+}
+
 // Fundamental matrix (the role of cv2.findFundamentalMat in _verify, matching.py:175-191):
 // normalized 8-point algorithm.  Each image's 8 points are shifted to their centroid and scaled
 // to mean distance sqrt(2); the 8 x 9 system [u x, u y, u, v x, v y, v, x, y, 1] f = 0 is
 // reduced by Gaussian elimination with full pivoting (largest remaining |entry|, rows then
 // columns), the one column left unpivoted is set to 1 and the rest back-substituted, so no
-// entry of F is assumed non-zero; F = Tb^T F^ Ta.  No rank-2 projection.  Error: cv2's
-// FM_RANSAC one, the larger of the two squared point-to-epipolar-line distances.
+// entry of F is assumed non-zero; the normalized solution is projected to rank 2 (as cv2's
+// models are: its 7-point solutions by construction, its 8-point one by SVD), then
+// F = Tb^T F^ Ta.  Error: cv2's FM_RANSAC one, the larger of the two squared
+// point-to-epipolar-line distances.
 struct ModelF {
     static constexpr int K = 8;
+    static constexpr bool refit = false;
     __device__ static bool solve(const double* xa, const double* ya, const double* xb, const double* yb, double* m) {
         double ca[2] = {0.0, 0.0}, cb[2] = {0.0, 0.0};
         for (int k = 0; k < 8; k++) {
@@ -174,6 +233,7 @@ struct ModelF {
         }
         double fh[9];
         for (int k = 0; k < 9; k++) fh[perm[k]] = fz[k];
+        rank2_project(fh);
         // F = Tb^T F^ Ta with T = [[s, 0, -s cx], [0, s, -s cy], [0, 0, 1]]
         const double Ta[3][3] = {{sa, 0.0, -sa * ca[0]}, {0.0, sa, -sa * ca[1]}, {0.0, 0.0, 1.0}};
         const double Tb[3][3] = {{sb, 0.0, -sb * cb[0]}, {0.0, sb, -sb * cb[1]}, {0.0, 0.0, 1.0}};
@@ -202,6 +262,7 @@ struct ModelF {
 // from 2 points, M = [[a, -b, tx], [b, a, ty]]; stored as m[0..5] = a, -b, tx, b, a, ty.
 struct ModelS {
     static constexpr int K = 2;
+    static constexpr bool refit = false;
     __device__ static bool solve(const double* xa, const double* ya, const double* xb, const double* yb, double* m) {
         const double dx = xa[1] - xa[0], dy = ya[1] - ya[0];
         const double ex = xb[1] - xb[0], ey = yb[1] - yb[0];
@@ -320,9 +381,162 @@ __global__ __launch_bounds__(256) void k_ransac(const long long* __restrict__ of
         inlier[o + i] = (uint8_t)(have && M::inlier(h, A[2 * i], A[2 * i + 1], Bq[2 * i], Bq[2 * i + 1], thr2));
 }
 
+// Least-squares refit of each pair's homography on the inliers of its winning hypothesis (the
+// role of the Levenberg-Marquardt refinement cv2.findHomography runs on the RANSAC inliers):
+// each image's inliers shifted to their centroid and scaled to mean distance sqrt(2), the
+// normal equations of the DLT rows (h33 = 1) summed over the inliers, solved by Gaussian
+// elimination with partial pivoting, then H = Tb^-1 H^ Ta scaled to h33 = 1.  The inlier flags
+// and count stay those of the winning hypothesis.  A pair with fewer than 5 inliers, a pivot
+// below 1e-12 of the largest diagonal entry, |h33| below 1e-12 of the largest entry or a
+// non-finite result keeps the hypothesis' model.  Sums: lane l adds matches l, l + 64, ... in
+// order (a non-inlier adds 0.0), then the 64 lane sums are added in lane order, so a CPU
+// restatement gives the same bits.  One 64-lane block per pair.
+constexpr int RF_LANES = 64;
+constexpr int RF_TERMS = 44;     // the 36 upper-triangle entries of the 8 x 8 matrix, then the 8 of the right side
+
+template <int M>
+__device__ __forceinline__ void lane_sums(const double (&v)[M], double (*part)[RF_LANES + 1], double* tot) {
+    const int lane = threadIdx.x;
+#pragma unroll
+    for (int e = 0; e < M; e++) part[e][lane] = v[e];
+    __syncthreads();
+    for (int e = lane; e < M; e += RF_LANES) {
+        double t = 0.0;
+        for (int l = 0; l < RF_LANES; l++) t = t + part[e][l];
+        tot[e] = t;
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(RF_LANES) void k_refit_h(const long long* __restrict__ off, const double* __restrict__ pa,
+                                                     const double* __restrict__ pb, const uint8_t* __restrict__ inlier,
+                                                     const int* __restrict__ n_in, double* __restrict__ H_out) {
+    __shared__ double s_part[RF_TERMS][RF_LANES + 1];
+    __shared__ double s_tot[RF_TERMS];
+    const int lane = threadIdx.x, p = blockIdx.x;
+    const int cnt = n_in[p];
+    if (cnt < 5) return;
+    const long long o = off[p];
+    const int n = (int)(off[p + 1] - o);
+    const double* A = pa + 2 * o;
+    const double* B = pb + 2 * o;
+    const uint8_t* fl = inlier + o;
+    const double dn = (double)cnt;
+    double c4[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = lane; i < n; i += RF_LANES) {
+        const bool in = fl[i] != 0;
+        c4[0] = c4[0] + (in ? A[2 * i] : 0.0);
+        c4[1] = c4[1] + (in ? A[2 * i + 1] : 0.0);
+        c4[2] = c4[2] + (in ? B[2 * i] : 0.0);
+        c4[3] = c4[3] + (in ? B[2 * i + 1] : 0.0);
+    }
+    lane_sums<4>(c4, s_part, s_tot);
+    const double cax = s_tot[0] / dn, cay = s_tot[1] / dn, cbx = s_tot[2] / dn, cby = s_tot[3] / dn;
+    double r2[2] = {0.0, 0.0};
+    for (int i = lane; i < n; i += RF_LANES) {
+        const bool in = fl[i] != 0;
+        const double dxa = A[2 * i] - cax, dya = A[2 * i + 1] - cay;
+        const double dxb = B[2 * i] - cbx, dyb = B[2 * i + 1] - cby;
+        r2[0] = r2[0] + (in ? sqrt(dxa * dxa + dya * dya) : 0.0);
+        r2[1] = r2[1] + (in ? sqrt(dxb * dxb + dyb * dyb) : 0.0);
+    }
+    lane_sums<2>(r2, s_part, s_tot);
+    const double ra = s_tot[0], rb = s_tot[1];
+    if (!(ra > 1e-12 && rb > 1e-12)) return;
+    const double sa = 1.4142135623730951 * dn / ra, sb = 1.4142135623730951 * dn / rb;
+    double acc[RF_TERMS];
+#pragma unroll
+    for (int e = 0; e < RF_TERMS; e++) acc[e] = 0.0;
+    for (int i = lane; i < n; i += RF_LANES) {
+        const bool in = fl[i] != 0;
+        const double x = (A[2 * i] - cax) * sa, y = (A[2 * i + 1] - cay) * sa;
+        const double u = (B[2 * i] - cbx) * sb, v = (B[2 * i + 1] - cby) * sb;
+        const double r0[8] = {x, y, 1.0, 0.0, 0.0, 0.0, -(u * x), -(u * y)};
+        const double r1[8] = {0.0, 0.0, 0.0, x, y, 1.0, -(v * x), -(v * y)};
+        int e = 0;
+#pragma unroll
+        for (int a = 0; a < 8; a++)
+#pragma unroll
+            for (int b = a; b < 8; b++, e++) acc[e] = acc[e] + (in ? r0[a] * r0[b] + r1[a] * r1[b] : 0.0);
+#pragma unroll
+        for (int a = 0; a < 8; a++) acc[36 + a] = acc[36 + a] + (in ? r0[a] * u + r1[a] * v : 0.0);
+    }
+    lane_sums<RF_TERMS>(acc, s_part, s_tot);
+    if (lane != 0) return;
+    double N[8][9];
+    {
+        int e = 0;
+        for (int a = 0; a < 8; a++)
+            for (int b = a; b < 8; b++, e++) N[a][b] = N[b][a] = s_tot[e];
+        for (int a = 0; a < 8; a++) N[a][8] = s_tot[36 + a];
+    }
+    double dmax = 0.0;
+    for (int a = 0; a < 8; a++) dmax = fmax(dmax, N[a][a]);
+    const double tol = 1e-12 * dmax;
+    for (int c = 0; c < 8; c++) {
+        int piv = c;
+        double best = fabs(N[c][c]);
+        for (int r = c + 1; r < 8; r++) {
+            const double a = fabs(N[r][c]);
+            if (a > best) {
+                best = a;
+                piv = r;
+            }
+        }
+        if (!(best > tol)) return;
+        if (piv != c)
+            for (int k = 0; k < 9; k++) {
+                const double t = N[c][k];
+                N[c][k] = N[piv][k];
+                N[piv][k] = t;
+            }
+        for (int r = c + 1; r < 8; r++) {
+            const double f = N[r][c] / N[c][c];
+            for (int k = c; k < 9; k++) N[r][k] = N[r][k] - f * N[c][k];
+        }
+    }
+    double h[9];
+    for (int c = 7; c >= 0; c--) {
+        double s = N[c][8];
+        for (int k = c + 1; k < 8; k++) s = s - N[c][k] * h[k];
+        h[c] = s / N[c][c];
+    }
+    h[8] = 1.0;
+    // H = Tb^-1 H^ Ta, Ta = [[sa, 0, -sa cax], [0, sa, -sa cay], [0, 0, 1]], Tb^-1 = [[1/sb, 0, cbx], [0, 1/sb, cby], [0, 0, 1]]
+    const double Ta[3][3] = {{sa, 0.0, -sa * cax}, {0.0, sa, -sa * cay}, {0.0, 0.0, 1.0}};
+    const double ib = 1.0 / sb;
+    const double Tbi[3][3] = {{ib, 0.0, cbx}, {0.0, ib, cby}, {0.0, 0.0, 1.0}};
+    double G[3][3], Hr[9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) G[i][j] = (h[3 * i] * Ta[0][j] + h[3 * i + 1] * Ta[1][j]) + h[3 * i + 2] * Ta[2][j];
+    double hmax = 0.0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            Hr[3 * i + j] = (Tbi[i][0] * G[0][j] + Tbi[i][1] * G[1][j]) + Tbi[i][2] * G[2][j];
+            hmax = fmax(hmax, fabs(Hr[3 * i + j]));
+        }
+    const double h33 = Hr[8];
+    if (!(fabs(h33) > 1e-12 * hmax)) return;
+    bool finite = true;
+    for (int k = 0; k < 9; k++) {
+        Hr[k] = Hr[k] / h33;
+        finite &= isfinite(Hr[k]);
+    }
+    if (!finite) return;
+    for (int k = 0; k < 9; k++) H_out[9 * p + k] = Hr[k];
+}
+
 }  // namespace fsx
 
 using namespace fsx;
+
+template <class M>
+static void launch_refit(int n_pair, const long long* off_dev, const double* pa_dev, const double* pb_dev,
+                         const uint8_t* inlier_dev, const int* n_in_dev, double* M_dev, hipStream_t stream) {
+    if (M::refit)
+        hipLaunchKernelGGL(k_refit_h, dim3(n_pair), dim3(RF_LANES), 0, stream, off_dev, pa_dev, pb_dev, inlier_dev,
+                           n_in_dev, M_dev);
+}
 
 template <class M>
 static int launch_ransac(int n_pair, const long long* off_dev, const double* pa_dev, const double* pb_dev, int n_hyp,
@@ -333,6 +547,7 @@ static int launch_ransac(int n_pair, const long long* off_dev, const double* pa_
     FSX_REQUIRE(off_dev && pa_dev && pb_dev && inlier_dev && H_dev && n_in_dev, "null pointer argument");
     hipLaunchKernelGGL(k_ransac<M>, dim3(n_pair), dim3(256), 0, (hipStream_t)stream, off_dev, pa_dev, pb_dev, n_hyp,
                        thr * thr, seed, nullptr, nullptr, inlier_dev, H_dev, n_in_dev);
+    launch_refit<M>(n_pair, off_dev, pa_dev, pb_dev, inlier_dev, n_in_dev, H_dev, (hipStream_t)stream);
     FSX_LAUNCH_CHECK();
     return FSX_OK;
 }
@@ -347,6 +562,7 @@ static int launch_ransac_per_pair(int n_pair, const long long* off_dev, const do
                 "null pointer argument");
     hipLaunchKernelGGL(k_ransac<M>, dim3(n_pair), dim3(256), 0, (hipStream_t)stream, off_dev, pa_dev, pb_dev, n_hyp,
                        0.0, 0ull, seed_dev, thr_dev, inlier_dev, M_dev, n_in_dev);
+    launch_refit<M>(n_pair, off_dev, pa_dev, pb_dev, inlier_dev, n_in_dev, M_dev, (hipStream_t)stream);
     FSX_LAUNCH_CHECK();
     return FSX_OK;
 }
@@ -356,8 +572,9 @@ extern "C" {
 // Deterministic RANSAC for n_pair sets of matches: pair p's matches are rows off_dev[p] ..
 // off_dev[p+1] of pa_dev / pb_dev (float64 (x, y) in image a / image b).  n_hyp hypotheses
 // per pair (hypothesis h draws its sample with a counter hash of (seed, p, h)); the one with
-// the most inliers wins (ties: lowest h).  Writes inlier_dev (one byte per match), H_dev (9
-// float64 per pair, all zero when no model) and n_in_dev (per pair).
+// the most inliers wins (ties: lowest h).  Writes inlier_dev (one byte per match) and n_in_dev
+// (per pair) from the winner, and H_dev (9 float64 per pair, all zero when no model): the
+// winner refit on its inliers by k_refit_h.
 int fsx_ransac_homography(int n_pair, const long long* off_dev, const double* pa_dev, const double* pb_dev, int n_hyp,
                           double thr, unsigned long long seed, uint8_t* inlier_dev, double* H_dev, int* n_in_dev,
                           void* stream) {

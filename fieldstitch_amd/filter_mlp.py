@@ -158,7 +158,8 @@ def flight_candidates(seed, workers=8, radius=60.0, ratio=0.8, n_hyp=512, thr=3.
     a candidate), y (float32 device labels: 1 = the plain pipeline's match and a RANSAC
     inlier), plain (bool numpy: accepted by the ratio test and the one-query-per-target rule)
     and counts.  keep_matcher=True adds the flight's PairMatcher ("matcher", after its plain run),
-    the per-photo keypoint sizes / angles and per-query arrays for checking the pruning stage."""
+    the per-photo keypoint sizes / angles and positions, the labelling RANSAC's settings and
+    per-query arrays for checking the pruning stage."""
     import torch
     from . import features, synth
     from .matching import PairMatcher
@@ -208,7 +209,7 @@ def flight_candidates(seed, workers=8, radius=60.0, ratio=0.8, n_hyp=512, thr=3.
            "n_plain": int(plain.sum()), "n_inliers": int(label.sum())}
     if keep_matcher:
         out.update({"matcher": pm, "sizes": sizes, "angles": angles, "query_offsets": qoff, "plain_results": res,
-                    "query_label": label > 0.5})
+                    "query_label": label > 0.5, "xy": xys, "ransac": (n_hyp, thr, 0x5EED + seed)})
     return out
 
 

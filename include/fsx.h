@@ -307,17 +307,19 @@ int fsx_mlp_to_bf16(const float *in, void *out, int n, void *stream);
  * with reproducible sampling): pair p's matches are rows off_dev[p] .. off_dev[p+1] (int64
  * device prefix) of pa_dev / pb_dev (float64 (x, y) in image a / b); n_hyp hypotheses per pair
  * from splitmix64 counter draws, 4-point DLT in float64, inlier iff reprojection error < thr.
- * Writes inlier_dev (byte per match), H_dev (9 float64 per pair, zeros without a model) and
- * n_in_dev. */
+ * Writes inlier_dev (byte per match) and n_in_dev from the winning hypothesis, and H_dev (9
+ * float64 per pair, zeros without a model): the winner refit by normalized least squares on its
+ * inliers when it has at least 5 (as cv2.findHomography refines its RANSAC model), h33 = 1. */
 int fsx_ransac_homography(int n_pair, const long long *off_dev, const double *pa_dev, const double *pb_dev, int n_hyp,
                           double thr, unsigned long long seed, uint8_t *inlier_dev, double *H_dev, int *n_in_dev,
                           void *stream);
 
 /* Deterministic fundamental-matrix RANSAC (the role of cv2.findFundamentalMat in _verify,
  * matching.py:175-191): same layout and sampling as fsx_ransac_homography; normalized 8-point
- * algorithm (full-pivot elimination of the 8 x 9 system, no entry of F fixed), inlier iff the
- * larger squared point-to-epipolar-line distance < thr^2.  F_dev: 9 float64 per pair (row-major,
- * x_b^T F x_a = 0). */
+ * algorithm (full-pivot elimination of the 8 x 9 system, no entry of F fixed, each solution
+ * projected to rank 2 as cv2's models are), inlier iff the larger squared
+ * point-to-epipolar-line distance < thr^2.  F_dev: 9 float64 per pair (row-major,
+ * x_b^T F x_a = 0, rank 2). */
 int fsx_ransac_fundamental(int n_pair, const long long *off_dev, const double *pa_dev, const double *pb_dev,
                            int n_hyp, double thr, unsigned long long seed, uint8_t *inlier_dev, double *F_dev,
                            int *n_in_dev, void *stream);

@@ -1,8 +1,9 @@
 """CPU restatement of the deterministic RANSAC (test infrastructure only): the same
 counter-hash sampling, model solves (4-point homography DLT with partial pivoting; normalized
-8-point fundamental matrix with full pivoting; 2-point similarity; float64, no contraction,
-sums in the device's order) and inlier tests as fieldstitch_amd/csrc/verify.hip, vectorised
-over the hypotheses, so every count and label can be compared exactly."""
+8-point fundamental matrix with full pivoting and a Jacobi rank-2 projection; 2-point
+similarity; float64, no contraction, sums in the device's order), inlier tests and
+least-squares refit of the winning homography as fieldstitch_amd/csrc/verify.hip, vectorised
+over the hypotheses, so every count, label and model can be compared exactly."""
 from __future__ import annotations
 
 import numpy as np
@@ -91,10 +92,59 @@ def inliers(h, a, b, thr2):
     return good & (dx * dx + dy * dy < thr2)
 
 
+def rank2_project(f):
+    """Batched closest rank-2 matrix (verify.hip rank2_project): f (m, 9) row-major, scaled by
+    its largest |entry|, then f (I - v v^T) with v the smallest-eigenvalue eigenvector of f^T f
+    from six cyclic Jacobi sweeps (rotations (0,1), (0,2), (1,2)); rows with f = 0 unchanged."""
+    f = np.array(f, np.float64)
+    m = f.shape[0]
+    mx = np.abs(f).max(axis=1)
+    live = mx > 0.0
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        f = np.where(live[:, None], f / np.where(live, mx, 1.0)[:, None], f)
+        S = np.zeros((m, 3, 3))
+        V = np.zeros((m, 3, 3))
+        for i in range(3):
+            for j in range(3):
+                S[:, i, j] = (f[:, i] * f[:, j] + f[:, 3 + i] * f[:, 3 + j]) + f[:, 6 + i] * f[:, 6 + j]
+            V[:, i, i] = 1.0
+        for _ in range(6):
+            for p, q in ((0, 1), (0, 2), (1, 2)):
+                apq = S[:, p, q].copy()
+                rot = apq != 0.0
+                theta = (S[:, q, q] - S[:, p, p]) / (2.0 * np.where(rot, apq, 1.0))
+                t = np.where(theta >= 0.0, 1.0, -1.0) / (np.abs(theta) + np.sqrt(theta * theta + 1.0))
+                c = 1.0 / np.sqrt(t * t + 1.0)
+                sn = t * c
+                c = np.where(rot, c, 1.0)
+                sn = np.where(rot, sn, 0.0)
+                S0, V0 = S.copy(), V.copy()
+                for k in range(3):
+                    S[:, k, p] = np.where(rot, c * S0[:, k, p] - sn * S0[:, k, q], S0[:, k, p])
+                    S[:, k, q] = np.where(rot, sn * S0[:, k, p] + c * S0[:, k, q], S0[:, k, q])
+                    V[:, k, p] = np.where(rot, c * V0[:, k, p] - sn * V0[:, k, q], V0[:, k, p])
+                    V[:, k, q] = np.where(rot, sn * V0[:, k, p] + c * V0[:, k, q], V0[:, k, q])
+                S1 = S.copy()
+                for k in range(3):
+                    S[:, p, k] = np.where(rot, c * S1[:, p, k] - sn * S1[:, q, k], S1[:, p, k])
+                    S[:, q, k] = np.where(rot, sn * S1[:, p, k] + c * S1[:, q, k], S1[:, q, k])
+        d = np.stack([S[:, 0, 0], S[:, 1, 1], S[:, 2, 2]], axis=1)
+        kmin = np.zeros(m, np.int64)
+        for k in (1, 2):
+            kmin = np.where(d[:, k] < d[np.arange(m), kmin], k, kmin)
+        v = V[np.arange(m), :, kmin]
+        out = f.copy()
+        for i in range(3):
+            w = (f[:, 3 * i] * v[:, 0] + f[:, 3 * i + 1] * v[:, 1]) + f[:, 3 * i + 2] * v[:, 2]
+            for j in range(3):
+                out[:, 3 * i + j] = f[:, 3 * i + j] - w * v[:, j]
+    return np.where(live[:, None], out, f)
+
+
 def solve_f8(xa, ya, xb, yb):
     """Batched normalized 8-point fundamental matrix: arrays (m, 8) -> (F (m, 9), valid (m,)).
     Centroid / mean-distance normalization per image, full-pivot elimination of the 8 x 9
-    system, free unknown = 1, F = Tb^T F^ Ta (verify.hip ModelF)."""
+    system, free unknown = 1, projected to rank 2, F = Tb^T F^ Ta (verify.hip ModelF)."""
     m = xa.shape[0]
     r_ = np.arange(m)
     ca = [np.zeros(m), np.zeros(m)]
@@ -149,6 +199,7 @@ def solve_f8(xa, ya, xb, yb):
             fz[:, c] = -s_ / A[:, c, c]
         fh = np.zeros((m, 9))
         fh[r_[:, None], perm] = fz
+        fh = rank2_project(fh)
         z = np.zeros(m)
         o = np.ones(m)
         Ta = [[sa, z, -sa * ca[0]], [z, sa, -sa * ca[1]], [z, z, o]]
@@ -205,6 +256,93 @@ def inliers_s(M, a, b, thr2):
     return dx * dx + dy * dy < thr2
 
 
+def _lane_sums(vals, flags, lanes=64):
+    """verify.hip lane_sums over matches: vals (n, E) with non-inliers as 0.0; lane l adds rows
+    l, l + 64, ... in order, then the 64 lane sums are added in lane order -> (E,)."""
+    n, E = vals.shape
+    v = np.where(flags[:, None], vals, 0.0)
+    R = -(-n // lanes)
+    pad = np.zeros((R * lanes, E))
+    pad[:n] = v
+    pad = pad.reshape(R, lanes, E)
+    part = np.zeros((lanes, E))
+    for r in range(R):
+        part = part + pad[r]
+    tot = np.zeros(E)
+    for l in range(lanes):
+        tot = tot + part[l]
+    return tot
+
+
+def refit_h(a, b, flags, h):
+    """verify.hip k_refit_h: the homography h (9,) refit on the inliers `flags` by normalized
+    least squares (h33 = 1), or h unchanged (fewer than 5 inliers or a degenerate system)."""
+    cnt = int(flags.sum())
+    if cnt < 5:
+        return h
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    dn = float(cnt)
+    c4 = _lane_sums(np.c_[a, b], flags)
+    cax, cay, cbx, cby = c4[0] / dn, c4[1] / dn, c4[2] / dn, c4[3] / dn
+    dxa, dya = a[:, 0] - cax, a[:, 1] - cay
+    dxb, dyb = b[:, 0] - cbx, b[:, 1] - cby
+    r2 = _lane_sums(np.c_[np.sqrt(dxa * dxa + dya * dya), np.sqrt(dxb * dxb + dyb * dyb)], flags)
+    ra, rb = r2[0], r2[1]
+    if not (ra > 1e-12 and rb > 1e-12):
+        return h
+    sa = 1.4142135623730951 * dn / ra
+    sb = 1.4142135623730951 * dn / rb
+    x, y = (a[:, 0] - cax) * sa, (a[:, 1] - cay) * sa
+    u, v = (b[:, 0] - cbx) * sb, (b[:, 1] - cby) * sb
+    z, o = np.zeros(len(a)), np.ones(len(a))
+    r0 = [x, y, o, z, z, z, -(u * x), -(u * y)]
+    r1 = [z, z, z, x, y, o, -(v * x), -(v * y)]
+    cols = [r0[i] * r0[j] + r1[i] * r1[j] for i in range(8) for j in range(i, 8)]
+    cols += [r0[i] * u + r1[i] * v for i in range(8)]
+    tot = _lane_sums(np.stack(cols, axis=1), flags)
+    N = np.zeros((8, 9))
+    e = 0
+    for i in range(8):
+        for j in range(i, 8):
+            N[i, j] = N[j, i] = tot[e]
+            e += 1
+    N[:, 8] = tot[36:]
+    tol = 1e-12 * max(0.0, N[np.arange(8), np.arange(8)].max())
+    for c in range(8):
+        col = np.abs(N[c:, c])
+        piv = c + int(np.argmax(col))
+        if not (col[piv - c] > tol):
+            return h
+        if piv != c:
+            N[[c, piv]] = N[[piv, c]]
+        for r in range(c + 1, 8):
+            f = N[r, c] / N[c, c]
+            N[r, c:] = N[r, c:] - f * N[c, c:]
+    hh = np.zeros(9)
+    for c in range(7, -1, -1):
+        s_ = N[c, 8]
+        for k in range(c + 1, 8):
+            s_ = s_ - N[c, k] * hh[k]
+        hh[c] = s_ / N[c, c]
+    hh[8] = 1.0
+    Ta = [[sa, 0.0, -sa * cax], [0.0, sa, -sa * cay], [0.0, 0.0, 1.0]]
+    ib = 1.0 / sb
+    Tbi = [[ib, 0.0, cbx], [0.0, ib, cby], [0.0, 0.0, 1.0]]
+    G = [[(hh[3 * i] * Ta[0][j] + hh[3 * i + 1] * Ta[1][j]) + hh[3 * i + 2] * Ta[2][j] for j in range(3)]
+         for i in range(3)]
+    Hr = np.array([(Tbi[i][0] * G[0][j] + Tbi[i][1] * G[1][j]) + Tbi[i][2] * G[2][j] for i in range(3)
+                   for j in range(3)])
+    hmax = np.abs(Hr).max()
+    if not (abs(Hr[8]) > 1e-12 * hmax):
+        return h
+    with np.errstate(all="ignore"):
+        Hr = Hr / Hr[8]
+    if not np.isfinite(Hr).all():
+        return h
+    return Hr
+
+
 MODELS = {"homography": (4, solve_h4, inliers), "fundamental": (8, solve_f8, inliers_f),
           "similarity": (2, solve_s2, inliers_s)}
 
@@ -226,4 +364,6 @@ def ransac_pair(p, a, b, n_hyp, thr, seed, model="homography"):
     if cnt.max() < 0:
         return np.zeros(n, bool), np.zeros(9), 0
     best = int(np.argmax(cnt))          # first maximum = lowest hypothesis index
-    return test(h[best:best + 1], a, b, thr * thr)[0], h[best], int(cnt[best])
+    flags = test(h[best:best + 1], a, b, thr * thr)[0]
+    model_out = refit_h(a, b, flags, h[best]) if model == "homography" else h[best]
+    return flags, model_out, int(cnt[best])

@@ -122,3 +122,84 @@ def test_fundamental_and_similarity_degenerate():
     for model in ("fundamental", "similarity"):
         flags, M, cnt = ro.ransac_pair(0, a, b, 64, 1.0, 1, model=model)
         assert cnt == 0 and not flags.any() and not M.any()
+
+
+def test_rank2_projection_is_the_svd_truncation():
+    """The Jacobi projection equals the textbook closest rank-2 matrix (numpy's SVD with the
+    smallest singular value zeroed) of the scaled matrix, and leaves a rank-2 matrix as it is."""
+    rng = np.random.default_rng(11)
+    f = rng.normal(size=(200, 9))
+    f[:20, 6:] = f[:20, :3] * 2.0 - f[:20, 3:6]              # already rank 2
+    f[20:40] *= np.array([1.0, 1e-3, 1.0, 1e-6, 1.0, 1.0, 1e3, 1.0, 1.0])    # badly scaled
+    got = ro.rank2_project(f)
+    for k in range(len(f)):
+        m = (f[k] / np.abs(f[k]).max()).reshape(3, 3)
+        U, s, Vt = np.linalg.svd(m)
+        want = (U[:, :2] * s[:2]) @ Vt[:2]
+        np.testing.assert_allclose(got[k].reshape(3, 3), want, atol=1e-12 * s[0])
+        assert np.linalg.svd(got[k].reshape(3, 3), compute_uv=False)[2] <= 1e-12 * s[0]
+    np.testing.assert_allclose(got[:20], f[:20] / np.abs(f[:20]).max(1, keepdims=True), atol=1e-12)
+    assert not ro.rank2_project(np.zeros((1, 9))).any()
+
+
+def test_fundamental_ransac_model_is_rank2_and_labels_its_inliers():
+    """Planted two-view case: the winning F is rank 2, the flags and count are exactly the
+    inliers of that rank-2 model, and the true inliers are found."""
+    rng = np.random.default_rng(8)
+    a, b, Ft = _two_views(rng, 400, (15.0, 4.0, 0.5))
+    b = b + rng.normal(0.0, 0.3, b.shape)
+    out = rng.random(400) < 0.25
+    b[out] = rng.uniform(0, 2600, (int(out.sum()), 2))
+    flags, F, cnt = ro.ransac_pair(2, a, b, 1024, 1.0, 5, model="fundamental")
+    s = np.linalg.svd(F.reshape(3, 3), compute_uv=False)
+    assert s[2] <= 1e-12 * s[0]
+    np.testing.assert_array_equal(flags, ro.inliers_f(F[None], a, b, 1.0)[0])
+    assert cnt == flags.sum() and (flags & ~out).sum() >= 0.95 * (~out).sum() and (flags & out).sum() <= 3
+
+
+def _dlt_lstsq(a, b):
+    """The same normalized least-squares DLT with h33 = 1, solved by numpy's lstsq on the
+    stacked rows (not the normal equations)."""
+    ca, cb = a.mean(0), b.mean(0)
+    sa = np.sqrt(2.0) / np.linalg.norm(a - ca, axis=1).mean()
+    sb = np.sqrt(2.0) / np.linalg.norm(b - cb, axis=1).mean()
+    x, y = ((a - ca) * sa).T
+    u, v = ((b - cb) * sb).T
+    z, o = np.zeros(len(a)), np.ones(len(a))
+    M = np.r_[np.c_[x, y, o, z, z, z, -u * x, -u * y], np.c_[z, z, z, x, y, o, -v * x, -v * y]]
+    h = np.r_[np.linalg.lstsq(M, np.r_[u, v], rcond=None)[0], 1.0].reshape(3, 3)
+    Ta = np.array([[sa, 0, -sa * ca[0]], [0, sa, -sa * ca[1]], [0, 0, 1.0]])
+    Tb = np.array([[sb, 0, -sb * cb[0]], [0, sb, -sb * cb[1]], [0, 0, 1.0]])
+    H = np.linalg.inv(Tb) @ h @ Ta
+    return H / H[2, 2]
+
+
+def _reproj(H, a, b):
+    H = H.reshape(3, 3)
+    p = np.c_[a, np.ones(len(a))] @ H.T
+    return np.sqrt(np.mean(np.sum((p[:, :2] / p[:, 2:] - b) ** 2, 1)))
+
+
+def test_homography_refit_on_inliers():
+    """The winning homography is refit on its inliers: it equals an independent least-squares
+    solve of the same normalized system, fits the true inliers better than the 4-point winner,
+    and the flags / count stay the winner's."""
+    rng = np.random.default_rng(21)
+    H = np.array([[1.01, -0.04, 550.0], [0.03, 0.98, -60.0], [2e-6, -1e-6, 1.0]])
+    a, b, good = planted(rng, 900, 0.55, H)
+    b[good] += rng.normal(0.0, 0.8, (int(good.sum()), 2))
+    flags, h, cnt = ro.ransac_pair(4, a, b, 256, 3.0, 13)
+    K, solve, test = ro.MODELS["homography"]
+    idx, ok = ro.samples(13, 4, len(a), np.arange(256), K)
+    hyp, valid = solve(a[idx, 0], a[idx, 1], b[idx, 0], b[idx, 1])
+    counts = np.where(valid & ok, test(hyp, a, b, 9.0).sum(1), -1)
+    win = hyp[int(np.argmax(counts))]
+    np.testing.assert_array_equal(flags, test(win[None], a, b, 9.0)[0])
+    assert cnt == counts.max() == flags.sum()
+    np.testing.assert_allclose(h.reshape(3, 3), _dlt_lstsq(a[flags], b[flags]), rtol=1e-9, atol=1e-9)
+    assert _reproj(h, a[good], b[good]) < _reproj(win, a[good], b[good])
+    assert _reproj(h, a[good], b[good]) < 1.2
+    # fewer than 5 inliers: the hypothesis' model is kept
+    few = np.zeros(len(a), bool)
+    few[np.nonzero(flags)[0][:4]] = True
+    assert ro.refit_h(a, b, few, win) is win
