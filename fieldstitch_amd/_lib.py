@@ -86,6 +86,9 @@ SIGNATURES = {
     "fsx_mlp_colsum": [_vp, _i, _i, _i, _i, _vp, _vp, _vp],
     "fsx_mlp_adam": [_vp, _vp, _vp, _vp, _vp, _i, C.c_float, C.c_float, C.c_float, C.c_float, _i, _vp],
     "fsx_mlp_to_bf16": [_vp, _vp, _i, _vp],
+    "fsx_mlp_infer_pack_bytes": [C.POINTER(_sz)],
+    "fsx_mlp_infer_pack": [_vp, _vp, _vp, _vp],
+    "fsx_mlp_infer": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "fsx_match_candidate_offsets": [_i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), _i, C.POINTER(_i),
                                     C.POINTER(_i), C.c_double, C.POINTER(C.c_longlong)],
     "fsx_ransac_homography": [_i, _vp, _vp, _vp, _i, C.c_double, C.c_ulonglong, _vp, _vp, _vp, _vp],

@@ -119,6 +119,7 @@ __global__ __launch_bounds__(256) void k_splitk_sum(const float* __restrict__ pa
 // dZ2[b][j] = dz * w3[j] * (H2[b][j] > 0).  One wave per row, 4 rows per block; block
 // partials part[blk][0..257] = (sum_b dz H2[b][j] for j < 256, sum dz, sum loss).
 constexpr int HID = 256;
+constexpr int N_INP_MLP = 288;      // input width (264 features padded to the GEMM's K step)
 
 __global__ __launch_bounds__(256) void k_head(const unsigned short* __restrict__ H2, const float* __restrict__ w3,
                                               const float* __restrict__ b3, const float* __restrict__ y, int nb,
@@ -210,6 +211,145 @@ __global__ __launch_bounds__(256) void k_to_bf16(const float* __restrict__ in, u
 }  // namespace fsx
 
 using namespace fsx;
+
+// ---- fused inference -------------------------------------------------------------------
+// p = sigmoid(relu(relu(X W1 + b1) W2 + b2) . w3 + b3) for rows of X (bf16 [n][288]) in one
+// launch, the hidden activations rounded to bf16 as the training forward rounds them.  Each
+// wave owns 32 rows and computes the transposed products (hidden units as the MFMA's M, rows
+// as N), so a lane's accumulators are 16 hidden units of its own row: the 32x32x16 BF16 MFMA's
+// C layout holds hidden rows 8 (j / 4) + 4 (lane / 32) + j % 4 of a 32-unit tile, which after
+// bias, ReLU and rounding are exactly the lane's B operand for two k-steps of layer 2 when
+// that layer's weights take the same order (mi_k2).  The weights are packed once
+// (k_mlp_pack) in the operands' lane order, one 16-byte piece per lane, k-step and tile, in
+// 16 KB stages: 9 stages of layer 1 (2 k-steps each, all 8 hidden tiles, so a wave keeps 8
+// independent accumulators), then 8 of layer 2 (one 32-unit output tile, all 16 k-steps).  A
+// block stages them one at a time in LDS (double-buffered) for its 8 waves, so each weight
+// byte read from L2 serves 256 rows.
+constexpr int MI_K1 = N_INP_MLP / 16;            // 18 k-steps of layer 1
+constexpr int MI_K2 = HID / 16;                  // 16 k-steps of layer 2
+constexpr int MI_NT = HID / 32;                  // 8 hidden tiles per layer
+constexpr int MI_S1 = MI_K1 / 2;                 // 9 layer-1 stages of 2 k-steps
+constexpr int MI_STAGE = 8192;                   // bf16 per stage (16 KB)
+constexpr int MI_NSTAGE = MI_S1 + MI_NT;         // 17
+constexpr int MI_PACK = MI_NSTAGE * MI_STAGE;    // bf16 in the packed weights
+constexpr int MI_WAVES = 8, MI_ROWS = 32 * MI_WAVES;
+static_assert(2 * MI_NT * 64 * 8 == MI_STAGE && MI_K2 * 64 * 8 == MI_STAGE && 2 * MI_S1 == MI_K1, "stage layout");
+
+// hidden unit of k-step element i for lane half h in the layer-2 order (see above)
+__host__ __device__ __forceinline__ int mi_k2(int s, int h, int i) { return 16 * s + 8 * (i >> 2) + 4 * h + (i & 3); }
+
+__global__ __launch_bounds__(256) void k_mlp_pack(const unsigned short* __restrict__ W1, const unsigned short* __restrict__ W2,
+                                                  unsigned short* __restrict__ pk) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= MI_PACK) return;
+    const int g = e / MI_STAGE, r = e % MI_STAGE;
+    const int i = r & 7, l = (r >> 3) & 63;
+    if (g < MI_S1) {                                        // [k-step 2 g + sl][tile t][lane][i]
+        const int t = (r >> 9) & 7, sl = r >> 12;
+        pk[e] = W1[(16 * (2 * g + sl) + 8 * (l >> 5) + i) * HID + 32 * t + (l & 31)];
+    } else {                                                // tile g - 9: [k-step s][lane][i]
+        const int s = r >> 9;
+        pk[e] = W2[mi_k2(s, l >> 5, i) * HID + 32 * (g - MI_S1) + (l & 31)];
+    }
+}
+
+__global__ __launch_bounds__(64 * MI_WAVES) void k_mlp_infer(const unsigned short* __restrict__ X, int n,
+                                                            const unsigned short* __restrict__ pk,
+                                                            const float* __restrict__ b1, const float* __restrict__ b2,
+                                                            const float* __restrict__ w3, const float* __restrict__ b3,
+                                                            float* __restrict__ p_out) {
+    __shared__ __attribute__((aligned(16))) unsigned short s_w[2][MI_STAGE];
+    constexpr int NTH = 64 * MI_WAVES;
+    static_assert(2 * NTH * 8 == MI_STAGE, "stage copy: two 16-byte pieces per thread");
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, h = lane >> 5;
+    const long long row = (long long)blockIdx.x * MI_ROWS + wv * 32 + (lane & 31);
+    const bool live = row < n;
+    const uint4* xr = (const uint4*)(X + (size_t)(live ? row : 0) * N_INP_MLP);
+    // staging of the next stage: NP = 2 pieces per thread, kept in st0 / st1
+    uint4 st0 = ((const uint4*)pk)[tid], st1 = ((const uint4*)pk)[tid + NTH];
+    ((uint4*)s_w[0])[tid] = st0;
+    ((uint4*)s_w[0])[tid + NTH] = st1;
+    // layer-1 B operands of stage c: the lane's row, inputs 16 (2 c + sl) + 8 h .. + 7 (a lane
+    // past the last row reads row 0 and writes nothing)
+    uint4 x0 = xr[h], x1 = xr[2 + h];
+    __syncthreads();
+    mf32x16_t acc[MI_NT];
+#pragma unroll
+    for (int t = 0; t < MI_NT; t++)
+#pragma unroll
+        for (int j = 0; j < 16; j++) acc[t][j] = 0.0f;
+    for (int c = 0; c < MI_S1; c++) {
+        const uint4* nsrc = (const uint4*)(pk + (size_t)(c + 1) * MI_STAGE);   // stage c + 1 < 17 always exists
+        st0 = nsrc[tid];
+        st1 = nsrc[tid + NTH];
+        const int cn = c + 1 < MI_S1 ? c + 1 : c;
+        const uint4 y0 = xr[4 * cn + h], y1 = xr[4 * cn + 2 + h];
+        const uint4* w = (const uint4*)s_w[c & 1];
+#pragma unroll
+        for (int t = 0; t < MI_NT; t++)
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(mbf16x8_t, w[t * 64 + lane]),
+                                                             __builtin_bit_cast(mbf16x8_t, x0), acc[t], 0, 0, 0);
+#pragma unroll
+        for (int t = 0; t < MI_NT; t++)
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(mbf16x8_t, w[(MI_NT + t) * 64 + lane]),
+                                                             __builtin_bit_cast(mbf16x8_t, x1), acc[t], 0, 0, 0);
+        // the other buffer was last read before the previous barrier
+        ((uint4*)s_w[(c + 1) & 1])[tid] = st0;
+        ((uint4*)s_w[(c + 1) & 1])[tid + NTH] = st1;
+        x0 = y0;
+        x1 = y1;
+        __syncthreads();
+    }
+    // bias, ReLU, bf16: tile t's elements j = 4q .. 4q + 3 -> k-step 2 t + q / 2, pieces 2 (q % 2), + 1
+    unsigned hb[MI_K2][4];
+#pragma unroll
+    for (int t = 0; t < MI_NT; t++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const float4 bb = *(const float4*)(b1 + 32 * t + 8 * q + 4 * h);
+            const unsigned short r0 = m_f2bf(fmaxf(acc[t][4 * q] + bb.x, 0.0f));
+            const unsigned short r1 = m_f2bf(fmaxf(acc[t][4 * q + 1] + bb.y, 0.0f));
+            const unsigned short r2 = m_f2bf(fmaxf(acc[t][4 * q + 2] + bb.z, 0.0f));
+            const unsigned short r3 = m_f2bf(fmaxf(acc[t][4 * q + 3] + bb.w, 0.0f));
+            hb[2 * t + (q >> 1)][2 * (q & 1)] = (unsigned)r0 | ((unsigned)r1 << 16);
+            hb[2 * t + (q >> 1)][2 * (q & 1) + 1] = (unsigned)r2 | ((unsigned)r3 << 16);
+        }
+    float z = 0.0f;
+    for (int t = 0; t < MI_NT; t++) {
+        const int g = MI_S1 + t;
+        if (g + 1 < MI_NSTAGE) {
+            const uint4* nsrc = (const uint4*)(pk + (size_t)(g + 1) * MI_STAGE);
+            st0 = nsrc[tid];
+            st1 = nsrc[tid + NTH];
+        }
+        const uint4* w = (const uint4*)s_w[g & 1];
+        mf32x16_t a2;
+#pragma unroll
+        for (int j = 0; j < 16; j++) a2[j] = 0.0f;
+#pragma unroll
+        for (int s = 0; s < MI_K2; s++)
+            a2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                __builtin_bit_cast(mbf16x8_t, w[s * 64 + lane]),
+                __builtin_bit_cast(mbf16x8_t, make_uint4(hb[s][0], hb[s][1], hb[s][2], hb[s][3])), a2, 0, 0, 0);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const float4 bb = *(const float4*)(b2 + 32 * t + 8 * q + 4 * h);
+            const float4 ww = *(const float4*)(w3 + 32 * t + 8 * q + 4 * h);
+            z += m_bf2f(m_f2bf(fmaxf(a2[4 * q] + bb.x, 0.0f))) * ww.x;
+            z += m_bf2f(m_f2bf(fmaxf(a2[4 * q + 1] + bb.y, 0.0f))) * ww.y;
+            z += m_bf2f(m_f2bf(fmaxf(a2[4 * q + 2] + bb.z, 0.0f))) * ww.z;
+            z += m_bf2f(m_f2bf(fmaxf(a2[4 * q + 3] + bb.w, 0.0f))) * ww.w;
+        }
+        if (g + 1 < MI_NSTAGE) {
+            ((uint4*)s_w[(g + 1) & 1])[tid] = st0;
+            ((uint4*)s_w[(g + 1) & 1])[tid + NTH] = st1;
+        }
+        __syncthreads();
+    }
+    z += __shfl_xor(z, 32);
+    z += b3[0];
+    if (live && h == 0) p_out[row] = 1.0f / (1.0f + expf(-z));
+}
 
 extern "C" {
 
@@ -330,6 +470,41 @@ int fsx_mlp_adam(float* w, float* m, float* v, const float* g, void* w16, int n,
     const float c1 = 1.0f - powf(beta1, (float)step), c2 = 1.0f - powf(beta2, (float)step);
     hipLaunchKernelGGL(k_adam, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, w, m, v, g,
                        (unsigned short*)w16, n, lr, beta1, beta2, eps, c1, c2);
+    FSX_LAUNCH_CHECK();
+    return FSX_OK;
+}
+
+// Bytes of the packed inference weights (fsx_mlp_infer_pack).
+int fsx_mlp_infer_pack_bytes(size_t* bytes) {
+    FSX_REQUIRE(bytes, "null pointer argument");
+    *bytes = (size_t)MI_PACK * 2;
+    return FSX_OK;
+}
+
+// Packs the bf16 weights W1 ([288][256]) and W2 ([256][256]) for fsx_mlp_infer into `packed`
+// (fsx_mlp_infer_pack_bytes bytes, 16-byte aligned); run again after the weights change.
+int fsx_mlp_infer_pack(const void* W1, const void* W2, void* packed, void* stream) {
+    FSX_REQUIRE(W1 && W2 && packed, "null pointer argument");
+    FSX_REQUIRE(((uintptr_t)packed & 15) == 0, "packed weights must be 16-byte aligned");
+    hipLaunchKernelGGL(k_mlp_pack, dim3((MI_PACK + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                       (const unsigned short*)W1, (const unsigned short*)W2, (unsigned short*)packed);
+    FSX_LAUNCH_CHECK();
+    return FSX_OK;
+}
+
+// p_out[r] = sigmoid(relu(relu(X[r] W1 + b1) W2 + b2) . w3 + b3) for r < n, X bf16 [n][288]
+// (16-byte aligned rows), weights packed by fsx_mlp_infer_pack; hidden activations rounded to
+// bf16 as in the training forward (fsx_mlp_gemm epilogue 0 + fsx_mlp_head).
+int fsx_mlp_infer(const void* X, int n, const void* packed, const float* b1, const float* b2, const float* w3,
+                  const float* b3, float* p_out, void* stream) {
+    FSX_REQUIRE(n >= 0, "bad row count");
+    if (n == 0) return FSX_OK;
+    FSX_REQUIRE(X && packed && b1 && b2 && w3 && b3 && p_out, "null pointer argument");
+    FSX_REQUIRE(((uintptr_t)X & 15) == 0 && ((uintptr_t)packed & 15) == 0 && ((uintptr_t)b1 & 15) == 0 &&
+                    ((uintptr_t)b2 & 15) == 0 && ((uintptr_t)w3 & 15) == 0,
+                "X, packed weights, b1, b2 and w3 must be 16-byte aligned");
+    hipLaunchKernelGGL(k_mlp_infer, dim3((n + MI_ROWS - 1) / MI_ROWS), dim3(64 * MI_WAVES), 0, (hipStream_t)stream,
+                       (const unsigned short*)X, n, (const unsigned short*)packed, b1, b2, w3, b3, p_out);
     FSX_LAUNCH_CHECK();
     return FSX_OK;
 }

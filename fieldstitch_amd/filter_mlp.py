@@ -132,12 +132,24 @@ class MatchFilterMLP:
                       self.step_count, self._st())
         return loss
 
-    def predict(self, X16, chunk=65536):
-        """Probabilities for any number of rows (chunked)."""
+    def predict(self, X16, chunk=None):
+        """Probabilities for any number of rows: one fused launch (fsx_mlp_infer) over the
+        weights packed for it, packed again whenever an Adam step has changed them.  chunk is
+        accepted for compatibility and ignored."""
         torch = self.torch
+        self._check(X16)
         out = torch.empty(X16.shape[0], dtype=torch.float32, device=self.dev)
-        for s in range(0, X16.shape[0], chunk):
-            out[s:s + chunk] = self.forward(X16[s:s + chunk])
+        if X16.shape[0] == 0:
+            return out
+        if getattr(self, "_packed_at", None) != self.step_count:
+            nb = C.c_size_t()
+            _lib.call("fsx_mlp_infer_pack_bytes", C.byref(nb))
+            self._packed = torch.empty(nb.value // 2, dtype=torch.int16, device=self.dev)
+            _lib.call("fsx_mlp_infer_pack", self._P(self.W1_16), self._P(self.W2_16), self._P(self._packed), self._st())
+            self._packed_at = self.step_count
+        P = self._P
+        _lib.call("fsx_mlp_infer", P(X16), X16.shape[0], P(self._packed), P(self.p["b1"]), P(self.p["b2"]),
+                  P(self.p["w3"]), P(self.p["b3"]), P(out), self._st())
         return out
 
     def _check(self, X16):

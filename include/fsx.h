@@ -303,6 +303,16 @@ int fsx_mlp_adam(float *w, float *m, float *v, const float *g, void *w16, int n,
 /* float32 -> bf16 (round to nearest even). */
 int fsx_mlp_to_bf16(const float *in, void *out, int n, void *stream);
 
+/* Fused inference of the match filter (no counterpart in the reference): p_out[r] =
+ * sigmoid(relu(relu(X[r] W1 + b1) W2 + b2) . w3 + b3) for the n rows of X (bf16 [n][288],
+ * 16-byte aligned), hidden activations rounded to bf16 as in fsx_mlp_gemm + fsx_mlp_head.  The
+ * weights are first packed by fsx_mlp_infer_pack (W1 bf16 [288][256], W2 bf16 [256][256]) into
+ * fsx_mlp_infer_pack_bytes bytes of 16-byte aligned device memory; pack again after they change. */
+int fsx_mlp_infer_pack_bytes(size_t *bytes);
+int fsx_mlp_infer_pack(const void *W1, const void *W2, void *packed, void *stream);
+int fsx_mlp_infer(const void *X, int n, const void *packed, const float *b1, const float *b2, const float *w3,
+                  const float *b3, float *p_out, void *stream);
+
 /* Deterministic homography RANSAC (the verification step of match_pair, matching.py:226,
  * with reproducible sampling): pair p's matches are rows off_dev[p] .. off_dev[p+1] (int64
  * device prefix) of pa_dev / pb_dev (float64 (x, y) in image a / b); n_hyp hypotheses per pair

@@ -145,3 +145,37 @@ def test_training_learns_a_rule(gpu):
     p = net.predict(Xd[28672:]).cpu().numpy()
     acc = np.mean((p > 0.5) == (y[28672:] > 0.5))
     assert losses[-1] < 0.5 * losses[0] and acc > 0.9, (losses[0], losses[-1], acc)
+
+
+@pytest.mark.parametrize("n", [1, 255, 1000, 70000])
+def test_fused_inference(gpu, n):
+    """predict's single fused launch against the training forward (GEMM epilogues + head) and
+    the float64 oracle with the same bf16 roundings, on ragged row counts; the packed weights
+    follow an Adam step."""
+    import torch
+    from fieldstitch_amd.filter_mlp import MatchFilterMLP, N_IN
+    rng = np.random.default_rng(50 + n)
+    net = MatchFilterMLP(seed=7, lr=1e-2)
+    Xb = np.zeros((512, 288), np.float32)
+    Xb[:, :N_IN] = rng.normal(0, 1, (512, N_IN))
+    Xbd = torch.from_numpy(mo.bf16(Xb)).to(torch.bfloat16).cuda()
+    net.step(Xbd, torch.from_numpy((rng.random(512) < 0.4).astype(np.float32)).cuda())
+    for rep in range(2):
+        X = np.zeros((n, 288), np.float32)
+        X[:, :N_IN] = rng.normal(0, 1.5, (n, N_IN))
+        X = mo.bf16(X)
+        Xd = torch.from_numpy(X).to(torch.bfloat16).cuda()
+        print("fused inference n=%d rep=%d" % (n, rep), flush=True)
+        got = net.predict(Xd).cpu().numpy().astype(np.float64)
+        fwd = net.forward(Xd).cpu().numpy().astype(np.float64)
+        ref, _ = mo.forward(X.astype(np.float64), _sync_params(net), rounding=True)
+        d_fwd = np.abs(got - fwd)
+        d_ref = np.abs(got - ref)
+        print("max |fused - forward| %.3g, exact %.4f; max |fused - oracle| %.3g" %
+              (d_fwd.max(), np.mean(d_fwd == 0), d_ref.max()), flush=True)
+        # a hidden unit may round to the other bf16 neighbour when the two float32 sums differ
+        # in their last bits; that moves p by far less than 1e-2
+        assert d_fwd.max() <= 1e-2 and d_ref.max() <= 1e-2
+        assert np.mean(d_fwd <= 1e-5) > 0.95 and np.mean(d_ref <= 1e-4) > 0.95
+        net.step(Xbd, torch.from_numpy((rng.random(512) < 0.4).astype(np.float32)).cuda())   # weights move
+    assert net.predict(Xd[:0]).shape == (0,)
