@@ -28,6 +28,10 @@ HIPCC_FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 
+# Per-source additions.  fmatch.hip: MFMA accumulators in ordinary VGPRs rather than AGPRs, so
+# the screen's per-tile gate reads them without v_accvgpr_read copies.
+EXTRA_FLAGS = {"fmatch.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
+
 
 def hipcc() -> str:
     for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc"):
@@ -49,7 +53,8 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
     os.makedirs(OBJ_DIR, exist_ok=True)
     srcs = list(SOURCES)
     objs = [os.path.join(OBJ_DIR, os.path.splitext(s)[0] + ".o") for s in srcs]
-    cmds = [[hipcc()] + HIPCC_FLAGS + ["-I", CSRC, "-c", "-o", o, os.path.join(CSRC, s)] for s, o in zip(srcs, objs)]
+    cmds = [[hipcc()] + HIPCC_FLAGS + EXTRA_FLAGS.get(s, []) + ["-I", CSRC, "-c", "-o", o, os.path.join(CSRC, s)]
+            for s, o in zip(srcs, objs)]
     jobs = jobs or max(1, min(len(cmds), 16, os.cpu_count() or 1))
     with ThreadPoolExecutor(jobs) as ex:
         rcs = list(ex.map(lambda c: _run(c, verbose), cmds))

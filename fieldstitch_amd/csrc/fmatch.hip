@@ -248,21 +248,31 @@ typedef float f32x16_t __attribute__((ext_vector_type(16)));
 typedef int i32x8_t __attribute__((ext_vector_type(8)));
 
 
-// Branch-free insertion into an ascending list: the smaller of (d, td[j]) stays at j, the
-// larger moves on.  Equal screened distances keep their arrival order; the certificate only
-// needs the K-th value as a bound, so the order among ties never changes a result.
+// Branch-free insertion into an ascending list: the new list's entry j is the median of
+// (td[j-1], d, td[j]) (td[0]: the smaller of d and td[0]), so every entry comes from the old
+// list without a carried value; written from the top down in place.  d goes after entries
+// equal to it (arrival order among ties); the certificate only needs the K-th value as a
+// bound, so the order among ties never changes a result.  d = +inf changes nothing.
 template <int K>
 __device__ __forceinline__ void topk_insert(float d, int t, float* td, int* ti) {
 #pragma unroll
-    for (int j = 0; j < K; j++) {
-        const bool c = d < td[j];
-        const float od = td[j];
-        const int ot = ti[j];
-        td[j] = c ? d : od;
-        ti[j] = c ? t : ot;
-        d = c ? od : d;
-        t = c ? ot : t;
+    for (int j = K - 1; j >= 1; j--) {
+        const bool below_prev = d < td[j - 1];
+        const bool below = d < td[j];
+        ti[j] = below_prev ? ti[j - 1] : (below ? t : ti[j]);
+        td[j] = __builtin_amdgcn_fmed3f(td[j - 1], d, td[j]);
     }
+    const bool below0 = d < td[0];
+    ti[0] = below0 ? t : ti[0];
+    td[0] = below0 ? d : td[0];
+}
+
+// Score gate of a query with squared norm qn2 whose K-th screened distance is at most tau: a
+// target whose accumulator (a.b - |b|^2 / 2) does not exceed it has screened distance
+// qn2 - 2 acc >= tau (the margin covers the float32 roundings of both sides); -inf while
+// tau is +inf (the list is not full).
+__device__ __forceinline__ float gate_of(float qn2, float tau) {
+    return 0.5f * (qn2 - tau) - 0x1p-20f * (fabsf(qn2) + fabsf(tau)) - 1e-30f;
 }
 
 template <int PREC, int K, int FS_TILE, int QB>                   // QB: query blocks per wave
@@ -403,10 +413,14 @@ __global__ __launch_bounds__(256) void k_f_screen(const FBatch B) {
             }
 #pragma unroll
             for (int u = 0; u < QB; u++) {
-                unsigned m = 0;
+                // the gate: one compare of the lane's largest value against its threshold
+                float mx = acc[u][0];
 #pragma unroll
-                for (int e = 0; e < 16; e++) m |= acc[u][e] > hthr[u] ? (1u << e) : 0u;
-                if (__any(m != 0)) {
+                for (int e = 1; e < 16; e++) mx = fmaxf(mx, acc[u][e]);
+                if (__any(mx > hthr[u])) {
+                    unsigned m = 0;
+#pragma unroll
+                    for (int e = 0; e < 16; e++) m |= acc[u][e] > hthr[u] ? (1u << e) : 0u;
                     // the lane's 16 values to its LDS slot (float4 chunks swizzled by lane & 3
                     // against bank conflicts), then one iteration per passing element
                     float* my = s_acc + ((wv * 64 + lane) << 4);
@@ -419,7 +433,8 @@ __global__ __launch_bounds__(256) void k_f_screen(const FBatch B) {
                     }
                     if (m) {
                         // software-pipelined: the next set element's value is read while the
-                        // current one is inserted
+                        // current one is inserted; an element the moved threshold now rejects
+                        // is inserted as +inf, which changes nothing (no branch in the loop)
                         int e = __builtin_ctz(m);
                         m &= m - 1;
                         float v = my[4 * ((e >> 2) ^ sw) + (e & 3)];
@@ -428,12 +443,11 @@ __global__ __launch_bounds__(256) void k_f_screen(const FBatch B) {
                             const int en = more ? __builtin_ctz(m) : e;
                             const float vn = my[4 * ((en >> 2) ^ sw) + (en & 3)];
                             m = more ? (m & (m - 1)) : 0u;
-                            if (v > hthr[u]) {             // the threshold may have moved
-                                const int r = h * 32 + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3);
-                                topk_insert<K>(qn2[u] - 2.0f * v, base + r, td[u], ti[u]);
-                                const float tau = td[u][K - 1];
-                                hthr[u] = 0.5f * (qn2[u] - tau) - 0x1p-20f * (fabsf(qn2[u]) + fabsf(tau)) - 1e-30f;
-                            }
+                            const bool take = v > hthr[u];
+                            const int r = h * 32 + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3);
+                            topk_insert<K>(take ? qn2[u] - 2.0f * v : INFINITY, base + r, td[u], ti[u]);
+                            const float tau = td[u][K - 1];
+                            hthr[u] = take ? fmaxf(hthr[u], gate_of(qn2[u], tau)) : hthr[u];
                             if (!more) break;
                             e = en;
                             v = vn;
