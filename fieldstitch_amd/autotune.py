@@ -1,8 +1,9 @@
 """Autotuner of libfsx's launch shapes (no counterpart in the reference).
 
-Tunables (fsx_tune_set keys): orientation blocks per image, refinement blocks per image, and the
-float matcher's targets per LDS tile and blocks of 32 queries per wave.  Every choice gives identical results (they only change
-how work is split over workgroups), so the tuner times the candidates on the caller's own
+Tunables (fsx_tune_set keys): orientation blocks per image, refinement blocks per image, the
+float matcher's targets per LDS tile and blocks of 32 queries per wave, and the pyramid blur's
+row strip (output rows per tile).  Every choice gives identical results (they only change how
+work is split over workgroups), so the tuner times the candidates on the caller's own
 workload with HIP events and keeps the fastest.  Choices are cached per (device, key,
 workload signature) in memory, and in the JSON file named by FSX_TUNE_CACHE if set.
 """
@@ -16,9 +17,9 @@ import numpy as np
 
 from . import _lib
 
-ORIENT_BLOCKS, REFINE_BLOCKS, FSCREEN_TILE, FSCREEN_QB = 0, 1, 2, 3
+ORIENT_BLOCKS, REFINE_BLOCKS, FSCREEN_TILE, FSCREEN_QB, BLUR_ROWS = 0, 1, 2, 3, 4
 CANDIDATES = {ORIENT_BLOCKS: (1024, 2048, 4096, 8192), REFINE_BLOCKS: (64, 128, 256, 512),
-              FSCREEN_TILE: (64, 128), FSCREEN_QB: (1, 2)}
+              FSCREEN_TILE: (64, 128), FSCREEN_QB: (1, 2), BLUR_ROWS: (8, 16, 32)}
 _cache = {}
 
 
@@ -84,12 +85,15 @@ def tune(key, run, stage, signature, reps=3):
 
 
 def tune_features(fb, imgs, reps=3):
-    """Orientation and refinement grids for a FeatureBatch on its images."""
+    """Pyramid row strip, orientation and refinement grids for a FeatureBatch on its images
+    (capture its graph after tuning: a captured graph keeps its launch shapes)."""
     sig = ("features", tuple(fb.shapes), fb.max_kp)
     run = lambda ev: fb.run(imgs, events=ev)     # noqa: E731
-    out = {"orientation_blocks": tune(ORIENT_BLOCKS, run, (3, 4, 8), sig, reps),
+    out = {"blur_rows": tune(BLUR_ROWS, run, (0, 1, 8), sig, reps),
+           "orientation_blocks": tune(ORIENT_BLOCKS, run, (3, 4, 8), sig, reps),
            "refine_blocks": tune(REFINE_BLOCKS, run, (2, 3, 8), sig, reps)}
-    out["chosen"] = {"orientation_blocks": get(ORIENT_BLOCKS), "refine_blocks": get(REFINE_BLOCKS)}
+    out["chosen"] = {"blur_rows": get(BLUR_ROWS), "orientation_blocks": get(ORIENT_BLOCKS),
+                     "refine_blocks": get(REFINE_BLOCKS)}
     return out
 
 

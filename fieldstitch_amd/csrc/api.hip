@@ -8,7 +8,7 @@ namespace fsx {
 static thread_local char g_err[1024];
 
 // tunable launch shapes: orientation blocks / image, refinement blocks / image, float-screen tile
-static int g_tune[TUNE_N_KEYS] = {4096, 256, 64, 1};
+static int g_tune[TUNE_N_KEYS] = {4096, 256, 64, 1, 16};
 int tune_get(int key) { return g_tune[key]; }
 
 void set_error(const char* fmt, ...) {
@@ -80,6 +80,8 @@ int fsx_tune_set(int key, int value) {
         FSX_REQUIRE(value >= 16 && value <= 65536, "blocks per image must be in [16, 65536] (got %d)", value);
     if (key == TUNE_FSCREEN_TILE) FSX_REQUIRE(value == 64 || value == 128, "tile must be 64 or 128 (got %d)", value);
     if (key == TUNE_FSCREEN_QB) FSX_REQUIRE(value == 1 || value == 2, "query blocks must be 1 or 2 (got %d)", value);
+    if (key == TUNE_BLUR_ROWS)
+        FSX_REQUIRE(value == 8 || value == 16 || value == 32, "blur rows must be 8, 16 or 32 (got %d)", value);
     g_tune[key] = value;
     return FSX_OK;
 }

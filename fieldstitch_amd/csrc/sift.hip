@@ -150,17 +150,17 @@ __device__ __forceinline__ float up2_value(const uint8_t* __restrict__ img, int 
 
 // Tile: 256 input columns, TWo = 224 output columns (the largest multiple of 32 <= 256 - 2*R2:
 // tiles start on 128-byte boundaries of their row, so where rows are 128-byte aligned, as in
-// octave 0 of most photo widths, each row segment is written as whole lines) x 16 output rows,
-// 256 threads.
-// Vertical pass: a thread owns one input column and the 16 output rows; it streams the
-// 16 + 2*R2 input values of its column once from global memory (coalesced across the wave),
+// octave 0 of most photo widths, each row segment is written as whole lines) x TH output rows
+// (the row strip: 8, 16 or 32, the tuning key TUNE_BLUR_ROWS), 256 threads.
+// Vertical pass: a thread owns one input column and the TH output rows; it streams the
+// TH + 2*R2 input values of its column once from global memory (coalesced across the wave),
 // converts each to float64 once and accumulates from registers.  The float32-rounded result
 // goes to an LDS mid-tile (row pitch 257 floats).  Horizontal pass: a thread owns 16
 // consecutive outputs of one row, reading 16 + 2*R2 mid values once.  Epilogue: lanes along
-// x, coalesced stores of the gauss layer (the next launch's input).
+// x, coalesced stores of the gauss layer (the next launch's input).  A taller strip reads
+// fewer halo rows per output (TH + 2*R2 per TH) but needs more registers and LDS per block.
 constexpr int BL_SW = 256;
 constexpr int BL_NT = 256;
-constexpr int BL_TH = 16;
 constexpr int BL_CB = 16;
 
 template <int R2>
@@ -209,7 +209,7 @@ __device__ __forceinline__ float blur_point(const double* w, const HalfTaps& tp)
     return (float)ex;
 }
 
-template <int R2, bool BASE>
+template <int R2, bool BASE, int BL_TH>
 __global__ __launch_bounds__(BL_NT) void k_blur(const SiftJob* __restrict__ jobs, int o, int layer, HalfTaps tp) {
     constexpr int TWo = blur_tw<R2>();
     constexpr int NW = BL_TH + 2 * R2;        // vertical window
@@ -221,7 +221,7 @@ __global__ __launch_bounds__(BL_NT) void k_blur(const SiftJob* __restrict__ jobs
     static_assert(BL_SW == BL_NT && BL_TH % NWV == 0 && BL_TH % 2 == 0, "one input column per thread, even rows");
     __shared__ float s_mid[BL_TH][BL_SW + 1];
     __shared__ float s_out[BL_TH][TWo + 1];   // odd pitch: the horizontal pass writes conflict-free;
-                                              // 30.8 KB in all: 5 blocks per CU where VGPRs allow
+                                              // 30.8 KB in all at 16 rows: 5 blocks per CU where VGPRs allow
     int tx, ty, tz;
     xcd_tile(tx, ty, tz);
     const SiftJob& J = jobs[tz];
@@ -243,7 +243,7 @@ __global__ __launch_bounds__(BL_NT) void k_blur(const SiftJob* __restrict__ jobs
         const int gx = reflect101(x0 - R2 + c, C);
         const int ybase = y0 - R2;
         float f[NW];
-        // base layer: ybase = 16*ty - R2 has the parity of R2, so in the interior the source
+        // base layer: ybase = TH*ty - R2 has the parity of R2, so in the interior the source
         // rows of each upsampled row are fixed offsets from rb = (ybase >> 1) - 1; the thread
         // loads its two source columns over those NI rows once (the same float32 operations
         // as up2_value, no clamping needed inside [rb, rb + NI))
@@ -1611,23 +1611,27 @@ static int fill_job(SiftJob* J, const SiftPlan& p, const uint8_t* img, int ld, d
     return FSX_OK;
 }
 
-template <int R2>
+// one blur launch with the tuned row strip (TUNE_BLUR_ROWS: 8, 16 or 32 output rows per tile)
+template <int R2, bool BASE>
 static void launch_blur(const SiftJob* jobs, int n_img, int o, int layer, int R, int C, const HalfTaps& h,
                         hipStream_t st) {
     constexpr int TWo = blur_tw<R2>();
-    dim3 grid((C + TWo - 1) / TWo, (R + BL_TH - 1) / BL_TH, n_img);
-    hipLaunchKernelGGL((k_blur<R2, false>), grid, dim3(BL_NT), 0, st, jobs, o, layer, h);
+    const int th = tune_get(TUNE_BLUR_ROWS);
+    dim3 grid((C + TWo - 1) / TWo, (R + th - 1) / th, n_img);
+    if (th == 8) hipLaunchKernelGGL((k_blur<R2, BASE, 8>), grid, dim3(BL_NT), 0, st, jobs, o, layer, h);
+    else if (th == 32) hipLaunchKernelGGL((k_blur<R2, BASE, 32>), grid, dim3(BL_NT), 0, st, jobs, o, layer, h);
+    else hipLaunchKernelGGL((k_blur<R2, BASE, 16>), grid, dim3(BL_NT), 0, st, jobs, o, layer, h);
 }
 
 static int launch_layer(const SiftJob* jobs, int n_img, int o, int layer, int R, int C, hipStream_t st) {
     const TapSet& T = taps();
     const HalfTaps& h = T.h[layer];
     switch (T.r2[layer]) {
-        case 5: launch_blur<5>(jobs, n_img, o, layer, R, C, h, st); break;
-        case 6: launch_blur<6>(jobs, n_img, o, layer, R, C, h, st); break;
-        case 8: launch_blur<8>(jobs, n_img, o, layer, R, C, h, st); break;
-        case 10: launch_blur<10>(jobs, n_img, o, layer, R, C, h, st); break;
-        case 13: launch_blur<13>(jobs, n_img, o, layer, R, C, h, st); break;
+        case 5: launch_blur<5, false>(jobs, n_img, o, layer, R, C, h, st); break;
+        case 6: launch_blur<6, false>(jobs, n_img, o, layer, R, C, h, st); break;
+        case 8: launch_blur<8, false>(jobs, n_img, o, layer, R, C, h, st); break;
+        case 10: launch_blur<10, false>(jobs, n_img, o, layer, R, C, h, st); break;
+        case 13: launch_blur<13, false>(jobs, n_img, o, layer, R, C, h, st); break;
         default:
             set_error("unexpected blur half-width %d", T.r2[layer]);
             return FSX_EINVAL;
@@ -1658,9 +1662,7 @@ static int run_pipeline(const SiftJob* all, int n_all, const Group* groups, int 
     for (int gi = 0; gi < n_groups; gi++) {
         const Group& g = groups[gi];
         const SiftPlan& p = g.plan;
-        constexpr int TWo = blur_tw<5>();
-        dim3 grid((p.C[0] + TWo - 1) / TWo, (p.R[0] + BL_TH - 1) / BL_TH, g.n);
-        hipLaunchKernelGGL((k_blur<5, true>), grid, dim3(BL_NT), 0, st, g.jobs, 0, 0, T.h[0]);
+        launch_blur<5, true>(g.jobs, g.n, 0, 0, p.R[0], p.C[0], T.h[0], st);
         FSX_LAUNCH_CHECK();
         for (int o = 0; o < p.n_oct; o++)
             for (int layer = 1; layer <= N_LAYERS + 2; layer++) {

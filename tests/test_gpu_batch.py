@@ -67,18 +67,20 @@ def test_tunables_do_not_change_results(gpu):
     hosts = _images(5, SHAPES)
     dev = [torch.from_numpy(h).cuda() for h in hosts]
     fb = FeatureBatch(SHAPES, nfeat=600, contrast=0.04, edge=10.0)
-    old = (autotune.get(autotune.ORIENT_BLOCKS), autotune.get(autotune.REFINE_BLOCKS))
+    keys = (autotune.ORIENT_BLOCKS, autotune.REFINE_BLOCKS, autotune.BLUR_ROWS)
+    old = [autotune.get(k) for k in keys]
     snaps = []
     try:
-        for ob, rb in ((1024, 64), (8192, 512), (4096, 256)):
-            autotune.set(autotune.ORIENT_BLOCKS, ob)
-            autotune.set(autotune.REFINE_BLOCKS, rb)
+        for vals in ((1024, 64, 8), (8192, 512, 32), (4096, 256, 16)):
+            for k, v in zip(keys, vals):
+                autotune.set(k, v)
+            print("tunables", vals, flush=True)
             fb.run(dev)
             torch.cuda.synchronize()
             snaps.append(_snapshot(fb))
     finally:
-        autotune.set(autotune.ORIENT_BLOCKS, old[0])
-        autotune.set(autotune.REFINE_BLOCKS, old[1])
+        for k, v in zip(keys, old):
+            autotune.set(k, v)
     assert _same(snaps[0], snaps[1]) and _same(snaps[1], snaps[2])
 
 
@@ -93,6 +95,7 @@ def test_autotune_features_and_float_matcher(gpu):
     out = autotune.tune_features(fb, dev, reps=2)
     assert out["chosen"]["orientation_blocks"] in autotune.CANDIDATES[autotune.ORIENT_BLOCKS]
     assert out["chosen"]["refine_blocks"] in autotune.CANDIDATES[autotune.REFINE_BLOCKS]
+    assert out["chosen"]["blur_rows"] in autotune.CANDIDATES[autotune.BLUR_ROWS]
     rng = np.random.default_rng(2)
     da, db, _ = synth.descriptor_pair(rng, 900, 1100)
     fm = FloatMatcher([da, db], [(0, 1)], ratio=0.8, precision="bf16")

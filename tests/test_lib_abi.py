@@ -91,3 +91,24 @@ def test_noalloc_matcher_rejects_what_exceeds_its_limits():
         a = list(_noalloc_args([100, 100], [(0, 1)]))
         a[15] = None
         _lib.call("fsx_match_batch_noalloc", *a)
+
+
+def test_tune_keys_validated():
+    """Launch-shape tunables: accepted values round-trip, others are refused with a message
+    (host code only)."""
+    import ctypes as C
+    from fieldstitch_amd import _lib, autotune
+    for key, vals in autotune.CANDIDATES.items():
+        old = autotune.get(key)
+        try:
+            for v in vals:
+                autotune.set(key, v)
+                assert autotune.get(key) == v
+        finally:
+            autotune.set(key, old)
+    assert autotune.get(autotune.BLUR_ROWS) == 16
+    L = _lib.load()
+    for key, bad in ((autotune.BLUR_ROWS, 12), (autotune.FSCREEN_TILE, 96), (autotune.FSCREEN_QB, 3), (5, 1)):
+        assert L.fsx_tune_set(key, bad) != 0
+    v = C.c_int()
+    assert L.fsx_tune_get(5, C.byref(v)) != 0
