@@ -148,9 +148,10 @@ __device__ __forceinline__ float up2_value(const uint8_t* __restrict__ img, int 
     return 0.75f * t_i + 0.25f * t_n;
 }
 
-// Tile: 256 input columns, TWo = 224 output columns (the largest multiple of 32 <= 256 - 2*R2:
-// tiles start on 128-byte boundaries of their row, so where rows are 128-byte aligned, as in
-// octave 0 of most photo widths, each row segment is written as whole lines) x TH output rows
+// Tile: 256 input columns, TWo output columns (the largest multiple of 16 <= 256 - 2*R2: 240
+// for R2 <= 8, 224 above; tiles start on 64-byte boundaries of their row, so where rows are
+// 64-byte aligned, as in octave 0 of most photo widths, each row segment is written as whole
+// 64-byte sectors, and the vertical pass computes at most 16 + 2*R2 columns it does not store) x TH output rows
 // (the row strip: 8, 16 or 32, the tuning key TUNE_BLUR_ROWS), 256 threads.
 // Vertical pass: a thread owns one input column and the TH output rows; it streams the
 // TH + 2*R2 input values of its column once from global memory (coalesced across the wave),
@@ -164,7 +165,7 @@ constexpr int BL_NT = 256;
 constexpr int BL_CB = 16;
 
 template <int R2>
-constexpr int blur_tw() { return (BL_SW - 2 * R2) & ~31; }
+constexpr int blur_tw() { return (BL_SW - 2 * R2) & ~15; }
 
 // Workgroups are dispatched to the 8 XCDs round-robin by linear id; remap so that XCD k gets a
 // contiguous range of row-major tiles (a band of tile rows), keeping vertically adjacent tiles,
@@ -221,7 +222,7 @@ __global__ __launch_bounds__(BL_NT) void k_blur(const SiftJob* __restrict__ jobs
     static_assert(BL_SW == BL_NT && BL_TH % NWV == 0 && BL_TH % 2 == 0, "one input column per thread, even rows");
     __shared__ float s_mid[BL_TH][BL_SW + 1];
     __shared__ float s_out[BL_TH][TWo + 1];   // odd pitch: the horizontal pass writes conflict-free;
-                                              // 30.8 KB in all at 16 rows: 5 blocks per CU where VGPRs allow
+                                              // 31.9 KB in all at 16 rows: 5 blocks per CU where VGPRs allow
     int tx, ty, tz;
     xcd_tile(tx, ty, tz);
     const SiftJob& J = jobs[tz];
