@@ -290,7 +290,7 @@ def bench_chain(fl, desc_dev, xy_dev, counts, shapes, pairs, reps=2):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(reps):
-        res = match_flow.run_pairs(pairs, images, cfg)
+        res = match_flow.run_pairs(pairs, images, cfg, stats=stats)     # stage_ms: the last rep's
     torch.cuda.synchronize()
     t = (time.perf_counter() - t0) / reps
     matched = [r for r in res if r[2] is not None]

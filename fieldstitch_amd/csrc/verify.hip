@@ -526,6 +526,89 @@ __global__ __launch_bounds__(RF_LANES) void k_refit_h(const long long* __restric
     for (int k = 0; k < 9; k++) H_out[9 * p + k] = Hr[k];
 }
 
+// out[k] = xy[idx[k]] (float64 (x, y) rows) for k < n; an index outside [0, n_xy) gives NaN.
+__global__ __launch_bounds__(256) void k_gather_xy(long long n, const int* __restrict__ idx,
+                                                   const double* __restrict__ xy, long long n_xy,
+                                                   double* __restrict__ out) {
+    const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const int i = idx[k];
+    const bool ok = i >= 0 && (long long)i < n_xy;
+    const double2 v = ok ? ((const double2*)xy)[i] : make_double2(__builtin_nan(""), __builtin_nan(""));
+    ((double2*)out)[k] = v;
+}
+
+// Least-squares similarity b ~ s R(theta) a + t of each pair's inlier rows (one 256-thread block
+// per pair): centroids, then over the centred rows den = sum |a|^2, c = sum a.b / den,
+// s = sum a x b / den, t = mb - [[c, -s], [s, c]] ma; out[4p..4p+3] = (atan2(s, c), hypot(c, s),
+// tx, ty), or NaN when fewer than 2 inliers or den <= 1e-12 n (the points coincide).  Sums:
+// per thread over rows tid, tid + 256, ... in order, then a fixed tree over the threads.
+__device__ __forceinline__ void block_sum5(double* v, double (*red)[257]) {
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int e = 0; e < 5; e++) red[e][tid] = v[e];
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+        if (tid < w)
+#pragma unroll
+            for (int e = 0; e < 5; e++) red[e][tid] = red[e][tid] + red[e][tid + w];
+        __syncthreads();
+    }
+#pragma unroll
+    for (int e = 0; e < 5; e++) v[e] = red[e][0];
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_similarity_fit(const long long* __restrict__ off,
+                                                        const double* __restrict__ pa, const double* __restrict__ pb,
+                                                        const uint8_t* __restrict__ inlier, double* __restrict__ out) {
+    __shared__ double red[5][257];
+    const int p = blockIdx.x, tid = threadIdx.x;
+    const long long o = off[p];
+    const int n = (int)(off[p + 1] - o);
+    const double* A = pa + 2 * o;
+    const double* B = pb + 2 * o;
+    const uint8_t* f = inlier + o;
+    double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int i = tid; i < n; i += 256)
+        if (f[i]) {
+            v[0] = v[0] + A[2 * i];
+            v[1] = v[1] + A[2 * i + 1];
+            v[2] = v[2] + B[2 * i];
+            v[3] = v[3] + B[2 * i + 1];
+            v[4] = v[4] + 1.0;
+        }
+    block_sum5(v, red);
+    const double cnt = v[4];
+    const double nan = __builtin_nan("");
+    if (!(cnt >= 2.0)) {
+        if (tid < 4) out[4 * p + tid] = nan;
+        return;
+    }
+    const double max_ = v[0] / cnt, may = v[1] / cnt, mbx = v[2] / cnt, mby = v[3] / cnt;
+    double w[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int i = tid; i < n; i += 256)
+        if (f[i]) {
+            const double ax = A[2 * i] - max_, ay = A[2 * i + 1] - may;
+            const double bx = B[2 * i] - mbx, by = B[2 * i + 1] - mby;
+            w[0] = w[0] + (ax * ax + ay * ay);
+            w[1] = w[1] + (ax * bx + ay * by);
+            w[2] = w[2] + (ax * by - ay * bx);
+        }
+    block_sum5(w, red);
+    if (tid != 0) return;
+    const double den = w[0];
+    if (!(den > 1e-12 * cnt)) {
+        for (int e = 0; e < 4; e++) out[4 * p + e] = nan;
+        return;
+    }
+    const double c = w[1] / den, sn = w[2] / den;
+    out[4 * p] = atan2(sn, c);
+    out[4 * p + 1] = hypot(c, sn);
+    out[4 * p + 2] = mbx - (c * max_ - sn * may);
+    out[4 * p + 3] = mby - (sn * max_ + c * may);
+}
+
 }  // namespace fsx
 
 using namespace fsx;
@@ -617,6 +700,35 @@ int fsx_ransac_batch(int model, int n_pair, const long long* off_dev, const doub
         default:
             FSX_REQUIRE(false, "model must be 0 (homography), 1 (fundamental) or 2 (similarity)");
     }
+}
+
+// out_dev[k] (two float64) = xy_dev[idx_dev[k]] for k < n; an index outside [0, n_xy) gives
+// NaN.  Gathers the matched positions of many pairs from all photos' positions in one launch.
+int fsx_gather_xy(long long n, const int* idx_dev, const double* xy_dev, long long n_xy, double* out_dev,
+                  void* stream) {
+    FSX_REQUIRE(n >= 0 && n_xy >= 0, "bad gather sizes");
+    if (n == 0) return FSX_OK;
+    FSX_REQUIRE(idx_dev && xy_dev && out_dev, "null pointer argument");
+    FSX_REQUIRE(((uintptr_t)xy_dev & 15) == 0 && ((uintptr_t)out_dev & 15) == 0, "xy / out must be 16-byte aligned");
+    hipLaunchKernelGGL(k_gather_xy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, n, idx_dev,
+                       xy_dev, n_xy, out_dev);
+    FSX_LAUNCH_CHECK();
+    return FSX_OK;
+}
+
+// Least-squares similarity (rotation, scale, translation) of each pair's inlier rows: pair p's
+// rows are off_dev[p] .. off_dev[p+1] of pa_dev / pb_dev (float64 (x, y)), inlier_dev one byte
+// per row; out_dev[4p .. 4p+3] = (angle rad, scale, tx, ty) of b ~ s R a + t, NaN when the pair
+// has fewer than 2 inliers or they coincide.
+int fsx_similarity_fit_batch(int n_pair, const long long* off_dev, const double* pa_dev, const double* pb_dev,
+                             const uint8_t* inlier_dev, double* out_dev, void* stream) {
+    FSX_REQUIRE(n_pair >= 0, "bad pair count");
+    if (n_pair == 0) return FSX_OK;
+    FSX_REQUIRE(off_dev && pa_dev && pb_dev && inlier_dev && out_dev, "null pointer argument");
+    hipLaunchKernelGGL(k_similarity_fit, dim3(n_pair), dim3(256), 0, (hipStream_t)stream, off_dev, pa_dev, pb_dev,
+                       inlier_dev, out_dev);
+    FSX_LAUNCH_CHECK();
+    return FSX_OK;
 }
 
 }  // extern "C"

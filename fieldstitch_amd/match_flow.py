@@ -32,6 +32,7 @@ from __future__ import annotations
 import ctypes as C
 import math
 import os
+import time
 import traceback
 
 import numpy as np
@@ -179,14 +180,53 @@ def _concat_rows(rows):
     return off, np.ascontiguousarray(data, np.float64).reshape(-1, 2)
 
 
-def _prior_keep(cfg, items):
-    """items: [(i, j, ua rows, ub rows)] -> per item a bool keep mask (matching.py:221-225):
-    one fsx_prior_keep_batch for all of them."""
+class Positions:
+    """All photos' undistorted positions in one device float64 array: photo i's rows start at
+    base[i], so a pair's matched rows are gathered by index (fsx_gather_xy) instead of being
+    indexed, packed and uploaded on the host."""
+
+    def __init__(self, arrays):
+        torch = _torch()
+        self.base = {}
+        parts = []
+        o = 0
+        for i, a in arrays.items():
+            t = a if isinstance(a, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(a, np.float64))
+            t = t.to("cuda", torch.float64).reshape(-1, 2)
+            self.base[i] = o
+            o += int(t.shape[0])
+            parts.append(t)
+        if o >= 2 ** 31:
+            raise ValueError("%d positions in all; at most 2^31 - 1" % o)
+        self.n = o
+        self.xy = torch.cat(parts).contiguous() if o else torch.zeros((1, 2), dtype=torch.float64, device="cuda")
+
+    def rows(self, i, q):
+        """Global row numbers of photo i's features q."""
+        return self.base[i] + np.asarray(q, np.int64)
+
+    def gather(self, idx_list):
+        """[(m_k,) row numbers] -> (int64 offsets, device float64 (sum m_k, 2) rows)."""
+        torch = _torch()
+        lens = np.array([len(x) for x in idx_list], np.int64)
+        off = np.r_[0, np.cumsum(lens)].astype(np.int64)
+        tot = int(off[-1])
+        out = torch.empty((max(tot, 1), 2), dtype=torch.float64, device="cuda")
+        if tot:
+            idx = torch.from_numpy(np.concatenate(idx_list).astype(np.int32)).to("cuda")
+            _lib.call("fsx_gather_xy", tot, C.c_void_p(idx.data_ptr()), C.c_void_p(self.xy.data_ptr()), self.n,
+                      C.c_void_p(out.data_ptr()), C.c_void_p(_lib.stream_handle()))
+        return off, out
+
+
+def _prior_keep(cfg, P, items):
+    """items: [(i, j, rows in i, rows in j)] (global row numbers of P) -> per item a bool keep
+    mask (matching.py:221-225): one gather and one fsx_prior_keep_batch for all of them."""
     if not items:
         return []
     torch = _torch()
-    off, ua = _concat_rows([it[2] for it in items])
-    _, ub = _concat_rows([it[3] for it in items])
+    off, d_ua = P.gather([it[2] for it in items])
+    _, d_ub = P.gather([it[3] for it in items])
     if off[-1] == 0:
         return [np.zeros(0, bool) for _ in items]
     cams = np.zeros((len(items), 43), np.float64)
@@ -197,33 +237,25 @@ def _prior_keep(cfg, items):
             tol[k] = cfg.prior_tol * cfg.side(j)
     dev = "cuda"
     d_off = torch.from_numpy(off).to(dev)
-    d_ua = torch.from_numpy(ua).to(dev)
-    d_ub = torch.from_numpy(ub).to(dev)
     d_cams = torch.from_numpy(cams).to(dev)
     d_tol = torch.from_numpy(tol).to(dev)
     keep = torch.empty(int(off[-1]), dtype=torch.uint8, device=dev)
-    P = C.c_void_p
-    _lib.call("fsx_prior_keep_batch", len(items), P(d_off.data_ptr()), P(d_ua.data_ptr()), P(d_ub.data_ptr()),
-              P(d_cams.data_ptr()), P(d_tol.data_ptr()), int(np.diff(off).max()), P(keep.data_ptr()),
-              P(_lib.stream_handle()))
+    Pv = C.c_void_p
+    _lib.call("fsx_prior_keep_batch", len(items), Pv(d_off.data_ptr()), Pv(d_ua.data_ptr()), Pv(d_ub.data_ptr()),
+              Pv(d_cams.data_ptr()), Pv(d_tol.data_ptr()), int(np.diff(off).max()), Pv(keep.data_ptr()),
+              Pv(_lib.stream_handle()))
     k = keep.cpu().numpy().astype(bool)
     return [k[off[n]:off[n + 1]] for n in range(len(items))]
 
 
-def ransac_batch(model, pa_list, pb_list, thr, seeds, n_hyp):
-    """RANSAC of `model` for every pair, each with its own threshold and seed (one launch).
-    Returns (inlier masks, models (n, 9), counts)."""
+def _ransac_dev(model, off, d_a, d_b, thr, seeds, n_hyp):
+    """fsx_ransac_batch on device rows (pair k's rows off[k] .. off[k+1] of d_a / d_b).
+    Returns (inlier masks, models (n, 9), counts, the device inlier bytes)."""
     torch = _torch()
-    n = len(pa_list)
-    if n == 0:
-        return [], np.zeros((0, 9)), np.zeros(0, np.int64)
-    off, A = _concat_rows(pa_list)
-    _, B = _concat_rows(pb_list)
+    n = len(off) - 1
     dev = "cuda"
     tot = max(int(off[-1]), 1)
     d_off = torch.from_numpy(off).to(dev)
-    d_a = torch.from_numpy(A if off[-1] else np.zeros((1, 2))).to(dev)
-    d_b = torch.from_numpy(B if off[-1] else np.zeros((1, 2))).to(dev)
     d_thr = torch.from_numpy(np.asarray(thr, np.float64).reshape(n)).to(dev)
     d_seed = torch.from_numpy(np.asarray([s & 0x7FFFFFFFFFFFFFFF if s < (1 << 63) else s - (1 << 64) for s in seeds],
                                          np.int64)).to(dev)
@@ -235,13 +267,50 @@ def ransac_batch(model, pa_list, pb_list, thr, seeds, n_hyp):
               int(n_hyp), P(d_thr.data_ptr()), P(d_seed.data_ptr()), P(inl.data_ptr()), P(M.data_ptr()),
               P(cnt.data_ptr()), P(_lib.stream_handle()))
     flags = inl.cpu().numpy().astype(bool)
-    return [flags[off[k]:off[k + 1]] for k in range(n)], M.cpu().numpy(), cnt.cpu().numpy().astype(np.int64)
+    return ([flags[off[k]:off[k + 1]] for k in range(n)], M.cpu().numpy(), cnt.cpu().numpy().astype(np.int64),
+            (d_off, inl))
+
+
+def ransac_batch(model, pa_list, pb_list, thr, seeds, n_hyp):
+    """RANSAC of `model` for every pair, each with its own threshold and seed (one launch), on
+    host (n_k, 2) position arrays.  Returns (inlier masks, models (n, 9), counts)."""
+    torch = _torch()
+    if not pa_list:
+        return [], np.zeros((0, 9)), np.zeros(0, np.int64)
+    off, A = _concat_rows(pa_list)
+    _, B = _concat_rows(pb_list)
+    d_a = torch.from_numpy(A if off[-1] else np.zeros((1, 2))).to("cuda")
+    d_b = torch.from_numpy(B if off[-1] else np.zeros((1, 2))).to("cuda")
+    flags, M, cnt, _ = _ransac_dev(model, off, d_a, d_b, thr, seeds, n_hyp)
+    return flags, M, cnt
+
+
+def ransac_rows(model, P, ia_list, ib_list, thr, seeds, n_hyp, fit=False):
+    """ransac_batch on the rows ia_list[k] / ib_list[k] of the Positions P, gathered on the
+    device.  With fit=True also the least-squares similarity of each pair's inliers
+    (fsx_similarity_fit_batch): a (n, 4) array of (angle, scale, tx, ty), NaN rows without a
+    fit.  Returns (inlier masks, models, counts[, fits])."""
+    torch = _torch()
+    if not ia_list:
+        empty = ([], np.zeros((0, 9)), np.zeros(0, np.int64))
+        return empty + (np.zeros((0, 4)),) if fit else empty
+    off, d_a = P.gather(ia_list)
+    _, d_b = P.gather(ib_list)
+    flags, M, cnt, (d_off, inl) = _ransac_dev(model, off, d_a, d_b, thr, seeds, n_hyp)
+    if not fit:
+        return flags, M, cnt
+    n = len(ia_list)
+    out = torch.empty((n, 4), dtype=torch.float64, device="cuda")
+    Pv = C.c_void_p
+    _lib.call("fsx_similarity_fit_batch", n, Pv(d_off.data_ptr()), Pv(d_a.data_ptr()), Pv(d_b.data_ptr()),
+              Pv(inl.data_ptr()), Pv(out.data_ptr()), Pv(_lib.stream_handle()))
+    return flags, M, cnt, out.cpu().numpy()
 
 
 def similarity_fit(a, b):
     """Least-squares similarity b ~ s R(theta) a + t of matched points (closed form on the
     centred sets): (theta rad, s, tx, ty), or None for fewer than 2 points or points that
-    coincide."""
+    coincide.  The host form of fsx_similarity_fit_batch (the chain uses the device one)."""
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     if len(a) < 2:
@@ -270,7 +339,8 @@ def _error_tuple(i, j, e):
 def run_pairs(tasks, images, cfg, form="mfma", filt=None, stats=None):
     """The match_pair chain for every (i, j) of `tasks` over `images` (index -> FlowImage).
     Returns the tuples in task order.  filt = (net, tau) prunes the guided stage's candidates
-    with the learned filter.  stats (dict, optional) receives per-stage pair counts."""
+    with the learned filter.  stats (dict, optional) receives per-stage pair counts and host
+    wall times (stage_ms)."""
     tasks = [(int(i), int(j)) for i, j in tasks]
     try:
         return _chain(tasks, images, cfg, form, filt, stats)
@@ -289,8 +359,10 @@ def _chain(tasks, images, cfg, form, filt, stats):
             out[k] = (i, j, None, None, None, 0)
         else:
             live.append(k)
-    und = {i: images[i].und_host() for k in live for i in tasks[k]}
+    marks = [("start", time.perf_counter())]
+    P = Positions({i: images[i].und for i in dict.fromkeys(i for k in live for i in tasks[k])})
     result = {}            # task index -> (q, t, n_in)
+    marks.append(("positions", time.perf_counter()))
 
     # 1-2. coarse matches of the strongest features, prior filter
     n1 = cfg.n_coarse
@@ -298,37 +370,41 @@ def _chain(tasks, images, cfg, form, filt, stats):
     cm = _dense_matches(images, [tasks[k] for k in coarse], 0.85, prefix=n1, form=form)
     cm = dict(zip(coarse, cm))
     pri = [k for k in coarse if cfg.prior_on(*tasks[k]) and len(cm[k][0])]
-    keeps = _prior_keep(cfg, [(tasks[k][0], tasks[k][1], und[tasks[k][0]][cm[k][0]], und[tasks[k][1]][cm[k][1]])
-                              for k in pri])
+    keeps = _prior_keep(cfg, P, [(tasks[k][0], tasks[k][1], P.rows(tasks[k][0], cm[k][0]), P.rows(tasks[k][1], cm[k][1]))
+                                 for k in pri])
     for k, m in zip(pri, keeps):
         cm[k] = (cm[k][0][m], cm[k][1][m])
+    marks.append(("coarse_and_prior", time.perf_counter()))
 
     # 3. homography RANSAC on the coarse set
     hk = [k for k in coarse if len(cm[k][0]) >= 12]
-    hflags, hM, _ = ransac_batch("homography", [und[tasks[k][0]][cm[k][0]] for k in hk],
-                                 [und[tasks[k][1]][cm[k][1]] for k in hk],
-                                 [0.012 * cfg.side(tasks[k][0]) for k in hk],
-                                 [pair_seed(*tasks[k], 0x1) for k in hk], H_ITERS)
+    hflags, hM, _ = ransac_rows("homography", P, [P.rows(tasks[k][0], cm[k][0]) for k in hk],
+                                [P.rows(tasks[k][1], cm[k][1]) for k in hk],
+                                [0.012 * cfg.side(tasks[k][0]) for k in hk],
+                                [pair_seed(*tasks[k], 0x1) for k in hk], H_ITERS)
     gk = [k for k, f in zip(hk, hflags) if f.sum() >= 10]
     Hm = {k: hM[n].reshape(3, 3) for n, k in enumerate(hk)}
+    marks.append(("homography", time.perf_counter()))
 
     # 4. guided matching under the homography
     gm = _guided_matches(images, [tasks[k] for k in gk], [Hm[k] for k in gk],
                          [cfg.guided_radius * cfg.side(tasks[k][1]) for k in gk], cfg.guided_k, 0.85, form, filt)
     vk = [k for k, (q, _) in zip(gk, gm) if len(q) >= cfg.min_inliers]
     gm = dict(zip(gk, gm))
+    marks.append(("guided", time.perf_counter()))
 
     # 5. fundamental-matrix verification of the guided matches
-    for k, (q, t, n_in) in zip(vk, _verify(cfg, tasks, und, [(k, gm[k]) for k in vk])):
+    for k, (q, t, n_in) in zip(vk, _verify(cfg, tasks, P, [(k, gm[k]) for k in vk])):
         if q is not None and n_in >= max(cfg.min_inliers, 40):
             result[k] = (q, t, n_in)
+    marks.append(("verify", time.perf_counter()))
 
     # 6. fallback: exhaustive matching, prior filter, verification
     fb = [k for k in live if k not in result]
     fm = dict(zip(fb, _dense_matches(images, [tasks[k] for k in fb], cfg.ratio, form=form)))
     pri = [k for k in fb if cfg.prior_on(*tasks[k]) and len(fm[k][0])]
-    keeps = _prior_keep(cfg, [(tasks[k][0], tasks[k][1], und[tasks[k][0]][fm[k][0]], und[tasks[k][1]][fm[k][1]])
-                              for k in pri])
+    keeps = _prior_keep(cfg, P, [(tasks[k][0], tasks[k][1], P.rows(tasks[k][0], fm[k][0]), P.rows(tasks[k][1], fm[k][1]))
+                                 for k in pri])
     for k, m in zip(pri, keeps):
         fm[k] = (fm[k][0][m], fm[k][1][m])
     enough = []
@@ -337,35 +413,38 @@ def _chain(tasks, images, cfg, form, filt, stats):
             out[k] = (tasks[k][0], tasks[k][1], None, None, None, 0)
         else:
             enough.append(k)
-    for k, (q, t, n_in) in zip(enough, _verify(cfg, tasks, und, [(k, fm[k]) for k in enough])):
+    for k, (q, t, n_in) in zip(enough, _verify(cfg, tasks, P, [(k, fm[k]) for k in enough])):
         if q is None or n_in < cfg.min_inliers:
             out[k] = (tasks[k][0], tasks[k][1], None, None, None, n_in)
         else:
             result[k] = (q, t, n_in)
 
+    marks.append(("fallback", time.perf_counter()))
     # 7. similarity of every verified set
     rk = sorted(result)
-    sims = _similarities(cfg, tasks, und, [(k, result[k]) for k in rk])
+    sims = _similarities(cfg, tasks, P, [(k, result[k]) for k in rk])
     for k, sim in zip(rk, sims):
         q, t, n_in = result[k]
         out[k] = (tasks[k][0], tasks[k][1], np.asarray(q).astype(np.int32), np.asarray(t).astype(np.int32), sim, n_in)
+    marks.append(("similarity", time.perf_counter()))
     if stats is not None:
         stats.update({"pairs": n_task, "coarse": len(coarse), "homography": len(hk), "guided": len(gk),
-                      "guided_verified": len(vk), "fallback": len(fb), "matched": len(rk)})
+                      "guided_verified": len(vk), "fallback": len(fb), "matched": len(rk),
+                      "stage_ms": {b[0]: round(1e3 * (b[1] - a[1]), 3) for a, b in zip(marks, marks[1:])}})
     return out
 
 
-def _verify(cfg, tasks, und, items):
+def _verify(cfg, tasks, P, items):
     """matching.py:175-191 for items [(task index, (q, t))]: fundamental-matrix RANSAC at
     ransac_thr (fewer than 8 matches: no model) -> [(q, t of the inliers, count) or
     (None, None, 0)]."""
     res = [(None, None, 0)] * len(items)
     ok = [n for n, (_, (q, _)) in enumerate(items) if len(q) >= 8]
-    flags, _, cnt = ransac_batch("fundamental",
-                                 [und[tasks[items[n][0]][0]][items[n][1][0]] for n in ok],
-                                 [und[tasks[items[n][0]][1]][items[n][1][1]] for n in ok],
-                                 [cfg.ransac_thr] * len(ok), [pair_seed(*tasks[items[n][0]], 0xF) for n in ok],
-                                 cfg.f_iters)
+    flags, _, cnt = ransac_rows("fundamental", P,
+                                [P.rows(tasks[items[n][0]][0], items[n][1][0]) for n in ok],
+                                [P.rows(tasks[items[n][0]][1], items[n][1][1]) for n in ok],
+                                [cfg.ransac_thr] * len(ok), [pair_seed(*tasks[items[n][0]], 0xF) for n in ok],
+                                cfg.f_iters)
     for n, f, c in zip(ok, flags, cnt):
         if c > 0:
             q, t = items[n][1]
@@ -373,18 +452,19 @@ def _verify(cfg, tasks, und, items):
     return res
 
 
-def _similarities(cfg, tasks, und, items):
+def _similarities(cfg, tasks, P, items):
     """matching.py:194-202 for items [(task index, (q, t, n_in))]: similarity RANSAC at
-    0.02 max(W_i, H_i), then the least-squares fit on its inliers; None when it fails."""
+    0.02 max(W_i, H_i), then the least-squares fit on its inliers (both on the device); None
+    when it fails."""
     res = [None] * len(items)
     ok = [n for n, (_, (q, _, _)) in enumerate(items) if len(q) >= 2]
-    pa = [und[tasks[items[n][0]][0]][items[n][1][0]] for n in ok]
-    pb = [und[tasks[items[n][0]][1]][items[n][1][1]] for n in ok]
-    flags, _, cnt = ransac_batch("similarity", pa, pb, [0.02 * cfg.side(tasks[items[n][0]][0]) for n in ok],
-                                 [pair_seed(*tasks[items[n][0]], 0x5) for n in ok], S_ITERS)
-    for n, a, b, f, c in zip(ok, pa, pb, flags, cnt):
-        if c >= 2:
-            res[n] = similarity_fit(a[f], b[f])
+    _, _, cnt, fits = ransac_rows("similarity", P, [P.rows(tasks[items[n][0]][0], items[n][1][0]) for n in ok],
+                                  [P.rows(tasks[items[n][0]][1], items[n][1][1]) for n in ok],
+                                  [0.02 * cfg.side(tasks[items[n][0]][0]) for n in ok],
+                                  [pair_seed(*tasks[items[n][0]], 0x5) for n in ok], S_ITERS, fit=True)
+    for n, c, fit in zip(ok, cnt, fits):
+        if c >= 2 and np.isfinite(fit).all():
+            res[n] = (float(fit[0]), float(fit[1]), float(fit[2]), float(fit[3]))
     return res
 
 

@@ -588,8 +588,8 @@ def _verify(c, i, j, ui, uj, q, t):
     GPU -> (q, t of the inliers, count) or (None, None, 0)."""
     from . import match_flow
     cfg = match_flow.FlowConfig(c)
-    und = {i: np.asarray(ui, np.float64), j: np.asarray(uj, np.float64)}
-    return match_flow._verify(cfg, [(i, j)], und, [(0, (np.asarray(q), np.asarray(t)))])[0]
+    pos = match_flow.Positions({i: np.asarray(ui, np.float64), j: np.asarray(uj, np.float64)})
+    return match_flow._verify(cfg, [(i, j)], pos, [(0, (np.asarray(q), np.asarray(t)))])[0]
 
 
 def _similarity(c, i, j, ui, uj, q, t):
@@ -598,7 +598,7 @@ def _similarity(c, i, j, ui, uj, q, t):
     from . import match_flow
     try:
         cfg = match_flow.FlowConfig(c)
-        und = {i: np.asarray(ui, np.float64), j: np.asarray(uj, np.float64)}
-        return match_flow._similarities(cfg, [(i, j)], und, [(0, (np.asarray(q), np.asarray(t), 0))])[0]
+        pos = match_flow.Positions({i: np.asarray(ui, np.float64), j: np.asarray(uj, np.float64)})
+        return match_flow._similarities(cfg, [(i, j)], pos, [(0, (np.asarray(q), np.asarray(t), 0))])[0]
     except Exception:  # noqa: BLE001 - the reference returns None on any failure here
         return None

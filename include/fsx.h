@@ -383,6 +383,19 @@ int fsx_ransac_batch(int model, int n_pair, const long long *off_dev, const doub
                      int n_hyp, const double *thr_dev, const unsigned long long *seed_dev, uint8_t *inlier_dev,
                      double *M_dev, int *n_in_dev, void *stream);
 
+/* Gather of matched positions (the host-side fancy indexing ua[q], ub[t] of matching.py:180,
+ * 196, 226 for many pairs at once): out_dev[k] = xy_dev[idx_dev[k]] (two float64 each) for
+ * k < n; an index outside [0, n_xy) gives NaN.  xy_dev and out_dev 16-byte aligned. */
+int fsx_gather_xy(long long n, const int *idx_dev, const double *xy_dev, long long n_xy, double *out_dev,
+                  void *stream);
+
+/* The least-squares similarity of _similarity (matching.py:194-202, cv2's refit on the RANSAC
+ * inliers) for many pairs: pair p's rows off_dev[p] .. off_dev[p+1] of pa_dev / pb_dev, one
+ * inlier byte per row; out_dev[4p..4p+3] = (angle rad, scale, tx, ty) of b ~ s R a + t, NaN
+ * with fewer than 2 inliers or coincident points. */
+int fsx_similarity_fit_batch(int n_pair, const long long *off_dev, const double *pa_dev, const double *pb_dev,
+                             const uint8_t *inlier_dev, double *out_dev, void *stream);
+
 /* ---------------------------------------------------------------------------------------
  * Keypoint geometry around the matcher
  * ------------------------------------------------------------------------------------- */

@@ -66,3 +66,60 @@ def test_ransac_models_equal_oracle(gpu, model, thr):
         np.testing.assert_allclose(Ms[p], rm, rtol=1e-10, atol=1e-12)
     if model == "fundamental":
         assert cnt[0] > 0.5 * 700 and cnt[4] > 0.85 * 1500
+
+
+def test_gather_and_similarity_fit(gpu):
+    """fsx_gather_xy against numpy indexing (out-of-range indices give NaN rows), and
+    fsx_similarity_fit_batch against numpy's least-squares solve of the same similarity on
+    each pair's inliers (fewer than 2 inliers and coincident points give NaN)."""
+    import ctypes as C
+    import math
+    import torch
+    from fieldstitch_amd import _lib
+    rng = np.random.default_rng(21)
+    xy = rng.uniform(-500, 3000, (1000, 2))
+    idx = rng.integers(0, 1000, 5000).astype(np.int32)
+    idx[[3, 77, 4999]] = [-1, 1000, 2 ** 31 - 1]
+    d_xy = torch.from_numpy(xy).cuda()
+    d_idx = torch.from_numpy(idx).cuda()
+    out = torch.empty((5000, 2), dtype=torch.float64, device="cuda")
+    P = lambda t: C.c_void_p(t.data_ptr())    # noqa: E731
+    st = C.c_void_p(_lib.stream_handle())
+    _lib.call("fsx_gather_xy", 5000, P(d_idx), P(d_xy), 1000, P(out), st)
+    g = out.cpu().numpy()
+    bad = (idx < 0) | (idx >= 1000)
+    np.testing.assert_array_equal(g[~bad], xy[idx[~bad]])
+    assert np.isnan(g[bad]).all()
+
+    sets, flags = [], []
+    for k, n in enumerate((300, 2, 1, 0, 50, 40)):
+        a = rng.uniform(0, 2000, (n, 2))
+        ang, sc = rng.uniform(-0.5, 0.5), rng.uniform(0.8, 1.2)
+        R = sc * np.array([[math.cos(ang), -math.sin(ang)], [math.sin(ang), math.cos(ang)]])
+        b = a @ R.T + rng.uniform(-100, 100, 2) + rng.normal(0, 0.5, (n, 2))
+        f = rng.random(n) < 0.8
+        if k == 1:
+            f[:] = True
+        if k == 5:
+            a[:] = a[0]                     # coincident points
+        sets.append((a, b))
+        flags.append(f)
+    off = np.r_[0, np.cumsum([len(a) for a, _ in sets])].astype(np.int64)
+    A = np.concatenate([a for a, _ in sets]) if off[-1] else np.zeros((1, 2))
+    B = np.concatenate([b for _, b in sets])
+    Fl = np.concatenate(flags).astype(np.uint8)
+    d = [torch.from_numpy(np.ascontiguousarray(x)).cuda() for x in (off, A, B, Fl)]
+    fit = torch.empty((len(sets), 4), dtype=torch.float64, device="cuda")
+    _lib.call("fsx_similarity_fit_batch", len(sets), P(d[0]), P(d[1]), P(d[2]), P(d[3]), P(fit), st)
+    fit = fit.cpu().numpy()
+    for k, ((a, b), f) in enumerate(zip(sets, flags)):
+        a, b = a[f], b[f]
+        if len(a) < 2 or k == 5:
+            assert np.isnan(fit[k]).all(), k
+            continue
+        M = np.zeros((2 * len(a), 4))
+        M[0::2] = np.c_[a[:, 0], -a[:, 1], np.ones(len(a)), np.zeros(len(a))]
+        M[1::2] = np.c_[a[:, 1], a[:, 0], np.zeros(len(a)), np.ones(len(a))]
+        (sa, sb, tx, ty), *_ = np.linalg.lstsq(M, b.reshape(-1), rcond=None)
+        want = (math.atan2(sb, sa), math.hypot(sa, sb), tx, ty)
+        np.testing.assert_allclose(fit[k], want, rtol=1e-9, atol=1e-7)
