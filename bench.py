@@ -617,14 +617,17 @@ def bench_filter(args):
 def filter_stage_times(d, net, tau, reps=5):
     """Matching + verification of the test flight's pairs with and without the pruning stage:
     guided matching (PairMatcher.run, or run_filtered with the network and tau) through the host
-    match lists, then homography RANSAC over every pair's list with the labelling settings.
+    match lists, then homography RANSAC (a seed per pair, positions gathered on the device, as
+    match_pairs verifies) over every pair's list with the labelling threshold and hypotheses.
     Median of `reps` runs each; also the matches reaching verification and the inliers found."""
     import torch
-    from fieldstitch_amd.verify import ransac_homography
+    from fieldstitch_amd import match_flow
     pm, xy = d["matcher"], d["xy"]
     n_hyp, thr, seed = d["ransac"]
     sz = [torch.as_tensor(np.asarray(v, np.float32)).cuda() for v in d["sizes"]]
     an = [torch.as_tensor(np.asarray(v, np.float32)).cuda() for v in d["angles"]]
+    pos = match_flow.Positions({i: x for i, x in enumerate(xy)})     # every photo's positions on the device
+    seeds = [match_flow.pair_seed(i, j, 0x1) for i, j in pm.pairs]
 
     def once(filtered):
         torch.cuda.synchronize()
@@ -635,9 +638,9 @@ def filter_stage_times(d, net, tau, reps=5):
             pm.run("mfma")
         res = pm.results()
         t1 = time.perf_counter()
-        pa = [xy[i][q] for (i, j), (q, _) in zip(pm.pairs, res)]
-        pb = [xy[j][t] for (i, j), (_, t) in zip(pm.pairs, res)]
-        _, _, cnt = ransac_homography(pa, pb, n_hyp=n_hyp, thr=thr, seed=seed)
+        _, _, cnt = match_flow.ransac_rows("homography", pos, [pos.rows(i, q) for (i, _), (q, _) in zip(pm.pairs, res)],
+                                           [pos.rows(j, t) for (_, j), (_, t) in zip(pm.pairs, res)],
+                                           [thr] * len(res), seeds, n_hyp)
         t2 = time.perf_counter()
         return t1 - t0, t2 - t1, sum(len(q) for q, _ in res), int(np.sum(cnt))
 

@@ -26,6 +26,13 @@ from __future__ import annotations
 import numpy as np
 
 
+def _first_of_runs(s):
+    """True at the first element of each run of equal values of s (empty for empty s)."""
+    f = np.ones(len(s), bool)
+    f[1:] = s[1:] != s[:-1]
+    return f
+
+
 def unique(q: np.ndarray, t: np.ndarray, dd: np.ndarray):
     """matching.py:103-110: keep one query per target (smallest dd, then earliest position);
     output sorted by target."""
@@ -33,7 +40,7 @@ def unique(q: np.ndarray, t: np.ndarray, dd: np.ndarray):
         return q, t
     order = np.lexsort((dd, t))
     t_s = t[order]
-    first = np.r_[True, t_s[1:] != t_s[:-1]]
+    first = _first_of_runs(t_s)
     keep = order[first]
     return q[keep], t[keep]
 
@@ -175,7 +182,7 @@ def match_fp64(da, db, ratio: float, qs=None, cands=None, block: int = 1024):
     d = np.concatenate(out_d).astype(np.float64)
     order = np.lexsort((q, d, t))
     t_s = t[order]
-    first = np.r_[True, t_s[1:] != t_s[:-1]]
+    first = _first_of_runs(t_s)
     keep = order[first]
     return q[keep], t[keep], d[keep]
 
@@ -221,7 +228,7 @@ def match_fp64_fast(da, db, ratio: float, block: int = 2048):
     d = np.concatenate(out_d)
     order = np.lexsort((q, d, t))
     t_s = t[order]
-    first = np.r_[True, t_s[1:] != t_s[:-1]]
+    first = _first_of_runs(t_s)
     keep = order[first]
     return q[keep], t[keep], d[keep]
 
@@ -271,7 +278,7 @@ def guided_fp64_fast(da, xy_a, db, xy_b, Hm, wh_b, radius: float, ratio: float):
     dd = b1[acc_]
     o = np.lexsort((q, dd, t))
     t_s = t[o]
-    f = np.r_[True, t_s[1:] != t_s[:-1]]
+    f = _first_of_runs(t_s)
     keep = o[f]
     return q[keep], t[keep]
 
