@@ -1007,7 +1007,9 @@ def main():
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "launches_per_step": n_launch, "algorithmic_bytes_per_launch": pyr_b / n_launch,
                      "avg_launch_us": stages["pyramid"] * 1e3 / n_launch,
-                     "algorithmic_bytes_per_step": pyr_b, "time_ms_per_step": stages["pyramid"]},
+                     "algorithmic_bytes_per_step": pyr_b, "time_ms_per_step": stages["pyramid"],
+                     "note": "k_blur is VALU-issue-bound: float64 taps in the reference's order, ~86 % VALU busy by "
+                             "counters, ~2.3 ms of VALU issue per step at 100 % (profiles/pmc_k_blur_8x2640x1978.md)"},
     }
     log("sift timed: %.3f ms per step; matching next" % (t_sift * 1e3))
     res["matching"], host, pairs, Hm = bench_matching(fb, fl, shapes, args)

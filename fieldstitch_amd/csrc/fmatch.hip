@@ -252,7 +252,8 @@ typedef int i32x8_t __attribute__((ext_vector_type(8)));
 // (td[j-1], d, td[j]) (td[0]: the smaller of d and td[0]), so every entry comes from the old
 // list without a carried value; written from the top down in place.  d goes after entries
 // equal to it (arrival order among ties); the certificate only needs the K-th value as a
-// bound, so the order among ties never changes a result.  d = +inf changes nothing.
+// bound, so the order among ties never changes a result.  A d no smaller than td[K-1]
+// (+inf included) changes nothing.
 template <int K>
 __device__ __forceinline__ void topk_insert(float d, int t, float* td, int* ti) {
 #pragma unroll
@@ -433,8 +434,11 @@ __global__ __launch_bounds__(256) void k_f_screen(const FBatch B) {
                     }
                     if (m) {
                         // software-pipelined: the next set element's value is read while the
-                        // current one is inserted; an element the moved threshold now rejects
-                        // is inserted as +inf, which changes nothing (no branch in the loop)
+                        // current one is inserted.  No per-element threshold test: an element
+                        // no better than the list's K-th entry leaves the list unchanged, so
+                        // the gate is refreshed once, after the loop.  Row of element e:
+                        // (e / 4) * 8 + (lane / 32) * 4 + e % 4 of this 32-row step.
+                        const int rb = base + h * 32 + (lane >> 5) * 4;
                         int e = __builtin_ctz(m);
                         m &= m - 1;
                         float v = my[4 * ((e >> 2) ^ sw) + (e & 3)];
@@ -443,15 +447,12 @@ __global__ __launch_bounds__(256) void k_f_screen(const FBatch B) {
                             const int en = more ? __builtin_ctz(m) : e;
                             const float vn = my[4 * ((en >> 2) ^ sw) + (en & 3)];
                             m = more ? (m & (m - 1)) : 0u;
-                            const bool take = v > hthr[u];
-                            const int r = h * 32 + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3);
-                            topk_insert<K>(take ? qn2[u] - 2.0f * v : INFINITY, base + r, td[u], ti[u]);
-                            const float tau = td[u][K - 1];
-                            hthr[u] = take ? fmaxf(hthr[u], gate_of(qn2[u], tau)) : hthr[u];
+                            topk_insert<K>(qn2[u] - 2.0f * v, rb + ((e & 12) << 1) + (e & 3), td[u], ti[u]);
                             if (!more) break;
                             e = en;
                             v = vn;
                         }
+                        hthr[u] = fmaxf(hthr[u], gate_of(qn2[u], td[u][K - 1]));
                     }
                     // both half-waves of a query gate with the tighter threshold: an element of
                     // the union's top-K lies below both halves' K-th values
