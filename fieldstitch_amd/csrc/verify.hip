@@ -71,13 +71,14 @@ __device__ bool solve_h4(const double* xa, const double* ya, const double* xb, c
     return true;
 }
 
+// reprojection error of (x, y) -> (u, v) below sqrt(thr2), without a division:
+// |H p / w - q|^2 < thr2  <=>  |H p - w q|^2 < thr2 w^2 (w != 0)
 __device__ __forceinline__ bool is_inlier(const double* h, double x, double y, double u, double v, double thr2) {
     const double w = (h[6] * x + h[7] * y) + h[8];
     if (!(fabs(w) > 1e-12)) return false;
-    const double px = ((h[0] * x + h[1] * y) + h[2]) / w;
-    const double py = ((h[3] * x + h[4] * y) + h[5]) / w;
-    const double dx = px - u, dy = py - v;
-    return dx * dx + dy * dy < thr2;
+    const double ex = ((h[0] * x + h[1] * y) + h[2]) - u * w;
+    const double ey = ((h[3] * x + h[4] * y) + h[5]) - v * w;
+    return ex * ex + ey * ey < thr2 * (w * w);
 }
 
 // ---- models: K points per sample, solve(K points) -> 9 parameters, inlier test -------------
@@ -252,9 +253,9 @@ struct ModelF {
                      c1 = (F[2] * u + F[5] * v) + F[8];
         const double n2 = a2 * a2 + b2 * b2, n1 = a1 * a1 + b1 * b1;
         if (!(n1 > 1e-30 && n2 > 1e-30)) return false;
+        // max(d2^2 / n2, d1^2 / n1) < thr2, without a division (n1, n2 > 0)
         const double d2 = (u * a2 + v * b2) + c2, d1 = (x * a1 + y * b1) + c1;
-        const double e2 = d2 * d2 / n2, e1 = d1 * d1 / n1;
-        return (e1 > e2 ? e1 : e2) < thr2;
+        return d2 * d2 < thr2 * n2 && d1 * d1 < thr2 * n1;
     }
 };
 

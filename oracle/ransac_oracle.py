@@ -80,16 +80,16 @@ def solve_h4(xa, ya, xb, yb):
 
 
 def inliers(h, a, b, thr2):
-    """h (m, 9); a, b (n, 2) -> (m, n) bool."""
+    """h (m, 9); a, b (n, 2) -> (m, n) bool: reprojection error below sqrt(thr2), tested as
+    |H p - w q|^2 < thr2 w^2 (no division, as verify.hip)."""
     x, y, u, v = a[None, :, 0], a[None, :, 1], b[None, :, 0], b[None, :, 1]
     H = [h[:, k:k + 1] for k in range(9)]
     with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
         w = (H[6] * x + H[7] * y) + H[8]
         good = np.abs(w) > 1e-12
-        px = ((H[0] * x + H[1] * y) + H[2]) / w
-        py = ((H[3] * x + H[4] * y) + H[5]) / w
-    dx, dy = px - u, py - v
-    return good & (dx * dx + dy * dy < thr2)
+        ex = ((H[0] * x + H[1] * y) + H[2]) - u * w
+        ey = ((H[3] * x + H[4] * y) + H[5]) - v * w
+        return good & (ex * ex + ey * ey < thr2 * (w * w))
 
 
 def rank2_project(f):
@@ -214,7 +214,8 @@ def solve_f8(xa, ya, xb, yb):
 
 
 def inliers_f(F, a, b, thr2):
-    """F (m, 9); a, b (n, 2) -> (m, n) bool: max squared point-to-epipolar-line distance < thr2."""
+    """F (m, 9); a, b (n, 2) -> (m, n) bool: max squared point-to-epipolar-line distance < thr2,
+    tested as d^2 < thr2 * |line normal|^2 on both lines (no division, as verify.hip)."""
     x, y, u, v = a[None, :, 0], a[None, :, 1], b[None, :, 0], b[None, :, 1]
     f = [F[:, k:k + 1] for k in range(9)]
     with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
@@ -224,10 +225,8 @@ def inliers_f(F, a, b, thr2):
         n1 = a1 * a1 + b1 * b1
         d2 = (u * a2 + v * b2) + c2
         d1 = (x * a1 + y * b1) + c1
-        e2 = d2 * d2 / n2
-        e1 = d1 * d1 / n1
-        e = np.where(e1 > e2, e1, e2)
-    return (n1 > 1e-30) & (n2 > 1e-30) & (e < thr2)
+        ok = (d2 * d2 < thr2 * n2) & (d1 * d1 < thr2 * n1)
+    return (n1 > 1e-30) & (n2 > 1e-30) & ok
 
 
 def solve_s2(xa, ya, xb, yb):
