@@ -18,6 +18,7 @@
 //   fallback: one block per uncertain query, exact float64 over all targets;
 //   unique / compact: per target the smallest (b1, q), then (q, t) lists in target order.
 #include "fsx_common.h"
+#include <float.h>
 #include <math.h>
 #include <string.h>
 #include <algorithm>
@@ -247,33 +248,39 @@ typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
 typedef int i32x8_t __attribute__((ext_vector_type(8)));
 
-
-// Branch-free insertion into an ascending list: the new list's entry j is the median of
-// (td[j-1], d, td[j]) (td[0]: the smaller of d and td[0]), so every entry comes from the old
-// list without a carried value; written from the top down in place.  d goes after entries
-// equal to it (arrival order among ties); the certificate only needs the K-th value as a
-// bound, so the order among ties never changes a result.  A d no smaller than td[K-1]
-// (+inf included) changes nothing.
-template <int K>
-__device__ __forceinline__ void topk_insert(float d, int t, float* td, int* ti) {
-#pragma unroll
-    for (int j = K - 1; j >= 1; j--) {
-        const bool below_prev = d < td[j - 1];
-        const bool below = d < td[j];
-        ti[j] = below_prev ? ti[j - 1] : (below ? t : ti[j]);
-        td[j] = __builtin_amdgcn_fmed3f(td[j - 1], d, td[j]);
-    }
-    const bool below0 = d < td[0];
-    ti[0] = below0 ? t : ti[0];
-    td[0] = below0 ? d : td[0];
+// Candidate lists hold keys: a key is a non-negative float whose low `ib` bits are replaced by
+// the target's index (ib = bits of the target count), so one float compare orders by screened
+// distance truncated to the key's mantissa bits, ties by index, and one v_med3_f32 per slot
+// inserts (no separate index array, no selects).  The truncated value (index bits cleared) is
+// <= the screened distance, so the K-th key's value bounds every target outside the list from
+// below: such a target's key is larger, hence its truncated distance, and with it its screened
+// distance, is at least that value.  An empty slot holds +inf; real keys are finite (the
+// distance is finite and its exponent field is kept).
+__device__ __forceinline__ float make_key(float d, unsigned himask, unsigned idx) {
+    return __uint_as_float((__float_as_uint(fmaxf(d, 0.0f)) & himask) | idx);
 }
 
-// Score gate of a query with squared norm qn2 whose K-th screened distance is at most tau: a
+// Insertion into an ascending key list: the new entry j is the median of (kl[j-1], key,
+// kl[j]) (entry 0: the smaller of key and kl[0]), written from the top down in place.  A key
+// above kl[K-1] (+inf included) changes nothing.
+template <int K>
+__device__ __forceinline__ void key_insert(float key, float* kl) {
+#pragma unroll
+    for (int j = K - 1; j >= 1; j--) kl[j] = __builtin_amdgcn_fmed3f(kl[j - 1], key, kl[j]);
+    kl[0] = fminf(kl[0], key);
+}
+
+// Score gate of a query with squared norm qn2 whose K-th screened distance is at least tau: a
 // target whose accumulator (a.b - |b|^2 / 2) does not exceed it has screened distance
-// qn2 - 2 acc >= tau (the margin covers the float32 roundings of both sides); -inf while
-// tau is +inf (the list is not full).
+// qn2 - 2 acc >= tau (the margin covers the float32 roundings of both sides).
 __device__ __forceinline__ float gate_of(float qn2, float tau) {
     return 0.5f * (qn2 - tau) - 0x1p-20f * (fabsf(qn2) + fabsf(tau)) - 1e-30f;
+}
+
+// the gate of a key list: -FLT_MAX while it has an empty slot (every real target enters;
+// rows past the end carry -inf and never do)
+__device__ __forceinline__ float gate_of_list(float qn2, float kK, unsigned himask) {
+    return kK == INFINITY ? -FLT_MAX : gate_of(qn2, __uint_as_float(__float_as_uint(kK) & himask));
 }
 
 template <int PREC, int K, int FS_TILE, int QB>                   // QB: query blocks per wave
@@ -286,26 +293,27 @@ __global__ __launch_bounds__(256) void k_f_screen(const FBatch B) {
     constexpr int NV = FS_TILE * ROWB / 16 / 256;                 // 16-byte pieces per thread per tile
     static_assert(NV * 16 * 256 == FS_TILE * ROWB && FS_TILE <= 256, "tile staging layout");
     constexpr int NBUF = 2;                                       // next tile fetched during this one
-    constexpr int SBYTES = NBUF * TBYTES > 256 * QB * K * 8 ? NBUF * TBYTES : 256 * QB * K * 8;
+    constexpr int SBYTES = NBUF * TBYTES > 256 * QB * K * 4 ? NBUF * TBYTES : 256 * QB * K * 4;
     __shared__ __attribute__((aligned(16))) uint8_t s_rows[SBYTES];
     __shared__ __attribute__((aligned(16))) float s_c0[NBUF][FS_TILE];   // -|b|^2 / 2, -inf past the end
     __shared__ int s_sx[NBUF][FS_TILE];                                   // FP8: E8M0 scale of the row
     __shared__ __attribute__((aligned(16))) float s_acc[256 * 16];        // per lane: one fragment's values
-    float* s_md = (float*)s_rows;
-    int* s_mi = (int*)(s_rows + 256 * QB * K * 4);
+    float* s_mk = (float*)s_rows;
     const FPair P = B.pair[blockIdx.y];
     const FSet sa = B.set[P.a], sb = B.set[P.b];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     if ((int)blockIdx.x * 128 * QB >= sa.n) return;                 // block-uniform
     const bool pair_ok = sa.n >= 2 && sb.n >= 2;
-    // query operands, norms and top-K lists of the QB blocks
+    // key layout of this pair: target indices below 2^ib
+    const int ib = 32 - __clz(max(sb.n, 1));
+    const unsigned himask = ~((1u << ib) - 1u);
+    // query operands, norms and key lists of the QB blocks
     bf16x8_t qb[QB][8];
     i32x8_t q8[QB][2];
     int qsx[QB];
     float qn2[QB], hthr[QB];
     bool valid[QB];
-    float td[QB][K];
-    int ti[QB][K];
+    float kl[QB][K];
 #pragma unroll
     for (int u = 0; u < QB; u++) {
         const int q = blockIdx.x * 128 * QB + (wv * QB + u) * 32 + (lane & 31);
@@ -325,12 +333,10 @@ __global__ __launch_bounds__(256) void k_f_screen(const FBatch B) {
             qsx[u] = (int)B.sexp[ka];
         }
         qn2[u] = (float)B.n2[ka];
-        hthr[u] = valid[u] ? -INFINITY : INFINITY;       // empty list: every real target enters
+        // an invalid query gates at +inf: hthr - acc is never negative, nothing enters
+        hthr[u] = valid[u] ? -FLT_MAX : INFINITY;
 #pragma unroll
-        for (int j = 0; j < K; j++) {
-            td[u][j] = INFINITY;
-            ti[u][j] = 0x7fffffff;
-        }
+        for (int j = 0; j < K; j++) kl[u][j] = INFINITY;
     }
     uint4 pv[NV];
     float pc0 = 0.0f;
@@ -414,45 +420,44 @@ __global__ __launch_bounds__(256) void k_f_screen(const FBatch B) {
             }
 #pragma unroll
             for (int u = 0; u < QB; u++) {
-                // the gate: one compare of the lane's largest value against its threshold
-                float mx = acc[u][0];
+                // pass mask of the lane's 16 values: bit e = acc[e] > hthr, the sign of
+                // hthr - acc[e] (never -0 or NaN: hthr is finite or +inf, acc finite or -inf),
+                // shifted in by one funnel shift per element
+                unsigned m = 0;
 #pragma unroll
-                for (int e = 1; e < 16; e++) mx = fmaxf(mx, acc[u][e]);
-                if (__any(mx > hthr[u])) {
-                    unsigned m = 0;
-#pragma unroll
-                    for (int e = 0; e < 16; e++) m |= acc[u][e] > hthr[u] ? (1u << e) : 0u;
-                    // the lane's 16 values to its LDS slot (float4 chunks swizzled by lane & 3
-                    // against bank conflicts), then one iteration per passing element
-                    float* my = s_acc + ((wv * 64 + lane) << 4);
-                    const int sw = lane & 3;
+                for (int e = 15; e >= 0; e--)
+                    m = __builtin_amdgcn_alignbit(m, __float_as_uint(hthr[u] - acc[u][e]), 31);
+                if (__any(m != 0u)) {
                     if (m) {
+                        // the lane's 16 values to its LDS slot (float4 chunks swizzled by lane & 3
+                        // against bank conflicts), then one iteration per passing element,
+                        // software-pipelined: the next element's value is read while the
+                        // current one is inserted.  No per-element threshold test: a key above
+                        // the list's K-th leaves it unchanged, so the gate is refreshed once,
+                        // after the loop.  Row of element e: (e / 4) * 8 + (lane / 32) * 4 + e % 4.
+                        float* my = s_acc + ((wv * 64 + lane) << 4);
+                        const int sw = lane & 3;
 #pragma unroll
                         for (int c = 0; c < 4; c++)
                             *(float4*)(my + 4 * (c ^ sw)) =
                                 make_float4(acc[u][4 * c], acc[u][4 * c + 1], acc[u][4 * c + 2], acc[u][4 * c + 3]);
-                    }
-                    if (m) {
-                        // software-pipelined: the next set element's value is read while the
-                        // current one is inserted.  No per-element threshold test: an element
-                        // no better than the list's K-th entry leaves the list unchanged, so
-                        // the gate is refreshed once, after the loop.  Row of element e:
-                        // (e / 4) * 8 + (lane / 32) * 4 + e % 4 of this 32-row step.
-                        const int rb = base + h * 32 + (lane >> 5) * 4;
+                        const unsigned rb = (unsigned)(base + h * 32 + (lane >> 5) * 4);
                         int e = __builtin_ctz(m);
                         m &= m - 1;
                         float v = my[4 * ((e >> 2) ^ sw) + (e & 3)];
                         while (true) {
-                            const bool more = m != 0;
-                            const int en = more ? __builtin_ctz(m) : e;
+                            const bool more_e = m != 0;
+                            const int en = more_e ? __builtin_ctz(m) : e;
                             const float vn = my[4 * ((en >> 2) ^ sw) + (en & 3)];
-                            m = more ? (m & (m - 1)) : 0u;
-                            topk_insert<K>(qn2[u] - 2.0f * v, rb + ((e & 12) << 1) + (e & 3), td[u], ti[u]);
-                            if (!more) break;
+                            m = more_e ? (m & (m - 1)) : 0u;
+                            // screened distance qn2 - 2v (2v is exact: one rounding)
+                            const float d = __builtin_fmaf(-2.0f, v, qn2[u]);
+                            key_insert<K>(make_key(d, himask, rb + ((e & 12) << 1) + (e & 3)), kl[u]);
+                            if (!more_e) break;
                             e = en;
                             v = vn;
                         }
-                        hthr[u] = fmaxf(hthr[u], gate_of(qn2[u], td[u][K - 1]));
+                        hthr[u] = fmaxf(hthr[u], gate_of_list(qn2[u], kl[u][K - 1], himask));
                     }
                     // both half-waves of a query gate with the tighter threshold: an element of
                     // the union's top-K lies below both halves' K-th values
@@ -468,23 +473,22 @@ __global__ __launch_bounds__(256) void k_f_screen(const FBatch B) {
 #pragma unroll
     for (int u = 0; u < QB; u++)
 #pragma unroll
-        for (int j = 0; j < K; j++) {
-            s_md[(u * 256 + tid) * K + j] = td[u][j];
-            s_mi[(u * 256 + tid) * K + j] = ti[u][j];
-        }
+        for (int j = 0; j < K; j++) s_mk[(u * 256 + tid) * K + j] = kl[u][j];
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < QB; u++) {
         if (lane < 32 && valid[u]) {
 #pragma unroll
-            for (int j = 0; j < K; j++)
-                topk_insert<K>(s_md[(u * 256 + tid + 32) * K + j], s_mi[(u * 256 + tid + 32) * K + j], td[u], ti[u]);
+            for (int j = 0; j < K; j++) key_insert<K>(s_mk[(u * 256 + tid + 32) * K + j], kl[u]);
             const int q = blockIdx.x * 128 * QB + (wv * QB + u) * 32 + lane;
             const long long g = P.q_off + q;
 #pragma unroll
             for (int j = 0; j < K; j++) {
-                B.topd[g * FK_MAX + j] = td[u][j];
-                B.topi[g * FK_MAX + j] = ti[u][j];
+                // value: the truncated screened distance (a lower bound of it); -1: empty slot
+                const bool full = kl[u][j] != INFINITY;
+                const unsigned kb = __float_as_uint(kl[u][j]);
+                B.topd[g * FK_MAX + j] = full ? __uint_as_float(kb & himask) : INFINITY;
+                B.topi[g * FK_MAX + j] = full ? (int)(kb & ~himask) : -1;
             }
         }
     }
