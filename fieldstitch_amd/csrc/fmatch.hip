@@ -131,10 +131,11 @@ __device__ __forceinline__ unsigned f2e4m3(float v) {
     return sign | code;
 }
 
-__device__ __forceinline__ double e4m3_value(unsigned c) {
-    const double s = (c & 0x80u) ? -1.0 : 1.0;
+// the value of an e4m3 code as a float (exact: at most 4 significant bits, exponents -9..8)
+__device__ __forceinline__ float e4m3_float(unsigned c) {
     const unsigned ex = (c >> 3) & 15u, m = c & 7u;
-    return s * (ex == 0 ? (double)m * 0x1p-9 : ldexp(1.0 + (double)m / 8.0, (int)ex - 7));
+    const float mag = ex == 0 ? (float)m * 0x1p-9f : __uint_as_float(((ex + 120u) << 23) | (m << 20));
+    return (c & 0x80u) ? -mag : mag;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -192,12 +193,15 @@ __global__ __launch_bounds__(256) void k_f_quant(const FBatch B) {
                 ex = ex < -127 ? -127 : (ex > 127 ? 127 : ex);
             }
             sc = ldexpf(1.0f, ex);
+            // x / sc as x * 2^-ex: both are the correctly rounded x 2^-ex (2^-ex is a float
+            // for |ex| <= 127), the multiply without a divide sequence
+            const float isc = ldexpf(1.0f, -ex);
             unsigned w = 0;
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                const unsigned c = f2e4m3(fminf(fmaxf(x[u] / sc, -448.0f), 448.0f));
+                const unsigned c = f2e4m3(fminf(fmaxf(x[u] * isc, -448.0f), 448.0f));
                 w |= c << (8 * u);
-                const double d = (double)x[u] - (double)sc * e4m3_value(c);
+                const double d = (double)x[u] - (double)sc * (double)e4m3_float(c);
                 e2 += d * d;
             }
             ((unsigned*)(B.q8 + k * FD))[j] = w;
@@ -267,7 +271,8 @@ template <int K>
 __device__ __forceinline__ void key_insert(float key, float* kl) {
 #pragma unroll
     for (int j = K - 1; j >= 1; j--) kl[j] = __builtin_amdgcn_fmed3f(kl[j - 1], key, kl[j]);
-    kl[0] = fminf(kl[0], key);
+    // entry 0: both are non-negative floats or +inf, ordered as unsigned integers
+    kl[0] = __uint_as_float(min(__float_as_uint(kl[0]), __float_as_uint(key)));
 }
 
 // Score gate of a query with squared norm qn2 whose K-th screened distance is at least tau: a
@@ -297,7 +302,11 @@ __global__ __launch_bounds__(256) void k_f_screen(const FBatch B) {
     __shared__ __attribute__((aligned(16))) uint8_t s_rows[SBYTES];
     __shared__ __attribute__((aligned(16))) float s_c0[NBUF][FS_TILE];   // -|b|^2 / 2, -inf past the end
     __shared__ int s_sx[NBUF][FS_TILE];                                   // FP8: E8M0 scale of the row
-    __shared__ __attribute__((aligned(16))) float s_acc[256 * 16];        // per lane: one fragment's values
+    // FP8 stages each lane's 16 accumulator values in LDS for its insertion loop; BF16 inserts
+    // straight from the registers, element by element (its MFMA time per step is twice FP8's,
+    // so the per-element branches cost relatively less, and the LDS it saves buys a fourth wave)
+    constexpr bool STAGED = PREC == 2;
+    __shared__ __attribute__((aligned(16))) float s_acc[STAGED ? 256 * 16 : 4];
     float* s_mk = (float*)s_rows;
     const FPair P = B.pair[blockIdx.y];
     const FSet sa = B.set[P.a], sb = B.set[P.b];
@@ -418,50 +427,75 @@ __global__ __launch_bounds__(256) void k_f_screen(const FBatch B) {
                                                                                  qsx[u]);
                 }
             }
+            // row of element e: (e / 4) * 8 + (lane / 32) * 4 + e % 4 = rb + e + (e & 12)
+            const unsigned rb = (unsigned)(base + h * 32 + (lane >> 5) * 4);
 #pragma unroll
             for (int u = 0; u < QB; u++) {
-                // pass mask of the lane's 16 values: bit e = acc[e] > hthr, the sign of
-                // hthr - acc[e] (never -0 or NaN: hthr is finite or +inf, acc finite or -inf),
-                // shifted in by one funnel shift per element
-                unsigned m = 0;
+                bool ins = false;                      // wave-uniform: some lane inserted
+                if (STAGED) {
+                    // pass mask of the lane's 16 values: bit e = acc[e] > hthr, the sign of
+                    // hthr - acc[e] (never -0 or NaN: hthr is finite or +inf, acc finite or
+                    // -inf), shifted in by one funnel shift per element
+                    unsigned m = 0;
 #pragma unroll
-                for (int e = 15; e >= 0; e--)
-                    m = __builtin_amdgcn_alignbit(m, __float_as_uint(hthr[u] - acc[u][e]), 31);
-                if (__any(m != 0u)) {
-                    if (m) {
-                        // the lane's 16 values to its LDS slot (float4 chunks swizzled by lane & 3
-                        // against bank conflicts), then one iteration per passing element,
-                        // software-pipelined: the next element's value is read while the
-                        // current one is inserted.  No per-element threshold test: a key above
-                        // the list's K-th leaves it unchanged, so the gate is refreshed once,
-                        // after the loop.  Row of element e: (e / 4) * 8 + (lane / 32) * 4 + e % 4.
-                        float* my = s_acc + ((wv * 64 + lane) << 4);
-                        const int sw = lane & 3;
+                    for (int e = 15; e >= 0; e--)
+                        m = __builtin_amdgcn_alignbit(m, __float_as_uint(hthr[u] - acc[u][e]), 31);
+                    if (__any(m != 0u)) {
+                        ins = true;
+                        if (m) {
+                            // the lane's 16 values to its LDS slot (float4 chunks swizzled by
+                            // lane & 3 against bank conflicts: element e at e ^ 4 (lane & 3)), then
+                            // one iteration per passing element, software-pipelined: the next
+                            // element's value is read while the current one is inserted
+                            float* my = s_acc + ((wv * 64 + lane) << 4);
+                            const int sw4 = (lane & 3) << 2;
 #pragma unroll
-                        for (int c = 0; c < 4; c++)
-                            *(float4*)(my + 4 * (c ^ sw)) =
-                                make_float4(acc[u][4 * c], acc[u][4 * c + 1], acc[u][4 * c + 2], acc[u][4 * c + 3]);
-                        const unsigned rb = (unsigned)(base + h * 32 + (lane >> 5) * 4);
-                        int e = __builtin_ctz(m);
-                        m &= m - 1;
-                        float v = my[4 * ((e >> 2) ^ sw) + (e & 3)];
-                        while (true) {
-                            const bool more_e = m != 0;
-                            const int en = more_e ? __builtin_ctz(m) : e;
-                            const float vn = my[4 * ((en >> 2) ^ sw) + (en & 3)];
-                            m = more_e ? (m & (m - 1)) : 0u;
-                            // screened distance qn2 - 2v (2v is exact: one rounding)
-                            const float d = __builtin_fmaf(-2.0f, v, qn2[u]);
-                            key_insert<K>(make_key(d, himask, rb + ((e & 12) << 1) + (e & 3)), kl[u]);
-                            if (!more_e) break;
-                            e = en;
-                            v = vn;
+                            for (int c = 0; c < 4; c++)
+                                *(float4*)(my + ((4 * c) ^ sw4)) =
+                                    make_float4(acc[u][4 * c], acc[u][4 * c + 1], acc[u][4 * c + 2], acc[u][4 * c + 3]);
+                            int e = __builtin_ctz(m);
+                            m &= m - 1;
+                            float v = my[e ^ sw4];
+                            while (true) {
+                                const bool more_e = m != 0;
+                                const int en = more_e ? __builtin_ctz(m) : e;
+                                const float vn = my[en ^ sw4];
+                                m &= m - 1;
+                                // screened distance qn2 - 2v (2v is exact: one rounding)
+                                const float d = __builtin_fmaf(-2.0f, v, qn2[u]);
+                                key_insert<K>(make_key(d, himask, rb + (unsigned)(e + (e & 12))), kl[u]);
+                                if (!more_e) break;
+                                e = en;
+                                v = vn;
+                            }
                         }
-                        hthr[u] = fmaxf(hthr[u], gate_of_list(qn2[u], kl[u][K - 1], himask));
                     }
+                } else {
+                    // one compare per element; an element passing in some lane is inserted
+                    // from its accumulator register, an element passing in none costs the
+                    // compare and a wave-uniform skip
+#pragma unroll
+                    for (int e = 0; e < 16; e++) {
+                        const bool pass = acc[u][e] > hthr[u];
+                        if (__any(pass)) {
+                            ins = true;
+                            if (pass) {
+                                const float d = __builtin_fmaf(-2.0f, acc[u][e], qn2[u]);
+                                key_insert<K>(make_key(d, himask, rb + (unsigned)(e + (e & 12))), kl[u]);
+                            }
+                        }
+                    }
+                }
+                // both forms insert against the threshold of the step's start (a key above the
+                // list's K-th leaves it unchanged) and refresh the gate once per step
+                if (ins) {
+                    hthr[u] = fmaxf(hthr[u], gate_of_list(qn2[u], kl[u][K - 1], himask));
                     // both half-waves of a query gate with the tighter threshold: an element of
-                    // the union's top-K lies below both halves' K-th values
-                    hthr[u] = fmaxf(hthr[u], __shfl_xor(hthr[u], 32));
+                    // the union's top-K lies below both halves' K-th values.  The swap leaves
+                    // the partner's value in one word and the lane's own in the other.
+                    const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(hthr[u]),
+                                                                     __float_as_uint(hthr[u]), false, false);
+                    hthr[u] = fmaxf(__uint_as_float(sw[0]), __uint_as_float(sw[1]));
                 }
             }
         }
