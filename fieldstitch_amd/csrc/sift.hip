@@ -156,13 +156,12 @@ __device__ __forceinline__ float up2_value(const uint8_t* __restrict__ img, int 
 // Vertical pass: a thread owns one input column and the TH output rows; it streams the
 // TH + 2*R2 input values of its column once from global memory (coalesced across the wave),
 // converts each to float64 once and accumulates from registers.  The float32-rounded result
-// goes to an LDS mid-tile (row pitch 257 floats).  Horizontal pass: a thread owns 16
-// consecutive outputs of one row, reading 16 + 2*R2 mid values once.  Epilogue: lanes along
+// goes to an LDS mid-tile (row pitch 257 floats).  Horizontal pass: a thread owns CB (about
+// TWo * TH / 256) consecutive outputs of one row, reading CB + 2*R2 mid values once.  Epilogue: lanes along
 // x, coalesced stores of the gauss layer (the next launch's input).  A taller strip reads
 // fewer halo rows per output (TH + 2*R2 per TH) but needs more registers and LDS per block.
 constexpr int BL_SW = 256;
 constexpr int BL_NT = 256;
-constexpr int BL_CB = 16;
 
 template <int R2>
 constexpr int blur_tw() { return (BL_SW - 2 * R2) & ~15; }
@@ -214,8 +213,12 @@ template <int R2, bool BASE, int BL_TH>
 __global__ __launch_bounds__(BL_NT) void k_blur(const SiftJob* __restrict__ jobs, int o, int layer, HalfTaps tp) {
     constexpr int TWo = blur_tw<R2>();
     constexpr int NW = BL_TH + 2 * R2;        // vertical window
-    constexpr int HW = BL_CB + 2 * R2;        // horizontal window
-    constexpr int NG = (TWo + BL_CB - 1) / BL_CB;
+    // horizontal tasks: CB consecutive outputs of one row each, sized so that the TH x TWo
+    // outputs make (about) one task per thread: 14 outputs per task for TWo = 224 at TH = 16,
+    // so every lane of the four waves works in both passes
+    constexpr int CB = (TWo * BL_TH + BL_NT - 1) / BL_NT;
+    constexpr int HW = CB + 2 * R2;           // horizontal window
+    constexpr int NG = (TWo + CB - 1) / CB;
     constexpr int NWV = BL_NT / 64;           // waves
     constexpr int EPR = BL_TH / NWV;          // epilogue rows per wave
     constexpr int EPC = (TWo + 63) / 64;      // epilogue column steps per lane
@@ -296,7 +299,7 @@ __global__ __launch_bounds__(BL_NT) void k_blur(const SiftJob* __restrict__ jobs
     __syncthreads();
     for (int task = tid; task < BL_TH * NG; task += BL_NT) {
         const int ly = task % BL_TH, cg = task / BL_TH;
-        const int cbase = cg * BL_CB;
+        const int cbase = cg * CB;
         float fv[HW];
 #pragma unroll
         for (int k = 0; k < HW; k++) fv[k] = (cbase + k < BL_SW) ? s_mid[ly][cbase + k] : 0.0f;
@@ -304,7 +307,7 @@ __global__ __launch_bounds__(BL_NT) void k_blur(const SiftJob* __restrict__ jobs
 #pragma unroll
         for (int k = 0; k < HW; k++) v[k] = (double)fv[k];
 #pragma unroll
-        for (int j = 0; j < BL_CB; j++) {
+        for (int j = 0; j < CB; j++) {
             if (cbase + j >= TWo) continue;
             s_out[ly][cbase + j] = blur_point<R2>(v + j, tp);
         }
