@@ -191,7 +191,9 @@ def bench_matching(fb, fl, shapes, args):
     pairs = list(fl.pairs)
     Hm = [fl.prior[p] for p in pairs]
     pm = PairMatcher(images, pairs, Hm, radius=RADIUS, kc=0, ratio=RATIO)
-    out = {"pairs": len(pairs), "radius_px": RADIUS, "ratio": RATIO, "kc": "all in window"}
+    from fieldstitch_amd import autotune
+    out = {"pairs": len(pairs), "radius_px": RADIUS, "ratio": RATIO, "kc": "all in window",
+           "autotune": autotune.tune_matcher(pm, "mfma", reps=3)}
     results = {}
     for form in ("mfma", "plain"):
         for _ in range(max(1, args.warmup)):
@@ -468,7 +470,7 @@ def bench_flight400(args, tracks=True):
         Hm.append(P)
     images = [(u8_all[i, :counts[i]], xy_all[i, :counts[i]], (shapes[i][1], shapes[i][0])) for i in range(n)]
     pm = PairMatcher(images, pl, Hm, radius=RADIUS, kc=0, ratio=RATIO)
-    out_m = {}
+    out_m = {"autotune": autotune.tune_matcher(pm, "mfma", reps=3)}
     res = {}
     for form in ("mfma", "plain"):
         pm.run(form, graph=True)

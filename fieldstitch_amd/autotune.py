@@ -1,8 +1,9 @@
 """Autotuner of libfsx's launch shapes (no counterpart in the reference).
 
 Tunables (fsx_tune_set keys): orientation blocks per image, refinement blocks per image, the
-float matcher's targets per LDS tile and blocks of 32 queries per wave, and the pyramid blur's
-row strip (output rows per tile).  Every choice gives identical results (they only change how
+float matcher's targets per LDS tile and blocks of 32 queries per wave, the pyramid blur's
+row strip (output rows per tile) and the guided matcher's contraction workgroups (one per
+chunk of 32 queries, or a fixed number of persistent ones striding over all pairs' chunks).  Every choice gives identical results (they only change how
 work is split over workgroups), so the tuner times the candidates on the caller's own
 workload with HIP events and keeps the fastest.  Choices are cached per (device, key,
 workload signature) in memory, and in the JSON file named by FSX_TUNE_CACHE if set.
@@ -17,9 +18,10 @@ import numpy as np
 
 from . import _lib
 
-ORIENT_BLOCKS, REFINE_BLOCKS, FSCREEN_TILE, FSCREEN_QB, BLUR_ROWS = 0, 1, 2, 3, 4
+ORIENT_BLOCKS, REFINE_BLOCKS, FSCREEN_TILE, FSCREEN_QB, BLUR_ROWS, MATCH_BLOCKS = 0, 1, 2, 3, 4, 5
 CANDIDATES = {ORIENT_BLOCKS: (1024, 2048, 4096, 8192), REFINE_BLOCKS: (64, 128, 256, 512),
-              FSCREEN_TILE: (64, 128), FSCREEN_QB: (1, 2), BLUR_ROWS: (8, 16, 32)}
+              FSCREEN_TILE: (64, 128), FSCREEN_QB: (1, 2), BLUR_ROWS: (8, 16, 32),
+              MATCH_BLOCKS: (0, 2048, 4096, 8192)}
 _cache = {}
 
 
@@ -106,3 +108,13 @@ def tune_float_matcher(fm, reps=2):
     times = tune(FSCREEN_TILE, run, (1, 2, 5), sig, reps)
     return {"query_blocks": qb, "tile": times, "chosen": get(FSCREEN_TILE),
             "chosen_query_blocks": get(FSCREEN_QB)}
+
+
+def tune_matcher(pm, form="mfma", reps=3):
+    """Workgroups of a PairMatcher's MFMA contraction: one per (chunk, pair), or 2048 / 4096 /
+    8192 persistent ones (capture its graph after tuning: a captured graph keeps its launch
+    shapes)."""
+    sig = ("match", form, len(pm.pairs), tuple(pm.nk[:8]), len(pm.nk))
+    run = lambda ev: pm.run(form, events=ev)     # noqa: E731
+    times = tune(MATCH_BLOCKS, run, (1, 2, 4), sig, reps)
+    return {"workgroups": times, "chosen": get(MATCH_BLOCKS)}

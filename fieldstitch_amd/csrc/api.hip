@@ -8,7 +8,7 @@ namespace fsx {
 static thread_local char g_err[1024];
 
 // tunable launch shapes: orientation blocks / image, refinement blocks / image, float-screen tile
-static int g_tune[TUNE_N_KEYS] = {4096, 256, 64, 1, 16};
+static int g_tune[TUNE_N_KEYS] = {4096, 256, 64, 1, 16, 0};
 int tune_get(int key) { return g_tune[key]; }
 
 void set_error(const char* fmt, ...) {
@@ -82,6 +82,9 @@ int fsx_tune_set(int key, int value) {
     if (key == TUNE_FSCREEN_QB) FSX_REQUIRE(value == 1 || value == 2, "query blocks must be 1 or 2 (got %d)", value);
     if (key == TUNE_BLUR_ROWS)
         FSX_REQUIRE(value == 8 || value == 16 || value == 32, "blur rows must be 8, 16 or 32 (got %d)", value);
+    if (key == TUNE_MATCH_BLOCKS)
+        FSX_REQUIRE(value == 0 || (value >= 256 && value <= 65536),
+                    "matcher workgroups must be 0 (one per chunk) or in [256, 65536] (got %d)", value);
     g_tune[key] = value;
     return FSX_OK;
 }

@@ -565,7 +565,11 @@ __device__ void query_direct(const MatchBatch& B, const MPair& P, const MImage& 
         }
 }
 
-__global__ __launch_bounds__(64) void k_m_mfma(const MatchBatch B) {
+// One 64-lane workgroup per MFMA chunk.  PERSIST = false: the grid is (chunk, pair);
+// PERSIST = true: a fixed number of persistent workgroups stride over the (pair, chunk) items,
+// pair-major, max_chunks items per pair (chunks past an image's count are skipped).
+template <bool PERSIST>
+__global__ __launch_bounds__(64) void k_m_mfma(const MatchBatch B, int max_chunks) {
     __shared__ int s_t[MF_CAP];
     __shared__ double s_cx[MF_CAP], s_cy[MF_CAP];
     __shared__ __attribute__((aligned(16))) int s_cn[MF_CAP];
@@ -575,12 +579,14 @@ __global__ __launch_bounds__(64) void k_m_mfma(const MatchBatch B) {
     __shared__ int s_i1[64];
     const int lane = threadIdx.x;
     const int qc = lane & 31;
-    {
-        const MPair P = B.pair[blockIdx.y];      // records by value: loaded once, never reloaded
+    const int n_items = PERSIST ? max_chunks * B.n_pair : 1;       // < 2^31 (checked at launch)
+    for (int item = PERSIST ? (int)blockIdx.x : 0; item < n_items; item += PERSIST ? (int)gridDim.x : 1) {
+        const int chunk = PERSIST ? item % max_chunks : (int)blockIdx.x;
+        const MPair P = B.pair[PERSIST ? item / max_chunks : (int)blockIdx.y];   // records by value
         const MImage ia = B.img[P.a];
         const MImage ib = B.img[P.b];
-        if ((int)blockIdx.x >= B.n_chunks_img[P.a]) return;      // block-uniform
-        const int2 chk = B.chunks[ia.chunk_off + blockIdx.x];
+        if (chunk >= B.n_chunks_img[P.a]) continue;      // block-uniform
+        const int2 chk = B.chunks[ia.chunk_off + chunk];
         const int kk = chk.x + qc;
         const bool valid = qc < chk.y;
         const long long sa = ia.off + (valid ? kk : 0);
@@ -599,7 +605,8 @@ __global__ __launch_bounds__(64) void k_m_mfma(const MatchBatch B) {
                 B.best[g] = -1;
                 B.cbest[g] = -1;
             }
-            return;
+            __syncthreads();               // s_mark is cleared again by the next item
+            continue;
         }
         double td = B.r2;
         int ti = 0x7fffffff;
@@ -1167,7 +1174,18 @@ static int launch_sequence(const long long* info, int form, hipEvent_t* ev, hipS
         if (form == 0)
             hipLaunchKernelGGL(k_m_plain, dim3((unsigned)((max_na + 255) / 256), n_pair), dim3(256), 0, st, b);
         else
-            hipLaunchKernelGGL(k_m_mfma, dim3((unsigned)info[FSX_MATCH_INFO_LEN - 1], n_pair), dim3(64), 0, st, b);
+        {
+            // the contraction's workgroups: one per (chunk, pair), or a fixed number of
+            // persistent ones striding over the items (TUNE_MATCH_BLOCKS)
+            const int mc = (int)info[FSX_MATCH_INFO_LEN - 1], pers = tune_get(TUNE_MATCH_BLOCKS);
+            const long long items = (long long)mc * n_pair;
+            if (pers > 0 && items > 0 && items < (1ll << 31)) {
+                hipLaunchKernelGGL(k_m_mfma<true>, dim3((unsigned)std::min<long long>(pers, items)), dim3(64), 0, st, b,
+                                   mc);
+            } else {
+                hipLaunchKernelGGL(k_m_mfma<false>, dim3((unsigned)std::max(mc, 1), n_pair), dim3(64), 0, st, b, 0);
+            }
+        }
         FSX_LAUNCH_CHECK();
     }
     if (ev) FSX_HIP(hipEventRecord(ev[2], st));

@@ -177,3 +177,52 @@ def test_graph_replays_equal_eager(gpu, form):
             assert np.array_equal(q0, q1) and np.array_equal(t0, t1), "replay %d differs" % rep
         m.run("plain" if form == "mfma" else "mfma")      # eager work in between
         torch.cuda.synchronize()
+
+
+def test_persistent_workgroups_equal_oracle(gpu):
+    """The MFMA contraction run by a fixed number of persistent workgroups (striding over all
+    pairs' chunks) gives the one-workgroup-per-chunk result and the FP64 oracle's, on pairs of
+    uneven sizes including one whose prior puts every query outside the target photo (all its
+    chunks inactive); then the autotuner's choice is one of the candidates."""
+    from fieldstitch_amd import autotune
+    from fieldstitch_amd.matching import PairMatcher
+    rng = np.random.default_rng(41)
+    images, pairs, Hms, expect = [], [], [], []
+    for p, (na, nb) in enumerate(((300, 900), (2500, 1200), (1100, 3000), (700, 700))):
+        da, xy_a, db, xy_b, Hm, wh = make_pair(rng, na, nb, W=700 + 40 * p)
+        if p == 3:
+            Hm = Hm.copy()
+            Hm[0, 2] += 5000.0           # every predicted position far outside photo b
+        images += [(da, xy_a, wh), (db, xy_b, wh)]
+        pairs.append((2 * p, 2 * p + 1))
+        Hms.append(Hm)
+        expect.append(oracle_guided(da, xy_a, db, xy_b, Hm, wh, 45.0, 0, 0.8))
+    pairs.append((2, 5))                 # photos shared between pairs
+    Hms.append(np.eye(3))
+    expect.append(oracle_guided(images[2][0], images[2][1], images[5][0], images[5][1], np.eye(3), images[5][2],
+                                45.0, 0, 0.8))
+    m = PairMatcher(images, pairs, Hms, radius=45.0, kc=0, ratio=0.8)
+    old = autotune.get(autotune.MATCH_BLOCKS)
+    got = {}
+    try:
+        for wg in (0, 256, 4096):
+            autotune.set(autotune.MATCH_BLOCKS, wg)
+            print("case persistent workgroups", wg, flush=True)
+            m.run("mfma")
+            got[wg] = m.results()
+    finally:
+        autotune.set(autotune.MATCH_BLOCKS, old)
+    assert len(expect[3][0]) == 0 and len(expect[1][0]) > 50
+    for wg, res in got.items():
+        for (q, t), (qe, te) in zip(res, expect):
+            np.testing.assert_array_equal(q, qe)
+            np.testing.assert_array_equal(t, te)
+    try:
+        tuned = autotune.tune_matcher(m, "mfma", reps=2)
+        assert tuned["chosen"] in autotune.CANDIDATES[autotune.MATCH_BLOCKS]
+        m.run("mfma")
+        for (q, t), (qe, te) in zip(m.results(), expect):
+            np.testing.assert_array_equal(q, qe)
+            np.testing.assert_array_equal(t, te)
+    finally:
+        autotune.set(autotune.MATCH_BLOCKS, old)

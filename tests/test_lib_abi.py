@@ -108,7 +108,9 @@ def test_tune_keys_validated():
             autotune.set(key, old)
     assert autotune.get(autotune.BLUR_ROWS) == 16
     L = _lib.load()
-    for key, bad in ((autotune.BLUR_ROWS, 12), (autotune.FSCREEN_TILE, 96), (autotune.FSCREEN_QB, 3), (5, 1)):
+    for key, bad in ((autotune.BLUR_ROWS, 12), (autotune.FSCREEN_TILE, 96), (autotune.FSCREEN_QB, 3),
+                     (autotune.MATCH_BLOCKS, 100), (autotune.MATCH_BLOCKS, -1), (6, 1)):
         assert L.fsx_tune_set(key, bad) != 0
+    assert autotune.get(autotune.MATCH_BLOCKS) == 0      # default: one workgroup per chunk
     v = C.c_int()
-    assert L.fsx_tune_get(5, C.byref(v)) != 0
+    assert L.fsx_tune_get(6, C.byref(v)) != 0
