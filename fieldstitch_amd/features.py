@@ -13,6 +13,7 @@
 from __future__ import annotations
 
 import ctypes as C
+import os
 import time
 from collections import OrderedDict
 
@@ -307,17 +308,63 @@ class FeatureBatch:
 
 # ---- GPU feature worker (features.py:270-381) ---------------------------------------------------
 
-def read_gray(path):
-    """Decode one photo for the worker: ``.npy`` files holding a 2-D uint8 gray image (already
-    reduced) are read directly.  JPEG decoding, reduction and CLAHE (features.py:60-71) are
-    outside this library; other paths raise, and the worker reports that photo as failed."""
+def read_gray(path, reduce=1):
+    """Decode one photo to a 2-D uint8 gray image (features.py:18-26, decode(buf, reduce,
+    color=False)).  ``.npy``: a stored gray image, already reduced (``reduce`` is not applied).
+    Anything else goes through Pillow: a JPEG is decoded reduced by 1, 2, 4 or 8 in the DCT
+    domain (libjpeg's scaled decode, which cv2's IMREAD_REDUCED_GRAYSCALE_<r> uses for JPEG;
+    the output is ceil(w / r) x ceil(h / r)) to its luma plane, with EXIF orientation ignored
+    as IMREAD_IGNORE_ORIENTATION asks; other formats Pillow reads are decoded at full size and
+    only with reduce 1.  Missing or unreadable files raise, and the worker reports that photo as
+    failed (features.py:335-336)."""
     p = str(path)
-    if not p.endswith(".npy"):
-        raise ValueError("no decoder for %s (the worker reads .npy gray images)" % p.rsplit("/", 1)[-1])
-    g = np.load(p, allow_pickle=False)
-    if g.dtype != np.uint8 or g.ndim != 2:
-        raise ValueError("%s: expected a 2-D uint8 array, got %s %s" % (p.rsplit("/", 1)[-1], g.dtype, g.shape))
-    return np.ascontiguousarray(g)
+    if p.endswith(".npy"):
+        g = np.load(p, allow_pickle=False)
+        if g.dtype != np.uint8 or g.ndim != 2:
+            raise ValueError("%s: expected a 2-D uint8 array, got %s %s" % (p.rsplit("/", 1)[-1], g.dtype, g.shape))
+        return np.ascontiguousarray(g)
+    return _pil_decode(p, reduce, "L")
+
+
+def _pil_decode(path, reduce, mode):
+    """Pillow decode of `path` to `mode` ("L" or "RGB"), JPEG reduced in the DCT domain."""
+    from PIL import Image
+    if reduce not in (1, 2, 4, 8):
+        raise ValueError("reduce must be 1, 2, 4 or 8 (got %r)" % (reduce,))
+    with Image.open(path) as im:
+        w, h = im.size
+        if im.format == "JPEG":
+            if reduce > 1 or im.mode != mode:
+                # draft takes the largest libjpeg scale 1/s with s <= w // (requested width)
+                # (and height alike): asking for the floor sizes gives s = reduce, and libjpeg's
+                # output is ceil(w / reduce) x ceil(h / reduce)
+                im.draft(mode, (max(1, w // reduce), max(1, h // reduce)))
+        elif reduce != 1:
+            raise ValueError("%s: reduced decoding is for JPEG photos (%s given)" % (path.rsplit("/", 1)[-1], im.format))
+        out = np.asarray(im.convert(mode), dtype=np.uint8)
+    exp = ((h + reduce - 1) // reduce, (w + reduce - 1) // reduce)
+    if out.shape[:2] != exp:
+        raise ValueError("%s: decoded %s, expected %s" % (path.rsplit("/", 1)[-1], out.shape[:2], exp))
+    return np.ascontiguousarray(out)
+
+
+def prepare_task(task):
+    """The reference's prepare (features.py:60-71) for a worker task (idx, path, out_npz,
+    thumb_path, reduce, nfeat), without its CLAHE step (run the worker with ``clahe=True`` so the
+    GPU equalises each batch): if ``thumb_path`` is given and missing, the photo is decoded in
+    colour, a thumbnail of ((w + 1) * reduce // 8) x ((h + 1) * reduce // 8) (area average;
+    the colour image itself at reduce 8) is written as JPEG quality 88 and the gray image is its
+    luma; otherwise the gray image is decoded directly (read_gray)."""
+    path, thumb, reduce = str(task[1]), task[3], int(task[4])
+    if thumb and not os.path.exists(thumb) and not path.endswith(".npy"):
+        from PIL import Image
+        col = _pil_decode(path, reduce, "RGB")
+        im = Image.fromarray(col)
+        if reduce < 8:
+            im = im.resize(((col.shape[1] + 1) * reduce // 8, (col.shape[0] + 1) * reduce // 8), Image.BOX)
+        im.save(thumb, format="JPEG", quality=88)
+        return np.ascontiguousarray(np.asarray(Image.fromarray(col).convert("L"), dtype=np.uint8))
+    return read_gray(path, reduce)
 
 
 def _gpu_ready():
@@ -353,12 +400,13 @@ def gpu_worker_main(task_q, res_q, n_threads, wid, batch=16, decode=read_gray, c
     sift_gpu settings: max_kp = 2.2 nfeat, contrast 0.015, edge 12), writes each feature file
     with _save and posts ("ok", (idx, n, err, timings)).  Also posted: ("gpu", wid, info) when
     ready, ("gpu_off", wid, msg) if the GPU cannot be used (without cpu_fallback every photo is
-    then reported failed), and ("exit", "gpu%d" % wid) at the end.  thumb_path and
-    reduce belong to the decoder the reference uses and are not interpreted.  With
-    ``clahe=True`` the decoder returns the plain gray image and the contrast equalisation of
-    features.py:70-71 (clip 2.0, 8 x 8 tiles) runs on the GPU on each photo of the batch.
-    decode_task (optional): called with the whole task tuple instead of decode(path), for a
-    decoder that needs reduce / thumb_path (the reference's prepare, features.py:60-71).
+    then reported failed), and ("exit", "gpu%d" % wid) at the end.  The default decoder,
+    read_gray, reads .npy gray images and JPEG photos reduced by the task's reduce; another
+    ``decode`` is called with the path alone.  With ``clahe=True`` the decoder returns the plain
+    gray image and the contrast equalisation of features.py:70-71 (clip 2.0, 8 x 8 tiles) runs
+    on the GPU on each photo of the batch.  decode_task (optional): called with the whole task
+    tuple instead, for a decoder that needs reduce / thumb_path: prepare_task is the
+    reference's prepare (features.py:60-71) with its CLAHE left to ``clahe=True``.
     cpu_fallback (optional): the CPU path of features.py:355-364 - called with a task tuple, it
     returns (idx, n, err, timings) (the reference's features.extract); when given, the photos go
     to it while the GPU is unavailable ("gpu_off") or after a GPU batch fails (then "gpu_off" is
@@ -390,7 +438,12 @@ def gpu_worker_main(task_q, res_q, n_threads, wid, batch=16, decode=read_gray, c
                 continue
             try:
                 t1 = time.perf_counter()
-                g2 = decode_task(task) if decode_task is not None else decode(task[1])
+                if decode_task is not None:
+                    g2 = decode_task(task)
+                elif decode is read_gray:
+                    g2 = read_gray(task[1], int(task[4]))      # the task's reduce
+                else:
+                    g2 = decode(task[1])
                 tt["decode"] = time.perf_counter() - t1
                 tt["bytes"] = int(g2.nbytes)
                 ready.put((task, g2, tt, None))

@@ -43,14 +43,15 @@ def test_worker_without_gpu_reports_every_photo(tmp_path, monkeypatch):
         raise RuntimeError("no ROCm GPU visible")
     monkeypatch.setattr(features, "_gpu_ready", no_gpu)
     tasks = _tasks(tmp_path, [(60, 80), (64, 96)])
-    tasks.append((7, str(tmp_path / "missing.jpg"), str(tmp_path / "7.npz"), None, 1, 1500))
+    (tmp_path / "notaphoto.jpg").write_bytes(b"not an image")
+    tasks.append((7, str(tmp_path / "notaphoto.jpg"), str(tmp_path / "7.npz"), None, 1, 1500))
     msgs = _run(tasks)
     assert msgs[0][0] == "gpu_off" and "no ROCm GPU" in msgs[0][2]
     assert msgs[-1] == ("exit", "gpu0")
     ok = {m[1][0]: m[1] for m in msgs if m[0] == "ok"}
     assert sorted(ok) == [0, 1, 7]
     assert all(r[1] == 0 and r[2] for r in ok.values())
-    assert "no decoder" in ok[7][2] and "GPU unavailable" in ok[0][2]
+    assert "cannot identify" in ok[7][2] and "GPU unavailable" in ok[0][2]
 
 
 @pytest.mark.gpu
@@ -109,7 +110,7 @@ def _cpu_extract(task):
 
 def test_worker_falls_back_to_cpu_without_gpu(tmp_path, monkeypatch):
     """No GPU and a cpu_fallback: every task of the reference's tuple form goes to the CPU
-    path (the photo that no decoder reads included), as features.py:355-364 does."""
+    path (the photo that cannot be decoded included), as features.py:355-364 does."""
     from fieldstitch_amd import features
 
     def no_gpu():
