@@ -191,18 +191,22 @@ __device__ __forceinline__ void xcd_tile(int& tx, int& ty, int& tz) {
 // mantissa bits below float32 precision read 0x10000000; if acc's low 29 bits are more than
 // 256 ulps away from that pattern, no midpoint separates acc from the reference's value and
 // both round to the same float32 (rounding is monotone).  Otherwise (~1 output in 2^20), and
-// for 0 < acc < 2^-120 where float32 is subnormal, the reference's unfused order is evaluated.
+// where the float32 result is below 2^-120 (near float32's subnormal range, 2^-126, whose
+// rounding points differ) but not 0, the reference's unfused order is evaluated.
+// The test costs four instructions: low29 - (0x10000000 - 256) <= 512 is evaluated as
+// (lo << 3) - (0x0FFFFF00 << 3) <= 4096 in 32 bits (the shift drops the bits above the low 29,
+// one shift-add), and 0 < r < 2^-120 as bits(r) - 1 < 0x037FFFFF on the float32 result r
+// (an acc just below 2^-120 that rounds up to it is >= 2^-126, where the midpoint argument holds).
 template <int R2>
 __device__ __forceinline__ float blur_point(const double* w, const HalfTaps& tp) {
     double acc = w[R2] * tp.k[0];
 #pragma unroll
     for (int j = R2; j >= 1; --j) acc = __builtin_fma(w[R2 - j] + w[R2 + j], tp.k[j], acc);
-    const unsigned long long bits = (unsigned long long)__double_as_longlong(acc);
-    const unsigned low29 = (unsigned)bits & 0x1fffffffu;
-    const bool near_mid = low29 - (0x10000000u - 256u) <= 512u;
-    // acc >= 0: the biased exponent; 0 itself is exact (all terms are zero)
-    const bool tiny = bits != 0ull && (unsigned)(bits >> 52) < 1023u - 120u;
-    if (__builtin_expect(!(near_mid || tiny), 1)) return (float)acc;
+    const unsigned lo = (unsigned)__double_as_longlong(acc);
+    const float r = (float)acc;
+    const bool near_mid = (lo << 3) - (0x0FFFFF00u << 3) <= 4096u;
+    const bool tiny = __float_as_uint(r) - 1u < 0x037FFFFFu;       // acc >= 0: r >= 0
+    if (__builtin_expect(!(near_mid || tiny), 1)) return r;
     double ex = w[R2] * tp.k[0];
 #pragma unroll
     for (int j = R2; j >= 1; --j) ex += (w[R2 - j] + w[R2 + j]) * tp.k[j];

@@ -188,7 +188,11 @@ int64_t fso_fused_blur_check(const float *win, int64_t n, int r2, const float *k
             memcpy(&bits, &acc, 8);
             const uint32_t low29 = (uint32_t)bits & 0x1fffffffu, m = (uint32_t)margin;
             const int near_mid = (uint32_t)(low29 - (0x10000000u - m)) <= 2u * m;
-            const int tiny = bits != 0 && (uint32_t)(bits >> 52) < 1023u - 120u;
+            /* the device tests the float32 result: 0 < r < 2^-120 */
+            const float r = (float)acc;
+            uint32_t rb;
+            memcpy(&rb, &r, 4);
+            const int tiny = rb - 1u < 0x037FFFFFu;
             if (!(near_mid || tiny)) {
                 out = (float)acc;
             } else {

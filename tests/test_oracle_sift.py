@@ -279,3 +279,21 @@ def test_fused_blur_midpoint_bits_test(layer):
     assert bad_g == 0 and fb_g < 50, fb_g
     Z = np.zeros((1000, 2 * r2 + 1), np.float32)
     assert so.fused_blur_check(Z, kh, 256, use_check=2) == (0, 0)
+    # results below 2^-120 (float32 subnormal territory) always take the reference order
+    T = (rng.random((2000, 2 * r2 + 1)) * 2.0 ** -124).astype(np.float32)
+    bad_t, fb_t = so.fused_blur_check(T, kh, 256, use_check=2)
+    assert bad_t == 0 and fb_t == len(T), (bad_t, fb_t)
+
+
+def test_midpoint_window_as_one_shift_add():
+    """The device evaluates low29 - (0x10000000 - 256) <= 512 (32-bit unsigned) as
+    (lo << 3) - (0x0FFFFF00 << 3) <= 4096: equal on every low word (exhaustive around the
+    window's edges, random elsewhere)."""
+    rng = np.random.default_rng(5)
+    edges = np.concatenate([np.arange(-2000, 2000) + c for c in (0x0FFFFF00, 0x10000100, 0x1FFFFF00)]) % 2 ** 32
+    lo = np.concatenate([edges, rng.integers(0, 2 ** 32, 2_000_000), [0, 2 ** 32 - 1, 0x0FFFFF00 | 0xE0000000]])
+    lo = lo.astype(np.uint64)
+    low29 = lo & 0x1FFFFFFF
+    a = ((low29 - (0x10000000 - 256)) % 2 ** 32) <= 512
+    b = (((lo << 3) % 2 ** 32 - (0x0FFFFF00 << 3)) % 2 ** 32) <= 4096
+    assert np.array_equal(a, b) and a.sum() > 1000

@@ -160,3 +160,46 @@ def test_worker_decode_task_gets_the_task_tuple(tmp_path, monkeypatch):
     msgs = _run(tasks, decode_task=dec)
     ok = [m[1] for m in msgs if m[0] == "ok"]
     assert seen == [(0, "thumb.jpg", 2)] and ok[0][1] == 0 and "cannot decode image" in ok[0][2]
+
+
+@pytest.mark.gpu
+def test_worker_jpeg_flight_with_prepare(tmp_path, gpu):
+    """JPEG photos through the worker as the pipeline feeds them: prepare_task decodes each
+    reduced by the task's factor and writes its thumbnail, the GPU equalises (clahe=True) and
+    extracts; the feature files equal sift_gpu on the CPU restatement's CLAHE of the same
+    decoded image."""
+    from PIL import Image
+    from fieldstitch_amd import features, synth
+    from oracle import clahe_oracle
+    rng = np.random.default_rng(12)
+    tasks = []
+    for k, (h, w, r) in enumerate(((480, 640, 2), (600, 820, 2), (480, 640, 1))):
+        mos = synth.Mosaic(seed=300 + k)
+        mos.polys = synth.random_polygons(rng, 10, 0, 0, w, h)
+        g = mos.render(np.eye(3)[:2], h, w, rng)
+        rgb = np.stack([g, np.clip(g.astype(int) + 12, 0, 255), np.clip(g.astype(int) - 9, 0, 255)], -1)
+        p = tmp_path / ("photo%d.jpg" % k)
+        Image.fromarray(rgb.astype(np.uint8)).save(p, format="JPEG", quality=93)
+        tasks.append((k, str(p), str(tmp_path / ("%d.npz" % k)), str(tmp_path / ("th%d.jpg" % k)), r, 1200))
+    expect = {}
+    for t in tasks:
+        g = features.read_gray(t[1], t[4])
+        expect[t[0]] = features.prepare_task((t[0], t[1], None, str(tmp_path / ("x%d.jpg" % t[0])), t[4], t[5]))
+        assert expect[t[0]].shape == g.shape
+    print("case jpeg worker", flush=True)
+    msgs = _run(tasks, n_threads=2, batch=3, decode_task=features.prepare_task, clahe=True)
+    ok = {m[1][0]: m[1] for m in msgs if m[0] == "ok"}
+    assert msgs[0][0] == "gpu" and sorted(ok) == [0, 1, 2]
+    for idx, path, out, thumb, r, nfeat in tasks:
+        _, n, err, _ = ok[idx]
+        assert err is None and n > 10
+        with Image.open(thumb) as th:
+            assert th.format == "JPEG"
+        g = clahe_oracle.apply(expect[idx])
+        xy, desc, resp = per_image_path(g, nfeat)
+        ref = str(tmp_path / ("ref%d.npz" % idx))
+        assert features._save(ref, xy, desc, resp, g.shape[1], g.shape[0]) == n
+        a, b = np.load(out), np.load(ref)
+        for key in ("xy", "resp", "red_wh"):
+            np.testing.assert_array_equal(a[key], b[key])
+        assert np.abs(a["desc"].astype(int) - b["desc"].astype(int)).max() <= 1
