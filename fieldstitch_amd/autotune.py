@@ -20,7 +20,7 @@ from . import _lib
 
 ORIENT_BLOCKS, REFINE_BLOCKS, FSCREEN_TILE, FSCREEN_QB, BLUR_ROWS, MATCH_BLOCKS = 0, 1, 2, 3, 4, 5
 CANDIDATES = {ORIENT_BLOCKS: (1024, 2048, 4096, 8192), REFINE_BLOCKS: (64, 128, 256, 512),
-              FSCREEN_TILE: (64, 128), FSCREEN_QB: (1, 2), BLUR_ROWS: (8, 16, 32),
+              FSCREEN_TILE: (64, 128), FSCREEN_QB: (1, 2), BLUR_ROWS: (8, 16, 24, 32),
               MATCH_BLOCKS: (0, 2048, 4096, 8192)}
 _cache = {}
 

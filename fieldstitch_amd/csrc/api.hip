@@ -81,7 +81,8 @@ int fsx_tune_set(int key, int value) {
     if (key == TUNE_FSCREEN_TILE) FSX_REQUIRE(value == 64 || value == 128, "tile must be 64 or 128 (got %d)", value);
     if (key == TUNE_FSCREEN_QB) FSX_REQUIRE(value == 1 || value == 2, "query blocks must be 1 or 2 (got %d)", value);
     if (key == TUNE_BLUR_ROWS)
-        FSX_REQUIRE(value == 8 || value == 16 || value == 32, "blur rows must be 8, 16 or 32 (got %d)", value);
+        FSX_REQUIRE(value == 8 || value == 16 || value == 24 || value == 32, "blur rows must be 8, 16, 24 or 32 (got %d)",
+                    value);
     if (key == TUNE_MATCH_BLOCKS)
         FSX_REQUIRE(value == 0 || (value >= 256 && value <= 65536),
                     "matcher workgroups must be 0 (one per chunk) or in [256, 65536] (got %d)", value);

@@ -28,7 +28,7 @@ enum TuneKey {
     TUNE_REFINE_BLOCKS = 1,    // refinement blocks per image
     TUNE_FSCREEN_TILE = 2,     // float matcher: targets per LDS tile (64 or 128)
     TUNE_FSCREEN_QB = 3,       // float matcher: blocks of 32 queries per wave (1 or 2)
-    TUNE_BLUR_ROWS = 4,        // pyramid blur: output rows per tile, the row strip (8, 16 or 32)
+    TUNE_BLUR_ROWS = 4,        // pyramid blur: output rows per tile, the row strip (8, 16, 24 or 32)
     TUNE_MATCH_BLOCKS = 5,     // guided matcher contraction: 0 = one workgroup per (chunk, pair),
                                // N = N persistent workgroups striding over the items
     TUNE_N_KEYS = 6

@@ -152,7 +152,7 @@ __device__ __forceinline__ float up2_value(const uint8_t* __restrict__ img, int 
 // for R2 <= 8, 224 above; tiles start on 64-byte boundaries of their row, so where rows are
 // 64-byte aligned, as in octave 0 of most photo widths, each row segment is written as whole
 // 64-byte sectors, and the vertical pass computes at most 16 + 2*R2 columns it does not store) x TH output rows
-// (the row strip: 8, 16 or 32, the tuning key TUNE_BLUR_ROWS), 256 threads.
+// (the row strip: 8, 16, 24 or 32, the tuning key TUNE_BLUR_ROWS), 256 threads.
 // Vertical pass: a thread owns one input column and the TH output rows; it streams the
 // TH + 2*R2 input values of its column once from global memory (coalesced across the wave),
 // converts each to float64 once and accumulates from registers.  The float32-rounded result
@@ -217,10 +217,11 @@ template <int R2, bool BASE, int BL_TH>
 __global__ __launch_bounds__(BL_NT) void k_blur(const SiftJob* __restrict__ jobs, int o, int layer, HalfTaps tp) {
     constexpr int TWo = blur_tw<R2>();
     constexpr int NW = BL_TH + 2 * R2;        // vertical window
-    // horizontal tasks: CB consecutive outputs of one row each, sized so that the TH x TWo
-    // outputs make (about) one task per thread: 14 outputs per task for TWo = 224 at TH = 16,
-    // so every lane of the four waves works in both passes
-    constexpr int CB = (TWo * BL_TH + BL_NT - 1) / BL_NT;
+    // horizontal tasks: CB consecutive outputs of one row each, at most 256 / TH tasks per row,
+    // so that the TH x TWo outputs make at most one task per thread (14 outputs per task for
+    // TWo = 224 at TH = 16: every lane of the four waves works in both passes)
+    constexpr int NGMAX = BL_NT / BL_TH;
+    constexpr int CB = (TWo + NGMAX - 1) / NGMAX;
     constexpr int HW = CB + 2 * R2;           // horizontal window
     constexpr int NG = (TWo + CB - 1) / CB;
     constexpr int NWV = BL_NT / 64;           // waves
@@ -1619,7 +1620,7 @@ static int fill_job(SiftJob* J, const SiftPlan& p, const uint8_t* img, int ld, d
     return FSX_OK;
 }
 
-// one blur launch with the tuned row strip (TUNE_BLUR_ROWS: 8, 16 or 32 output rows per tile)
+// one blur launch with the tuned row strip (TUNE_BLUR_ROWS: 8, 16, 24 or 32 output rows per tile)
 template <int R2, bool BASE>
 static void launch_blur(const SiftJob* jobs, int n_img, int o, int layer, int R, int C, const HalfTaps& h,
                         hipStream_t st) {
@@ -1627,6 +1628,7 @@ static void launch_blur(const SiftJob* jobs, int n_img, int o, int layer, int R,
     const int th = tune_get(TUNE_BLUR_ROWS);
     dim3 grid((C + TWo - 1) / TWo, (R + th - 1) / th, n_img);
     if (th == 8) hipLaunchKernelGGL((k_blur<R2, BASE, 8>), grid, dim3(BL_NT), 0, st, jobs, o, layer, h);
+    else if (th == 24) hipLaunchKernelGGL((k_blur<R2, BASE, 24>), grid, dim3(BL_NT), 0, st, jobs, o, layer, h);
     else if (th == 32) hipLaunchKernelGGL((k_blur<R2, BASE, 32>), grid, dim3(BL_NT), 0, st, jobs, o, layer, h);
     else hipLaunchKernelGGL((k_blur<R2, BASE, 16>), grid, dim3(BL_NT), 0, st, jobs, o, layer, h);
 }

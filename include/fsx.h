@@ -31,7 +31,9 @@ int fsx_last_error(char *buf, size_t len);
  * fieldstitch_amd.autotune): key 0 = orientation blocks per image (16..65536, default 4096),
  * 1 = refinement blocks per image (16..65536, default 256), 2 = float matcher targets per LDS
  * tile (64 or 128, default 64), 3 = float matcher blocks of 32 queries per wave (1 or 2,
- * default 1), 4 = pyramid blur output rows per tile, the row strip (8, 16 or 32, default 16).
+ * default 1), 4 = pyramid blur output rows per tile, the row strip (8, 16, 24 or 32, default
+ * 16), 5 = guided matcher contraction workgroups (0 = one per chunk of 32 queries, the
+ * default, or 256..65536 persistent ones striding over every pair's chunks).
  * Every value gives identical results.  Not synchronised with running work: set between
  * batches (a captured graph keeps the shapes it was captured with). */
 int fsx_tune_set(int key, int value);
