@@ -325,61 +325,132 @@ __device__ bool hypothesis(const double* A, const double* Bq, unsigned long long
 // seeds / thrs (device, per pair; may be null): pair p draws with (seeds[p], 0, h) instead of
 // (seed, p, h) and tests against thrs[p]^2 instead of thr2, so its result depends on its own
 // seed only, not on its position in the batch.
+struct RansacPair {
+    int p;                       // pair index of the draws
+    unsigned long long seed;
+    double thr2;
+    long long o;                 // first match row
+    int n;                       // matches
+};
+
+__device__ __forceinline__ RansacPair ransac_pair(int slot, const long long* __restrict__ off, double thr2_all,
+                                                  unsigned long long seed_all, const unsigned long long* __restrict__ seeds,
+                                                  const double* __restrict__ thrs) {
+    RansacPair r;
+    r.p = seeds ? 0 : slot;
+    r.seed = seeds ? seeds[slot] : seed_all;
+    r.thr2 = thrs ? thrs[slot] * thrs[slot] : thr2_all;
+    r.o = off[slot];
+    r.n = (int)(off[slot + 1] - r.o);
+    return r;
+}
+
+// the block's best (inliers + 1) << 32 | (0xffffffff - hypothesis) over hypotheses
+// h0 + tid, h0 + tid + step, ... < n_hyp: the largest key has the most inliers, ties the lowest
+// hypothesis; 0 when no hypothesis of the block gave a model (256 threads)
+template <class M>
+__device__ unsigned long long ransac_score(const RansacPair& R, const double* A, const double* Bq, int h0, int step,
+                                          int n_hyp) {
+    __shared__ unsigned long long s_key[256];
+    const int tid = threadIdx.x;
+    unsigned long long key = 0ull;
+    if (R.n >= M::K) {
+        for (int hy = h0 + tid; hy < n_hyp; hy += step) {
+            double h[9];
+            if (!hypothesis<M>(A, Bq, R.seed, R.p, hy, R.n, h)) continue;
+            int cnt = 0;
+            for (int i = 0; i < R.n; i++) cnt += M::inlier(h, A[2 * i], A[2 * i + 1], Bq[2 * i], Bq[2 * i + 1], R.thr2);
+            const unsigned long long k = ((unsigned long long)(cnt + 1) << 32) | (0xffffffffu - (unsigned)hy);
+            key = k > key ? k : key;
+        }
+    }
+    s_key[tid] = key;
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+        if (tid < w) s_key[tid] = s_key[tid + w] > s_key[tid] ? s_key[tid + w] : s_key[tid];
+        __syncthreads();
+    }
+    const unsigned long long best = s_key[0];
+    __syncthreads();
+    return best;
+}
+
+// the winner of key: rebuild its model (the same draws and solve), write the model, its inlier
+// count and the inlier flags of every match (256 threads)
+template <class M>
+__device__ void ransac_finish(const RansacPair& R, int slot, const double* A, const double* Bq, unsigned long long key,
+                              uint8_t* __restrict__ inlier, double* __restrict__ H_out, int* __restrict__ n_in) {
+    __shared__ double s_h[9];
+    __shared__ int s_have;
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        const int bc = (int)(key >> 32) - 1, bh = (int)(0xffffffffu - (unsigned)key);
+        double h[9];
+        const bool have = bc >= 0 && hypothesis<M>(A, Bq, R.seed, R.p, bh, R.n, h);
+        for (int k = 0; k < 9; k++) s_h[k] = have ? h[k] : 0.0;
+        s_have = have;
+        n_in[slot] = have ? bc : 0;
+        for (int k = 0; k < 9; k++) H_out[9 * slot + k] = s_h[k];
+    }
+    __syncthreads();
+    const bool have = s_have != 0;
+    double h[9];
+    for (int k = 0; k < 9; k++) h[k] = s_h[k];
+    for (int i = tid; i < R.n; i += 256)
+        inlier[R.o + i] = (uint8_t)(have && M::inlier(h, A[2 * i], A[2 * i + 1], Bq[2 * i], Bq[2 * i + 1], R.thr2));
+}
+
+// one block per pair: every hypothesis, then the winner
 template <class M>
 __global__ __launch_bounds__(256) void k_ransac(const long long* __restrict__ off, const double* __restrict__ pa,
                                                 const double* __restrict__ pb, int n_hyp, double thr2_all,
                                                 unsigned long long seed_all, const unsigned long long* __restrict__ seeds,
                                                 const double* __restrict__ thrs, uint8_t* __restrict__ inlier,
                                                 double* __restrict__ H_out, int* __restrict__ n_in) {
-    __shared__ int s_cnt[256], s_hyp[256];
-    __shared__ double s_h[9];
-    __shared__ int s_have;
-    const int tid = threadIdx.x;
-    const int pb_ = blockIdx.x;                                  // slot in the batch
-    const int p = seeds ? 0 : pb_;                               // pair index of the draws
-    const unsigned long long seed = seeds ? seeds[pb_] : seed_all;
-    const double thr2 = thrs ? thrs[pb_] * thrs[pb_] : thr2_all;
-    const long long o = off[pb_];
-    const int n = (int)(off[pb_ + 1] - o);
-    const double* A = pa + 2 * o;
-    const double* Bq = pb + 2 * o;
-    int best = -1, best_h = 0x7fffffff;
-    if (n >= M::K) {
-        for (int hy = tid; hy < n_hyp; hy += 256) {
-            double h[9];
-            if (!hypothesis<M>(A, Bq, seed, p, hy, n, h)) continue;
-            int cnt = 0;
-            for (int i = 0; i < n; i++) cnt += M::inlier(h, A[2 * i], A[2 * i + 1], Bq[2 * i], Bq[2 * i + 1], thr2);
-            if (cnt > best || (cnt == best && hy < best_h)) {
-                best = cnt;
-                best_h = hy;
-            }
+    const int slot = blockIdx.x;
+    const RansacPair R = ransac_pair(slot, off, thr2_all, seed_all, seeds, thrs);
+    const double* A = pa + 2 * R.o;
+    const double* Bq = pb + 2 * R.o;
+    const unsigned long long key = ransac_score<M>(R, A, Bq, 0, 256, n_hyp);
+    ransac_finish<M>(R, slot, A, Bq, key, inlier, H_out, n_in);
+}
+
+// Small batches: S <= 9 blocks per pair (gridDim.y) each score the hypotheses
+// h = 256 s + tid + 256 S k; block s leaves its best key in word s of the pair's 9-double model
+// slot, which k_ransac_pick reads back before it writes the model there.
+template <class M>
+__global__ __launch_bounds__(256) void k_ransac_split(const long long* __restrict__ off, const double* __restrict__ pa,
+                                                      const double* __restrict__ pb, int n_hyp, double thr2_all,
+                                                      unsigned long long seed_all,
+                                                      const unsigned long long* __restrict__ seeds,
+                                                      const double* __restrict__ thrs, double* __restrict__ H_out) {
+    const int slot = blockIdx.x;
+    const RansacPair R = ransac_pair(slot, off, thr2_all, seed_all, seeds, thrs);
+    const unsigned long long key =
+        ransac_score<M>(R, pa + 2 * R.o, pb + 2 * R.o, 256 * (int)blockIdx.y, 256 * (int)gridDim.y, n_hyp);
+    if (threadIdx.x == 0) ((unsigned long long*)H_out)[9 * slot + blockIdx.y] = key;
+}
+
+template <class M>
+__global__ __launch_bounds__(256) void k_ransac_pick(const long long* __restrict__ off, const double* __restrict__ pa,
+                                                     const double* __restrict__ pb, int n_split, double thr2_all,
+                                                     unsigned long long seed_all,
+                                                     const unsigned long long* __restrict__ seeds,
+                                                     const double* __restrict__ thrs, uint8_t* __restrict__ inlier,
+                                                     double* __restrict__ H_out, int* __restrict__ n_in) {
+    __shared__ unsigned long long s_best;
+    const int slot = blockIdx.x;
+    const RansacPair R = ransac_pair(slot, off, thr2_all, seed_all, seeds, thrs);
+    if (threadIdx.x == 0) {
+        unsigned long long key = 0ull;
+        for (int sp = 0; sp < n_split; sp++) {
+            const unsigned long long k = ((const unsigned long long*)H_out)[9 * slot + sp];
+            key = k > key ? k : key;
         }
+        s_best = key;
     }
-    s_cnt[tid] = best;
-    s_hyp[tid] = best_h;
-    __syncthreads();
-    if (tid == 0) {
-        int bc = -1, bh = 0x7fffffff;
-        for (int u = 0; u < 256; u++)
-            if (s_cnt[u] > bc || (s_cnt[u] == bc && s_hyp[u] < bh)) {
-                bc = s_cnt[u];
-                bh = s_hyp[u];
-            }
-        // rebuild the winning model (the same draws and solve as above)
-        double h[9];
-        const bool have = bc >= 0 && hypothesis<M>(A, Bq, seed, p, bh, n, h);
-        for (int k = 0; k < 9; k++) s_h[k] = have ? h[k] : 0.0;
-        s_have = have;
-        n_in[pb_] = have ? bc : 0;
-        for (int k = 0; k < 9; k++) H_out[9 * pb_ + k] = s_h[k];
-    }
-    __syncthreads();
-    const bool have = s_have != 0;
-    double h[9];
-    for (int k = 0; k < 9; k++) h[k] = s_h[k];
-    for (int i = tid; i < n; i += 256)
-        inlier[o + i] = (uint8_t)(have && M::inlier(h, A[2 * i], A[2 * i + 1], Bq[2 * i], Bq[2 * i + 1], thr2));
+    __syncthreads();                 // the keys are read before ransac_finish overwrites the slot
+    ransac_finish<M>(R, slot, pa + 2 * R.o, pb + 2 * R.o, s_best, inlier, H_out, n_in);
 }
 
 // Least-squares refit of each pair's homography on the inliers of its winning hypothesis (the
@@ -622,6 +693,25 @@ static void launch_refit(int n_pair, const long long* off_dev, const double* pa_
                            n_in_dev, M_dev);
 }
 
+// Hypothesis scoring and the winner: one block per pair, or, for batches too small to fill the
+// GPU's 256 CUs twice, S = min(9, ceil(512 / n_pair)) blocks per pair and a pick launch.  Both
+// find the same winner (most inliers, then the lowest hypothesis).
+template <class M>
+static void launch_score(int n_pair, const long long* off_dev, const double* pa_dev, const double* pb_dev, int n_hyp,
+                         double thr2, unsigned long long seed, const unsigned long long* seed_dev, const double* thr_dev,
+                         uint8_t* inlier_dev, double* M_dev, int* n_in_dev, hipStream_t stream) {
+    const int split = std::min(std::min(9, (512 + n_pair - 1) / n_pair), std::max(1, (n_hyp + 255) / 256));
+    if (split <= 1) {
+        hipLaunchKernelGGL(k_ransac<M>, dim3(n_pair), dim3(256), 0, stream, off_dev, pa_dev, pb_dev, n_hyp, thr2, seed,
+                           seed_dev, thr_dev, inlier_dev, M_dev, n_in_dev);
+        return;
+    }
+    hipLaunchKernelGGL(k_ransac_split<M>, dim3(n_pair, split), dim3(256), 0, stream, off_dev, pa_dev, pb_dev, n_hyp,
+                       thr2, seed, seed_dev, thr_dev, M_dev);
+    hipLaunchKernelGGL(k_ransac_pick<M>, dim3(n_pair), dim3(256), 0, stream, off_dev, pa_dev, pb_dev, split, thr2, seed,
+                       seed_dev, thr_dev, inlier_dev, M_dev, n_in_dev);
+}
+
 template <class M>
 static int launch_ransac(int n_pair, const long long* off_dev, const double* pa_dev, const double* pb_dev, int n_hyp,
                          double thr, unsigned long long seed, uint8_t* inlier_dev, double* H_dev, int* n_in_dev,
@@ -629,8 +719,8 @@ static int launch_ransac(int n_pair, const long long* off_dev, const double* pa_
     FSX_REQUIRE(n_pair >= 0 && n_hyp >= 1 && n_hyp <= (1 << 20) && thr > 0, "bad RANSAC arguments");
     if (n_pair == 0) return FSX_OK;
     FSX_REQUIRE(off_dev && pa_dev && pb_dev && inlier_dev && H_dev && n_in_dev, "null pointer argument");
-    hipLaunchKernelGGL(k_ransac<M>, dim3(n_pair), dim3(256), 0, (hipStream_t)stream, off_dev, pa_dev, pb_dev, n_hyp,
-                       thr * thr, seed, nullptr, nullptr, inlier_dev, H_dev, n_in_dev);
+    launch_score<M>(n_pair, off_dev, pa_dev, pb_dev, n_hyp, thr * thr, seed, nullptr, nullptr, inlier_dev, H_dev,
+                    n_in_dev, (hipStream_t)stream);
     launch_refit<M>(n_pair, off_dev, pa_dev, pb_dev, inlier_dev, n_in_dev, H_dev, (hipStream_t)stream);
     FSX_LAUNCH_CHECK();
     return FSX_OK;
@@ -644,8 +734,8 @@ static int launch_ransac_per_pair(int n_pair, const long long* off_dev, const do
     if (n_pair == 0) return FSX_OK;
     FSX_REQUIRE(off_dev && pa_dev && pb_dev && thr_dev && seed_dev && inlier_dev && M_dev && n_in_dev,
                 "null pointer argument");
-    hipLaunchKernelGGL(k_ransac<M>, dim3(n_pair), dim3(256), 0, (hipStream_t)stream, off_dev, pa_dev, pb_dev, n_hyp,
-                       0.0, 0ull, seed_dev, thr_dev, inlier_dev, M_dev, n_in_dev);
+    launch_score<M>(n_pair, off_dev, pa_dev, pb_dev, n_hyp, 0.0, 0ull, seed_dev, thr_dev, inlier_dev, M_dev, n_in_dev,
+                    (hipStream_t)stream);
     launch_refit<M>(n_pair, off_dev, pa_dev, pb_dev, inlier_dev, n_in_dev, M_dev, (hipStream_t)stream);
     FSX_LAUNCH_CHECK();
     return FSX_OK;

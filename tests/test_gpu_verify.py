@@ -123,3 +123,27 @@ def test_gather_and_similarity_fit(gpu):
         (sa, sb, tx, ty), *_ = np.linalg.lstsq(M, b.reshape(-1), rcond=None)
         want = (math.atan2(sb, sa), math.hypot(sa, sb), tx, ty)
         np.testing.assert_allclose(fit[k], want, rtol=1e-9, atol=1e-7)
+
+
+@pytest.mark.parametrize("model,thr", [("homography", 3.0), ("fundamental", 1.0), ("similarity", 4.0)])
+def test_small_batch_split_equals_one_block_per_pair(gpu, model, thr):
+    """A small batch scores each pair's hypotheses on several blocks and picks the winner in a
+    second launch; a batch of 600+ pairs uses one block per pair.  The first pairs of both
+    (same slots, same draws) give identical flags, counts and models."""
+    from fieldstitch_amd.verify import ransac
+    rng = np.random.default_rng(21)
+    H = np.array([[0.99, -0.05, 600.0], [0.05, 0.99, -40.0], [1e-6, -2e-6, 1.0]])
+    if model == "fundamental":
+        sets = [_views(rng, n, (12.0, 3.0, 1.0), f) for n, f in ((700, 0.7), (60, 0.5), (1500, 0.9))]
+    else:
+        sets = [_planted(rng, n, f, H if model == "homography" else np.array([[0.9, -0.2, 30.0], [0.2, 0.9, 5.0],
+                                                                             [0.0, 0.0, 1.0]]))
+                for n, f in ((700, 0.7), (60, 0.5), (1500, 0.9))]
+    pad = [sets[0]] * 620
+    print("case split", model, flush=True)
+    small = ransac(model, [a for a, _ in sets], [b for _, b in sets], n_hyp=2000, thr=thr, seed=77)
+    big = ransac(model, [a for a, _ in sets + pad], [b for _, b in sets + pad], n_hyp=2000, thr=thr, seed=77)
+    for p in range(len(sets)):
+        np.testing.assert_array_equal(small[0][p], big[0][p])
+        np.testing.assert_array_equal(small[1][p], big[1][p])
+        assert small[2][p] == big[2][p] and small[2][p] > 0
