@@ -252,10 +252,9 @@ struct ModelF {
         const double a1 = (F[0] * u + F[3] * v) + F[6], b1 = (F[1] * u + F[4] * v) + F[7],
                      c1 = (F[2] * u + F[5] * v) + F[8];
         const double n2 = a2 * a2 + b2 * b2, n1 = a1 * a1 + b1 * b1;
-        if (!(n1 > 1e-30 && n2 > 1e-30)) return false;
-        // max(d2^2 / n2, d1^2 / n1) < thr2, without a division (n1, n2 > 0)
+        // max(d2^2 / n2, d1^2 / n1) < thr2, without a division (n1, n2 > 0), and no branch
         const double d2 = (u * a2 + v * b2) + c2, d1 = (x * a1 + y * b1) + c1;
-        return d2 * d2 < thr2 * n2 && d1 * d1 < thr2 * n1;
+        return (n1 > 1e-30) & (n2 > 1e-30) & (d2 * d2 < thr2 * n2) & (d1 * d1 < thr2 * n1);
     }
 };
 
@@ -348,20 +347,47 @@ __device__ __forceinline__ RansacPair ransac_pair(int slot, const long long* __r
 // the block's best (inliers + 1) << 32 | (0xffffffff - hypothesis) over hypotheses
 // h0 + tid, h0 + tid + step, ... < n_hyp: the largest key has the most inliers, ties the lowest
 // hypothesis; 0 when no hypothesis of the block gave a model (256 threads)
+// Hypotheses go in rounds of one per thread; each round streams the matches through LDS in
+// tiles of 256 (one match loaded per thread, then every thread tests its hypothesis against
+// the tile: the same address for all lanes, a broadcast read).
 template <class M>
 __device__ unsigned long long ransac_score(const RansacPair& R, const double* A, const double* Bq, int h0, int step,
                                           int n_hyp) {
     __shared__ unsigned long long s_key[256];
+    __shared__ double4 s_m[256];                   // (x, y) in a, (u, v) in b
     const int tid = threadIdx.x;
     unsigned long long key = 0ull;
-    if (R.n >= M::K) {
-        for (int hy = h0 + tid; hy < n_hyp; hy += step) {
+    if (R.n >= M::K) {                             // block-uniform
+        for (int hr = h0; hr < n_hyp; hr += step) {
+            const int hy = hr + tid;
             double h[9];
-            if (!hypothesis<M>(A, Bq, R.seed, R.p, hy, R.n, h)) continue;
+            const bool ok = hy < n_hyp && hypothesis<M>(A, Bq, R.seed, R.p, hy, R.n, h);
             int cnt = 0;
-            for (int i = 0; i < R.n; i++) cnt += M::inlier(h, A[2 * i], A[2 * i + 1], Bq[2 * i], Bq[2 * i + 1], R.thr2);
+            for (int t0 = 0; t0 < R.n; t0 += 256) {
+                const int i = t0 + tid;
+                __syncthreads();                   // the previous tile has been read
+                if (i < R.n) s_m[tid] = make_double4(A[2 * i], A[2 * i + 1], Bq[2 * i], Bq[2 * i + 1]);
+                __syncthreads();
+                const int m = min(256, R.n - t0);
+                if (ok) {
+                    // four matches per iteration: independent float64 chains in flight
+                    int k = 0;
+                    for (; k + 4 <= m; k += 4) {
+                        const double4 v0 = s_m[k], v1 = s_m[k + 1], v2 = s_m[k + 2], v3 = s_m[k + 3];
+                        const int c0 = M::inlier(h, v0.x, v0.y, v0.z, v0.w, R.thr2);
+                        const int c1 = M::inlier(h, v1.x, v1.y, v1.z, v1.w, R.thr2);
+                        const int c2 = M::inlier(h, v2.x, v2.y, v2.z, v2.w, R.thr2);
+                        const int c3 = M::inlier(h, v3.x, v3.y, v3.z, v3.w, R.thr2);
+                        cnt += (c0 + c1) + (c2 + c3);
+                    }
+                    for (; k < m; k++) {
+                        const double4 v = s_m[k];
+                        cnt += M::inlier(h, v.x, v.y, v.z, v.w, R.thr2);
+                    }
+                }
+            }
             const unsigned long long k = ((unsigned long long)(cnt + 1) << 32) | (0xffffffffu - (unsigned)hy);
-            key = k > key ? k : key;
+            if (ok) key = k > key ? k : key;
         }
     }
     s_key[tid] = key;
