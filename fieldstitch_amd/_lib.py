@@ -78,6 +78,7 @@ SIGNATURES = {
                            C.POINTER(_i), C.POINTER(C.c_double), _vp],
     "fsx_fp8_layout_probe": [_vp, _vp],
     "fsx_fp8_scale_probe": [_vp, _vp],
+    "fsx_fp8_cvt_probe": [_vp, _vp],
     "fsx_mlp_gemm": [_i, _i, _i, _vp, _i, _vp, _i, _i, _i, _i, _vp, _vp, _i, _vp, _vp, _vp],
     "fsx_mlp_splitk_scratch": [_i, _i, _i, C.POINTER(_sz)],
     "fsx_mlp_gemm_splitk": [_i, _i, _vp, _i, _vp, _i, _i, _i, _i, _vp, _i, _vp, _i, _vp],

@@ -196,12 +196,17 @@ __global__ __launch_bounds__(256) void k_f_quant(const FBatch B) {
             // x / sc as x * 2^-ex: both are the correctly rounded x 2^-ex (2^-ex is a float
             // for |ex| <= 127), the multiply without a divide sequence
             const float isc = ldexpf(1.0f, -ex);
-            unsigned w = 0;
+            // e4m3 codes by the hardware conversion (round to nearest even; the same code as
+            // f2e4m3 for every float in [-448, 448] up to the sign of zero: fsx_fp8_cvt_probe
+            // checks all of them)
+            float y[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) y[u] = fminf(fmaxf(x[u] * isc, -448.0f), 448.0f);
+            unsigned w = __builtin_amdgcn_cvt_pk_fp8_f32(y[0], y[1], 0, false);
+            w = __builtin_amdgcn_cvt_pk_fp8_f32(y[2], y[3], w, true);
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                const unsigned c = f2e4m3(fminf(fmaxf(x[u] * isc, -448.0f), 448.0f));
-                w |= c << (8 * u);
-                const double d = (double)x[u] - (double)sc * (double)e4m3_float(c);
+                const double d = (double)x[u] - (double)sc * (double)e4m3_float((w >> (8 * u)) & 0xffu);
                 e2 += d * d;
             }
             ((unsigned*)(B.q8 + k * FD))[j] = w;
@@ -852,6 +857,34 @@ __global__ void k_fp8_scale_probe(float* out) {
     for (int i = 0; i < 16; i++) out[lane * 16 + i] = acc[i];
 }
 
+// The hardware e4m3 conversion against f2e4m3 on every float in [-448, 448] (2^31 + 2 bit
+// patterns: magnitudes 0 .. 448 with both signs; a +0 code for -0 counts as equal, both are
+// zero): out[0] = mismatches, out[1] = the smallest mismatching bit pattern (all ones if none).
+__global__ void k_fp8_cvt_init(unsigned long long* out) {
+    out[0] = 0ull;
+    out[1] = ~0ull;
+}
+
+__global__ __launch_bounds__(256) void k_fp8_cvt_probe(unsigned long long* out) {
+    const unsigned long long n = 2ull * (0x43E00000ull + 1ull);
+    unsigned long long bad = 0ull, first = ~0ull;
+    for (unsigned long long i = (unsigned long long)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (unsigned long long)gridDim.x * 256) {
+        const unsigned b = (unsigned)(i >> 1) | ((i & 1ull) ? 0x80000000u : 0u);
+        const float x = __uint_as_float(b);
+        const unsigned hw = __builtin_amdgcn_cvt_pk_fp8_f32(x, 0.0f, 0, false) & 0xffu;
+        const unsigned sw = f2e4m3(x);
+        if (hw != sw && ((hw | sw) & 0x7fu)) {         // +0 and -0 codes both count as zero
+            bad++;
+            first = first < b ? first : b;
+        }
+    }
+    if (bad) {
+        atomicAdd(&out[0], bad);
+        atomicMin(&out[1], first);
+    }
+}
+
 // ------------------------------------------------------------------------------------------
 // Host side
 
@@ -1075,6 +1108,14 @@ int fsx_fmatch_results(int n_set, const int* n_kp, int n_pair, const int* pa, co
 int fsx_fp8_scale_probe(float* out_dev, void* stream) {
     FSX_REQUIRE(out_dev, "null pointer argument");
     hipLaunchKernelGGL(k_fp8_scale_probe, dim3(1), dim3(64), 0, (hipStream_t)stream, out_dev);
+    FSX_LAUNCH_CHECK();
+    return FSX_OK;
+}
+
+int fsx_fp8_cvt_probe(unsigned long long* out_dev, void* stream) {
+    FSX_REQUIRE(out_dev, "null pointer argument");
+    hipLaunchKernelGGL(k_fp8_cvt_init, dim3(1), dim3(1), 0, (hipStream_t)stream, out_dev);
+    hipLaunchKernelGGL(k_fp8_cvt_probe, dim3(2048), dim3(256), 0, (hipStream_t)stream, out_dev);
     FSX_LAUNCH_CHECK();
     return FSX_OK;
 }

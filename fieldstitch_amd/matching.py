@@ -495,6 +495,17 @@ def fp8_scale_probe():
     return out.view(64, 16).cpu().numpy()
 
 
+def fp8_cvt_probe():
+    """The hardware e4m3 conversion the FP8 quantization uses against a software
+    round-to-nearest-even conversion on every float in [-448, 448]: (mismatches, smallest
+    mismatching bit pattern or None)."""
+    import torch
+    out = torch.zeros(2, dtype=torch.int64, device="cuda")
+    _lib.call("fsx_fp8_cvt_probe", C.c_void_p(out.data_ptr()), C.c_void_p(_lib.stream_handle()))
+    bad, first = out.cpu().numpy().view(np.uint64).tolist()
+    return int(bad), (None if first == 2 ** 64 - 1 else int(first))
+
+
 def _guided(A, B, Hm, radius, kc, ratio, form="mfma"):
     """matching.py:113-152: match A's features to B's keypoints within `radius` of their
     predicted position Hm(A.und), the kc spatially nearest (kc >= len(B) or 0: all)."""

@@ -268,6 +268,13 @@ int fsx_fp8_layout_probe(float *out_dev, void *stream);
  * power-of-two per-descriptor scales enter the MFMA (no reference counterpart). */
 int fsx_fp8_scale_probe(float *out_dev, void *stream);
 
+/* The float matcher's FP8 quantization converts with the hardware e4m3 conversion; this
+ * compares it with a software round-to-nearest-even e4m3 conversion on every float in
+ * [-448, 448] (a zero code of either sign for -0 counts as equal): out_dev[0] (uint64) =
+ * mismatches, out_dev[1] = the smallest mismatching bit pattern (all ones if none).  No
+ * reference counterpart. */
+int fsx_fp8_cvt_probe(unsigned long long *out_dev, void *stream);
+
 /* ---------------------------------------------------------------------------------------
  * Learned match filter: BF16 MLP kernels (no reference counterpart; trained on the inlier
  * labels of the plain pipeline, used to prune candidates before the exact ratio test)

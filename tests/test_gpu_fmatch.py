@@ -37,6 +37,14 @@ def test_fp8_scale_layout(gpu):
                 assert out[lane, i * 4 + j] == Cm[i * 8 + (lane >> 5) * 4 + j, lane & 31]
 
 
+def test_fp8_hardware_conversion_is_rne_e4m3(gpu):
+    """The FP8 quantization's hardware conversion gives the software round-to-nearest-even
+    e4m3 code for every float in [-448, 448] (all 2^31 + 2 bit patterns)."""
+    from fieldstitch_amd import matching
+    bad, first = matching.fp8_cvt_probe()
+    assert bad == 0, (bad, None if first is None else hex(first))
+
+
 def _oracle(da, db, ratio):
     q, t, _ = mo.match_fp64(da, db, ratio)
     return q, t
