@@ -603,22 +603,40 @@ __global__ __launch_bounds__(256) void k_f_rescore(const FBatch B) {
     int iv = 0x7fffffff;
     float dt = INFINITY;
     float err = 0.0f;
+    bool cand = false;
     if (live && sub < kk) {
         iv = B.topi[g * FK_MAX + sub];
         dt = B.topd[g * FK_MAX + sub];
-        if (iv >= 0 && iv < sb.n) {   // a screened slot left empty (no finite distance) stays out
-            dv = dist64(B.desc + ka * FD, B.desc + (sb.off + iv) * FD);
-            err = (float)fabs((double)dt - dv);
-        } else {
-            iv = 0x7fffffff;
-        }
+        cand = iv >= 0 && iv < sb.n;  // a screened slot left empty (no finite distance) stays out
+    }
+    // Only candidates that can be among the two smallest exact distances are rescored.  The
+    // exact distance lies within eps of the screened one, which lies in [dt, dt (1 + 2^(ib-23)))
+    // (dt: the key's value, ib index bits truncated); U2, the second smallest upper bound of the
+    // group's candidates, bounds the second smallest exact distance b2, so a candidate with
+    // dt - eps >= U2 has distance >= b2 and changes neither b1, b2 nor the best index (a tie at
+    // b1 = b2 fails the ratio test whatever the index).  Its row is not read.
+    const double eps = live ? screen_eps(B, ka, P.b) : 0.0;
+    const int ib = 32 - __clz(max(sb.n, 1));
+    double m1 = cand ? (double)dt * (1.0 + ldexp(1.0, ib - 22)) + ldexp(1.0, ib - 149) + eps : INFINITY, m2 = INFINITY;
+#pragma unroll
+    for (int off = 1; off < 8; off <<= 1) {          // the two smallest over the 8 lanes of the group
+        const double o1 = __shfl_xor(m1, off), o2 = __shfl_xor(m2, off);
+        m2 = fmin(fmax(m1, o1), fmin(m2, o2));
+        m1 = fmin(m1, o1);
+    }
+    const bool use = cand && (double)dt - eps < m2;
+    if (use) {
+        dv = dist64(B.desc + ka * FD, B.desc + (sb.off + iv) * FD);
+        err = (float)fabs((double)dt - dv);
+    } else {
+        iv = 0x7fffffff;
     }
     // screening-error statistics: block partials (same-address atomics would serialise)
     {
         __shared__ double s_st[4][3];
         float emax = err;
         double esum = err;
-        int ecnt = (live && sub < kk) ? 1 : 0;
+        int ecnt = use ? 1 : 0;
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) {
             emax = fmaxf(emax, __shfl_xor(emax, off));
@@ -668,7 +686,6 @@ __global__ __launch_bounds__(256) void k_f_rescore(const FBatch B) {
         return;
     }
     // targets outside the candidates: true distance >= L
-    const double eps = screen_eps(B, ka, P.b);
     const double L = ((double)dK - eps) * (1.0 - 0x1p-40);
     const bool best_ok = b1 < L;
     const double lb2 = fmin(b2, L), lb1 = fmin(b1, L);
