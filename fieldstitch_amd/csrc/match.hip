@@ -312,18 +312,26 @@ __device__ __forceinline__ int predict_cell(const MatchBatch& B, const MPair& P,
 __device__ __forceinline__ bool sp_less(double d, int i, double d2, int i2) { return d < d2 || (d == d2 && i < i2); }
 
 // kc-th nearest spatial key of a query (kc > 0): the candidate set is {in radius and
-// (d2, idx) <= that key}; (r2, INT_MAX) when fewer than kc keypoints are in the radius
+// (d2, idx) <= that key}; (r2, INT_MAX) when fewer than kc keypoints are in the radius.  The KS
+// (>= kc) smallest keys are kept sorted in registers by a branch-free insertion network
+// (static indices only: a list indexed at run time would live in scratch memory).
 constexpr int KMAX = 64;
 
+template <int KS>
 __device__ void knn_threshold(const MatchBatch& B, const MPair& P, const MImage& ib, int cell, double px, double py,
                               double& td, int& ti) {
     td = B.r2;
     ti = 0x7fffffff;
     if (B.kc <= 0 || cell < 0 || P.dense) return;
-    double ld[KMAX];
-    int li[KMAX];
-    int nl = 0;
-    const int kc = B.kc < KMAX ? B.kc : KMAX;
+    double ld[KS];
+    int li[KS];
+#pragma unroll
+    for (int j = 0; j < KS; j++) {
+        ld[j] = INFINITY;
+        li[j] = 0x7fffffff;
+    }
+    int nl = 0;                                   // keypoints in the radius
+    const int kc = B.kc < KS ? B.kc : KS;
     int cx0, cy0;
     cell_xy(ib, cell, cx0, cy0);
     const int* cs = B.cell_start + ib.cell_off + P.b;
@@ -336,24 +344,33 @@ __device__ void knn_threshold(const MatchBatch& B, const MPair& P, const MImage&
                 const double dx = B.s_xy[2 * sb] - px, dy = B.s_xy[2 * sb + 1] - py;
                 const double d = dx * dx + dy * dy;
                 if (!(d < B.r2)) continue;
-                if (nl == kc && !sp_less(d, t, ld[nl - 1], li[nl - 1])) continue;
-                int j = nl < kc ? nl++ : kc - 1;
-                while (j > 0 && sp_less(d, t, ld[j - 1], li[j - 1])) {
-                    ld[j] = ld[j - 1];
-                    li[j] = li[j - 1];
-                    j--;
+                nl++;
+                if (!sp_less(d, t, ld[KS - 1], li[KS - 1])) continue;
+#pragma unroll
+                for (int j = KS - 1; j >= 1; j--) {
+                    const bool lp = sp_less(d, t, ld[j - 1], li[j - 1]);
+                    const bool lc = sp_less(d, t, ld[j], li[j]);
+                    ld[j] = lp ? ld[j - 1] : (lc ? d : ld[j]);
+                    li[j] = lp ? li[j - 1] : (lc ? t : li[j]);
                 }
-                ld[j] = d;
-                li[j] = t;
+                const bool l0 = sp_less(d, t, ld[0], li[0]);
+                ld[0] = l0 ? d : ld[0];
+                li[0] = l0 ? t : li[0];
             }
         }
-    if (nl == kc) {
-        td = ld[kc - 1];
-        ti = li[kc - 1];
+    if (nl >= kc) {
+#pragma unroll
+        for (int j = 0; j < KS; j++)
+            if (j == kc - 1) {
+                td = ld[j];
+                ti = li[j];
+            }
     }
 }
 
-// kc > 0: the kc-th nearest key of every query, one thread per query (before the contraction)
+// kc > 0: the kc-th nearest key of every query, one thread per query (before the contraction);
+// KS: the list length, the smallest of 8, 16, 24, 32, 48, 64 that holds kc
+template <int KS>
 __global__ __launch_bounds__(256) void k_m_knn(const MatchBatch B) {
     const MPair P = B.pair[blockIdx.y];      // records by value: loaded once, never reloaded
     const MImage ia = B.img[P.a];
@@ -365,7 +382,7 @@ __global__ __launch_bounds__(256) void k_m_knn(const MatchBatch B) {
     double px, py, td;
     int ti;
     const int cell = predict_cell(B, P, ia, ib, B.xy[2 * ka], B.xy[2 * ka + 1], px, py);
-    knn_threshold(B, P, ib, cell, px, py, td, ti);
+    knn_threshold<KS>(B, P, ib, cell, px, py, td, ti);
     B.kth[2 * g] = td;
     B.kth[2 * g + 1] = __longlong_as_double((long long)ti);
 }
@@ -1166,8 +1183,16 @@ static int launch_sequence(const long long* info, int form, hipEvent_t* ev, hipS
     if (ev) FSX_HIP(hipEventRecord(ev[0], st));
     hipLaunchKernelGGL(k_m_setup, dim3((unsigned)info[5]), dim3(SU_THREADS), 0, st, b);
     FSX_LAUNCH_CHECK();
-    if (nq > 0 && info[6] > 0)
-        hipLaunchKernelGGL(k_m_knn, dim3((unsigned)((max_na + 255) / 256), n_pair), dim3(256), 0, st, b);
+    if (nq > 0 && info[6] > 0) {
+        const dim3 g((unsigned)((max_na + 255) / 256), n_pair);
+        const long long kc = info[6];
+        if (kc <= 8) hipLaunchKernelGGL(k_m_knn<8>, g, dim3(256), 0, st, b);
+        else if (kc <= 16) hipLaunchKernelGGL(k_m_knn<16>, g, dim3(256), 0, st, b);
+        else if (kc <= 24) hipLaunchKernelGGL(k_m_knn<24>, g, dim3(256), 0, st, b);
+        else if (kc <= 32) hipLaunchKernelGGL(k_m_knn<32>, g, dim3(256), 0, st, b);
+        else if (kc <= 48) hipLaunchKernelGGL(k_m_knn<48>, g, dim3(256), 0, st, b);
+        else hipLaunchKernelGGL(k_m_knn<KMAX>, g, dim3(256), 0, st, b);
+    }
     FSX_LAUNCH_CHECK();
     if (ev) FSX_HIP(hipEventRecord(ev[1], st));
     if (nq > 0) {
