@@ -8,9 +8,10 @@
 // elimination with partial pivoting in float64 and counts matches whose reprojection error is
 // below thr; the hypothesis with the most inliers wins (ties: lowest h), and its inliers are
 // the labels; the winning homography is then refit on them.  The same scheme serves the
-// fundamental matrix (8 points, rank 2) and the similarity (2 points).  Plain float64
-// arithmetic without contraction, in a fixed order, so the CPU restatement reproduces every
-// count and model.
+// fundamental matrix (8 points, rank 2) and the similarity (2 points).  Float64 arithmetic in a
+// fixed order: the solves without contraction, the inlier tests with explicit fused
+// multiply-adds (fma, one rounding each), so the CPU restatement (plain operations and C's fma)
+// reproduces every count and model.
 #include "fsx_common.h"
 #include <math.h>
 
@@ -72,13 +73,13 @@ __device__ bool solve_h4(const double* xa, const double* ya, const double* xb, c
 }
 
 // reprojection error of (x, y) -> (u, v) below sqrt(thr2), without a division:
-// |H p / w - q|^2 < thr2  <=>  |H p - w q|^2 < thr2 w^2 (w != 0)
+// |H p / w - q|^2 < thr2  <=>  |H p - w q|^2 < thr2 w^2 (w != 0); fused multiply-adds in a
+// fixed nesting (the CPU restatement uses C's fma, so the bits agree), no branch
 __device__ __forceinline__ bool is_inlier(const double* h, double x, double y, double u, double v, double thr2) {
-    const double w = (h[6] * x + h[7] * y) + h[8];
-    if (!(fabs(w) > 1e-12)) return false;
-    const double ex = ((h[0] * x + h[1] * y) + h[2]) - u * w;
-    const double ey = ((h[3] * x + h[4] * y) + h[5]) - v * w;
-    return ex * ex + ey * ey < thr2 * (w * w);
+    const double w = fma(h[6], x, fma(h[7], y, h[8]));
+    const double ex = fma(-u, w, fma(h[0], x, fma(h[1], y, h[2])));
+    const double ey = fma(-v, w, fma(h[3], x, fma(h[4], y, h[5])));
+    return (fabs(w) > 1e-12) & (fma(ex, ex, ey * ey) < thr2 * (w * w));
 }
 
 // ---- models: K points per sample, solve(K points) -> 9 parameters, inlier test -------------
@@ -246,15 +247,19 @@ struct ModelF {
             for (int j = 0; j < 3; j++) m[3 * i + j] = (Tb[0][i] * G[0][j] + Tb[1][i] * G[1][j]) + Tb[2][i] * G[2][j];
         return true;
     }
+    // lines l2 = F x_a = (a2, b2, c2) in image b and l1 = F^T x_b = (a1, b1, .) in image a; the
+    // residual d = x_b^T F x_a is the same number for both lines (cv2 evaluates it twice, as two
+    // dot products equal in exact arithmetic), so it is formed once;
+    // max(d^2 / |l2.ab|^2, d^2 / |l1.ab|^2) < thr2 without a division, fused multiply-adds in a
+    // fixed nesting, no branch
     __device__ static bool inlier(const double* F, double x, double y, double u, double v, double thr2) {
-        const double a2 = (F[0] * x + F[1] * y) + F[2], b2 = (F[3] * x + F[4] * y) + F[5],
-                     c2 = (F[6] * x + F[7] * y) + F[8];
-        const double a1 = (F[0] * u + F[3] * v) + F[6], b1 = (F[1] * u + F[4] * v) + F[7],
-                     c1 = (F[2] * u + F[5] * v) + F[8];
-        const double n2 = a2 * a2 + b2 * b2, n1 = a1 * a1 + b1 * b1;
-        // max(d2^2 / n2, d1^2 / n1) < thr2, without a division (n1, n2 > 0), and no branch
-        const double d2 = (u * a2 + v * b2) + c2, d1 = (x * a1 + y * b1) + c1;
-        return (n1 > 1e-30) & (n2 > 1e-30) & (d2 * d2 < thr2 * n2) & (d1 * d1 < thr2 * n1);
+        const double a2 = fma(F[0], x, fma(F[1], y, F[2])), b2 = fma(F[3], x, fma(F[4], y, F[5])),
+                     c2 = fma(F[6], x, fma(F[7], y, F[8]));
+        const double a1 = fma(F[0], u, fma(F[3], v, F[6])), b1 = fma(F[1], u, fma(F[4], v, F[7]));
+        const double n2 = fma(a2, a2, b2 * b2), n1 = fma(a1, a1, b1 * b1);
+        const double d = fma(u, a2, fma(v, b2, c2));
+        const double dd = d * d;
+        return (n1 > 1e-30) & (n2 > 1e-30) & (dd < thr2 * n2) & (dd < thr2 * n1);
     }
 };
 
@@ -281,9 +286,8 @@ struct ModelS {
         return true;
     }
     __device__ static bool inlier(const double* m, double x, double y, double u, double v, double thr2) {
-        const double px = (m[0] * x + m[1] * y) + m[2], py = (m[3] * x + m[4] * y) + m[5];
-        const double dx = px - u, dy = py - v;
-        return dx * dx + dy * dy < thr2;
+        const double dx = fma(m[0], x, fma(m[1], y, m[2])) - u, dy = fma(m[3], x, fma(m[4], y, m[5])) - v;
+        return fma(dx, dx, dy * dy) < thr2;
     }
 };
 
@@ -347,47 +351,36 @@ __device__ __forceinline__ RansacPair ransac_pair(int slot, const long long* __r
 // the block's best (inliers + 1) << 32 | (0xffffffff - hypothesis) over hypotheses
 // h0 + tid, h0 + tid + step, ... < n_hyp: the largest key has the most inliers, ties the lowest
 // hypothesis; 0 when no hypothesis of the block gave a model (256 threads)
-// Hypotheses go in rounds of one per thread; each round streams the matches through LDS in
-// tiles of 256 (one match loaded per thread, then every thread tests its hypothesis against
-// the tile: the same address for all lanes, a broadcast read).
+// Hypotheses go in rounds of one per thread; every lane tests its hypothesis against the
+// matches in order.  A match's index is the same for all lanes, so its four coordinates are
+// read by scalar loads into SGPRs (no LDS staging, no barrier, no vector memory traffic) and
+// used as the scalar operand of the float64 instructions.
 template <class M>
-__device__ unsigned long long ransac_score(const RansacPair& R, const double* A, const double* Bq, int h0, int step,
-                                          int n_hyp) {
+__device__ unsigned long long ransac_score(const RansacPair& R, const double* __restrict__ A,
+                                          const double* __restrict__ Bq, int h0, int step, int n_hyp) {
     __shared__ unsigned long long s_key[256];
-    __shared__ double4 s_m[256];                   // (x, y) in a, (u, v) in b
     const int tid = threadIdx.x;
     unsigned long long key = 0ull;
     if (R.n >= M::K) {                             // block-uniform
         for (int hr = h0; hr < n_hyp; hr += step) {
             const int hy = hr + tid;
             double h[9];
+#pragma unroll
+            for (int k = 0; k < 9; k++) h[k] = 0.0;
             const bool ok = hy < n_hyp && hypothesis<M>(A, Bq, R.seed, R.p, hy, R.n, h);
             int cnt = 0;
-            for (int t0 = 0; t0 < R.n; t0 += 256) {
-                const int i = t0 + tid;
-                __syncthreads();                   // the previous tile has been read
-                if (i < R.n) s_m[tid] = make_double4(A[2 * i], A[2 * i + 1], Bq[2 * i], Bq[2 * i + 1]);
-                __syncthreads();
-                const int m = min(256, R.n - t0);
-                if (ok) {
-                    // four matches per iteration: independent float64 chains in flight
-                    int k = 0;
-                    for (; k + 4 <= m; k += 4) {
-                        const double4 v0 = s_m[k], v1 = s_m[k + 1], v2 = s_m[k + 2], v3 = s_m[k + 3];
-                        const int c0 = M::inlier(h, v0.x, v0.y, v0.z, v0.w, R.thr2);
-                        const int c1 = M::inlier(h, v1.x, v1.y, v1.z, v1.w, R.thr2);
-                        const int c2 = M::inlier(h, v2.x, v2.y, v2.z, v2.w, R.thr2);
-                        const int c3 = M::inlier(h, v3.x, v3.y, v3.z, v3.w, R.thr2);
-                        cnt += (c0 + c1) + (c2 + c3);
-                    }
-                    for (; k < m; k++) {
-                        const double4 v = s_m[k];
-                        cnt += M::inlier(h, v.x, v.y, v.z, v.w, R.thr2);
-                    }
-                }
+            // four matches per iteration: independent float64 chains in flight
+            int k = 0;
+            for (; k + 4 <= R.n; k += 4) {
+                const int c0 = M::inlier(h, A[2 * k], A[2 * k + 1], Bq[2 * k], Bq[2 * k + 1], R.thr2);
+                const int c1 = M::inlier(h, A[2 * k + 2], A[2 * k + 3], Bq[2 * k + 2], Bq[2 * k + 3], R.thr2);
+                const int c2 = M::inlier(h, A[2 * k + 4], A[2 * k + 5], Bq[2 * k + 4], Bq[2 * k + 5], R.thr2);
+                const int c3 = M::inlier(h, A[2 * k + 6], A[2 * k + 7], Bq[2 * k + 6], Bq[2 * k + 7], R.thr2);
+                cnt += (c0 + c1) + (c2 + c3);
             }
-            const unsigned long long k = ((unsigned long long)(cnt + 1) << 32) | (0xffffffffu - (unsigned)hy);
-            if (ok) key = k > key ? k : key;
+            for (; k < R.n; k++) cnt += M::inlier(h, A[2 * k], A[2 * k + 1], Bq[2 * k], Bq[2 * k + 1], R.thr2);
+            const unsigned long long kk = ((unsigned long long)(cnt + 1) << 32) | (0xffffffffu - (unsigned)hy);
+            if (ok) key = kk > key ? kk : key;
         }
     }
     s_key[tid] = key;
@@ -426,24 +419,15 @@ __device__ void ransac_finish(const RansacPair& R, int slot, const double* A, co
         inlier[R.o + i] = (uint8_t)(have && M::inlier(h, A[2 * i], A[2 * i + 1], Bq[2 * i], Bq[2 * i + 1], R.thr2));
 }
 
-// one block per pair: every hypothesis, then the winner
-template <class M>
-__global__ __launch_bounds__(256) void k_ransac(const long long* __restrict__ off, const double* __restrict__ pa,
-                                                const double* __restrict__ pb, int n_hyp, double thr2_all,
-                                                unsigned long long seed_all, const unsigned long long* __restrict__ seeds,
-                                                const double* __restrict__ thrs, uint8_t* __restrict__ inlier,
-                                                double* __restrict__ H_out, int* __restrict__ n_in) {
-    const int slot = blockIdx.x;
-    const RansacPair R = ransac_pair(slot, off, thr2_all, seed_all, seeds, thrs);
-    const double* A = pa + 2 * R.o;
-    const double* Bq = pb + 2 * R.o;
-    const unsigned long long key = ransac_score<M>(R, A, Bq, 0, 256, n_hyp);
-    ransac_finish<M>(R, slot, A, Bq, key, inlier, H_out, n_in);
+// Scoring: block (p, r) scores hypotheses 256 r .. 256 r + 255 of pair p and merges its best
+// key into word 0 of the pair's 9-double model slot (an atomic maximum; k_ransac_keys_zero
+// clears the words first), so a pair with many matches spreads over as many blocks as it has
+// rounds of 256 hypotheses.  k_ransac_pick reads the word back before it writes the model there.
+__global__ void k_ransac_keys_zero(int n_pair, double* __restrict__ H_out) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p < n_pair) ((unsigned long long*)H_out)[9 * (long long)p] = 0ull;
 }
 
-// Small batches: S <= 9 blocks per pair (gridDim.y) each score the hypotheses
-// h = 256 s + tid + 256 S k; block s leaves its best key in word s of the pair's 9-double model
-// slot, which k_ransac_pick reads back before it writes the model there.
 template <class M>
 __global__ __launch_bounds__(256) void k_ransac_split(const long long* __restrict__ off, const double* __restrict__ pa,
                                                       const double* __restrict__ pb, int n_hyp, double thr2_all,
@@ -452,14 +436,14 @@ __global__ __launch_bounds__(256) void k_ransac_split(const long long* __restric
                                                       const double* __restrict__ thrs, double* __restrict__ H_out) {
     const int slot = blockIdx.x;
     const RansacPair R = ransac_pair(slot, off, thr2_all, seed_all, seeds, thrs);
-    const unsigned long long key =
-        ransac_score<M>(R, pa + 2 * R.o, pb + 2 * R.o, 256 * (int)blockIdx.y, 256 * (int)gridDim.y, n_hyp);
-    if (threadIdx.x == 0) ((unsigned long long*)H_out)[9 * slot + blockIdx.y] = key;
+    const int h0 = 256 * (int)blockIdx.y;
+    const unsigned long long key = ransac_score<M>(R, pa + 2 * R.o, pb + 2 * R.o, h0, n_hyp, n_hyp);   // one round
+    if (threadIdx.x == 0 && key) atomicMax((unsigned long long*)H_out + 9 * (long long)slot, key);
 }
 
 template <class M>
 __global__ __launch_bounds__(256) void k_ransac_pick(const long long* __restrict__ off, const double* __restrict__ pa,
-                                                     const double* __restrict__ pb, int n_split, double thr2_all,
+                                                     const double* __restrict__ pb, double thr2_all,
                                                      unsigned long long seed_all,
                                                      const unsigned long long* __restrict__ seeds,
                                                      const double* __restrict__ thrs, uint8_t* __restrict__ inlier,
@@ -467,15 +451,8 @@ __global__ __launch_bounds__(256) void k_ransac_pick(const long long* __restrict
     __shared__ unsigned long long s_best;
     const int slot = blockIdx.x;
     const RansacPair R = ransac_pair(slot, off, thr2_all, seed_all, seeds, thrs);
-    if (threadIdx.x == 0) {
-        unsigned long long key = 0ull;
-        for (int sp = 0; sp < n_split; sp++) {
-            const unsigned long long k = ((const unsigned long long*)H_out)[9 * slot + sp];
-            key = k > key ? k : key;
-        }
-        s_best = key;
-    }
-    __syncthreads();                 // the keys are read before ransac_finish overwrites the slot
+    if (threadIdx.x == 0) s_best = ((const unsigned long long*)H_out)[9 * (long long)slot];
+    __syncthreads();                 // the key is read before ransac_finish overwrites the slot
     ransac_finish<M>(R, slot, pa + 2 * R.o, pb + 2 * R.o, s_best, inlier, H_out, n_in);
 }
 
@@ -719,22 +696,18 @@ static void launch_refit(int n_pair, const long long* off_dev, const double* pa_
                            n_in_dev, M_dev);
 }
 
-// Hypothesis scoring and the winner: one block per pair, or, for batches too small to fill the
-// GPU's 256 CUs twice, S = min(9, ceil(512 / n_pair)) blocks per pair and a pick launch.  Both
-// find the same winner (most inliers, then the lowest hypothesis).
+// Hypothesis scoring and the winner: the keys cleared, one block per pair and round of 256
+// hypotheses, then one block per pair rebuilds the winner (most inliers, then the lowest
+// hypothesis) and labels the matches.
 template <class M>
 static void launch_score(int n_pair, const long long* off_dev, const double* pa_dev, const double* pb_dev, int n_hyp,
                          double thr2, unsigned long long seed, const unsigned long long* seed_dev, const double* thr_dev,
                          uint8_t* inlier_dev, double* M_dev, int* n_in_dev, hipStream_t stream) {
-    const int split = std::min(std::min(9, (512 + n_pair - 1) / n_pair), std::max(1, (n_hyp + 255) / 256));
-    if (split <= 1) {
-        hipLaunchKernelGGL(k_ransac<M>, dim3(n_pair), dim3(256), 0, stream, off_dev, pa_dev, pb_dev, n_hyp, thr2, seed,
-                           seed_dev, thr_dev, inlier_dev, M_dev, n_in_dev);
-        return;
-    }
-    hipLaunchKernelGGL(k_ransac_split<M>, dim3(n_pair, split), dim3(256), 0, stream, off_dev, pa_dev, pb_dev, n_hyp,
+    const int rounds = (n_hyp + 255) / 256;
+    hipLaunchKernelGGL(k_ransac_keys_zero, dim3((unsigned)((n_pair + 255) / 256)), dim3(256), 0, stream, n_pair, M_dev);
+    hipLaunchKernelGGL(k_ransac_split<M>, dim3(n_pair, rounds), dim3(256), 0, stream, off_dev, pa_dev, pb_dev, n_hyp,
                        thr2, seed, seed_dev, thr_dev, M_dev);
-    hipLaunchKernelGGL(k_ransac_pick<M>, dim3(n_pair), dim3(256), 0, stream, off_dev, pa_dev, pb_dev, split, thr2, seed,
+    hipLaunchKernelGGL(k_ransac_pick<M>, dim3(n_pair), dim3(256), 0, stream, off_dev, pa_dev, pb_dev, thr2, seed,
                        seed_dev, thr_dev, inlier_dev, M_dev, n_in_dev);
 }
 

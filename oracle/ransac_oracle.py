@@ -1,12 +1,15 @@
 """CPU restatement of the deterministic RANSAC (test infrastructure only): the same
 counter-hash sampling, model solves (4-point homography DLT with partial pivoting; normalized
 8-point fundamental matrix with full pivoting and a Jacobi rank-2 projection; 2-point
-similarity; float64, no contraction, sums in the device's order), inlier tests and
+similarity; float64, no contraction, sums in the device's order), inlier tests (fused
+multiply-adds where the device fuses them, through the C fma of sift_oracle.c) and
 least-squares refit of the winning homography as fieldstitch_amd/csrc/verify.hip, vectorised
 over the hypotheses, so every count, label and model can be compared exactly."""
 from __future__ import annotations
 
 import numpy as np
+
+from oracle.sift_oracle import fma
 
 M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
 
@@ -81,15 +84,17 @@ def solve_h4(xa, ya, xb, yb):
 
 def inliers(h, a, b, thr2):
     """h (m, 9); a, b (n, 2) -> (m, n) bool: reprojection error below sqrt(thr2), tested as
-    |H p - w q|^2 < thr2 w^2 (no division, as verify.hip)."""
+    |H p - w q|^2 < thr2 w^2 (no division), with verify.hip's fused multiply-adds:
+    w = fma(h6, x, fma(h7, y, h8)), ex = fma(-u, w, fma(h0, x, fma(h1, y, h2))), ey likewise,
+    |e|^2 = fma(ex, ex, ey * ey)."""
     x, y, u, v = a[None, :, 0], a[None, :, 1], b[None, :, 0], b[None, :, 1]
     H = [h[:, k:k + 1] for k in range(9)]
     with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
-        w = (H[6] * x + H[7] * y) + H[8]
+        w = fma(H[6], x, fma(H[7], y, H[8]))
         good = np.abs(w) > 1e-12
-        ex = ((H[0] * x + H[1] * y) + H[2]) - u * w
-        ey = ((H[3] * x + H[4] * y) + H[5]) - v * w
-        return good & (ex * ex + ey * ey < thr2 * (w * w))
+        ex = fma(-u, w, fma(H[0], x, fma(H[1], y, H[2])))
+        ey = fma(-v, w, fma(H[3], x, fma(H[4], y, H[5])))
+        return good & (fma(ex, ex, ey * ey) < thr2 * (w * w))
 
 
 def rank2_project(f):
@@ -215,17 +220,20 @@ def solve_f8(xa, ya, xb, yb):
 
 def inliers_f(F, a, b, thr2):
     """F (m, 9); a, b (n, 2) -> (m, n) bool: max squared point-to-epipolar-line distance < thr2,
-    tested as d^2 < thr2 * |line normal|^2 on both lines (no division, as verify.hip)."""
+    tested as d^2 < thr2 * |line normal|^2 on both lines (no division), with verify.hip's fused
+    multiply-adds: lines (a2, b2, c2) = F x_a and (a1, b1) = the first two of F^T x_b, each
+    entry fma(f_i, x, fma(f_j, y, f_k)); normals n = fma(a, a, b * b); the one residual
+    d = x_b^T F x_a = fma(u, a2, fma(v, b2, c2)) for both lines."""
     x, y, u, v = a[None, :, 0], a[None, :, 1], b[None, :, 0], b[None, :, 1]
     f = [F[:, k:k + 1] for k in range(9)]
     with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
-        a2, b2, c2 = (f[0] * x + f[1] * y) + f[2], (f[3] * x + f[4] * y) + f[5], (f[6] * x + f[7] * y) + f[8]
-        a1, b1, c1 = (f[0] * u + f[3] * v) + f[6], (f[1] * u + f[4] * v) + f[7], (f[2] * u + f[5] * v) + f[8]
-        n2 = a2 * a2 + b2 * b2
-        n1 = a1 * a1 + b1 * b1
-        d2 = (u * a2 + v * b2) + c2
-        d1 = (x * a1 + y * b1) + c1
-        ok = (d2 * d2 < thr2 * n2) & (d1 * d1 < thr2 * n1)
+        a2, b2, c2 = fma(f[0], x, fma(f[1], y, f[2])), fma(f[3], x, fma(f[4], y, f[5])), fma(f[6], x, fma(f[7], y, f[8]))
+        a1, b1 = fma(f[0], u, fma(f[3], v, f[6])), fma(f[1], u, fma(f[4], v, f[7]))
+        n2 = fma(a2, a2, b2 * b2)
+        n1 = fma(a1, a1, b1 * b1)
+        d = fma(u, a2, fma(v, b2, c2))
+        dd = d * d
+        ok = (dd < thr2 * n2) & (dd < thr2 * n1)
     return (n1 > 1e-30) & (n2 > 1e-30) & ok
 
 
@@ -247,12 +255,14 @@ def solve_s2(xa, ya, xb, yb):
 
 
 def inliers_s(M, a, b, thr2):
+    """M (m, 9); a, b (n, 2) -> (m, n) bool: |M x_a - x_b|^2 < thr2, verify.hip's fused form:
+    dx = fma(m0, x, fma(m1, y, m2)) - u, dy likewise, fma(dx, dx, dy * dy)."""
     x, y, u, v = a[None, :, 0], a[None, :, 1], b[None, :, 0], b[None, :, 1]
     m = [M[:, k:k + 1] for k in range(6)]
     with np.errstate(invalid="ignore", over="ignore"):
-        dx = ((m[0] * x + m[1] * y) + m[2]) - u
-        dy = ((m[3] * x + m[4] * y) + m[5]) - v
-    return dx * dx + dy * dy < thr2
+        dx = fma(m[0], x, fma(m[1], y, m[2])) - u
+        dy = fma(m[3], x, fma(m[4], y, m[5])) - v
+        return fma(dx, dx, dy * dy) < thr2
 
 
 def _lane_sums(vals, flags, lanes=64):

@@ -694,3 +694,9 @@ void fso_rootsift_u8(const float *desc, int64_t m, uint8_t *out) {
         }
     }
 }
+
+/* out[i] = fma(a[i], b[i], c[i]): a * b + c with one rounding (C99 fma, correctly rounded), the
+ * fused multiply-add of the device's float64 inlier tests (verify.hip), which numpy lacks */
+void fso_fma(const double *a, const double *b, const double *c, double *out, int64_t n) {
+    for (int64_t i = 0; i < n; i++) out[i] = fma(a[i], b[i], c[i]);
+}

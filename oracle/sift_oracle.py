@@ -39,6 +39,8 @@ def lib():
         if not os.path.exists(LIB_PATH):
             build()
         L = C.CDLL(LIB_PATH)
+        _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+        L.fso_fma.argtypes = [_f64p, _f64p, _f64p, _f64p, C.c_int64]
         L.fso_gauss_kernel.argtypes = [C.c_double, _f32p]
         L.fso_gauss_kernel.restype = C.c_int
         L.fso_sigmas.argtypes = [np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")]
@@ -76,6 +78,17 @@ def lib():
         L.fso_fused_blur_check.restype = C.c_int64
         _lib = L
     return _lib
+
+
+def fma(a, b, c) -> np.ndarray:
+    """a * b + c with a single rounding (C99 fma), elementwise with numpy broadcasting."""
+    a, b, c = np.broadcast_arrays(np.asarray(a, np.float64), np.asarray(b, np.float64), np.asarray(c, np.float64))
+    shape = a.shape
+    a, b, c = (np.ascontiguousarray(t).reshape(-1) for t in (a, b, c))
+    out = np.empty(a.size, np.float64)
+    if a.size:
+        lib().fso_fma(a, b, c, out, a.size)
+    return out.reshape(shape)
 
 
 def gauss_kernel(sigma: float) -> np.ndarray:

@@ -203,3 +203,56 @@ def test_homography_refit_on_inliers():
     few = np.zeros(len(a), bool)
     few[np.nonzero(flags)[0][:4]] = True
     assert ro.refit_h(a, b, few, win) is win
+
+
+def test_fma_is_correctly_rounded():
+    """The oracle's fused multiply-add (C99 fma) equals a * b + c computed exactly in rationals
+    and rounded once; the unfused expression differs on the cases built to show it."""
+    from fractions import Fraction
+    from oracle.sift_oracle import fma
+    rng = np.random.default_rng(31)
+    a = rng.normal(size=2000) * 10.0 ** rng.integers(-6, 7, 2000)
+    b = rng.normal(size=2000) * 10.0 ** rng.integers(-6, 7, 2000)
+    c = -(a * b) * (1.0 + rng.normal(size=2000) * 1e-12)     # heavy cancellation
+    e = 2.0 ** -52
+    a = np.r_[a, 1.0 + e, 3.0, 0.1]
+    b = np.r_[b, 1.0 - e, 1.0 / 3.0, 10.0]
+    c = np.r_[c, -1.0, -1.0, -1.0]
+    got = fma(a, b, c)
+    want = np.array([float(Fraction(x) * Fraction(y) + Fraction(z)) for x, y, z in zip(a, b, c)])
+    np.testing.assert_array_equal(got, want)
+    assert got[-3] == -e * e and (a[-3] * b[-3] + c[-3]) == 0.0
+    assert np.count_nonzero(got != a * b + c) > 1000
+    # broadcasting as numpy's operators
+    assert fma(np.ones((3, 1)), np.arange(4.0)[None, :], 1.0).shape == (3, 4)
+
+
+def test_inlier_tests_equal_the_geometric_errors():
+    """Each model's fused inlier test agrees with its geometric error computed the long way
+    (homography: |H x_a / w - x_b|^2; fundamental: the larger squared distance of x_b to the
+    line F x_a and of x_a to the line F^T x_b; similarity: |M x_a - x_b|^2) against thr^2,
+    wherever the error is not within 1e-9 relative of the threshold."""
+    rng = np.random.default_rng(32)
+    a = rng.uniform(0, 2640, (3000, 2))
+    H = np.array([[1.01, -0.04, 550.0], [0.03, 0.98, -60.0], [2e-6, -1e-6, 1.0]])
+    w = a @ H[2, :2] + H[2, 2]
+    b = np.c_[(a @ H[0, :2] + H[0, 2]) / w, (a @ H[1, :2] + H[1, 2]) / w] + rng.normal(0, 3.0, a.shape)
+    ha, hb = np.c_[a, np.ones(len(a))], np.c_[b, np.ones(len(b))]
+    p = ha @ H.T
+    err_h = np.sum((p[:, :2] / p[:, 2:] - b) ** 2, 1)
+    F = rng.normal(size=(3, 3))
+    l2, l1 = ha @ F.T, hb @ F
+    err_f = np.maximum(np.sum(hb * l2, 1) ** 2 / (l2[:, 0] ** 2 + l2[:, 1] ** 2),
+                       np.sum(ha * l1, 1) ** 2 / (l1[:, 0] ** 2 + l1[:, 1] ** 2))
+    M = np.array([1.02, -0.05, 30.0, 0.05, 1.02, -12.0, 0.0, 0.0, 1.0])
+    err_s = np.sum((a @ M.reshape(3, 3)[:2, :2].T + M.reshape(3, 3)[:2, 2] - b) ** 2, 1)
+    for name, model, err in (("homography", H.ravel(), err_h), ("fundamental", F.ravel(), err_f),
+                             ("similarity", M, err_s)):
+        thr2 = float(np.median(err))
+        got = ro.MODELS[name][2](model[None], a, b, thr2)[0]
+        clear = np.abs(err - thr2) > 1e-9 * thr2
+        assert 0.3 < got.mean() < 0.7, name
+        np.testing.assert_array_equal(got[clear], (err < thr2)[clear], err_msg=name)
+    # a transposed F is a different test
+    got_t = ro.inliers_f(F.T.ravel()[None], a, b, float(np.median(err_f)))[0]
+    assert (got_t != (err_f < np.median(err_f))).mean() > 0.1
