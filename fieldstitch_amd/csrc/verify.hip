@@ -138,10 +138,13 @@ __device__ void rank2_project(double* f) {
                 S[q][k] = sn * spk + c * sqk;
             }
         }
-    int kmin = 0;
-    for (int k = 1; k < 3; k++)
-        if (S[k][k] < S[kmin][kmin]) kmin = k;
-    const double v0 = V[0][kmin], v1 = V[1][kmin], v2 = V[2][kmin];
+    // kmin: the smallest diagonal entry, ties to the lower index (selects, no indexed access)
+    const bool k1 = S[1][1] < S[0][0];
+    const double s01 = k1 ? S[1][1] : S[0][0];
+    const bool k2 = S[2][2] < s01;
+    const double v0 = k2 ? V[0][2] : (k1 ? V[0][1] : V[0][0]);
+    const double v1 = k2 ? V[1][2] : (k1 ? V[1][1] : V[1][0]);
+    const double v2 = k2 ? V[2][2] : (k1 ? V[2][1] : V[2][0]);
     for (int i = 0; i < 3; i++) {
         const double w = (f[3 * i] * v0 + f[3 * i + 1] * v1) + f[3 * i + 2] * v2;
         f[3 * i] = f[3 * i] - w * v0;
@@ -250,16 +253,16 @@ struct ModelF {
     // lines l2 = F x_a = (a2, b2, c2) in image b and l1 = F^T x_b = (a1, b1, .) in image a; the
     // residual d = x_b^T F x_a is the same number for both lines (cv2 evaluates it twice, as two
     // dot products equal in exact arithmetic), so it is formed once;
-    // max(d^2 / |l2.ab|^2, d^2 / |l1.ab|^2) < thr2 without a division, fused multiply-adds in a
-    // fixed nesting, no branch
+    // max(d^2 / n2, d^2 / n1) < thr2  <=>  d^2 < thr2 min(n1, n2) (n = |line normal|^2, a zero
+    // normal never passes, as cv2's infinite error), fused multiply-adds in a fixed nesting, no
+    // branch.  fmin returns the other operand for one NaN, as numpy's fmin.
     __device__ static bool inlier(const double* F, double x, double y, double u, double v, double thr2) {
         const double a2 = fma(F[0], x, fma(F[1], y, F[2])), b2 = fma(F[3], x, fma(F[4], y, F[5])),
                      c2 = fma(F[6], x, fma(F[7], y, F[8]));
         const double a1 = fma(F[0], u, fma(F[3], v, F[6])), b1 = fma(F[1], u, fma(F[4], v, F[7]));
         const double n2 = fma(a2, a2, b2 * b2), n1 = fma(a1, a1, b1 * b1);
         const double d = fma(u, a2, fma(v, b2, c2));
-        const double dd = d * d;
-        return (n1 > 1e-30) & (n2 > 1e-30) & (dd < thr2 * n2) & (dd < thr2 * n1);
+        return d * d < thr2 * fmin(n1, n2);
     }
 };
 
@@ -368,17 +371,18 @@ __device__ unsigned long long ransac_score(const RansacPair& R, const double* __
 #pragma unroll
             for (int k = 0; k < 9; k++) h[k] = 0.0;
             const bool ok = hy < n_hyp && hypothesis<M>(A, Bq, R.seed, R.p, hy, R.n, h);
-            int cnt = 0;
-            // four matches per iteration: independent float64 chains in flight
+            // four matches per iteration: independent float64 chains in flight, one counter each
+            // (an add with carry-in per test)
+            int c0 = 0, c1 = 0, c2 = 0, c3 = 0;
             int k = 0;
             for (; k + 4 <= R.n; k += 4) {
-                const int c0 = M::inlier(h, A[2 * k], A[2 * k + 1], Bq[2 * k], Bq[2 * k + 1], R.thr2);
-                const int c1 = M::inlier(h, A[2 * k + 2], A[2 * k + 3], Bq[2 * k + 2], Bq[2 * k + 3], R.thr2);
-                const int c2 = M::inlier(h, A[2 * k + 4], A[2 * k + 5], Bq[2 * k + 4], Bq[2 * k + 5], R.thr2);
-                const int c3 = M::inlier(h, A[2 * k + 6], A[2 * k + 7], Bq[2 * k + 6], Bq[2 * k + 7], R.thr2);
-                cnt += (c0 + c1) + (c2 + c3);
+                c0 += M::inlier(h, A[2 * k], A[2 * k + 1], Bq[2 * k], Bq[2 * k + 1], R.thr2);
+                c1 += M::inlier(h, A[2 * k + 2], A[2 * k + 3], Bq[2 * k + 2], Bq[2 * k + 3], R.thr2);
+                c2 += M::inlier(h, A[2 * k + 4], A[2 * k + 5], Bq[2 * k + 4], Bq[2 * k + 5], R.thr2);
+                c3 += M::inlier(h, A[2 * k + 6], A[2 * k + 7], Bq[2 * k + 6], Bq[2 * k + 7], R.thr2);
             }
-            for (; k < R.n; k++) cnt += M::inlier(h, A[2 * k], A[2 * k + 1], Bq[2 * k], Bq[2 * k + 1], R.thr2);
+            for (; k < R.n; k++) c0 += M::inlier(h, A[2 * k], A[2 * k + 1], Bq[2 * k], Bq[2 * k + 1], R.thr2);
+            const int cnt = (c0 + c1) + (c2 + c3);
             const unsigned long long kk = ((unsigned long long)(cnt + 1) << 32) | (0xffffffffu - (unsigned)hy);
             if (ok) key = kk > key ? kk : key;
         }

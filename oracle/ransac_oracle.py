@@ -223,7 +223,7 @@ def inliers_f(F, a, b, thr2):
     tested as d^2 < thr2 * |line normal|^2 on both lines (no division), with verify.hip's fused
     multiply-adds: lines (a2, b2, c2) = F x_a and (a1, b1) = the first two of F^T x_b, each
     entry fma(f_i, x, fma(f_j, y, f_k)); normals n = fma(a, a, b * b); the one residual
-    d = x_b^T F x_a = fma(u, a2, fma(v, b2, c2)) for both lines."""
+    d = x_b^T F x_a = fma(u, a2, fma(v, b2, c2)) for both lines: d^2 < thr2 * fmin(n1, n2)."""
     x, y, u, v = a[None, :, 0], a[None, :, 1], b[None, :, 0], b[None, :, 1]
     f = [F[:, k:k + 1] for k in range(9)]
     with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
@@ -232,9 +232,7 @@ def inliers_f(F, a, b, thr2):
         n2 = fma(a2, a2, b2 * b2)
         n1 = fma(a1, a1, b1 * b1)
         d = fma(u, a2, fma(v, b2, c2))
-        dd = d * d
-        ok = (dd < thr2 * n2) & (dd < thr2 * n1)
-    return (n1 > 1e-30) & (n2 > 1e-30) & ok
+        return d * d < thr2 * np.fmin(n1, n2)
 
 
 def solve_s2(xa, ya, xb, yb):

@@ -256,3 +256,13 @@ def test_inlier_tests_equal_the_geometric_errors():
     # a transposed F is a different test
     got_t = ro.inliers_f(F.T.ravel()[None], a, b, float(np.median(err_f)))[0]
     assert (got_t != (err_f < np.median(err_f))).mean() > 0.1
+
+
+def test_fundamental_zero_normal_never_passes():
+    """A point whose epipolar line has a zero normal (here F = 0, and F with a zero first two
+    columns for every point) is no inlier, as under cv2's infinite error."""
+    a = np.random.default_rng(33).uniform(0, 100, (50, 2))
+    assert not ro.inliers_f(np.zeros((1, 9)), a, a, 1e30).any()
+    F = np.zeros((1, 9))
+    F[0, [2, 5, 8]] = 1.0                      # l2 = (1, 1, 1) for every point, l1 = (0, 0, .)
+    assert not ro.inliers_f(F, a, a, 1e30).any()
