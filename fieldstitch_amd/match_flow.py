@@ -90,19 +90,34 @@ class FlowConfig:
         self.f_iters = int(ctx.get("ransac_iters", F_ITERS))
         self.ratio = float(ctx.get("ratio", 0.8))
         self._cams = None
+        self._halves = None
         if all(k in ctx for k in ("K", "R", "C", "zg")):
             self._cams = (ctx["K"], ctx["R"], ctx["C"], ctx["zg"])
 
     def prior_on(self, i, j):
         return bool(self.has_rot[i] and self.has_rot[j] and self.use_prior and self._cams is not None)
 
+    def _cam_rows(self):
+        """Per camera: the 22 float64 of its source half of the prior record (K^-1, R, C, z_g)
+        and the 21 of its target half (R, K, C), computed once."""
+        if self._halves is None:
+            K, R, Cc, zg = self._cams
+            n = len(K)
+            src = np.zeros((n, 22), np.float64)
+            dst = np.zeros((n, 21), np.float64)
+            for c in range(n):
+                Kc = np.asarray(K[c], np.float64)
+                src[c] = np.concatenate([np.linalg.inv(Kc).ravel(), np.asarray(R[c], np.float64).ravel(),
+                                         np.asarray(Cc[c], np.float64).ravel(), [float(zg[c])]])
+                dst[c] = np.concatenate([np.asarray(R[c], np.float64).ravel(), Kc.ravel(),
+                                         np.asarray(Cc[c], np.float64).ravel()])
+            self._halves = (src, dst)
+        return self._halves
+
     def cams(self, i, j):
         """The 43 float64 of the ground-plane prior from camera i into camera j."""
-        K, R, Cc, zg = self._cams
-        return np.concatenate([np.linalg.inv(np.asarray(K[i], np.float64)).ravel(), np.asarray(R[i], np.float64).ravel(),
-                               np.asarray(Cc[i], np.float64).ravel(), [float(zg[i])],
-                               np.asarray(R[j], np.float64).ravel(), np.asarray(K[j], np.float64).ravel(),
-                               np.asarray(Cc[j], np.float64).ravel()])
+        src, dst = self._cam_rows()
+        return np.concatenate([src[i], dst[j]])
 
     def side(self, i):
         return max(self.W[i], self.H[i])
@@ -180,6 +195,18 @@ def _concat_rows(rows):
     return off, np.ascontiguousarray(data, np.float64).reshape(-1, 2)
 
 
+class _Rows:
+    """Features q of the photo whose rows start at base in a Positions array."""
+    __slots__ = ("base", "q")
+
+    def __init__(self, base, q):
+        self.base = base
+        self.q = q
+
+    def __len__(self):
+        return len(self.q)
+
+
 class Positions:
     """All photos' undistorted positions in one device float64 array: photo i's rows start at
     base[i], so a pair's matched rows are gathered by index (fsx_gather_xy) instead of being
@@ -202,18 +229,24 @@ class Positions:
         self.xy = torch.cat(parts).contiguous() if o else torch.zeros((1, 2), dtype=torch.float64, device="cuda")
 
     def rows(self, i, q):
-        """Global row numbers of photo i's features q."""
-        return self.base[i] + np.asarray(q, np.int64)
+        """Photo i's features q as a row reference for gather (global row numbers base[i] + q;
+        the addition is done once for a whole list, in gather)."""
+        return _Rows(self.base[i], q)
 
     def gather(self, idx_list):
-        """[(m_k,) row numbers] -> (int64 offsets, device float64 (sum m_k, 2) rows)."""
+        """[row references of rows(), or (m_k,) global row numbers] -> (int64 offsets, device
+        float64 (sum m_k, 2) rows)."""
         torch = _torch()
-        lens = np.array([len(x) for x in idx_list], np.int64)
-        off = np.r_[0, np.cumsum(lens)].astype(np.int64)
+        lens = np.fromiter((len(x.q) if isinstance(x, _Rows) else len(x) for x in idx_list), np.int64,
+                           count=len(idx_list))
+        off = np.zeros(len(idx_list) + 1, np.int64)
+        np.cumsum(lens, out=off[1:])
         tot = int(off[-1])
         out = torch.empty((max(tot, 1), 2), dtype=torch.float64, device="cuda")
         if tot:
-            idx = torch.from_numpy(np.concatenate(idx_list).astype(np.int32)).to("cuda")
+            base = np.fromiter((x.base if isinstance(x, _Rows) else 0 for x in idx_list), np.int64, count=len(idx_list))
+            rows = np.concatenate([x.q if isinstance(x, _Rows) else x for x in idx_list]).astype(np.int64, copy=False)
+            idx = torch.from_numpy((rows + np.repeat(base, lens)).astype(np.int32)).to("cuda")
             _lib.call("fsx_gather_xy", tot, C.c_void_p(idx.data_ptr()), C.c_void_p(self.xy.data_ptr()), self.n,
                       C.c_void_p(out.data_ptr()), C.c_void_p(_lib.stream_handle()))
         return off, out
@@ -231,10 +264,14 @@ def _prior_keep(cfg, P, items):
         return [np.zeros(0, bool) for _ in items]
     cams = np.zeros((len(items), 43), np.float64)
     tol = np.full(len(items), -1.0)
-    for k, (i, j, _, _) in enumerate(items):
-        if cfg.prior_on(i, j):
-            cams[k] = cfg.cams(i, j)
-            tol[k] = cfg.prior_tol * cfg.side(j)
+    on = np.fromiter((cfg.prior_on(it[0], it[1]) for it in items), bool, count=len(items))
+    if on.any():
+        src, dst = cfg._cam_rows()
+        ii = np.fromiter((it[0] for it in items), np.int64, count=len(items))[on]
+        jj = np.fromiter((it[1] for it in items), np.int64, count=len(items))[on]
+        cams[on, :22] = src[ii]
+        cams[on, 22:] = dst[jj]
+        tol[on] = cfg.prior_tol * np.maximum(np.asarray(cfg.W, np.float64)[jj], np.asarray(cfg.H, np.float64)[jj])
     dev = "cuda"
     d_off = torch.from_numpy(off).to(dev)
     d_cams = torch.from_numpy(cams).to(dev)
@@ -246,6 +283,15 @@ def _prior_keep(cfg, P, items):
               Pv(_lib.stream_handle()))
     k = keep.cpu().numpy().astype(bool)
     return [k[off[n]:off[n + 1]] for n in range(len(items))]
+
+
+class _Masks(list):
+    """Per-pair inlier masks (views of one flat bool array; .flat and .off keep the whole)."""
+
+    def __init__(self, flat, off):
+        super().__init__(flat[off[k]:off[k + 1]] for k in range(len(off) - 1))
+        self.flat = flat[:int(off[-1])]
+        self.off = off
 
 
 def _ransac_dev(model, off, d_a, d_b, thr, seeds, n_hyp):
@@ -266,9 +312,8 @@ def _ransac_dev(model, off, d_a, d_b, thr, seeds, n_hyp):
     _lib.call("fsx_ransac_batch", _MODEL[model], n, P(d_off.data_ptr()), P(d_a.data_ptr()), P(d_b.data_ptr()),
               int(n_hyp), P(d_thr.data_ptr()), P(d_seed.data_ptr()), P(inl.data_ptr()), P(M.data_ptr()),
               P(cnt.data_ptr()), P(_lib.stream_handle()))
-    flags = inl.cpu().numpy().astype(bool)
-    return ([flags[off[k]:off[k + 1]] for k in range(n)], M.cpu().numpy(), cnt.cpu().numpy().astype(np.int64),
-            (d_off, inl))
+    flags = inl.cpu().numpy().view(bool)
+    return (_Masks(flags, off), M.cpu().numpy(), cnt.cpu().numpy().astype(np.int64), (d_off, inl))
 
 
 def ransac_batch(model, pa_list, pb_list, thr, seeds, n_hyp):
@@ -378,11 +423,11 @@ def _chain(tasks, images, cfg, form, filt, stats):
 
     # 3. homography RANSAC on the coarse set
     hk = [k for k in coarse if len(cm[k][0]) >= 12]
-    hflags, hM, _ = ransac_rows("homography", P, [P.rows(tasks[k][0], cm[k][0]) for k in hk],
+    _, hM, hcnt = ransac_rows("homography", P, [P.rows(tasks[k][0], cm[k][0]) for k in hk],
                                 [P.rows(tasks[k][1], cm[k][1]) for k in hk],
                                 [0.012 * cfg.side(tasks[k][0]) for k in hk],
                                 [pair_seed(*tasks[k], 0x1) for k in hk], H_ITERS)
-    gk = [k for k, f in zip(hk, hflags) if f.sum() >= 10]
+    gk = [k for k, c in zip(hk, hcnt.tolist()) if c >= 10]          # the winner's count = its flags' sum
     Hm = {k: hM[n].reshape(3, 3) for n, k in enumerate(hk)}
     marks.append(("homography", time.perf_counter()))
 
@@ -423,9 +468,13 @@ def _chain(tasks, images, cfg, form, filt, stats):
     # 7. similarity of every verified set
     rk = sorted(result)
     sims = _similarities(cfg, tasks, P, [(k, result[k]) for k in rk])
-    for k, sim in zip(rk, sims):
-        q, t, n_in = result[k]
-        out[k] = (tasks[k][0], tasks[k][1], np.asarray(q).astype(np.int32), np.asarray(t).astype(np.int32), sim, n_in)
+    if rk:                       # int32 (q, t) of every pair: one narrowing, per-pair views
+        lens = [len(result[k][0]) for k in rk]
+        q32 = np.concatenate([np.asarray(result[k][0]) for k in rk]).astype(np.int32)
+        t32 = np.concatenate([np.asarray(result[k][1]) for k in rk]).astype(np.int32)
+        ends = np.cumsum(lens).tolist()
+        for k, sim, e, n in zip(rk, sims, ends, lens):
+            out[k] = (tasks[k][0], tasks[k][1], q32[e - n:e], t32[e - n:e], sim, result[k][2])
     marks.append(("similarity", time.perf_counter()))
     if stats is not None:
         stats.update({"pairs": n_task, "coarse": len(coarse), "homography": len(hk), "guided": len(gk),
@@ -445,10 +494,16 @@ def _verify(cfg, tasks, P, items):
                                 [P.rows(tasks[items[n][0]][1], items[n][1][1]) for n in ok],
                                 [cfg.ransac_thr] * len(ok), [pair_seed(*tasks[items[n][0]], 0xF) for n in ok],
                                 cfg.f_iters)
-    for n, f, c in zip(ok, flags, cnt):
+    if not ok:
+        return res
+    # the inliers of all pairs by one mask over the concatenated matches, split per pair
+    qf = np.concatenate([items[n][1][0] for n in ok])[flags.flat]
+    tf = np.concatenate([items[n][1][1] for n in ok])[flags.flat]
+    n_f = np.add.reduceat(flags.flat.astype(np.int64), flags.off[:-1]).tolist()    # every pair has >= 8 rows
+    ends = np.cumsum(n_f).tolist()
+    for n, c, e, m in zip(ok, cnt.tolist(), ends, n_f):
         if c > 0:
-            q, t = items[n][1]
-            res[n] = (q[f], t[f], int(f.sum()))
+            res[n] = (qf[e - m:e], tf[e - m:e], m)
     return res
 
 
@@ -462,9 +517,10 @@ def _similarities(cfg, tasks, P, items):
                                   [P.rows(tasks[items[n][0]][1], items[n][1][1]) for n in ok],
                                   [0.02 * cfg.side(tasks[items[n][0]][0]) for n in ok],
                                   [pair_seed(*tasks[items[n][0]], 0x5) for n in ok], S_ITERS, fit=True)
-    for n, c, fit in zip(ok, cnt, fits):
-        if c >= 2 and np.isfinite(fit).all():
-            res[n] = (float(fit[0]), float(fit[1]), float(fit[2]), float(fit[3]))
+    good = ((cnt >= 2) & np.isfinite(fits).all(1)).tolist() if ok else []
+    for n, g, fit in zip(ok, good, fits.tolist()):
+        if g:
+            res[n] = tuple(fit)
     return res
 
 

@@ -347,12 +347,15 @@ class PairMatcher:
                   t.ctypes.data_as(C.POINTER(C.c_int)), flags, C.c_void_p(st))
         if flags[2]:
             raise RuntimeError("matcher chunk table overflow (flags %d)" % flags[2])
-        out = []
-        for p in range(self.n_pair):
-            o = self._offsets[p]
-            n = counts[p]
-            out.append((q[o:o + n].astype(np.int64), t[o:o + n].astype(np.int64)))
-        return out
+        if self.n_pair == 0:
+            return []
+        # the pairs' segments packed and widened once, then handed out as views
+        cnt = list(counts)[:self.n_pair]
+        offs = list(self._offsets)[:self.n_pair]
+        qc = np.concatenate([q[o:o + n] for o, n in zip(offs, cnt)]).astype(np.int64)
+        tc = np.concatenate([t[o:o + n] for o, n in zip(offs, cnt)]).astype(np.int64)
+        ends = np.cumsum(cnt).tolist()
+        return [(qc[e - n:e], tc[e - n:e]) for e, n in zip(ends, cnt)]
 
 
 def _query_state(m: "PairMatcher", stream=None):
