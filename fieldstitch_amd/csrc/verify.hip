@@ -28,6 +28,41 @@ __device__ __forceinline__ unsigned long long draw(unsigned long long seed, int 
     return splitmix64(seed ^ splitmix64(((unsigned long long)p << 40) ^ ((unsigned long long)h << 8) ^ (unsigned)j));
 }
 
+// Step C of the partial-pivot elimination of the 8 x 9 DLT system: the pivot is the largest
+// |A[r][C]| of rows C.. (the first of equal ones), swapped to row C; the rows below are reduced.
+// Compile-time indices only (the swap is a select), so the system stays in registers; only
+// columns >= C are swapped, the others are not read again.  false: pivot below 1e-12.
+template <int C>
+__device__ __forceinline__ bool h4_step(double (&A)[8][9]) {
+    int piv = C;
+    double best = fabs(A[C][C]);
+#pragma unroll
+    for (int r = C + 1; r < 8; r++) {
+        const double a = fabs(A[r][C]);
+        const bool gt = a > best;
+        best = gt ? a : best;
+        piv = gt ? r : piv;
+    }
+    if (best < 1e-12) return false;
+#pragma unroll
+    for (int r = C + 1; r < 8; r++) {
+        const bool sw = piv == r;
+#pragma unroll
+        for (int k = C; k < 9; k++) {
+            const double t = A[C][k];
+            A[C][k] = sw ? A[r][k] : t;
+            A[r][k] = sw ? t : A[r][k];
+        }
+    }
+#pragma unroll
+    for (int r = C + 1; r < 8; r++) {
+        const double f = A[r][C] / A[C][C];
+#pragma unroll
+        for (int k = C; k < 9; k++) A[r][k] = A[r][k] - f * A[C][k];
+    }
+    return true;
+}
+
 // solve the 4-point DLT; returns false when a pivot is below 1e-12 (degenerate sample)
 __device__ bool solve_h4(const double* xa, const double* ya, const double* xb, const double* yb, double* h) {
     double A[8][9];
@@ -41,30 +76,13 @@ __device__ bool solve_h4(const double* xa, const double* ya, const double* xb, c
         r1[0] = 0.0; r1[1] = 0.0; r1[2] = 0.0; r1[3] = x; r1[4] = y; r1[5] = 1.0;
         r1[6] = -(v * x); r1[7] = -(v * y); r1[8] = v;
     }
-    for (int c = 0; c < 8; c++) {
-        int piv = c;
-        double best = fabs(A[c][c]);
-        for (int r = c + 1; r < 8; r++) {
-            const double a = fabs(A[r][c]);
-            if (a > best) {
-                best = a;
-                piv = r;
-            }
-        }
-        if (best < 1e-12) return false;
-        if (piv != c)
-            for (int k = 0; k < 9; k++) {
-                const double t = A[c][k];
-                A[c][k] = A[piv][k];
-                A[piv][k] = t;
-            }
-        for (int r = c + 1; r < 8; r++) {
-            const double f = A[r][c] / A[c][c];
-            for (int k = c; k < 9; k++) A[r][k] = A[r][k] - f * A[c][k];
-        }
-    }
+    if (!h4_step<0>(A) || !h4_step<1>(A) || !h4_step<2>(A) || !h4_step<3>(A) || !h4_step<4>(A) || !h4_step<5>(A) ||
+        !h4_step<6>(A) || !h4_step<7>(A))
+        return false;
+#pragma unroll
     for (int c = 7; c >= 0; c--) {
         double s = A[c][8];
+#pragma unroll
         for (int k = c + 1; k < 8; k++) s = s - A[c][k] * h[k];
         h[c] = s / A[c][c];
     }
@@ -153,6 +171,58 @@ __device__ void rank2_project(double* f) {
     }
 }
 
+// Step C of the full-pivot elimination of the 8 x 9 system: the pivot is the largest |entry|
+// of rows C.. and columns C.. (first in row-major order), swapped to (C, C); the rows below are
+// reduced.  Every index is a compile-time constant (the swaps and the column permutation are
+// selects), so the system stays in registers instead of scratch memory.  Only the entries read
+// later are swapped: columns >= C of the pivot rows.  false: pivot below 1e-12 (degenerate).
+template <int C>
+__device__ __forceinline__ bool f8_step(double (&A)[8][9], int (&perm)[9]) {
+    int pr = C, pc = C;
+    double best = -1.0;
+#pragma unroll
+    for (int r = C; r < 8; r++)
+#pragma unroll
+        for (int k = C; k < 9; k++) {
+            const double a = fabs(A[r][k]);
+            const bool gt = a > best;
+            best = gt ? a : best;
+            pr = gt ? r : pr;
+            pc = gt ? k : pc;
+        }
+    if (best < 1e-12) return false;
+#pragma unroll
+    for (int r = C + 1; r < 8; r++) {
+        const bool sw = pr == r;
+#pragma unroll
+        for (int k = C; k < 9; k++) {
+            const double t = A[C][k];
+            A[C][k] = sw ? A[r][k] : t;
+            A[r][k] = sw ? t : A[r][k];
+        }
+    }
+#pragma unroll
+    for (int k = C + 1; k < 9; k++) {
+        const bool sw = pc == k;
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            const double t = A[r][C];
+            A[r][C] = sw ? A[r][k] : t;
+            A[r][k] = sw ? t : A[r][k];
+        }
+        const int t = perm[C];
+        perm[C] = sw ? perm[k] : t;
+        perm[k] = sw ? t : perm[k];
+    }
+#pragma unroll
+    for (int r = C + 1; r < 8; r++) {
+        const double f = A[r][C] / A[C][C];
+#pragma unroll
+        for (int k = C; k < 9; k++) A[r][k] = A[r][k] - f * A[C][k];
+    }
+    return true;
+}
+
 // Fundamental matrix (the role of cv2.findFundamentalMat in _verify, matching.py:175-191):
 // normalized 8-point algorithm.  Each image's 8 points are shifted to their centroid and scaled
 // to mean distance sqrt(2); the 8 x 9 system [u x, u y, u, v x, v y, v, x, y, 1] f = 0 is
@@ -194,50 +264,28 @@ struct ModelF {
             r[6] = x; r[7] = y; r[8] = 1.0;
         }
         int perm[9];
+#pragma unroll
         for (int k = 0; k < 9; k++) perm[k] = k;
-        for (int c = 0; c < 8; c++) {
-            int pr = c, pc = c;
-            double best = -1.0;
-            for (int r = c; r < 8; r++)
-                for (int k = c; k < 9; k++) {
-                    const double a = fabs(A[r][k]);
-                    if (a > best) {
-                        best = a;
-                        pr = r;
-                        pc = k;
-                    }
-                }
-            if (best < 1e-12) return false;
-            if (pr != c)
-                for (int k = 0; k < 9; k++) {
-                    const double t = A[c][k];
-                    A[c][k] = A[pr][k];
-                    A[pr][k] = t;
-                }
-            if (pc != c) {
-                for (int r = 0; r < 8; r++) {
-                    const double t = A[r][c];
-                    A[r][c] = A[r][pc];
-                    A[r][pc] = t;
-                }
-                const int t = perm[c];
-                perm[c] = perm[pc];
-                perm[pc] = t;
-            }
-            for (int r = c + 1; r < 8; r++) {
-                const double f = A[r][c] / A[c][c];
-                for (int k = c; k < 9; k++) A[r][k] = A[r][k] - f * A[c][k];
-            }
-        }
+        if (!f8_step<0>(A, perm) || !f8_step<1>(A, perm) || !f8_step<2>(A, perm) || !f8_step<3>(A, perm) ||
+            !f8_step<4>(A, perm) || !f8_step<5>(A, perm) || !f8_step<6>(A, perm) || !f8_step<7>(A, perm))
+            return false;
         double fz[9];   // solution in pivot order; the free (last) one is 1
         fz[8] = 1.0;
+#pragma unroll
         for (int c = 7; c >= 0; c--) {
             double s = 0.0;
+#pragma unroll
             for (int k = c + 1; k < 9; k++) s = s + A[c][k] * fz[k];
             fz[c] = -s / A[c][c];
         }
-        double fh[9];
-        for (int k = 0; k < 9; k++) fh[perm[k]] = fz[k];
+        double fh[9];   // fh[perm[k]] = fz[k]
+#pragma unroll
+        for (int j = 0; j < 9; j++) {
+            double v = 0.0;
+#pragma unroll
+            for (int k = 0; k < 9; k++) v = perm[k] == j ? fz[k] : v;
+            fh[j] = v;
+        }
         rank2_project(fh);
         // F = Tb^T F^ Ta with T = [[s, 0, -s cx], [0, s, -s cy], [0, 0, 1]]
         const double Ta[3][3] = {{sa, 0.0, -sa * ca[0]}, {0.0, sa, -sa * ca[1]}, {0.0, 0.0, 1.0}};
@@ -351,42 +399,33 @@ __device__ __forceinline__ RansacPair ransac_pair(int slot, const long long* __r
     return r;
 }
 
-// the block's best (inliers + 1) << 32 | (0xffffffff - hypothesis) over hypotheses
-// h0 + tid, h0 + tid + step, ... < n_hyp: the largest key has the most inliers, ties the lowest
-// hypothesis; 0 when no hypothesis of the block gave a model (256 threads)
-// Hypotheses go in rounds of one per thread; every lane tests its hypothesis against the
-// matches in order.  A match's index is the same for all lanes, so its four coordinates are
-// read by scalar loads into SGPRs (no LDS staging, no barrier, no vector memory traffic) and
-// used as the scalar operand of the float64 instructions.
+// Scoring.  Key of hypothesis hy with model h: (inliers + 1) << 32 | (0xffffffff - hy), so the
+// largest key has the most inliers, ties the lowest hypothesis; 0 when it gave no model.  Every
+// lane tests its own hypothesis against the matches in order.  A match's index is the same for
+// all lanes, so its four coordinates are read by scalar loads into SGPRs (no LDS staging, no
+// barrier, no vector memory traffic) and used as the scalar operand of the float64
+// instructions; four matches per iteration, one counter each (an add with carry-in per test).
 template <class M>
-__device__ unsigned long long ransac_score(const RansacPair& R, const double* __restrict__ A,
-                                          const double* __restrict__ Bq, int h0, int step, int n_hyp) {
+__device__ __forceinline__ unsigned long long score_key(const RansacPair& R, const double* __restrict__ A,
+                                                        const double* __restrict__ Bq, const double (&h)[9], bool ok,
+                                                        int hy) {
+    int c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+    int k = 0;
+    for (; k + 4 <= R.n; k += 4) {
+        c0 += M::inlier(h, A[2 * k], A[2 * k + 1], Bq[2 * k], Bq[2 * k + 1], R.thr2);
+        c1 += M::inlier(h, A[2 * k + 2], A[2 * k + 3], Bq[2 * k + 2], Bq[2 * k + 3], R.thr2);
+        c2 += M::inlier(h, A[2 * k + 4], A[2 * k + 5], Bq[2 * k + 4], Bq[2 * k + 5], R.thr2);
+        c3 += M::inlier(h, A[2 * k + 6], A[2 * k + 7], Bq[2 * k + 6], Bq[2 * k + 7], R.thr2);
+    }
+    for (; k < R.n; k++) c0 += M::inlier(h, A[2 * k], A[2 * k + 1], Bq[2 * k], Bq[2 * k + 1], R.thr2);
+    const int cnt = (c0 + c1) + (c2 + c3);
+    return ok ? ((unsigned long long)(cnt + 1) << 32) | (0xffffffffu - (unsigned)hy) : 0ull;
+}
+
+// the largest key of the block (256 threads)
+__device__ unsigned long long block_max_key(unsigned long long key) {
     __shared__ unsigned long long s_key[256];
     const int tid = threadIdx.x;
-    unsigned long long key = 0ull;
-    if (R.n >= M::K) {                             // block-uniform
-        for (int hr = h0; hr < n_hyp; hr += step) {
-            const int hy = hr + tid;
-            double h[9];
-#pragma unroll
-            for (int k = 0; k < 9; k++) h[k] = 0.0;
-            const bool ok = hy < n_hyp && hypothesis<M>(A, Bq, R.seed, R.p, hy, R.n, h);
-            // four matches per iteration: independent float64 chains in flight, one counter each
-            // (an add with carry-in per test)
-            int c0 = 0, c1 = 0, c2 = 0, c3 = 0;
-            int k = 0;
-            for (; k + 4 <= R.n; k += 4) {
-                c0 += M::inlier(h, A[2 * k], A[2 * k + 1], Bq[2 * k], Bq[2 * k + 1], R.thr2);
-                c1 += M::inlier(h, A[2 * k + 2], A[2 * k + 3], Bq[2 * k + 2], Bq[2 * k + 3], R.thr2);
-                c2 += M::inlier(h, A[2 * k + 4], A[2 * k + 5], Bq[2 * k + 4], Bq[2 * k + 5], R.thr2);
-                c3 += M::inlier(h, A[2 * k + 6], A[2 * k + 7], Bq[2 * k + 6], Bq[2 * k + 7], R.thr2);
-            }
-            for (; k < R.n; k++) c0 += M::inlier(h, A[2 * k], A[2 * k + 1], Bq[2 * k], Bq[2 * k + 1], R.thr2);
-            const int cnt = (c0 + c1) + (c2 + c3);
-            const unsigned long long kk = ((unsigned long long)(cnt + 1) << 32) | (0xffffffffu - (unsigned)hy);
-            if (ok) key = kk > key ? kk : key;
-        }
-    }
     s_key[tid] = key;
     __syncthreads();
     for (int w = 128; w >= 1; w >>= 1) {
@@ -440,8 +479,62 @@ __global__ __launch_bounds__(256) void k_ransac_split(const long long* __restric
                                                       const double* __restrict__ thrs, double* __restrict__ H_out) {
     const int slot = blockIdx.x;
     const RansacPair R = ransac_pair(slot, off, thr2_all, seed_all, seeds, thrs);
-    const int h0 = 256 * (int)blockIdx.y;
-    const unsigned long long key = ransac_score<M>(R, pa + 2 * R.o, pb + 2 * R.o, h0, n_hyp, n_hyp);   // one round
+    const double* A = pa + 2 * R.o;
+    const double* Bq = pb + 2 * R.o;
+    const int hy = 256 * (int)blockIdx.y + (int)threadIdx.x;
+    double h[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) h[k] = 0.0;
+    const bool ok = hy < n_hyp && R.n >= M::K && hypothesis<M>(A, Bq, R.seed, R.p, hy, R.n, h);
+    const unsigned long long key = block_max_key(score_key<M>(R, A, Bq, h, ok, hy));
+    if (threadIdx.x == 0 && key) atomicMax((unsigned long long*)H_out + 9 * (long long)slot, key);
+}
+
+// The same scoring in two launches, for the models whose solve needs many registers (the
+// 4-point homography, the 8-point F: a fused kernel runs at the solve's occupancy).
+// k_ransac_solve: block (x, r) solves hypotheses 256 r .. 256 r + 255 of pair p0 + x into the
+// plane layout hyp[q * plane + x * n_hyp + h] (q = 0..8; valid[x * n_hyp + h] = 1 when the
+// sample gave a model); k_ransac_score's block (x, r) scores the same hypotheses.
+template <class M>
+__global__ __launch_bounds__(256) void k_ransac_solve(const long long* __restrict__ off, const double* __restrict__ pa,
+                                                      const double* __restrict__ pb, int n_hyp,
+                                                      unsigned long long seed_all,
+                                                      const unsigned long long* __restrict__ seeds, int p0,
+                                                      long long plane, double* __restrict__ hyp,
+                                                      uint8_t* __restrict__ valid) {
+    const int hy = 256 * (int)blockIdx.y + (int)threadIdx.x;
+    if (hy >= n_hyp) return;
+    const RansacPair R = ransac_pair(p0 + (int)blockIdx.x, off, 0.0, seed_all, seeds, nullptr);
+    double h[9];
+    const bool ok = R.n >= M::K && hypothesis<M>(pa + 2 * R.o, pb + 2 * R.o, R.seed, R.p, hy, R.n, h);
+    const long long i = (long long)blockIdx.x * n_hyp + hy;
+#pragma unroll
+    for (int q = 0; q < 9; q++) hyp[q * plane + i] = ok ? h[q] : 0.0;
+    valid[i] = ok ? 1 : 0;
+}
+
+template <class M>
+__global__ __launch_bounds__(256) void k_ransac_score(const long long* __restrict__ off, const double* __restrict__ pa,
+                                                      const double* __restrict__ pb, int n_hyp, double thr2_all,
+                                                      unsigned long long seed_all,
+                                                      const unsigned long long* __restrict__ seeds,
+                                                      const double* __restrict__ thrs, int p0, long long plane,
+                                                      const double* __restrict__ hyp,
+                                                      const uint8_t* __restrict__ valid, double* __restrict__ H_out) {
+    const int slot = p0 + (int)blockIdx.x;
+    const RansacPair R = ransac_pair(slot, off, thr2_all, seed_all, seeds, thrs);
+    const int hy = 256 * (int)blockIdx.y + (int)threadIdx.x;
+    const long long i = (long long)blockIdx.x * n_hyp + hy;
+    double h[9];
+    bool ok = false;
+#pragma unroll
+    for (int q = 0; q < 9; q++) h[q] = 0.0;
+    if (hy < n_hyp) {
+        ok = valid[i] != 0;
+#pragma unroll
+        for (int q = 0; q < 9; q++) h[q] = hyp[q * plane + i];
+    }
+    const unsigned long long key = block_max_key(score_key<M>(R, pa + 2 * R.o, pb + 2 * R.o, h, ok, hy));
     if (threadIdx.x == 0 && key) atomicMax((unsigned long long*)H_out + 9 * (long long)slot, key);
 }
 
@@ -702,17 +795,40 @@ static void launch_refit(int n_pair, const long long* off_dev, const double* pa_
 
 // Hypothesis scoring and the winner: the keys cleared, one block per pair and round of 256
 // hypotheses, then one block per pair rebuilds the winner (most inliers, then the lowest
-// hypothesis) and labels the matches.
+// hypothesis) and labels the matches.  The homography and F solve into a stream-ordered
+// temporary first (pairs in chunks of at most RANSAC_HYP_BYTES of hypotheses), the similarity
+// solves inside its scoring kernel.
+constexpr long long RANSAC_HYP_BYTES = 256ll << 20;
+
 template <class M>
-static void launch_score(int n_pair, const long long* off_dev, const double* pa_dev, const double* pb_dev, int n_hyp,
-                         double thr2, unsigned long long seed, const unsigned long long* seed_dev, const double* thr_dev,
-                         uint8_t* inlier_dev, double* M_dev, int* n_in_dev, hipStream_t stream) {
+static int launch_score(int n_pair, const long long* off_dev, const double* pa_dev, const double* pb_dev, int n_hyp,
+                        double thr2, unsigned long long seed, const unsigned long long* seed_dev, const double* thr_dev,
+                        uint8_t* inlier_dev, double* M_dev, int* n_in_dev, hipStream_t stream) {
     const int rounds = (n_hyp + 255) / 256;
     hipLaunchKernelGGL(k_ransac_keys_zero, dim3((unsigned)((n_pair + 255) / 256)), dim3(256), 0, stream, n_pair, M_dev);
-    hipLaunchKernelGGL(k_ransac_split<M>, dim3(n_pair, rounds), dim3(256), 0, stream, off_dev, pa_dev, pb_dev, n_hyp,
-                       thr2, seed, seed_dev, thr_dev, M_dev);
+    if constexpr (M::K <= 2) {
+        hipLaunchKernelGGL(k_ransac_split<M>, dim3(n_pair, rounds), dim3(256), 0, stream, off_dev, pa_dev, pb_dev,
+                           n_hyp, thr2, seed, seed_dev, thr_dev, M_dev);
+    } else {
+        const long long per_pair = (long long)n_hyp * (9 * sizeof(double) + 1);
+        const int chunk = (int)std::max(1ll, std::min((long long)n_pair, RANSAC_HYP_BYTES / per_pair));
+        const long long plane = (long long)chunk * n_hyp;
+        void* tmp = nullptr;
+        FSX_HIP(hipMallocAsync(&tmp, (size_t)(plane * (9 * sizeof(double) + 1)), stream));
+        double* hyp = (double*)tmp;
+        uint8_t* valid = (uint8_t*)(hyp + 9 * plane);
+        for (int p0 = 0; p0 < n_pair; p0 += chunk) {
+            const int cp = std::min(chunk, n_pair - p0);
+            hipLaunchKernelGGL(k_ransac_solve<M>, dim3(cp, rounds), dim3(256), 0, stream, off_dev, pa_dev, pb_dev,
+                               n_hyp, seed, seed_dev, p0, plane, hyp, valid);
+            hipLaunchKernelGGL(k_ransac_score<M>, dim3(cp, rounds), dim3(256), 0, stream, off_dev, pa_dev, pb_dev,
+                               n_hyp, thr2, seed, seed_dev, thr_dev, p0, plane, hyp, valid, M_dev);
+        }
+        FSX_HIP(hipFreeAsync(tmp, stream));
+    }
     hipLaunchKernelGGL(k_ransac_pick<M>, dim3(n_pair), dim3(256), 0, stream, off_dev, pa_dev, pb_dev, thr2, seed,
                        seed_dev, thr_dev, inlier_dev, M_dev, n_in_dev);
+    return FSX_OK;
 }
 
 template <class M>
@@ -722,8 +838,9 @@ static int launch_ransac(int n_pair, const long long* off_dev, const double* pa_
     FSX_REQUIRE(n_pair >= 0 && n_hyp >= 1 && n_hyp <= (1 << 20) && thr > 0, "bad RANSAC arguments");
     if (n_pair == 0) return FSX_OK;
     FSX_REQUIRE(off_dev && pa_dev && pb_dev && inlier_dev && H_dev && n_in_dev, "null pointer argument");
-    launch_score<M>(n_pair, off_dev, pa_dev, pb_dev, n_hyp, thr * thr, seed, nullptr, nullptr, inlier_dev, H_dev,
-                    n_in_dev, (hipStream_t)stream);
+    const int rc = launch_score<M>(n_pair, off_dev, pa_dev, pb_dev, n_hyp, thr * thr, seed, nullptr, nullptr,
+                                   inlier_dev, H_dev, n_in_dev, (hipStream_t)stream);
+    if (rc != FSX_OK) return rc;
     launch_refit<M>(n_pair, off_dev, pa_dev, pb_dev, inlier_dev, n_in_dev, H_dev, (hipStream_t)stream);
     FSX_LAUNCH_CHECK();
     return FSX_OK;
@@ -737,8 +854,9 @@ static int launch_ransac_per_pair(int n_pair, const long long* off_dev, const do
     if (n_pair == 0) return FSX_OK;
     FSX_REQUIRE(off_dev && pa_dev && pb_dev && thr_dev && seed_dev && inlier_dev && M_dev && n_in_dev,
                 "null pointer argument");
-    launch_score<M>(n_pair, off_dev, pa_dev, pb_dev, n_hyp, 0.0, 0ull, seed_dev, thr_dev, inlier_dev, M_dev, n_in_dev,
-                    (hipStream_t)stream);
+    const int rc = launch_score<M>(n_pair, off_dev, pa_dev, pb_dev, n_hyp, 0.0, 0ull, seed_dev, thr_dev, inlier_dev,
+                                   M_dev, n_in_dev, (hipStream_t)stream);
+    if (rc != FSX_OK) return rc;
     launch_refit<M>(n_pair, off_dev, pa_dev, pb_dev, inlier_dev, n_in_dev, M_dev, (hipStream_t)stream);
     FSX_LAUNCH_CHECK();
     return FSX_OK;
