@@ -330,7 +330,10 @@ int fsx_mlp_infer(const void *X, int n, const void *packed, const float *b1, con
  * from splitmix64 counter draws, 4-point DLT in float64, inlier iff reprojection error < thr.
  * Writes inlier_dev (byte per match) and n_in_dev from the winning hypothesis, and H_dev (9
  * float64 per pair, zeros without a model): the winner refit by normalized least squares on its
- * inliers when it has at least 5 (as cv2.findHomography refines its RANSAC model), h33 = 1. */
+ * inliers when it has at least 5 (as cv2.findHomography refines its RANSAC model), h33 = 1.
+ * The homography and fundamental-matrix forms (here and in fsx_ransac_batch) take a
+ * stream-ordered temporary for their hypotheses (hipMallocAsync / hipFreeAsync on `stream`, at
+ * most 256 MB, or 73 n_hyp bytes when one pair needs more); the similarity takes none. */
 int fsx_ransac_homography(int n_pair, const long long *off_dev, const double *pa_dev, const double *pb_dev, int n_hyp,
                           double thr, unsigned long long seed, uint8_t *inlier_dev, double *H_dev, int *n_in_dev,
                           void *stream);
