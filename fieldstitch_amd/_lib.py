@@ -100,6 +100,7 @@ SIGNATURES = {
     "fsx_match_accepted": [_vp, C.POINTER(C.c_longlong), _vp, _vp, _vp],
     "fsx_match_filter_rows": [_vp, C.POINTER(C.c_longlong), _vp, _vp, _vp, _vp, C.c_longlong, _vp, _vp],
     "fsx_match_publish": [_vp, C.POINTER(C.c_longlong), _vp, _vp, C.c_longlong, _vp, C.c_float, _vp],
+    "fsx_match_pack": [_vp, C.POINTER(C.c_longlong), _vp, C.c_longlong, _vp, _vp, _vp],
     "fsx_ransac_batch": [_i, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp],
     "fsx_gather_xy": [C.c_longlong, _vp, _vp, C.c_longlong, _vp, _vp],
     "fsx_similarity_fit_batch": [_i, _vp, _vp, _vp, _vp, _vp, _vp],

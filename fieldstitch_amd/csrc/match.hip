@@ -981,6 +981,22 @@ __global__ __launch_bounds__(CP_THREADS) void k_m_compact(const MatchBatch B) {
     if (tid == 0) B.out_n[blockIdx.x] = carry;
 }
 
+// The last run's per-pair lists packed on the device: pair p's list (out_n[p] entries at its
+// output slot o_off) to rows poff[p] .. of q_out / t_out; a list that would pass row `cap` is
+// not written.
+__global__ __launch_bounds__(256) void k_m_pack(const MatchBatch B, const long long* __restrict__ poff, long long cap,
+                                               int* __restrict__ q_out, int* __restrict__ t_out) {
+    const int p = blockIdx.x;
+    const long long o = B.pair[p].o_off;
+    const int n = B.out_n[p];
+    const long long d = poff[p];
+    if (d < 0 || n < 0 || d + n > cap) return;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        q_out[d + i] = B.out_q[o + i];
+        t_out[d + i] = B.out_t[o + i];
+    }
+}
+
 // ------------------------------------------------------------------------------------------
 // Host side
 
@@ -1379,6 +1395,25 @@ int fsx_match_results(int n_img, const int* n_kp, const int* W, const int* H, in
     if (t) FSX_HIP(hipMemcpyAsync(t, w + M.off_outt, 4 * (size_t)M.n_out, hipMemcpyDeviceToHost, st));
     if (flags) FSX_HIP(hipMemcpyAsync(flags, w + M.off_cnt, 16, hipMemcpyDeviceToHost, st));
     FSX_HIP(hipStreamSynchronize(st));
+    return FSX_OK;
+}
+
+// The last batch's (q, t) lists packed on the device (asynchronous): pair p's counts[p]
+// entries (the counts of fsx_match_results) to rows poff_dev[p] .. of q_dev / t_dev (int32,
+// cap rows each); poff_dev: n_pair int64 row offsets, normally the exclusive prefix sums of the
+// counts.  Only the matches cross to the host afterwards, not the per-query output slots.
+int fsx_match_pack(const void* ws, const long long* info, const long long* poff_dev, long long cap, int* q_dev,
+                   int* t_dev, void* stream) {
+    FSX_REQUIRE(ws && info && poff_dev && q_dev && t_dev, "null pointer argument");
+    FSX_REQUIRE(info[0] == 0x6673786d && info[7] == (long long)(intptr_t)ws, "launch_info does not match ws");
+    FSX_REQUIRE(cap >= 0, "negative capacity");
+    MatchBatch b;
+    memcpy(&b, info + 8, sizeof(b));
+    const int n_pair = (int)info[2];
+    if (n_pair <= 0) return FSX_OK;
+    hipLaunchKernelGGL(k_m_pack, dim3((unsigned)n_pair), dim3(256), 0, (hipStream_t)stream, b, poff_dev, cap, q_dev,
+                       t_dev);
+    FSX_LAUNCH_CHECK();
     return FSX_OK;
 }
 

@@ -334,28 +334,47 @@ class PairMatcher:
         self._graphs[form] = g
         return g
 
-    def results(self, stream=None):
-        """List of (q int64, t int64) per pair, sorted by t (synchronises)."""
-        st = _lib.stream_handle(stream)
+    def results_dev(self, stream=None):
+        """The last run's lists packed on the device: (counts (n_pair,) int64, row offsets
+        (n_pair + 1,) int64, q, t int32 device tensors) with pair p's (q, t) in rows off[p] ..
+        off[p + 1], sorted by t.  Synchronises once, for the counts; only the matches are
+        moved (fsx_match_pack), not the per-query output slots."""
+        import torch
+        ts = torch.cuda.current_stream() if stream is None else stream
+        st = _lib.stream_handle(ts)
         npair = max(self.n_pair, 1)
         counts = (C.c_int * npair)()
-        q = np.zeros(max(self.capacity, 1), np.int32)
-        t = np.zeros(max(self.capacity, 1), np.int32)
         flags = (C.c_int * 4)()
         _lib.call("fsx_match_results", self.n_img, self._nk, self._W, self._H, self.n_pair, self._pa, self._pb,
-                  self.radius, C.c_void_p(self.ws.data_ptr()), counts, q.ctypes.data_as(C.POINTER(C.c_int)),
-                  t.ctypes.data_as(C.POINTER(C.c_int)), flags, C.c_void_p(st))
+                  self.radius, C.c_void_p(self.ws.data_ptr()), counts, None, None, flags, C.c_void_p(st))
         if flags[2]:
             raise RuntimeError("matcher chunk table overflow (flags %d)" % flags[2])
+        cnt = np.ctypeslib.as_array(counts)[:self.n_pair].astype(np.int64)
+        off = np.zeros(self.n_pair + 1, np.int64)
+        np.cumsum(cnt, out=off[1:])
+        tot = int(off[-1])
+        with torch.cuda.stream(ts):
+            q = torch.empty(max(tot, 1), dtype=torch.int32, device=self.ws.device)
+            t = torch.empty(max(tot, 1), dtype=torch.int32, device=self.ws.device)
+            if tot:
+                self._pack_off = torch.from_numpy(off[:-1].copy()).to(self.ws.device)
+                _lib.call("fsx_match_pack", C.c_void_p(self.ws.data_ptr()), self._info,
+                          C.c_void_p(self._pack_off.data_ptr()), tot, C.c_void_p(q.data_ptr()),
+                          C.c_void_p(t.data_ptr()), C.c_void_p(st))
+        return cnt, off, q[:tot], t[:tot]
+
+    def results(self, stream=None):
+        """List of (q int64, t int64) per pair, sorted by t (synchronises): the lists packed on
+        the device, copied to the host once and handed out as views."""
+        import torch
         if self.n_pair == 0:
             return []
-        # the pairs' segments packed and widened once, then handed out as views
-        cnt = list(counts)[:self.n_pair]
-        offs = list(self._offsets)[:self.n_pair]
-        qc = np.concatenate([q[o:o + n] for o, n in zip(offs, cnt)]).astype(np.int64)
-        tc = np.concatenate([t[o:o + n] for o, n in zip(offs, cnt)]).astype(np.int64)
-        ends = np.cumsum(cnt).tolist()
-        return [(qc[e - n:e], tc[e - n:e]) for e, n in zip(ends, cnt)]
+        _, off, q, t = self.results_dev(stream)
+        with torch.cuda.stream(torch.cuda.current_stream() if stream is None else stream):
+            qh = q.cpu().numpy().astype(np.int64)
+            th = t.cpu().numpy().astype(np.int64)
+        o = off.tolist()
+        return [(qh[o[p]:o[p + 1]], th[o[p]:o[p + 1]]) for p in range(self.n_pair)]
 
 
 def _query_state(m: "PairMatcher", stream=None):

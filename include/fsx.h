@@ -178,6 +178,15 @@ int fsx_match_results(int n_img, const int *n_kp, const int *W, const int *H, in
                       const int *pb, double radius, const void *ws_dev, int *counts, int *q, int *t, int *flags,
                       void *stream);
 
+/* The last batch's (q, t) lists packed on the device (asynchronous; no counterpart in the
+ * reference, whose per-pair lists are host arrays from the start, matching.py:249): pair p's
+ * counts[p] entries (as fsx_match_results reports them) to rows poff_dev[p] .. of q_dev / t_dev
+ * (int32, cap rows each; a list that would pass cap is not written); poff_dev: n_pair int64,
+ * normally the exclusive prefix sums of the counts.  launch_info: as filled by
+ * fsx_match_prepare for ws_dev. */
+int fsx_match_pack(const void *ws_dev, const long long *launch_info, const long long *poff_dev, long long cap,
+                   int *q_dev, int *t_dev, void *stream);
+
 /* fsx_match_batch for callers that cannot allocate (code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc).  No counterpart in the reference.  Same
  * match set and arguments as fsx_match_batch, but the call allocates nothing (no device,
