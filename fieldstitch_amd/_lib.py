@@ -103,6 +103,8 @@ SIGNATURES = {
     "fsx_match_pack": [_vp, C.POINTER(C.c_longlong), _vp, C.c_longlong, _vp, _vp, _vp],
     "fsx_ransac_batch": [_i, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp],
     "fsx_gather_xy": [C.c_longlong, _vp, _vp, C.c_longlong, _vp, _vp],
+    "fsx_gather_lists": [_i, _vp, _vp, _vp, C.c_longlong, _vp, _vp, _vp, _vp, _vp, _vp, C.c_longlong, _vp, _vp, _vp,
+                         _vp, _vp, _vp],
     "fsx_similarity_fit_batch": [_i, _vp, _vp, _vp, _vp, _vp, _vp],
     "fsx_prior_keep_batch": [_i, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp],
     "fsx_match_query_state": [_i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), _i, C.POINTER(_i), C.POINTER(_i),

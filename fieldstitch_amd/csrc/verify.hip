@@ -710,6 +710,61 @@ __global__ __launch_bounds__(256) void k_gather_xy(long long n, const int* __res
     ((double2*)out)[k] = v;
 }
 
+// Lists of matches moved between the chain's stages on the device (one 256-thread block per
+// list): list L's rows src_off[L] .. + cnt[L] of q_src / t_src (those with keep[row] != 0 when
+// keep is given, in order) go to rows dst_off[L] .. of q_dst / t_dst, and, when pa_dst / pb_dst
+// are given, the positions xy[base_a[L] + q] / xy[base_b[L] + t] to the same rows (NaN for a row
+// outside [0, n_xy)); rows at or past dst_cap are not written; kept[L] = rows taken.
+__global__ __launch_bounds__(256) void k_gather_lists(const long long* __restrict__ src_off,
+                                                      const int* __restrict__ cnt,
+                                                      const long long* __restrict__ dst_off, long long dst_cap,
+                                                      const long long* __restrict__ base_a,
+                                                      const long long* __restrict__ base_b,
+                                                      const int* __restrict__ q_src, const int* __restrict__ t_src,
+                                                      const uint8_t* __restrict__ keep,
+                                                      const double* __restrict__ xy, long long n_xy,
+                                                      int* __restrict__ q_dst, int* __restrict__ t_dst,
+                                                      double* __restrict__ pa_dst, double* __restrict__ pb_dst,
+                                                      int* __restrict__ kept) {
+    __shared__ int s_w[4];
+    const int L = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const long long so = src_off[L], d0 = dst_off[L];
+    const int n = cnt[L];
+    const long long ba = base_a ? base_a[L] : 0, bb = base_b ? base_b[L] : 0;
+    const double2 nan2 = make_double2(__builtin_nan(""), __builtin_nan(""));
+    int carry = 0;
+    for (int b0 = 0; b0 < n; b0 += 256) {
+        const int i = b0 + tid;
+        const bool in = i < n && (!keep || keep[so + i] != 0);
+        const unsigned long long bal = __ballot(in);
+        if (lane == 0) s_w[wv] = __popcll(bal);
+        __syncthreads();
+        int pre = 0, tot = 0;
+#pragma unroll
+        for (int w2 = 0; w2 < 4; w2++) {
+            pre += w2 < wv ? s_w[w2] : 0;
+            tot += s_w[w2];
+        }
+        __syncthreads();                 // s_w is read before the next round writes it
+        const long long d = d0 + carry + pre + __popcll(bal & ((1ull << lane) - 1ull));
+        if (in && d >= 0 && d < dst_cap) {
+            const int q = q_src[so + i], t = t_src[so + i];
+            if (q_dst) q_dst[d] = q;
+            if (t_dst) t_dst[d] = t;
+            if (pa_dst) {
+                const long long r = ba + q;
+                ((double2*)pa_dst)[d] = r >= 0 && r < n_xy ? ((const double2*)xy)[r] : nan2;
+            }
+            if (pb_dst) {
+                const long long r = bb + t;
+                ((double2*)pb_dst)[d] = r >= 0 && r < n_xy ? ((const double2*)xy)[r] : nan2;
+            }
+        }
+        carry += tot;
+    }
+    if (kept && tid == 0) kept[L] = carry;
+}
+
 // Least-squares similarity b ~ s R(theta) a + t of each pair's inlier rows (one 256-thread block
 // per pair): centroids, then over the centred rows den = sum |a|^2, c = sum a.b / den,
 // s = sum a x b / den, t = mb - [[c, -s], [s, c]] ma; out[4p..4p+3] = (atan2(s, c), hypot(c, s),
@@ -924,6 +979,32 @@ int fsx_gather_xy(long long n, const int* idx_dev, const double* xy_dev, long lo
     FSX_REQUIRE(((uintptr_t)xy_dev & 15) == 0 && ((uintptr_t)out_dev & 15) == 0, "xy / out must be 16-byte aligned");
     hipLaunchKernelGGL(k_gather_xy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, n, idx_dev,
                        xy_dev, n_xy, out_dev);
+    FSX_LAUNCH_CHECK();
+    return FSX_OK;
+}
+
+// The chain's lists of matches moved on the device between its stages: list L's cnt_dev[L]
+// rows from src_off_dev[L] of q_src_dev / t_src_dev (int32 feature indices; with keep_dev, only
+// rows whose byte is non-zero, in order) to rows dst_off_dev[L] .. of q_dst_dev / t_dst_dev, and
+// optionally their positions xy_dev[base_a_dev[L] + q] / xy_dev[base_b_dev[L] + t] (float64
+// (x, y), NaN outside [0, n_xy)) to pa_dst_dev / pb_dst_dev; rows at or past dst_cap are
+// skipped; kept_dev[L] (optional) = rows taken.  Any of q_dst, t_dst, pa_dst, pb_dst may be
+// null.  xy and the position outputs 16-byte aligned.
+int fsx_gather_lists(int n_list, const long long* src_off_dev, const int* cnt_dev, const long long* dst_off_dev,
+                     long long dst_cap, const long long* base_a_dev, const long long* base_b_dev,
+                     const int* q_src_dev, const int* t_src_dev, const uint8_t* keep_dev, const double* xy_dev,
+                     long long n_xy, int* q_dst_dev, int* t_dst_dev, double* pa_dst_dev, double* pb_dst_dev,
+                     int* kept_dev, void* stream) {
+    FSX_REQUIRE(n_list >= 0 && dst_cap >= 0 && n_xy >= 0, "bad list arguments");
+    if (n_list == 0) return FSX_OK;
+    FSX_REQUIRE(src_off_dev && cnt_dev && dst_off_dev && q_src_dev && t_src_dev, "null pointer argument");
+    FSX_REQUIRE((!pa_dst_dev || base_a_dev) && (!pb_dst_dev || base_b_dev) && (!(pa_dst_dev || pb_dst_dev) || xy_dev),
+                "positions need xy_dev and their row bases");
+    FSX_REQUIRE(((uintptr_t)xy_dev & 15) == 0 && ((uintptr_t)pa_dst_dev & 15) == 0 && ((uintptr_t)pb_dst_dev & 15) == 0,
+                "xy / positions must be 16-byte aligned");
+    hipLaunchKernelGGL(k_gather_lists, dim3((unsigned)n_list), dim3(256), 0, (hipStream_t)stream, src_off_dev, cnt_dev,
+                       dst_off_dev, dst_cap, base_a_dev, base_b_dev, q_src_dev, t_src_dev, keep_dev, xy_dev, n_xy,
+                       q_dst_dev, t_dst_dev, pa_dst_dev, pb_dst_dev, kept_dev);
     FSX_LAUNCH_CHECK();
     return FSX_OK;
 }

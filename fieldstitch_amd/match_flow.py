@@ -21,6 +21,11 @@ for all the pairs that reach it in one batch:
   7. similarity  RANSAC of a 2-point similarity at 0.02 max(W_i, H_i), then its least-squares
                  fit on the inliers (matching.py:194-202).
 
+The match lists stay on the device from one stage to the next (packed per pair; the
+matcher's output packed by fsx_match_pack, lists chosen, cut to the prior's survivors or to
+RANSAC inliers, and gathered with their positions by fsx_gather_lists): the host handles only
+per-pair counts, models and fits, and copies the verified lists once at the end.
+
 Results are the reference's tuples: (i, j, q int32, t int32, sim, n_inliers), or
 (i, j, None, None, None, n) when matching fails.  A stage that raises sends its pairs through
 the chain one by one, so a failing pair reports ("error", message) in the sim slot
@@ -130,64 +135,6 @@ def _torch():
     return torch
 
 
-def _dense_matches(images, pairs, ratio, prefix=None, form="mfma"):
-    """Exhaustive ratio matching of every (i, j) in `pairs` (the first `prefix` features of
-    each photo when given): one PairMatcher batch.  Returns [(q, t)]."""
-    from .matching import PairMatcher
-    if not pairs:
-        return []
-    torch = _torch()
-    used = sorted({i for p in pairs for i in p})
-    slot = {i: k for k, i in enumerate(used)}
-    zeros = {}
-    sets = []
-    for i in used:
-        d = images[i].desc if prefix is None else images[i].desc[:prefix]
-        n = int(d.shape[0])
-        if n not in zeros:
-            zeros[n] = torch.zeros((n, 2), dtype=torch.float64, device=d.device)
-        sets.append((d, zeros[n], (1, 1)))
-    pm = PairMatcher(sets, [(slot[i], slot[j]) for i, j in pairs], None, radius=1.0, ratio=ratio)
-    pm.run(form)
-    out = pm.results()
-    # the reference returns nothing when either side has fewer than 2 features (matching.py:88)
-    for k, (i, j) in enumerate(pairs):
-        if min(sets[slot[i]][0].shape[0], sets[slot[j]][0].shape[0]) < 2:
-            out[k] = (np.zeros(0, np.int64), np.zeros(0, np.int64))
-    return out
-
-
-def _guided_matches(images, pairs, Hms, radii, kc, ratio, form="mfma", filt=None):
-    """Guided matching of each pair under its homography, window radius radii[k]: one
-    PairMatcher batch per distinct radius.  filt = (net, tau): the learned filter prunes the
-    ratio-accepted queries scoring below tau before the one-query-per-target rule."""
-    from .matching import PairMatcher
-    out = [None] * len(pairs)
-    groups = {}
-    for k, r in enumerate(radii):
-        groups.setdefault(float(r), []).append(k)
-    for r, ks in groups.items():
-        used = sorted({i for k in ks for i in pairs[k]})
-        slot = {i: s for s, i in enumerate(used)}
-        sets = [(images[i].desc, images[i].und, (images[i].W, images[i].H)) for i in used]
-        sub = [(slot[pairs[k][0]], slot[pairs[k][1]]) for k in ks]
-        kcs = [0 if kc >= images[pairs[k][1]].n else kc for k in ks]
-        # kc >= n_b means every candidate in the window (cKDTree's k clipped to n_b)
-        for kc_val in sorted(set(kcs)):
-            ks2 = [k for k, v in zip(ks, kcs) if v == kc_val]
-            sub2 = [(slot[pairs[k][0]], slot[pairs[k][1]]) for k in ks2]
-            pm = PairMatcher(sets, sub2, [Hms[k] for k in ks2], radius=r, kc=kc_val, ratio=ratio)
-            if filt is None:
-                pm.run(form)
-            else:
-                net, tau = filt
-                pm.run_filtered(form, net, tau, [images[i].size for i in used], [images[i].angle for i in used])
-            for k, res in zip(ks2, pm.results()):
-                out[k] = res
-        del sub
-    return out
-
-
 def _concat_rows(rows):
     lens = np.array([len(r) for r in rows], np.int64)
     off = np.r_[0, np.cumsum(lens)].astype(np.int64)
@@ -228,6 +175,10 @@ class Positions:
         self.n = o
         self.xy = torch.cat(parts).contiguous() if o else torch.zeros((1, 2), dtype=torch.float64, device="cuda")
 
+    def base_of(self, photos):
+        """Row bases of photos (int64; a photo not held gives a base no row reaches)."""
+        return np.array([self.base.get(int(i), -(1 << 62)) for i in photos], np.int64)
+
     def rows(self, i, q):
         """Photo i's features q as a row reference for gather (global row numbers base[i] + q;
         the addition is done once for a whole list, in gather)."""
@@ -250,39 +201,6 @@ class Positions:
             _lib.call("fsx_gather_xy", tot, C.c_void_p(idx.data_ptr()), C.c_void_p(self.xy.data_ptr()), self.n,
                       C.c_void_p(out.data_ptr()), C.c_void_p(_lib.stream_handle()))
         return off, out
-
-
-def _prior_keep(cfg, P, items):
-    """items: [(i, j, rows in i, rows in j)] (global row numbers of P) -> per item a bool keep
-    mask (matching.py:221-225): one gather and one fsx_prior_keep_batch for all of them."""
-    if not items:
-        return []
-    torch = _torch()
-    off, d_ua = P.gather([it[2] for it in items])
-    _, d_ub = P.gather([it[3] for it in items])
-    if off[-1] == 0:
-        return [np.zeros(0, bool) for _ in items]
-    cams = np.zeros((len(items), 43), np.float64)
-    tol = np.full(len(items), -1.0)
-    on = np.fromiter((cfg.prior_on(it[0], it[1]) for it in items), bool, count=len(items))
-    if on.any():
-        src, dst = cfg._cam_rows()
-        ii = np.fromiter((it[0] for it in items), np.int64, count=len(items))[on]
-        jj = np.fromiter((it[1] for it in items), np.int64, count=len(items))[on]
-        cams[on, :22] = src[ii]
-        cams[on, 22:] = dst[jj]
-        tol[on] = cfg.prior_tol * np.maximum(np.asarray(cfg.W, np.float64)[jj], np.asarray(cfg.H, np.float64)[jj])
-    dev = "cuda"
-    d_off = torch.from_numpy(off).to(dev)
-    d_cams = torch.from_numpy(cams).to(dev)
-    d_tol = torch.from_numpy(tol).to(dev)
-    keep = torch.empty(int(off[-1]), dtype=torch.uint8, device=dev)
-    Pv = C.c_void_p
-    _lib.call("fsx_prior_keep_batch", len(items), Pv(d_off.data_ptr()), Pv(d_ua.data_ptr()), Pv(d_ub.data_ptr()),
-              Pv(d_cams.data_ptr()), Pv(d_tol.data_ptr()), int(np.diff(off).max()), Pv(keep.data_ptr()),
-              Pv(_lib.stream_handle()))
-    k = keep.cpu().numpy().astype(bool)
-    return [k[off[n]:off[n + 1]] for n in range(len(items))]
 
 
 class _Masks(list):
@@ -395,6 +313,234 @@ def run_pairs(tasks, images, cfg, form="mfma", filt=None, stats=None):
         return [run_pairs([t], images, cfg, form, filt)[0] for t in tasks]
 
 
+# ---- the chain's match lists on the device --------------------------------------------------
+
+class _DevLists:
+    """Match lists on the device: list k holds the (q, t) feature pairs of photos a[k] -> b[k] in
+    rows start[k] .. start[k] + cnt[k] of the int32 tensors q and t (host start / cnt int64)."""
+
+    def __init__(self, q, t, start, cnt, a, b):
+        self.q, self.t = q, t
+        self.start = np.asarray(start, np.int64)
+        self.cnt = np.asarray(cnt, np.int64)
+        self.a = np.asarray(a, np.int64)
+        self.b = np.asarray(b, np.int64)
+
+    def __len__(self):
+        return len(self.cnt)
+
+    def off(self):
+        """Row offsets (n + 1) of lists packed in order (as _move writes them)."""
+        o = np.zeros(len(self.cnt) + 1, np.int64)
+        np.cumsum(self.cnt, out=o[1:])
+        return o
+
+
+def _dev_i64(*arrays):
+    """Host int64 arrays of one length -> one device (k, n) int64 tensor (a single upload)."""
+    torch = _torch()
+    return torch.from_numpy(np.ascontiguousarray(np.stack([np.asarray(a, np.int64) for a in arrays]))).to("cuda")
+
+
+def _move(sources, n_out, P=None):
+    """A new _DevLists of n_out lists packed in order: for each source (L, src, dst, keep),
+    list dst[i] = L's list src[i], only its rows whose byte in keep (device uint8 over L's rows,
+    or None: all rows) is set.  With the Positions P also the rows' positions (pa, pb) from one
+    gather.  Returns (lists, pa, pb)."""
+    torch = _torch()
+    cnt = np.zeros(n_out, np.int64)
+    a = np.zeros(n_out, np.int64)
+    b = np.zeros(n_out, np.int64)
+    plans = []
+    for L, src, dst, keep in sources:
+        src = np.asarray(src, np.int64)
+        dst = np.asarray(dst, np.int64)
+        if len(src) == 0:
+            continue
+        d_src = _dev_i64(L.start[src])
+        d_cnt = torch.from_numpy(L.cnt[src].astype(np.int32)).to("cuda")
+        if keep is None:
+            c = L.cnt[src]
+        else:                   # counting pass: the rows each list keeps
+            kept = torch.zeros(len(src), dtype=torch.int32, device="cuda")
+            _lib.call("fsx_gather_lists", len(src), C.c_void_p(d_src.data_ptr()), C.c_void_p(d_cnt.data_ptr()),
+                      C.c_void_p(d_src.data_ptr()), 0, None, None, C.c_void_p(L.q.data_ptr()),
+                      C.c_void_p(L.t.data_ptr()), C.c_void_p(keep.data_ptr()), None, 0, None, None, None, None,
+                      C.c_void_p(kept.data_ptr()), C.c_void_p(_lib.stream_handle()))
+            c = kept.cpu().numpy().astype(np.int64)
+        cnt[dst] = c
+        a[dst] = L.a[src]
+        b[dst] = L.b[src]
+        plans.append((L, src, dst, keep, d_src, d_cnt))
+    start = np.zeros(n_out, np.int64)
+    if n_out:
+        start[1:] = np.cumsum(cnt)[:-1]
+    tot = int(cnt.sum())
+    q = torch.empty(max(tot, 1), dtype=torch.int32, device="cuda")
+    t = torch.empty(max(tot, 1), dtype=torch.int32, device="cuda")
+    pa = pb = None
+    if P is not None:
+        pa = torch.empty((max(tot, 1), 2), dtype=torch.float64, device="cuda")
+        pb = torch.empty((max(tot, 1), 2), dtype=torch.float64, device="cuda")
+    Pv = C.c_void_p
+    for L, src, dst, keep, d_src, d_cnt in plans:
+        if P is not None:
+            d_aux = _dev_i64(start[dst], P.base_of(L.a[src]), P.base_of(L.b[src]))
+            d_dst, d_ba, d_bb = (Pv(d_aux[r].data_ptr()) for r in range(3))
+        else:
+            d_aux = _dev_i64(start[dst])
+            d_dst, d_ba, d_bb = Pv(d_aux[0].data_ptr()), None, None
+        _lib.call("fsx_gather_lists", len(src), Pv(d_src.data_ptr()), Pv(d_cnt.data_ptr()), d_dst, tot, d_ba, d_bb,
+                  Pv(L.q.data_ptr()), Pv(L.t.data_ptr()), None if keep is None else Pv(keep.data_ptr()),
+                  Pv(P.xy.data_ptr()) if P is not None else None, P.n if P is not None else 0, Pv(q.data_ptr()),
+                  Pv(t.data_ptr()), Pv(pa.data_ptr()) if pa is not None else None,
+                  Pv(pb.data_ptr()) if pb is not None else None, None, Pv(_lib.stream_handle()))
+    return _DevLists(q, t, start, cnt, a, b), pa, pb
+
+
+def _lists_of_matcher(pm, pairs):
+    """The last run of PairMatcher pm (pairs [(i, j)] in its order) as _DevLists."""
+    cnt, off, q, t = pm.results_dev()
+    return _DevLists(q, t, off[:-1], cnt, [i for i, _ in pairs], [j for _, j in pairs])
+
+
+def _dense_dev(images, pairs, ratio, prefix=None, form="mfma"):
+    """_dense_matches with the lists left on the device."""
+    from .matching import PairMatcher
+    torch = _torch()
+    if not pairs:
+        z = torch.empty(1, dtype=torch.int32, device="cuda")
+        return _DevLists(z, z, [], [], [], [])
+    used = sorted({i for p in pairs for i in p})
+    slot = {i: k for k, i in enumerate(used)}
+    zeros = {}
+    sets = []
+    for i in used:
+        d = images[i].desc if prefix is None else images[i].desc[:prefix]
+        n = int(d.shape[0])
+        if n not in zeros:
+            zeros[n] = torch.zeros((n, 2), dtype=torch.float64, device=d.device)
+        sets.append((d, zeros[n], (1, 1)))
+    pm = PairMatcher(sets, [(slot[i], slot[j]) for i, j in pairs], None, radius=1.0, ratio=ratio)
+    pm.run(form)
+    L = _lists_of_matcher(pm, pairs)
+    # the reference returns nothing when either side has fewer than 2 features (matching.py:88)
+    for k, (i, j) in enumerate(pairs):
+        if min(sets[slot[i]][0].shape[0], sets[slot[j]][0].shape[0]) < 2:
+            L.cnt[k] = 0
+    return L
+
+
+def _guided_dev(images, pairs, Hms, radii, kc, ratio, form="mfma", filt=None):
+    """_guided_matches with the lists left on the device, in the order of `pairs`."""
+    from .matching import PairMatcher
+    torch = _torch()
+    if not pairs:
+        z = torch.empty(1, dtype=torch.int32, device="cuda")
+        return _DevLists(z, z, [], [], [], [])
+    parts = []
+    groups = {}
+    for k, r in enumerate(radii):
+        groups.setdefault(float(r), []).append(k)
+    for r, ks in groups.items():
+        used = sorted({i for k in ks for i in pairs[k]})
+        slot = {i: s for s, i in enumerate(used)}
+        sets = [(images[i].desc, images[i].und, (images[i].W, images[i].H)) for i in used]
+        kcs = [0 if kc >= images[pairs[k][1]].n else kc for k in ks]
+        # kc >= n_b means every candidate in the window (cKDTree's k clipped to n_b)
+        for kc_val in sorted(set(kcs)):
+            ks2 = [k for k, v in zip(ks, kcs) if v == kc_val]
+            sub2 = [(slot[pairs[k][0]], slot[pairs[k][1]]) for k in ks2]
+            pm = PairMatcher(sets, sub2, [Hms[k] for k in ks2], radius=r, kc=kc_val, ratio=ratio)
+            if filt is None:
+                pm.run(form)
+            else:
+                net, tau = filt
+                pm.run_filtered(form, net, tau, [images[i].size for i in used], [images[i].angle for i in used])
+            parts.append((_lists_of_matcher(pm, [pairs[k] for k in ks2]), ks2))
+    if len(parts) == 1 and parts[0][1] == list(range(len(pairs))):
+        return parts[0][0]
+    L, _, _ = _move([(Lp, np.arange(len(ks2)), ks2, None) for Lp, ks2 in parts], len(pairs))
+    return L
+
+
+def _prior_dev(cfg, P, L, sel):
+    """L with lists sel (indices; prior on, non-empty) cut to the matches that pass the
+    ground-plane prior (matching.py:221-225): one gather, one fsx_prior_keep_batch, one
+    compaction."""
+    torch = _torch()
+    if not len(sel):
+        return L
+    sel = np.asarray(sel, np.int64)
+    G, pa, pb = _move([(L, sel, np.arange(len(sel)), None)], len(sel), P)
+    off = G.off()
+    cams = np.zeros((len(sel), 43), np.float64)
+    src, dst = cfg._cam_rows()
+    cams[:, :22] = src[G.a]
+    cams[:, 22:] = dst[G.b]
+    tol = cfg.prior_tol * np.maximum(np.asarray(cfg.W, np.float64)[G.b], np.asarray(cfg.H, np.float64)[G.b])
+    keep = torch.empty(max(int(off[-1]), 1), dtype=torch.uint8, device="cuda")
+    d_off = torch.from_numpy(off).to("cuda")
+    d_cams = torch.from_numpy(cams).to("cuda")
+    d_tol = torch.from_numpy(tol).to("cuda")
+    Pv = C.c_void_p
+    _lib.call("fsx_prior_keep_batch", len(sel), Pv(d_off.data_ptr()), Pv(pa.data_ptr()), Pv(pb.data_ptr()),
+              Pv(d_cams.data_ptr()), Pv(d_tol.data_ptr()), int(G.cnt.max()), Pv(keep.data_ptr()),
+              Pv(_lib.stream_handle()))
+    rest = np.setdiff1d(np.arange(len(L)), sel)
+    out, _, _ = _move([(L, rest, rest, None), (G, np.arange(len(sel)), sel, keep)], len(L))
+    return out
+
+
+def _ransac_lists(model, off, pa, pb, thr, seeds, n_hyp, want_model=False, fit=False):
+    """fsx_ransac_batch on lists packed by _move (rows off[k] .. off[k+1] of pa / pb), flags
+    left on the device.  Returns (device inlier bytes, counts, models (n, 9) or None, similarity
+    fits (n, 4) or None)."""
+    torch = _torch()
+    n = len(off) - 1
+    tot = max(int(off[-1]), 1)
+    d_off = torch.from_numpy(np.asarray(off, np.int64)).to("cuda")
+    d_thr = torch.from_numpy(np.asarray(thr, np.float64).reshape(n)).to("cuda")
+    d_seed = torch.from_numpy(np.asarray([s & 0x7FFFFFFFFFFFFFFF if s < (1 << 63) else s - (1 << 64) for s in seeds],
+                                         np.int64)).to("cuda")
+    inl = torch.zeros(tot, dtype=torch.uint8, device="cuda")
+    M = torch.zeros((n, 9), dtype=torch.float64, device="cuda")
+    cnt = torch.zeros(n, dtype=torch.int32, device="cuda")
+    Pv = C.c_void_p
+    _lib.call("fsx_ransac_batch", _MODEL[model], n, Pv(d_off.data_ptr()), Pv(pa.data_ptr()), Pv(pb.data_ptr()),
+              int(n_hyp), Pv(d_thr.data_ptr()), Pv(d_seed.data_ptr()), Pv(inl.data_ptr()), Pv(M.data_ptr()),
+              Pv(cnt.data_ptr()), Pv(_lib.stream_handle()))
+    fits = None
+    if fit:
+        fits = torch.empty((n, 4), dtype=torch.float64, device="cuda")
+        _lib.call("fsx_similarity_fit_batch", n, Pv(d_off.data_ptr()), Pv(pa.data_ptr()), Pv(pb.data_ptr()),
+                  Pv(inl.data_ptr()), Pv(fits.data_ptr()), Pv(_lib.stream_handle()))
+    return (inl, cnt.cpu().numpy().astype(np.int64), M.cpu().numpy() if want_model else None,
+            fits.cpu().numpy() if fit else None)
+
+
+def _verify_dev(cfg, tasks, keys, P, L, sel):
+    """_verify on lists sel of L (keys: their task indices): F RANSAC of the lists with >= 8
+    matches, then each list cut to its inliers on the device.  Returns (the inlier lists of the
+    verified pairs, [(key, list index, n_in)] for them, {key: n_in} for every list of sel)."""
+    sel = [n for n in sel if L.cnt[n] >= 8]
+    n_in = {}
+    if not sel:
+        return None, [], n_in
+    V, pa, pb = _move([(L, sel, np.arange(len(sel)), None)], len(sel), P)
+    inl, cnt, _, _ = _ransac_lists("fundamental", V.off(), pa, pb, [cfg.ransac_thr] * len(sel),
+                                   [pair_seed(*tasks[keys[n]], 0xF) for n in sel], cfg.f_iters)
+    ok = [m for m in range(len(sel)) if cnt[m] > 0]
+    I, _, _ = _move([(V, ok, np.arange(len(ok)), inl)], len(ok))
+    if not np.array_equal(I.cnt, cnt[ok]):
+        raise RuntimeError("inlier flags disagree with the RANSAC counts")
+    done = []
+    for r, m in enumerate(ok):
+        n_in[keys[sel[m]]] = int(I.cnt[r])
+        done.append((keys[sel[m]], r, int(I.cnt[r])))
+    return I, done, n_in
+
+
 def _chain(tasks, images, cfg, form, filt, stats):
     n_task = len(tasks)
     out = [None] * n_task
@@ -406,79 +552,86 @@ def _chain(tasks, images, cfg, form, filt, stats):
             live.append(k)
     marks = [("start", time.perf_counter())]
     P = Positions({i: images[i].und for i in dict.fromkeys(i for k in live for i in tasks[k])})
-    result = {}            # task index -> (q, t, n_in)
+    result = {}            # task index -> (inlier lists, list index, n_in)
     marks.append(("positions", time.perf_counter()))
 
     # 1-2. coarse matches of the strongest features, prior filter
     n1 = cfg.n_coarse
     coarse = [k for k in live if cfg.guided and images[tasks[k][0]].n > n1 // 2]
-    cm = _dense_matches(images, [tasks[k] for k in coarse], 0.85, prefix=n1, form=form)
-    cm = dict(zip(coarse, cm))
-    pri = [k for k in coarse if cfg.prior_on(*tasks[k]) and len(cm[k][0])]
-    keeps = _prior_keep(cfg, P, [(tasks[k][0], tasks[k][1], P.rows(tasks[k][0], cm[k][0]), P.rows(tasks[k][1], cm[k][1]))
-                                 for k in pri])
-    for k, m in zip(pri, keeps):
-        cm[k] = (cm[k][0][m], cm[k][1][m])
+    Cm = _dense_dev(images, [tasks[k] for k in coarse], 0.85, prefix=n1, form=form)
+    Cm = _prior_dev(cfg, P, Cm, [n for n, k in enumerate(coarse) if cfg.prior_on(*tasks[k]) and Cm.cnt[n] > 0])
     marks.append(("coarse_and_prior", time.perf_counter()))
 
     # 3. homography RANSAC on the coarse set
-    hk = [k for k in coarse if len(cm[k][0]) >= 12]
-    _, hM, hcnt = ransac_rows("homography", P, [P.rows(tasks[k][0], cm[k][0]) for k in hk],
-                                [P.rows(tasks[k][1], cm[k][1]) for k in hk],
-                                [0.012 * cfg.side(tasks[k][0]) for k in hk],
-                                [pair_seed(*tasks[k], 0x1) for k in hk], H_ITERS)
-    gk = [k for k, c in zip(hk, hcnt.tolist()) if c >= 10]          # the winner's count = its flags' sum
-    Hm = {k: hM[n].reshape(3, 3) for n, k in enumerate(hk)}
+    hsel = [n for n in range(len(coarse)) if Cm.cnt[n] >= 12]
+    hk = [coarse[n] for n in hsel]
+    Hm = {}
+    gk = []
+    if hsel:
+        Hl, pa, pb = _move([(Cm, hsel, np.arange(len(hsel)), None)], len(hsel), P)
+        _, hcnt, hM, _ = _ransac_lists("homography", Hl.off(), pa, pb, [0.012 * cfg.side(tasks[k][0]) for k in hk],
+                                       [pair_seed(*tasks[k], 0x1) for k in hk], H_ITERS, want_model=True)
+        gk = [k for k, c in zip(hk, hcnt.tolist()) if c >= 10]          # the winner's count = its flags' sum
+        Hm = {k: hM[n].reshape(3, 3) for n, k in enumerate(hk)}
     marks.append(("homography", time.perf_counter()))
 
     # 4. guided matching under the homography
-    gm = _guided_matches(images, [tasks[k] for k in gk], [Hm[k] for k in gk],
-                         [cfg.guided_radius * cfg.side(tasks[k][1]) for k in gk], cfg.guided_k, 0.85, form, filt)
-    vk = [k for k, (q, _) in zip(gk, gm) if len(q) >= cfg.min_inliers]
-    gm = dict(zip(gk, gm))
+    G = _guided_dev(images, [tasks[k] for k in gk], [Hm[k] for k in gk],
+                    [cfg.guided_radius * cfg.side(tasks[k][1]) for k in gk], cfg.guided_k, 0.85, form, filt)
+    vsel = [n for n in range(len(gk)) if G.cnt[n] >= cfg.min_inliers]
     marks.append(("guided", time.perf_counter()))
 
     # 5. fundamental-matrix verification of the guided matches
-    for k, (q, t, n_in) in zip(vk, _verify(cfg, tasks, P, [(k, gm[k]) for k in vk])):
-        if q is not None and n_in >= max(cfg.min_inliers, 40):
-            result[k] = (q, t, n_in)
+    need = max(cfg.min_inliers, 40)
+    I1, done, _ = _verify_dev(cfg, tasks, gk, P, G, vsel)
+    for k, r, n_in in done:
+        if n_in >= need:
+            result[k] = (I1, r, n_in)
     marks.append(("verify", time.perf_counter()))
 
     # 6. fallback: exhaustive matching, prior filter, verification
     fb = [k for k in live if k not in result]
-    fm = dict(zip(fb, _dense_matches(images, [tasks[k] for k in fb], cfg.ratio, form=form)))
-    pri = [k for k in fb if cfg.prior_on(*tasks[k]) and len(fm[k][0])]
-    keeps = _prior_keep(cfg, P, [(tasks[k][0], tasks[k][1], P.rows(tasks[k][0], fm[k][0]), P.rows(tasks[k][1], fm[k][1]))
-                                 for k in pri])
-    for k, m in zip(pri, keeps):
-        fm[k] = (fm[k][0][m], fm[k][1][m])
+    Fm = _dense_dev(images, [tasks[k] for k in fb], cfg.ratio, form=form)
+    Fm = _prior_dev(cfg, P, Fm, [n for n, k in enumerate(fb) if cfg.prior_on(*tasks[k]) and Fm.cnt[n] > 0])
     enough = []
-    for k in fb:
-        if len(fm[k][0]) < 20:
+    for n, k in enumerate(fb):
+        if Fm.cnt[n] < 20:
             out[k] = (tasks[k][0], tasks[k][1], None, None, None, 0)
         else:
-            enough.append(k)
-    for k, (q, t, n_in) in zip(enough, _verify(cfg, tasks, P, [(k, fm[k]) for k in enough])):
-        if q is None or n_in < cfg.min_inliers:
-            out[k] = (tasks[k][0], tasks[k][1], None, None, None, n_in)
-        else:
-            result[k] = (q, t, n_in)
+            enough.append(n)
+    I2, done2, n_in2 = _verify_dev(cfg, tasks, fb, P, Fm, enough)
+    for n in enough:
+        k = fb[n]
+        if k not in n_in2 or n_in2[k] < cfg.min_inliers:
+            out[k] = (tasks[k][0], tasks[k][1], None, None, None, n_in2.get(k, 0))
+    for k, r, n_in in done2:
+        if n_in >= cfg.min_inliers:
+            result[k] = (I2, r, n_in)
 
     marks.append(("fallback", time.perf_counter()))
-    # 7. similarity of every verified set
+    # 7. similarity of every verified set: the lists in task order on the device, one RANSAC
+    # launch with its fits, then one copy of all verified lists to the host
     rk = sorted(result)
-    sims = _similarities(cfg, tasks, P, [(k, result[k]) for k in rk])
-    if rk:                       # int32 (q, t) of every pair: one narrowing, per-pair views
-        lens = [len(result[k][0]) for k in rk]
-        q32 = np.concatenate([np.asarray(result[k][0]) for k in rk]).astype(np.int32)
-        t32 = np.concatenate([np.asarray(result[k][1]) for k in rk]).astype(np.int32)
-        ends = np.cumsum(lens).tolist()
-        for k, sim, e, n in zip(rk, sims, ends, lens):
-            out[k] = (tasks[k][0], tasks[k][1], q32[e - n:e], t32[e - n:e], sim, result[k][2])
+    if rk:
+        srcs = []
+        for Lx in {id(result[k][0]): result[k][0] for k in rk}.values():
+            ks = [n for n, k in enumerate(rk) if result[k][0] is Lx]
+            srcs.append((Lx, [result[rk[n]][1] for n in ks], ks, None))
+        R, pa, pb = _move(srcs, len(rk), P)
+        _, scnt, _, fits = _ransac_lists("similarity", R.off(), pa, pb, [0.02 * cfg.side(tasks[k][0]) for k in rk],
+                                         [pair_seed(*tasks[k], 0x5) for k in rk], S_ITERS, fit=True)
+        good = ((scnt >= 2) & np.isfinite(fits).all(1)).tolist()
+        tot = int(R.cnt.sum())
+        q32 = R.q[:tot].cpu().numpy()
+        t32 = R.t[:tot].cpu().numpy()
+        o = R.off().tolist()
+        for n, (k, g, fit) in enumerate(zip(rk, good, fits.tolist())):
+            out[k] = (tasks[k][0], tasks[k][1], q32[o[n]:o[n + 1]], t32[o[n]:o[n + 1]], tuple(fit) if g else None,
+                      result[k][2])
     marks.append(("similarity", time.perf_counter()))
     if stats is not None:
         stats.update({"pairs": n_task, "coarse": len(coarse), "homography": len(hk), "guided": len(gk),
-                      "guided_verified": len(vk), "fallback": len(fb), "matched": len(rk),
+                      "guided_verified": len(vsel), "fallback": len(fb), "matched": len(rk),
                       "stage_ms": {b[0]: round(1e3 * (b[1] - a[1]), 3) for a, b in zip(marks, marks[1:])}})
     return out
 

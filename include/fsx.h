@@ -410,6 +410,20 @@ int fsx_ransac_batch(int model, int n_pair, const long long *off_dev, const doub
 int fsx_gather_xy(long long n, const int *idx_dev, const double *xy_dev, long long n_xy, double *out_dev,
                   void *stream);
 
+/* The chain's lists of matches moved between its stages on the device (no counterpart in the
+ * reference, whose lists stay host arrays, matching.py:180-233): list L's cnt_dev[L] rows from
+ * src_off_dev[L] of q_src_dev / t_src_dev (int32 feature indices; with keep_dev, only rows
+ * whose byte is non-zero, in order) to rows dst_off_dev[L] .. of q_dst_dev / t_dst_dev, and
+ * optionally their positions xy_dev[base_a_dev[L] + q] / xy_dev[base_b_dev[L] + t] (float64
+ * (x, y), NaN outside [0, n_xy)) to pa_dst_dev / pb_dst_dev; rows at or past dst_cap are
+ * skipped; kept_dev[L] (optional) = rows taken.  Any output may be null; xy and the position
+ * outputs 16-byte aligned.  Asynchronous. */
+int fsx_gather_lists(int n_list, const long long *src_off_dev, const int *cnt_dev, const long long *dst_off_dev,
+                     long long dst_cap, const long long *base_a_dev, const long long *base_b_dev,
+                     const int *q_src_dev, const int *t_src_dev, const uint8_t *keep_dev, const double *xy_dev,
+                     long long n_xy, int *q_dst_dev, int *t_dst_dev, double *pa_dst_dev, double *pb_dst_dev,
+                     int *kept_dev, void *stream);
+
 /* The least-squares similarity of _similarity (matching.py:194-202, cv2's refit on the RANSAC
  * inliers) for many pairs: pair p's rows off_dev[p] .. off_dev[p+1] of pa_dev / pb_dev, one
  * inlier byte per row; out_dev[4p..4p+3] = (angle rad, scale, tx, ty) of b ~ s R a + t, NaN
