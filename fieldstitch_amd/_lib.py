@@ -102,6 +102,7 @@ SIGNATURES = {
     "fsx_match_publish": [_vp, C.POINTER(C.c_longlong), _vp, _vp, C.c_longlong, _vp, C.c_float, _vp],
     "fsx_match_pack": [_vp, C.POINTER(C.c_longlong), _vp, C.c_longlong, _vp, _vp, _vp],
     "fsx_ransac_batch": [_i, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp],
+    "fsx_ransac_batch_adaptive": [_i, _i, _vp, _vp, _vp, _i, C.c_double, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "fsx_gather_xy": [C.c_longlong, _vp, _vp, C.c_longlong, _vp, _vp],
     "fsx_gather_lists": [_i, _vp, _vp, _vp, C.c_longlong, _vp, _vp, _vp, _vp, _vp, _vp, C.c_longlong, _vp, _vp, _vp,
                          _vp, _vp, _vp],

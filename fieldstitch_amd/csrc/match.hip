@@ -369,14 +369,17 @@ __device__ void knn_threshold(const MatchBatch& B, const MPair& P, const MImage&
 }
 
 // kc > 0: the kc-th nearest key of every query, one thread per query (before the contraction);
-// KS: the list length, the smallest of 8, 16, 24, 32, 48, 64 that holds kc
+// KS: the list length, the smallest of 8, 16, 24, 32, 48, 64 that holds kc.  Threads take the
+// queries in the cell-sorted order of image a (k_m_setup's s_orig), so the predictions of a
+// wave's queries lie close together in b and its lanes scan mostly the same candidate cells.
 template <int KS>
 __global__ __launch_bounds__(256) void k_m_knn(const MatchBatch B) {
     const MPair P = B.pair[blockIdx.y];      // records by value: loaded once, never reloaded
     const MImage ia = B.img[P.a];
     const MImage ib = B.img[P.b];
-    const int q = blockIdx.x * 256 + threadIdx.x;
-    if (q >= ia.n) return;
+    const int sq = blockIdx.x * 256 + threadIdx.x;
+    if (sq >= ia.n) return;
+    const int q = B.s_orig[ia.off + sq];
     const long long g = P.q_off + q;
     const long long ka = ia.off + q;
     double px, py, td;

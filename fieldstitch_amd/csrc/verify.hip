@@ -471,17 +471,21 @@ __global__ void k_ransac_keys_zero(int n_pair, double* __restrict__ H_out) {
     if (p < n_pair) ((unsigned long long*)H_out)[9 * (long long)p] = 0ull;
 }
 
+// r0: the first round of the launch (hypotheses 256 (r0 + blockIdx.y) + threadIdx.x); live
+// (optional, per pair): blocks of pairs whose flag is 0 exit at once (the adaptive form).
 template <class M>
 __global__ __launch_bounds__(256) void k_ransac_split(const long long* __restrict__ off, const double* __restrict__ pa,
                                                       const double* __restrict__ pb, int n_hyp, double thr2_all,
                                                       unsigned long long seed_all,
                                                       const unsigned long long* __restrict__ seeds,
-                                                      const double* __restrict__ thrs, double* __restrict__ H_out) {
+                                                      const double* __restrict__ thrs, double* __restrict__ H_out,
+                                                      int r0, const int* __restrict__ live) {
     const int slot = blockIdx.x;
+    if (live && !live[slot]) return;
     const RansacPair R = ransac_pair(slot, off, thr2_all, seed_all, seeds, thrs);
     const double* A = pa + 2 * R.o;
     const double* Bq = pb + 2 * R.o;
-    const int hy = 256 * (int)blockIdx.y + (int)threadIdx.x;
+    const int hy = 256 * (r0 + (int)blockIdx.y) + (int)threadIdx.x;
     double h[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) h[k] = 0.0;
@@ -492,22 +496,26 @@ __global__ __launch_bounds__(256) void k_ransac_split(const long long* __restric
 
 // The same scoring in two launches, for the models whose solve needs many registers (the
 // 4-point homography, the 8-point F: a fused kernel runs at the solve's occupancy).
-// k_ransac_solve: block (x, r) solves hypotheses 256 r .. 256 r + 255 of pair p0 + x into the
-// plane layout hyp[q * plane + x * n_hyp + h] (q = 0..8; valid[x * n_hyp + h] = 1 when the
-// sample gave a model); k_ransac_score's block (x, r) scores the same hypotheses.
+// k_ransac_solve: block (x, r) solves hypotheses hy = 256 (r0 + r) .. + 255 of pair p0 + x into
+// the plane layout hyp[q * plane + x * hstride + (hy - 256 r0)] (q = 0..8; valid[...] = 1 when
+// the sample gave a model; hstride = the hypotheses one launch holds per pair);
+// k_ransac_score's block (x, r) scores the same hypotheses.  live as in k_ransac_split.
 template <class M>
 __global__ __launch_bounds__(256) void k_ransac_solve(const long long* __restrict__ off, const double* __restrict__ pa,
                                                       const double* __restrict__ pb, int n_hyp,
                                                       unsigned long long seed_all,
                                                       const unsigned long long* __restrict__ seeds, int p0,
                                                       long long plane, double* __restrict__ hyp,
-                                                      uint8_t* __restrict__ valid) {
-    const int hy = 256 * (int)blockIdx.y + (int)threadIdx.x;
-    if (hy >= n_hyp) return;
+                                                      uint8_t* __restrict__ valid, int r0, int hstride,
+                                                      const int* __restrict__ live) {
+    const int hl = 256 * (int)blockIdx.y + (int)threadIdx.x;
+    const int hy = 256 * r0 + hl;
+    if (hy >= n_hyp || hl >= hstride) return;
+    if (live && !live[p0 + (int)blockIdx.x]) return;
     const RansacPair R = ransac_pair(p0 + (int)blockIdx.x, off, 0.0, seed_all, seeds, nullptr);
     double h[9];
     const bool ok = R.n >= M::K && hypothesis<M>(pa + 2 * R.o, pb + 2 * R.o, R.seed, R.p, hy, R.n, h);
-    const long long i = (long long)blockIdx.x * n_hyp + hy;
+    const long long i = (long long)blockIdx.x * hstride + hl;
 #pragma unroll
     for (int q = 0; q < 9; q++) hyp[q * plane + i] = ok ? h[q] : 0.0;
     valid[i] = ok ? 1 : 0;
@@ -520,16 +528,19 @@ __global__ __launch_bounds__(256) void k_ransac_score(const long long* __restric
                                                       const unsigned long long* __restrict__ seeds,
                                                       const double* __restrict__ thrs, int p0, long long plane,
                                                       const double* __restrict__ hyp,
-                                                      const uint8_t* __restrict__ valid, double* __restrict__ H_out) {
+                                                      const uint8_t* __restrict__ valid, double* __restrict__ H_out,
+                                                      int r0, int hstride, const int* __restrict__ live) {
     const int slot = p0 + (int)blockIdx.x;
+    if (live && !live[slot]) return;
     const RansacPair R = ransac_pair(slot, off, thr2_all, seed_all, seeds, thrs);
-    const int hy = 256 * (int)blockIdx.y + (int)threadIdx.x;
-    const long long i = (long long)blockIdx.x * n_hyp + hy;
+    const int hl = 256 * (int)blockIdx.y + (int)threadIdx.x;
+    const int hy = 256 * r0 + hl;
+    const long long i = (long long)blockIdx.x * hstride + hl;
     double h[9];
     bool ok = false;
 #pragma unroll
     for (int q = 0; q < 9; q++) h[q] = 0.0;
-    if (hy < n_hyp) {
+    if (hy < n_hyp && hl < hstride) {
         ok = valid[i] != 0;
 #pragma unroll
         for (int q = 0; q < 9; q++) h[q] = hyp[q * plane + i];
@@ -551,6 +562,49 @@ __global__ __launch_bounds__(256) void k_ransac_pick(const long long* __restrict
     if (threadIdx.x == 0) s_best = ((const unsigned long long*)H_out)[9 * (long long)slot];
     __syncthreads();                 // the key is read before ransac_finish overwrites the slot
     ransac_finish<M>(R, slot, pa + 2 * R.o, pb + 2 * R.o, s_best, inlier, H_out, n_in);
+}
+
+// Adaptive stopping, the rule by which cv2's RANSAC estimators end early: stop once the
+// hypotheses drawn reach log(1 - confidence) / log(1 - w^K), w the inlier ratio of the best
+// model so far, K its sample size.  Evaluated after each round of 256 hypotheses (R rounds
+// done) as (1 - w^K)^(256 R) <= 1 - confidence, by multiplications only: w = c / n, w^K by
+// K - 1 successive products, the 256th power by 8 squarings, then R - 1 products; the power
+// underflowing to 0 means stop.  A restatement with the same operations decides identically.
+template <int K>
+__device__ __forceinline__ bool ransac_confident(int c, int n, int rounds, double omc) {
+    const double w = (double)c / (double)n;
+    double wk = w;
+#pragma unroll
+    for (int i = 1; i < K; i++) wk = wk * w;
+    double q = 1.0 - wk;
+#pragma unroll
+    for (int i = 0; i < 8; i++) q = q * q;
+    double y = q;
+    for (int i = 1; i < rounds; i++) y = y * q;
+    return y <= omc;
+}
+
+__global__ void k_ransac_adapt_init(int n_pair, double* __restrict__ H_out, int* __restrict__ live) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p < n_pair) {
+        ((unsigned long long*)H_out)[9 * (long long)p] = 0ull;
+        live[p] = 1;
+    }
+}
+
+// After a round: done = hypotheses drawn per live pair so far (min(256 (r + 1), max_hyp)).  A
+// live pair records done in used[] and stops when done reaches max_hyp, when it has fewer than
+// K matches, or when the inlier count of its best key so far satisfies the rule above.
+template <class M>
+__global__ void k_ransac_adapt(int n_pair, const long long* __restrict__ off, int done, int max_hyp, double omc,
+                               const double* __restrict__ H_out, int* __restrict__ live, int* __restrict__ used) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= n_pair || !live[p]) return;
+    used[p] = done;
+    const int n = (int)(off[p + 1] - off[p]);
+    const unsigned long long key = ((const unsigned long long*)H_out)[9 * (long long)p];
+    const int c = key ? (int)(key >> 32) - 1 : 0;
+    if (done >= max_hyp || n < M::K || (c > 0 && ransac_confident<M::K>(c, n, done / 256, omc))) live[p] = 0;
 }
 
 // Least-squares refit of each pair's homography on the inliers of its winning hypothesis (the
@@ -863,7 +917,7 @@ static int launch_score(int n_pair, const long long* off_dev, const double* pa_d
     hipLaunchKernelGGL(k_ransac_keys_zero, dim3((unsigned)((n_pair + 255) / 256)), dim3(256), 0, stream, n_pair, M_dev);
     if constexpr (M::K <= 2) {
         hipLaunchKernelGGL(k_ransac_split<M>, dim3(n_pair, rounds), dim3(256), 0, stream, off_dev, pa_dev, pb_dev,
-                           n_hyp, thr2, seed, seed_dev, thr_dev, M_dev);
+                           n_hyp, thr2, seed, seed_dev, thr_dev, M_dev, 0, (const int*)nullptr);
     } else {
         const long long per_pair = (long long)n_hyp * (9 * sizeof(double) + 1);
         const int chunk = (int)std::max(1ll, std::min((long long)n_pair, RANSAC_HYP_BYTES / per_pair));
@@ -875,9 +929,10 @@ static int launch_score(int n_pair, const long long* off_dev, const double* pa_d
         for (int p0 = 0; p0 < n_pair; p0 += chunk) {
             const int cp = std::min(chunk, n_pair - p0);
             hipLaunchKernelGGL(k_ransac_solve<M>, dim3(cp, rounds), dim3(256), 0, stream, off_dev, pa_dev, pb_dev,
-                               n_hyp, seed, seed_dev, p0, plane, hyp, valid);
+                               n_hyp, seed, seed_dev, p0, plane, hyp, valid, 0, n_hyp, (const int*)nullptr);
             hipLaunchKernelGGL(k_ransac_score<M>, dim3(cp, rounds), dim3(256), 0, stream, off_dev, pa_dev, pb_dev,
-                               n_hyp, thr2, seed, seed_dev, thr_dev, p0, plane, hyp, valid, M_dev);
+                               n_hyp, thr2, seed, seed_dev, thr_dev, p0, plane, hyp, valid, M_dev, 0, n_hyp,
+                               (const int*)nullptr);
         }
         FSX_HIP(hipFreeAsync(tmp, stream));
     }
@@ -913,6 +968,55 @@ static int launch_ransac_per_pair(int n_pair, const long long* off_dev, const do
                                    M_dev, n_in_dev, (hipStream_t)stream);
     if (rc != FSX_OK) return rc;
     launch_refit<M>(n_pair, off_dev, pa_dev, pb_dev, inlier_dev, n_in_dev, M_dev, (hipStream_t)stream);
+    FSX_LAUNCH_CHECK();
+    return FSX_OK;
+}
+
+// The adaptive form for pairs with their own thresholds and seeds: rounds of 256 hypotheses,
+// each launched for every pair but run only by the pairs still live, k_ransac_adapt after each
+// round; the winner is the best key over the rounds a pair ran (most inliers, then the lowest
+// hypothesis), picked, labelled and refit as in the fixed form.  One stream-ordered temporary
+// holds the flags and one round's hypotheses (homography, F).
+template <class M>
+static int launch_ransac_adaptive(int n_pair, const long long* off_dev, const double* pa_dev, const double* pb_dev,
+                                  int max_hyp, double conf, const double* thr_dev,
+                                  const unsigned long long* seed_dev, uint8_t* inlier_dev, double* M_dev,
+                                  int* n_in_dev, int* used_dev, hipStream_t stream) {
+    const int rounds = (max_hyp + 255) / 256;
+    const double omc = 1.0 - conf;
+    const unsigned pair_blocks = (unsigned)((n_pair + 255) / 256);
+    const long long per_pair = 256ll * (9 * sizeof(double) + 1);
+    const int chunk = M::K <= 2 ? 0 : (int)std::max(1ll, std::min((long long)n_pair, RANSAC_HYP_BYTES / per_pair));
+    const long long plane = (long long)chunk * 256;
+    const size_t flag_bytes = ((size_t)n_pair * 2 * sizeof(int) + 255) & ~(size_t)255;
+    void* tmp = nullptr;
+    FSX_HIP(hipMallocAsync(&tmp, flag_bytes + (size_t)(plane * (9 * sizeof(double) + 1)), stream));
+    int* live = (int*)tmp;
+    int* used = used_dev ? used_dev : live + n_pair;
+    double* hyp = (double*)((char*)tmp + flag_bytes);
+    uint8_t* valid = (uint8_t*)(hyp + 9 * plane);
+    hipLaunchKernelGGL(k_ransac_adapt_init, dim3(pair_blocks), dim3(256), 0, stream, n_pair, M_dev, live);
+    for (int r = 0; r < rounds; r++) {
+        if constexpr (M::K <= 2) {
+            hipLaunchKernelGGL(k_ransac_split<M>, dim3(n_pair, 1), dim3(256), 0, stream, off_dev, pa_dev, pb_dev,
+                               max_hyp, 0.0, 0ull, seed_dev, thr_dev, M_dev, r, (const int*)live);
+        } else {
+            for (int p0 = 0; p0 < n_pair; p0 += chunk) {
+                const int cp = std::min(chunk, n_pair - p0);
+                hipLaunchKernelGGL(k_ransac_solve<M>, dim3(cp, 1), dim3(256), 0, stream, off_dev, pa_dev, pb_dev,
+                                   max_hyp, 0ull, seed_dev, p0, plane, hyp, valid, r, 256, (const int*)live);
+                hipLaunchKernelGGL(k_ransac_score<M>, dim3(cp, 1), dim3(256), 0, stream, off_dev, pa_dev, pb_dev,
+                                   max_hyp, 0.0, 0ull, seed_dev, thr_dev, p0, plane, hyp, valid, M_dev, r, 256,
+                                   (const int*)live);
+            }
+        }
+        hipLaunchKernelGGL(k_ransac_adapt<M>, dim3(pair_blocks), dim3(256), 0, stream, n_pair, off_dev,
+                           std::min(256 * (r + 1), max_hyp), max_hyp, omc, (const double*)M_dev, live, used);
+    }
+    hipLaunchKernelGGL(k_ransac_pick<M>, dim3(n_pair), dim3(256), 0, stream, off_dev, pa_dev, pb_dev, 0.0, 0ull,
+                       seed_dev, thr_dev, inlier_dev, M_dev, n_in_dev);
+    FSX_HIP(hipFreeAsync(tmp, stream));
+    launch_refit<M>(n_pair, off_dev, pa_dev, pb_dev, inlier_dev, n_in_dev, M_dev, stream);
     FSX_LAUNCH_CHECK();
     return FSX_OK;
 }
@@ -964,6 +1068,36 @@ int fsx_ransac_batch(int model, int n_pair, const long long* off_dev, const doub
         case 2:
             return launch_ransac_per_pair<ModelS>(n_pair, off_dev, pa_dev, pb_dev, n_hyp, thr_dev, seed_dev,
                                                   inlier_dev, M_dev, n_in_dev, stream);
+        default:
+            FSX_REQUIRE(false, "model must be 0 (homography), 1 (fundamental) or 2 (similarity)");
+    }
+}
+
+// fsx_ransac_batch with adaptive stopping: at most max_hyp hypotheses per pair, drawn in rounds
+// of 256 until the inlier ratio w of the pair's best model so far gives
+// (1 - w^K)^(hypotheses drawn) <= 1 - confidence (K = 4, 8, 2 for models 0, 1, 2).  used_dev
+// (optional, int per pair): hypotheses drawn.  A pair's result depends on its own seed and
+// matches only.
+int fsx_ransac_batch_adaptive(int model, int n_pair, const long long* off_dev, const double* pa_dev,
+                              const double* pb_dev, int max_hyp, double confidence, const double* thr_dev,
+                              const unsigned long long* seed_dev, uint8_t* inlier_dev, double* M_dev, int* n_in_dev,
+                              int* used_dev, void* stream) {
+    FSX_REQUIRE(n_pair >= 0 && max_hyp >= 1 && max_hyp <= (1 << 20), "bad RANSAC arguments");
+    FSX_REQUIRE(confidence > 0.0 && confidence < 1.0, "confidence must lie in (0, 1)");
+    if (n_pair == 0) return FSX_OK;
+    FSX_REQUIRE(off_dev && pa_dev && pb_dev && thr_dev && seed_dev && inlier_dev && M_dev && n_in_dev,
+                "null pointer argument");
+    const hipStream_t st = (hipStream_t)stream;
+    switch (model) {
+        case 0:
+            return launch_ransac_adaptive<ModelH>(n_pair, off_dev, pa_dev, pb_dev, max_hyp, confidence, thr_dev,
+                                                  seed_dev, inlier_dev, M_dev, n_in_dev, used_dev, st);
+        case 1:
+            return launch_ransac_adaptive<ModelF>(n_pair, off_dev, pa_dev, pb_dev, max_hyp, confidence, thr_dev,
+                                                  seed_dev, inlier_dev, M_dev, n_in_dev, used_dev, st);
+        case 2:
+            return launch_ransac_adaptive<ModelS>(n_pair, off_dev, pa_dev, pb_dev, max_hyp, confidence, thr_dev,
+                                                  seed_dev, inlier_dev, M_dev, n_in_dev, used_dev, st);
         default:
             FSX_REQUIRE(false, "model must be 0 (homography), 1 (fundamental) or 2 (similarity)");
     }

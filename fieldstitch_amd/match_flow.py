@@ -8,7 +8,7 @@ for all the pairs that reach it in one batch:
                  0.85 (matching.py:219-220): one dense PairMatcher batch;
   2. prior       the ground-plane prior filter (matching.py:221-225): one fsx_prior_keep_batch;
   3. homography  RANSAC on pairs with >= 12 coarse matches, threshold 0.012 max(W_i, H_i)
-                 (matching.py:226): one fsx_ransac_batch, a seed per pair;
+                 (matching.py:226): one fsx_ransac_batch_adaptive, a seed per pair;
   4. guided      pairs whose homography has >= 10 inliers: guided matching in a window of
                  guided_radius max(W_j, H_j) around the homography's prediction, the guided_k
                  nearest, ratio 0.85 (matching.py:227-229): one PairMatcher batch per window
@@ -20,6 +20,10 @@ for all the pairs that reach it in one batch:
                  with success at min_inliers (matching.py:236-246);
   7. similarity  RANSAC of a 2-point similarity at 0.02 max(W_i, H_i), then its least-squares
                  fit on the inliers (matching.py:194-202).
+
+The three RANSAC stages stop as cv2's estimators do: a pair draws hypotheses in rounds of 256
+until its best model's inlier ratio w makes (1 - w^K)^(hypotheses drawn) <= 1 - confidence,
+at most the call's iteration cap (confidence / cap of each call in H_CONF, F_CONF, S_CONF).
 
 The match lists stay on the device from one stage to the next (packed per pair; the
 matcher's output packed by fsx_match_pack, lists chosen, cut to the prior's survivors or to
@@ -44,9 +48,13 @@ import numpy as np
 
 from . import _lib
 
-H_ITERS = 2000        # cv2.findHomography's default iteration cap
-F_ITERS = 4096        # fixed count (cv2 stops at 0.999 confidence, at most 10000)
-S_ITERS = 2000        # cv2.estimateAffinePartial2D's default iteration cap
+# each estimator's iteration cap and confidence, as the reference calls cv2 (matching.py:226
+# findHomography with its defaults, :180 findFundamentalMat(..., 0.999, 10000), :196
+# estimateAffinePartial2D with its defaults); the device RANSAC stops as they do, once
+# the best model's inlier ratio w gives (1 - w^K)^(hypotheses drawn) <= 1 - confidence
+H_ITERS, H_CONF = 2000, 0.995
+F_ITERS, F_CONF = 10000, 0.999
+S_ITERS, S_CONF = 2000, 0.99
 _MODEL = {"homography": 0, "fundamental": 1, "similarity": 2}
 
 
@@ -212,9 +220,22 @@ class _Masks(list):
         self.off = off
 
 
-def _ransac_dev(model, off, d_a, d_b, thr, seeds, n_hyp):
-    """fsx_ransac_batch on device rows (pair k's rows off[k] .. off[k+1] of d_a / d_b).
-    Returns (inlier masks, models (n, 9), counts, the device inlier bytes)."""
+def _ransac_call(model, n, d_off, pa, pb, n_hyp, d_thr, d_seed, inl, M, cnt, conf):
+    """fsx_ransac_batch (conf None: n_hyp hypotheses per pair) or fsx_ransac_batch_adaptive
+    (at most n_hyp, stopping at confidence conf)."""
+    Pv = C.c_void_p
+    head = (_MODEL[model], n, Pv(d_off.data_ptr()), Pv(pa.data_ptr()), Pv(pb.data_ptr()), int(n_hyp))
+    tail = (Pv(d_thr.data_ptr()), Pv(d_seed.data_ptr()), Pv(inl.data_ptr()), Pv(M.data_ptr()), Pv(cnt.data_ptr()))
+    if conf is None:
+        _lib.call("fsx_ransac_batch", *head, *tail, Pv(_lib.stream_handle()))
+    else:
+        _lib.call("fsx_ransac_batch_adaptive", *head, float(conf), *tail, None, Pv(_lib.stream_handle()))
+
+
+def _ransac_dev(model, off, d_a, d_b, thr, seeds, n_hyp, conf=None):
+    """fsx_ransac_batch (or, with conf, the adaptive form) on device rows (pair k's rows off[k]
+    .. off[k+1] of d_a / d_b).  Returns (inlier masks, models (n, 9), counts, the device inlier
+    bytes)."""
     torch = _torch()
     n = len(off) - 1
     dev = "cuda"
@@ -226,17 +247,15 @@ def _ransac_dev(model, off, d_a, d_b, thr, seeds, n_hyp):
     inl = torch.zeros(tot, dtype=torch.uint8, device=dev)
     M = torch.zeros((n, 9), dtype=torch.float64, device=dev)
     cnt = torch.zeros(n, dtype=torch.int32, device=dev)
-    P = C.c_void_p
-    _lib.call("fsx_ransac_batch", _MODEL[model], n, P(d_off.data_ptr()), P(d_a.data_ptr()), P(d_b.data_ptr()),
-              int(n_hyp), P(d_thr.data_ptr()), P(d_seed.data_ptr()), P(inl.data_ptr()), P(M.data_ptr()),
-              P(cnt.data_ptr()), P(_lib.stream_handle()))
+    _ransac_call(model, n, d_off, d_a, d_b, n_hyp, d_thr, d_seed, inl, M, cnt, conf)
     flags = inl.cpu().numpy().view(bool)
     return (_Masks(flags, off), M.cpu().numpy(), cnt.cpu().numpy().astype(np.int64), (d_off, inl))
 
 
-def ransac_batch(model, pa_list, pb_list, thr, seeds, n_hyp):
-    """RANSAC of `model` for every pair, each with its own threshold and seed (one launch), on
-    host (n_k, 2) position arrays.  Returns (inlier masks, models (n, 9), counts)."""
+def ransac_batch(model, pa_list, pb_list, thr, seeds, n_hyp, conf=None):
+    """RANSAC of `model` for every pair, each with its own threshold and seed (one launch; with
+    conf, adaptive stopping at that confidence, n_hyp the cap), on host (n_k, 2) position
+    arrays.  Returns (inlier masks, models (n, 9), counts)."""
     torch = _torch()
     if not pa_list:
         return [], np.zeros((0, 9)), np.zeros(0, np.int64)
@@ -244,11 +263,11 @@ def ransac_batch(model, pa_list, pb_list, thr, seeds, n_hyp):
     _, B = _concat_rows(pb_list)
     d_a = torch.from_numpy(A if off[-1] else np.zeros((1, 2))).to("cuda")
     d_b = torch.from_numpy(B if off[-1] else np.zeros((1, 2))).to("cuda")
-    flags, M, cnt, _ = _ransac_dev(model, off, d_a, d_b, thr, seeds, n_hyp)
+    flags, M, cnt, _ = _ransac_dev(model, off, d_a, d_b, thr, seeds, n_hyp, conf)
     return flags, M, cnt
 
 
-def ransac_rows(model, P, ia_list, ib_list, thr, seeds, n_hyp, fit=False):
+def ransac_rows(model, P, ia_list, ib_list, thr, seeds, n_hyp, fit=False, conf=None):
     """ransac_batch on the rows ia_list[k] / ib_list[k] of the Positions P, gathered on the
     device.  With fit=True also the least-squares similarity of each pair's inliers
     (fsx_similarity_fit_batch): a (n, 4) array of (angle, scale, tx, ty), NaN rows without a
@@ -259,7 +278,7 @@ def ransac_rows(model, P, ia_list, ib_list, thr, seeds, n_hyp, fit=False):
         return empty + (np.zeros((0, 4)),) if fit else empty
     off, d_a = P.gather(ia_list)
     _, d_b = P.gather(ib_list)
-    flags, M, cnt, (d_off, inl) = _ransac_dev(model, off, d_a, d_b, thr, seeds, n_hyp)
+    flags, M, cnt, (d_off, inl) = _ransac_dev(model, off, d_a, d_b, thr, seeds, n_hyp, conf)
     if not fit:
         return flags, M, cnt
     n = len(ia_list)
@@ -334,6 +353,18 @@ class _DevLists:
         o = np.zeros(len(self.cnt) + 1, np.int64)
         np.cumsum(self.cnt, out=o[1:])
         return o
+
+
+def _to_host(*xs):
+    """Device tensors -> numpy arrays through pinned host buffers: one DMA each on the current
+    stream, then one synchronisation (a pageable copy of tens of MB goes through many small
+    staged copies at a few GB/s)."""
+    torch = _torch()
+    hs = [torch.empty(x.shape, dtype=x.dtype, pin_memory=True) for x in xs]
+    for h, x in zip(hs, xs):
+        h.copy_(x, non_blocking=True)
+    torch.cuda.current_stream().synchronize()
+    return [h.numpy() for h in hs]
 
 
 def _dev_i64(*arrays):
@@ -492,10 +523,10 @@ def _prior_dev(cfg, P, L, sel):
     return out
 
 
-def _ransac_lists(model, off, pa, pb, thr, seeds, n_hyp, want_model=False, fit=False):
-    """fsx_ransac_batch on lists packed by _move (rows off[k] .. off[k+1] of pa / pb), flags
-    left on the device.  Returns (device inlier bytes, counts, models (n, 9) or None, similarity
-    fits (n, 4) or None)."""
+def _ransac_lists(model, off, pa, pb, thr, seeds, n_hyp, want_model=False, fit=False, conf=None):
+    """fsx_ransac_batch (with conf: fsx_ransac_batch_adaptive) on lists packed by _move (rows
+    off[k] .. off[k+1] of pa / pb), flags left on the device.  Returns (device inlier bytes,
+    counts, models (n, 9) or None, similarity fits (n, 4) or None)."""
     torch = _torch()
     n = len(off) - 1
     tot = max(int(off[-1]), 1)
@@ -507,9 +538,7 @@ def _ransac_lists(model, off, pa, pb, thr, seeds, n_hyp, want_model=False, fit=F
     M = torch.zeros((n, 9), dtype=torch.float64, device="cuda")
     cnt = torch.zeros(n, dtype=torch.int32, device="cuda")
     Pv = C.c_void_p
-    _lib.call("fsx_ransac_batch", _MODEL[model], n, Pv(d_off.data_ptr()), Pv(pa.data_ptr()), Pv(pb.data_ptr()),
-              int(n_hyp), Pv(d_thr.data_ptr()), Pv(d_seed.data_ptr()), Pv(inl.data_ptr()), Pv(M.data_ptr()),
-              Pv(cnt.data_ptr()), Pv(_lib.stream_handle()))
+    _ransac_call(model, n, d_off, pa, pb, n_hyp, d_thr, d_seed, inl, M, cnt, conf)
     fits = None
     if fit:
         fits = torch.empty((n, 4), dtype=torch.float64, device="cuda")
@@ -529,7 +558,7 @@ def _verify_dev(cfg, tasks, keys, P, L, sel):
         return None, [], n_in
     V, pa, pb = _move([(L, sel, np.arange(len(sel)), None)], len(sel), P)
     inl, cnt, _, _ = _ransac_lists("fundamental", V.off(), pa, pb, [cfg.ransac_thr] * len(sel),
-                                   [pair_seed(*tasks[keys[n]], 0xF) for n in sel], cfg.f_iters)
+                                   [pair_seed(*tasks[keys[n]], 0xF) for n in sel], cfg.f_iters, conf=F_CONF)
     ok = [m for m in range(len(sel)) if cnt[m] > 0]
     I, _, _ = _move([(V, ok, np.arange(len(ok)), inl)], len(ok))
     if not np.array_equal(I.cnt, cnt[ok]):
@@ -570,7 +599,8 @@ def _chain(tasks, images, cfg, form, filt, stats):
     if hsel:
         Hl, pa, pb = _move([(Cm, hsel, np.arange(len(hsel)), None)], len(hsel), P)
         _, hcnt, hM, _ = _ransac_lists("homography", Hl.off(), pa, pb, [0.012 * cfg.side(tasks[k][0]) for k in hk],
-                                       [pair_seed(*tasks[k], 0x1) for k in hk], H_ITERS, want_model=True)
+                                       [pair_seed(*tasks[k], 0x1) for k in hk], H_ITERS, want_model=True,
+                                       conf=H_CONF)
         gk = [k for k, c in zip(hk, hcnt.tolist()) if c >= 10]          # the winner's count = its flags' sum
         Hm = {k: hM[n].reshape(3, 3) for n, k in enumerate(hk)}
     marks.append(("homography", time.perf_counter()))
@@ -619,11 +649,10 @@ def _chain(tasks, images, cfg, form, filt, stats):
             srcs.append((Lx, [result[rk[n]][1] for n in ks], ks, None))
         R, pa, pb = _move(srcs, len(rk), P)
         _, scnt, _, fits = _ransac_lists("similarity", R.off(), pa, pb, [0.02 * cfg.side(tasks[k][0]) for k in rk],
-                                         [pair_seed(*tasks[k], 0x5) for k in rk], S_ITERS, fit=True)
+                                         [pair_seed(*tasks[k], 0x5) for k in rk], S_ITERS, fit=True, conf=S_CONF)
         good = ((scnt >= 2) & np.isfinite(fits).all(1)).tolist()
         tot = int(R.cnt.sum())
-        q32 = R.q[:tot].cpu().numpy()
-        t32 = R.t[:tot].cpu().numpy()
+        q32, t32 = _to_host(R.q[:tot], R.t[:tot])
         o = R.off().tolist()
         for n, (k, g, fit) in enumerate(zip(rk, good, fits.tolist())):
             out[k] = (tasks[k][0], tasks[k][1], q32[o[n]:o[n + 1]], t32[o[n]:o[n + 1]], tuple(fit) if g else None,
@@ -646,7 +675,7 @@ def _verify(cfg, tasks, P, items):
                                 [P.rows(tasks[items[n][0]][0], items[n][1][0]) for n in ok],
                                 [P.rows(tasks[items[n][0]][1], items[n][1][1]) for n in ok],
                                 [cfg.ransac_thr] * len(ok), [pair_seed(*tasks[items[n][0]], 0xF) for n in ok],
-                                cfg.f_iters)
+                                cfg.f_iters, conf=F_CONF)
     if not ok:
         return res
     # the inliers of all pairs by one mask over the concatenated matches, split per pair
@@ -669,7 +698,8 @@ def _similarities(cfg, tasks, P, items):
     _, _, cnt, fits = ransac_rows("similarity", P, [P.rows(tasks[items[n][0]][0], items[n][1][0]) for n in ok],
                                   [P.rows(tasks[items[n][0]][1], items[n][1][1]) for n in ok],
                                   [0.02 * cfg.side(tasks[items[n][0]][0]) for n in ok],
-                                  [pair_seed(*tasks[items[n][0]], 0x5) for n in ok], S_ITERS, fit=True)
+                                  [pair_seed(*tasks[items[n][0]], 0x5) for n in ok], S_ITERS, fit=True,
+                                  conf=S_CONF)
     good = ((cnt >= 2) & np.isfinite(fits).all(1)).tolist() if ok else []
     for n, g, fit in zip(ok, good, fits.tolist()):
         if g:

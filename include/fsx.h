@@ -404,6 +404,21 @@ int fsx_ransac_batch(int model, int n_pair, const long long *off_dev, const doub
                      int n_hyp, const double *thr_dev, const unsigned long long *seed_dev, uint8_t *inlier_dev,
                      double *M_dev, int *n_in_dev, void *stream);
 
+/* fsx_ransac_batch with adaptive stopping, as cv2's estimators stop (cv2.findHomography's
+ * confidence 0.995 / 2000 iterations at matching.py:226, findFundamentalMat's 0.999 / 10000 at
+ * :180, estimateAffinePartial2D's 0.99 / 2000 at :196): at most max_hyp hypotheses per pair,
+ * drawn in rounds of 256 until the inlier ratio w of the pair's best model so far gives
+ * (1 - w^K)^(hypotheses drawn) <= 1 - confidence (sample size K = 4, 8, 2 for models 0, 1, 2),
+ * evaluated by multiplications only (w = c / n, w^K by K - 1 products, the power by squarings
+ * and products) so a restatement decides identically.  The winner is the best hypothesis of the
+ * rounds drawn (most inliers, then the lowest index).  used_dev (optional, int per pair):
+ * hypotheses drawn.  0 < confidence < 1.  Takes one stream-ordered temporary (flags and one
+ * round of hypotheses). */
+int fsx_ransac_batch_adaptive(int model, int n_pair, const long long *off_dev, const double *pa_dev,
+                              const double *pb_dev, int max_hyp, double confidence, const double *thr_dev,
+                              const unsigned long long *seed_dev, uint8_t *inlier_dev, double *M_dev, int *n_in_dev,
+                              int *used_dev, void *stream);
+
 /* Gather of matched positions (the host-side fancy indexing ua[q], ub[t] of matching.py:180,
  * 196, 226 for many pairs at once): out_dev[k] = xy_dev[idx_dev[k]] (two float64 each) for
  * k < n; an index outside [0, n_xy) gives NaN.  xy_dev and out_dev 16-byte aligned. */

@@ -6,7 +6,10 @@ with the oracle's exact parts in the places of the GPU kernels:
   ratio ties (matching.py:86-152);
 * the ground-plane prior: oracle.geom_oracle.ground_predict (matching.py:155-172);
 * cv2's estimators: the deterministic RANSAC restated in oracle.ransac_oracle, each pair's
-  stage drawing from its own seed (pair slot 0), as the batched GPU chain does;
+  stage drawing from its own seed (pair slot 0), as the batched GPU chain does, with adaptive
+  stopping at the confidence and iteration cap of each cv2 call (findHomography's defaults
+  0.995 / 2000, findFundamentalMat(..., 0.999, 10000), estimateAffinePartial2D's defaults
+  0.99 / 2000);
 * the similarity: that RANSAC's inliers fitted by numpy's least-squares solver.
 
 Stage order, thresholds, counts and return tuples follow matching.py:205-253 line by line.
@@ -20,6 +23,11 @@ import numpy as np
 from . import geom_oracle as go
 from . import match_oracle as mo
 from . import ransac_oracle as ro
+
+
+H_CAP, H_CONF = 2000, 0.995      # cv2.findHomography defaults (matching.py:226)
+F_CAP, F_CONF = 10000, 0.999     # cv2.findFundamentalMat(..., 0.999, 10000) (matching.py:180)
+S_CAP, S_CONF = 2000, 0.99       # cv2.estimateAffinePartial2D defaults (matching.py:196)
 
 
 def pair_seed(i, j, salt):
@@ -38,8 +46,8 @@ def _verify(ctx, i, j, ua, ub, q, t):
     """matching.py:175-191 with the restated fundamental-matrix RANSAC."""
     if len(q) < 8:
         return None, None, 0
-    f, _, cnt = ro.ransac_pair(0, ua[q], ub[t], int(ctx.get("ransac_iters", 4096)), float(ctx["ransac_thr"]),
-                               pair_seed(i, j, 0xF), "fundamental")
+    f, _, cnt, _ = ro.ransac_pair_adaptive(0, ua[q], ub[t], int(ctx.get("ransac_iters", F_CAP)), F_CONF,
+                                           float(ctx["ransac_thr"]), pair_seed(i, j, 0xF), "fundamental")
     if cnt == 0:
         return None, None, 0
     return q[f], t[f], int(f.sum())
@@ -50,8 +58,8 @@ def _similarity(ctx, i, j, ua, ub, q, t):
     if len(q) < 2:
         return None
     a, b = ua[q], ub[t]
-    f, _, cnt = ro.ransac_pair(0, a, b, 2000, 0.02 * max(ctx["W"][i], ctx["H"][i]), pair_seed(i, j, 0x5),
-                               "similarity")
+    f, _, cnt, _ = ro.ransac_pair_adaptive(0, a, b, S_CAP, S_CONF, 0.02 * max(ctx["W"][i], ctx["H"][i]),
+                                           pair_seed(i, j, 0x5), "similarity")
     if cnt < 2:
         return None
     a, b = a[f], b[f]
@@ -78,8 +86,9 @@ def match_pair(ctx, feats, i, j):
             m = _prior_keep(ctx, i, j, ua[q1], ub[t1])
             q1, t1 = q1[m], t1[m]
         if len(q1) >= 12:
-            hm, Hm, cnt = ro.ransac_pair(0, ua[q1], ub[t1], 2000, 0.012 * max(ctx["W"][i], ctx["H"][i]),
-                                         pair_seed(i, j, 0x1), "homography")
+            hm, Hm, cnt, _ = ro.ransac_pair_adaptive(0, ua[q1], ub[t1], H_CAP, H_CONF,
+                                                     0.012 * max(ctx["W"][i], ctx["H"][i]), pair_seed(i, j, 0x1),
+                                                     "homography")
             if cnt > 0 and hm.sum() >= 10:
                 radius = ctx["guided_radius"] * max(ctx["W"][j], ctx["H"][j])
                 kc = ctx.get("guided_k", 24)

@@ -4,7 +4,9 @@ counter-hash sampling, model solves (4-point homography DLT with partial pivotin
 similarity; float64, no contraction, sums in the device's order), inlier tests (fused
 multiply-adds where the device fuses them, through the C fma of sift_oracle.c) and
 least-squares refit of the winning homography as fieldstitch_amd/csrc/verify.hip, vectorised
-over the hypotheses, so every count, label and model can be compared exactly."""
+over the hypotheses, so every count, label and model can be compared exactly; and the adaptive
+stopping of fsx_ransac_batch_adaptive (rounds of 256 hypotheses, the stopping rule in the same
+float64 multiplications), pinned against the closed form log(1 - conf) / log(1 - w^K)."""
 from __future__ import annotations
 
 import numpy as np
@@ -374,3 +376,55 @@ def ransac_pair(p, a, b, n_hyp, thr, seed, model="homography"):
     flags = test(h[best:best + 1], a, b, thr * thr)[0]
     model_out = refit_h(a, b, flags, h[best]) if model == "homography" else h[best]
     return flags, model_out, int(cnt[best])
+
+
+def confident(c, n, K, rounds, omc):
+    """The adaptive stopping rule of fsx_ransac_batch_adaptive (verify.hip ransac_confident):
+    (1 - w^K)^(256 rounds) <= omc (= 1 - confidence) with w = c / n, by the same float64
+    multiplications in the same order: w^K by K - 1 successive products, the 256th power by 8
+    squarings, then rounds - 1 products."""
+    w = float(c) / float(n)
+    wk = w
+    for _ in range(K - 1):
+        wk = wk * w
+    q = 1.0 - wk
+    for _ in range(8):
+        q = q * q
+    y = q
+    for _ in range(rounds - 1):
+        y = y * q
+    return y <= omc
+
+
+def ransac_pair_adaptive(p, a, b, max_hyp, conf, thr, seed, model="homography"):
+    """The adaptive form (fsx_ransac_batch_adaptive) for pair index p: hypotheses in rounds of
+    256 (at most max_hyp), the best so far kept (most inliers, then the lowest hypothesis);
+    after each round the pair stops when it has drawn max_hyp, has fewer than K matches, or
+    its best count c > 0 satisfies confident(c, n, K, rounds done, 1 - conf).  Returns
+    (inlier flags (n,), model (9,), count, hypotheses drawn)."""
+    K, solve, test = MODELS[model]
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    n = len(a)
+    omc = 1.0 - conf
+    best_c, best_h = -1, None
+    used = 0
+    for r in range((max_hyp + 255) // 256):
+        h1 = min(256 * (r + 1), max_hyp)
+        used = h1
+        if n >= K:
+            idx, ok = samples(seed, p, n, np.arange(256 * r, h1), K)
+            safe = np.where(ok[:, None], idx, 0)
+            h, valid = solve(a[safe, 0], a[safe, 1], b[safe, 0], b[safe, 1])
+            valid &= ok
+            cnt = np.where(valid, test(h, a, b, thr * thr).sum(1), -1)
+            k = int(np.argmax(cnt))             # first maximum = lowest hypothesis of the round
+            if cnt[k] > best_c:                 # equal counts keep the earlier round's winner
+                best_c, best_h = int(cnt[k]), h[k]
+        if h1 >= max_hyp or n < K or (best_c > 0 and confident(best_c, n, K, h1 // 256, omc)):
+            break
+    if best_c < 0:
+        return np.zeros(n, bool), np.zeros(9), 0, used
+    flags = test(best_h[None], a, b, thr * thr)[0]
+    model_out = refit_h(a, b, flags, best_h) if model == "homography" else best_h
+    return flags, model_out, best_c, used

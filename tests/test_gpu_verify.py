@@ -147,3 +147,58 @@ def test_small_batch_split_equals_one_block_per_pair(gpu, model, thr):
         np.testing.assert_array_equal(small[0][p], big[0][p])
         np.testing.assert_array_equal(small[1][p], big[1][p])
         assert small[2][p] == big[2][p] and small[2][p] > 0
+
+
+@pytest.mark.parametrize("model,thr,conf,cap", [("homography", 3.0, 0.995, 2000), ("fundamental", 1.0, 0.999, 1500),
+                                                 ("similarity", 4.0, 0.99, 2000)])
+def test_adaptive_batch_equals_oracle(gpu, model, thr, conf, cap):
+    """fsx_ransac_batch_adaptive against ransac_oracle.ransac_pair_adaptive, pair by pair:
+    counts, inlier flags, models and hypotheses drawn, on sets that stop after one round (all
+    inliers), after a few (70 %), that never stop before the cap (few inliers), and sets with
+    fewer matches than the sample size, none, or only degenerate samples."""
+    import ctypes as C
+    import torch
+    from fieldstitch_amd import _lib
+    from fieldstitch_amd.match_flow import _MODEL
+    rng = np.random.default_rng(41)
+    H = np.array([[0.99, -0.05, 600.0], [0.05, 0.99, -40.0], [1e-6, -2e-6, 1.0]])
+    if model == "homography":
+        sets = [_planted(rng, 600, 1.0, H), _planted(rng, 900, 0.7, H), _planted(rng, 400, 0.25, H)]
+    else:
+        sets = [_views(rng, 600, (12.0, 3.0, 1.0), 1.0, noise=0.05), _views(rng, 900, (20.0, 2.0, 0.0), 0.7),
+                _views(rng, 400, (2.0, 9.0, 0.3), 0.25)]
+    sets += [(sets[1][0][:3], sets[1][1][:3]), (np.zeros((0, 2)), np.zeros((0, 2)))]
+    pt = np.tile([[100.0, 200.0]], (20, 1))
+    sets.append((pt, pt + 5.0))
+    n = len(sets)
+    off = np.zeros(n + 1, np.int64)
+    off[1:] = np.cumsum([len(a) for a, _ in sets])
+    A = torch.from_numpy(np.concatenate([a for a, _ in sets] + [np.zeros((1, 2))])).cuda()
+    B = torch.from_numpy(np.concatenate([b for _, b in sets] + [np.zeros((1, 2))])).cuda()
+    seeds = [0x1234 + 977 * k for k in range(n)]
+    d_off = torch.from_numpy(off).cuda()
+    d_thr = torch.full((n,), thr, dtype=torch.float64, device="cuda")
+    d_seed = torch.tensor(seeds, dtype=torch.int64, device="cuda")
+    inl = torch.zeros(int(off[-1]) + 1, dtype=torch.uint8, device="cuda")
+    M = torch.zeros((n, 9), dtype=torch.float64, device="cuda")
+    cnt = torch.zeros(n, dtype=torch.int32, device="cuda")
+    used = torch.full((n,), -1, dtype=torch.int32, device="cuda")
+    P = C.c_void_p
+    print("adaptive", model, [len(a) for a, _ in sets], flush=True)
+    _lib.call("fsx_ransac_batch_adaptive", _MODEL[model], n, P(d_off.data_ptr()), P(A.data_ptr()), P(B.data_ptr()),
+              cap, conf, P(d_thr.data_ptr()), P(d_seed.data_ptr()), P(inl.data_ptr()), P(M.data_ptr()),
+              P(cnt.data_ptr()), P(used.data_ptr()), P(_lib.stream_handle()))
+    torch.cuda.synchronize()
+    flags, Ms, cs, us = inl.cpu().numpy().view(bool), M.cpu().numpy(), cnt.cpu().numpy(), used.cpu().numpy()
+    rounds = []
+    for p, (a, b) in enumerate(sets):
+        rf, rm, rc, ru = ro.ransac_pair_adaptive(0, a, b, cap, conf, thr, seeds[p], model=model)
+        print("pair", p, len(a), rc, ru, flush=True)
+        assert (cs[p], us[p]) == (rc, ru)
+        np.testing.assert_array_equal(flags[off[p]:off[p + 1]], rf)
+        np.testing.assert_allclose(Ms[p], rm, rtol=1e-10, atol=1e-12)
+        rounds.append(ru)
+    # one round with all inliers; a degenerate set never gets a model and runs to the cap; few
+    # inliers reach the cap for the 4- and 8-point models (the 2-point one stops after 512)
+    assert rounds[0] == 256 and 256 <= rounds[1] < cap and rounds[-1] == cap
+    assert rounds[2] == (512 if model == "similarity" else cap)

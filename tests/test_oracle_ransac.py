@@ -266,3 +266,62 @@ def test_fundamental_zero_normal_never_passes():
     F = np.zeros((1, 9))
     F[0, [2, 5, 8]] = 1.0                      # l2 = (1, 1, 1) for every point, l1 = (0, 0, .)
     assert not ro.inliers_f(F, a, a, 1e30).any()
+
+
+# ---- adaptive stopping (fsx_ransac_batch_adaptive) ---------------------------------------------
+
+def test_stopping_rule_is_the_closed_form():
+    """confident(c, n, K, R, 1 - conf) holds exactly when 256 R >= log(1 - conf) / log(1 - w^K)
+    (the iteration count cv2's estimators stop at), away from the boundary; a power computed
+    with one squaring too few or too many, or w^K with a wrong exponent, fails it."""
+    import math
+    checked = 0
+    for K in (2, 4, 8):
+        for conf in (0.99, 0.995, 0.999):
+            for n in (37, 500, 2612):
+                for c in range(1, n + 1, max(1, n // 97)):
+                    w = c / n
+                    denom = math.log1p(-(w ** K)) if w < 1 else -math.inf
+                    need = math.log(1.0 - conf) / denom if denom != 0 else math.inf
+                    for R in (1, 2, 3, 5, 8, 16, 39, 40):
+                        if math.isfinite(need) and abs(256 * R - need) <= 1e-9 * need:
+                            continue
+                        assert ro.confident(c, n, K, R, 1.0 - conf) == (256 * R >= need), (K, conf, n, c, R)
+                        checked += 1
+    assert checked > 10000
+
+
+def test_unreachable_confidence_equals_the_fixed_count():
+    """With 1 - conf = 2^-52 and few inliers the rule never holds within the cap, so the
+    adaptive form draws every hypothesis and returns what the fixed-count RANSAC returns."""
+    rng = np.random.default_rng(31)
+    H = np.array([[0.99, -0.05, 600.0], [0.05, 0.99, -40.0], [1e-6, -2e-6, 1.0]])
+    a, b, _ = planted(rng, 400, 0.3, H)
+    conf = 1.0 - 2.0 ** -52
+    for model, cap, thr in (("homography", 700, 3.0), ("fundamental", 600, 1.0)):
+        f1, m1, c1 = ro.ransac_pair(0, a, b, cap, thr, 77, model=model)
+        f2, m2, c2, used = ro.ransac_pair_adaptive(0, a, b, cap, conf, thr, 77, model=model)
+        assert used == cap and c1 == c2
+        np.testing.assert_array_equal(f1, f2)
+        np.testing.assert_array_equal(m1, m2)
+
+
+def test_adaptive_stops_at_the_first_confident_round():
+    """All inliers: the first round's best model has w = 1, so one round of 256; 70 % inliers:
+    the pair stops once 256 R reaches the closed-form count for its final inlier ratio, far
+    below the cap, with the planted inliers found; fewer than K matches: one round, no model."""
+    import math
+    rng = np.random.default_rng(32)
+    n = 300
+    a = rng.uniform(0, 2000, (n, 2))
+    b = a * 1.02 + [15.0, -7.0]
+    f, M, c, used = ro.ransac_pair_adaptive(0, a, b, 2000, 0.99, 3.0, 5, model="similarity")
+    assert used == 256 and c == n and f.all()
+    H = np.array([[0.99, -0.05, 600.0], [0.05, 0.99, -40.0], [1e-6, -2e-6, 1.0]])
+    a, b, good = planted(rng, 900, 0.7, H)
+    f, h, c, used = ro.ransac_pair_adaptive(0, a, b, 2000, 0.995, 3.0, 11)
+    need = math.log(0.005) / math.log1p(-((c / 900) ** 4))
+    assert used % 256 == 0 and 256 <= used < 2000 and used >= need
+    np.testing.assert_array_equal(f, good)
+    f, h, c, used = ro.ransac_pair_adaptive(0, a[:3], b[:3], 2000, 0.995, 3.0, 11)
+    assert used == 256 and c == 0 and not f.any()
