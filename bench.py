@@ -290,13 +290,16 @@ def bench_chain(fl, desc_dev, xy_dev, counts, shapes, pairs, reps=2):
     stats = {}
     res = match_flow.run_pairs(pairs, images, cfg, stats=stats)          # warm-up (allocations, kernels)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
+    ts = []
     for _ in range(reps):
+        t0 = time.perf_counter()
         res = match_flow.run_pairs(pairs, images, cfg, stats=stats)     # stage_ms: the last rep's
-    torch.cuda.synchronize()
-    t = (time.perf_counter() - t0) / reps
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    t = sum(ts) / reps
     matched = [r for r in res if r[2] is not None]
-    return {"pairs": len(pairs), "ms": t * 1e3, "pairs_per_s": len(pairs) / t, "stage_pairs": stats,
+    return {"pairs": len(pairs), "ms": t * 1e3, "ms_reps": [round(x * 1e3, 2) for x in ts],
+            "pairs_per_s": len(pairs) / t, "stage_pairs": stats,
             "matched_pairs": len(matched), "verified_matches_mean": float(np.mean([r[5] for r in matched])) if matched
             else 0.0, "errors": int(sum(1 for r in res if isinstance(r[4], tuple) and r[4] and r[4][0] == "error"))}
 
@@ -493,7 +496,7 @@ def bench_flight400(args, tracks=True):
                                    Hm[p], (shapes[j][1], shapes[j][0]), RADIUS, RATIO)
         ok += int(np.array_equal(q, res["mfma"][p][0]) and np.array_equal(t, res["mfma"][p][1]))
     log("flight400: matching timed (%d pairs), chain next" % len(pl))
-    chain = bench_chain(fl, u8_all, xy_all, counts, shapes, pl, reps=1)
+    chain = bench_chain(fl, u8_all, xy_all, counts, shapes, pl, reps=3)
     log("flight400: chain %.1f ms" % chain["ms"])
     if not tracks:
         return _flight400_record(args, n, t_sift, alg, counts, tuned, t_pairs, pl, out_m, same_forms, sample, ok,

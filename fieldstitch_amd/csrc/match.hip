@@ -368,11 +368,131 @@ __device__ void knn_threshold(const MatchBatch& B, const MPair& P, const MImage&
     }
 }
 
-// kc > 0: the kc-th nearest key of every query, one thread per query (before the contraction);
-// KS: the list length, the smallest of 8, 16, 24, 32, 48, 64 that holds kc.  Threads take the
-// queries in the cell-sorted order of image a (k_m_setup's s_orig), so the predictions of a
+// The same key by binned selection (the insertion network above costs O(KS) per inserted
+// keypoint, and with kc ~ half the window's keypoints most of them are inserted).  Pass 1 counts
+// the in-radius keypoints per distance bin, bin = min(floor(d * KNN_NB / r2), KNN_NB - 1):
+// monotone in d, so each bin holds a contiguous run of the (d, idx) order; the bin holding the
+// kc-th key and the count `below` it follow.  Pass 2 keeps the KNN_SMALL smallest (d, idx) keys
+// of that bin in an exact register list; the key of rank kc - below in it is the kc-th overall.
+// When that rank exceeds KNN_SMALL (clusters, equal distances) td = -1 marks the query for
+// k_m_knn_exact (knn_threshold).  cnt: this thread's KNN_NB counters in LDS, KNN_STRIDE words
+// apart.
+constexpr int KNN_NB = 32;
+constexpr int KNN_SMALL = 8;
+constexpr int KNN_STRIDE = 256;
+
+// The 3x3 cells around (cx0, cy0) as at most six runs of consecutive cell ids (a row's cells are
+// consecutive inside a 4x4 tile of cell_id), so their keypoints are runs of the sorted order;
+// f(s0, s1) is called for each run's sorted rows [s0, s1).
+template <class F>
+__device__ __forceinline__ void for_window_runs(const MImage& ib, const int* cs, int cx0, int cy0, F&& f) {
+    const int xlo = max(cx0 - 1, 0), xhi = min(cx0 + 1, ib.gw - 1);
+    const int xm = min(xhi, xlo | 3);             // last cell of xlo's tile within [xlo, xhi]
+#pragma unroll
+    for (int dy = -1; dy <= 1; dy++) {
+        const int cy = cy0 + dy;
+        if (cy < 0 || cy >= ib.gh) continue;
+        f(cs[cell_id(ib, xlo, cy)], cs[cell_id(ib, xm, cy) + 1]);
+        if (xm < xhi) f(cs[cell_id(ib, xm + 1, cy)], cs[cell_id(ib, xhi, cy) + 1]);
+    }
+}
+
+// squared distances of sorted rows s .. s + 3 of run [s, s1) from (px, py); rows at or past s1
+// read row s1 - 1 and give +inf
+__device__ __forceinline__ void dist4(const MatchBatch& B, long long off, int s, int s1, double px, double py,
+                                      double (&d)[4]) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const long long sb = off + min(s + k, s1 - 1);
+        const double dx = B.s_xy[2 * sb] - px, dy = B.s_xy[2 * sb + 1] - py;
+        d[k] = s + k < s1 ? dx * dx + dy * dy : INFINITY;
+    }
+}
+
+__device__ void knn_threshold_binned(const MatchBatch& B, const MPair& P, const MImage& ib, int cell, double px,
+                                     double py, double& td, int& ti, int* cnt) {
+    td = B.r2;
+    ti = 0x7fffffff;
+    if (B.kc <= 0 || cell < 0 || P.dense) return;
+#pragma unroll
+    for (int b = 0; b < KNN_NB; b++) cnt[b * KNN_STRIDE] = 0;
+    const double r2 = B.r2;
+    const double inv = (double)KNN_NB / r2;
+    const int kc = B.kc;
+    int cx0, cy0;
+    cell_xy(ib, cell, cx0, cy0);
+    const int* cs = B.cell_start + ib.cell_off + P.b;
+    const long long off = ib.off;
+    int nl = 0;
+    for_window_runs(ib, cs, cx0, cy0, [&](int s0, int s1) {
+        for (int s = s0; s < s1; s += 4) {
+            double d[4];
+            dist4(B, off, s, s1, px, py, d);
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (d[k] < r2) {
+                    nl++;
+                    atomicAdd(&cnt[min((int)(d[k] * inv), KNN_NB - 1) * KNN_STRIDE], 1);
+                }
+        }
+    });
+    if (nl < kc) return;
+    int below = 0, bs = KNN_NB - 1;
+    for (int b = 0; b < KNN_NB; b++) {
+        const int c = cnt[b * KNN_STRIDE];
+        if (below + c >= kc) {
+            bs = b;
+            break;
+        }
+        below += c;
+    }
+    const int r = kc - below;                    // rank of the kc-th key inside bin bs, >= 1
+    if (r > KNN_SMALL) {
+        td = -1.0;
+        return;
+    }
+    double ld[KNN_SMALL];
+    int li[KNN_SMALL];
+#pragma unroll
+    for (int j = 0; j < KNN_SMALL; j++) {
+        ld[j] = INFINITY;
+        li[j] = 0x7fffffff;
+    }
+    for_window_runs(ib, cs, cx0, cy0, [&](int s0, int s1) {
+        for (int s = s0; s < s1; s += 4) {
+            double d[4];
+            dist4(B, off, s, s1, px, py, d);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const double dk = d[k];
+                if (!(dk < r2) || min((int)(dk * inv), KNN_NB - 1) != bs) continue;
+                const int t = B.s_orig[off + s + k];
+                if (!sp_less(dk, t, ld[KNN_SMALL - 1], li[KNN_SMALL - 1])) continue;
+#pragma unroll
+                for (int j = KNN_SMALL - 1; j >= 1; j--) {
+                    const bool lp = sp_less(dk, t, ld[j - 1], li[j - 1]);
+                    const bool lc = sp_less(dk, t, ld[j], li[j]);
+                    ld[j] = lp ? ld[j - 1] : (lc ? dk : ld[j]);
+                    li[j] = lp ? li[j - 1] : (lc ? t : li[j]);
+                }
+                const bool l0 = sp_less(dk, t, ld[0], li[0]);
+                ld[0] = l0 ? dk : ld[0];
+                li[0] = l0 ? t : li[0];
+            }
+        }
+    });
+#pragma unroll
+    for (int j = 0; j < KNN_SMALL; j++)
+        if (j == r - 1) {
+            td = ld[j];
+            ti = li[j];
+        }
+}
+
+// kc > 0: the kc-th nearest key of every query, one thread per query (before the contraction),
+// by binned selection; queries it marks (kth = -1) are decided by k_m_knn_exact.  Threads take
+// the queries in the cell-sorted order of image a (k_m_setup's s_orig), so the predictions of a
 // wave's queries lie close together in b and its lanes scan mostly the same candidate cells.
-template <int KS>
 __global__ __launch_bounds__(256) void k_m_knn(const MatchBatch B) {
     const MPair P = B.pair[blockIdx.y];      // records by value: loaded once, never reloaded
     const MImage ia = B.img[P.a];
@@ -381,6 +501,27 @@ __global__ __launch_bounds__(256) void k_m_knn(const MatchBatch B) {
     if (sq >= ia.n) return;
     const int q = B.s_orig[ia.off + sq];
     const long long g = P.q_off + q;
+    const long long ka = ia.off + q;
+    double px, py, td;
+    int ti;
+    __shared__ int s_cnt[KNN_NB * KNN_STRIDE];
+    const int cell = predict_cell(B, P, ia, ib, B.xy[2 * ka], B.xy[2 * ka + 1], px, py);
+    knn_threshold_binned(B, P, ib, cell, px, py, td, ti, s_cnt + threadIdx.x);
+    B.kth[2 * g] = td;
+    B.kth[2 * g + 1] = __longlong_as_double((long long)ti);
+}
+
+// the queries k_m_knn marked: the insertion network over all their candidates; KS the list
+// length, the smallest of 8, 16, 24, 32, 48, 64 that holds kc
+template <int KS>
+__global__ __launch_bounds__(256) void k_m_knn_exact(const MatchBatch B) {
+    const MPair P = B.pair[blockIdx.y];
+    const MImage ia = B.img[P.a];
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= ia.n) return;
+    const long long g = P.q_off + q;
+    if (!(B.kth[2 * g] < 0.0)) return;
+    const MImage ib = B.img[P.b];
     const long long ka = ia.off + q;
     double px, py, td;
     int ti;
@@ -1205,12 +1346,13 @@ static int launch_sequence(const long long* info, int form, hipEvent_t* ev, hipS
     if (nq > 0 && info[6] > 0) {
         const dim3 g((unsigned)((max_na + 255) / 256), n_pair);
         const long long kc = info[6];
-        if (kc <= 8) hipLaunchKernelGGL(k_m_knn<8>, g, dim3(256), 0, st, b);
-        else if (kc <= 16) hipLaunchKernelGGL(k_m_knn<16>, g, dim3(256), 0, st, b);
-        else if (kc <= 24) hipLaunchKernelGGL(k_m_knn<24>, g, dim3(256), 0, st, b);
-        else if (kc <= 32) hipLaunchKernelGGL(k_m_knn<32>, g, dim3(256), 0, st, b);
-        else if (kc <= 48) hipLaunchKernelGGL(k_m_knn<48>, g, dim3(256), 0, st, b);
-        else hipLaunchKernelGGL(k_m_knn<KMAX>, g, dim3(256), 0, st, b);
+        hipLaunchKernelGGL(k_m_knn, g, dim3(256), 0, st, b);
+        if (kc <= 8) hipLaunchKernelGGL(k_m_knn_exact<8>, g, dim3(256), 0, st, b);
+        else if (kc <= 16) hipLaunchKernelGGL(k_m_knn_exact<16>, g, dim3(256), 0, st, b);
+        else if (kc <= 24) hipLaunchKernelGGL(k_m_knn_exact<24>, g, dim3(256), 0, st, b);
+        else if (kc <= 32) hipLaunchKernelGGL(k_m_knn_exact<32>, g, dim3(256), 0, st, b);
+        else if (kc <= 48) hipLaunchKernelGGL(k_m_knn_exact<48>, g, dim3(256), 0, st, b);
+        else hipLaunchKernelGGL(k_m_knn_exact<KMAX>, g, dim3(256), 0, st, b);
     }
     FSX_LAUNCH_CHECK();
     if (ev) FSX_HIP(hipEventRecord(ev[1], st));

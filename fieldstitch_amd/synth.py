@@ -191,6 +191,18 @@ def _render_job(job):
     return mos.render(M, h, w, np.random.default_rng(noise_seed))
 
 
+def _pool_context():
+    """Process start method of the render pools: fork, unless this process has initialised the
+    GPU runtime (children forked from such a process can crash), then spawn (the workers import
+    only numpy and this module)."""
+    import multiprocessing as mp
+    import sys
+    torch = sys.modules.get("torch")
+    if torch is not None and torch.cuda.is_initialized():
+        return mp.get_context("spawn")
+    return mp.get_context("fork")
+
+
 def flight(seed: int = 2, rows: int = 2, cols: int = 4, W: int = 2640, H: int = 1978,
            front: float = 0.75, side: float = 0.65, rot_jitter: float = 3.0, scale_jitter: float = 0.02,
            prior_sigma: float = 3.0, lowres_frac: float = 0.0, poly_density: float = 40.0 / (1368 * 912),
@@ -230,9 +242,8 @@ def flight(seed: int = 2, rows: int = 2, cols: int = 4, W: int = 2640, H: int = 
             centres.append((cu, cv))
             jobs.append((mos, M, h, w, int(rng.integers(1 << 62))))
     if workers > 1:
-        import multiprocessing as mp
         images = []
-        with mp.get_context("fork").Pool(min(workers, len(jobs))) as pool:
+        with _pool_context().Pool(min(workers, len(jobs))) as pool:
             for im in pool.imap(_render_job, jobs):
                 images.append(im)
                 if progress and len(images) % 25 == 0:
