@@ -213,6 +213,49 @@ __device__ __forceinline__ float blur_point(const double* w, const HalfTaps& tp)
     return (float)ex;
 }
 
+// blur_point for a thread's group of outputs: each output's fused result is rounded and its two
+// check quantities folded into running minima (mid: (lo << 3) - (0x0FFFFF00 << 3), tiny:
+// bits(r) - 1, both unsigned), so the group costs one branch instead of one per output; when
+// any output of the group fails the check (mid <= 4096 or tiny < 0x037FFFFF) the group is
+// evaluated again output by output with blur_point.  Either way every output equals
+// blur_point's.
+template <int R2>
+__device__ __forceinline__ double blur_acc(const double* w, const HalfTaps& tp) {
+    double acc = w[R2] * tp.k[0];
+#pragma unroll
+    for (int j = R2; j >= 1; --j) acc = __builtin_fma(w[R2 - j] + w[R2 + j], tp.k[j], acc);
+    return acc;
+}
+
+__device__ __forceinline__ float blur_round(double acc, unsigned& mid, unsigned& tiny) {
+    const unsigned lo = (unsigned)__double_as_longlong(acc);
+    const float r = (float)acc;
+    mid = min(mid, (lo << 3) - (0x0FFFFF00u << 3));
+    tiny = min(tiny, __float_as_uint(r) - 1u);
+    return r;
+}
+
+// (the empty asm with a memory clobber keeps the group's stores ahead of the branch: without it
+// the compiler sinks them below, since the per-output path overwrites the same words, and holds
+// every rounded value of the group in registers until then)
+__device__ __forceinline__ bool blur_group_unsure(unsigned mid, unsigned tiny) {
+    asm volatile("" ::: "memory");
+    return mid <= 4096u || tiny < 0x037FFFFFu;
+}
+
+// the per-output path's copy of a window, passed through an opaque move so that none of its
+// pair sums is shared with the group's fast evaluation (shared sums would stay live across the
+// branch, all of them at once)
+template <int N>
+__device__ __forceinline__ void blur_launder(const double* w, double* out) {
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        double x = w[k];
+        asm volatile("" : "+v"(x));
+        out[k] = x;
+    }
+}
+
 template <int R2, bool BASE, int BL_TH>
 __global__ __launch_bounds__(BL_NT) void k_blur(const SiftJob* __restrict__ jobs, int o, int layer, HalfTaps tp) {
     constexpr int TWo = blur_tw<R2>();
@@ -251,55 +294,67 @@ __global__ __launch_bounds__(BL_NT) void k_blur(const SiftJob* __restrict__ jobs
         const int c = tid;
         const int gx = reflect101(x0 - R2 + c, C);
         const int ybase = y0 - R2;
-        float f[NW];
-        // base layer: ybase = TH*ty - R2 has the parity of R2, so in the interior the source
-        // rows of each upsampled row are fixed offsets from rb = (ybase >> 1) - 1; the thread
-        // loads its two source columns over those NI rows once (the same float32 operations
-        // as up2_value, no clamping needed inside [rb, rb + NI))
-        constexpr int PAR = R2 & 1;
-        constexpr int NI = ((PAR + NW - 1) >> 1) + 3;
-        const int rb = (ybase >> 1) - 1;
-        if (BASE && ybase >= 0 && rb >= 0 && rb + NI <= J.H) {
-            const float s255 = (float)(1.0 / 255.0);
-            const int ix = gx >> 1;
-            const int nx = (gx & 1) ? min(ix + 1, J.W - 1) : max(ix - 1, 0);
-            const __attribute__((address_space(1))) uint8_t* src = gptr(J.img) + (size_t)rb * J.ld;
-            float bi[NI], bn[NI];
+        // the thread's input column over the window rows, as float64 (called again by the
+        // group's per-output path, so that the window is not held across the group)
+        auto load_window = [&](double (&w)[NW]) {
+            float f[NW];
+            // base layer: ybase = TH*ty - R2 has the parity of R2, so in the interior the source
+            // rows of each upsampled row are fixed offsets from rb = (ybase >> 1) - 1; the thread
+            // loads its two source columns over those NI rows once (the same float32 operations
+            // as up2_value, no clamping needed inside [rb, rb + NI))
+            constexpr int PAR = R2 & 1;
+            constexpr int NI = ((PAR + NW - 1) >> 1) + 3;
+            const int rb = (ybase >> 1) - 1;
+            if (BASE && ybase >= 0 && rb >= 0 && rb + NI <= J.H) {
+                const float s255 = (float)(1.0 / 255.0);
+                const int ix = gx >> 1;
+                const int nx = (gx & 1) ? min(ix + 1, J.W - 1) : max(ix - 1, 0);
+                const __attribute__((address_space(1))) uint8_t* src = gptr(J.img) + (size_t)rb * J.ld;
+                float bi[NI], bn[NI];
 #pragma unroll
-            for (int r = 0; r < NI; r++) {
-                bi[r] = (float)src[(size_t)r * J.ld + ix] * s255;
-                bn[r] = (float)src[(size_t)r * J.ld + nx] * s255;
-            }
+                for (int r = 0; r < NI; r++) {
+                    bi[r] = (float)src[(size_t)r * J.ld + ix] * s255;
+                    bn[r] = (float)src[(size_t)r * J.ld + nx] * s255;
+                }
 #pragma unroll
-            for (int k = 0; k < NW; k++) {
-                const int iy = ((PAR + k) >> 1) + 1;
-                const int ny = ((PAR + k) & 1) ? iy + 1 : iy - 1;
-                const float t_i = 0.75f * bi[iy] + 0.25f * bi[ny];
-                const float t_n = 0.75f * bn[iy] + 0.25f * bn[ny];
-                f[k] = 0.75f * t_i + 0.25f * t_n;
-            }
-        } else if (ybase >= 0 && ybase + NW <= R) {
-            // interior rows (block-uniform test): direct addressing, no reflection
-            if (BASE) {
+                for (int k = 0; k < NW; k++) {
+                    const int iy = ((PAR + k) >> 1) + 1;
+                    const int ny = ((PAR + k) & 1) ? iy + 1 : iy - 1;
+                    const float t_i = 0.75f * bi[iy] + 0.25f * bi[ny];
+                    const float t_n = 0.75f * bn[iy] + 0.25f * bn[ny];
+                    f[k] = 0.75f * t_i + 0.25f * t_n;
+                }
+            } else if (ybase >= 0 && ybase + NW <= R) {
+                // interior rows (block-uniform test): direct addressing, no reflection
+                if (BASE) {
 #pragma unroll
-                for (int k = 0; k < NW; k++) f[k] = up2_value(J.img, J.H, J.W, J.ld, ybase + k, gx);
+                    for (int k = 0; k < NW; k++) f[k] = up2_value(J.img, J.H, J.W, J.ld, ybase + k, gx);
+                } else {
+                    const __attribute__((address_space(1))) float* col = gptr(in) + (size_t)ybase * C + gx;
+#pragma unroll
+                    for (int k = 0; k < NW; k++) f[k] = col[(size_t)k * C];
+                }
             } else {
-                const __attribute__((address_space(1))) float* col = gptr(in) + (size_t)ybase * C + gx;
 #pragma unroll
-                for (int k = 0; k < NW; k++) f[k] = col[(size_t)k * C];
+                for (int k = 0; k < NW; k++) {
+                    const int gy = reflect101(ybase + k, R);
+                    f[k] = BASE ? up2_value(J.img, J.H, J.W, J.ld, gy, gx) : gptr(in)[(size_t)gy * C + gx];
+                }
             }
-        } else {
 #pragma unroll
-            for (int k = 0; k < NW; k++) {
-                const int gy = reflect101(ybase + k, R);
-                f[k] = BASE ? up2_value(J.img, J.H, J.W, J.ld, gy, gx) : gptr(in)[(size_t)gy * C + gx];
-            }
-        }
+            for (int k = 0; k < NW; k++) w[k] = (double)f[k];
+        };
         double w[NW];
+        load_window(w);
+        unsigned mid = ~0u, tiny = ~0u;
 #pragma unroll
-        for (int k = 0; k < NW; k++) w[k] = (double)f[k];
+        for (int i = 0; i < BL_TH; i++) s_mid[i][c] = blur_round(blur_acc<R2>(w + i, tp), mid, tiny);
+        if (__builtin_expect(blur_group_unsure(mid, tiny), 0)) {
+            double wl[NW];
+            load_window(wl);
 #pragma unroll
-        for (int i = 0; i < BL_TH; i++) s_mid[i][c] = blur_point<R2>(w + i, tp);
+            for (int i = 0; i < BL_TH; i++) s_mid[i][c] = blur_point<R2>(wl + i, tp);
+        }
     }
     __syncthreads();
     for (int task = tid; task < BL_TH * NG; task += BL_NT) {
@@ -311,10 +366,23 @@ __global__ __launch_bounds__(BL_NT) void k_blur(const SiftJob* __restrict__ jobs
         double v[HW];
 #pragma unroll
         for (int k = 0; k < HW; k++) v[k] = (double)fv[k];
+        unsigned mid = ~0u, tiny = ~0u;
 #pragma unroll
         for (int j = 0; j < CB; j++) {
             if (cbase + j >= TWo) continue;
-            s_out[ly][cbase + j] = blur_point<R2>(v + j, tp);
+            s_out[ly][cbase + j] = blur_round(blur_acc<R2>(v + j, tp), mid, tiny);
+        }
+        if (__builtin_expect(blur_group_unsure(mid, tiny), 0)) {
+            // the window again from LDS (not held across the group)
+            double vl[HW];
+#pragma unroll
+            for (int k = 0; k < HW; k++) vl[k] = (double)((cbase + k < BL_SW) ? s_mid[ly][cbase + k] : 0.0f);
+            blur_launder<HW>(vl, vl);
+#pragma unroll
+            for (int j = 0; j < CB; j++) {
+                if (cbase + j >= TWo) continue;
+                s_out[ly][cbase + j] = blur_point<R2>(vl + j, tp);
+            }
         }
     }
     __syncthreads();
