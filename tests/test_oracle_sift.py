@@ -297,3 +297,36 @@ def test_midpoint_window_as_one_shift_add():
     a = ((low29 - (0x10000000 - 256)) % 2 ** 32) <= 512
     b = (((lo << 3) % 2 ** 32 - (0x0FFFFF00 << 3)) % 2 ** 32) <= 4096
     assert np.array_equal(a, b) and a.sum() > 1000
+
+
+def test_grouped_rounding_check_flags_exactly_the_groups_with_an_uncertain_output():
+    """The blur folds each output's check quantities into two running unsigned minima, mid =
+    (lo << 3) - (0x0FFFFF00 << 3) and tiny = bits(r) - 1 (32-bit), and re-evaluates the group
+    output by output iff min(mid) <= 4096 or min(tiny) < 0x037FFFFF.  Groups of 16 and 14
+    float64 results, some steered into the midpoint window or below 2^-120 (and exact zeros,
+    which must not count as tiny): the group flag equals 'some output fails the per-output
+    check'."""
+    rng = np.random.default_rng(9)
+    M = np.uint64(2 ** 32)
+    for size in (16, 14):
+        acc = rng.uniform(0.0, 1.0, (4000, size))
+        bits = acc.view(np.uint64).copy()
+        pick = rng.random(acc.shape) < 0.004                      # into the midpoint window
+        bits[pick] = (bits[pick] & ~np.uint64(0x1FFFFFFF)) | np.uint64(0x10000000) + rng.integers(
+            -300, 300, int(pick.sum())).astype(np.uint64)
+        acc = bits.view(np.float64)
+        tiny_pick = rng.random(acc.shape) < 0.003                 # below 2^-120, non-zero
+        acc[tiny_pick] = rng.uniform(2.0 ** -140, 2.0 ** -121, int(tiny_pick.sum()))
+        acc[rng.random(acc.shape) < 0.01] = 0.0
+        lo = acc.view(np.uint64) & np.uint64(0xFFFFFFFF)
+        r = acc.astype(np.float32)
+        mid = ((lo << np.uint64(3)) % M - np.uint64(0x0FFFFF00 << 3)) % M
+        tiny = (r.view(np.uint32).astype(np.uint64) - np.uint64(1)) % M
+        per_output = (mid <= 4096) | (tiny < 0x037FFFFF)
+        # the per-output check in its own terms: near a float32 midpoint, or 0 < r < 2^-120
+        low29 = acc.view(np.uint64) & np.uint64(0x1FFFFFFF)
+        near = ((low29 - np.uint64(0x10000000 - 256)) % M) <= 512
+        assert np.array_equal(per_output, near | ((r > 0) & (r < np.float32(2.0 ** -120))))
+        group = (mid.min(1) <= 4096) | (tiny.min(1) < 0x037FFFFF)
+        assert np.array_equal(group, per_output.any(1))
+        assert 0 < group.sum() < len(group)
