@@ -97,7 +97,7 @@ __device__ __forceinline__ bool is_inlier(const double* h, double x, double y, d
     const double w = fma(h[6], x, fma(h[7], y, h[8]));
     const double ex = fma(-u, w, fma(h[0], x, fma(h[1], y, h[2])));
     const double ey = fma(-v, w, fma(h[3], x, fma(h[4], y, h[5])));
-    return (fabs(w) > 1e-12) & (fma(ex, ex, ey * ey) < thr2 * (w * w));
+    return ((int)(fabs(w) > 1e-12) & (int)(fma(ex, ex, ey * ey) < thr2 * (w * w))) != 0;   // both tests, no branch
 }
 
 // ---- models: K points per sample, solve(K points) -> 9 parameters, inlier test -------------
