@@ -68,7 +68,7 @@ def sift_bytes(H, W):
     return H * W + 4 * P0 + 118 * sumP
 
 
-# ---- CPU baseline: the reference's CPU worker pool (BASELINE.md plan) -----------------------
+# ---- CPU baseline: the reference's CPU worker pool shape -------------------------------------
 # A spawn pool of os.cpu_count() - 1 single-threaded workers (sysinfo.py:28-35, pipeline.py:14-18);
 # photos and features reach the workers through files they memory-map, so the timed maps carry
 # only indices.

@@ -374,7 +374,7 @@ int fsx_match_filter_features(void *ws_dev, const long long *launch_info, const 
                               const float *angle_dev, void *X_dev, void *stream);
 
 /* The matcher in stages, so that the learned match filter can prune between the contraction
- * and the one-query-per-target rule (no reference counterpart: the north star's pruning stage
+ * and the one-query-per-target rule (no reference counterpart: the learned filter's pruning stage
  * ahead of matching.py:146-152's acceptance and the verification of :226-229).  On a workspace
  * prepared by fsx_match_prepare:
  *   fsx_match_contract: setup, kc-nearest prefilter and contraction (`form` as
